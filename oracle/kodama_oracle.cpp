@@ -1,0 +1,1483 @@
+// CPU oracle for the KODAMA label-evolution hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// A C++ restatement of the reference algorithms, used as the checker of the HIP
+// path. It is C++ rather than C because exact trajectory replay depends on the
+// libstdc++ random streams the reference draws from (std::mt19937_64,
+// std::uniform_int_distribution, std::shuffle, std::discrete_distribution,
+// std::uniform_real_distribution, std::normal_distribution); calling the same
+// library with the same sequence of requests reproduces them.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+// library. The product path never does.
+//
+// Pinning (what checks each function; the tests live in tests/test_oracle_*.py):
+//   oracle_make_folds        vs reference make_folds (oracle/_ref) + fold-size invariants
+//   oracle_distance          vs reference HNSW distances on exhaustive searches (_ref)
+//   oracle_exact_knn         vs brute force on tiny inputs, reference exhaustive HNSW (_ref)
+//   oracle_knn_vote          hand-built ties (count, score, label) + empty-row fallback
+//   oracle_objective         closed form acc*sqrt(1-sum p^2), single-class -inf
+//   oracle_core_knn          trajectories vs reference CoreKNN_CPU (_ref, golden fixtures)
+//   oracle_knncv             vs reference KNNCV_CPU (_ref, golden fixtures)
+//   oracle_knn_graph         brute force on tiny inputs; reference HNSW graph recall
+//   oracle_dissimilarity     brute-force co-assignment counts + reference (_ref)
+//   oracle_plsldacv (fp32)   vs reference PLSLDACV_CPU (_ref), bit-exact predictions
+//   oracle_plsldacv (fp64)   two-class closed form (rank-1 S), w'Gw = 1, orthonormal loadings
+//   oracle_core_plslda       trajectories vs reference CorePLSLDA_CPU (_ref)
+
+#include "kodama_oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+using IVec = std::vector<int>;
+
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  } catch (...) {
+    g_error = "unknown exception";
+    return 1;
+  }
+}
+
+IVec to_vec(const int32_t* p, int n) { return p ? IVec(p, p + n) : IVec(); }
+
+// ---------------------------------------------------------------------------
+// Folds. Follows common.cpp:108-214 (detail::make_folds): one mt19937_64 seeded
+// with the fold seed; unconstrained rows are either shuffled and dealt i % F, or
+// (stratified) shuffled per label and dealt to the least-loaded fold. Constrained
+// rows move as groups (majority label per group).
+// ---------------------------------------------------------------------------
+IVec make_folds(const IVec& labels, const IVec& constrain, int n_folds, bool stratified,
+                std::uint64_t seed) {
+  if (n_folds < 2) throw std::invalid_argument("Number of folds must be at least 2.");
+  const int n = static_cast<int>(labels.size());
+  std::mt19937_64 rng(seed);
+  IVec out(static_cast<size_t>(n), 0);
+  if (constrain.empty()) {
+    if (stratified) {
+      std::map<int, IVec> rows_of_label;
+      for (int i = 0; i < n; ++i) rows_of_label[labels[i]].push_back(i);
+      IVec load(static_cast<size_t>(n_folds), 0);
+      for (auto& entry : rows_of_label) {
+        IVec& rows = entry.second;
+        std::shuffle(rows.begin(), rows.end(), rng);
+        std::fill(load.begin(), load.end(), 0);
+        for (int r : rows) {
+          const int f = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+          out[r] = f;
+          ++load[f];
+        }
+      }
+    } else {
+      IVec perm(static_cast<size_t>(n));
+      std::iota(perm.begin(), perm.end(), 0);
+      std::shuffle(perm.begin(), perm.end(), rng);
+      for (size_t i = 0; i < perm.size(); ++i) out[perm[i]] = static_cast<int>(i % n_folds);
+    }
+    return out;
+  }
+
+  std::map<int, IVec> members;
+  for (int i = 0; i < n; ++i) members[constrain[i]].push_back(i);
+  struct Group { int key; int label; int size; };
+  std::vector<Group> groups;
+  for (const auto& kv : members) {
+    std::map<int, int> tally;
+    for (int r : kv.second) ++tally[labels[r]];
+    int best = tally.begin()->first, best_n = tally.begin()->second;
+    for (const auto& t : tally) {
+      if (t.second > best_n) { best = t.first; best_n = t.second; }
+    }
+    groups.push_back({kv.first, best, static_cast<int>(kv.second.size())});
+  }
+  IVec group_fold(groups.size(), 0);
+  if (stratified) {
+    std::map<int, std::vector<size_t>> groups_of_label;
+    for (size_t g = 0; g < groups.size(); ++g) groups_of_label[groups[g].label].push_back(g);
+    IVec load(static_cast<size_t>(n_folds), 0);
+    for (auto& entry : groups_of_label) {
+      auto& ids = entry.second;
+      std::shuffle(ids.begin(), ids.end(), rng);
+      std::fill(load.begin(), load.end(), 0);
+      for (size_t g : ids) {
+        const int f = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        group_fold[g] = f;
+        load[f] += groups[g].size;
+      }
+    }
+  } else {
+    std::vector<size_t> perm(groups.size());
+    std::iota(perm.begin(), perm.end(), 0);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    for (size_t i = 0; i < perm.size(); ++i) group_fold[perm[i]] = static_cast<int>(i % n_folds);
+  }
+  size_t g = 0;
+  for (const auto& kv : members) {
+    for (int r : kv.second) out[r] = group_fold[g];
+    ++g;
+  }
+  return out;
+}
+
+IVec sorted_unique(IVec v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Distances. The reference's CPU kNN (native_knn.cpp:41-95, taken on any x86-64
+// with AVX2+FMA) accumulates 32 lane partials with fused multiply-adds over blocks
+// of 32 dimensions, reduces them lane-wise (s0+s1)+(s2+s3), then 8 -> 4 lanes,
+// then (l0+l1)+(l2+l3), and adds the remaining dimensions one at a time with a
+// fused multiply-add (GCC contracts `r += d*d` inside the avx2,fma function).
+// Squared L2 for Euclidean; minus the dot product for cosine / inner product.
+// ---------------------------------------------------------------------------
+float ref_order_reduce(const float* a, const float* b, int p, bool euclid) {
+  float acc[32];
+  for (int s = 0; s < 32; ++s) acc[s] = 0.0f;
+  int d = 0;
+  for (; d + 31 < p; d += 32) {
+    for (int s = 0; s < 32; ++s) {
+      if (euclid) {
+        const float t = a[d + s] - b[d + s];
+        acc[s] = std::fma(t, t, acc[s]);
+      } else {
+        acc[s] = std::fma(a[d + s], b[d + s], acc[s]);
+      }
+    }
+  }
+  float v8[8];
+  for (int l = 0; l < 8; ++l) v8[l] = (acc[l] + acc[8 + l]) + (acc[16 + l] + acc[24 + l]);
+  float v4[4];
+  for (int l = 0; l < 4; ++l) v4[l] = v8[l] + v8[4 + l];
+  float r = (v4[0] + v4[1]) + (v4[2] + v4[3]);
+  for (; d < p; ++d) {
+    if (euclid) {
+      const float t = a[d] - b[d];
+      r = std::fma(t, t, r);
+    } else {
+      r = std::fma(a[d], b[d], r);
+    }
+  }
+  return euclid ? r : -r;
+}
+
+float distance_of(const float* a, const float* b, int p, int metric) {
+  return ref_order_reduce(a, b, p, metric == 2);
+}
+
+// Row preparation for cosine search. variant 0 = core.cpp:159-193
+// (make_search_matrix: long double sum of squares, double scale); variant 1 =
+// native_knn.cpp:1005-1035 (prepare_native_matrix: float sum, float scale).
+std::vector<float> prepare_rows(const float* x, int n, int p, int metric, int variant) {
+  std::vector<float> out(x, x + static_cast<size_t>(n) * p);
+  if (metric != 0) return out;
+  for (int r = 0; r < n; ++r) {
+    const float* row = x + static_cast<size_t>(r) * p;
+    float* dst = out.data() + static_cast<size_t>(r) * p;
+    if (variant == 0) {
+      long double ss = 0.0L;
+      for (int j = 0; j < p; ++j) ss += static_cast<long double>(row[j]) * static_cast<long double>(row[j]);
+      const double nrm = std::sqrt(static_cast<double>(ss));
+      const double scale = nrm > 0.0 && std::isfinite(nrm) ? 1.0 / nrm : 1.0;
+      for (int j = 0; j < p; ++j) dst[j] = static_cast<float>(static_cast<double>(row[j]) * scale);
+    } else {
+      float ss = 0.0f;
+      for (int j = 0; j < p; ++j) ss += row[j] * row[j];
+      const float scale = ss > 0.0f && std::isfinite(ss) ? 1.0f / std::sqrt(ss) : 1.0f;
+      for (int j = 0; j < p; ++j) dst[j] = row[j] * scale;
+    }
+  }
+  return out;
+}
+
+// Exact k nearest neighbours, ascending (distance, index); exclude[q] (if given)
+// removes one training index per query. Width-k output, -1 / +inf padded beyond
+// min(k, available), as the reference search reports (native_knn.cpp:218-266).
+void exact_knn(const float* train, int nt, const float* query, int nq, int p, int k,
+               int metric, const int32_t* exclude, int32_t* out_idx, float* out_dist) {
+  std::vector<std::pair<float, int>> cand;
+  cand.reserve(static_cast<size_t>(nt));
+  for (int q = 0; q < nq; ++q) {
+    cand.clear();
+    const int ex = exclude ? exclude[q] : -1;
+    for (int t = 0; t < nt; ++t) {
+      if (t == ex) continue;
+      cand.emplace_back(distance_of(query + static_cast<size_t>(q) * p,
+                                    train + static_cast<size_t>(t) * p, p, metric), t);
+    }
+    const int kk = std::min<int>(k, static_cast<int>(cand.size()));
+    std::partial_sort(cand.begin(), cand.begin() + kk, cand.end(),
+                      [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+                        return a.first < b.first || (a.first == b.first && a.second < b.second);
+                      });
+    for (int j = 0; j < k; ++j) {
+      const size_t o = static_cast<size_t>(q) * k + j;
+      out_idx[o] = j < kk ? cand[j].second : -1;
+      out_dist[o] = j < kk ? cand[j].first : std::numeric_limits<float>::infinity();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-fold kNN tables (core.cpp:463-584 with an exact search in place of HNSW,
+// flattened as core.cpp:311-341): row i of a fold's validation set gets the
+// min(k, |train|) nearest training rows of that fold, as landmark-row ids, with
+// score = -internal distance (native_knn.cpp:1153-1155).
+// ---------------------------------------------------------------------------
+struct KnnTables {
+  IVec folds;
+  int width = 0;
+  IVec nbr;                 // [n * width]
+  std::vector<float> score; // [n * width]
+};
+
+KnnTables precompute_tables(const float* x, int n, int p, const IVec& folds, int k, int metric) {
+  if (k < 1) throw std::invalid_argument("KNNOptions::k must be positive.");
+  KnnTables t;
+  t.folds = folds;
+  t.width = k;
+  t.nbr.assign(static_cast<size_t>(n) * k, -1);
+  t.score.assign(static_cast<size_t>(n) * k, -std::numeric_limits<float>::infinity());
+  const std::vector<float> xs = prepare_rows(x, n, p, metric, 0);
+  for (int f : sorted_unique(folds)) {
+    IVec val, tr;
+    for (int i = 0; i < n; ++i) (folds[i] == f ? val : tr).push_back(i);
+    if (tr.empty()) throw std::runtime_error("KNN fold has no training samples.");
+    const int kf = std::min<int>(k, static_cast<int>(tr.size()));
+    std::vector<float> trx(tr.size() * p), vx(val.size() * p);
+    for (size_t i = 0; i < tr.size(); ++i) std::memcpy(&trx[i * p], &xs[static_cast<size_t>(tr[i]) * p], sizeof(float) * p);
+    for (size_t i = 0; i < val.size(); ++i) std::memcpy(&vx[i * p], &xs[static_cast<size_t>(val[i]) * p], sizeof(float) * p);
+    std::vector<int32_t> idx(val.size() * kf);
+    std::vector<float> dist(val.size() * kf);
+    exact_knn(trx.data(), static_cast<int>(tr.size()), vx.data(), static_cast<int>(val.size()), p, kf,
+              metric, nullptr, idx.data(), dist.data());
+    for (size_t q = 0; q < val.size(); ++q) {
+      for (int j = 0; j < kf; ++j) {
+        const int id = idx[q * kf + j];
+        const size_t o = static_cast<size_t>(val[q]) * k + j;
+        if (id >= 0) t.nbr[o] = tr[id];
+        t.score[o] = -dist[q * kf + j];
+      }
+    }
+  }
+  return t;
+}
+
+// Majority label with ties to the smallest label (core.cpp:292-303, 810-824).
+int majority_smallest(const std::map<int, int>& counts) {
+  int best = counts.empty() ? 0 : counts.begin()->first;
+  int best_n = -1;
+  for (const auto& kv : counts) {
+    if (kv.second > best_n) { best = kv.first; best_n = kv.second; }
+  }
+  return best;
+}
+
+// The kNN CV vote (core.cpp:784-914 with the fallbacks of 258-309): per validation
+// row, labels of its neighbours are tallied; the winner has the most votes, then
+// the largest sum of scores (double, neighbour-rank order), then the smallest
+// label. Rows with no neighbour take the majority label of their training fold.
+IVec vote_predict(const KnnTables& t, const IVec& labels) {
+  const int n = static_cast<int>(labels.size());
+  IVec pred(static_cast<size_t>(n), 0);
+  std::map<int, int> all;
+  for (int l : labels) ++all[l];
+  const int global_fallback = majority_smallest(all);
+  std::vector<int> empty_rows;
+  for (int i = 0; i < n; ++i) {
+    std::map<int, std::pair<int, double>> tally;
+    for (int j = 0; j < t.width; ++j) {
+      const int nb = t.nbr[static_cast<size_t>(i) * t.width + j];
+      if (nb < 0) continue;
+      auto& e = tally[labels[nb]];
+      e.first += 1;
+      e.second += static_cast<double>(t.score[static_cast<size_t>(i) * t.width + j]);
+    }
+    if (tally.empty()) {
+      pred[i] = global_fallback;
+      empty_rows.push_back(i);
+      continue;
+    }
+    int best = 0, best_n = -1;
+    double best_s = -std::numeric_limits<double>::infinity();
+    for (const auto& kv : tally) {
+      const int c = kv.second.first;
+      const double s = kv.second.second;
+      if (c > best_n || (c == best_n && s > best_s) || (c == best_n && s == best_s && kv.first < best)) {
+        best = kv.first; best_n = c; best_s = s;
+      }
+    }
+    pred[i] = best;
+  }
+  if (!empty_rows.empty()) {
+    for (int f : sorted_unique(t.folds)) {
+      std::map<int, int> train_counts;
+      for (const auto& kv : all) train_counts[kv.first] = kv.second;
+      for (int i = 0; i < n; ++i) if (t.folds[i] == f) --train_counts[labels[i]];
+      const int fb = majority_smallest(train_counts);
+      for (int r : empty_rows) if (t.folds[r] == f) pred[r] = fb;
+    }
+  }
+  return pred;
+}
+
+// ---------------------------------------------------------------------------
+// Objective (core.cpp:916-972). Class frequencies are visited in the iteration
+// order of an unordered_map filled in sample order after reserve(n), as the
+// reference does, so the double sum of p^2 rounds identically.
+// ---------------------------------------------------------------------------
+struct Flags {
+  int cycles;
+  bool knn;
+  std::uint64_t seed;
+  bool shake, evolutionary, guarded, absorption, coarsening, adaptive;
+  bool guidance, policy_adaptive, transition, stochastic, diversity, pls_coarsening, fragmentation, reject_single;
+};
+
+Flags flags_of(const oracle_core_options* o) {
+  Flags f;
+  f.cycles = o->cycles;
+  f.knn = o->classifier == 1;
+  f.seed = o->seed;
+  f.shake = o->shake; f.evolutionary = o->evolutionary_search; f.guarded = o->guarded_diversity;
+  f.absorption = o->many_to_one_absorption; f.coarsening = o->auto_class_coarsening;
+  f.adaptive = o->adaptive_proposal_size;
+  f.guidance = o->prediction_guidance; f.policy_adaptive = o->policy_adaptive_proposal_size;
+  f.transition = o->transition_proposal; f.stochastic = o->stochastic_acceptance;
+  f.diversity = o->diversity_multiplier; f.pls_coarsening = o->pls_transition_coarsening;
+  f.fragmentation = o->pls_fragmentation_penalty; f.reject_single = o->reject_single_class;
+  return f;
+}
+
+double objective(const IVec& labels, double acc, const Flags& f) {
+  const bool diversity = f.guarded && f.diversity;
+  const bool fragmentation = f.coarsening && !f.knn && f.fragmentation;
+  const bool reject_single = f.guarded && f.reject_single;
+  if (!diversity && !fragmentation && !reject_single) return acc;
+  if (labels.empty()) return acc;
+  std::unordered_map<int, int> counts;
+  counts.reserve(labels.size());
+  for (int l : labels) counts[l]++;
+  const int k = static_cast<int>(counts.size());
+  if (k < 1) return acc;
+  if (reject_single && k == 1) return -std::numeric_limits<double>::infinity();
+  const double n = static_cast<double>(labels.size());
+  double score = acc;
+  if (diversity) {
+    double same = 0.0;
+    for (const auto& kv : counts) {
+      const double q = static_cast<double>(kv.second) / n;
+      same += q * q;
+    }
+    score *= std::sqrt(std::max(0.0, 1.0 - same));
+  }
+  if (fragmentation && k > 1) {
+    double h = 0.0;
+    for (const auto& kv : counts) {
+      const double q = static_cast<double>(kv.second) / n;
+      if (q > 0.0) h -= q * std::log(q);
+    }
+    const double hmax = n > 1.0 ? std::log(n) : 0.0;
+    const double code_cost = hmax > 0.0 ? std::clamp(h / hmax, 0.0, 1.0) : 0.0;
+    const double keff = std::exp(h);
+    const double frag = std::max(0.0, std::log(static_cast<double>(k) / std::max(1.0, keff)));
+    const double frag_cost = frag / (1.0 + frag);
+    score -= (1.0 - acc) * std::max(code_cost, frag_cost);
+  }
+  return score;
+}
+
+double accuracy(const IVec& truth, const IVec& pred) {
+  if (truth.empty()) return 0.0;
+  size_t ok = 0;
+  for (size_t i = 0; i < truth.size(); ++i) ok += truth[i] == pred[i] ? 1u : 0u;
+  return static_cast<double>(ok) / static_cast<double>(truth.size());
+}
+
+// ---------------------------------------------------------------------------
+// Transition proposals (core.cpp:974-1226).
+// ---------------------------------------------------------------------------
+struct TransitionStats {
+  std::map<int, int> sizes;
+  std::map<int, int> movable;
+  std::map<int, std::map<int, int>> flows;  // label -> (previous prediction -> rows)
+};
+
+TransitionStats transition_stats(const IVec& labels, const IVec& prev_pred, const IVec& fixed) {
+  TransitionStats s;
+  for (size_t i = 0; i < labels.size(); ++i) {
+    ++s.sizes[labels[i]];
+    ++s.flows[labels[i]][prev_pred[i]];
+    if (fixed.empty() || fixed[i] != 1) ++s.movable[labels[i]];
+  }
+  return s;
+}
+
+// core.cpp:995-1108: merge unstable small classes into the class most of their
+// members were predicted as, with a budget set by the class fragmentation.
+bool coarsen_classes(IVec& labels, const IVec& fixed, const TransitionStats& st, std::mt19937_64& rng) {
+  const int classes = static_cast<int>(st.sizes.size());
+  if (classes <= 2) return false;
+  const double n = static_cast<double>(labels.size());
+  double h = 0.0;
+  for (const auto& kv : st.sizes) {
+    const double q = static_cast<double>(kv.second) / n;
+    if (q > 0.0) h -= q * std::log(q);
+  }
+  const double keff = std::exp(h);
+  const double frag = std::max(0.0, std::log(static_cast<double>(classes) / std::max(1.0, keff)));
+  double flow_entropy = 0.0;
+  struct Cand { int source; int dest; double score; };
+  std::vector<Cand> cands;
+  for (const auto& kv : st.sizes) {
+    const int src = kv.first;
+    const int src_n = kv.second;
+    const auto mv = st.movable.find(src);
+    const int movable = mv == st.movable.end() ? 0 : mv->second;
+    if (movable <= 0) continue;
+    int dst = src, dst_n = src_n, best = 0;
+    double row_h = 0.0;
+    const auto& row = st.flows.at(src);
+    for (const auto& e : row) {
+      const double q = static_cast<double>(e.second) / static_cast<double>(src_n);
+      if (q > 0.0) row_h -= q * std::log(q);
+      const auto cs = st.sizes.find(e.first);
+      if (e.first != src && cs != st.sizes.end() &&
+          (e.second > best || (e.second == best && cs->second > dst_n) ||
+           (e.second == best && cs->second == dst_n && e.first < dst))) {
+        dst = e.first; dst_n = cs->second; best = e.second;
+      }
+    }
+    flow_entropy += (static_cast<double>(src_n) / n) * row_h;
+    if (dst == src || best == 0) continue;
+    const double stay = static_cast<double>(row.count(src) ? row.at(src) : 0) / static_cast<double>(src_n);
+    cands.push_back({src, dst, (1.0 / std::sqrt(static_cast<double>(movable))) * (1.0 - stay)});
+  }
+  if (cands.empty()) return false;
+  const double temp = frag / (frag + flow_entropy + 1.0);
+  const double redundant = std::max(0.0, static_cast<double>(classes) - keff);
+  int budget = static_cast<int>(std::ceil(temp * redundant));
+  if (budget < 1 && frag > 0.0) budget = 1;
+  budget = std::min(budget, classes - 2);
+  if (budget <= 0) return false;
+  std::shuffle(cands.begin(), cands.end(), rng);
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+  std::unordered_map<int, int> remap;
+  remap.reserve(st.sizes.size());
+  for (const auto& kv : st.sizes) remap[kv.first] = kv.first;
+  int merged = 0;
+  for (const Cand& c : cands) {
+    if (merged >= budget) break;
+    if (remap[c.source] != c.source) continue;
+    int d = c.dest;
+    while (remap[d] != d) d = remap[d];
+    if (d == c.source) continue;
+    remap[c.source] = d;
+    ++merged;
+  }
+  if (merged == 0) return false;
+  bool changed = false;
+  for (size_t i = 0; i < labels.size(); ++i) {
+    if (!fixed.empty() && fixed[i] == 1) continue;
+    int d = labels[i];
+    while (remap[d] != d) d = remap[d];
+    if (d != labels[i]) { labels[i] = d; changed = true; }
+  }
+  return changed;
+}
+
+// core.cpp:1110-1226: each class votes for the class it flows into most beyond
+// chance; one target (weighted by its surplus) absorbs all its sources.
+bool absorb_many_to_one(IVec& labels, const IVec& fixed, const TransitionStats& st, std::mt19937_64& rng) {
+  const int classes = static_cast<int>(st.sizes.size());
+  if (classes <= 2) return false;
+  const double n = static_cast<double>(labels.size());
+  struct Src { int source; double surplus; };
+  struct Target { int target = 0; double weight = 0.0; std::vector<Src> sources; };
+  std::map<int, Target> by_target;
+  for (const auto& kv : st.sizes) {
+    const int src = kv.first;
+    const int src_n = kv.second;
+    const auto mv = st.movable.find(src);
+    if (mv == st.movable.end() || mv->second <= 0) continue;
+    const auto& row = st.flows.at(src);
+    const int stay = row.count(src) ? row.at(src) : 0;
+    int tgt = src, best = 0, tgt_n = src_n;
+    double surplus_best = 0.0;
+    for (const auto& e : row) {
+      if (e.first == src) continue;
+      const auto cs = st.sizes.find(e.first);
+      if (cs == st.sizes.end()) continue;
+      const double expected = static_cast<double>(src_n) * static_cast<double>(cs->second) / std::max(1.0, n);
+      const double surplus = static_cast<double>(e.second) - expected;
+      if (surplus > surplus_best || (surplus == surplus_best && e.second > best) ||
+          (surplus == surplus_best && e.second == best && cs->second > tgt_n) ||
+          (surplus == surplus_best && e.second == best && cs->second == tgt_n && e.first < tgt)) {
+        tgt = e.first; tgt_n = cs->second; best = e.second; surplus_best = surplus;
+      }
+    }
+    if (tgt == src || surplus_best <= 0.0 || best <= stay) continue;
+    Target& t = by_target[tgt];
+    t.target = tgt;
+    t.weight += surplus_best;
+    t.sources.push_back({src, surplus_best});
+  }
+  if (by_target.empty()) return false;
+  std::vector<Target> multi, single;
+  for (auto& kv : by_target) {
+    std::sort(kv.second.sources.begin(), kv.second.sources.end(), [](const Src& a, const Src& b) {
+      if (a.surplus != b.surplus) return a.surplus > b.surplus;
+      return a.source < b.source;
+    });
+    (kv.second.sources.size() > 1 ? multi : single).push_back(std::move(kv.second));
+  }
+  std::vector<Target>& moves = multi.empty() ? single : multi;
+  if (moves.empty()) return false;
+  std::vector<double> w;
+  for (const Target& t : moves) w.push_back(std::max(t.weight, std::numeric_limits<double>::min()));
+  std::discrete_distribution<int> pick(w.begin(), w.end());
+  const Target& chosen = moves[static_cast<size_t>(pick(rng))];
+  std::unordered_map<int, int> remap;
+  for (const Src& s : chosen.sources) if (s.source != chosen.target) remap[s.source] = chosen.target;
+  if (remap.empty()) return false;
+  IVec proposed = labels;
+  bool changed = false;
+  for (size_t i = 0; i < proposed.size(); ++i) {
+    if (!fixed.empty() && fixed[i] == 1) continue;
+    const auto it = remap.find(proposed[i]);
+    if (it == remap.end()) continue;
+    proposed[i] = it->second;
+    changed = true;
+  }
+  if (!changed) return false;
+  const int after = static_cast<int>(sorted_unique(proposed).size());
+  if (after <= 1 || after >= classes) return false;
+  labels.swap(proposed);
+  return true;
+}
+
+// core.cpp:1228-1254: how many groups to relabel this cycle (smoothstep-annealed
+// upper bound when adaptive).
+int proposal_count(int groups, int cycle, int cycles, bool adaptive, std::mt19937_64& rng) {
+  if (groups <= 1) return std::max(0, groups);
+  if (!adaptive || cycles <= 1) {
+    std::uniform_int_distribution<int> d(1, groups);
+    return d(rng);
+  }
+  const double t = std::clamp(static_cast<double>(cycle + 1) / static_cast<double>(cycles + 1), 0.0, 1.0);
+  const double smooth = t * t * (3.0 - 2.0 * t);
+  const double temp = std::clamp(1.0 - smooth, 0.0, 1.0);
+  const int hi = std::max(1, 1 + static_cast<int>(std::floor(static_cast<double>(groups - 1) * temp)));
+  std::uniform_int_distribution<int> d(1, hi);
+  return d(rng);
+}
+
+// ---------------------------------------------------------------------------
+// Label evolution (core.cpp:1268-1549, maximize_core).
+// ---------------------------------------------------------------------------
+template <class Predict>
+void evolve(int n, const IVec& initial, const IVec& constrain, const IVec& fixed_in, const Flags& f,
+            Predict&& predict, oracle_core_result* out) {
+  if (f.cycles < 0) throw std::invalid_argument("Core cycles must be non-negative.");
+  const int T = f.cycles;
+  IVec clbest = initial, clbest_dirty = initial;
+  std::vector<double> vacc(T, -1.0), vscore(T, -1.0);
+  IVec psize(T, 0), active(T, 0);
+  std::vector<uint8_t> acc_flag(T, 0), impr(T, 0), temp_flag(T, 0);
+  int counters[ORACLE_N_COUNTERS] = {0};
+  enum { kCompleted, kProposals, kBest, kCurAcc, kStochAtt, kStochAcc, kCurRej, kCoarse, kAbsorb,
+         kTransAtt, kTransAcc, kM2OAtt, kM2OAcc, kPlsAtt, kPlsAcc, kCvEval, kSuccess };
+  const IVec fixed = fixed_in.empty() ? IVec(static_cast<size_t>(n), 0) : fixed_in;
+
+  // Groups: one per row without constraints, else one per distinct constraint
+  // value in ascending order, members in row order.
+  const bool singletons = constrain.empty();
+  std::vector<IVec> members;
+  IVec groups;
+  if (singletons) {
+    groups.resize(n);
+    std::iota(groups.begin(), groups.end(), 0);
+  } else {
+    std::map<int, IVec> by_key;
+    for (int i = 0; i < n; ++i) by_key[constrain[i]].push_back(i);
+    for (auto& kv : by_key) {
+      groups.push_back(static_cast<int>(members.size()));
+      members.push_back(kv.second);
+    }
+  }
+
+  std::mt19937_64 rng(f.seed);
+  IVec predbest = predict(clbest);
+  counters[kCvEval] = 1;
+  double accbest = f.shake ? 0.0 : accuracy(clbest, predbest);
+  double scorebest = f.shake ? -std::numeric_limits<double>::infinity() : objective(clbest, accbest, f);
+  IVec cur = clbest, curpred = predbest;
+  double cur_acc = accbest, cur_score = scorebest;
+  bool success = false;
+
+  for (int cycle = 0; cycle < T; ++cycle) {
+    IVec cl = f.evolutionary ? cur : clbest;
+    const IVec cl_dirty = cl;
+    const IVec& guide = f.evolutionary ? curpred : predbest;
+    if (!groups.empty()) {
+      const int take = proposal_count(static_cast<int>(groups.size()), cycle, T,
+                                      f.adaptive && f.policy_adaptive, rng);
+      psize[cycle] = take;
+      IVec order = groups;
+      std::shuffle(order.begin(), order.end(), rng);
+      order.resize(static_cast<size_t>(take));
+      IVec emp_labels, emp_counts;
+      if (!f.guidance) {
+        std::map<int, int> tally;
+        for (int l : cl) ++tally[l];
+        for (const auto& kv : tally) { emp_labels.push_back(kv.first); emp_counts.push_back(kv.second); }
+      }
+      for (int g : order) {
+        IVec elig;
+        if (singletons) {
+          if (fixed[g] != 1) elig.push_back(g);
+        } else {
+          for (int r : members[g]) if (fixed[r] != 1) elig.push_back(r);
+        }
+        if (elig.empty()) continue;
+        IVec cand, cnt;
+        if (f.guidance) {
+          std::map<int, int> tally;
+          for (int r : elig) ++tally[guide[r]];
+          for (const auto& kv : tally) { cand.push_back(kv.first); cnt.push_back(kv.second); }
+        } else {
+          cand = emp_labels;
+          cnt = emp_counts;
+        }
+        if (cand.empty()) continue;
+        std::discrete_distribution<int> pick(cnt.begin(), cnt.end());
+        const int repl = cand[static_cast<size_t>(pick(rng))];
+        for (int r : elig) cl[r] = repl;
+      }
+    }
+
+    bool coarsened = false, absorbed = false;
+    const bool try_coarsen = f.transition && f.coarsening && f.pls_coarsening;
+    const bool try_absorb = f.transition && f.absorption;
+    TransitionStats st;
+    if (try_coarsen || try_absorb) {
+      ++counters[kTransAtt];
+      st = transition_stats(cl, guide, fixed);
+    }
+    if (try_coarsen) {
+      ++counters[kPlsAtt];
+      coarsened = coarsen_classes(cl, fixed, st, rng);
+      if (coarsened) ++counters[kCoarse];
+    }
+    if (try_absorb) {
+      ++counters[kM2OAtt];
+      if (coarsened) {
+        const TransitionStats st2 = transition_stats(cl, guide, fixed);
+        absorbed = absorb_many_to_one(cl, fixed, st2, rng);
+      } else {
+        absorbed = absorb_many_to_one(cl, fixed, st, rng);
+      }
+      if (absorbed) ++counters[kAbsorb];
+    }
+
+    const IVec pred = predict(cl);
+    ++counters[kCvEval];
+    ++counters[kProposals];
+    const double acc = accuracy(cl, pred);
+    const double score = objective(cl, acc, f);
+    const bool improves = score > scorebest;
+    if (improves) {
+      predbest = pred;
+      clbest = cl;
+      clbest_dirty = cl_dirty;
+      accbest = acc;
+      scorebest = score;
+      ++counters[kBest];
+    }
+    bool accepted = improves, improving = improves, stochastic = false;
+    if (f.evolutionary) {
+      const double cooling = 1.0 - static_cast<double>(cycle + 1) / static_cast<double>(std::max(1, T));
+      const double temperature = std::max(1.0e-9, 0.10 * std::max(0.0, 1.0 - cur_acc) * cooling);
+      bool take = score >= cur_score;
+      improving = take;
+      if (!take && f.stochastic && temperature > 1.0e-9) {
+        ++counters[kStochAtt];
+        std::uniform_real_distribution<double> u(0.0, 1.0);
+        take = u(rng) < std::exp((score - cur_score) / temperature);
+        stochastic = take;
+      }
+      if (take) {
+        cur = cl;
+        curpred = pred;
+        cur_acc = acc;
+        cur_score = score;
+        ++counters[kCurAcc];
+        if (stochastic) ++counters[kStochAcc];
+      } else {
+        ++counters[kCurRej];
+      }
+      accepted = take;
+    }
+    acc_flag[cycle] = accepted ? 1 : 0;
+    impr[cycle] = accepted && improving ? 1 : 0;
+    temp_flag[cycle] = stochastic ? 1 : 0;
+    if (accepted && (coarsened || absorbed)) {
+      ++counters[kTransAcc];
+      if (coarsened) ++counters[kPlsAcc];
+      if (absorbed) ++counters[kM2OAcc];
+    }
+    active[cycle] = static_cast<int>(sorted_unique(f.evolutionary ? cur : clbest).size());
+    vacc[cycle] = accbest;
+    vscore[cycle] = scorebest;
+    counters[kCompleted] = cycle + 1;
+    if (acc == 1.0 && std::isfinite(score)) success = true;
+  }
+  counters[kSuccess] = success ? 1 : 0;
+
+  std::memcpy(out->clbest, clbest.data(), sizeof(int32_t) * n);
+  std::memcpy(out->clbest_dirty, clbest_dirty.data(), sizeof(int32_t) * n);
+  std::memcpy(out->cvpredbest, predbest.data(), sizeof(int32_t) * n);
+  for (int c = 0; c < T; ++c) {
+    out->vect_acc[c] = vacc[c];
+    out->vect_score[c] = vscore[c];
+    out->proposal_size[c] = psize[c];
+    out->active_classes[c] = active[c];
+    out->accepted[c] = acc_flag[c];
+    out->improving_acceptance[c] = impr[c];
+    out->temperature_acceptance[c] = temp_flag[c];
+  }
+  std::memcpy(out->counters, counters, sizeof(counters));
+  out->accbest = accbest;
+  out->scorebest = scorebest;
+}
+
+// ---------------------------------------------------------------------------
+// PLS-LDA (plscv.cpp). Templated on the working precision T: T = float mirrors
+// the reference's float32 arithmetic operation for operation (its accumulators
+// that are double stay double), T = double is the FP64 oracle.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Mat {
+  int r = 0, c = 0;
+  std::vector<T> v;
+  Mat() = default;
+  Mat(int rr, int cc) : r(rr), c(cc), v(static_cast<size_t>(rr) * cc, T(0)) {}
+  T& operator()(int i, int j) { return v[static_cast<size_t>(i) * c + j]; }
+  T operator()(int i, int j) const { return v[static_cast<size_t>(i) * c + j]; }
+};
+
+// Per-training-set centring and scaling (plscv.cpp:669-698). The scale
+// accumulator starts at 1, as in the reference.
+template <class T>
+void center_scale(const float* x, int p, const IVec& rows, bool center, bool scale,
+                  std::vector<T>& mean, std::vector<T>& sd) {
+  mean.assign(p, T(0));
+  sd.assign(p, T(1));
+  if (center) {
+    for (int r : rows) for (int j = 0; j < p; ++j) mean[j] += static_cast<T>(x[static_cast<size_t>(r) * p + j]);
+    const T inv = rows.empty() ? T(0) : T(1) / static_cast<T>(rows.size());
+    for (T& m : mean) m *= inv;
+  }
+  if (scale) {
+    for (int r : rows) for (int j = 0; j < p; ++j) {
+      const T d = static_cast<T>(x[static_cast<size_t>(r) * p + j]) - mean[j];
+      sd[j] += d * d;
+    }
+    for (int j = 0; j < p; ++j) {
+      const T s = std::sqrt(sd[j] / std::max(T(1), static_cast<T>(rows.size() - 1)));
+      sd[j] = s > T(0) && std::isfinite(s) ? s : T(1);
+    }
+  }
+}
+
+template <class T>
+Mat<T> standardize(const float* x, int p, const IVec& rows, const std::vector<T>& mean, const std::vector<T>& sd) {
+  Mat<T> out(static_cast<int>(rows.size()), p);
+  for (int i = 0; i < out.r; ++i)
+    for (int j = 0; j < p; ++j)
+      out(i, j) = (static_cast<T>(x[static_cast<size_t>(rows[i]) * p + j]) - mean[j]) / sd[j];
+  return out;
+}
+
+template <class T>
+Mat<T> gram_of(const Mat<T>& x) {
+  Mat<T> g(x.c, x.c);
+  for (int a = 0; a < x.c; ++a)
+    for (int b = 0; b <= a; ++b) {
+      T v = T(0);
+      for (int i = 0; i < x.r; ++i) v += x(i, a) * x(i, b);
+      g(a, b) = v;
+      g(b, a) = v;
+    }
+  return g;
+}
+
+// dot / mat-vec helpers with double accumulation (plscv.cpp:774-814).
+template <class T>
+T dotd(const std::vector<T>& a, const std::vector<T>& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  return static_cast<T>(s);
+}
+template <class T>
+T norm2(const std::vector<T>& a) { return std::sqrt(std::max(T(0), dotd(a, a))); }
+template <class T>
+T power_norm(const std::vector<T>& a) {
+  double s = 0.0;
+  for (T v : a) s += static_cast<double>(v) * static_cast<double>(v);
+  if (!std::isfinite(s) || s < 0.0) return std::numeric_limits<T>::infinity();
+  if (std::is_same<T, float>::value) {
+    if (s <= static_cast<double>(std::numeric_limits<float>::max())) return std::sqrt(static_cast<T>(s));
+    return static_cast<T>(std::sqrt(s));
+  }
+  return static_cast<T>(std::sqrt(s));
+}
+template <class T>
+std::vector<T> matvec(const Mat<T>& a, const std::vector<T>& x) {
+  std::vector<T> o(a.r, T(0));
+  for (int i = 0; i < a.r; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < a.c; ++j) s += static_cast<double>(a(i, j)) * static_cast<double>(x[j]);
+    o[i] = static_cast<T>(s);
+  }
+  return o;
+}
+template <class T>
+std::vector<T> tmatvec(const Mat<T>& a, const std::vector<T>& x) {
+  std::vector<T> o(a.c, T(0));
+  for (int j = 0; j < a.c; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < a.r; ++i) s += static_cast<double>(a(i, j)) * static_cast<double>(x[i]);
+    o[j] = static_cast<T>(s);
+  }
+  return o;
+}
+
+// S = X'(Y - ybar) through per-class column sums (plscv.cpp:700-746).
+template <class T>
+Mat<T> label_crossprod(const Mat<T>& x, const IVec& y, const IVec& classes) {
+  const int p = x.c, m = static_cast<int>(classes.size());
+  std::vector<T> s(static_cast<size_t>(p) * m, T(0)), xsum(p, T(0)), counts(m, T(0));
+  IVec code(y.size());
+  for (size_t i = 0; i < y.size(); ++i) {
+    code[i] = static_cast<int>(std::lower_bound(classes.begin(), classes.end(), y[i]) - classes.begin());
+    counts[code[i]] += T(1);
+  }
+  for (int i = 0; i < x.r; ++i)
+    for (int j = 0; j < p; ++j) {
+      const T v = x(i, j);
+      xsum[j] += v;
+      s[j + static_cast<size_t>(code[i]) * p] += v;
+    }
+  const T inv = x.r > 0 ? T(1) / static_cast<T>(x.r) : T(0);
+  for (int c = 0; c < m; ++c) {
+    const T ym = counts[c] * inv;
+    for (int j = 0; j < p; ++j) s[j + static_cast<size_t>(c) * p] -= ym * xsum[j];
+  }
+  Mat<T> out(p, m);
+  for (int c = 0; c < m; ++c) for (int j = 0; j < p; ++j) out(j, c) = s[j + static_cast<size_t>(c) * p];
+  return out;
+}
+
+struct Degenerate : std::runtime_error { Degenerate() : std::runtime_error("degenerate PLS fit") {} };
+
+// Power-refresh SIMPLS (plscv.cpp:816-955): per component, the previous weight
+// (or N(0,1) from mt19937(1) for the first) is refreshed by two S S' power steps,
+// normalised, scaled to unit score norm, and the loading is Gram-Schmidt
+// orthogonalised against earlier loadings before S is deflated.
+template <class T>
+Mat<T> simpls_power(const Mat<T>& x, Mat<T> s, int max_components, const Mat<T>* gram) {
+  const int rank_cap = std::min({max_components, x.c, std::max(1, x.r - 1)});
+  const Mat<T> s0 = s;
+  Mat<T> w(x.c, rank_cap), vb(x.c, rank_cap);
+  std::vector<T> prev;
+  int fitted = 0;
+  auto refresh = [](const Mat<T>& cp, std::vector<T>& cand) {
+    for (int it = 0; it < 2; ++it) {
+      const std::vector<T> r = tmatvec(cp, cand);
+      cand = matvec(cp, r);
+    }
+  };
+  for (int a = 0; a < rank_cap; ++a) {
+    std::vector<T> wt(x.c, T(0));
+    if (!prev.empty()) {
+      wt = prev;
+    } else {
+      std::mt19937 gen(1u);
+      std::normal_distribution<float> nd(0.0f, 1.0f);
+      for (T& v : wt) v = static_cast<T>(nd(gen));
+    }
+    refresh(s, wt);
+    T nrm = power_norm(wt);
+    if (prev.empty() && (!std::isfinite(nrm) || nrm <= T(1.0e-10f))) {
+      int strongest = -1;
+      double strongest_sq = 0.0;
+      for (int c = 0; c < s.c; ++c) {
+        double sq = 0.0;
+        for (int j = 0; j < s.r; ++j) sq += static_cast<double>(s(j, c)) * static_cast<double>(s(j, c));
+        if (std::isfinite(sq) && sq > strongest_sq) { strongest_sq = sq; strongest = c; }
+      }
+      if (strongest >= 0 && strongest_sq > 1.0e-20) {
+        for (int j = 0; j < s.r; ++j) wt[j] = s(j, strongest);
+        refresh(s, wt);
+        nrm = power_norm(wt);
+      }
+    }
+    if (!std::isfinite(nrm) || nrm <= T(1.0e-10f)) break;
+    for (T& v : wt) v /= nrm;
+    std::vector<T> load;
+    if (gram != nullptr) {
+      load = matvec(*gram, wt);
+      const T tt = dotd(wt, load);
+      if (!std::isfinite(tt) || tt <= T(1.0e-20f)) break;
+      const T inv = T(1) / std::sqrt(tt);
+      for (T& v : wt) v *= inv;
+      for (T& v : load) v *= inv;
+    } else {
+      std::vector<T> t(x.r, T(0));
+      for (int i = 0; i < x.r; ++i) {
+        T v = T(0);
+        for (int j = 0; j < x.c; ++j) v += x(i, j) * wt[j];
+        t[i] = v;
+      }
+      const T tn = norm2(t);
+      if (!std::isfinite(tn) || tn <= T(1.0e-10f)) break;
+      const T inv = T(1) / tn;
+      for (T& v : t) v *= inv;
+      for (T& v : wt) v *= inv;
+      load = tmatvec(x, t);
+    }
+    prev = wt;
+    for (int b = 0; b < a; ++b) {
+      T proj = T(0);
+      for (int j = 0; j < x.c; ++j) proj += vb(j, b) * load[j];
+      for (int j = 0; j < x.c; ++j) load[j] -= proj * vb(j, b);
+    }
+    const T ln = norm2(load);
+    if (!std::isfinite(ln) || ln <= T(1.0e-10f)) break;
+    for (T& v : load) v /= ln;
+    for (int j = 0; j < x.c; ++j) { w(j, a) = wt[j]; vb(j, a) = load[j]; }
+    std::vector<T> vs(s.c, T(0));
+    for (int c = 0; c < s.c; ++c) {
+      T v = T(0);
+      for (int j = 0; j < s.r; ++j) v += load[j] * s(j, c);
+      vs[c] = v;
+    }
+    for (int j = 0; j < s.r; ++j) for (int c = 0; c < s.c; ++c) s(j, c) -= load[j] * vs[c];
+    fitted = a + 1;
+  }
+  (void)s0;
+  if (fitted < 1) throw Degenerate();
+  if (fitted == rank_cap) return w;
+  Mat<T> out(x.c, fitted);
+  for (int a = 0; a < fitted; ++a) for (int j = 0; j < x.c; ++j) out(j, a) = w(j, a);
+  return out;
+}
+
+// Cholesky solve (plscv.cpp:1379-1412).
+template <class T>
+bool chol_solve(const Mat<T>& m, const Mat<T>& rhs, Mat<T>& sol) {
+  const int n = m.r;
+  Mat<T> lo(n, n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c <= r; ++c) {
+      T v = m(r, c);
+      for (int k = 0; k < c; ++k) v -= lo(r, k) * lo(c, k);
+      if (r == c) {
+        if (!std::isfinite(v) || v <= T(0)) return false;
+        lo(r, c) = std::sqrt(v);
+      } else {
+        const T d = lo(c, c);
+        if (!std::isfinite(d) || d <= T(0)) return false;
+        lo(r, c) = v / d;
+      }
+    }
+  sol = rhs;
+  for (int col = 0; col < rhs.c; ++col) {
+    for (int r = 0; r < n; ++r) {
+      T v = sol(r, col);
+      for (int k = 0; k < r; ++k) v -= lo(r, k) * sol(k, col);
+      sol(r, col) = v / lo(r, r);
+    }
+    for (int r = n - 1; r >= 0; --r) {
+      T v = sol(r, col);
+      for (int k = r + 1; k < n; ++k) v -= lo(k, r) * sol(k, col);
+      sol(r, col) = v / lo(r, r);
+    }
+  }
+  return true;
+}
+
+template <class T>
+struct LdaFit { Mat<T> linear; std::vector<T> constant; };
+
+// LDA from score statistics with the ridge grid (plscv.cpp:1419-1487).
+template <class T>
+LdaFit<T> lda_fit(Mat<T> sums, const IVec& counts, const Mat<T>& tt, int n_train) {
+  const int C = sums.r, A = sums.c;
+  for (int c = 0; c < C; ++c) {
+    const T inv = counts[c] > 0 ? T(1) / static_cast<T>(counts[c]) : T(0);
+    for (int a = 0; a < A; ++a) sums(c, a) *= inv;
+  }
+  Mat<T> pooled(A, A);
+  const T df = static_cast<T>(std::max(1, n_train - C));
+  T trace = T(0);
+  for (int r = 0; r < A; ++r) {
+    for (int c2 = 0; c2 <= r; ++c2) {
+      T between = T(0);
+      for (int c = 0; c < C; ++c) between += static_cast<T>(counts[c]) * sums(c, r) * sums(c, c2);
+      const T cov = (tt(r, c2) - between) / df;
+      pooled(r, c2) = cov;
+      pooled(c2, r) = cov;
+    }
+    trace += pooled(r, r);
+  }
+  Mat<T> rhs(A, C);
+  for (int c = 0; c < C; ++c) for (int a = 0; a < A; ++a) rhs(a, c) = sums(c, a);
+  const T rs = std::isfinite(trace) && trace > T(0) ? trace / static_cast<T>(std::max(1, A)) : T(1);
+  const float grid[] = {1e-8f, 1e-6f, 1e-5f, 1e-4f, 1e-3f, 1e-2f};
+  Mat<T> sol;
+  bool ok = false;
+  for (float g : grid) {
+    Mat<T> cov = pooled;
+    const T lam = static_cast<T>(g) * rs;
+    for (int a = 0; a < A; ++a) cov(a, a) += lam;
+    if (chol_solve(cov, rhs, sol)) { ok = true; break; }
+  }
+  if (!ok) throw std::runtime_error("PLS-LDA Cholesky factorization failed.");
+  LdaFit<T> m{Mat<T>(C, A), std::vector<T>(C, T(0))};
+  for (int c = 0; c < C; ++c) {
+    T dm = T(0);
+    for (int a = 0; a < A; ++a) {
+      m.linear(c, a) = sol(a, c);
+      dm += sums(c, a) * m.linear(c, a);
+    }
+    const T prior = std::max(static_cast<T>(counts[c]) / static_cast<T>(std::max(1, n_train)),
+                             static_cast<T>(std::numeric_limits<float>::min()));
+    m.constant[c] = T(-0.5) * dm + std::log(prior);
+  }
+  return m;
+}
+
+template <class T>
+struct FoldOut {
+  IVec pred;                 // per validation row
+  int components = 1;
+  Mat<T> weights;            // p x rank (empty on fallback)
+  Mat<T> val_scores;         // n_val x A
+  Mat<T> discriminant;       // n_val x C
+  IVec classes;
+};
+
+// One fold of PLS-LDA CV (plscv.cpp:2340-2438 with the streamed predictor
+// 1544-1608). `gram` selects the Gram-matrix loading path of the fold cache.
+template <class T>
+FoldOut<T> pls_fold(const float* x, int p, const IVec& labels, const IVec& train, const IVec& val,
+                    const oracle_core_options* o, bool use_gram) {
+  FoldOut<T> out;
+  std::vector<T> mean, sd;
+  center_scale<T>(x, p, train, o->center != 0, o->scale != 0, mean, sd);
+  const Mat<T> xt = standardize<T>(x, p, train, mean, sd);
+  const Mat<T> xv = standardize<T>(x, p, val, mean, sd);
+  IVec y(train.size());
+  for (size_t i = 0; i < train.size(); ++i) y[i] = labels[train[i]];
+  const IVec classes = sorted_unique(y);
+  out.classes = classes;
+  if (classes.size() <= 1) {
+    const int lab = classes.empty() ? (labels.empty() ? 0 : labels.front()) : classes.front();
+    out.pred.assign(val.size(), lab);
+    out.components = 1;
+    return out;
+  }
+  const int rank_cap = std::max(1, std::min({o->max_components, p, std::max(1, xt.r - 1)}));
+  Mat<T> g;
+  const bool gram_path = use_gram && xt.c <= xt.r;
+  if (gram_path) g = gram_of(xt);
+  Mat<T> w;
+  try {
+    w = simpls_power<T>(xt, label_crossprod(xt, y, classes), rank_cap, gram_path ? &g : nullptr);
+  } catch (const Degenerate&) {
+    std::map<int, int> tally;
+    for (int l : y) ++tally[l];
+    int lab = classes.front(), best = -1;
+    for (const auto& kv : tally) if (kv.second > best) { lab = kv.first; best = kv.second; }
+    out.pred.assign(val.size(), lab);
+    out.components = 1;
+    return out;
+  }
+  const int requested = o->fixed_components > 0 ? o->fixed_components : o->max_components;
+  const int A = std::min(requested, w.c);
+  out.components = A;
+  out.weights = w;
+  const int C = static_cast<int>(classes.size());
+  IVec code(y.size()), counts(C, 0);
+  for (size_t i = 0; i < y.size(); ++i) {
+    code[i] = static_cast<int>(std::lower_bound(classes.begin(), classes.end(), y[i]) - classes.begin());
+    ++counts[code[i]];
+  }
+  Mat<T> sums(C, A), tt(A, A);
+  std::vector<T> row(A, T(0));
+  for (int i = 0; i < xt.r; ++i) {
+    std::fill(row.begin(), row.end(), T(0));
+    for (int j = 0; j < p; ++j) {
+      const T v = xt(i, j);
+      for (int a = 0; a < A; ++a) row[a] += v * w(j, a);
+    }
+    for (int a = 0; a < A; ++a) sums(code[i], a) += row[a];
+    for (int r = 0; r < A; ++r) for (int c2 = 0; c2 <= r; ++c2) tt(r, c2) += row[r] * row[c2];
+  }
+  for (int r = 0; r < A; ++r) for (int c2 = 0; c2 < r; ++c2) tt(c2, r) = tt(r, c2);
+  const LdaFit<T> m = lda_fit(sums, counts, tt, xt.r);
+  out.val_scores = Mat<T>(xv.r, A);
+  out.discriminant = Mat<T>(xv.r, C);
+  out.pred.assign(val.size(), classes.front());
+  for (int i = 0; i < xv.r; ++i) {
+    std::fill(row.begin(), row.end(), T(0));
+    for (int j = 0; j < p; ++j) {
+      const T v = xv(i, j);
+      for (int a = 0; a < A; ++a) row[a] += v * w(j, a);
+    }
+    for (int a = 0; a < A; ++a) out.val_scores(i, a) = row[a];
+    int best = 0;
+    T best_s = -std::numeric_limits<T>::infinity();
+    for (int c = 0; c < C; ++c) {
+      T sc = m.constant[c];
+      for (int a = 0; a < A; ++a) sc += row[a] * m.linear(c, a);
+      out.discriminant(i, c) = sc;
+      if (sc > best_s) { best_s = sc; best = c; }
+    }
+    out.pred[i] = classes[best];
+  }
+  return out;
+}
+
+// F-fold PLS-LDA CV (plscv.cpp:2293-2507). With the fold cache (non-stratified
+// folds and a data epoch, as CorePLSLDA runs it) folds come from the first call
+// and the Gram loading path is used; otherwise folds are drawn from the labels.
+template <class T>
+IVec plsldacv(const float* x, int n, int p, const IVec& labels, const IVec& constrain,
+              const oracle_core_options* o, const IVec* cached_folds, IVec* folds_out, int* selected) {
+  if (o->max_components < 1) throw std::invalid_argument("PLSOptions::max_components must be positive.");
+  const bool cache = o->stratified == 0 && o->pls_fold_cache != 0;
+  const IVec folds = (cache && cached_folds) ? *cached_folds
+                                               : make_folds(labels, constrain, o->folds, o->stratified != 0, o->cv_seed);
+  if (folds_out) *folds_out = folds;
+  IVec pred(static_cast<size_t>(n), labels.empty() ? 0 : labels.front());
+  int evaluated = o->fixed_components > 0 ? o->fixed_components : o->max_components;
+  for (int f : sorted_unique(folds)) {
+    IVec val, tr;
+    for (int i = 0; i < n; ++i) (folds[i] == f ? val : tr).push_back(i);
+    const FoldOut<T> fo = pls_fold<T>(x, p, labels, tr, val, o, cache);
+    for (size_t i = 0; i < val.size(); ++i) pred[val[i]] = fo.pred[i];
+    evaluated = std::min(evaluated, fo.components);
+  }
+  if (selected) *selected = std::min(evaluated, o->max_components);
+  return pred;
+}
+
+// ---------------------------------------------------------------------------
+// KODAMA dissimilarity (kodama_matrix.cpp:2685-2776): per edge, the fraction of
+// runs that put both ends in the same (non-zero) class rescales (1 + d); rows are
+// re-sorted by (distance, neighbour id).
+// ---------------------------------------------------------------------------
+void dissimilarity(int32_t* idx, float* dist, int samples, int K, const int32_t* res, int runs,
+                   bool in_one, bool out_one) {
+  std::vector<std::pair<float, int>> row(K);
+  for (int i = 0; i < samples; ++i) {
+    for (int j = 0; j < K; ++j) {
+      const size_t o = static_cast<size_t>(i) * K + j;
+      const int stored = idx[o];
+      const int nb = stored >= 0 && in_one ? stored - 1 : stored;
+      float d = dist[o];
+      if (nb < 0 || nb >= samples || !std::isfinite(d)) {
+        row[j] = {std::numeric_limits<float>::infinity(), stored};
+        continue;
+      }
+      int same = 0, valid = 0;
+      for (int r = 0; r < runs; ++r) {
+        const int a = res[static_cast<size_t>(r) * samples + i];
+        const int b = res[static_cast<size_t>(r) * samples + nb];
+        const int v = a != 0 && b != 0;
+        valid += v;
+        same += v && a == b;
+      }
+      if (same == 0 || valid == 0) {
+        d = std::numeric_limits<float>::infinity();
+      } else {
+        const double agree = static_cast<double>(same) / static_cast<double>(valid);
+        d = static_cast<float>((1.0 + static_cast<double>(d)) / (agree * agree));
+      }
+      row[j] = {d, out_one ? nb + 1 : nb};
+    }
+    std::stable_sort(row.begin(), row.end(), [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+      if (a.first != b.first) return a.first < b.first;
+      return a.second < b.second;
+    });
+    for (int j = 0; j < K; ++j) {
+      const size_t o = static_cast<size_t>(i) * K + j;
+      dist[o] = row[j].first;
+      idx[o] = row[j].second;
+    }
+  }
+}
+
+void copy_fold_out(const IVec& folds, int32_t* out) {
+  if (out) std::memcpy(out, folds.data(), sizeof(int32_t) * folds.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_error.c_str(); }
+
+int oracle_make_folds(const int32_t* labels, int32_t n, const int32_t* constrain, int32_t folds,
+                      int32_t stratified, uint64_t seed, int32_t* out) {
+  return guarded([&] {
+    const IVec f = make_folds(IVec(labels, labels + n), to_vec(constrain, n), folds, stratified != 0, seed);
+    std::memcpy(out, f.data(), sizeof(int32_t) * n);
+  });
+}
+
+float oracle_distance(const float* a, const float* b, int32_t p, int32_t metric) {
+  return distance_of(a, b, p, metric);
+}
+
+void oracle_prepare_rows(const float* x, int32_t n, int32_t p, int32_t metric, int32_t variant, float* out) {
+  const std::vector<float> r = prepare_rows(x, n, p, metric, variant);
+  std::memcpy(out, r.data(), sizeof(float) * r.size());
+}
+
+int oracle_exact_knn(const float* train, int32_t nt, const float* query, int32_t nq, int32_t p,
+                     int32_t k, int32_t metric, const int32_t* exclude, int32_t* out_idx, float* out_dist) {
+  return guarded([&] { exact_knn(train, nt, query, nq, p, k, metric, exclude, out_idx, out_dist); });
+}
+
+int oracle_precompute_knn_cv(const float* x, int32_t n, int32_t p, const int32_t* folds, int32_t k,
+                             int32_t metric, int32_t* nbr, float* score) {
+  return guarded([&] {
+    const KnnTables t = precompute_tables(x, n, p, IVec(folds, folds + n), k, metric);
+    std::memcpy(nbr, t.nbr.data(), sizeof(int32_t) * t.nbr.size());
+    std::memcpy(score, t.score.data(), sizeof(float) * t.score.size());
+  });
+}
+
+int oracle_knn_vote(const int32_t* nbr, const float* score, int32_t n, int32_t width,
+                    const int32_t* folds, const int32_t* labels, int32_t* pred) {
+  return guarded([&] {
+    KnnTables t;
+    t.folds.assign(folds, folds + n);
+    t.width = width;
+    t.nbr.assign(nbr, nbr + static_cast<size_t>(n) * width);
+    t.score.assign(score, score + static_cast<size_t>(n) * width);
+    const IVec r = vote_predict(t, IVec(labels, labels + n));
+    std::memcpy(pred, r.data(), sizeof(int32_t) * n);
+  });
+}
+
+double oracle_objective(const int32_t* labels, int32_t n, double acc, const oracle_core_options* opt) {
+  return objective(IVec(labels, labels + n), acc, flags_of(opt));
+}
+
+int oracle_core_knn(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                    const int32_t* fixed, const oracle_core_options* opt, oracle_core_result* out) {
+  return guarded([&] {
+    const IVec init(labels, labels + n);
+    const IVec cons = to_vec(constrain, n);
+    const IVec folds = make_folds(init, cons, opt->folds, opt->stratified != 0, opt->cv_seed);
+    const KnnTables t = precompute_tables(x, n, p, folds, opt->k, opt->metric);
+    Flags f = flags_of(opt);
+    f.knn = true;
+    evolve(n, init, cons, to_vec(fixed, n), f, [&](const IVec& l) { return vote_predict(t, l); }, out);
+    copy_fold_out(folds, out->fold_assignments);
+  });
+}
+
+int oracle_core_knn_tables(const int32_t* nbr, const float* score, int32_t n, int32_t width,
+                           const int32_t* folds, const int32_t* labels, const int32_t* constrain,
+                           const int32_t* fixed, const oracle_core_options* opt, oracle_core_result* out) {
+  return guarded([&] {
+    KnnTables t;
+    t.folds.assign(folds, folds + n);
+    t.width = width;
+    t.nbr.assign(nbr, nbr + static_cast<size_t>(n) * width);
+    t.score.assign(score, score + static_cast<size_t>(n) * width);
+    Flags f = flags_of(opt);
+    f.knn = true;
+    evolve(n, IVec(labels, labels + n), to_vec(constrain, n), to_vec(fixed, n), f,
+           [&](const IVec& l) { return vote_predict(t, l); }, out);
+    copy_fold_out(t.folds, out->fold_assignments);
+  });
+}
+
+// KNNCV (knncv.cpp:366-419) with an exact search: per fold, the k nearest
+// training rows vote with (count, sum of scores, smallest label); rows with no
+// neighbour keep the label of the fold's first training row.
+int oracle_knncv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                 const oracle_core_options* opt, int32_t* pred, int32_t* folds_out) {
+  return guarded([&] {
+    if (opt->k < 1) throw std::invalid_argument("KNNOptions::k must be positive.");
+    const IVec lab(labels, labels + n);
+    const IVec folds = make_folds(lab, to_vec(constrain, n), opt->folds, opt->stratified != 0, opt->cv_seed);
+    const std::vector<float> xs = prepare_rows(x, n, p, opt->metric, 1);
+    IVec out(static_cast<size_t>(n), 0);
+    for (int f : sorted_unique(folds)) {
+      IVec val, tr;
+      for (int i = 0; i < n; ++i) (folds[i] == f ? val : tr).push_back(i);
+      if (tr.empty()) throw std::runtime_error("KNN fold has no training samples.");
+      const int kf = std::min<int>(opt->k, static_cast<int>(tr.size()));
+      std::vector<float> trx(tr.size() * p), vx(val.size() * p);
+      for (size_t i = 0; i < tr.size(); ++i) std::memcpy(&trx[i * p], &xs[static_cast<size_t>(tr[i]) * p], sizeof(float) * p);
+      for (size_t i = 0; i < val.size(); ++i) std::memcpy(&vx[i * p], &xs[static_cast<size_t>(val[i]) * p], sizeof(float) * p);
+      std::vector<int32_t> idx(val.size() * kf);
+      std::vector<float> dist(val.size() * kf);
+      exact_knn(trx.data(), static_cast<int>(tr.size()), vx.data(), static_cast<int>(val.size()), p, kf,
+                opt->metric, nullptr, idx.data(), dist.data());
+      for (size_t q = 0; q < val.size(); ++q) {
+        std::map<int, std::pair<int, double>> tally;
+        for (int j = 0; j < kf; ++j) {
+          const int id = idx[q * kf + j];
+          if (id < 0) continue;
+          auto& e = tally[lab[tr[id]]];
+          e.first += 1;
+          e.second += static_cast<double>(-dist[q * kf + j]);
+        }
+        int best = lab[tr.front()];
+        if (!tally.empty()) {
+          best = tally.begin()->first;
+          int bn = tally.begin()->second.first;
+          double bs = tally.begin()->second.second;
+          for (const auto& kv : tally) {
+            if (kv.second.first > bn || (kv.second.first == bn && kv.second.second > bs) ||
+                (kv.second.first == bn && kv.second.second == bs && kv.first < best)) {
+              best = kv.first; bn = kv.second.first; bs = kv.second.second;
+            }
+          }
+        }
+        out[val[q]] = best;
+      }
+    }
+    std::memcpy(pred, out.data(), sizeof(int32_t) * n);
+    copy_fold_out(folds, folds_out);
+  });
+}
+
+// Exact self kNN graph (graph_cluster.cpp:790-837 contract): k = min(k, n-1),
+// self excluded, 1-based ids, public distance sqrt(max(0, d2)) for Euclidean,
+// max(0, 1 + internal) for cosine, 1 + internal for inner product
+// (native_knn.cpp:1157-1164).
+int oracle_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_t metric,
+                     int32_t* idx_one_based, float* dist) {
+  return guarded([&] {
+    if (n < 2) throw std::invalid_argument("KODAMAKNNGraph requires at least two rows.");
+    const int kk = std::max(1, std::min(k, n - 1));
+    const std::vector<float> xs = prepare_rows(x, n, p, metric, 1);
+    IVec ex(n);
+    std::iota(ex.begin(), ex.end(), 0);
+    std::vector<int32_t> idx(static_cast<size_t>(n) * kk);
+    std::vector<float> d(static_cast<size_t>(n) * kk);
+    exact_knn(xs.data(), n, xs.data(), n, p, kk, metric, ex.data(), idx.data(), d.data());
+    for (size_t i = 0; i < idx.size(); ++i) {
+      idx_one_based[i] = idx[i] >= 0 ? idx[i] + 1 : -1;
+      const float v = d[i];
+      dist[i] = metric == 2 ? std::sqrt(std::max(0.0f, v)) : (metric == 0 ? std::max(0.0f, 1.0f + v) : 1.0f + v);
+    }
+  });
+}
+
+int oracle_dissimilarity(int32_t* indices, float* distances, int32_t samples, int32_t neighbors,
+                         const int32_t* res, int32_t runs, int32_t input_one_based, int32_t output_one_based) {
+  return guarded([&] {
+    dissimilarity(indices, distances, samples, neighbors, res, runs, input_one_based != 0, output_one_based != 0);
+  });
+}
+
+int oracle_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                    const oracle_core_options* opt, int32_t precision, int32_t* pred, int32_t* folds_out,
+                    int32_t* selected_components) {
+  return guarded([&] {
+    const IVec lab(labels, labels + n);
+    IVec folds;
+    int sel = 1;
+    const IVec r = precision == 32
+      ? plsldacv<float>(x, n, p, lab, to_vec(constrain, n), opt, nullptr, &folds, &sel)
+      : plsldacv<double>(x, n, p, lab, to_vec(constrain, n), opt, nullptr, &folds, &sel);
+    std::memcpy(pred, r.data(), sizeof(int32_t) * n);
+    copy_fold_out(folds, folds_out);
+    if (selected_components) *selected_components = sel;
+  });
+}
+
+// Per-fold detail of one PLS-LDA fold for tolerance checks: weights [p x rank]
+// (row-major, first `rank` columns), validation scores [n_val x A], discriminant
+// scores [n_val x C] and the class list. Buffers must hold p*max_components,
+// n*max_components, n*n_classes_max (n) and n entries.
+int oracle_pls_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* folds,
+                           int32_t fold, const oracle_core_options* opt, int32_t precision, double* weights,
+                           int32_t* rank, double* val_scores, double* discriminants, int32_t* classes,
+                           int32_t* n_classes, int32_t* n_val) {
+  return guarded([&] {
+    IVec val, tr;
+    for (int i = 0; i < n; ++i) (folds[i] == fold ? val : tr).push_back(i);
+    const IVec lab(labels, labels + n);
+    auto emit = [&](const auto& fo) {
+      *rank = fo.weights.c;
+      for (int j = 0; j < fo.weights.r; ++j)
+        for (int a = 0; a < fo.weights.c; ++a)
+          weights[static_cast<size_t>(j) * fo.weights.c + a] = static_cast<double>(fo.weights(j, a));
+      for (size_t i = 0; i < fo.val_scores.v.size(); ++i) val_scores[i] = static_cast<double>(fo.val_scores.v[i]);
+      for (size_t i = 0; i < fo.discriminant.v.size(); ++i) discriminants[i] = static_cast<double>(fo.discriminant.v[i]);
+      for (size_t i = 0; i < fo.classes.size(); ++i) classes[i] = fo.classes[i];
+      *n_classes = static_cast<int>(fo.classes.size());
+      *n_val = static_cast<int>(val.size());
+    };
+    const bool cache = opt->stratified == 0 && opt->pls_fold_cache != 0;
+    if (precision == 32) emit(pls_fold<float>(x, p, lab, tr, val, opt, cache));
+    else emit(pls_fold<double>(x, p, lab, tr, val, opt, cache));
+  });
+}
+
+int oracle_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                       const int32_t* fixed, const oracle_core_options* opt, int32_t precision,
+                       oracle_core_result* out) {
+  return guarded([&] {
+    const IVec init(labels, labels + n);
+    const IVec cons = to_vec(constrain, n);
+    oracle_core_options o = *opt;
+    o.pls_fold_cache = 1;
+    // The fold cache is keyed on data, not labels: folds come from the first call.
+    const IVec first_folds = make_folds(init, cons, o.folds, o.stratified != 0, o.cv_seed);
+    Flags f = flags_of(&o);
+    f.knn = false;
+    evolve(n, init, cons, to_vec(fixed, n), f, [&](const IVec& l) {
+      return precision == 32 ? plsldacv<float>(x, n, p, l, cons, &o, &first_folds, nullptr, nullptr)
+                             : plsldacv<double>(x, n, p, l, cons, &o, &first_folds, nullptr, nullptr);
+    }, out);
+    copy_fold_out(first_folds, out->fold_assignments);
+  });
+}
+
+}  // extern "C"

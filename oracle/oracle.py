@@ -1,0 +1,381 @@
+"""ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/liboracle.so`` (this project's C++ restatement of the reference
+algorithms) and, where it was built, ``oracle/_ref/libkodama_ref.so`` (the
+reference's own C++ core behind ``ref_driver.cpp``). Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
+module, and only as the checker or as the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libkodama_ref.so")
+
+N_COUNTERS = 17
+COUNTER_NAMES = (
+    "cycles_completed", "proposals_evaluated", "best_state_updates",
+    "current_state_accepts", "stochastic_state_attempts", "stochastic_state_accepts",
+    "current_state_rejections", "coarsening_moves", "absorption_moves",
+    "transition_attempted", "transition_accepted", "many_to_one_attempted",
+    "many_to_one_accepted", "pls_coarsening_attempted", "pls_coarsening_accepted",
+    "cv_evaluations", "success",
+)
+
+
+class CoreOptions(C.Structure):
+    """Mirror of ``oracle_core_options`` (oracle/kodama_oracle.h)."""
+
+    _fields_ = [
+        ("cycles", C.c_int32), ("classifier", C.c_int32), ("seed", C.c_uint64),
+        ("shake", C.c_int32), ("evolutionary_search", C.c_int32),
+        ("guarded_diversity", C.c_int32), ("many_to_one_absorption", C.c_int32),
+        ("auto_class_coarsening", C.c_int32), ("adaptive_proposal_size", C.c_int32),
+        ("prediction_guidance", C.c_int32), ("policy_adaptive_proposal_size", C.c_int32),
+        ("transition_proposal", C.c_int32), ("stochastic_acceptance", C.c_int32),
+        ("diversity_multiplier", C.c_int32), ("pls_transition_coarsening", C.c_int32),
+        ("pls_fragmentation_penalty", C.c_int32), ("reject_single_class", C.c_int32),
+        ("folds", C.c_int32), ("stratified", C.c_int32), ("cv_seed", C.c_uint64),
+        ("k", C.c_int32), ("metric", C.c_int32),
+        ("max_components", C.c_int32), ("fixed_components", C.c_int32),
+        ("center", C.c_int32), ("scale", C.c_int32), ("pls_fold_cache", C.c_int32),
+        ("hnsw_ef_search", C.c_int32), ("n_threads", C.c_int32),
+    ]
+
+
+def core_options(**kw) -> CoreOptions:
+    """Defaults follow kodama::CoreOptions (kodama.hpp:294-314) with KNN k=30."""
+    d = dict(cycles=100, classifier=1, seed=1, shake=0, evolutionary_search=0,
+             guarded_diversity=0, many_to_one_absorption=0, auto_class_coarsening=0,
+             adaptive_proposal_size=1, prediction_guidance=1,
+             policy_adaptive_proposal_size=1, transition_proposal=1,
+             stochastic_acceptance=1, diversity_multiplier=1,
+             pls_transition_coarsening=1, pls_fragmentation_penalty=1,
+             reject_single_class=1, folds=10, stratified=1, cv_seed=1, k=30, metric=2,
+             max_components=10, fixed_components=0, center=1, scale=1,
+             pls_fold_cache=1, hnsw_ef_search=4096, n_threads=1)
+    d.update(kw)
+    o = CoreOptions()
+    for key, v in d.items():
+        setattr(o, key, int(v))
+    return o
+
+
+class CoreResult(C.Structure):
+    _fields_ = [
+        ("clbest", C.POINTER(C.c_int32)), ("clbest_dirty", C.POINTER(C.c_int32)),
+        ("cvpredbest", C.POINTER(C.c_int32)), ("fold_assignments", C.POINTER(C.c_int32)),
+        ("vect_acc", C.POINTER(C.c_double)), ("vect_score", C.POINTER(C.c_double)),
+        ("proposal_size", C.POINTER(C.c_int32)), ("active_classes", C.POINTER(C.c_int32)),
+        ("accepted", C.POINTER(C.c_uint8)), ("improving_acceptance", C.POINTER(C.c_uint8)),
+        ("temperature_acceptance", C.POINTER(C.c_uint8)), ("counters", C.POINTER(C.c_int32)),
+        ("accbest", C.c_double), ("scorebest", C.c_double),
+    ]
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr_or_null(a, ct):
+    return None if a is None else _p(a, ct)
+
+
+class _CoreBuffers:
+    def __init__(self, n: int, cycles: int):
+        self.arrays = dict(
+            clbest=np.zeros(n, np.int32), clbest_dirty=np.zeros(n, np.int32),
+            cvpredbest=np.zeros(n, np.int32), fold_assignments=np.zeros(n, np.int32),
+            vect_acc=np.zeros(max(cycles, 1), np.float64),
+            vect_score=np.zeros(max(cycles, 1), np.float64),
+            proposal_size=np.zeros(max(cycles, 1), np.int32),
+            active_classes=np.zeros(max(cycles, 1), np.int32),
+            accepted=np.zeros(max(cycles, 1), np.uint8),
+            improving_acceptance=np.zeros(max(cycles, 1), np.uint8),
+            temperature_acceptance=np.zeros(max(cycles, 1), np.uint8),
+            counters=np.zeros(N_COUNTERS, np.int32))
+        self.cycles = cycles
+        r = CoreResult()
+        for k, a in self.arrays.items():
+            ct = {np.int32: C.c_int32, np.float64: C.c_double, np.uint8: C.c_uint8}[a.dtype.type]
+            setattr(r, k, _p(a, ct))
+        self.struct = r
+
+    def result(self) -> dict:
+        out = {}
+        for k, a in self.arrays.items():
+            if k in ("vect_acc", "vect_score", "proposal_size", "active_classes", "accepted",
+                     "improving_acceptance", "temperature_acceptance"):
+                out[k] = a[: self.cycles].copy()
+            elif k == "counters":
+                out.update({name: int(v) for name, v in zip(COUNTER_NAMES, a)})
+            else:
+                out[k] = a.copy()
+        out["accbest"] = float(self.struct.accbest)
+        out["scorebest"] = float(self.struct.scorebest)
+        return out
+
+
+def build(with_reference: bool = False) -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if with_reference and os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+
+
+class _Lib:
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.prefix = prefix
+        err = getattr(self.lib, prefix + "last_error")
+        err.restype = C.c_char_p
+        self._err = err
+
+    def call(self, name, *args):
+        fn = getattr(self.lib, self.prefix + name)
+        rc = fn(*args)
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}{name}: {self._err().decode()}")
+
+
+class _Api:
+    """Common calls that both the oracle and the reference driver export."""
+
+    def __init__(self, lib: _Lib):
+        self._l = lib
+
+    def make_folds(self, labels, folds=10, stratified=True, seed=1, constrain=None):
+        lab = _i32(labels)
+        con = _i32(constrain)
+        out = np.zeros(lab.size, np.int32)
+        self._l.call("make_folds", _p(lab, C.c_int32), C.c_int32(lab.size),
+                     _ptr_or_null(con, C.c_int32), C.c_int32(folds), C.c_int32(int(stratified)),
+                     C.c_uint64(seed), _p(out, C.c_int32))
+        return out
+
+    def core_knn(self, x, labels, opt: CoreOptions, constrain=None, fixed=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        buf = _CoreBuffers(lab.size, opt.cycles)
+        self._l.call("core_knn", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+                     _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.byref(buf.struct))
+        return buf.result()
+
+    def knncv(self, x, labels, opt: CoreOptions, constrain=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        pred = np.zeros(lab.size, np.int32)
+        folds = np.zeros(lab.size, np.int32)
+        self._l.call("knncv", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt),
+                     _p(pred, C.c_int32), _p(folds, C.c_int32))
+        return pred, folds
+
+
+class Oracle(_Api):
+    def __init__(self):
+        super().__init__(_Lib(ORACLE_SO, "oracle_"))
+        L = self._l.lib
+        L.oracle_distance.restype = C.c_float
+        L.oracle_objective.restype = C.c_double
+
+    def distance(self, a, b, metric=2):
+        a = _f32(a)
+        b = _f32(b)
+        return float(self._l.lib.oracle_distance(_p(a, C.c_float), _p(b, C.c_float),
+                                                  C.c_int32(a.size), C.c_int32(metric)))
+
+    def exact_knn(self, train, query, k, metric=2, exclude=None):
+        tr = _f32(train)
+        q = _f32(query)
+        idx = np.zeros((q.shape[0], k), np.int32)
+        dist = np.zeros((q.shape[0], k), np.float32)
+        self._l.call("exact_knn", _p(tr, C.c_float), C.c_int32(tr.shape[0]), _p(q, C.c_float),
+                     C.c_int32(q.shape[0]), C.c_int32(tr.shape[1]), C.c_int32(k), C.c_int32(metric),
+                     _ptr_or_null(_i32(exclude), C.c_int32), _p(idx, C.c_int32), _p(dist, C.c_float))
+        return idx, dist
+
+    def precompute_knn_cv(self, x, folds, k, metric=2):
+        x = _f32(x)
+        f = _i32(folds)
+        nbr = np.zeros((x.shape[0], k), np.int32)
+        score = np.zeros((x.shape[0], k), np.float32)
+        self._l.call("precompute_knn_cv", _p(x, C.c_float), C.c_int32(x.shape[0]),
+                     C.c_int32(x.shape[1]), _p(f, C.c_int32), C.c_int32(k), C.c_int32(metric),
+                     _p(nbr, C.c_int32), _p(score, C.c_float))
+        return nbr, score
+
+    def knn_vote(self, nbr, score, folds, labels):
+        nbr = _i32(nbr)
+        score = _f32(score)
+        f = _i32(folds)
+        lab = _i32(labels)
+        pred = np.zeros(lab.size, np.int32)
+        self._l.call("knn_vote", _p(nbr, C.c_int32), _p(score, C.c_float), C.c_int32(nbr.shape[0]),
+                     C.c_int32(nbr.shape[1]), _p(f, C.c_int32), _p(lab, C.c_int32), _p(pred, C.c_int32))
+        return pred
+
+    def objective(self, labels, accuracy, opt: CoreOptions):
+        lab = _i32(labels)
+        return float(self._l.lib.oracle_objective(_p(lab, C.c_int32), C.c_int32(lab.size),
+                                                   C.c_double(accuracy), C.byref(opt)))
+
+    def core_knn_tables(self, nbr, score, folds, labels, opt: CoreOptions, constrain=None, fixed=None):
+        nbr = _i32(nbr)
+        score = _f32(score)
+        f = _i32(folds)
+        lab = _i32(labels)
+        buf = _CoreBuffers(lab.size, opt.cycles)
+        self._l.call("core_knn_tables", _p(nbr, C.c_int32), _p(score, C.c_float),
+                     C.c_int32(nbr.shape[0]), C.c_int32(nbr.shape[1]), _p(f, C.c_int32),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+                     _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.byref(buf.struct))
+        return buf.result()
+
+    def knn_graph(self, x, k, metric=2):
+        x = _f32(x)
+        kk = max(1, min(k, x.shape[0] - 1))
+        idx = np.zeros((x.shape[0], kk), np.int32)
+        dist = np.zeros((x.shape[0], kk), np.float32)
+        self._l.call("knn_graph", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     C.c_int32(k), C.c_int32(metric), _p(idx, C.c_int32), _p(dist, C.c_float))
+        return idx, dist
+
+    def dissimilarity(self, indices, distances, res, input_one_based=True, output_one_based=True):
+        idx = np.array(indices, dtype=np.int32, copy=True, order="C")
+        dist = np.array(distances, dtype=np.float32, copy=True, order="C")
+        res = _i32(res)
+        self._l.call("dissimilarity", _p(idx, C.c_int32), _p(dist, C.c_float), C.c_int32(idx.shape[0]),
+                     C.c_int32(idx.shape[1]), _p(res, C.c_int32), C.c_int32(res.shape[0]),
+                     C.c_int32(int(input_one_based)), C.c_int32(int(output_one_based)))
+        return idx, dist
+
+    def plsldacv(self, x, labels, opt: CoreOptions, precision=64, constrain=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        pred = np.zeros(lab.size, np.int32)
+        folds = np.zeros(lab.size, np.int32)
+        sel = C.c_int32(0)
+        self._l.call("plsldacv", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt),
+                     C.c_int32(precision), _p(pred, C.c_int32), _p(folds, C.c_int32), C.byref(sel))
+        return pred, folds, sel.value
+
+    def pls_fold_detail(self, x, labels, folds, fold, opt: CoreOptions, precision=64):
+        x = _f32(x)
+        lab = _i32(labels)
+        f = _i32(folds)
+        n, p = x.shape
+        A = opt.max_components
+        w = np.zeros(p * A, np.float64)
+        vs = np.zeros(n * A, np.float64)
+        disc = np.zeros(n * n, np.float64)
+        cls = np.zeros(n, np.int32)
+        rank = C.c_int32(0)
+        ncls = C.c_int32(0)
+        nval = C.c_int32(0)
+        self._l.call("pls_fold_detail", _p(x, C.c_float), C.c_int32(n), C.c_int32(p), _p(lab, C.c_int32),
+                     _p(f, C.c_int32), C.c_int32(fold), C.byref(opt), C.c_int32(precision),
+                     _p(w, C.c_double), C.byref(rank), _p(vs, C.c_double), _p(disc, C.c_double),
+                     _p(cls, C.c_int32), C.byref(ncls), C.byref(nval))
+        r, nc, nv = rank.value, ncls.value, nval.value
+        a = min(opt.fixed_components or opt.max_components, r) if r > 0 else 0
+        return dict(weights=w[: p * r].reshape(p, r) if r else np.zeros((p, 0)),
+                    val_scores=vs[: nv * a].reshape(nv, a) if a else np.zeros((nv, 0)),
+                    discriminant=disc[: nv * nc].reshape(nv, nc) if r else np.zeros((nv, 0)),
+                    classes=cls[:nc].copy(), rank=r)
+
+    def core_plslda(self, x, labels, opt: CoreOptions, precision=64, constrain=None, fixed=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        buf = _CoreBuffers(lab.size, opt.cycles)
+        self._l.call("core_plslda", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+                     _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.c_int32(precision),
+                     C.byref(buf.struct))
+        return buf.result()
+
+
+class Reference(_Api):
+    """The reference's own CPU core (oracle/_ref), where it was built."""
+
+    def __init__(self):
+        super().__init__(_Lib(REF_SO, "ref_"))
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_SO)
+
+    def hnsw_search(self, train, query, k, metric=2, ef_search=4096):
+        tr = _f32(train)
+        q = _f32(query)
+        idx = np.zeros((q.shape[0], k), np.int32)
+        dist = np.zeros((q.shape[0], k), np.float32)
+        self._l.call("hnsw_search", _p(tr, C.c_float), C.c_int32(tr.shape[0]), _p(q, C.c_float),
+                     C.c_int32(q.shape[0]), C.c_int32(tr.shape[1]), C.c_int32(k), C.c_int32(metric),
+                     C.c_int32(ef_search), _p(idx, C.c_int32), _p(dist, C.c_float))
+        return idx, dist
+
+    def core_knn_graph(self, idx, dist, labels, opt: CoreOptions, constrain=None, fixed=None):
+        idx = _i32(idx)
+        dist = _f32(dist)
+        lab = _i32(labels)
+        buf = _CoreBuffers(lab.size, opt.cycles)
+        self._l.call("core_knn_graph", _p(idx, C.c_int32), _p(dist, C.c_float), C.c_int32(idx.shape[0]),
+                     C.c_int32(idx.shape[1]), _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+                     _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.byref(buf.struct))
+        return buf.result()
+
+    def plsldacv(self, x, labels, opt: CoreOptions, constrain=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        pred = np.zeros(lab.size, np.int32)
+        folds = np.zeros(lab.size, np.int32)
+        sel = C.c_int32(0)
+        self._l.call("plsldacv", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt),
+                     _p(pred, C.c_int32), _p(folds, C.c_int32), C.byref(sel))
+        return pred, folds, sel.value
+
+    def core_plslda(self, x, labels, opt: CoreOptions, constrain=None, fixed=None):
+        x = _f32(x)
+        lab = _i32(labels)
+        buf = _CoreBuffers(lab.size, opt.cycles)
+        self._l.call("core_plslda", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+                     _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.byref(buf.struct))
+        return buf.result()
+
+    def dissimilarity(self, indices, distances, res):
+        idx = np.array(indices, dtype=np.int32, copy=True, order="C")
+        dist = np.array(distances, dtype=np.float32, copy=True, order="C")
+        res = _i32(res)
+        self._l.call("dissimilarity", _p(idx, C.c_int32), _p(dist, C.c_float), C.c_int32(idx.shape[0]),
+                     C.c_int32(idx.shape[1]), _p(res, C.c_int32), C.c_int32(res.shape[0]))
+        return idx, dist
+
+    def knn_graph(self, x, k, metric=2, n_threads=1):
+        x = _f32(x)
+        idx = np.zeros((x.shape[0], k), np.int32)
+        dist = np.zeros((x.shape[0], k), np.float32)
+        self._l.call("knn_graph", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     C.c_int32(k), C.c_int32(metric), C.c_int32(n_threads), _p(idx, C.c_int32),
+                     _p(dist, C.c_float))
+        return idx, dist
