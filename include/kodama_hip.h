@@ -1,0 +1,217 @@
+/*
+ * kodama_hip -- MI355X (gfx950) HIP backend of the KODAMA.matrix hot path.
+ *
+ * C ABI of libkodama_hip.so: plain pointers, sizes and integer status codes,
+ * no exceptions and no C++ or framework types across the boundary. Every entry
+ * point returns KODAMA_HIP_OK (0) or an error code; kodama_hip_last_error()
+ * returns the message of the last failure on the calling thread (the pattern of
+ * the reference's embedding launchers, visualization.cpp:2594-2596).
+ *
+ * Host pointers unless a name says otherwise. Labels are arbitrary int32 class
+ * values as in the reference; neighbour ids are 0-based unless the function name
+ * says one_based. Entry points are re-entrant: concurrent callers on different
+ * threads each get their own device workspaces (kodama_matrix.cpp:2651-2682 runs
+ * *_CUDA from many lane threads at once).
+ *
+ * Each declaration cites the reference interface it replaces (file:line in
+ * tkcaccia/KODAMA).
+ */
+#ifndef KODAMA_HIP_H
+#define KODAMA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KODAMA_HIP_OK 0
+#define KODAMA_HIP_EINVAL 1   /* std::invalid_argument in the reference */
+#define KODAMA_HIP_ERUNTIME 2 /* std::runtime_error / device failure */
+#define KODAMA_HIP_ENODEV 3   /* no usable gfx950 device */
+
+/* DistanceMetric order (kodama.hpp:27-31). */
+#define KODAMA_HIP_COSINE 0
+#define KODAMA_HIP_INNER_PRODUCT 1
+#define KODAMA_HIP_EUCLIDEAN 2
+
+/* Flat mirror of kodama::CoreOptions with its KNNOptions / CorePLSLDAOptions
+ * (kodama.hpp:149-179, 294-314) and EvolutionPolicy (kodama.hpp:49-61),
+ * restricted to the fields the hot path reads. */
+typedef struct kodama_hip_core_options {
+  int32_t cycles;                 /* CoreOptions::cycles */
+  int32_t classifier;             /* 0 = PLS_LDA, 1 = KNN (CoreClassifier) */
+  uint64_t seed;                  /* CoreOptions::seed */
+  int32_t shake;
+  int32_t evolutionary_search;
+  int32_t guarded_diversity;
+  int32_t many_to_one_absorption;
+  int32_t auto_class_coarsening;
+  int32_t adaptive_proposal_size;
+  int32_t prediction_guidance;    /* EvolutionPolicy fields ... */
+  int32_t policy_adaptive_proposal_size;
+  int32_t transition_proposal;
+  int32_t stochastic_acceptance;
+  int32_t diversity_multiplier;
+  int32_t pls_transition_coarsening;
+  int32_t pls_fragmentation_penalty;
+  int32_t reject_single_class;
+  int32_t folds;                  /* FoldOptions::folds */
+  int32_t stratified;             /* FoldOptions::stratified */
+  uint64_t cv_seed;               /* FoldOptions::seed */
+  int32_t k;                      /* KNNOptions::k */
+  int32_t metric;                 /* KNNOptions::metric */
+  int32_t max_components;         /* CorePLSLDAOptions::max_components */
+  int32_t fixed_components;       /* CorePLSLDAOptions::fixed_components */
+  int32_t center;
+  int32_t scale;
+  int32_t pls_fold_cache;         /* 1 = data_epoch != 0 (folds cached per run) */
+  int32_t device;                 /* KNNOptions::gpu_device / PLSOptions::gpu_device */
+  int32_t n_threads;              /* host threads for the batched engine (0 = auto) */
+} kodama_hip_core_options;
+
+/* Outputs of one label-evolution run (kodama::CoreResult, kodama.hpp:316-349).
+ * Caller-allocated; per-sample arrays hold n entries, per-cycle arrays `cycles`.
+ * counters[KODAMA_HIP_N_COUNTERS]: cycles_completed, proposals_evaluated,
+ * best_state_updates, current_state_accepts, stochastic_state_attempts,
+ * stochastic_state_accepts, current_state_rejections, coarsening_moves,
+ * absorption_moves, transition_attempted, transition_accepted,
+ * many_to_one_attempted, many_to_one_accepted, pls_coarsening_attempted,
+ * pls_coarsening_accepted, cv_evaluations, success. */
+typedef struct kodama_hip_core_result {
+  int32_t* clbest;
+  int32_t* clbest_dirty;
+  int32_t* cvpredbest;
+  int32_t* fold_assignments;      /* may be NULL */
+  double* vect_acc;
+  double* vect_score;
+  int32_t* proposal_size;
+  int32_t* active_classes;
+  uint8_t* accepted;
+  uint8_t* improving_acceptance;
+  uint8_t* temperature_acceptance;
+  int32_t* counters;
+  double accbest;
+  double scorebest;
+} kodama_hip_core_result;
+
+#define KODAMA_HIP_N_COUNTERS 17
+
+/* ---- library -------------------------------------------------------------- */
+const char* kodama_hip_last_error(void);
+/* Number of visible gfx950 devices (hipGetDeviceCount + arch check). */
+int kodama_hip_device_count(int32_t* count);
+/* Library version string, e.g. "kodama_hip 0.1.0 gfx950". */
+const char* kodama_hip_version(void);
+
+/* ---- device seam: exact kNN ------------------------------------------------
+ * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).
+ * train [train_rows x dimensions], query [query_rows x dimensions] row-major
+ * float32 (rows already normalised for cosine). Output [query_rows x k]:
+ * ascending internal distance (squared L2, or minus the inner product), ties by
+ * smaller train index, -1 / +inf beyond min(k, available). query_train_indices
+ * (may be NULL) excludes one train index per query. Distances are summed in the
+ * reference CPU kernel's order (native_knn.cpp:41-95), so they are bit-identical
+ * to the reference's exhaustive search. */
+int kodama_hip_exact_knn_search(const float* train, int32_t train_rows, const float* query,
+                                int32_t query_rows, int32_t dimensions, int32_t k,
+                                int32_t metric, int32_t device,
+                                const int32_t* query_train_indices, int32_t* out_indices,
+                                float* out_distances);
+
+/* ---- device seam: resident kNN vote graph ----------------------------------
+ * Replace NativeCudaKNNVoteGraph, native_cuda_build_knn_vote_graph and
+ * native_cuda_knn_vote_predict_into (native_cuda_backend.hpp:92-131, 277-296).
+ * neighbor_rows/scores: [samples x neighbors], -1 / -inf padded. The graph stays
+ * resident on the device; each predict uploads dense label codes (0..C-1) and
+ * returns the winning code per row (count, then score sum, then smaller code),
+ * or fallback_code for rows without neighbours. */
+typedef struct kodama_hip_vote_graph kodama_hip_vote_graph;
+int kodama_hip_build_knn_vote_graph(const int32_t* neighbor_rows, const float* scores,
+                                    int32_t samples, int32_t neighbors, int32_t device,
+                                    kodama_hip_vote_graph** out_graph);
+int kodama_hip_knn_vote_predict(kodama_hip_vote_graph* graph, const int32_t* label_codes,
+                                int32_t fallback_code, int32_t* prediction_codes);
+void kodama_hip_release_knn_vote_graph(kodama_hip_vote_graph* graph);
+
+/* ---- kodama-cpp entry points (Backend::HIP twins) -------------------------- */
+
+/* kodama::KNNCV_CUDA (kodama.hpp:871-876; knncv.cpp:422-488) with the exact
+ * search. x [n x p] row-major; constrain may be NULL. Outputs predicted labels
+ * and fold assignments [n]; *global_accuracy may be NULL. */
+int kodama_hip_knncv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                     const int32_t* constrain, const kodama_hip_core_options* options,
+                     int32_t* predicted_labels, int32_t* fold_assignments,
+                     double* global_accuracy);
+
+/* kodama::CoreKNN_CUDA (kodama.hpp:1010-1016; core.cpp:1782-1812): exact per-fold
+ * kNN tables on the device, then maximize_core with the device vote. constrain
+ * and fixed may be NULL. */
+int kodama_hip_core_knn(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                        const int32_t* constrain, const int32_t* fixed,
+                        const kodama_hip_core_options* options,
+                        kodama_hip_core_result* result);
+
+/* kodama::CoreKNNGraph_CUDA (kodama.hpp:1043-1050; core.cpp:1684-1700): the
+ * caller's graph [samples x neighbors] (0- or 1-based per index_base: 0 auto,
+ * 1 zero-based, 2 one-based; GraphIndexBase order) with distances. */
+int kodama_hip_core_knn_graph(const int32_t* graph_indices, const float* graph_distances,
+                              int32_t samples, int32_t neighbors, int32_t index_base,
+                              const int32_t* labels, const int32_t* constrain,
+                              const int32_t* fixed, const kodama_hip_core_options* options,
+                              kodama_hip_core_result* result);
+
+/* kodama::PLSLDACV_CUDA (kodama.hpp:920-925; plscv.cpp:2510-2717): F-fold
+ * PLS-LDA CV. Outputs predictions, folds and the selected component count. */
+int kodama_hip_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                        const int32_t* constrain, const kodama_hip_core_options* options,
+                        int32_t* predicted_labels, int32_t* fold_assignments,
+                        int32_t* selected_components);
+
+/* kodama::CorePLSLDA_CUDA (kodama.hpp:1002-1008; core.cpp:1720-1749). */
+int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                           const int32_t* constrain, const int32_t* fixed,
+                           const kodama_hip_core_options* options,
+                           kodama_hip_core_result* result);
+
+/* kodama::KODAMAKNNGraph_CUDA (kodama.hpp:1524-1527; graph_cluster.cpp:790-837):
+ * exact self-kNN graph, k = min(k, n-1), self excluded; 1-based ids and public
+ * distances (sqrt of squared L2 for Euclidean). Output [n x min(k, n-1)]. */
+int kodama_hip_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_t metric,
+                         int32_t device, int32_t* indices_one_based, float* distances);
+
+/* kodama::KODAMADissimilarityInPlace with Backend::CUDA (kodama.hpp:1181-1189;
+ * kodama_matrix.cpp:2685-2776, 4383-4463): 1-based graph [samples x neighbors]
+ * rewritten in place from the run labels res [runs x samples]. */
+int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distances,
+                                     int32_t samples, int32_t neighbors,
+                                     const int32_t* run_labels, int32_t runs,
+                                     int32_t device);
+
+/* ---- batched label evolution over M runs ----------------------------------
+ * The M-run scheduler of KODAMAMatrix_CUDA (kodama_matrix.cpp:2651-2682,
+ * 3823-3850) around CoreKNN_CUDA (3141-3144), extended to a different landmark
+ * count per run. x [n x p]; run r evolves the rows
+ * landmark_rows[run_offsets[r] .. run_offsets[r+1]) starting from
+ * initial_labels (same indexing); folds are make_folds(options->cv_seed + r)
+ * unless fold_ids is non-NULL (same indexing as landmark_rows). Seeds:
+ * options->seed + r for the proposals. mode 0 = plain (proposals on the host
+ * with the reference's random streams: per run bit-identical to
+ * kodama_hip_core_knn); mode 1 = device-resident (proposal, transition, vote
+ * and acceptance on the device, one graph launch per cycle, counter-based
+ * random streams; same algorithm, different random draws).
+ * Outputs: clbest (same indexing as landmark_rows), accbest[runs],
+ * vect_acc[runs x cycles]; timings[4] (may be NULL): precompute seconds,
+ * evolution seconds, steps completed, device vote seconds. */
+int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs,
+                              const int32_t* run_offsets, const int32_t* landmark_rows,
+                              const int32_t* initial_labels, const int32_t* fold_ids,
+                              const kodama_hip_core_options* options, int32_t mode,
+                              int32_t* clbest, double* accbest, double* vect_acc,
+                              double* timings);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KODAMA_HIP_H */

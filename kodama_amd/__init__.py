@@ -1,0 +1,246 @@
+"""kodama_amd -- the KODAMA.matrix hot path on AMD MI355X (gfx950).
+
+Host-side mirror of the reference's user API for the cross-validated classifiers
+and label-evolution loops (R/kodama.R:160-349: KNNCV, PLSLDACV, CoreKNN,
+CorePLSLDA), the exact kNN graph (KODAMA.graph's builder) and the KODAMA
+dissimilarity, with the same argument names, defaults and error behaviour. Every
+function runs on the HIP backend through libkodama_hip.so; there is no CPU path.
+Results are dicts with the fields of the reference's R lists (kodama_r.cpp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import (CoreOptions, InvalidArgument, KodamaHipError, NoDevice,  # noqa: F401
+                      core_options, device_count)
+
+__all__ = [
+    "KNNCV", "PLSLDACV", "CoreKNN", "CorePLSLDA", "CoreKNNGraph", "KODAMA_knn_graph",
+    "KODAMA_dissimilarity", "core_knn", "core_plslda", "core_knn_batch", "exact_knn_search",
+    "KnnVoteGraph", "core_options", "device_count", "version",
+]
+
+
+def version() -> str:
+    return N.lib().kodama_hip_version().decode()
+
+
+def _metric(metric) -> int:
+    if isinstance(metric, (list, tuple)):
+        metric = metric[0]
+    if metric not in N.METRICS:
+        raise ValueError(f"metric must be one of {tuple(N.METRICS)}")
+    return N.METRICS[metric]
+
+
+def _backend(backend) -> None:
+    if backend not in (None, "hip", "auto"):
+        raise ValueError("kodama_amd runs the 'hip' backend only")
+
+
+def _matrix(data) -> np.ndarray:
+    x = N.f32(data)
+    if x.ndim != 2:
+        raise ValueError("data must be a 2-D matrix")
+    return x
+
+
+# ---- exact kNN seam ---------------------------------------------------------
+
+def exact_knn_search(train, query, k, metric="euclidean", exclude=None, gpu_device=0):
+    """Exact kNN (native_cuda_exact_knn_search contract): internal distances."""
+    tr, q = _matrix(train), _matrix(query)
+    ex = N.i32(exclude)
+    idx = np.zeros((q.shape[0], k), np.int32)
+    dist = np.zeros((q.shape[0], k), np.float32)
+    N.call("kodama_hip_exact_knn_search", N.ptr(tr), C.c_int32(tr.shape[0]), N.ptr(q),
+           C.c_int32(q.shape[0]), C.c_int32(tr.shape[1]), C.c_int32(k), C.c_int32(_metric(metric)),
+           C.c_int32(gpu_device), N.ptr(ex), N.ptr(idx), N.ptr(dist))
+    return idx, dist
+
+
+class KnnVoteGraph:
+    """Resident vote graph (NativeCudaKNNVoteGraph contract)."""
+
+    def __init__(self, neighbor_rows, scores, gpu_device=0):
+        nb = N.i32(neighbor_rows)
+        sc = N.f32(scores)
+        self.samples, self.neighbors = nb.shape
+        h = C.c_void_p()
+        N.call("kodama_hip_build_knn_vote_graph", N.ptr(nb), N.ptr(sc), C.c_int32(self.samples),
+               C.c_int32(self.neighbors), C.c_int32(gpu_device), C.byref(h))
+        self._h = h
+
+    def predict(self, label_codes, fallback_code=0):
+        codes = N.i32(label_codes)
+        out = np.zeros(self.samples, np.int32)
+        N.call("kodama_hip_knn_vote_predict", self._h, N.ptr(codes), C.c_int32(fallback_code), N.ptr(out))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().kodama_hip_release_knn_vote_graph(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- cross-validated classifiers --------------------------------------------
+
+def KNNCV(data, labels, constrain=None, folds=10, stratified=True, seed=1, k=10,
+          metric=("cosine", "inner_product", "euclidean"), backend=None, n_cores=1, gpu_device=0):
+    """k-nearest-neighbour cross-validation (R/kodama.R:160-186)."""
+    _backend(backend)
+    x = _matrix(data)
+    lab = N.i32(labels)
+    opt = core_options(k=k, metric=_metric(metric), folds=folds, stratified=int(bool(stratified)),
+                       cv_seed=seed, device=gpu_device, n_threads=n_cores)
+    pred = np.zeros(x.shape[0], np.int32)
+    fold = np.zeros(x.shape[0], np.int32)
+    acc = C.c_double(0.0)
+    N.call("kodama_hip_knncv", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), N.ptr(lab),
+           N.ptr(N.i32(constrain)), C.byref(opt), N.ptr(pred), N.ptr(fold), C.byref(acc))
+    return dict(predicted_labels=pred, true_labels=lab.copy(), fold_assignments=fold,
+                global_accuracy=acc.value, backend="hip")
+
+
+def PLSLDACV(data, labels, constrain=None, folds=10, stratified=True, seed=1, ncomp=None,
+             center=True, scale=True, backend=None, n_cores=1, gpu_device=0):
+    """PLS-LDA cross-validation (R/kodama.R:212-239)."""
+    _backend(backend)
+    x = _matrix(data)
+    lab = N.i32(labels)
+    ncomp = min(50, x.shape[1]) if ncomp is None else int(ncomp)
+    opt = core_options(classifier=0, folds=folds, stratified=int(bool(stratified)), cv_seed=seed,
+                       max_components=ncomp, fixed_components=ncomp, center=int(bool(center)),
+                       scale=int(bool(scale)), pls_fold_cache=0, device=gpu_device, n_threads=n_cores)
+    pred = np.zeros(x.shape[0], np.int32)
+    fold = np.zeros(x.shape[0], np.int32)
+    sel = C.c_int32(0)
+    N.call("kodama_hip_plsldacv", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), N.ptr(lab),
+           N.ptr(N.i32(constrain)), C.byref(opt), N.ptr(pred), N.ptr(fold), C.byref(sel))
+    return dict(predicted_labels=pred, true_labels=lab.copy(), fold_assignments=fold,
+                selected_components=sel.value, global_accuracy=float(np.mean(pred == lab)),
+                backend="hip")
+
+
+# ---- label evolution ----------------------------------------------------------
+
+def core_knn(data, labels, options: CoreOptions, constrain=None, fix=None) -> dict:
+    """kodama::CoreKNN with full CoreOptions (core.cpp:1866-1876)."""
+    x = _matrix(data)
+    lab = N.i32(labels)
+    buf = N.CoreBuffers(lab.size, options.cycles)
+    N.call("kodama_hip_core_knn", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), N.ptr(lab),
+           N.ptr(N.i32(constrain)), N.ptr(N.i32(fix)), C.byref(options), C.byref(buf.struct))
+    out = buf.result()
+    out["backend"] = "hip"
+    return out
+
+
+def core_plslda(data, labels, options: CoreOptions, constrain=None, fix=None) -> dict:
+    """kodama::CorePLSLDA with full CoreOptions (core.cpp:1854-1864)."""
+    x = _matrix(data)
+    lab = N.i32(labels)
+    buf = N.CoreBuffers(lab.size, options.cycles)
+    N.call("kodama_hip_core_plslda", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), N.ptr(lab),
+           N.ptr(N.i32(constrain)), N.ptr(N.i32(fix)), C.byref(options), C.byref(buf.struct))
+    out = buf.result()
+    out["backend"] = "hip"
+    return out
+
+
+def CoreKNN(data, labels, constrain=None, fix=None, cycles=100, folds=10, stratified=True, seed=1, k=30,
+            metric=("euclidean", "cosine", "inner_product"), backend=None, n_cores=4, gpu_device=0):
+    """kNN label evolution (R/kodama.R:266-296 -> core_impl_cpp, kodama_r.cpp:1118-1174)."""
+    _backend(backend)
+    opt = core_options(cycles=cycles, classifier=1, seed=seed, folds=folds, stratified=int(bool(stratified)),
+                       cv_seed=seed, k=k, metric=_metric(metric), device=gpu_device, n_threads=n_cores)
+    return core_knn(data, labels, opt, constrain=constrain, fix=fix)
+
+
+def CorePLSLDA(data, labels, constrain=None, fix=None, cycles=100, folds=10, stratified=True, seed=1,
+               ncomp=None, backend=None, n_cores=4, gpu_device=0):
+    """PLS-LDA label evolution (R/kodama.R:322-349 -> core_impl_cpp)."""
+    _backend(backend)
+    x = _matrix(data)
+    ncomp = min(50, x.shape[1]) if ncomp is None else int(ncomp)
+    opt = core_options(cycles=cycles, classifier=0, seed=seed, folds=folds, stratified=int(bool(stratified)),
+                       cv_seed=seed, max_components=ncomp, fixed_components=ncomp, device=gpu_device,
+                       n_threads=n_cores)
+    return core_plslda(x, labels, opt, constrain=constrain, fix=fix)
+
+
+def CoreKNNGraph(graph_indices, graph_distances, labels, options: CoreOptions, constrain=None, fix=None,
+                 index_base="auto") -> dict:
+    """kodama::CoreKNNGraph on a caller graph (core.cpp:1684-1700)."""
+    idx = N.i32(graph_indices)
+    dist = N.f32(graph_distances)
+    lab = N.i32(labels)
+    base = {"auto": 0, "zero": 1, "one": 2}[index_base]
+    buf = N.CoreBuffers(lab.size, options.cycles)
+    N.call("kodama_hip_core_knn_graph", N.ptr(idx), N.ptr(dist), C.c_int32(idx.shape[0]),
+           C.c_int32(idx.shape[1]), C.c_int32(base), N.ptr(lab), N.ptr(N.i32(constrain)),
+           N.ptr(N.i32(fix)), C.byref(options), C.byref(buf.struct))
+    out = buf.result()
+    out["backend"] = "hip"
+    return out
+
+
+def core_knn_batch(data, run_rows, initial_labels, options: CoreOptions, mode="plain", fold_ids=None):
+    """Label evolution of M runs in one batch (the M-run scheduler of KODAMAMatrix).
+
+    run_rows / initial_labels / fold_ids: lists of per-run int arrays. mode
+    "plain" replays the reference random streams per run (bit-identical to
+    core_knn on that run); "device" keeps proposals, votes and acceptance on the
+    device. Returns clbest per run, accbest, vect_acc [M x cycles] and timings.
+    """
+    x = _matrix(data)
+    M = len(run_rows)
+    offsets = np.zeros(M + 1, np.int32)
+    offsets[1:] = np.cumsum([len(r) for r in run_rows])
+    rows = N.i32(np.concatenate([np.asarray(r) for r in run_rows]))
+    init = N.i32(np.concatenate([np.asarray(l) for l in initial_labels]))
+    folds = None if fold_ids is None else N.i32(np.concatenate([np.asarray(f) for f in fold_ids]))
+    clbest = np.zeros(int(offsets[-1]), np.int32)
+    accbest = np.zeros(M, np.float64)
+    vect = np.zeros(M * max(options.cycles, 1), np.float64)
+    timings = np.zeros(4, np.float64)
+    m = {"plain": 0, "device": 1}[mode]
+    N.call("kodama_hip_core_knn_batch", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+           N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), C.c_int32(m),
+           N.ptr(clbest), N.ptr(accbest), N.ptr(vect), N.ptr(timings))
+    return dict(clbest=[clbest[offsets[r]:offsets[r + 1]].copy() for r in range(M)], accbest=accbest,
+                vect_acc=vect[: M * options.cycles].reshape(M, options.cycles),
+                precompute_seconds=float(timings[0]), evolution_seconds=float(timings[1]),
+                steps=int(timings[2]), vote_seconds=float(timings[3]))
+
+
+# ---- graph and dissimilarity ----------------------------------------------------
+
+def KODAMA_knn_graph(data, k=30, metric="euclidean", gpu_device=0):
+    """Exact self-kNN graph (KODAMAKNNGraph_CUDA contract): 1-based ids, public distances."""
+    x = _matrix(data)
+    kk = max(1, min(int(k), x.shape[0] - 1))
+    idx = np.zeros((x.shape[0], kk), np.int32)
+    dist = np.zeros((x.shape[0], kk), np.float32)
+    N.call("kodama_hip_knn_graph", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(k),
+           C.c_int32(_metric(metric)), C.c_int32(gpu_device), N.ptr(idx), N.ptr(dist))
+    return dict(indices=idx, distances=dist, neighbors=kk, index_base="one")
+
+
+def KODAMA_dissimilarity(indices, distances, res, gpu_device=0):
+    """KODAMADissimilarityInPlace on a 1-based graph; returns corrected copies."""
+    idx = np.array(indices, dtype=np.int32, order="C", copy=True)
+    dist = np.array(distances, dtype=np.float32, order="C", copy=True)
+    r = N.i32(res)
+    N.call("kodama_hip_dissimilarity_inplace", N.ptr(idx), N.ptr(dist), C.c_int32(idx.shape[0]),
+           C.c_int32(idx.shape[1]), N.ptr(r), C.c_int32(r.shape[0]), C.c_int32(gpu_device))
+    return idx, dist

@@ -1,0 +1,62 @@
+// Kernel parameter blocks and host launchers shared by the HIP translation
+// units and the C++ host engine of libkodama_hip.so.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace kodama_hip {
+
+constexpr int kMaxK = 64;       // neighbours kept per row by the register top-k
+constexpr int kMaxDims = 256;   // feature dimensions of the register query path
+
+// Exact kNN. MODE 0 (query vs train): query/train row-major, out_index/out_dist
+// [query_rows x k], internal distances. MODE 1 (per-fold tables of a batch of
+// runs): rows[] are the data rows of each run, concatenated with run_offsets;
+// a row's candidates are the rows of its run in another fold; output goes to the
+// transposed padded vote layout [k x pad_total] as padded row ids and scores.
+struct KnnTopkParams {
+  const float* train = nullptr;
+  const float* query = nullptr;
+  int train_rows = 0;
+  int query_rows = 0;
+  int dims = 0;
+  int k = 0;
+  const int* exclude = nullptr;      // MODE 0, per query, may be null
+  const int* rows = nullptr;         // MODE 1
+  const int* folds = nullptr;        // MODE 1, per concatenated row
+  const int* run_offsets = nullptr;  // MODE 1, [runs + 1]
+  const int* pad_offsets = nullptr;  // MODE 1, [runs + 1]
+  int pad_total = 0;                 // MODE 1
+  int* out_index = nullptr;
+  float* out_dist = nullptr;         // MODE 0 distances, MODE 1 scores (-distance)
+};
+
+// Batched vote over the padded layout (each run's rows start at a multiple of
+// 256, so every block belongs to one run).
+struct VoteParams {
+  const int* neighbors = nullptr;    // [k x pad_total] padded row ids or -1
+  const float* scores = nullptr;     // [k x pad_total]
+  const int* labels = nullptr;       // [pad_total] label codes
+  const int* block_run = nullptr;    // [blocks]
+  const int* run_offsets = nullptr;  // [runs + 1] unpadded offsets (row counts)
+  const int* pad_offsets = nullptr;  // [runs + 1]
+  const int* fallback_code = nullptr;  // [runs] code for rows without neighbours, may be null
+  int pad_total = 0;
+  int k = 0;
+  int classes = 0;                   // codes per run (array stride)
+  int* predictions = nullptr;        // [pad_total]
+  int* counts = nullptr;             // [runs x classes] label counts, null = skip reductions
+  int* firsts = nullptr;             // [runs x classes] first local row of each label
+  int* matches = nullptr;            // [runs] rows with prediction == label
+};
+
+hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,
+                           hipStream_t stream);
+hipError_t launch_vote(const VoteParams& prm, int blocks, hipStream_t stream);
+hipError_t launch_vote_reset(int* counts, int* firsts, int* matches, int runs, int classes,
+                             hipStream_t stream);
+
+}  // namespace kodama_hip
