@@ -1,0 +1,200 @@
+"""Benchmark: CV label-evolution steps/s of the KODAMA.matrix kNN loop on one MI355X.
+
+Workload (the batched-M-runs configuration): X = 100,000 x 50 float32 from a
+20-cluster Gaussian mixture (Zipf(1.1) cluster sizes, sigma 1, means
+U[-4,4]^50); classifier kNN with k = 30; M = 100 runs, each on its own landmark
+subset of U[1000, 5000] rows with 10 uneven folds (Dirichlet(1) fold shares);
+starting labels = nearest of min(300, L) random landmark centres (one k-means
+assignment step); Tcycle = --steps cycles. A step is one (run, cycle) CV
+evaluation with its proposal and accept/reject; steps/s = M * cycles / wall time
+of the label-evolution phase (host proposals, device votes and syncs included;
+the one-time per-fold exact kNN precompute is excluded and reported apart).
+
+Two forms run on the same inputs:
+  plain   host proposals with the reference's random streams (per run
+          bit-identical to CoreKNN), one batched device vote per cycle;
+  device  proposal, absorption, vote, objective and acceptance on the device,
+          one hipGraph launch per cycle, counter-based random streams.
+`value` is the device form's steps/s; both forms' mean final CV accuracy is
+reported ("equal accuracy" = within 0.5 points).
+
+The roofline object covers the batched vote kernel (HBM-bound): algorithmic
+bytes per launch = sum over runs of L * (8k + 28), divided by the kernel's
+average duration measured with HIP events on its launch stream.
+
+cpu_baseline: the reference's own CPU label-evolution loop (oracle/_ref,
+CoreKNNGraph_CPU on exact 64-NN landmark graphs of the same runs), run on
+`cores` host threads, one run per thread at a time.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--runs 100]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CV label-evolution steps/s (n=100k,k=30), plain vs tuned at equal CV accuracy"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10):
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    x, _ = D.config3(n=n, p=p)
+    sizes = D.randint(6, 1, runs, 1000, 5000)
+    rows, fold_ids, init = [], [], []
+    for r in range(runs):
+        L = int(sizes[r])
+        keys = D.uniform(6, 100 + r, n)
+        rw = np.sort(np.argsort(keys, kind="stable")[:L]).astype(np.int32)
+        # Uneven folds: Dirichlet(1) shares, at least one row per fold.
+        e = -np.log1p(-D.uniform(7, r, folds))
+        share = e / e.sum()
+        cnt = np.maximum(1, np.floor(share * (L - folds)).astype(np.int64) + 1)
+        cnt[np.argmax(cnt)] += L - cnt.sum()
+        perm = np.argsort(D.uniform(7, 1000 + r, L), kind="stable")
+        f = np.empty(L, np.int32)
+        f[perm] = np.repeat(np.arange(folds, dtype=np.int32), cnt)
+        # Starting labels: nearest of min(300, L) random landmark centres.
+        nc = min(300, L)
+        centres = x[rw[np.argsort(D.uniform(8, r, L), kind="stable")[:nc]]]
+        idx, _ = K.exact_knn_search(centres, x[rw], 1, "euclidean")
+        rows.append(rw)
+        fold_ids.append(f)
+        init.append((idx[:, 0] + 1).astype(np.int32))
+    return x, rows, fold_ids, init
+
+
+def cpu_baseline(x, rows, init, k, cycles, cores, sample_runs):
+    """Reference CoreKNNGraph_CPU label evolution, `cores` runs at a time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kodama_amd as K
+    import oracle as O
+    kind = "reference" if O.Reference.available() else "port"
+    eng = O.Reference() if kind == "reference" else O.Oracle()
+    graphs = []
+    for r in range(sample_runs):
+        xl = x[rows[r]]
+        ex = np.arange(xl.shape[0], dtype=np.int32)
+        idx, d2 = K.exact_knn_search(xl, xl, 64, "euclidean", exclude=ex)
+        graphs.append((idx, np.sqrt(np.maximum(d2, 0)).astype(np.float32)))
+    opt = O.core_options(cycles=cycles, k=k, folds=10, stratified=0, evolutionary_search=1,
+                         guarded_diversity=1, many_to_one_absorption=1)
+
+    def one(r):
+        o = O.core_options(**{f: getattr(opt, f) for f, _ in O.CoreOptions._fields_})
+        o.seed = 1 + r
+        o.cv_seed = 1 + r
+        idx, d = graphs[r]
+        if kind == "reference":
+            return eng.core_knn_graph(idx, d, init[r], o)["accbest"]
+        nbr = np.where(idx >= 0, idx, -1)[:, :k]
+        score = -(d[:, :k] ** 2)
+        folds = O.Oracle().make_folds(init[r], 10, 0, 1 + r)
+        return eng.core_knn_tables(nbr, score, folds, init[r], o)["accbest"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        accs = list(pool.map(one, range(sample_runs)))
+    wall = time.perf_counter() - t0
+    steps = sample_runs * cycles
+    return dict(value=steps / wall, unit="steps/s", cores=cores, kind=kind,
+                sample=f"{sample_runs} runs x {cycles} cycles of this workload "
+                       f"(CoreKNNGraph_CPU on exact 64-NN landmark graphs), {cores} threads",
+                wall_seconds=wall, mean_accbest=float(np.mean(accs)))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="label-evolution cycles per run (Tcycle)")
+    ap.add_argument("--warmup", type=int, default=5, help="cycles of an untimed warm-up evolution")
+    ap.add_argument("--runs", type=int, default=100)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--k", type=int, default=30)
+    ap.add_argument("--forms", default="plain,device")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported"}))
+        return 2
+
+    import kodama_amd as K
+    log(f"{K.version()}; devices={K.device_count()}")
+    t0 = time.perf_counter()
+    x, rows, fold_ids, init = make_workload(args.runs, args.n, 50, args.k)
+    L = np.array([len(r) for r in rows])
+    log(f"workload ready in {time.perf_counter() - t0:.1f}s: runs={args.runs} landmarks total={L.sum()}")
+    base = dict(k=args.k, folds=10, stratified=0, seed=1, cv_seed=1, evolutionary_search=1,
+                guarded_diversity=1, many_to_one_absorption=1, n_threads=0)
+    forms = [f for f in args.forms.split(",") if f]
+    res = {}
+    for form in forms:
+        if args.warmup > 0:
+            K.core_knn_batch(x, rows[:8], init[:8], K.core_options(cycles=args.warmup, **base),
+                             mode=form, fold_ids=fold_ids[:8])
+        out = K.core_knn_batch(x, rows, init, K.core_options(cycles=args.steps, **base), mode=form,
+                               fold_ids=fold_ids)
+        sps = args.runs * args.steps / out["evolution_seconds"]
+        res[form] = dict(steps_per_s=sps, evolution_seconds=out["evolution_seconds"],
+                         precompute_seconds=out["precompute_seconds"],
+                         mean_accbest=float(np.mean(out["accbest"])),
+                         vote_kernel_ms=out["vote_seconds"] * 1e3)
+        log(f"{form}: {sps:,.0f} steps/s, evolution {out['evolution_seconds']:.3f}s, "
+            f"precompute {out['precompute_seconds']:.3f}s, mean accbest {res[form]['mean_accbest']:.4f}, "
+            f"vote kernel {out['vote_seconds'] * 1e3:.3f} ms")
+    head = "device" if "device" in res else forms[0]
+    vote_s = res[head]["vote_kernel_ms"] * 1e-3
+    algo_bytes = float(L.sum()) * (8 * args.k + 28)
+    achieved = algo_bytes / vote_s / 1e9 if vote_s > 0 else 0.0
+    line = {
+        "metric": METRIC,
+        "value": res[head]["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * res[head]["evolution_seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched M runs: X=100000x50 20-cluster Zipf(1.1) mixture, kNN k=30, M=100, "
+                        "landmarks U[1000,5000], 10 uneven folds, Tcycle=50",
+            "n": args.n, "p": 50, "k": args.k, "runs": args.runs, "cycles": args.steps,
+            "landmarks_total": int(L.sum()), "form": head,
+        },
+        "forms": res,
+        "roofline": {"bound": "hbm", "kernel": "vote_kernel<32>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": algo_bytes},
+    }
+    if "plain" in res and "device" in res:
+        line["accuracy_gap_pt"] = 100.0 * (res["device"]["mean_accbest"] - res["plain"]["mean_accbest"])
+        line["speedup_device_vs_plain"] = res["device"]["steps_per_s"] / res["plain"]["steps_per_s"]
+    if not args.no_cpu_baseline:
+        cores = min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(x, rows, init, args.k, args.steps, cores, min(args.runs, 64))
+        log(f"cpu baseline ({cb['kind']}): {cb['value']:,.0f} steps/s on {cores} threads")
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -292,6 +292,12 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
       folds[r] = fold_ids ? IVec(fold_ids + offsets[r], fold_ids + offsets[r + 1])
                           : make_folds(IVec(lab, lab + L), IVec(), options->folds, options->stratified != 0,
                                        options->cv_seed + static_cast<std::uint64_t>(r));
+      {
+        // Two folds or more: every row then has training rows, so the exact
+        // search gives it at least one neighbour.
+        const auto mm = std::minmax_element(folds[r].begin(), folds[r].end());
+        require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
+      }
       encode_labels(lab, L, values[r], codes[r]);
       classes = std::max(classes, static_cast<int>(values[r].size()));
       std::copy(folds[r].begin(), folds[r].end(), fold_concat.begin() + offsets[r]);
@@ -312,7 +318,7 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
         ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
       }
       evolve_plain(ev, batch, folds, options->cycles, options->n_threads);
-      vote_seconds = batch.vote_ms_total() * 1e-3;
+      vote_seconds = batch.vote_ms_total() * 1e-3 / std::max(1, batch.vote_calls());
       for (int r = 0; r < runs; ++r) {
         const IVec& b = ev[r].best_codes();
         for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];

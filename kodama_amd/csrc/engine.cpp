@@ -731,8 +731,8 @@ KnnBatch::KnnBatch(int device, const float* x, int n, int p, const IVec& run_off
 void KnnBatch::vote_all(bool reduce) {
   const hipStream_t s = stream_.get();
   d_labels_.upload(h_labels_.get(), static_cast<size_t>(pad_total_), s);
-  timer_.start(s);
   if (reduce) check(launch_vote_reset(d_counts_.get(), d_firsts_.get(), d_matches_.get(), runs_, classes_, s), "vote reset");
+  timer_.start(s);
   VoteParams prm;
   prm.neighbors = d_nbr_.get();
   prm.scores = d_score_.get();

@@ -2,6 +2,7 @@
 // evolve_device.cpp).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "engine.h"

@@ -1,0 +1,56 @@
+"""The device-resident evolution: same algorithm as the plain form, different
+random draws. Checked by exact invariants and by accuracy against the plain form."""
+import numpy as np
+import pytest
+
+import kodama_amd as K
+from kodama_amd import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n_runs=12, n=6000, seed=31):
+    x, comp = D.config3(n=n, seed=seed)
+    rows, init, folds = [], [], []
+    for r in range(n_runs):
+        L = int(D.randint(seed, 50 + r, 1, 300, 1200)[0])
+        rw = np.sort(np.argsort(D.uniform(seed, 200 + r, n))[:L]).astype(np.int32)
+        lab = D.randint(seed, 400 + r, L, 1, 40).astype(np.int32)
+        f = (np.argsort(D.uniform(seed, 600 + r, L)) % 10).astype(np.int32)
+        rows.append(rw)
+        init.append(lab)
+        folds.append(f)
+    return x, rows, init, folds
+
+
+OPTS = dict(cycles=25, k=15, folds=10, stratified=0, seed=3, cv_seed=3, evolutionary_search=1,
+            guarded_diversity=1, many_to_one_absorption=1)
+
+
+def test_device_accuracy_is_the_vote_of_the_returned_labels(oracle):
+    x, rows, init, folds = _batch()
+    out = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
+    for r in range(len(rows)):
+        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], OPTS["k"])
+        cl = out["clbest"][r]
+        pred = oracle.knn_vote(nbr, score, folds[r], cl)
+        assert out["accbest"][r] == np.mean(pred == cl), r
+        assert set(np.unique(cl)) <= set(np.unique(init[r]))
+        assert out["vect_acc"][r][-1] == out["accbest"][r]
+
+
+def test_device_reaches_plain_accuracy():
+    x, rows, init, folds = _batch(n_runs=24)
+    plain = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="plain", fold_ids=folds)
+    dev = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
+    gap = abs(np.mean(dev["accbest"]) - np.mean(plain["accbest"]))
+    assert gap < 0.02, (np.mean(dev["accbest"]), np.mean(plain["accbest"]))
+
+
+def test_device_is_deterministic():
+    x, rows, init, folds = _batch(n_runs=4)
+    a = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
+    b = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
+    for r in range(4):
+        assert np.array_equal(a["clbest"][r], b["clbest"][r])
+    assert np.array_equal(a["accbest"], b["accbest"])
