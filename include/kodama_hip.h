@@ -202,8 +202,9 @@ int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distance
  * random streams; same algorithm, different random draws).
  * Outputs: clbest (same indexing as landmark_rows), accbest[runs],
  * vect_acc[runs x cycles]; timings[4] (may be NULL): precompute seconds,
- * evolution seconds, steps completed, average duration of one batched vote
- * launch in seconds (HIP events on the launch stream). */
+ * evolution seconds, steps completed, and the device seconds of the dominant
+ * kernel (HIP events on its launch stream): mode 0 the average batched vote
+ * launch, mode 1 the single persistent evolution launch. */
 int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs,
                               const int32_t* run_offsets, const int32_t* landmark_rows,
                               const int32_t* initial_labels, const int32_t* fold_ids,

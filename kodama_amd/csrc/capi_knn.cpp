@@ -328,7 +328,7 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
     } else {
       DeviceEvolution dev(batch, values, codes, folds, *options);
       dev.run(options->cycles);
-      vote_seconds = dev.vote_seconds();
+      vote_seconds = dev.kernel_seconds();
       dev.export_results(clbest, accbest, vect_acc);
     }
     const double t_evo = seconds_since(t1);

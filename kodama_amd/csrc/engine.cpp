@@ -620,6 +620,7 @@ void KnnBatch::allocate(const IVec& run_offsets) {
   for (int r = 0; r < runs_; ++r) {
     const int L = run_offsets[r + 1] - run_offsets[r];
     if (L < 1) throw InvalidArgument("every run needs at least one row.");
+    max_rows_ = std::max(max_rows_, L);
     const int padded = (L + 255) / 256 * 256;
     pad_offsets_[r + 1] = pad_offsets_[r] + padded;
     for (int b = 0; b < padded / 256; ++b) block_run.push_back(r);
@@ -742,6 +743,7 @@ void KnnBatch::vote_all(bool reduce) {
   prm.pad_offsets = d_pad_off_.get();
   prm.pad_total = pad_total_;
   prm.k = k_;
+  prm.max_rows = max_rows_;
   prm.classes = classes_;
   prm.predictions = d_pred_.get();
   if (reduce) {

@@ -117,6 +117,7 @@ class KnnBatch {
   int k() const { return k_; }
   int pad_offset(int r) const { return pad_offsets_[r]; }
   int pad_total() const { return pad_total_; }
+  int max_rows() const { return max_rows_; }
   int classes() const { return classes_; }
   hipStream_t stream() const { return stream_.get(); }
 
@@ -151,7 +152,7 @@ class KnnBatch {
   void find_empty_rows();
 
   Stream stream_;
-  int runs_ = 0, k_ = 0, pad_total_ = 0, classes_ = 0, blocks_ = 0;
+  int runs_ = 0, k_ = 0, pad_total_ = 0, classes_ = 0, blocks_ = 0, max_rows_ = 0;
   IVec run_offsets_, pad_offsets_;
   DeviceBuffer<int> d_nbr_, d_labels_, d_pred_, d_block_run_, d_run_off_, d_pad_off_;
   DeviceBuffer<float> d_score_;

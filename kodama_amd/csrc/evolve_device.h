@@ -1,5 +1,4 @@
-// Device-resident label evolution of a batch of runs (declared here, defined in
-// evolve_device.cpp).
+// Device-resident label evolution of a batch of runs (evolve_device.cpp).
 #pragma once
 
 #include <memory>
@@ -14,14 +13,18 @@ class DeviceEvolution {
   DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values, const std::vector<IVec>& codes,
                   const std::vector<IVec>& folds, const kodama_hip_core_options& options);
   ~DeviceEvolution();
+  DeviceEvolution(const DeviceEvolution&) = delete;
+  DeviceEvolution& operator=(const DeviceEvolution&) = delete;
+  // Runs `cycles` more cycles of every run in one persistent launch; blocks.
   void run(int cycles);
-  double vote_seconds() const { return vote_seconds_; }
+  // Duration of the last launch (HIP events on the batch stream).
+  double kernel_seconds() const { return kernel_seconds_; }
   void export_results(int* clbest, double* accbest, double* vect_acc);
 
  private:
   struct Impl;
   Impl* impl_ = nullptr;
-  double vote_seconds_ = 0.0;
+  double kernel_seconds_ = 0.0;
 };
 
 }  // namespace kodama_hip

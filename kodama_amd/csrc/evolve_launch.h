@@ -1,51 +1,51 @@
-// Parameter block and launchers of the device-resident evolution kernels
-// (evolve_kernels.hip).
+// Parameter block and launchers of the persistent device-resident evolution
+// kernel (evolve_kernels.hip).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace kodama_hip {
 
+constexpr int kEvoThreads = 1024;           // one workgroup per run
+constexpr size_t kEvoMaxLds = 160 * 1024;   // LDS of one CDNA4 compute unit
 constexpr int kEvoAccepted = 1;
 constexpr int kEvoImproved = 2;
-constexpr int kEvoAbsorbed = 4;
 constexpr int kEvoStateStride = 4;  // cur_acc, cur_score, best_acc, best_score
 
 struct EvoParams {
   int runs = 0;
-  int classes = 0;       // label codes per run (array stride)
-  int cycles = 0;
-  int guarded = 1;       // diversity-guarded objective
-  const int* block_run = nullptr;
-  const int* run_offsets = nullptr;
-  const int* pad_offsets = nullptr;
-  int* cycle = nullptr;              // device cycle counter
-  const uint64_t* run_key = nullptr; // [runs]
-  const int* perm_half = nullptr;    // [runs] Feistel half width
-  int* cur = nullptr;
-  int* curpred = nullptr;
-  int* best = nullptr;
-  int* bestpred = nullptr;
-  int* prop = nullptr;
-  int* proppred = nullptr;
-  int* sizes = nullptr;              // [runs x classes]
-  int* flows = nullptr;              // [runs x classes x classes]
-  int* remap = nullptr;              // [runs x classes]
-  int* counts = nullptr;             // vote reductions [runs x classes]
-  int* firsts = nullptr;
-  int* matches = nullptr;            // [runs]
-  double* state = nullptr;           // [runs x kEvoStateStride]
-  int* decision = nullptr;           // [runs]
-  double* history = nullptr;         // [runs x cycles] best accuracy per cycle
+  int classes = 0;        // label codes per run (array stride), <= 1024
+  int cycles = 0;         // total cycles of the evolution (annealing, history stride)
+  int cycle_begin = 0;    // first cycle of this launch
+  int cycle_count = 0;    // cycles run by this launch
+  int init = 0;           // evaluate the starting labels first
+  int guarded = 1;        // diversity-guarded objective
+  int k = 0;
+  int max_rows = 0;       // largest run, <= 8191
+  int pad_total = 0;
+  const int* run_offsets = nullptr;          // [runs + 1]
+  const int* pad_offsets = nullptr;          // [runs + 1]
+  const unsigned short* neighbors = nullptr; // [k x pad_total] local ids, 0xFFFF = none
+  const float* scores = nullptr;             // [k x pad_total]
+  const uint64_t* run_key = nullptr;         // [runs]
+  const int* perm_half = nullptr;            // [runs] Feistel half width
+  unsigned short* cur = nullptr;             // [pad_total] run state between launches
+  unsigned short* curpred = nullptr;
+  unsigned short* best = nullptr;
+  unsigned short* bestpred = nullptr;
+  double* state = nullptr;                   // [runs x kEvoStateStride]
+  int* flows = nullptr;                      // [runs x classes x classes], zero between cycles
+  double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
 };
 
-hipError_t launch_evo_clear(const EvoParams& p, hipStream_t s);
-hipError_t launch_evo_propose(const EvoParams& p, int blocks, hipStream_t s);
-hipError_t launch_evo_absorb(const EvoParams& p, hipStream_t s);
-hipError_t launch_evo_remap(const EvoParams& p, int blocks, hipStream_t s);
-hipError_t launch_evo_accept(const EvoParams& p, int init, hipStream_t s);
-hipError_t launch_evo_commit(const EvoParams& p, int blocks, hipStream_t s);
+size_t evo_lds_bytes(int classes, int max_rows);
+hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
+// 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids.
+hipError_t launch_evo_narrow(const int* nbr32, const int* pad_offsets, const int* block_run, unsigned short* nbr16,
+                             size_t total, int pad_total, hipStream_t s);
+hipError_t launch_evo_zero(int* a, size_t n, hipStream_t s);
 
 }  // namespace kodama_hip

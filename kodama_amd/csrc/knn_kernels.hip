@@ -11,12 +11,13 @@
 // index order and only a strictly smaller distance displaces a kept one, which
 // realises the (distance, index) order of native_knn.cpp:121-123.
 //
-// vote_kernel: one thread per validation row of every run of a batch. The k
-// neighbour labels are tallied in registers; the winner has the most votes, then
-// the largest double sum of scores (added in neighbour-rank order), then the
-// smallest label code (core.cpp:871-911). The block also reduces, per run, the
-// number of rows whose prediction equals the label, the class counts of the
-// labels and the first row of each class (the objective's inputs).
+// vote_kernel: one thread per validation row of every run of a batch; each block
+// belongs to one run and stages that run's labels in LDS. The k neighbour labels
+// are tallied in registers; the winner has the most votes, then the largest
+// double sum of scores (added in neighbour-rank order), then the smallest label
+// code (core.cpp:871-911). The block also reduces, per run, the number of rows
+// whose prediction equals the label, the class counts of the labels and the
+// first row of each class (the objective's inputs).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -217,90 +218,207 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
   }
 }
 
-template <int KMAX>
-__global__ void __launch_bounds__(kBlock) vote_kernel(VoteParams prm) {
-  extern __shared__ int smem[];
-  int* hist = smem;                    // [classes]
-  int* first = smem + prm.classes;     // [classes]
-  __shared__ int block_matches;
-
-  const int run = prm.block_run[blockIdx.x];
-  const int row = blockIdx.x * kBlock + threadIdx.x;
-  const int local = row - prm.pad_offsets[run];
-  const int L = prm.run_offsets[run + 1] - prm.run_offsets[run];
-  const bool active = local < L;
-  const bool reduce = prm.counts != nullptr;
-
-  if (reduce) {
-    for (int c = threadIdx.x; c < prm.classes; c += kBlock) {
-      hist[c] = 0;
-      first[c] = INT_MAX;
-    }
-    if (threadIdx.x == 0) block_matches = 0;
-    __syncthreads();
+// Label codes of a row's neighbours (-1 for empty slots). All neighbour ids are
+// loaded first and all label lookups issued next, with no branch on a loaded
+// value in between, so each batch of loads is in flight together. With STAGE the
+// run's labels (already remapped) sit in LDS at lab_s[local id]; otherwise they
+// are gathered from HBM and remapped here. Returns the number of neighbours.
+template <int KMAX, bool STAGE>
+__device__ __forceinline__ int load_codes(const VoteParams& prm, int row, int pad_base, const int* lab_s,
+                                          const int* remap, bool remapping, int (&code)[KMAX]) {
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    code[j] = j < prm.k ? prm.neighbors[static_cast<size_t>(j) * prm.pad_total + row] : -1;
   }
-
-  int best_code = -1;
-  int label = -1;
-  if (active) {
-    label = prm.labels[row];
-    int code[KMAX];
-    float sc[KMAX];
+  int gathered[KMAX];
 #pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-      code[j] = -1;
-      sc[j] = 0.0f;
-      if (j < prm.k) {
-        const size_t o = static_cast<size_t>(j) * prm.pad_total + row;
-        const int nb = prm.neighbors[o];
-        if (nb >= 0) {
-          code[j] = prm.labels[nb];
-          sc[j] = prm.scores[o];
-        }
-      }
+  for (int j = 0; j < KMAX; ++j) {
+    if (STAGE) gathered[j] = lab_s[code[j] >= 0 ? code[j] - pad_base : 0];
+    else gathered[j] = prm.labels[code[j] >= 0 ? code[j] : row];
+  }
+  int valid = 0;
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    if (code[j] >= 0) {
+      int c = gathered[j];
+      if (!STAGE && remapping && remap[c] >= 0) c = remap[c];
+      code[j] = c;
+      ++valid;
     }
-    int best_count = -1;
-    double best_sum = -INFINITY;
+  }
+  return valid;
+}
+
+// The full vote of one row: most votes, then the larger double sum of scores
+// in neighbour-rank order, then the smaller label (core.cpp:871-911).
+template <int KMAX>
+__device__ int full_tally(const VoteParams& prm, int row, const int (&code)[KMAX], int valid) {
+  // Pass 1, integers only, stopping once no unseen label can reach the best.
+  int best_code = -1, best_count = 0, seen = 0;
+  bool tie = false;
 #pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-      bool fresh = code[j] >= 0;
+  for (int j = 0; j < KMAX; ++j) {
+    if (code[j] >= 0 && best_count <= valid - seen) {
+      bool fresh = true;
 #pragma unroll
       for (int i = 0; i < j; ++i) fresh = fresh && (code[i] != code[j]);
       if (fresh) {
         int cnt = 0;
-        double sum = 0.0;
 #pragma unroll
-        for (int i = j; i < KMAX; ++i) {
-          if (code[i] == code[j]) {
-            ++cnt;
-            sum += static_cast<double>(sc[i]);
-          }
-        }
-        if (cnt > best_count || (cnt == best_count && (sum > best_sum || (sum == best_sum && code[j] < best_code)))) {
-          best_code = code[j];
+        for (int i = j; i < KMAX; ++i) cnt += code[i] == code[j] ? 1 : 0;
+        seen += cnt;
+        if (cnt > best_count) {
           best_count = cnt;
-          best_sum = sum;
+          best_code = code[j];
+          tie = false;
+        } else if (cnt == best_count) {
+          tie = true;
         }
       }
     }
+  }
+  if (!tie) return best_code;
+  // Pass 2, only when several labels share the best count.
+  float sc[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) sc[j] = code[j] >= 0 ? prm.scores[static_cast<size_t>(j) * prm.pad_total + row] : 0.0f;
+  double best_sum = -INFINITY;
+  best_code = -1;
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    if (code[j] >= 0) {
+      bool fresh = true;
+#pragma unroll
+      for (int i = 0; i < j; ++i) fresh = fresh && (code[i] != code[j]);
+      if (fresh) {
+        int cnt = 0;
+#pragma unroll
+        for (int i = j; i < KMAX; ++i) cnt += code[i] == code[j] ? 1 : 0;
+        if (cnt == best_count) {
+          double sum = 0.0;
+#pragma unroll
+          for (int i = j; i < KMAX; ++i) {
+            if (code[i] == code[j]) sum += static_cast<double>(sc[i]);
+          }
+          if (sum > best_sum || (sum == best_sum && code[j] < best_code)) {
+            best_sum = sum;
+            best_code = code[j];
+          }
+        }
+      }
+    }
+  }
+  return best_code;
+}
+
+template <int KMAX, bool STAGE>
+__global__ void __launch_bounds__(kBlock) vote_kernel(VoteParams prm) {
+  extern __shared__ int smem[];
+  const int C = prm.classes;
+  int* hist = smem;              // [C] label counts of this block
+  int* first = smem + C;         // [C] first row of each label in this block
+  int* remap = smem + 2 * C;     // [C] absorption remap of this run (device form)
+  int* queue = smem + 3 * C;     // [kBlock] rows left to the full tally
+  int* pred_s = queue + kBlock;  // [kBlock] their predictions
+  int* scal = pred_s + kBlock;   // block matches, queue length
+  int* lab_s = scal + 2;         // [rows of the run] staged labels (STAGE)
+
+  const int run = prm.block_run[blockIdx.x];
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  const int pad_base = prm.pad_offsets[run];
+  const int local = row - pad_base;
+  const int L = prm.run_offsets[run + 1] - prm.run_offsets[run];
+  const bool active = local < L;
+  const bool reduce = prm.counts != nullptr;
+  const bool remapping = prm.remap != nullptr && (prm.decision[run] & prm.remap_flag) != 0;
+  const size_t cb = static_cast<size_t>(run) * C;
+  // Reductions are double-buffered by cycle parity when a cycle counter is given.
+  const int parity = prm.cycle != nullptr ? (*prm.cycle & 1) : 0;
+  const size_t red = static_cast<size_t>(parity) * prm.runs * C;
+
+  for (int c = threadIdx.x; c < C; c += kBlock) {
+    hist[c] = 0;
+    first[c] = INT_MAX;
+    remap[c] = remapping ? prm.remap[cb + c] : -1;
+  }
+  if (threadIdx.x == 0) {
+    scal[0] = 0;
+    scal[1] = 0;
+  }
+  __syncthreads();
+  if (STAGE) {
+    // The run's labels, remapped. The remap is idempotent (targets map to
+    // themselves), so a label another block already rewrote stages the same.
+    for (int i = threadIdx.x; i < L; i += kBlock) {
+      const int c = prm.labels[pad_base + i];
+      lab_s[i] = remapping && remap[c] >= 0 ? remap[c] : c;
+    }
+    __syncthreads();
+  }
+
+  // Fast path: a label holding a strict majority of the votes is the unique
+  // winner. Other rows are queued and tallied densely below, so the quadratic
+  // tally only runs on full waves of mixed rows.
+  int best_code = -1;
+  int label = -1;
+  bool queued = false;
+  if (active) {
+    if (STAGE) {
+      label = lab_s[local];
+      if (remapping && label != prm.labels[row]) prm.relabel[row] = label;
+    } else {
+      label = prm.labels[row];
+      if (remapping && remap[label] >= 0) {
+        label = remap[label];
+        prm.relabel[row] = label;
+      }
+    }
+    int code[KMAX];
+    const int valid = load_codes<KMAX, STAGE>(prm, row, pad_base, lab_s, remap, remapping, code);
+    if (valid > 0) {
+      int c0 = 0;
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j) c0 += code[j] == code[0] ? 1 : 0;
+      if (2 * c0 > valid) {
+        best_code = code[0];
+      } else {
+        queue[atomicAdd(&scal[1], 1)] = threadIdx.x;
+        queued = true;
+      }
+    }
+  }
+  __syncthreads();
+  const int qn = scal[1];
+  for (int q = threadIdx.x; q < qn; q += kBlock) {
+    const int t = queue[q];
+    const int qrow = blockIdx.x * kBlock + t;
+    int code[KMAX];
+    const int valid = load_codes<KMAX, STAGE>(prm, qrow, pad_base, lab_s, remap, remapping, code);
+    pred_s[t] = full_tally<KMAX>(prm, qrow, code, valid);
+  }
+  __syncthreads();
+  if (active) {
+    if (queued) best_code = pred_s[threadIdx.x];
     if (best_code < 0) best_code = prm.fallback_code != nullptr ? prm.fallback_code[run] : -1;
     prm.predictions[row] = best_code;
   }
 
   if (!reduce) return;
   if (active) {
-    if (best_code == label) atomicAdd(&block_matches, 1);
+    if (best_code == label) atomicAdd(&scal[0], 1);
     atomicAdd(&hist[label], 1);
-    atomicMin(&first[label], local);
+    if (prm.firsts != nullptr) atomicMin(&first[label], local);
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < prm.classes; c += kBlock) {
+  for (int c = threadIdx.x; c < C; c += kBlock) {
     if (hist[c] != 0) {
-      atomicAdd(&prm.counts[static_cast<size_t>(run) * prm.classes + c], hist[c]);
-      atomicMin(&prm.firsts[static_cast<size_t>(run) * prm.classes + c], first[c]);
+      atomicAdd(&prm.counts[red + cb + c], hist[c]);
+      if (prm.firsts != nullptr) atomicMin(&prm.firsts[red + cb + c], first[c]);
     }
   }
-  if (threadIdx.x == 0 && block_matches != 0) atomicAdd(&prm.matches[run], block_matches);
+  if (threadIdx.x == 0 && scal[0] != 0) {
+    atomicAdd(&prm.matches[static_cast<size_t>(parity) * prm.runs + run], scal[0]);
+  }
 }
 
 __global__ void vote_reset_kernel(int* counts, int* firsts, int* matches, int runs, int classes) {
@@ -348,16 +466,27 @@ hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int 
   return launch_topk_p<256>(prm, mode, euclid, gx, runs, stream);
 }
 
+template <bool STAGE>
+static void launch_vote_k(const VoteParams& prm, dim3 grid, size_t shmem, hipStream_t stream) {
+  const dim3 block(kBlock);
+  if (prm.k <= 8) hipLaunchKernelGGL((vote_kernel<8, STAGE>), grid, block, shmem, stream, prm);
+  else if (prm.k <= 16) hipLaunchKernelGGL((vote_kernel<16, STAGE>), grid, block, shmem, stream, prm);
+  else if (prm.k <= 32) hipLaunchKernelGGL((vote_kernel<32, STAGE>), grid, block, shmem, stream, prm);
+  else hipLaunchKernelGGL((vote_kernel<64, STAGE>), grid, block, shmem, stream, prm);
+}
+
 hipError_t launch_vote(const VoteParams& prm, int blocks, hipStream_t stream) {
   if (prm.k < 1 || prm.k > kMaxK) return hipErrorInvalidValue;
   if (blocks == 0) return hipSuccess;
-  const size_t shmem = prm.counts != nullptr ? sizeof(int) * 2 * static_cast<size_t>(prm.classes) : 0;
-  if (shmem > 64 * 1024) return hipErrorInvalidValue;
-  const dim3 grid(blocks), block(kBlock);
-  if (prm.k <= 8) hipLaunchKernelGGL((vote_kernel<8>), grid, block, shmem, stream, prm);
-  else if (prm.k <= 16) hipLaunchKernelGGL((vote_kernel<16>), grid, block, shmem, stream, prm);
-  else if (prm.k <= 32) hipLaunchKernelGGL((vote_kernel<32>), grid, block, shmem, stream, prm);
-  else hipLaunchKernelGGL((vote_kernel<64>), grid, block, shmem, stream, prm);
+  const size_t base = sizeof(int) * (3 * static_cast<size_t>(prm.classes) + 2 * kBlock + 2);
+  if (base > 64 * 1024) return hipErrorInvalidValue;
+  // Stage the run's labels in LDS when they fit (48 KiB of labels at most).
+  const size_t staged = base + sizeof(int) * static_cast<size_t>(prm.max_rows);
+  if (prm.max_rows > 0 && staged <= 64 * 1024 && prm.max_rows <= 12288) {
+    launch_vote_k<true>(prm, dim3(blocks), staged, stream);
+  } else {
+    launch_vote_k<false>(prm, dim3(blocks), base, stream);
+  }
   return hipGetLastError();
 }
 

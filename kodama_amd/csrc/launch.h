@@ -47,10 +47,20 @@ struct VoteParams {
   int pad_total = 0;
   int k = 0;
   int classes = 0;                   // codes per run (array stride)
+  int max_rows = 0;                  // largest run (LDS label staging when it fits)
   int* predictions = nullptr;        // [pad_total]
   int* counts = nullptr;             // [runs x classes] label counts, null = skip reductions
-  int* firsts = nullptr;             // [runs x classes] first local row of each label
+  int* firsts = nullptr;             // [runs x classes] first local row of each label, may be null
   int* matches = nullptr;            // [runs] rows with prediction == label
+  int runs = 0;                      // stride of the parity double buffers
+  const int* cycle = nullptr;        // device cycle counter: reductions use buffer (cycle & 1)
+  // Device form: labels of absorbed runs go through remap [runs x classes]
+  // (-1 = unchanged) when decision[run] & remap_flag; the row's own remapped
+  // label is written to relabel (the labels buffer itself).
+  const int* remap = nullptr;
+  const int* decision = nullptr;
+  int remap_flag = 0;
+  int* relabel = nullptr;
 };
 
 hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,

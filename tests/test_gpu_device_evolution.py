@@ -27,11 +27,13 @@ OPTS = dict(cycles=25, k=15, folds=10, stratified=0, seed=3, cv_seed=3, evolutio
             guarded_diversity=1, many_to_one_absorption=1)
 
 
-def test_device_accuracy_is_the_vote_of_the_returned_labels(oracle):
+@pytest.mark.parametrize("k", [15, 30])
+def test_device_accuracy_is_the_vote_of_the_returned_labels(oracle, k):
     x, rows, init, folds = _batch()
-    out = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
+    opts = dict(OPTS, k=k)
+    out = K.core_knn_batch(x, rows, init, K.core_options(**opts), mode="device", fold_ids=folds)
     for r in range(len(rows)):
-        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], OPTS["k"])
+        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], k)
         cl = out["clbest"][r]
         pred = oracle.knn_vote(nbr, score, folds[r], cl)
         assert out["accbest"][r] == np.mean(pred == cl), r
