@@ -26,7 +26,7 @@ struct DeviceEvolution::Impl {
   std::vector<IVec> values;
   int runs = 0, classes = 0, cycles_cap = 0;
   DeviceBuffer<unsigned short> nbr16, cur, curpred, best, bestpred;
-  DeviceBuffer<int> flows, perm_half;
+  DeviceBuffer<int> flows, perm_bits;
   DeviceBuffer<uint64_t> run_key;
   DeviceBuffer<double> state, history;
   EvoParams prm;
@@ -49,7 +49,9 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.classes = batch.classes();
   m.cycles_cap = std::max(1, options.cycles);
   if (m.classes > 1024) throw InvalidArgument("device-resident evolution supports up to 1024 classes per run.");
-  if (batch.max_rows() > 8191) throw InvalidArgument("device-resident evolution supports up to 8191 rows per run.");
+  if (batch.max_rows() > kEvoMaxRows) {
+    throw InvalidArgument("device-resident evolution supports up to 8191 rows per run.");
+  }
   if (evo_lds_bytes(m.classes, batch.max_rows()) > kEvoMaxLds) {
     throw InvalidArgument("device-resident evolution: the run state does not fit the LDS of one compute unit.");
   }
@@ -59,13 +61,13 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.nbr16.resize(static_cast<size_t>(batch.k()) * pad);
   for (auto* b : {&m.cur, &m.curpred, &m.best, &m.bestpred}) b->resize(static_cast<size_t>(pad));
   m.flows.resize(flow_items);
-  m.perm_half.resize(static_cast<size_t>(m.runs));
+  m.perm_bits.resize(static_cast<size_t>(m.runs));
   m.run_key.resize(static_cast<size_t>(m.runs));
   m.state.resize(static_cast<size_t>(m.runs) * kEvoStateStride);
   m.history.resize(static_cast<size_t>(m.runs) * m.cycles_cap);
 
   std::vector<unsigned short> h_init(static_cast<size_t>(pad), 0);
-  std::vector<int> h_half(static_cast<size_t>(m.runs));
+  std::vector<int> h_bits(static_cast<size_t>(m.runs));
   std::vector<uint64_t> h_key(static_cast<size_t>(m.runs));
   for (int r = 0; r < m.runs; ++r) {
     for (size_t i = 0; i < codes[r].size(); ++i) {
@@ -74,16 +76,16 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     const int n = static_cast<int>(codes[r].size());
     int bits = 1;
     while ((1 << bits) < n) ++bits;
-    h_half[r] = (bits + 1) / 2;
+    h_bits[r] = bits;
     h_key[r] = splitmix(options.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
   }
   m.cur.upload(h_init.data(), h_init.size(), s);
   m.curpred.upload(h_init.data(), h_init.size(), s);
-  m.perm_half.upload(h_half.data(), h_half.size(), s);
+  m.perm_bits.upload(h_bits.data(), h_bits.size(), s);
   m.run_key.upload(h_key.data(), h_key.size(), s);
   check(launch_evo_zero(m.flows.get(), m.flows.size(), s), "evo_zero");
-  check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_pad_offsets().get(), batch.d_block_run().get(),
-                          m.nbr16.get(), m.nbr16.size(), pad, s),
+  check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
+                          batch.d_block_run().get(), m.nbr16.get(), m.nbr16.size(), pad, s),
         "evo_narrow");
 
   EvoParams& p = m.prm;
@@ -99,7 +101,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.neighbors = m.nbr16.get();
   p.scores = batch.d_scores().get();
   p.run_key = m.run_key.get();
-  p.perm_half = m.perm_half.get();
+  p.perm_bits = m.perm_bits.get();
   p.cur = m.cur.get();
   p.curpred = m.curpred.get();
   p.best = m.best.get();

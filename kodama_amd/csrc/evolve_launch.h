@@ -9,11 +9,12 @@
 
 namespace kodama_hip {
 
-constexpr int kEvoThreads = 1024;           // one workgroup per run
+constexpr int kEvoThreads = 512;            // one workgroup per run
 constexpr size_t kEvoMaxLds = 160 * 1024;   // LDS of one CDNA4 compute unit
 constexpr int kEvoAccepted = 1;
 constexpr int kEvoImproved = 2;
 constexpr int kEvoStateStride = 4;  // cur_acc, cur_score, best_acc, best_score
+constexpr int kEvoMaxRows = 8191;   // rows of one run (13-bit fields of the absorption keys)
 
 struct EvoParams {
   int runs = 0;
@@ -24,14 +25,14 @@ struct EvoParams {
   int init = 0;           // evaluate the starting labels first
   int guarded = 1;        // diversity-guarded objective
   int k = 0;
-  int max_rows = 0;       // largest run, <= 8191
+  int max_rows = 0;       // largest run, <= kEvoMaxRows
   int pad_total = 0;
   const int* run_offsets = nullptr;          // [runs + 1]
   const int* pad_offsets = nullptr;          // [runs + 1]
-  const unsigned short* neighbors = nullptr; // [k x pad_total] local ids, 0xFFFF = none
+  const unsigned short* neighbors = nullptr; // [k x pad_total] local ids, run rows = none
   const float* scores = nullptr;             // [k x pad_total]
   const uint64_t* run_key = nullptr;         // [runs]
-  const int* perm_half = nullptr;            // [runs] Feistel half width
+  const int* perm_bits = nullptr;            // [runs] ceil(log2 rows), at least 1
   unsigned short* cur = nullptr;             // [pad_total] run state between launches
   unsigned short* curpred = nullptr;
   unsigned short* best = nullptr;
@@ -43,9 +44,10 @@ struct EvoParams {
 
 size_t evo_lds_bytes(int classes, int max_rows);
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
-// 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids.
-hipError_t launch_evo_narrow(const int* nbr32, const int* pad_offsets, const int* block_run, unsigned short* nbr16,
-                             size_t total, int pad_total, hipStream_t s);
+// 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids; a
+// missing neighbour (-1) becomes the run's row count, its sentinel slot.
+hipError_t launch_evo_narrow(const int* nbr32, const int* run_offsets, const int* pad_offsets, const int* block_run,
+                             unsigned short* nbr16, size_t total, int pad_total, hipStream_t s);
 hipError_t launch_evo_zero(int* a, size_t n, hipStream_t s);
 
 }  // namespace kodama_hip

@@ -27,7 +27,7 @@ OPTS = dict(cycles=25, k=15, folds=10, stratified=0, seed=3, cv_seed=3, evolutio
             guarded_diversity=1, many_to_one_absorption=1)
 
 
-@pytest.mark.parametrize("k", [15, 30])
+@pytest.mark.parametrize("k", [5, 15, 30, 40])
 def test_device_accuracy_is_the_vote_of_the_returned_labels(oracle, k):
     x, rows, init, folds = _batch()
     opts = dict(OPTS, k=k)
@@ -41,12 +41,36 @@ def test_device_accuracy_is_the_vote_of_the_returned_labels(oracle, k):
         assert out["vect_acc"][r][-1] == out["accbest"][r]
 
 
+def _centre_batch(n_runs=24, n=6000, seed=31, centres=40):
+    """Runs whose starting labels are the nearest of `centres` random rows (one
+    k-means assignment step), as KODAMA starts from a clustering."""
+    x, _ = D.config3(n=n, seed=seed)
+    rows, init, folds = [], [], []
+    for r in range(n_runs):
+        L = int(D.randint(seed, 50 + r, 1, 300, 1200)[0])
+        rw = np.sort(np.argsort(D.uniform(seed, 200 + r, n))[:L]).astype(np.int32)
+        c = x[rw[np.argsort(D.uniform(seed, 800 + r, L))[:centres]]]
+        idx, _ = K.exact_knn_search(c, x[rw], 1, "euclidean")
+        rows.append(rw)
+        init.append((idx[:, 0] + 1).astype(np.int32))
+        folds.append((np.argsort(D.uniform(seed, 600 + r, L)) % 10).astype(np.int32))
+    return x, rows, init, folds
+
+
 def test_device_reaches_plain_accuracy():
-    x, rows, init, folds = _batch(n_runs=24)
-    plain = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="plain", fold_ids=folds)
-    dev = K.core_knn_batch(x, rows, init, K.core_options(**OPTS), mode="device", fold_ids=folds)
-    gap = abs(np.mean(dev["accbest"]) - np.mean(plain["accbest"]))
-    assert gap < 0.02, (np.mean(dev["accbest"]), np.mean(plain["accbest"]))
+    """Equal accuracy = the device form's mean best CV accuracy over 4 seeds x 24
+    runs is no more than half a point below the plain form's. (From random
+    starting labels single runs of either form scatter widely with the draws.)"""
+    x, rows, init, folds = _centre_batch()
+    means = {}
+    for mode in ("plain", "device"):
+        acc = []
+        for seed in (1, 2, 3, 4):
+            opts = dict(OPTS, seed=seed, cv_seed=seed)
+            out = K.core_knn_batch(x, rows, init, K.core_options(**opts), mode=mode, fold_ids=folds)
+            acc.append(np.mean(out["accbest"]))
+        means[mode] = float(np.mean(acc))
+    assert means["device"] >= means["plain"] - 0.005, means
 
 
 def test_device_is_deterministic():
