@@ -1,0 +1,92 @@
+// C ABI of libkodama_hip.so, graph half: the exact self-kNN graph
+// (KODAMAKNNGraph_CUDA, graph_cluster.cpp:790-837) and the KODAMA dissimilarity
+// rewrite (KODAMADissimilarityInPlace, kodama_matrix.cpp:2685-2776, 4383-4463).
+#include <algorithm>
+#include <numeric>
+
+#include "../../include/kodama_hip.h"
+#include "capi_common.h"
+#include "engine.h"
+#include "graph_launch.h"
+#include "launch.h"
+
+using namespace kodama_hip;
+
+namespace {
+
+void require(bool ok, const char* msg) {
+  if (!ok) throw InvalidArgument(msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kodama_hip_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_t metric, int32_t device,
+                         int32_t* indices_one_based, float* distances) {
+  return guarded([&] {
+    require(x != nullptr && indices_one_based != nullptr && distances != nullptr, "null pointer.");
+    require(n >= 2 && p >= 1, "KODAMAKNNGraph requires at least two rows.");
+    require(metric >= KODAMA_HIP_COSINE && metric <= KODAMA_HIP_EUCLIDEAN, "unknown metric.");
+    const int kk = std::max(1, std::min(k, n - 1));
+    require(kk <= kMaxK, "graph neighbours above 64 are not supported by the HIP kNN kernel.");
+    require(p <= kMaxDims, "the HIP kNN kernel supports 1..256 feature columns.");
+    // Cosine rows are normalised as the reference's exact search expects
+    // (native_knn.cpp:1005-1035).
+    const std::vector<float> xs = prepare_rows(x, n, p, metric, 1);
+    select_device(device);
+    Stream st(device);
+    const hipStream_t s = st.get();
+    DeviceBuffer<float> d_x(static_cast<size_t>(n) * p);
+    DeviceBuffer<int> d_ex(static_cast<size_t>(n));
+    DeviceBuffer<int> d_idx(static_cast<size_t>(n) * kk);
+    DeviceBuffer<float> d_dist(static_cast<size_t>(n) * kk);
+    std::vector<int> ex(static_cast<size_t>(n));
+    std::iota(ex.begin(), ex.end(), 0);
+    d_x.upload(xs.data(), d_x.size(), s);
+    d_ex.upload(ex.data(), ex.size(), s);
+    KnnTopkParams prm;
+    prm.train = d_x.get();
+    prm.query = d_x.get();
+    prm.train_rows = n;
+    prm.query_rows = n;
+    prm.dims = p;
+    prm.k = kk;
+    prm.exclude = d_ex.get();
+    prm.out_index = d_idx.get();
+    prm.out_dist = d_dist.get();
+    check(launch_knn_topk(prm, 0, metric == KODAMA_HIP_EUCLIDEAN, n, 1, s), "knn_topk launch");
+    check(launch_graph_publish(d_idx.get(), d_dist.get(), d_idx.size(), metric, s), "graph_publish launch");
+    d_idx.download(indices_one_based, d_idx.size(), s);
+    d_dist.download(distances, d_dist.size(), s);
+    st.sync();
+  });
+}
+
+int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distances, int32_t samples,
+                                     int32_t neighbors, const int32_t* run_labels, int32_t runs, int32_t device) {
+  return guarded([&] {
+    if (runs <= 0 || samples <= 0 || neighbors <= 0) return;  // the reference leaves the graph as is
+    require(indices_one_based != nullptr && distances != nullptr && run_labels != nullptr, "null pointer.");
+    require(dissimilarity_lds_bytes(neighbors) <= 64 * 1024, "dissimilarity supports up to 1365 neighbours per row.");
+    select_device(device);
+    Stream st(device);
+    const hipStream_t s = st.get();
+    const int stride = (runs + 3) / 4 * 4;
+    const size_t edges = static_cast<size_t>(samples) * neighbors;
+    DeviceBuffer<int> d_idx(edges), d_res(static_cast<size_t>(runs) * samples),
+      d_lab(static_cast<size_t>(samples) * stride);
+    DeviceBuffer<float> d_dist(edges);
+    d_idx.upload(indices_one_based, edges, s);
+    d_dist.upload(distances, edges, s);
+    d_res.upload(run_labels, d_res.size(), s);
+    check(launch_labels_transpose(d_res.get(), runs, samples, stride, d_lab.get(), s), "labels_transpose launch");
+    check(launch_dissimilarity(d_idx.get(), d_dist.get(), samples, neighbors, d_lab.get(), stride, true, true, s),
+          "dissimilarity launch");
+    d_idx.download(indices_one_based, edges, s);
+    d_dist.download(distances, edges, s);
+    st.sync();
+  });
+}
+
+}  // extern "C"
