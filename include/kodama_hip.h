@@ -174,6 +174,21 @@ int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* 
                            const kodama_hip_core_options* options,
                            kodama_hip_core_result* result);
 
+/* No counterpart in the reference (its fold internals are not exported): the
+ * detail of one PLS-LDA fold, for callers that check the FP64 device results
+ * (tests against an FP64 restatement). Evaluates `labels` under the caller's
+ * fold ids and reports fold `fold` (a fold id): SIMPLS weights [p x rank]
+ * row-major (rank 0 for single-class or degenerate folds), validation scores
+ * [n_val x components] and LDA discriminants [n_val x n_classes] for the
+ * fold's validation rows in ascending order, and the fold's training classes
+ * (label values, ascending). Buffers: weights p x max(max_components, 1),
+ * val_scores and discriminants n x n, classes n. */
+int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                                  const int32_t* folds, int32_t fold,
+                                  const kodama_hip_core_options* options, double* weights,
+                                  int32_t* rank, double* val_scores, double* discriminants,
+                                  int32_t* classes, int32_t* n_classes, int32_t* n_val);
+
 /* kodama::KODAMAKNNGraph_CUDA (kodama.hpp:1524-1527; graph_cluster.cpp:790-837):
  * exact self-kNN graph, k = min(k, n-1), self excluded; 1-based ids and public
  * distances (sqrt of squared L2 for Euclidean). Output [n x min(k, n-1)]. */
@@ -198,8 +213,9 @@ int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distance
  * options->seed + r for the proposals. mode 0 = plain (proposals on the host
  * with the reference's random streams: per run bit-identical to
  * kodama_hip_core_knn); mode 1 = device-resident (proposal, transition, vote
- * and acceptance on the device, one graph launch per cycle, counter-based
- * random streams; same algorithm, different random draws).
+ * and acceptance on the device, one persistent launch for all cycles with one
+ * workgroup per run, counter-based random streams; same algorithm, different
+ * random draws; runs of up to 8191 rows and 1024 classes).
  * Outputs: clbest (same indexing as landmark_rows), accbest[runs],
  * vect_acc[runs x cycles]; timings[4] (may be NULL): precompute seconds,
  * evolution seconds, steps completed, and the device seconds of the dominant

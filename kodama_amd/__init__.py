@@ -131,6 +131,29 @@ def PLSLDACV(data, labels, constrain=None, folds=10, stratified=True, seed=1, nc
                 backend="hip")
 
 
+def plslda_fold_detail(data, labels, folds, fold, options: CoreOptions) -> dict:
+    """One PLS-LDA fold evaluated on the device in FP64: SIMPLS weights,
+    validation scores and LDA discriminants (validation rows ascending)."""
+    x = _matrix(data)
+    lab = N.i32(labels)
+    fo = N.i32(folds)
+    n, p = x.shape
+    width = max(1, options.max_components)
+    w = np.zeros(p * width, np.float64)
+    vs = np.zeros(n * n, np.float64)
+    disc = np.zeros(n * n, np.float64)
+    cls = np.zeros(n, np.int32)
+    rank, ncls, nval = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    N.call("kodama_hip_plslda_fold_detail", N.ptr(x), C.c_int32(n), C.c_int32(p), N.ptr(lab), N.ptr(fo),
+           C.c_int32(fold), C.byref(options), N.ptr(w), C.byref(rank), N.ptr(vs), N.ptr(disc), N.ptr(cls),
+           C.byref(ncls), C.byref(nval))
+    r, nc, nv = rank.value, ncls.value, nval.value
+    a = min(options.fixed_components or options.max_components, r) if r > 0 else 0
+    return dict(weights=w[: p * r].reshape(p, r), val_scores=vs[: nv * a].reshape(nv, a),
+                discriminant=disc[: nv * nc].reshape(nv, nc) if r else np.zeros((nv, 0)),
+                classes=cls[:nc].copy(), rank=r)
+
+
 # ---- label evolution ----------------------------------------------------------
 
 def core_knn(data, labels, options: CoreOptions, constrain=None, fix=None) -> dict:

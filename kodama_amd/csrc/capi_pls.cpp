@@ -1,0 +1,154 @@
+// C ABI of libkodama_hip.so, PLS-LDA half: PLSLDACV, CorePLSLDA and the fold
+// detail used to check scores and discriminants against an FP64 restatement.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/kodama_hip.h"
+#include "capi_common.h"
+#include "engine.h"
+#include "pls_engine.h"
+
+using namespace kodama_hip;
+
+namespace {
+
+void require(bool ok, const char* msg) {
+  if (!ok) throw InvalidArgument(msg);
+}
+
+void validate(const float* x, int n, int p, const int* labels, const kodama_hip_core_options* o) {
+  require(x != nullptr, "MatrixView data pointer is null.");
+  require(n > 0 && p > 0, "Input matrix must have at least one row and one column.");
+  require(labels != nullptr, "labels size must match number of rows.");
+  require(o != nullptr, "null options.");
+  require(o->max_components >= 1, "PLSOptions::max_components must be positive.");
+  require(o->folds >= 2, "PLS cross-validation needs at least two folds.");
+}
+
+IVec vec_or_empty(const int* p, int n) { return p ? IVec(p, p + n) : IVec(); }
+
+IVec values_of(const IVec& codes, const IVec& values) {
+  IVec out(codes.size());
+  for (size_t i = 0; i < codes.size(); ++i) out[i] = values[codes[i]];
+  return out;
+}
+
+// Class statistics of a labelling for the evolution engine.
+Tally tally_of(const IVec& codes, const int* pred, int classes, IVec& counts, IVec& firsts) {
+  counts.assign(static_cast<size_t>(classes), 0);
+  firsts.assign(static_cast<size_t>(classes), -1);
+  Tally t;
+  for (size_t i = 0; i < codes.size(); ++i) {
+    const int c = codes[i];
+    if (firsts[c] < 0) firsts[c] = static_cast<int>(i);
+    ++counts[c];
+    t.matches += pred[i] == c ? 1 : 0;
+  }
+  t.counts = counts.data();
+  t.firsts = firsts.data();
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kodama_hip_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                        const kodama_hip_core_options* options, int32_t* predicted_labels,
+                        int32_t* fold_assignments, int32_t* selected_components) {
+  return guarded([&] {
+    validate(x, n, p, labels, options);
+    require(predicted_labels != nullptr, "null output.");
+    select_device(options->device);
+    IVec values, codes;
+    encode_labels(labels, n, values, codes);
+    const IVec lab(labels, labels + n);
+    const IVec folds = make_folds(lab, vec_or_empty(constrain, n), options->folds, options->stratified != 0,
+                                  options->cv_seed);
+    PlsCv pls(options->device, x, n, p, *options, static_cast<int>(values.size()));
+    IVec pred(static_cast<size_t>(n));
+    int sel = 0;
+    pls.evaluate(codes, folds, pred.data(), &sel);
+    for (int i = 0; i < n; ++i) predicted_labels[i] = values[pred[i]];
+    if (fold_assignments) std::copy(folds.begin(), folds.end(), fold_assignments);
+    if (selected_components) *selected_components = sel;
+  });
+}
+
+// CorePLSLDA: maximize_core with the PLS predictor (core.cpp:1720-1749). With
+// unstratified folds the fold cache keeps the folds of the first evaluation
+// (plscv.cpp:2312); stratified folds follow the labels of every evaluation.
+int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                           const int32_t* fixed, const kodama_hip_core_options* options,
+                           kodama_hip_core_result* result) {
+  return guarded([&] {
+    validate(x, n, p, labels, options);
+    require(result != nullptr, "null result.");
+    require(options->cycles >= 0, "Core cycles must be non-negative.");
+    select_device(options->device);
+    CoreFlags f = CoreFlags::from(*options);
+    f.knn = false;
+    IVec values, codes;
+    encode_labels(labels, n, values, codes);
+    const IVec cons = vec_or_empty(constrain, n);
+    const IVec init(labels, labels + n);
+    const bool cache = options->stratified == 0;
+    const IVec first_folds = make_folds(init, cons, options->folds, options->stratified != 0, options->cv_seed);
+    const int C = static_cast<int>(values.size());
+    PlsCv pls(options->device, x, n, p, *options, C);
+    RunEvolution run(values, codes, cons, vec_or_empty(fixed, n), f);
+    IVec pred(static_cast<size_t>(n)), counts, firsts;
+    auto evaluate = [&](const IVec& cl) {
+      const IVec folds = cache ? first_folds
+                               : make_folds(values_of(cl, values), cons, options->folds, true, options->cv_seed);
+      pls.evaluate(cl, folds, pred.data(), nullptr);
+      return tally_of(cl, pred.data(), C, counts, firsts);
+    };
+    {
+      const Tally t = evaluate(run.initial());
+      run.begin(pred.data(), t);
+    }
+    for (int cycle = 0; cycle < options->cycles; ++cycle) {
+      const IVec& cl = run.propose(cycle);
+      const Tally t = evaluate(cl);
+      run.finish(cycle, pred.data(), t);
+    }
+    run.export_result(result);
+    if (result->fold_assignments) std::copy(first_folds.begin(), first_folds.end(), result->fold_assignments);
+  });
+}
+
+int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                                  const int32_t* folds, int32_t fold, const kodama_hip_core_options* options,
+                                  double* weights, int32_t* rank, double* val_scores, double* discriminants,
+                                  int32_t* classes, int32_t* n_classes, int32_t* n_val) {
+  return guarded([&] {
+    validate(x, n, p, labels, options);
+    require(folds != nullptr && weights != nullptr && rank != nullptr && val_scores != nullptr &&
+              discriminants != nullptr && classes != nullptr && n_classes != nullptr && n_val != nullptr,
+            "null pointer.");
+    select_device(options->device);
+    IVec values, codes;
+    encode_labels(labels, n, values, codes);
+    const IVec fv(folds, folds + n);
+    IVec ids = fv;
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    const auto it = std::find(ids.begin(), ids.end(), fold);
+    require(it != ids.end(), "no such fold.");
+    PlsCv pls(options->device, x, n, p, *options, static_cast<int>(values.size()));
+    IVec pred(static_cast<size_t>(n));
+    pls.evaluate(codes, fv, pred.data(), nullptr);
+    const PlsFoldDetail d = pls.detail(static_cast<int>(it - ids.begin()));
+    *rank = d.rank;
+    *n_classes = static_cast<int>(d.classes.size());
+    *n_val = static_cast<int>(d.val_rows.size());
+    std::copy(d.weights.begin(), d.weights.end(), weights);
+    std::copy(d.val_scores.begin(), d.val_scores.end(), val_scores);
+    std::copy(d.discriminants.begin(), d.discriminants.end(), discriminants);
+    for (size_t c = 0; c < d.classes.size(); ++c) classes[c] = values[d.classes[c]];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+// Host side of the device PLS-LDA cross-validation: fold bookkeeping, the
+// class-grouped training rows of each evaluation, launches and result checks.
+// The fold partition and the evaluation order follow run_plscv_host
+// (plscv.cpp:2293-2507): folds in ascending id order, validation rows in
+// ascending order, the selected component count = min over folds.
+#include "pls_engine.h"
+
+#include <algorithm>
+#include <random>
+#include <stdexcept>
+
+#include "pls_launch.h"
+
+namespace kodama_hip {
+
+std::vector<double> pls_start_vector(int p) {
+  std::mt19937 gen(1u);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  std::vector<double> w(static_cast<size_t>(p));
+  for (double& v : w) v = static_cast<double>(nd(gen));
+  return w;
+}
+
+struct PlsCv::Impl {
+  int device = 0, n = 0, p = 0, classes = 0;
+  int requested = 0, max_components = 0, rank_cap = 0;
+  bool center = true, scale = true;
+  Stream stream;
+  DeviceBuffer<float> x;
+  DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, s0, gw, linear, constant, val_scores, disc;
+  DeviceBuffer<int> train_asc, train_off, train_sorted, seg_code, seg_off, seg_start, val_rows, val_off;
+  DeviceBuffer<int> pred, comps, rank;
+  IVec cached_folds, fold_ids, h_train_rows, h_train_off, h_val_rows, h_val_off;
+  IVec h_seg_code, h_seg_off, h_comps, h_rank;
+  int last_cmax = 0;
+
+  explicit Impl(int dev) : device(dev), stream(dev) {}
+
+  void rebuild_folds(const IVec& folds);
+};
+
+void PlsCv::Impl::rebuild_folds(const IVec& folds) {
+  fold_ids = folds;
+  std::sort(fold_ids.begin(), fold_ids.end());
+  fold_ids.erase(std::unique(fold_ids.begin(), fold_ids.end()), fold_ids.end());
+  const int F = static_cast<int>(fold_ids.size());
+  h_train_rows.clear();
+  h_val_rows.clear();
+  h_train_off.assign(1, 0);
+  h_val_off.assign(1, 0);
+  for (int fid : fold_ids) {
+    for (int i = 0; i < n; ++i) (folds[i] == fid ? h_val_rows : h_train_rows).push_back(i);
+    h_train_off.push_back(static_cast<int>(h_train_rows.size()));
+    h_val_off.push_back(static_cast<int>(h_val_rows.size()));
+  }
+  for (int f = 0; f < F; ++f) {
+    if (h_train_off[f + 1] == h_train_off[f]) throw std::runtime_error("PLS fold has no training samples.");
+  }
+  const hipStream_t s = stream.get();
+  train_asc.resize(h_train_rows.size());
+  train_sorted.resize(h_train_rows.size());
+  train_off.resize(h_train_off.size());
+  val_rows.resize(h_val_rows.size());
+  val_off.resize(h_val_off.size());
+  train_asc.upload(h_train_rows.data(), h_train_rows.size(), s);
+  train_off.upload(h_train_off.data(), h_train_off.size(), s);
+  val_rows.upload(h_val_rows.data(), h_val_rows.size(), s);
+  val_off.upload(h_val_off.data(), h_val_off.size(), s);
+  const size_t fp = static_cast<size_t>(F) * p;
+  mean.resize(fp);
+  sd.resize(fp);
+  xsum.resize(fp);
+  gram.resize(fp * p);
+  weights.resize(fp * kPlsMaxComponents);
+  loadings.resize(fp * kPlsMaxComponents);
+  gw.resize(fp * kPlsMaxComponents);
+  s0.resize(static_cast<size_t>(F) * classes * p);
+  linear.resize(static_cast<size_t>(F) * classes * kPlsMaxComponents);
+  constant.resize(static_cast<size_t>(F) * classes);
+  comps.resize(static_cast<size_t>(F));
+  rank.resize(static_cast<size_t>(F));
+  PlsCacheParams c;
+  c.x = x.get();
+  c.p = p;
+  c.folds = F;
+  c.center = center ? 1 : 0;
+  c.scale = scale ? 1 : 0;
+  c.train.rows = train_asc.get();
+  c.train.offsets = train_off.get();
+  c.mean = mean.get();
+  c.sd = sd.get();
+  c.xsum = xsum.get();
+  c.gram = gram.get();
+  check(launch_pls_cache(c, s), "pls_cache launch");
+  cached_folds = folds;
+}
+
+PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& o, int classes) {
+  if (o.max_components < 1) throw InvalidArgument("PLSOptions::max_components must be positive.");
+  if (p > kPlsMaxDims) throw InvalidArgument("the HIP PLS-LDA kernels support up to 256 feature columns.");
+  Impl* m = new Impl(device);
+  m->n = n;
+  m->p = p;
+  m->classes = std::max(1, classes);
+  m->max_components = o.max_components;
+  m->requested = o.fixed_components > 0 ? o.fixed_components : o.max_components;
+  m->rank_cap = std::min(o.max_components, p);
+  m->center = o.center != 0;
+  m->scale = o.scale != 0;
+  if (m->rank_cap > kPlsMaxComponents) {
+    delete m;
+    throw InvalidArgument("the HIP PLS-LDA kernels support up to 64 components.");
+  }
+  const hipStream_t s = m->stream.get();
+  m->x.resize(static_cast<size_t>(n) * p);
+  m->x.upload(x, m->x.size(), s);
+  const std::vector<double> w0 = pls_start_vector(p);
+  m->w0.resize(w0.size());
+  m->w0.upload(w0.data(), w0.size(), s);
+  m->pred.resize(static_cast<size_t>(n));
+  m->val_scores.resize(static_cast<size_t>(n) * kPlsMaxComponents);
+  m->disc.resize(static_cast<size_t>(n) * m->classes);
+  impl_ = m;
+}
+
+PlsCv::~PlsCv() { delete impl_; }
+
+void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected) {
+  Impl& m = *impl_;
+  if (static_cast<int>(codes.size()) != m.n || static_cast<int>(folds.size()) != m.n) {
+    throw InvalidArgument("labels and folds must have one entry per row.");
+  }
+  if (folds != m.cached_folds) m.rebuild_folds(folds);
+  const int F = static_cast<int>(m.fold_ids.size());
+  // Training rows of each fold grouped by class code (rows ascending inside a class).
+  IVec sorted;
+  sorted.reserve(m.h_train_rows.size());
+  m.h_seg_code.clear();
+  m.h_seg_off.assign(1, 0);
+  IVec seg_start;
+  std::vector<IVec> bucket(static_cast<size_t>(m.classes));
+  int cmax = 1;
+  for (int f = 0; f < F; ++f) {
+    for (auto& b : bucket) b.clear();
+    for (int i = m.h_train_off[f]; i < m.h_train_off[f + 1]; ++i) {
+      const int r = m.h_train_rows[i];
+      const int c = codes[r];
+      if (c < 0 || c >= m.classes) throw InvalidArgument("label code out of range.");
+      bucket[c].push_back(r);
+    }
+    int cf = 0;
+    for (int c = 0; c < m.classes; ++c) {
+      if (bucket[c].empty()) continue;
+      m.h_seg_code.push_back(c);
+      seg_start.push_back(static_cast<int>(sorted.size()));
+      sorted.insert(sorted.end(), bucket[c].begin(), bucket[c].end());
+      ++cf;
+    }
+    seg_start.push_back(static_cast<int>(sorted.size()));
+    m.h_seg_off.push_back(static_cast<int>(m.h_seg_code.size()));
+    cmax = std::max(cmax, cf);
+  }
+  if (static_cast<size_t>(cmax) * m.p > static_cast<size_t>(kPlsMaxSLds) ||
+      pls_eval_lds_bytes(m.p, cmax) > 160 * 1024) {
+    throw InvalidArgument("the HIP PLS-LDA kernel holds classes x columns <= 8192 in LDS.");
+  }
+  const hipStream_t s = m.stream.get();
+  m.train_sorted.upload(sorted.data(), sorted.size(), s);
+  m.seg_code.resize(m.h_seg_code.size());
+  m.seg_off.resize(m.h_seg_off.size());
+  m.seg_start.resize(seg_start.size());
+  m.seg_code.upload(m.h_seg_code.data(), m.h_seg_code.size(), s);
+  m.seg_off.upload(m.h_seg_off.data(), m.h_seg_off.size(), s);
+  m.seg_start.upload(seg_start.data(), seg_start.size(), s);
+  PlsEvalParams e;
+  e.x = m.x.get();
+  e.p = m.p;
+  e.folds = F;
+  e.requested = m.requested;
+  e.rank_cap = m.rank_cap;
+  e.w0 = m.w0.get();
+  e.mean = m.mean.get();
+  e.sd = m.sd.get();
+  e.xsum = m.xsum.get();
+  e.gram = m.gram.get();
+  e.train.rows = m.train_sorted.get();
+  e.train.offsets = m.train_off.get();
+  e.seg_code = m.seg_code.get();
+  e.seg_offsets = m.seg_off.get();
+  e.seg_start = m.seg_start.get();
+  e.val.rows = m.val_rows.get();
+  e.val.offsets = m.val_off.get();
+  e.max_classes = cmax;
+  e.weights = m.weights.get();
+  e.loadings = m.loadings.get();
+  e.s0 = m.s0.get();
+  e.gw = m.gw.get();
+  e.linear = m.linear.get();
+  e.constant = m.constant.get();
+  e.pred_code = m.pred.get();
+  e.components = m.comps.get();
+  e.rank = m.rank.get();
+  e.val_scores = m.val_scores.get();
+  e.discriminants = m.disc.get();
+  check(launch_pls_eval(e, s), "pls_eval launch");
+  m.last_cmax = cmax;
+  m.h_comps.assign(static_cast<size_t>(F), 0);
+  m.h_rank.assign(static_cast<size_t>(F), 0);
+  m.pred.download(pred_codes, static_cast<size_t>(m.n), s);
+  m.comps.download(m.h_comps.data(), m.h_comps.size(), s);
+  m.rank.download(m.h_rank.data(), m.h_rank.size(), s);
+  m.stream.sync();
+  int evaluated = m.requested;
+  for (int f = 0; f < F; ++f) {
+    if (m.h_comps[f] < 0) throw std::runtime_error("PLS-LDA Cholesky factorization failed.");
+    evaluated = std::min(evaluated, m.h_comps[f]);
+  }
+  if (selected) *selected = std::min(evaluated, m.max_components);
+}
+
+PlsFoldDetail PlsCv::detail(int fold) const {
+  const Impl& m = *impl_;
+  const int F = static_cast<int>(m.fold_ids.size());
+  if (fold < 0 || fold >= F || m.h_comps.empty()) throw InvalidArgument("no such fold in the last evaluation.");
+  PlsFoldDetail d;
+  d.rank = m.h_rank[fold];
+  d.components = m.h_comps[fold];
+  d.classes.assign(m.h_seg_code.begin() + m.h_seg_off[fold], m.h_seg_code.begin() + m.h_seg_off[fold + 1]);
+  d.val_rows.assign(m.h_val_rows.begin() + m.h_val_off[fold], m.h_val_rows.begin() + m.h_val_off[fold + 1]);
+  const int nv = static_cast<int>(d.val_rows.size());
+  const int cf = static_cast<int>(d.classes.size());
+  const hipStream_t s = m.stream.get();
+  std::vector<double> w(static_cast<size_t>(m.p) * kPlsMaxComponents);
+  std::vector<double> tv(static_cast<size_t>(nv) * kPlsMaxComponents);
+  std::vector<double> disc(static_cast<size_t>(nv) * m.last_cmax);
+  m.weights.download(w.data(), w.size(), s, static_cast<size_t>(fold) * m.p * kPlsMaxComponents);
+  m.val_scores.download(tv.data(), tv.size(), s, static_cast<size_t>(m.h_val_off[fold]) * kPlsMaxComponents);
+  m.disc.download(disc.data(), disc.size(), s, static_cast<size_t>(m.h_val_off[fold]) * m.last_cmax);
+  check(hipStreamSynchronize(s), "detail sync");
+  const bool fitted = d.rank > 0 && cf > 1;
+  const int A = fitted ? d.components : 0;
+  d.weights.assign(static_cast<size_t>(m.p) * (fitted ? d.rank : 0), 0.0);
+  for (int j = 0; j < m.p && fitted; ++j)
+    for (int a = 0; a < d.rank; ++a) d.weights[static_cast<size_t>(j) * d.rank + a] = w[static_cast<size_t>(j) * kPlsMaxComponents + a];
+  d.val_scores.assign(static_cast<size_t>(nv) * A, 0.0);
+  d.discriminants.assign(static_cast<size_t>(nv) * (fitted ? cf : 0), 0.0);
+  for (int i = 0; i < nv && fitted; ++i) {
+    for (int a = 0; a < A; ++a) d.val_scores[static_cast<size_t>(i) * A + a] = tv[static_cast<size_t>(i) * kPlsMaxComponents + a];
+    for (int c = 0; c < cf; ++c) d.discriminants[static_cast<size_t>(i) * cf + c] = disc[static_cast<size_t>(i) * m.last_cmax + c];
+  }
+  return d;
+}
+
+}  // namespace kodama_hip

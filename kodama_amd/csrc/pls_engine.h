@@ -1,0 +1,49 @@
+// Host side of the device PLS-LDA cross-validation (pls_engine.cpp): the data
+// stays on the device, the fold cache is rebuilt only when the folds change,
+// and each evaluation uploads the training rows grouped by class.
+#pragma once
+
+#include <vector>
+
+#include "engine.h"
+
+namespace kodama_hip {
+
+// Per-fold detail of one evaluation (validation rows in ascending order).
+struct PlsFoldDetail {
+  int rank = 0;                     // fitted SIMPLS components (0: degenerate)
+  int components = 0;
+  std::vector<double> weights;      // [p x rank]
+  std::vector<double> val_scores;   // [val rows x components]
+  std::vector<double> discriminants;  // [val rows x classes of the fold]
+  IVec classes;                     // label codes of the fold's training classes
+  IVec val_rows;
+};
+
+class PlsCv {
+ public:
+  // x [n x p] host rows; `classes` = number of label codes (0..classes-1).
+  PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& options, int classes);
+  ~PlsCv();
+  PlsCv(const PlsCv&) = delete;
+  PlsCv& operator=(const PlsCv&) = delete;
+
+  // One F-fold evaluation of the labelling `codes` under `folds` (same
+  // indexing). Writes the predicted code of every row; *selected = the
+  // component count PLSLDACV reports (min over folds, capped by
+  // max_components). Fold statistics are recomputed when `folds` differs from
+  // the previous call's.
+  void evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected);
+  // Detail of fold `fold` of the last evaluation.
+  PlsFoldDetail detail(int fold) const;
+
+ private:
+  struct Impl;
+  Impl* impl_ = nullptr;
+};
+
+// std::mt19937(1) + std::normal_distribution<float>(0, 1), the first power
+// iteration start of every SIMPLS fit (plscv.cpp:844-846), widened to double.
+std::vector<double> pls_start_vector(int p);
+
+}  // namespace kodama_hip

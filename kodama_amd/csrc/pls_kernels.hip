@@ -1,0 +1,495 @@
+// PLS-LDA cross-validation on the device (plscv.cpp:2293-2507 with the fold
+// cache 593-698, SIMPLS 816-955 and the LDA 1379-1510), all in FP64.
+//
+// pls_cache_kernel (one workgroup per fold, once per data epoch): training mean
+// and scale (the scale accumulator starts at 1, as train_center_scale_float
+// does), column sums of the standardised training rows and their Gram matrix
+// G = Xs'Xs, each summed in training-row order.
+//
+// pls_eval_kernel (one workgroup per fold, once per labelling):
+//   S      S[c] = sum_{rows of class c} xs - (n_c / n) xsum, from the class
+//          segments of the training rows (host-sorted by class, rows ascending
+//          inside a class), one thread per (class, column) in fixed order;
+//   SIMPLS power refresh (two S S' products), normalisation through G, one
+//          sequential Gram-Schmidt pass against earlier loadings, deflation;
+//   LDA    class score means W'(S0[c] + (n_c/n) xsum) / n_c and T'T = W'GW
+//          (equal to the sums over the training scores), pooled covariance,
+//          ridge grid, Cholesky solve;
+//   score  validation scores, discriminants, argmax (strict >, first wins).
+// Reductions are fixed trees (wave butterflies, waves summed in order), so a
+// result never depends on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "pls_launch.h"
+
+namespace kodama_hip {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kT = kPlsThreads;
+constexpr int kWaves = kT / kWave;
+constexpr int kA = kPlsMaxComponents;
+// The reference compares against float literals converted to the working type.
+constexpr double kNormFloor = static_cast<double>(1.0e-10f);
+constexpr double kTinyFloor = static_cast<double>(1.0e-20f);
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = kWave / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// Sum of v over the block; every thread gets the total. `red` holds kWaves
+// doubles; two barriers make back-to-back calls safe.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+}  // namespace
+
+// ---- fold cache ---------------------------------------------------------------
+
+__global__ void __launch_bounds__(kT) pls_cache_kernel(PlsCacheParams prm) {
+  constexpr int kTile = 16, kChunk = 32;
+  __shared__ double stage[kChunk][2 * kTile + 1];
+  const int f = blockIdx.x;
+  const int p = prm.p;
+  const int t0 = prm.train.offsets[f];
+  const int nt = prm.train.offsets[f + 1] - t0;
+  const int* rows = prm.train.rows + t0;
+  double* mean = prm.mean + static_cast<size_t>(f) * p;
+  double* sd = prm.sd + static_cast<size_t>(f) * p;
+  double* xsum = prm.xsum + static_cast<size_t>(f) * p;
+  double* gram = prm.gram + static_cast<size_t>(f) * p * p;
+  for (int j = threadIdx.x; j < p; j += kT) {
+    double m = 0.0;
+    if (prm.center) {
+      for (int i = 0; i < nt; ++i) m += static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]);
+      m *= nt > 0 ? 1.0 / static_cast<double>(nt) : 0.0;
+    }
+    double s = 1.0;
+    if (prm.scale) {
+      double acc = 1.0;
+      for (int i = 0; i < nt; ++i) {
+        const double d = static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]) - m;
+        acc += d * d;
+      }
+      s = sqrt(acc / fmax(1.0, static_cast<double>(nt - 1)));
+      s = s > 0.0 && isfinite(s) ? s : 1.0;
+    }
+    double xs = 0.0;
+    for (int i = 0; i < nt; ++i) xs += (static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]) - m) / s;
+    mean[j] = m;
+    sd[j] = s;
+    xsum[j] = xs;
+  }
+  __syncthreads();
+  // Gram in 16 x 16 tiles of the lower triangle; rows staged 32 at a time.
+  const int tiles = (p + kTile - 1) / kTile;
+  const int u = threadIdx.x / kTile, v = threadIdx.x % kTile;
+  for (int ta = 0; ta < tiles; ++ta) {
+    for (int tb = 0; tb <= ta; ++tb) {
+      const int a = ta * kTile + u, b = tb * kTile + v;
+      double acc = 0.0;
+      for (int c0 = 0; c0 < nt; c0 += kChunk) {
+        // Stage the two 16-column blocks of up to 32 standardised rows.
+        for (int e = threadIdx.x; e < kChunk * 2 * kTile; e += kT) {
+          const int r = e / (2 * kTile), q = e % (2 * kTile);
+          const int col = q < kTile ? ta * kTile + q : tb * kTile + (q - kTile);
+          double val = 0.0;
+          if (c0 + r < nt && col < p) {
+            val = (static_cast<double>(prm.x[static_cast<size_t>(rows[c0 + r]) * p + col]) - mean[col]) / sd[col];
+          }
+          stage[r][q] = val;
+        }
+        __syncthreads();
+        const int len = min(kChunk, nt - c0);
+        for (int r = 0; r < len; ++r) acc += stage[r][u] * stage[r][kTile + v];
+        __syncthreads();
+      }
+      if (a < p && b < p && b <= a) {
+        gram[static_cast<size_t>(a) * p + b] = acc;
+        gram[static_cast<size_t>(b) * p + a] = acc;
+      }
+    }
+  }
+}
+
+// ---- evaluation -------------------------------------------------------------
+
+__global__ void __launch_bounds__(kT) pls_eval_kernel(PlsEvalParams prm) {
+  extern __shared__ double pls_lds[];
+  __shared__ int s_int[4];
+  const int f = blockIdx.x;
+  const int p = prm.p;
+  const int tid = threadIdx.x;
+  const int t0 = prm.train.offsets[f];
+  const int nt = prm.train.offsets[f + 1] - t0;
+  const int sb = prm.seg_offsets[f];
+  const int cf = prm.seg_offsets[f + 1] - sb;
+  const int* seg_code = prm.seg_code + sb;
+  const int* seg_start = prm.seg_start + sb + f;  // cf + 1 entries into the concatenated training rows
+  const int v0 = prm.val.offsets[f];
+  const int nv = prm.val.offsets[f + 1] - v0;
+  const int* vrows = prm.val.rows + v0;
+  const int cmax = prm.max_classes;
+  const double* mean = prm.mean + static_cast<size_t>(f) * p;
+  const double* sd = prm.sd + static_cast<size_t>(f) * p;
+  const double* xsum = prm.xsum + static_cast<size_t>(f) * p;
+  const double* G = prm.gram + static_cast<size_t>(f) * p * p;
+  double* W = prm.weights + static_cast<size_t>(f) * p * kA;   // [j * kA + a]
+  double* V = prm.loadings + static_cast<size_t>(f) * p * kA;
+  double* S0 = prm.s0 + static_cast<size_t>(f) * cmax * p;  // undeflated S, [c * p + j]
+  double* GW = prm.gw + static_cast<size_t>(f) * p * kA;     // [j * kA + a]
+  double* LIN = prm.linear + static_cast<size_t>(f) * cmax * kA;  // [c * kA + a]
+  double* CONST = prm.constant + static_cast<size_t>(f) * cmax;
+  double* TV = prm.val_scores + static_cast<size_t>(v0) * kA;       // [i * kA + a]
+  double* DISC = prm.discriminants + static_cast<size_t>(v0) * cmax;  // [i * cmax + c]
+
+  double* S = pls_lds;                                  // [cmax * p], later LDA rows
+  double* wt = S + static_cast<size_t>(cmax) * p;       // [p]
+  double* ld = wt + p;                                  // [p]
+  double* rv = ld + p;                                  // [cmax]
+  double* red = rv + cmax;                              // [kWaves]
+  double* mat = red + kWaves;                           // [rc * rc] pooled covariance, then its factor
+  double* tt = mat + static_cast<size_t>(prm.rank_cap) * prm.rank_cap;  // [rc * rc] W'GW
+  double* mu = tt + static_cast<size_t>(prm.rank_cap) * prm.rank_cap;   // [cmax * rc] class score means
+
+  if (cf <= 1) {
+    // One class in the training fold: every validation row gets it.
+    const int code = cf == 1 ? seg_code[0] : 0;
+    for (int i = tid; i < nv; i += kT) prm.pred_code[vrows[i]] = code;
+    if (tid == 0) {
+      prm.components[f] = 1;
+      prm.rank[f] = 0;
+    }
+    return;
+  }
+
+  // -- S = X'(Y - ybar) -----------------------------------------------------------
+  const double inv_nt = 1.0 / static_cast<double>(nt);
+  for (int q = tid; q < cf * p; q += kT) {
+    const int c = q / p, j = q - c * p;
+    const int* r = prm.train.rows;
+    double acc = 0.0;
+    for (int i = seg_start[c]; i < seg_start[c + 1]; ++i) {
+      acc += (static_cast<double>(prm.x[static_cast<size_t>(r[i]) * p + j]) - mean[j]) / sd[j];
+    }
+    const double ym = static_cast<double>(seg_start[c + 1] - seg_start[c]) * inv_nt;
+    const double s = acc - ym * xsum[j];
+    S[q] = s;
+    S0[q] = s;
+  }
+  __syncthreads();
+
+  // -- power-refresh SIMPLS ----------------------------------------------------
+  const int rc = max(1, min(prm.rank_cap, max(1, nt - 1)));
+  int fitted = 0;
+  auto refresh = [&]() {
+    for (int it = 0; it < 2; ++it) {
+      for (int c = tid; c < cf; c += kT) {
+        double a = 0.0;
+        for (int j = 0; j < p; ++j) a += S[c * p + j] * wt[j];
+        rv[c] = a;
+      }
+      __syncthreads();
+      for (int j = tid; j < p; j += kT) {
+        double a = 0.0;
+        for (int c = 0; c < cf; ++c) a += S[c * p + j] * rv[c];
+        wt[j] = a;
+      }
+      __syncthreads();
+    }
+  };
+  auto norm_of = [&](const double* v) {
+    double part = 0.0;
+    for (int j = tid; j < p; j += kT) part += v[j] * v[j];
+    return block_sum(part, red);
+  };
+  for (int a = 0; a < rc; ++a) {
+    if (a == 0) {
+      for (int j = tid; j < p; j += kT) wt[j] = prm.w0[j];
+      __syncthreads();
+    }
+    refresh();
+    double ss = norm_of(wt);
+    double nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
+    if (a == 0 && (!isfinite(nrm) || nrm <= kNormFloor)) {
+      // The strongest column of S restarts the power iteration.
+      if (tid == 0) {
+        int strongest = -1;
+        double best = 0.0;
+        for (int c = 0; c < cf; ++c) {
+          double sq = 0.0;
+          for (int j = 0; j < p; ++j) sq += S[c * p + j] * S[c * p + j];
+          if (isfinite(sq) && sq > best) {
+            best = sq;
+            strongest = c;
+          }
+        }
+        s_int[0] = strongest >= 0 && best > 1.0e-20 ? strongest : -1;
+      }
+      __syncthreads();
+      const int strongest = s_int[0];
+      if (strongest >= 0) {
+        for (int j = tid; j < p; j += kT) wt[j] = S[strongest * p + j];
+        __syncthreads();
+        refresh();
+        ss = norm_of(wt);
+        nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
+      }
+    }
+    if (!isfinite(nrm) || nrm <= kNormFloor) break;
+    for (int j = tid; j < p; j += kT) wt[j] /= nrm;
+    __syncthreads();
+    for (int i = tid; i < p; i += kT) {
+      double s = 0.0;
+      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(i) * p + j] * wt[j];
+      ld[i] = s;
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int j = tid; j < p; j += kT) part += wt[j] * ld[j];
+    const double tnorm = block_sum(part, red);
+    if (!isfinite(tnorm) || tnorm <= kTinyFloor) break;
+    const double inv = 1.0 / sqrt(tnorm);
+    for (int j = tid; j < p; j += kT) {
+      wt[j] *= inv;
+      ld[j] *= inv;
+    }
+    __syncthreads();
+    // One sequential Gram-Schmidt pass against the earlier loadings.
+    for (int b = 0; b < a; ++b) {
+      double pp = 0.0;
+      for (int j = tid; j < p; j += kT) pp += V[static_cast<size_t>(j) * kA + b] * ld[j];
+      const double proj = block_sum(pp, red);
+      for (int j = tid; j < p; j += kT) ld[j] -= proj * V[static_cast<size_t>(j) * kA + b];
+      __syncthreads();
+    }
+    const double lnn = norm_of(ld);
+    const double ln = sqrt(fmax(0.0, lnn));
+    if (!isfinite(ln) || ln <= kNormFloor) break;
+    for (int j = tid; j < p; j += kT) {
+      ld[j] /= ln;
+      W[static_cast<size_t>(j) * kA + a] = wt[j];
+      V[static_cast<size_t>(j) * kA + a] = ld[j];
+    }
+    __syncthreads();
+    for (int c = tid; c < cf; c += kT) {
+      double s = 0.0;
+      for (int j = 0; j < p; ++j) s += ld[j] * S[c * p + j];
+      rv[c] = s;
+    }
+    __syncthreads();
+    for (int q = tid; q < cf * p; q += kT) {
+      const int c = q / p, j = q - c * p;
+      S[q] -= ld[j] * rv[c];
+    }
+    __syncthreads();
+    fitted = a + 1;
+  }
+
+  if (fitted < 1) {
+    // Degenerate fit: the training majority (the smaller code on ties).
+    const int code = [&] {
+      int best = seg_code[0], best_n = -1;
+      for (int c = 0; c < cf; ++c) {
+        const int cnt = seg_start[c + 1] - seg_start[c];
+        if (cnt > best_n) {
+          best = seg_code[c];
+          best_n = cnt;
+        }
+      }
+      return best;
+    }();
+    for (int i = tid; i < nv; i += kT) prm.pred_code[vrows[i]] = code;
+    if (tid == 0) {
+      prm.components[f] = 1;
+      prm.rank[f] = 0;
+    }
+    return;
+  }
+  const int A = min(prm.requested, fitted);
+  if (tid == 0) {
+    prm.components[f] = A;
+    prm.rank[f] = fitted;
+  }
+
+  // -- LDA statistics ----------------------------------------------------------
+  // GW = G W [p x A] (global scratch, [j * kA + a]).
+  for (int q = tid; q < p * A; q += kT) {
+    const int i = q / A, a = q - i * A;
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(i) * p + j] * W[static_cast<size_t>(j) * kA + a];
+    GW[static_cast<size_t>(i) * kA + a] = s;
+  }
+  // Class score means W'(S0[c] + (n_c / n) xsum) / n_c.
+  for (int q = tid; q < cf * A; q += kT) {
+    const int c = q / A, a = q - c * A;
+    const int cnt = seg_start[c + 1] - seg_start[c];
+    const double ym = static_cast<double>(cnt) * inv_nt;
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) s += W[static_cast<size_t>(j) * kA + a] * (S0[c * p + j] + ym * xsum[j]);
+    mu[c * A + a] = cnt > 0 ? s * (1.0 / static_cast<double>(cnt)) : 0.0;
+  }
+  __syncthreads();
+  // T'T = W' G W, then the pooled covariance.
+  const double df = static_cast<double>(max(1, nt - cf));
+  for (int q = tid; q < A * A; q += kT) {
+    const int r = q / A, c2 = q - r * A;
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) s += W[static_cast<size_t>(j) * kA + r] * GW[static_cast<size_t>(j) * kA + c2];
+    tt[q] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < A * A; q += kT) {
+    const int r = q / A, c2 = q - r * A;
+    const int lo = min(r, c2), hi = max(r, c2);
+    double between = 0.0;
+    for (int c = 0; c < cf; ++c) {
+      between += static_cast<double>(seg_start[c + 1] - seg_start[c]) * mu[c * A + hi] * mu[c * A + lo];
+    }
+    // The lower triangle is the one computed (symmetric use of T'T).
+    mat[q] = (tt[hi * A + lo] - between) / df;
+  }
+  __syncthreads();
+
+  // -- ridge grid + Cholesky (one wave; lanes over rows / right-hand sides) ------
+  // The factor is written over `mat` (lower triangle); the solution of each
+  // class column goes to LIN[c * kA + a].
+  if (tid < kWave) {
+    const int lane = tid;
+    double trace = 0.0;
+    for (int a = 0; a < A; ++a) trace += mat[a * A + a];
+    const double rs = isfinite(trace) && trace > 0.0 ? trace / static_cast<double>(max(1, A)) : 1.0;
+    const float grid[6] = {1e-8f, 1e-6f, 1e-5f, 1e-4f, 1e-3f, 1e-2f};
+    bool ok = false;
+    // tt is free now: it holds the factor while `mat` keeps the covariance.
+    double* lo = tt;
+    for (int g = 0; g < 6 && !ok; ++g) {
+      const double lam = static_cast<double>(grid[g]) * rs;
+      bool good = true;
+      for (int c = 0; c < A && good; ++c) {
+        // Diagonal entry (every lane computes it; lanes agree).
+        double v = mat[c * A + c] + lam;
+        for (int k = 0; k < c; ++k) v -= lo[c * A + k] * lo[c * A + k];
+        if (!isfinite(v) || v <= 0.0) {
+          good = false;
+          break;
+        }
+        const double d = sqrt(v);
+        if (lane == 0) lo[c * A + c] = d;
+        for (int r = c + 1 + lane; r < A; r += kWave) {
+          double w = mat[r * A + c];
+          for (int k = 0; k < c; ++k) w -= lo[r * A + k] * lo[c * A + k];
+          lo[r * A + c] = w / d;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      ok = good;
+    }
+    if (lane == 0) s_int[1] = ok ? 1 : 0;
+    if (ok) {
+      for (int c = lane; c < cf; c += kWave) {
+        double* sol = LIN + static_cast<size_t>(c) * kA;
+        for (int r = 0; r < A; ++r) {
+          double v = mu[c * A + r];
+          for (int k = 0; k < r; ++k) v -= lo[r * A + k] * sol[k];
+          sol[r] = v / lo[r * A + r];
+        }
+        for (int r = A - 1; r >= 0; --r) {
+          double v = sol[r];
+          for (int k = r + 1; k < A; ++k) v -= lo[k * A + r] * sol[k];
+          sol[r] = v / lo[r * A + r];
+        }
+        double dm = 0.0;
+        for (int a = 0; a < A; ++a) dm += mu[c * A + a] * sol[a];
+        const int cnt = seg_start[c + 1] - seg_start[c];
+        const double prior = fmax(static_cast<double>(cnt) / static_cast<double>(max(1, nt)),
+                                  static_cast<double>(FLT_MIN));
+        CONST[c] = -0.5 * dm + log(prior);
+      }
+    }
+  }
+  __syncthreads();
+  if (!s_int[1]) {
+    // The host reports the failed factorisation (components = -1).
+    if (tid == 0) prm.components[f] = -1;
+    return;
+  }
+
+  // -- validation scores and discriminants ----------------------------------------
+  for (int q = tid; q < nv * A; q += kT) {
+    const int i = q / A, a = q - i * A;
+    const size_t row = static_cast<size_t>(vrows[i]) * p;
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) {
+      s += ((static_cast<double>(prm.x[row + j]) - mean[j]) / sd[j]) * W[static_cast<size_t>(j) * kA + a];
+    }
+    TV[static_cast<size_t>(i) * kA + a] = s;
+  }
+  __syncthreads();
+  for (int q = tid; q < nv * cf; q += kT) {
+    const int i = q / cf, c = q - i * cf;
+    double sc = CONST[c];
+    for (int a = 0; a < A; ++a) sc += TV[static_cast<size_t>(i) * kA + a] * LIN[static_cast<size_t>(c) * kA + a];
+    DISC[static_cast<size_t>(i) * cmax + c] = sc;
+  }
+  __syncthreads();
+  for (int i = tid; i < nv; i += kT) {
+    int best = 0;
+    double best_s = -INFINITY;
+    for (int c = 0; c < cf; ++c) {
+      const double sc = DISC[static_cast<size_t>(i) * cmax + c];
+      if (sc > best_s) {
+        best_s = sc;
+        best = c;
+      }
+    }
+    prm.pred_code[vrows[i]] = seg_code[best];
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+
+size_t pls_eval_lds_bytes(int p, int max_classes) {
+  const int rc = p < kPlsMaxComponents ? p : kPlsMaxComponents;
+  return sizeof(double) * (static_cast<size_t>(max_classes) * p + 2 * static_cast<size_t>(p) + max_classes +
+                           kWaves + 2 * static_cast<size_t>(rc) * rc + static_cast<size_t>(max_classes) * rc);
+}
+
+hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s) {
+  if (prm.folds == 0) return hipSuccess;
+  if (prm.p < 1 || prm.p > kPlsMaxDims) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pls_cache_kernel, dim3(prm.folds), dim3(kT), 0, s, prm);
+  return hipGetLastError();
+}
+
+hipError_t launch_pls_eval(const PlsEvalParams& prm, hipStream_t s) {
+  if (prm.folds == 0) return hipSuccess;
+  if (prm.p < 1 || prm.p > kPlsMaxDims || prm.rank_cap < 1 || prm.rank_cap > kPlsMaxComponents ||
+      prm.rank_cap > prm.p || prm.requested < 1 || static_cast<size_t>(prm.max_classes) * prm.p > kPlsMaxSLds) {
+    return hipErrorInvalidValue;
+  }
+  const size_t shmem = pls_eval_lds_bytes(prm.p, prm.max_classes);
+  if (shmem > 160 * 1024) return hipErrorInvalidValue;
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_eval_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pls_eval_kernel, dim3(prm.folds), dim3(kT), shmem, s, prm);
+  return hipGetLastError();
+}
+
+}  // namespace kodama_hip

@@ -1,0 +1,132 @@
+"""HIP PLS-LDA (FP64 on the device) against the FP64 oracle -- scores and LDA
+discriminants within 1e-4 relative, predictions and CorePLSLDA trajectories
+identical -- and against the reference's FP32 golden fixtures."""
+import numpy as np
+import pytest
+
+import kodama_amd as K
+import oracle as O
+from _cases import assert_same_core, golden_core
+from conftest import golden
+from kodama_amd import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4  # PLS scores and LDA outputs vs the FP64 oracle
+
+
+def _opts(**kw):
+    d = dict(classifier=0, folds=10, stratified=0, cv_seed=3, max_components=5, fixed_components=5,
+             center=1, scale=1, pls_fold_cache=1)
+    d.update(kw)
+    return d
+
+
+def _rel_close(a, b, rtol=RTOL):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape
+    scale = np.maximum(np.abs(b), np.max(np.abs(b)) * 1e-6 + 1e-300)
+    err = np.max(np.abs(a - b) / scale) if a.size else 0.0
+    assert err <= rtol, err
+
+
+def _c2(n=2000):
+    x, truth = D.config2()
+    return x[:n], truth[:n]
+
+
+@pytest.mark.parametrize("strat", [0, 1])
+def test_plsldacv_predictions_match_oracle_fp64(oracle, strat):
+    x, truth = _c2()
+    o = _opts(stratified=strat, pls_fold_cache=0)  # the options PLSLDACV passes
+    got = K.PLSLDACV(x, truth, folds=10, stratified=bool(strat), seed=3, ncomp=5)
+    pred, folds, sel = oracle.plsldacv(x, truth, O.core_options(**o), precision=64)
+    assert np.array_equal(got["fold_assignments"], folds)
+    assert np.array_equal(got["predicted_labels"], pred)
+    assert got["selected_components"] == sel
+
+
+@pytest.mark.parametrize("strat", [1, 0])
+def test_plsldacv_matches_reference_golden(strat):
+    """The reference computes in FP32; the device in FP64. Their predictions agree
+    on this workload (no validation row sits within float rounding of a tie)."""
+    g = golden("plslda_c2.npz")
+    x, truth = _c2()
+    got = K.PLSLDACV(x, truth, folds=10, stratified=bool(strat), seed=3, ncomp=5)
+    assert np.array_equal(got["fold_assignments"], g[f"cv_folds_s{strat}"])
+    assert np.array_equal(got["predicted_labels"], g[f"cv_pred_s{strat}"])
+    assert got["selected_components"] == int(g[f"cv_sel_s{strat}"])
+
+
+@pytest.mark.parametrize("ncomp,classes", [(5, 8), (10, 3), (50, 8), (3, 2)])
+def test_fold_scores_and_discriminants_within_tolerance(oracle, ncomp, classes):
+    x, truth = D.config2(n=1500, seed=61)
+    lab = (truth % classes + 1).astype(np.int32)
+    folds = (np.argsort(D.uniform(9, 1, lab.size)) % 5).astype(np.int32)
+    o = _opts(max_components=ncomp, fixed_components=ncomp, folds=5)
+    for fold in (0, 3):
+        dev = K.plslda_fold_detail(x, lab, folds, fold, K.core_options(**o))
+        ref = oracle.pls_fold_detail(x, lab, folds, fold, O.core_options(**o), precision=64)
+        assert dev["rank"] == ref["rank"]
+        assert np.array_equal(dev["classes"], ref["classes"])
+        _rel_close(dev["weights"], ref["weights"])
+        _rel_close(dev["val_scores"], ref["val_scores"])
+        _rel_close(dev["discriminant"], ref["discriminant"])
+        # LDA posteriors (softmax of the discriminants)
+        def post(d):
+            e = np.exp(d - d.max(1, keepdims=True))
+            return e / e.sum(1, keepdims=True)
+        np.testing.assert_allclose(post(dev["discriminant"]), post(ref["discriminant"]), rtol=RTOL, atol=1e-12)
+
+
+def test_core_plslda_trajectories_match_oracle_fp64_and_reference(oracle):
+    g = golden("plslda_c2.npz")
+    x, _ = _c2()
+    init = g["init"]
+    for run in range(2):
+        kw = dict(cycles=20, classifier=0, folds=10, stratified=0, seed=1 + run, cv_seed=1 + run,
+                  max_components=5, fixed_components=5, evolutionary_search=1, guarded_diversity=1,
+                  many_to_one_absorption=1, auto_class_coarsening=1)
+        dev = K.core_plslda(x, init, K.core_options(**kw))
+        ref64 = oracle.core_plslda(x, init, O.core_options(**kw), precision=64)
+        assert_same_core(dev, ref64, f"fp64 run {run}")
+        assert_same_core(dev, golden_core(g, f"core_r{run}_"), f"reference run {run}")
+
+
+def test_core_plslda_stratified_api_defaults(oracle):
+    x, truth = D.config2(n=600, seed=71)
+    init = (truth % 4 + 1).astype(np.int32)
+    kw = dict(cycles=8, classifier=0, folds=5, stratified=1, seed=2, cv_seed=2, max_components=4,
+              fixed_components=4)
+    dev = K.core_plslda(x, init, K.core_options(**kw))
+    ref = oracle.core_plslda(x, init, O.core_options(**kw), precision=64)
+    assert_same_core(dev, ref, "stratified")
+
+
+def test_single_class_and_degenerate_folds(oracle):
+    # Constant data: no PLS direction, the selected component count is 1
+    # (tests/testthat/test-pls-degenerate.R:4-19).
+    x = np.ones((60, 4), np.float32)
+    lab = np.repeat([1, 2, 3], 20).astype(np.int32)
+    got = K.PLSLDACV(x, lab, folds=3, stratified=True, seed=1, ncomp=3)
+    pred, folds, sel = oracle.plsldacv(x, lab, O.core_options(**_opts(folds=3, stratified=1, cv_seed=1,
+                                                                       max_components=3, fixed_components=3,
+                                                                       pls_fold_cache=0)),
+                                       precision=64)
+    assert got["selected_components"] == 1 == sel
+    assert np.array_equal(got["predicted_labels"], pred)
+    # one class only
+    one = np.full(60, 7, np.int32)
+    xs, _ = D.config2(n=60, seed=3)
+    got = K.PLSLDACV(xs, one, folds=3, stratified=False, seed=1, ncomp=2)
+    assert np.all(got["predicted_labels"] == 7)
+
+
+def test_more_components_than_columns(oracle):
+    x, truth = D.config2(n=400, seed=81)
+    x = x[:, :6]
+    o = _opts(folds=4, max_components=10, fixed_components=10, pls_fold_cache=0)
+    got = K.PLSLDACV(x, truth, folds=4, stratified=False, seed=3, ncomp=10)
+    pred, folds, sel = oracle.plsldacv(x, truth, O.core_options(**o), precision=64)
+    assert np.array_equal(got["predicted_labels"], pred)
+    assert got["selected_components"] == sel
