@@ -29,11 +29,19 @@ HIP events on its launch stream:
           prediction (8k + 28 bytes with the per-run reductions), so bytes per
           launch = sum(L) * (8k + 28) over the average launch duration.
 
+`traffic` (HBM bytes per launch of that kernel) comes from the committed
+counter summary profiles/evo_persistent_kernel_traffic.txt when it was
+measured on this same workload (two rocprofv3 --pmc runs, FETCH_SIZE and
+WRITE_SIZE, scaled by what the counters read for a plain copy of known size:
+`python bench.py --copy-probe`); otherwise null.
+
 cpu_baseline: the reference's own CPU label-evolution loop (oracle/_ref,
 CoreKNNGraph_CPU on exact 64-NN landmark graphs of the same runs), run on
-`cores` host threads, one run per thread at a time.
+`cores` host threads, one run per thread at a time, in passes over all runs
+until at least --cpu-seconds of wall time.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--runs 100]
+    python bench.py --copy-probe     # the 256 MiB copy that calibrates the counters
 """
 from __future__ import annotations
 
@@ -85,8 +93,10 @@ def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10):
     return x, rows, fold_ids, init
 
 
-def cpu_baseline(x, rows, init, k, cycles, cores, sample_runs):
-    """Reference CoreKNNGraph_CPU label evolution, `cores` runs at a time."""
+def cpu_baseline(x, rows, init, k, cycles, cores, min_seconds):
+    """Reference CoreKNNGraph_CPU label evolution, `cores` runs at a time, in
+    passes over every run until min_seconds have elapsed."""
+    sample_runs = len(rows)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import kodama_amd as K
     import oracle as O
@@ -113,15 +123,53 @@ def cpu_baseline(x, rows, init, k, cycles, cores, sample_runs):
         folds = O.Oracle().make_folds(init[r], 10, 0, 1 + r)
         return eng.core_knn_tables(nbr, score, folds, init[r], o)["accbest"]
 
+    passes = 0
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=cores) as pool:
-        accs = list(pool.map(one, range(sample_runs)))
+        while True:
+            accs = list(pool.map(one, range(sample_runs)))
+            passes += 1
+            if time.perf_counter() - t0 >= min_seconds:
+                break
     wall = time.perf_counter() - t0
-    steps = sample_runs * cycles
+    steps = passes * sample_runs * cycles
     return dict(value=steps / wall, unit="steps/s", cores=cores, kind=kind,
-                sample=f"{sample_runs} runs x {cycles} cycles of this workload "
+                sample=f"{passes} passes over the {sample_runs} runs x {cycles} cycles of this workload "
                        f"(CoreKNNGraph_CPU on exact 64-NN landmark graphs), {cores} threads",
                 wall_seconds=wall, mean_accbest=float(np.mean(accs)))
+
+
+def copy_probe() -> int:
+    """A plain device-to-device copy of 256 MiB, three times: the counter
+    calibration (FETCH_SIZE / WRITE_SIZE per KiB actually moved)."""
+    import torch
+    a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
+    return 0
+
+
+def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
+    """HBM bytes per launch from the committed counter summary, if it covers
+    this kernel on this workload."""
+    path = os.path.join(ROOT, "profiles", "evo_persistent_kernel_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            key, val = line.split("=", 1)
+            kv[key.strip()] = val.strip()
+    try:
+        same = (kv["kernel"] == kernel and int(kv["runs"]) == runs and int(kv["cycles"]) == cycles
+                and int(kv["k"]) == k and int(kv["landmarks_total"]) == landmarks)
+        return float(kv["hbm_bytes_per_launch"]) if same else None
+    except (KeyError, ValueError):
+        return None
 
 
 def main() -> int:
@@ -134,7 +182,11 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=30)
     ap.add_argument("--forms", default="plain,device")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported"}))
         return 2
@@ -166,13 +218,14 @@ def main() -> int:
     head = "device" if "device" in res else forms[0]
     kern_s = res[head]["vote_kernel_ms"] * 1e-3
     if head == "device":
-        kernel = f"evo_persistent_kernel<{32 if args.k <= 32 else 64},{512}>" if args.k > 16 else \
-            f"evo_persistent_kernel<{8 if args.k <= 8 else 16},1024>"
+        kmax = 8 if args.k <= 8 else 16 if args.k <= 16 else 32 if args.k <= 32 else 64
+        kernel = f"evo_persistent_kernel<{kmax},512>"
         algo_bytes = float(args.steps + 1) * float(L.sum()) * (2 * args.k)
     else:
         kernel = "vote_kernel"
         algo_bytes = float(L.sum()) * (8 * args.k + 28)
     achieved = algo_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
+    traffic = measured_traffic(kernel, args.runs, args.steps, args.k, int(L.sum())) if head == "device" else None
     line = {
         "metric": METRIC,
         "value": res[head]["steps_per_s"],
@@ -195,14 +248,14 @@ def main() -> int:
         "forms": res,
         "roofline": {"bound": "hbm", "kernel": kernel, "kernel_ms": kern_s * 1e3, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": algo_bytes},
+                     "traffic": traffic, "algorithmic_bytes_per_launch": algo_bytes},
     }
     if "plain" in res and "device" in res:
         line["accuracy_gap_pt"] = 100.0 * (res["device"]["mean_accbest"] - res["plain"]["mean_accbest"])
         line["speedup_device_vs_plain"] = res["device"]["steps_per_s"] / res["plain"]["steps_per_s"]
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline(x, rows, init, args.k, args.steps, cores, min(args.runs, 64))
+        cb = cpu_baseline(x, rows, init, args.k, args.steps, cores, args.cpu_seconds)
         log(f"cpu baseline ({cb['kind']}): {cb['value']:,.0f} steps/s on {cores} threads")
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
