@@ -219,7 +219,7 @@ def main() -> int:
     kern_s = res[head]["vote_kernel_ms"] * 1e-3
     if head == "device":
         kmax = 8 if args.k <= 8 else 16 if args.k <= 16 else 32 if args.k <= 32 else 64
-        kernel = f"evo_persistent_kernel<{kmax},512>"
+        kernel = f"evo_persistent_kernel<{kmax},{1024 if kmax <= 32 else 512}>"
         algo_bytes = float(args.steps + 1) * float(L.sum()) * (2 * args.k)
     else:
         kernel = "vote_kernel"

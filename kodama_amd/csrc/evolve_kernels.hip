@@ -3,7 +3,7 @@
 // prediction-guided singleton relabelling, many-to-one absorption,
 // diversity-guarded objective, evolutionary acceptance).
 //
-// One workgroup of 512 threads owns one run for a whole launch:
+// One workgroup (1024 threads; 512 for k > 32) owns one run for a whole launch:
 // the run's current labels and predictions, the proposal, the class tables and
 // the absorption scratch live in LDS (16-bit label codes), so a launch runs any
 // number of cycles with only block barriers between phases. The neighbour table
@@ -293,23 +293,13 @@ __device__ __forceinline__ void vote_phase(int k, int pad_total, const Lds& s, c
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t below = (1ull << lane) - 1ull;
   int matches = 0;
-  // Up to 32 neighbours the next pair's ids are loaded while this pair votes.
-  constexpr bool kPrefetch = KMAX <= 32;
-  uint32_t next[kPrefetch ? KMAX : 1];
-  if constexpr (kPrefetch) {
-    if (static_cast<int>(threadIdx.x) < npairs) load_pair<KMAX>(nbr2, pad2, k, threadIdx.x, next);
-  }
-  // Uniform trip count: every lane reaches the ballots below.
+  // Uniform trip count: every lane reaches the ballots below. Latency is
+  // hidden by the workgroup's 16 waves rather than by prefetching (which would
+  // cost KMAX more registers per thread).
   for (int m0 = 0; m0 < npairs; m0 += THREADS) {
     const int m = m0 + threadIdx.x;
     uint32_t ids[KMAX];
-    if constexpr (kPrefetch) {
-#pragma unroll
-      for (int j = 0; j < KMAX; ++j) ids[j] = next[j];
-      if (m + THREADS < npairs) load_pair<KMAX>(nbr2, pad2, k, m + THREADS, next);
-    } else if (m < npairs) {
-      load_pair<KMAX>(nbr2, pad2, k, m, ids);
-    }
+    if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int i = 2 * m + h;
@@ -722,11 +712,11 @@ size_t evo_lds_bytes(int classes, int max_rows) {
          sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
 }
 
-// 512 threads per workgroup leave each thread 256 registers, which hold the
-// neighbour ids of two row pairs without spilling.
+// 1024 threads per workgroup (16 waves hide the gather latency) up to 32
+// neighbours; 512 for 64, whose id and key arrays need 256 registers.
 template <int KMAX>
 static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStream_t s) {
-  constexpr int threads = kThreads;
+  constexpr int threads = KMAX <= 32 ? kThreads : kThreads / 2;
   const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;

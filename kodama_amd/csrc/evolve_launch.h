@@ -9,7 +9,7 @@
 
 namespace kodama_hip {
 
-constexpr int kEvoThreads = 512;            // one workgroup per run
+constexpr int kEvoThreads = 1024;           // one workgroup per run (512 for k > 32)
 constexpr size_t kEvoMaxLds = 160 * 1024;   // LDS of one CDNA4 compute unit
 constexpr int kEvoAccepted = 1;
 constexpr int kEvoImproved = 2;
