@@ -104,6 +104,14 @@ int kodama_hip_device_count(int32_t* count);
 /* Library version string, e.g. "kodama_hip 0.1.0 gfx950". */
 const char* kodama_hip_version(void);
 
+/* No counterpart in the reference: for callers that check which path the last
+ * exact kNN search of the calling thread took (kodama_hip_exact_knn_search,
+ * kodama_hip_knn_graph). With at most 64 dimensions, k <= 32 and at least 4096
+ * train rows the search runs split-BF16 MFMA candidates with an exact FP32
+ * rerank and returns the number of queries that fell back to the exact scan;
+ * -1 when the exact scan served every query. Results are identical either way. */
+int kodama_hip_last_knn_fallbacks(void);
+
 /* ---- device seam: exact kNN ------------------------------------------------
  * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).
  * train [train_rows x dimensions], query [query_rows x dimensions] row-major
@@ -173,6 +181,19 @@ int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* 
                            const int32_t* constrain, const int32_t* fixed,
                            const kodama_hip_core_options* options,
                            kodama_hip_core_result* result);
+
+/* The M-run form of CorePLSLDA (the scheduler of KODAMAMatrix_CUDA,
+ * kodama_matrix.cpp:2651-2682, around CorePLSLDA_CUDA): same indexing as
+ * kodama_hip_core_knn_batch; proposals on the host with the reference's random
+ * streams (seed + r, folds from cv_seed + r unless fold_ids is given), one
+ * device evaluation of all runs x folds per cycle. Per run the result equals
+ * kodama_hip_core_plslda on the run's rows. timings[4] as for the kNN batch,
+ * timings[3] = average duration of the fused PLS-LDA evaluation kernel. */
+int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t runs,
+                                 const int32_t* run_offsets, const int32_t* landmark_rows,
+                                 const int32_t* initial_labels, const int32_t* fold_ids,
+                                 const kodama_hip_core_options* options, int32_t* clbest,
+                                 double* accbest, double* vect_acc, double* timings);
 
 /* No counterpart in the reference (its fold internals are not exported): the
  * detail of one PLS-LDA fold, for callers that check the FP64 device results

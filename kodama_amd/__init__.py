@@ -246,6 +246,31 @@ def core_knn_batch(data, run_rows, initial_labels, options: CoreOptions, mode="p
                 steps=int(timings[2]), vote_seconds=float(timings[3]))
 
 
+def core_plslda_batch(data, run_rows, initial_labels, options: CoreOptions, fold_ids=None):
+    """CorePLSLDA over M runs in one batch: host proposals with the reference
+    random streams per run (seed + r, folds from cv_seed + r), one device
+    evaluation of all runs x folds per cycle. Per run the result equals
+    core_plslda on that run's rows."""
+    x = _matrix(data)
+    M = len(run_rows)
+    offsets = np.zeros(M + 1, np.int32)
+    offsets[1:] = np.cumsum([len(r) for r in run_rows])
+    rows = N.i32(np.concatenate([np.asarray(r) for r in run_rows]))
+    init = N.i32(np.concatenate([np.asarray(l) for l in initial_labels]))
+    folds = None if fold_ids is None else N.i32(np.concatenate([np.asarray(f) for f in fold_ids]))
+    clbest = np.zeros(int(offsets[-1]), np.int32)
+    accbest = np.zeros(M, np.float64)
+    vect = np.zeros(M * max(options.cycles, 1), np.float64)
+    timings = np.zeros(4, np.float64)
+    N.call("kodama_hip_core_plslda_batch", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+           N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), N.ptr(clbest),
+           N.ptr(accbest), N.ptr(vect), N.ptr(timings))
+    return dict(clbest=[clbest[offsets[r]:offsets[r + 1]].copy() for r in range(M)], accbest=accbest,
+                vect_acc=vect[: M * options.cycles].reshape(M, options.cycles),
+                precompute_seconds=float(timings[0]), evolution_seconds=float(timings[1]),
+                steps=int(timings[2]), eval_kernel_seconds=float(timings[3]))
+
+
 # ---- graph and dissimilarity ----------------------------------------------------
 
 def KODAMA_knn_graph(data, k=30, metric="euclidean", gpu_device=0):
@@ -257,6 +282,12 @@ def KODAMA_knn_graph(data, k=30, metric="euclidean", gpu_device=0):
     N.call("kodama_hip_knn_graph", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(k),
            C.c_int32(_metric(metric)), C.c_int32(gpu_device), N.ptr(idx), N.ptr(dist))
     return dict(indices=idx, distances=dist, neighbors=kk, index_base="one")
+
+
+def last_knn_fallbacks() -> int:
+    """Queries of this thread's last exact kNN search / graph that the MFMA
+    candidate path handed to the exact scan; -1 when the exact scan ran alone."""
+    return int(N.lib().kodama_hip_last_knn_fallbacks())
 
 
 def KODAMA_dissimilarity(indices, distances, res, gpu_device=0):

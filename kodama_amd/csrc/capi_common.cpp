@@ -9,11 +9,27 @@ namespace kodama_hip {
 
 namespace {
 thread_local std::string g_last_error;
+thread_local int g_knn_fallbacks = -1;
 std::mutex g_arch_mutex;
 std::vector<int> g_arch_ok;  // -1 unknown, 0 not gfx950, 1 gfx950
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+int& last_knn_fallbacks() { return g_knn_fallbacks; }
+
+float max_row_norm2(const float* x, int rows, int p) {
+  double best = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) {
+      const double v = x[static_cast<size_t>(r) * p + j];
+      s += v * v;
+    }
+    best = s > best ? s : best;
+  }
+  return static_cast<float>(best * (1.0 + 1e-6));
+}
 
 void select_device(int device) {
   int count = 0;
@@ -41,6 +57,8 @@ extern "C" {
 const char* kodama_hip_last_error(void) { return kodama_hip::g_last_error.c_str(); }
 
 const char* kodama_hip_version(void) { return "kodama_hip 0.1.0 gfx950"; }
+
+int kodama_hip_last_knn_fallbacks(void) { return kodama_hip::g_knn_fallbacks; }
 
 int kodama_hip_device_count(int32_t* count) {
   return kodama_hip::guarded([&] {

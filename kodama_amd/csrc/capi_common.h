@@ -20,6 +20,14 @@ void set_last_error(const std::string& msg);
 // Makes `device` current after checking that it exists and is a gfx950 part.
 void select_device(int device);
 
+// Largest squared row norm (double accumulation, rounded up to float), the
+// scale of the MFMA candidate error bound.
+float max_row_norm2(const float* x, int rows, int p);
+
+// Queries of the calling thread's last exact kNN search that fell back to the
+// register scan (-1: the MFMA path was not used).
+int& last_knn_fallbacks();
+
 template <class F>
 int guarded(F&& f) {
   try {

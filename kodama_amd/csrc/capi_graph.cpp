@@ -8,6 +8,7 @@
 #include "capi_common.h"
 #include "engine.h"
 #include "graph_launch.h"
+#include "knn_mfma.h"
 #include "launch.h"
 
 using namespace kodama_hip;
@@ -45,17 +46,8 @@ int kodama_hip_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_
     std::iota(ex.begin(), ex.end(), 0);
     d_x.upload(xs.data(), d_x.size(), s);
     d_ex.upload(ex.data(), ex.size(), s);
-    KnnTopkParams prm;
-    prm.train = d_x.get();
-    prm.query = d_x.get();
-    prm.train_rows = n;
-    prm.query_rows = n;
-    prm.dims = p;
-    prm.k = kk;
-    prm.exclude = d_ex.get();
-    prm.out_index = d_idx.get();
-    prm.out_dist = d_dist.get();
-    check(launch_knn_topk(prm, 0, metric == KODAMA_HIP_EUCLIDEAN, n, 1, s), "knn_topk launch");
+    device_exact_knn(d_x.get(), n, d_x.get(), n, p, kk, metric, d_ex.get(), d_idx.get(), d_dist.get(),
+                     max_row_norm2(xs.data(), n, p), s, &last_knn_fallbacks());
     check(launch_graph_publish(d_idx.get(), d_dist.get(), d_idx.size(), metric, s), "graph_publish launch");
     d_idx.download(indices_one_based, d_idx.size(), s);
     d_dist.download(distances, d_dist.size(), s);

@@ -14,6 +14,7 @@
 #include "capi_common.h"
 #include "engine.h"
 #include "evolve_device.h"
+#include "knn_mfma.h"
 #include "launch.h"
 
 using namespace kodama_hip;
@@ -65,17 +66,9 @@ int kodama_hip_exact_knn_search(const float* train, int32_t train_rows, const fl
     d_train.upload(train, d_train.size(), s);
     d_query.upload(query, d_query.size(), s);
     if (query_train_indices) d_ex.upload(query_train_indices, d_ex.size(), s);
-    KnnTopkParams prm;
-    prm.train = d_train.get();
-    prm.query = d_query.get();
-    prm.train_rows = train_rows;
-    prm.query_rows = query_rows;
-    prm.dims = dimensions;
-    prm.k = k;
-    prm.exclude = query_train_indices ? d_ex.get() : nullptr;
-    prm.out_index = d_idx.get();
-    prm.out_dist = d_dist.get();
-    check(launch_knn_topk(prm, 0, metric == KODAMA_HIP_EUCLIDEAN, query_rows, 1, s), "knn_topk launch");
+    device_exact_knn(d_train.get(), train_rows, d_query.get(), query_rows, dimensions, k, metric,
+                     query_train_indices ? d_ex.get() : nullptr, d_idx.get(), d_dist.get(),
+                     max_row_norm2(train, train_rows, dimensions), s, &last_knn_fallbacks());
     d_idx.download(out_indices, d_idx.size(), s);
     d_dist.download(out_distances, d_dist.size(), s);
     st.sync();

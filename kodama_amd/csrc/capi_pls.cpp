@@ -1,6 +1,7 @@
 // C ABI of libkodama_hip.so, PLS-LDA half: PLSLDACV, CorePLSLDA and the fold
 // detail used to check scores and discriminants against an FP64 restatement.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -116,6 +117,118 @@ int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* 
     }
     run.export_result(result);
     if (result->fold_assignments) std::copy(first_folds.begin(), first_folds.end(), result->fold_assignments);
+  });
+}
+
+// The M-run form of CorePLSLDA: run r evolves its landmark rows with seed + r
+// and cv_seed + r (kodama_matrix.cpp:3113, 3123-3135); every run's proposals
+// come from its own host engine with the reference's random streams, and one
+// group + evaluation launch per cycle serves all runs x folds: the landmark
+// rows of all runs are concatenated and fold f of run r is fold r * F + f of
+// that matrix. Per run the result equals kodama_hip_core_plslda on its rows.
+int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
+                                 const int32_t* landmark_rows, const int32_t* initial_labels,
+                                 const int32_t* fold_ids, const kodama_hip_core_options* options,
+                                 int32_t* clbest, double* accbest, double* vect_acc, double* timings) {
+  return guarded([&] {
+    require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
+    require(runs >= 1, "runs must be positive.");
+    require(run_offsets != nullptr && landmark_rows != nullptr && initial_labels != nullptr, "null input.");
+    require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
+    require(options->max_components >= 1, "PLSOptions::max_components must be positive.");
+    require(options->folds >= 2, "PLS cross-validation needs at least two folds.");
+    require(options->cycles >= 0, "Core cycles must be non-negative.");
+    require(options->stratified == 0 || fold_ids == nullptr, "explicit folds are unstratified.");
+    select_device(options->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    const IVec offsets(run_offsets, run_offsets + runs + 1);
+    require(offsets[0] == 0, "run_offsets[0] must be 0.");
+    for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
+    const int total = offsets[runs];
+    std::vector<float> xcat(static_cast<size_t>(total) * p);
+    for (int i = 0; i < total; ++i) {
+      const int row = landmark_rows[i];
+      require(row >= 0 && row < n, "landmark row outside the data matrix.");
+      std::copy(x + static_cast<size_t>(row) * p, x + static_cast<size_t>(row + 1) * p,
+                xcat.begin() + static_cast<size_t>(i) * p);
+    }
+    std::vector<IVec> values(static_cast<size_t>(runs)), codes(static_cast<size_t>(runs));
+    int classes = 1;
+    for (int r = 0; r < runs; ++r) {
+      encode_labels(initial_labels + offsets[r], offsets[r + 1] - offsets[r], values[r], codes[r]);
+      classes = std::max(classes, static_cast<int>(values[r].size()));
+    }
+    const int F = options->folds;
+    const bool cache = options->stratified == 0;
+    auto run_folds = [&](int r, const IVec& labs) {
+      IVec fo = fold_ids ? IVec(fold_ids + offsets[r], fold_ids + offsets[r + 1])
+                         : make_folds(labs, IVec(), F, options->stratified != 0,
+                                      options->cv_seed + static_cast<std::uint64_t>(r));
+      for (int v : fo) require(v >= 0 && v < std::max(F, 1 << 20), "fold id out of range.");
+      return fo;
+    };
+    int fold_span = F;
+    if (fold_ids) {
+      for (int i = 0; i < total; ++i) fold_span = std::max(fold_span, fold_ids[i] + 1);
+    }
+    IVec gfolds(static_cast<size_t>(total));
+    auto set_folds = [&](int r, const IVec& fo) {
+      for (size_t i = 0; i < fo.size(); ++i) gfolds[offsets[r] + i] = r * fold_span + fo[i];
+    };
+    for (int r = 0; r < runs; ++r) {
+      const IVec fo = run_folds(r, IVec(initial_labels + offsets[r], initial_labels + offsets[r + 1]));
+      const auto mm = std::minmax_element(fo.begin(), fo.end());
+      require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
+      set_folds(r, fo);
+    }
+    PlsCv pls(options->device, xcat.data(), total, p, *options, classes, offsets);
+    std::vector<RunEvolution> ev;
+    ev.reserve(static_cast<size_t>(runs));
+    for (int r = 0; r < runs; ++r) {
+      CoreFlags f = CoreFlags::from(*options);
+      f.knn = false;
+      f.seed = options->seed + static_cast<std::uint64_t>(r);
+      ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
+    }
+    const double t_pre = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto t1 = std::chrono::steady_clock::now();
+    const int threads = resolve_threads(options->n_threads, runs);
+    IVec ccat(static_cast<size_t>(total)), pred(static_cast<size_t>(total));
+    auto evaluate_all = [&](int cycle) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+      for (int r = 0; r < runs; ++r) {
+        const IVec& cl = cycle < 0 ? ev[r].initial() : ev[r].propose(cycle);
+        std::copy(cl.begin(), cl.end(), ccat.begin() + offsets[r]);
+        if (!cache) set_folds(r, run_folds(r, values_of(cl, values[r])));
+      }
+      pls.evaluate(ccat, gfolds, pred.data(), nullptr);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+      for (int r = 0; r < runs; ++r) {
+        const IVec cl(ccat.begin() + offsets[r], ccat.begin() + offsets[r + 1]);
+        IVec counts, firsts;
+        const Tally t = tally_of(cl, pred.data() + offsets[r], static_cast<int>(values[r].size()), counts, firsts);
+        if (cycle < 0) ev[r].begin(pred.data() + offsets[r], t);
+        else ev[r].finish(cycle, pred.data() + offsets[r], t);
+      }
+    };
+    evaluate_all(-1);
+    for (int cycle = 0; cycle < options->cycles; ++cycle) evaluate_all(cycle);
+    const double t_evo = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    for (int r = 0; r < runs; ++r) {
+      const IVec& b = ev[r].best_codes();
+      for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];
+      accbest[r] = ev[r].accbest();
+      if (vect_acc) {
+        std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(),
+                  vect_acc + static_cast<size_t>(r) * options->cycles);
+      }
+    }
+    if (timings) {
+      timings[0] = t_pre;
+      timings[1] = t_evo;
+      timings[2] = static_cast<double>(runs) * options->cycles;
+      timings[3] = pls.eval_kernel_ms_average() * 1e-3;
+    }
   });
 }
 

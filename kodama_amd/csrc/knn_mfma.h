@@ -1,0 +1,55 @@
+// Split-BF16 MFMA exact kNN (knn_mfma.hip) and its host driver (knn_search.cpp).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+namespace kodama_hip {
+
+constexpr int kMfmaListLen = 8;       // candidates kept per (query, lane column)
+constexpr int kMfmaMaxK = 32;         // k served by the MFMA path
+constexpr int kMfmaMinTrain = 4096;   // smaller train sets use the register scan
+constexpr float kMfmaErrorScale = 8e-5f;  // E = scale * (|q|^2 + max |t|^2)
+
+struct KnnMfmaParams {
+  const float* train = nullptr;          // [train_rows x dims] f32 (exact re-score)
+  const float* query = nullptr;          // [query_rows x dims] f32
+  const unsigned short* train_hi = nullptr;  // [train_rows x 64] bf16 bits
+  const unsigned short* train_lo = nullptr;
+  const unsigned short* query_hi = nullptr;  // [query_rows x 64]
+  const unsigned short* query_lo = nullptr;
+  const float* train_norm2 = nullptr;    // [train_rows]
+  const float* query_norm2 = nullptr;    // [query_rows]
+  float train_norm2_max = 0.0f;
+  float error_scale = kMfmaErrorScale;
+  int train_rows = 0;
+  int query_rows = 0;
+  int dims = 0;
+  int k = 0;
+  const int* exclude = nullptr;          // [query_rows] train index to skip, may be null
+  float* cand_value = nullptr;           // [query_rows x 16 x kMfmaListLen]
+  int* cand_index = nullptr;
+  int* flags = nullptr;                  // [query_rows] 1 = needs the exact scan
+  int* out_index = nullptr;              // [query_rows x k]
+  float* out_dist = nullptr;             // internal distances
+};
+
+hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi, unsigned short* lo, float* norm2,
+                             hipStream_t s);
+hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s);
+// rows[i] of src [* x p] -> dst [count x p]; and the inverse scatter of [count x k] results.
+hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s);
+hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, const int* rows, int count, int k,
+                                  int* dst_idx, float* dst_dist, hipStream_t s);
+
+// Exact kNN of device-resident rows (internal distances, (distance, index)
+// order, -1 / +inf beyond the available rows), bit-identical to the register
+// scan: the MFMA path when dims <= 64, k <= 32 and train_rows >= 4096,
+// otherwise (and for every flagged query) the register scan. `train_norm2_max`
+// bounds the squared norms of the train rows (prepared rows for cosine).
+void device_exact_knn(const float* d_train, int train_rows, const float* d_query, int query_rows, int dims, int k,
+                      int metric, const int* d_exclude, int* d_idx, float* d_dist, float train_norm2_max,
+                      hipStream_t s, int* mfma_fallbacks = nullptr);
+
+}  // namespace kodama_hip

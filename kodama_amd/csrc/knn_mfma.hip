@@ -1,0 +1,418 @@
+// Exact kNN through split-BF16 MFMA candidates and an exact FP32 rerank.
+//
+// Every row x is split into bf16 parts hi = bf16(x) and lo = bf16(x - hi)
+// (64 dimensions, zero padded). The dot product of a query and a train row is
+// approximated by hi.hi + hi.lo + lo.hi, three v_mfma_f32_16x16x32_bf16 chains
+// accumulated in FP32 (the bf16 x bf16 products are exact in FP32). What the
+// split drops (lo.lo and the residuals below lo) is at most 3.01 * 2^-18
+// sum |q_i t_i| and the FP32 accumulation of 192 terms at most 192 * 2^-24 of
+// the same sum, so for squared Euclidean distances (and for minus the inner
+// product) the candidate value differs from the exact value by at most
+// E = 8e-5 (|q|^2 + max |t|^2), about twice the sum of these terms, the FP32
+// rounding of the norms and of the final combination, and the reference-order
+// rounding of the exact distance.
+//
+// knn_mfma_candidates_kernel: 512 threads, 128 queries per workgroup (8 waves
+// of 16 queries); train rows stream through LDS 64 at a time. MFMA output
+// lane l holds queries 4 (l / 16) + r (r = 0..3) against train column l % 16 of
+// each 16-row tile, so each lane keeps, per query, a sorted list of its 8
+// smallest candidate values over the train rows of its column class.
+//
+// knn_rerank_kernel: one wave per query. A_k = the k-th smallest candidate
+// value over the query's 16 lists. Any train row whose exact distance can be
+// among the k smallest has a candidate value <= A_k + 2E; if a full list's
+// largest value is <= A_k + 2E it may have dropped such a row, and the query
+// is flagged for the exact scan. Otherwise every candidate <= A_k + 2E is
+// re-scored exactly (the reference's FP32 summation order) and the k smallest
+// by (distance, index) are written: the same rows and values as the exact scan.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "knn_mfma.h"
+
+namespace kodama_hip {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+constexpr int kCandThreads = 512;
+constexpr int kQueriesPerBlock = 128;   // 8 waves x 16 queries
+constexpr int kChunk = 64;              // train rows per LDS stage
+constexpr int kRowStride = 72;          // bf16 per staged row (64 + 8 pad: spreads LDS banks)
+constexpr int kList = kMfmaListLen;     // candidates kept per (query, lane column)
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  // Round to nearest even (finite inputs).
+  const uint32_t b = __float_as_uint(x);
+  return static_cast<unsigned short>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float bf16_value(unsigned short h) {
+  return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Ascending list insert; a value equal to the largest kept one is dropped.
+__device__ __forceinline__ void list_insert(float (&d)[kList], int (&id)[kList], float v, int j) {
+  if (!(v < d[kList - 1])) return;
+  d[kList - 1] = v;
+  id[kList - 1] = j;
+#pragma unroll
+  for (int s = kList - 1; s > 0; --s) {
+    if (d[s] < d[s - 1]) {
+      const float td = d[s - 1];
+      const int ti = id[s - 1];
+      d[s - 1] = d[s];
+      id[s - 1] = id[s];
+      d[s] = td;
+      id[s] = ti;
+    }
+  }
+}
+
+// Reference-order FP32 distance (native_knn.cpp:41-95) for p <= 64, one thread.
+template <bool EUCLID>
+__device__ __forceinline__ float exact_distance(const float (&q)[64], const float* __restrict__ c, int p) {
+  const int nb = p >> 5;
+  float acc[32];
+#pragma unroll
+  for (int s = 0; s < 32; ++s) acc[s] = 0.0f;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    if (b < nb) {
+#pragma unroll
+      for (int s = 0; s < 32; ++s) {
+        const float cv = c[b * 32 + s];
+        if (EUCLID) {
+          const float t = __fsub_rn(q[b * 32 + s], cv);
+          acc[s] = __fmaf_rn(t, t, acc[s]);
+        } else {
+          acc[s] = __fmaf_rn(q[b * 32 + s], cv, acc[s]);
+        }
+      }
+    }
+  }
+  float v4[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const float lo = __fadd_rn(__fadd_rn(acc[l], acc[8 + l]), __fadd_rn(acc[16 + l], acc[24 + l]));
+    const float hi = __fadd_rn(__fadd_rn(acc[4 + l], acc[12 + l]), __fadd_rn(acc[20 + l], acc[28 + l]));
+    v4[l] = __fadd_rn(lo, hi);
+  }
+  float r = __fadd_rn(__fadd_rn(v4[0], v4[1]), __fadd_rn(v4[2], v4[3]));
+  const int tail = nb * 32;
+#pragma unroll
+  for (int d = 0; d < 64; ++d) {
+    if (d >= tail && d < p) {
+      const float cv = c[d];
+      if (EUCLID) {
+        const float t = __fsub_rn(q[d], cv);
+        r = __fmaf_rn(t, t, r);
+      } else {
+        r = __fmaf_rn(q[d], cv, r);
+      }
+    }
+  }
+  return EUCLID ? r : -r;
+}
+
+}  // namespace
+
+// Rows -> bf16 hi / lo parts [rows x 64] and FP32 squared norms. One thread per row.
+__global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, unsigned short* __restrict__ hi,
+                                  unsigned short* __restrict__ lo, float* __restrict__ norm2) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float* src = x + static_cast<size_t>(r) * p;
+  float s = 0.0f;
+  for (int d = 0; d < 64; ++d) {
+    const float v = d < p ? src[d] : 0.0f;
+    const unsigned short h = bf16_bits(v);
+    hi[static_cast<size_t>(r) * 64 + d] = h;
+    lo[static_cast<size_t>(r) * 64 + d] = bf16_bits(v - bf16_value(h));
+    s = fmaf(v, v, s);
+  }
+  norm2[r] = s;
+}
+
+template <bool EUCLID>
+__global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
+  __shared__ unsigned short s_hi[kChunk * kRowStride];
+  __shared__ unsigned short s_lo[kChunk * kRowStride];
+  __shared__ float s_tn[kChunk];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int grp = lane >> 4;       // k group of the operands, query group of the output
+  const int col = lane & 15;       // A row / B column / output column
+  const int q_tile = blockIdx.x * kQueriesPerBlock + wave * 16;
+
+  // A operands: query q_tile + col, dimensions kb * 32 + 8 grp .. + 7.
+  uint4 a_hi[2], a_lo[2];
+  {
+    const int q = q_tile + col;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      if (q < prm.query_rows) {
+        a_hi[kb] = *reinterpret_cast<const uint4*>(prm.query_hi + static_cast<size_t>(q) * 64 + kb * 32 + 8 * grp);
+        a_lo[kb] = *reinterpret_cast<const uint4*>(prm.query_lo + static_cast<size_t>(q) * 64 + kb * 32 + 8 * grp);
+      } else {
+        a_hi[kb] = make_uint4(0, 0, 0, 0);
+        a_lo[kb] = make_uint4(0, 0, 0, 0);
+      }
+    }
+  }
+  // This lane's output queries.
+  float qn[4];
+  int excl[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int q = q_tile + 4 * grp + r;
+    qn[r] = q < prm.query_rows ? prm.query_norm2[q] : 0.0f;
+    excl[r] = (q < prm.query_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
+  }
+  float kd[4][kList];
+  int ki[4][kList];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int s = 0; s < kList; ++s) {
+      kd[r][s] = INFINITY;
+      ki[r][s] = -1;
+    }
+  }
+
+  for (int c0 = 0; c0 < prm.train_rows; c0 += kChunk) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kChunk * 16; e += kCandThreads) {
+      // 64 rows x (8 hi + 8 lo uint4 parts).
+      const int row = e >> 4, part = e & 15;
+      const int tr = c0 + row;
+      const bool is_lo = part >= 8;
+      const int d8 = (part & 7) * 8;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (tr < prm.train_rows) {
+        const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(tr) * 64 + d8;
+        v = *reinterpret_cast<const uint4*>(src);
+      }
+      *reinterpret_cast<uint4*>((is_lo ? s_lo : s_hi) + row * kRowStride + d8) = v;
+    }
+    if (threadIdx.x < kChunk) {
+      s_tn[threadIdx.x] = c0 + static_cast<int>(threadIdx.x) < prm.train_rows ? prm.train_norm2[c0 + threadIdx.x] : 0.0f;
+    }
+    __syncthreads();
+    f32x4 acc[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) acc[tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      uint4 b_hi[4], b_lo[4];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        const int off = (tt * 16 + col) * kRowStride + kb * 32 + 8 * grp;
+        b_hi[tt] = *reinterpret_cast<const uint4*>(s_hi + off);
+        b_lo[tt] = *reinterpret_cast<const uint4*>(s_lo + off);
+      }
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a_hi[kb]), as_bf16x8(b_hi[tt]), acc[tt], 0, 0, 0);
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a_hi[kb]), as_bf16x8(b_lo[tt]), acc[tt], 0, 0, 0);
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a_lo[kb]), as_bf16x8(b_hi[tt]), acc[tt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      const int j = c0 + tt * 16 + col;
+      if (j < prm.train_rows) {
+        const float tn = s_tn[tt * 16 + col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (j != excl[r]) {
+            const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
+            list_insert(kd[r], ki[r], v, j);
+          }
+        }
+      }
+    }
+  }
+
+  // Lists: query q owns [16 columns x kList] entries, column-major by lane column.
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int q = q_tile + 4 * grp + r;
+    if (q < prm.query_rows) {
+      const size_t base = (static_cast<size_t>(q) * 16 + col) * kList;
+#pragma unroll
+      for (int s = 0; s < kList; ++s) {
+        prm.cand_value[base + s] = kd[r][s];
+        prm.cand_index[base + s] = ki[r][s];
+      }
+    }
+  }
+}
+
+// One wave per query; 4 queries per 256-thread workgroup.
+template <bool EUCLID>
+__global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
+  constexpr int kCand = 16 * kList;  // 128
+  __shared__ float s_val[4][kCand];
+  __shared__ int s_idx[4][kCand];
+  __shared__ float s_ex[4][kCand];
+  __shared__ int s_sel[4][kCand];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int q = blockIdx.x * 4 + wave;
+  const bool live = q < prm.query_rows;  // whole waves; no block barrier below
+  if (!live) return;
+  const int k = prm.k;
+  const int p = prm.dims;
+  float* val = s_val[wave];
+  int* idx = s_idx[wave];
+  for (int c = lane; c < kCand; c += kWave) {
+    val[c] = prm.cand_value[static_cast<size_t>(q) * kCand + c];
+    idx[c] = prm.cand_index[static_cast<size_t>(q) * kCand + c];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // A_k: the candidate of rank k - 1 in (value, index) order among valid ones.
+  int valid = 0;
+  float ak = INFINITY;
+  for (int c = lane; c < kCand; c += kWave) {
+    if (idx[c] < 0) continue;
+    int rank = 0;
+    for (int e = 0; e < kCand; ++e) {
+      if (idx[e] < 0) continue;
+      rank += (val[e] < val[c] || (val[e] == val[c] && idx[e] < idx[c])) ? 1 : 0;
+    }
+    if (rank == k - 1) ak = val[c];
+  }
+  for (int c = lane; c < kCand; c += kWave) valid += idx[c] >= 0 ? 1 : 0;
+#pragma unroll
+  for (int m = kWave / 2; m >= 1; m >>= 1) {
+    valid += __shfl_xor(valid, m);
+    ak = fminf(ak, __shfl_xor(ak, m));
+  }
+  // Fewer valid candidates than k: every one is kept.
+  const float bound = valid >= k ? ak + 2.0f * (prm.error_scale * (prm.query_norm2[q] + prm.train_norm2_max)) : INFINITY;
+  // A full list whose largest value is within the bound may have dropped a row.
+  bool overflow = false;
+  if (lane < 16) {
+    const int last = lane * kList + kList - 1;
+    overflow = idx[last] >= 0 && val[last] <= bound;
+  }
+  if (__ballot(overflow) != 0ull) {
+    if (lane == 0) prm.flags[q] = 1;
+    return;
+  }
+  if (lane == 0) prm.flags[q] = 0;
+  // Exact re-score of the candidates within the bound.
+  float qv[64];
+  const float* qrow = prm.query + static_cast<size_t>(q) * p;
+#pragma unroll
+  for (int d = 0; d < 64; ++d) qv[d] = d < p ? qrow[d] : 0.0f;
+  float* ex = s_ex[wave];
+  int* sel = s_sel[wave];
+  for (int c = lane; c < kCand; c += kWave) {
+    const bool use = idx[c] >= 0 && val[c] <= bound;
+    sel[c] = use ? 1 : 0;
+    ex[c] = use ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(idx[c]) * p, p) : INFINITY;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  int nsel = 0;
+  for (int c = lane; c < kCand; c += kWave) nsel += sel[c];
+#pragma unroll
+  for (int m = kWave / 2; m >= 1; m >>= 1) nsel += __shfl_xor(nsel, m);
+  for (int c = lane; c < kCand; c += kWave) {
+    if (!sel[c]) continue;
+    int rank = 0;
+    for (int e = 0; e < kCand; ++e) {
+      if (!sel[e]) continue;
+      rank += (ex[e] < ex[c] || (ex[e] == ex[c] && idx[e] < idx[c])) ? 1 : 0;
+    }
+    if (rank < k) {
+      prm.out_index[static_cast<size_t>(q) * k + rank] = idx[c];
+      prm.out_dist[static_cast<size_t>(q) * k + rank] = ex[c];
+    }
+  }
+  for (int s = nsel + lane; s < k; s += kWave) {
+    prm.out_index[static_cast<size_t>(q) * k + s] = -1;
+    prm.out_dist[static_cast<size_t>(q) * k + s] = INFINITY;
+  }
+}
+
+hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi, unsigned short* lo, float* norm2,
+                             hipStream_t s) {
+  if (rows == 0) return hipSuccess;
+  if (p < 1 || p > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(split_rows_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, s, x, rows, p,
+                     hi, lo, norm2);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s) {
+  if (prm.query_rows == 0) return hipSuccess;
+  if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.train_rows < 1) {
+    return hipErrorInvalidValue;
+  }
+  const unsigned blocks = static_cast<unsigned>((prm.query_rows + kQueriesPerBlock - 1) / kQueriesPerBlock);
+  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
+  if (euclid) {
+    hipLaunchKernelGGL(knn_mfma_candidates_kernel<true>, dim3(blocks), dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL(knn_rerank_kernel<true>, dim3(rblocks), dim3(256), 0, s, prm);
+  } else {
+    hipLaunchKernelGGL(knn_mfma_candidates_kernel<false>, dim3(blocks), dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL(knn_rerank_kernel<false>, dim3(rblocks), dim3(256), 0, s, prm);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace kodama_hip
+
+namespace kodama_hip {
+
+__global__ void gather_rows_kernel(const float* __restrict__ src, const int* __restrict__ rows, int count, int p,
+                                   float* __restrict__ dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(count) * p) return;
+  const int r = static_cast<int>(i / p), d = static_cast<int>(i % p);
+  dst[i] = src[static_cast<size_t>(rows[r]) * p + d];
+}
+
+__global__ void scatter_results_kernel(const int* __restrict__ src_idx, const float* __restrict__ src_dist,
+                                       const int* __restrict__ rows, int count, int k, int* __restrict__ dst_idx,
+                                       float* __restrict__ dst_dist) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(count) * k) return;
+  const int r = static_cast<int>(i / k), j = static_cast<int>(i % k);
+  dst_idx[static_cast<size_t>(rows[r]) * k + j] = src_idx[i];
+  dst_dist[static_cast<size_t>(rows[r]) * k + j] = src_dist[i];
+}
+
+hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s) {
+  const size_t total = static_cast<size_t>(count) * p;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, src,
+                     rows, count, p, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, const int* rows, int count, int k,
+                                  int* dst_idx, float* dst_dist, hipStream_t s) {
+  const size_t total = static_cast<size_t>(count) * k;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_results_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,
+                     src_idx, src_dist, rows, count, k, dst_idx, dst_dist);
+  return hipGetLastError();
+}
+
+}  // namespace kodama_hip

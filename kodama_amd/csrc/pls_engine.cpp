@@ -6,6 +6,7 @@
 #include "pls_engine.h"
 
 #include <algorithm>
+#include <map>
 #include <random>
 #include <stdexcept>
 
@@ -28,11 +29,14 @@ struct PlsCv::Impl {
   Stream stream;
   DeviceBuffer<float> x;
   DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, s0, gw, linear, constant, val_scores, disc;
-  DeviceBuffer<int> train_asc, train_off, train_sorted, seg_code, seg_off, seg_start, val_rows, val_off;
-  DeviceBuffer<int> pred, comps, rank;
+  DeviceBuffer<int> train_asc, train_off, train_sorted, seg_code, seg_start, seg_count, val_rows, val_off;
+  DeviceBuffer<int> codes, pred, comps, rank;
+  IVec groups;  // row ranges of the runs: [groups[g], groups[g+1])
   IVec cached_folds, fold_ids, h_train_rows, h_train_off, h_val_rows, h_val_off;
-  IVec h_seg_code, h_seg_off, h_comps, h_rank;
-  int last_cmax = 0;
+  IVec h_comps, h_rank;
+  EventTimer timer;
+  double eval_ms_total = 0.0;
+  int eval_calls = 0;
 
   explicit Impl(int dev) : device(dev), stream(dev) {}
 
@@ -40,16 +44,25 @@ struct PlsCv::Impl {
 };
 
 void PlsCv::Impl::rebuild_folds(const IVec& folds) {
-  fold_ids = folds;
-  std::sort(fold_ids.begin(), fold_ids.end());
-  fold_ids.erase(std::unique(fold_ids.begin(), fold_ids.end()), fold_ids.end());
+  // A fold belongs to the group (run) of its rows; its training rows are the
+  // other rows of that group only.
+  std::map<int, int> group_of;
+  for (int g = 0; g + 1 < static_cast<int>(groups.size()); ++g) {
+    for (int i = groups[g]; i < groups[g + 1]; ++i) {
+      const auto it = group_of.emplace(folds[i], g).first;
+      if (it->second != g) throw InvalidArgument("a fold id spans two runs.");
+    }
+  }
+  fold_ids.clear();
+  for (const auto& kv : group_of) fold_ids.push_back(kv.first);
   const int F = static_cast<int>(fold_ids.size());
   h_train_rows.clear();
   h_val_rows.clear();
   h_train_off.assign(1, 0);
   h_val_off.assign(1, 0);
   for (int fid : fold_ids) {
-    for (int i = 0; i < n; ++i) (folds[i] == fid ? h_val_rows : h_train_rows).push_back(i);
+    const int g = group_of[fid];
+    for (int i = groups[g]; i < groups[g + 1]; ++i) (folds[i] == fid ? h_val_rows : h_train_rows).push_back(i);
     h_train_off.push_back(static_cast<int>(h_train_rows.size()));
     h_val_off.push_back(static_cast<int>(h_val_rows.size()));
   }
@@ -79,6 +92,9 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
   constant.resize(static_cast<size_t>(F) * classes);
   comps.resize(static_cast<size_t>(F));
   rank.resize(static_cast<size_t>(F));
+  seg_code.resize(static_cast<size_t>(F) * classes);
+  seg_start.resize(static_cast<size_t>(F) * (classes + 1));
+  seg_count.resize(static_cast<size_t>(F));
   PlsCacheParams c;
   c.x = x.get();
   c.p = p;
@@ -95,12 +111,18 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
   cached_folds = folds;
 }
 
-PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& o, int classes) {
+PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& o, int classes,
+             const IVec& run_offsets) {
   if (o.max_components < 1) throw InvalidArgument("PLSOptions::max_components must be positive.");
   if (p > kPlsMaxDims) throw InvalidArgument("the HIP PLS-LDA kernels support up to 256 feature columns.");
   Impl* m = new Impl(device);
   m->n = n;
   m->p = p;
+  m->groups = run_offsets.empty() ? IVec{0, n} : run_offsets;
+  if (m->groups.front() != 0 || m->groups.back() != n) {
+    delete m;
+    throw InvalidArgument("run offsets must cover the rows.");
+  }
   m->classes = std::max(1, classes);
   m->max_components = o.max_components;
   m->requested = o.fixed_components > 0 ? o.fixed_components : o.max_components;
@@ -117,6 +139,12 @@ PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_opt
   const std::vector<double> w0 = pls_start_vector(p);
   m->w0.resize(w0.size());
   m->w0.upload(w0.data(), w0.size(), s);
+  if (static_cast<size_t>(m->classes) * p > static_cast<size_t>(kPlsMaxSLds) ||
+      pls_eval_lds_bytes(p, m->classes) > 160 * 1024) {
+    delete m;
+    throw InvalidArgument("the HIP PLS-LDA kernel holds classes x columns <= 8192 in LDS.");
+  }
+  m->codes.resize(static_cast<size_t>(n));
   m->pred.resize(static_cast<size_t>(n));
   m->val_scores.resize(static_cast<size_t>(n) * kPlsMaxComponents);
   m->disc.resize(static_cast<size_t>(n) * m->classes);
@@ -130,48 +158,24 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   if (static_cast<int>(codes.size()) != m.n || static_cast<int>(folds.size()) != m.n) {
     throw InvalidArgument("labels and folds must have one entry per row.");
   }
+  for (int c : codes) {
+    if (c < 0 || c >= m.classes) throw InvalidArgument("label code out of range.");
+  }
   if (folds != m.cached_folds) m.rebuild_folds(folds);
   const int F = static_cast<int>(m.fold_ids.size());
-  // Training rows of each fold grouped by class code (rows ascending inside a class).
-  IVec sorted;
-  sorted.reserve(m.h_train_rows.size());
-  m.h_seg_code.clear();
-  m.h_seg_off.assign(1, 0);
-  IVec seg_start;
-  std::vector<IVec> bucket(static_cast<size_t>(m.classes));
-  int cmax = 1;
-  for (int f = 0; f < F; ++f) {
-    for (auto& b : bucket) b.clear();
-    for (int i = m.h_train_off[f]; i < m.h_train_off[f + 1]; ++i) {
-      const int r = m.h_train_rows[i];
-      const int c = codes[r];
-      if (c < 0 || c >= m.classes) throw InvalidArgument("label code out of range.");
-      bucket[c].push_back(r);
-    }
-    int cf = 0;
-    for (int c = 0; c < m.classes; ++c) {
-      if (bucket[c].empty()) continue;
-      m.h_seg_code.push_back(c);
-      seg_start.push_back(static_cast<int>(sorted.size()));
-      sorted.insert(sorted.end(), bucket[c].begin(), bucket[c].end());
-      ++cf;
-    }
-    seg_start.push_back(static_cast<int>(sorted.size()));
-    m.h_seg_off.push_back(static_cast<int>(m.h_seg_code.size()));
-    cmax = std::max(cmax, cf);
-  }
-  if (static_cast<size_t>(cmax) * m.p > static_cast<size_t>(kPlsMaxSLds) ||
-      pls_eval_lds_bytes(m.p, cmax) > 160 * 1024) {
-    throw InvalidArgument("the HIP PLS-LDA kernel holds classes x columns <= 8192 in LDS.");
-  }
   const hipStream_t s = m.stream.get();
-  m.train_sorted.upload(sorted.data(), sorted.size(), s);
-  m.seg_code.resize(m.h_seg_code.size());
-  m.seg_off.resize(m.h_seg_off.size());
-  m.seg_start.resize(seg_start.size());
-  m.seg_code.upload(m.h_seg_code.data(), m.h_seg_code.size(), s);
-  m.seg_off.upload(m.h_seg_off.data(), m.h_seg_off.size(), s);
-  m.seg_start.upload(seg_start.data(), seg_start.size(), s);
+  m.codes.upload(codes.data(), codes.size(), s);
+  PlsGroupParams g;
+  g.train.rows = m.train_asc.get();
+  g.train.offsets = m.train_off.get();
+  g.codes = m.codes.get();
+  g.folds = F;
+  g.max_classes = m.classes;
+  g.sorted = m.train_sorted.get();
+  g.seg_code = m.seg_code.get();
+  g.seg_start = m.seg_start.get();
+  g.seg_count = m.seg_count.get();
+  check(launch_pls_group(g, s), "pls_group launch");
   PlsEvalParams e;
   e.x = m.x.get();
   e.p = m.p;
@@ -186,11 +190,11 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   e.train.rows = m.train_sorted.get();
   e.train.offsets = m.train_off.get();
   e.seg_code = m.seg_code.get();
-  e.seg_offsets = m.seg_off.get();
   e.seg_start = m.seg_start.get();
+  e.seg_count = m.seg_count.get();
   e.val.rows = m.val_rows.get();
   e.val.offsets = m.val_off.get();
-  e.max_classes = cmax;
+  e.max_classes = m.classes;
   e.weights = m.weights.get();
   e.loadings = m.loadings.get();
   e.s0 = m.s0.get();
@@ -202,20 +206,27 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   e.rank = m.rank.get();
   e.val_scores = m.val_scores.get();
   e.discriminants = m.disc.get();
+  m.timer.start(s);
   check(launch_pls_eval(e, s), "pls_eval launch");
-  m.last_cmax = cmax;
+  m.timer.stop(s);
   m.h_comps.assign(static_cast<size_t>(F), 0);
   m.h_rank.assign(static_cast<size_t>(F), 0);
   m.pred.download(pred_codes, static_cast<size_t>(m.n), s);
   m.comps.download(m.h_comps.data(), m.h_comps.size(), s);
   m.rank.download(m.h_rank.data(), m.h_rank.size(), s);
   m.stream.sync();
+  m.eval_ms_total += m.timer.ms();
+  ++m.eval_calls;
   int evaluated = m.requested;
   for (int f = 0; f < F; ++f) {
     if (m.h_comps[f] < 0) throw std::runtime_error("PLS-LDA Cholesky factorization failed.");
     evaluated = std::min(evaluated, m.h_comps[f]);
   }
   if (selected) *selected = std::min(evaluated, m.max_components);
+}
+
+double PlsCv::eval_kernel_ms_average() const {
+  return impl_->eval_calls ? impl_->eval_ms_total / impl_->eval_calls : 0.0;
 }
 
 PlsFoldDetail PlsCv::detail(int fold) const {
@@ -225,17 +236,23 @@ PlsFoldDetail PlsCv::detail(int fold) const {
   PlsFoldDetail d;
   d.rank = m.h_rank[fold];
   d.components = m.h_comps[fold];
-  d.classes.assign(m.h_seg_code.begin() + m.h_seg_off[fold], m.h_seg_code.begin() + m.h_seg_off[fold + 1]);
+  {
+    int count = 0;
+    m.seg_count.download(&count, 1, m.stream.get(), static_cast<size_t>(fold));
+    d.classes.assign(static_cast<size_t>(std::max(0, count)), 0);
+    m.seg_code.download(d.classes.data(), d.classes.size(), m.stream.get(), static_cast<size_t>(fold) * m.classes);
+    check(hipStreamSynchronize(m.stream.get()), "detail sync");
+  }
   d.val_rows.assign(m.h_val_rows.begin() + m.h_val_off[fold], m.h_val_rows.begin() + m.h_val_off[fold + 1]);
   const int nv = static_cast<int>(d.val_rows.size());
   const int cf = static_cast<int>(d.classes.size());
   const hipStream_t s = m.stream.get();
   std::vector<double> w(static_cast<size_t>(m.p) * kPlsMaxComponents);
   std::vector<double> tv(static_cast<size_t>(nv) * kPlsMaxComponents);
-  std::vector<double> disc(static_cast<size_t>(nv) * m.last_cmax);
+  std::vector<double> disc(static_cast<size_t>(nv) * m.classes);
   m.weights.download(w.data(), w.size(), s, static_cast<size_t>(fold) * m.p * kPlsMaxComponents);
   m.val_scores.download(tv.data(), tv.size(), s, static_cast<size_t>(m.h_val_off[fold]) * kPlsMaxComponents);
-  m.disc.download(disc.data(), disc.size(), s, static_cast<size_t>(m.h_val_off[fold]) * m.last_cmax);
+  m.disc.download(disc.data(), disc.size(), s, static_cast<size_t>(m.h_val_off[fold]) * m.classes);
   check(hipStreamSynchronize(s), "detail sync");
   const bool fitted = d.rank > 0 && cf > 1;
   const int A = fitted ? d.components : 0;
@@ -246,7 +263,7 @@ PlsFoldDetail PlsCv::detail(int fold) const {
   d.discriminants.assign(static_cast<size_t>(nv) * (fitted ? cf : 0), 0.0);
   for (int i = 0; i < nv && fitted; ++i) {
     for (int a = 0; a < A; ++a) d.val_scores[static_cast<size_t>(i) * A + a] = tv[static_cast<size_t>(i) * kPlsMaxComponents + a];
-    for (int c = 0; c < cf; ++c) d.discriminants[static_cast<size_t>(i) * cf + c] = disc[static_cast<size_t>(i) * m.last_cmax + c];
+    for (int c = 0; c < cf; ++c) d.discriminants[static_cast<size_t>(i) * cf + c] = disc[static_cast<size_t>(i) * m.classes + c];
   }
   return d;
 }

@@ -23,7 +23,11 @@ struct PlsFoldDetail {
 class PlsCv {
  public:
   // x [n x p] host rows; `classes` = number of label codes (0..classes-1).
-  PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& options, int classes);
+  // With run_offsets, rows [run_offsets[r], run_offsets[r+1]) form run r: a
+  // fold's ids belong to one run and it trains on that run's other rows only
+  // (empty: one run of all rows).
+  PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_options& options, int classes,
+        const IVec& run_offsets = IVec());
   ~PlsCv();
   PlsCv(const PlsCv&) = delete;
   PlsCv& operator=(const PlsCv&) = delete;
@@ -36,6 +40,8 @@ class PlsCv {
   void evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected);
   // Detail of fold `fold` of the last evaluation.
   PlsFoldDetail detail(int fold) const;
+  // Average duration of the fused evaluation kernel (HIP events).
+  double eval_kernel_ms_average() const;
 
  private:
   struct Impl;

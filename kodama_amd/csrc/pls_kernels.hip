@@ -22,6 +22,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 
 #include "pls_launch.h"
 
@@ -136,10 +137,9 @@ __global__ void __launch_bounds__(kT) pls_eval_kernel(PlsEvalParams prm) {
   const int tid = threadIdx.x;
   const int t0 = prm.train.offsets[f];
   const int nt = prm.train.offsets[f + 1] - t0;
-  const int sb = prm.seg_offsets[f];
-  const int cf = prm.seg_offsets[f + 1] - sb;
-  const int* seg_code = prm.seg_code + sb;
-  const int* seg_start = prm.seg_start + sb + f;  // cf + 1 entries into the concatenated training rows
+  const int cf = prm.seg_count[f];
+  const int* seg_code = prm.seg_code + static_cast<size_t>(f) * prm.max_classes;
+  const int* seg_start = prm.seg_start + static_cast<size_t>(f) * (prm.max_classes + 1);  // cf + 1 entries
   const int v0 = prm.val.offsets[f];
   const int nv = prm.val.offsets[f + 1] - v0;
   const int* vrows = prm.val.rows + v0;
@@ -462,7 +462,76 @@ __global__ void __launch_bounds__(kT) pls_eval_kernel(PlsEvalParams prm) {
   }
 }
 
+// ---- class grouping -----------------------------------------------------------
+
+// One workgroup per fold: class counts, the segment table (classes ascending),
+// then one wave scatters the rows in their original order -- per 64-row chunk,
+// the lanes of one class take consecutive slots in lane order -- so every class
+// segment lists its rows ascending.
+__global__ void __launch_bounds__(kT) pls_group_kernel(PlsGroupParams prm) {
+  extern __shared__ int grp_lds[];
+  const int C = prm.max_classes;
+  int* cnt = grp_lds;       // [C]
+  int* base = grp_lds + C;  // [C]
+  const int f = blockIdx.x;
+  const int t0 = prm.train.offsets[f];
+  const int nt = prm.train.offsets[f + 1] - t0;
+  const int* rows = prm.train.rows + t0;
+  for (int c = threadIdx.x; c < C; c += kT) cnt[c] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < nt; i += kT) atomicAdd(&cnt[prm.codes[rows[i]]], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0, acc = 0;
+    int* code = prm.seg_code + static_cast<size_t>(f) * C;
+    int* start = prm.seg_start + static_cast<size_t>(f) * (C + 1);
+    for (int c = 0; c < C; ++c) {
+      if (cnt[c] == 0) continue;
+      code[s] = c;
+      start[s] = t0 + acc;
+      base[c] = t0 + acc;
+      acc += cnt[c];
+      ++s;
+    }
+    start[s] = t0 + acc;
+    prm.seg_count[f] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int c0 = 0; c0 < nt; c0 += kWave) {
+      const int i = c0 + lane;
+      const int row = i < nt ? rows[i] : -1;
+      const int code = i < nt ? prm.codes[row] : -1;
+      uint64_t remaining = __ballot(i < nt);
+      while (remaining != 0ull) {
+        const int leader = __ffsll(static_cast<unsigned long long>(remaining)) - 1;
+        const int lc = __shfl(code, leader);
+        const uint64_t mask = __ballot(code == lc) & remaining;
+        const int b = base[lc];
+        if ((mask >> lane) & 1ull) prm.sorted[b + __popcll(mask & below)] = row;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == leader) base[lc] = b + __popcll(mask);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        remaining &= ~mask;
+      }
+    }
+  }
+}
+
 // ---- launchers ----------------------------------------------------------------
+
+hipError_t launch_pls_group(const PlsGroupParams& prm, hipStream_t s) {
+  if (prm.folds == 0) return hipSuccess;
+  if (prm.max_classes < 1) return hipErrorInvalidValue;
+  const size_t shmem = 2 * sizeof(int) * static_cast<size_t>(prm.max_classes);
+  if (shmem > 64 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pls_group_kernel, dim3(prm.folds), dim3(kT), shmem, s, prm);
+  return hipGetLastError();
+}
 
 size_t pls_eval_lds_bytes(int p, int max_classes) {
   const int rc = p < kPlsMaxComponents ? p : kPlsMaxComponents;

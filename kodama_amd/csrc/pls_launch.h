@@ -46,16 +46,17 @@ struct PlsEvalParams {
   const double* sd = nullptr;
   const double* xsum = nullptr;
   const double* gram = nullptr;
-  // Training rows grouped by class code (ascending code, rows ascending within
-  // a class): fold f's classes are seg_code[seg_offsets[f] .. seg_offsets[f+1])
-  // and class s of that range owns train.rows[seg_start[s] .. seg_start[s+1])
-  // (seg_start indexes the concatenated training rows; one extra entry per fold).
+  // Training rows grouped by class code (ascending code, rows ascending inside
+  // a class), as pls_group_kernel writes them: fold f has seg_count[f]
+  // classes, class s is seg_code[f * max_classes + s] and owns train.rows[i]
+  // for seg_start[f * (max_classes + 1) + s] <= i < seg_start[... + s + 1]
+  // (indices into the concatenated training rows).
   PlsFoldRows train;
-  const int* seg_code = nullptr;     // [total classes]
-  const int* seg_offsets = nullptr;  // [folds + 1]
-  const int* seg_start = nullptr;    // [total classes + folds]
+  const int* seg_code = nullptr;     // [folds x max_classes]
+  const int* seg_start = nullptr;    // [folds x (max_classes + 1)]
+  const int* seg_count = nullptr;    // [folds]
   PlsFoldRows val;
-  int max_classes = 0;               // largest class count of a fold
+  int max_classes = 0;               // label codes (0 .. max_classes - 1)
   // Scratch (device, per fold).
   double* weights = nullptr;         // [folds x p x kPlsMaxComponents], [j * kA + a]
   double* loadings = nullptr;        // [folds x p x kPlsMaxComponents]
@@ -73,7 +74,21 @@ struct PlsEvalParams {
   double* discriminants = nullptr;
 };
 
+// Groups each fold's training rows (ascending, `train`) by the class code of
+// each row (codes[row]) into `sorted`, stably, and writes the segment tables.
+struct PlsGroupParams {
+  PlsFoldRows train;
+  const int* codes = nullptr;        // [rows] label code per row
+  int folds = 0;
+  int max_classes = 0;
+  int* sorted = nullptr;             // [train total]
+  int* seg_code = nullptr;           // [folds x max_classes]
+  int* seg_start = nullptr;          // [folds x (max_classes + 1)]
+  int* seg_count = nullptr;          // [folds]
+};
+
 size_t pls_eval_lds_bytes(int p, int max_classes);
+hipError_t launch_pls_group(const PlsGroupParams& prm, hipStream_t s);
 hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s);
 hipError_t launch_pls_eval(const PlsEvalParams& prm, hipStream_t s);
 

@@ -130,3 +130,23 @@ def test_more_components_than_columns(oracle):
     pred, folds, sel = oracle.plsldacv(x, truth, O.core_options(**o), precision=64)
     assert np.array_equal(got["predicted_labels"], pred)
     assert got["selected_components"] == sel
+
+
+def test_core_plslda_batch_equals_independent_runs():
+    """The M-run batch is the per-run CorePLSLDA with seed + r and cv_seed + r."""
+    x, truth = D.config2(n=3000, seed=91)
+    rows, init = [], []
+    for r in range(4):
+        L = 400 + 150 * r
+        rw = np.sort(np.argsort(D.uniform(91, 10 + r, 3000))[:L]).astype(np.int32)
+        rows.append(rw)
+        init.append(((truth[rw] + r) % 6 + 1).astype(np.int32))
+    kw = dict(cycles=6, classifier=0, folds=5, stratified=0, seed=7, cv_seed=11, max_components=5,
+              fixed_components=5, evolutionary_search=1, guarded_diversity=1, many_to_one_absorption=1,
+              auto_class_coarsening=1)
+    batch = K.core_plslda_batch(x, rows, init, K.core_options(**kw))
+    for r in range(4):
+        one = K.core_plslda(x[rows[r]], init[r], K.core_options(**dict(kw, seed=7 + r, cv_seed=11 + r)))
+        assert np.array_equal(batch["clbest"][r], one["clbest"]), r
+        assert batch["accbest"][r] == one["accbest"], r
+        assert np.array_equal(batch["vect_acc"][r], one["vect_acc"]), r
