@@ -42,6 +42,7 @@ until at least --cpu-seconds of wall time.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--runs 100]
     python bench.py --copy-probe     # the 256 MiB copy that calibrates the counters
+    python bench.py --graph-rows 1000000   # exact 30-NN graph of 1M x 50 (own JSON line)
 """
 from __future__ import annotations
 
@@ -153,6 +154,24 @@ def copy_probe() -> int:
     return 0
 
 
+def graph_bench(rows: int, k: int) -> int:
+    """Exact self-kNN graph of the 20-cluster mixture (config-4 generator,
+    p = 50) through the split-BF16 MFMA path; one JSON line."""
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    x, _ = D.config4(n=rows)
+    K.KODAMA_knn_graph(x[:8192], k=k)  # warm-up (code objects, allocator)
+    t0 = time.perf_counter()
+    g = K.KODAMA_knn_graph(x, k=k)
+    wall = time.perf_counter() - t0
+    fb = K.last_knn_fallbacks()
+    flops = 2.0 * rows * rows * 50 * 3  # three bf16 products per MAC (hi.hi, hi.lo, lo.hi)
+    print(json.dumps({"graph_rows": rows, "k": k, "p": 50, "seconds": wall, "fallback_queries": fb,
+                      "candidate_bf16_tflops_per_s": flops / wall / 1e12,
+                      "neighbors": int(g["neighbors"])}), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
     """HBM bytes per launch from the committed counter summary, if it covers
     this kernel on this workload."""
@@ -184,9 +203,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
+    ap.add_argument("--graph-rows", type=int, default=0, help="time the exact k-NN graph of this many rows and exit")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
+    if args.graph_rows > 0:
+        return graph_bench(args.graph_rows, args.k)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported"}))
         return 2

@@ -21,20 +21,36 @@ uint64_t splitmix(uint64_t z) {
 }
 }  // namespace
 
+// All device state in one allocation (one hipMalloc instead of ten), laid
+// out so that the host-written part (starting labels, permutation widths, run
+// keys) is one upload and the exported part (best labels, state, history) one
+// download.
 struct DeviceEvolution::Impl {
   KnnBatch& batch;
   std::vector<IVec> values;
   int runs = 0, classes = 0, cycles_cap = 0;
-  DeviceBuffer<unsigned short> nbr16, cur, curpred, best, bestpred;
-  DeviceBuffer<int> flows, perm_bits;
-  DeviceBuffer<uint64_t> run_key;
-  DeviceBuffer<double> state, history;
+  DeviceBuffer<unsigned char> arena;
+  size_t up_begin = 0, up_bytes = 0, down_begin = 0, down_bytes = 0;
+  size_t off_best = 0, off_state = 0, off_history = 0;
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
+
+namespace {
+
+struct ArenaLayout {
+  size_t total = 0;
+  size_t take(size_t bytes) {
+    const size_t at = (total + 255) / 256 * 256;
+    total = at + bytes;
+    return at;
+  }
+};
+
+}  // namespace
 
 DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values, const std::vector<IVec>& codes,
                                  const std::vector<IVec>& folds, const kodama_hip_core_options& options) {
@@ -57,18 +73,33 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   }
   const size_t flow_items = static_cast<size_t>(m.runs) * m.classes * m.classes;
   const int pad = batch.pad_total();
+  const size_t label_bytes = sizeof(unsigned short) * static_cast<size_t>(pad);
   const hipStream_t s = batch.stream();
-  m.nbr16.resize(static_cast<size_t>(batch.k()) * pad);
-  for (auto* b : {&m.cur, &m.curpred, &m.best, &m.bestpred}) b->resize(static_cast<size_t>(pad));
-  m.flows.resize(flow_items);
-  m.perm_bits.resize(static_cast<size_t>(m.runs));
-  m.run_key.resize(static_cast<size_t>(m.runs));
-  m.state.resize(static_cast<size_t>(m.runs) * kEvoStateStride);
-  m.history.resize(static_cast<size_t>(m.runs) * m.cycles_cap);
 
-  std::vector<unsigned short> h_init(static_cast<size_t>(pad), 0);
-  std::vector<int> h_bits(static_cast<size_t>(m.runs));
-  std::vector<uint64_t> h_key(static_cast<size_t>(m.runs));
+  ArenaLayout lay;
+  const size_t off_nbr = lay.take(sizeof(unsigned short) * static_cast<size_t>(batch.k()) * pad);
+  const size_t off_flows = lay.take(sizeof(int) * flow_items);
+  const size_t off_curpred = lay.take(label_bytes);
+  const size_t off_cur = lay.take(label_bytes);  // upload: cur .. run_key
+  const size_t off_bits = lay.take(sizeof(int) * m.runs);
+  const size_t off_key = lay.take(sizeof(uint64_t) * m.runs);
+  const size_t off_bestpred = lay.take(label_bytes);
+  m.off_best = lay.take(label_bytes);  // download: best .. history
+  m.off_state = lay.take(sizeof(double) * m.runs * kEvoStateStride);
+  m.off_history = lay.take(sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap);
+  m.up_begin = off_cur;
+  m.up_bytes = off_key + sizeof(uint64_t) * m.runs - off_cur;
+  m.down_begin = m.off_best;
+  m.down_bytes = lay.total - m.off_best;
+  m.arena.resize(lay.total);
+  unsigned char* base = m.arena.get();
+
+  // Host image of the upload region: starting labels (their predictions start
+  // equal to them and are rewritten by the first evaluation), widths, keys.
+  std::vector<unsigned char> up(m.up_bytes, 0);
+  auto* h_init = reinterpret_cast<unsigned short*>(up.data());
+  auto* h_bits = reinterpret_cast<int*>(up.data() + (off_bits - off_cur));
+  auto* h_key = reinterpret_cast<uint64_t*>(up.data() + (off_key - off_cur));
   for (int r = 0; r < m.runs; ++r) {
     for (size_t i = 0; i < codes[r].size(); ++i) {
       h_init[batch.pad_offset(r) + i] = static_cast<unsigned short>(codes[r][i]);
@@ -79,13 +110,12 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     h_bits[r] = bits;
     h_key[r] = splitmix(options.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
   }
-  m.cur.upload(h_init.data(), h_init.size(), s);
-  m.curpred.upload(h_init.data(), h_init.size(), s);
-  m.perm_bits.upload(h_bits.data(), h_bits.size(), s);
-  m.run_key.upload(h_key.data(), h_key.size(), s);
-  check(launch_evo_zero(m.flows.get(), m.flows.size(), s), "evo_zero");
+  check(hipMemcpyAsync(base + m.up_begin, up.data(), m.up_bytes, hipMemcpyHostToDevice, s), "upload");
+  check(hipMemcpyAsync(base + off_curpred, base + off_cur, label_bytes, hipMemcpyDeviceToDevice, s), "copy");
+  check(launch_evo_zero(reinterpret_cast<int*>(base + off_flows), flow_items, s), "evo_zero");
   check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
-                          batch.d_block_run().get(), m.nbr16.get(), m.nbr16.size(), pad, s),
+                          batch.d_block_run().get(), reinterpret_cast<unsigned short*>(base + off_nbr),
+                          static_cast<size_t>(batch.k()) * pad, pad, s),
         "evo_narrow");
 
   EvoParams& p = m.prm;
@@ -98,18 +128,18 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.pad_total = pad;
   p.run_offsets = batch.d_run_offsets().get();
   p.pad_offsets = batch.d_pad_offsets().get();
-  p.neighbors = m.nbr16.get();
+  p.neighbors = reinterpret_cast<const unsigned short*>(base + off_nbr);
   p.scores = batch.d_scores().get();
-  p.run_key = m.run_key.get();
-  p.perm_bits = m.perm_bits.get();
-  p.cur = m.cur.get();
-  p.curpred = m.curpred.get();
-  p.best = m.best.get();
-  p.bestpred = m.bestpred.get();
-  p.state = m.state.get();
-  p.flows = m.flows.get();
-  p.history = m.history.get();
-  check(hipStreamSynchronize(s), "setup sync");
+  p.run_key = reinterpret_cast<const uint64_t*>(base + off_key);
+  p.perm_bits = reinterpret_cast<const int*>(base + off_bits);
+  p.cur = reinterpret_cast<unsigned short*>(base + off_cur);
+  p.curpred = reinterpret_cast<unsigned short*>(base + off_curpred);
+  p.best = reinterpret_cast<unsigned short*>(base + m.off_best);
+  p.bestpred = reinterpret_cast<unsigned short*>(base + off_bestpred);
+  p.state = reinterpret_cast<double*>(base + m.off_state);
+  p.flows = reinterpret_cast<int*>(base + off_flows);
+  p.history = reinterpret_cast<double*>(base + m.off_history);
+  check(hipStreamSynchronize(s), "setup sync");  // `up` is pageable host memory
   impl_ = impl.release();
 }
 
@@ -135,17 +165,20 @@ void DeviceEvolution::run(int cycles) {
 void DeviceEvolution::export_results(int* clbest, double* accbest, double* vect_acc) {
   Impl& m = *impl_;
   const hipStream_t s = m.batch.stream();
-  std::vector<unsigned short> best(static_cast<size_t>(m.batch.pad_total()));
-  std::vector<double> state(static_cast<size_t>(m.runs) * kEvoStateStride);
-  std::vector<double> hist(static_cast<size_t>(m.runs) * m.cycles_cap);
-  m.best.download(best.data(), best.size(), s);
-  m.state.download(state.data(), state.size(), s);
-  m.history.download(hist.data(), hist.size(), s);
+  std::vector<unsigned char> down(m.down_bytes);
+  check(hipMemcpyAsync(down.data(), m.arena.get() + m.down_begin, m.down_bytes, hipMemcpyDeviceToHost, s),
+        "download");
   check(hipStreamSynchronize(s), "export sync");
+  const auto* best = reinterpret_cast<const unsigned short*>(down.data() + (m.off_best - m.down_begin));
+  const auto* state = reinterpret_cast<const double*>(down.data() + (m.off_state - m.down_begin));
+  const auto* hist = reinterpret_cast<const double*>(down.data() + (m.off_history - m.down_begin));
   const IVec& off = m.batch.run_offsets();
   for (int r = 0; r < m.runs; ++r) {
     const int L = off[r + 1] - off[r];
-    for (int i = 0; i < L; ++i) clbest[off[r] + i] = m.values[r][best[m.batch.pad_offset(r) + i]];
+    const IVec& v = m.values[r];
+    const unsigned short* b = best + m.batch.pad_offset(r);
+    int* out = clbest + off[r];
+    for (int i = 0; i < L; ++i) out[i] = v[b[i]];
     accbest[r] = state[static_cast<size_t>(r) * kEvoStateStride + 2];
     if (vect_acc) {
       for (int c = 0; c < m.done_cycles; ++c) vect_acc[static_cast<size_t>(r) * m.done_cycles + c] =

@@ -56,6 +56,21 @@ __device__ __forceinline__ uint64_t draw(uint64_t key, uint64_t cycle, uint64_t 
   return mix64(key ^ mix64((cycle << 8) ^ stream));
 }
 
+// The thread index, read anew at the top of every cycle. Values derived from
+// threadIdx.x (row addresses, bounds masks) are loop-invariant across the cycle
+// loop; hoisted out of it they outlive the 128-register budget of a
+// 1024-thread workgroup and spill to scratch. From this opaque copy they are
+// recomputed each cycle instead, for a few instructions.
+__device__ __forceinline__ int opaque_tid() {
+  int t;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(static_cast<int>(threadIdx.x)));
+  return t;
+}
+
+// exp of the acceptance test, out of line: one lane calls it once per cycle,
+// and inlined its polynomial constants would be hoisted into registers.
+__device__ __noinline__ double accept_exp(double x) { return exp(x); }
+
 __device__ __forceinline__ double unit(uint64_t h) { return static_cast<double>(h >> 11) * 0x1.0p-53; }
 
 __device__ __forceinline__ uint32_t round_fn(uint32_t r, uint32_t key) {
@@ -225,7 +240,7 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
   int valid = 0;
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
-    const uint32_t c = j < k ? lab[nbr[static_cast<size_t>(j) * pad_total + i]] : kSentinel;
+    const uint32_t c = j < k ? lab[nbr[static_cast<uint32_t>(j * pad_total + i)]] : kSentinel;
     valid += c != kSentinel ? 1 : 0;
     key[j] = (c << 6) | static_cast<uint32_t>(j);
   }
@@ -246,23 +261,30 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
     }
   }
   if (!tie) return best_code;
-  // Ties are rare: the scores are read only here, all loads issued before the
-  // first sum (missing neighbours read rank 0 and are not summed).
-  float score[KMAX];
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const uint32_t rank = j < valid ? key[j] & 63u : 0u;
-    score[j] = scores[static_cast<size_t>(rank) * pad_total + i];
-  }
+  // Ties are rare: the scores are read only here, in groups of eight loads
+  // issued before their sums (missing neighbours read rank 0 and are not
+  // summed). Element offsets fit 32 bits (k * pad_total < 2^31, checked at
+  // launch), so each load is a scalar base plus one offset register.
+  constexpr int kGroup = KMAX < 8 ? KMAX : 8;
   double best_sum = -INFINITY, sum = 0.0;
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const bool fresh = j == 0 || (key[j] >> 6) != (key[j - 1] >> 6);
-    run = fresh ? 1 : run + 1;
-    sum = (fresh ? 0.0 : sum) + static_cast<double>(score[j]);
-    if (j < valid && run == best_count && sum > best_sum) {
-      best_sum = sum;
-      best_code = static_cast<int>(key[j] >> 6);
+  for (int j0 = 0; j0 < KMAX; j0 += kGroup) {
+    float score[kGroup];
+#pragma unroll
+    for (int t = 0; t < kGroup; ++t) {
+      const uint32_t rank = j0 + t < valid ? key[j0 + t] & 63u : 0u;
+      score[t] = scores[static_cast<uint32_t>(static_cast<int>(rank) * pad_total + i)];
+    }
+#pragma unroll
+    for (int t = 0; t < kGroup; ++t) {
+      const int j = j0 + t;
+      const bool fresh = j == 0 || (key[j] >> 6) != (key[j - 1] >> 6);
+      run = fresh ? 1 : run + 1;
+      sum = (fresh ? 0.0 : sum) + static_cast<double>(score[t]);
+      if (j < valid && run == best_count && sum > best_sum) {
+        best_sum = sum;
+        best_code = static_cast<int>(key[j] >> 6);
+      }
     }
   }
   return best_code;
@@ -284,20 +306,21 @@ __device__ __forceinline__ void load_pair(const uint32_t* __restrict__ nbr2, int
 // label, so this implies a strict majority of the present neighbours); the
 // rest are queued for the full vote.
 template <int KMAX, int THREADS>
-__device__ __forceinline__ void vote_phase(int k, int pad_total, const Lds& s, const unsigned short* __restrict__ nbr,
+__device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const Lds& s,
+                                           const unsigned short* __restrict__ nbr,
                                            const float* __restrict__ scores, const unsigned short* lab,
                                            unsigned short* pred, int n, bool count_labels) {
   const uint32_t* __restrict__ nbr2 = reinterpret_cast<const uint32_t*>(nbr);
   const int pad2 = pad_total >> 1;
   const int npairs = (n + 1) >> 1;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = tid & (kWave - 1);
   const uint64_t below = (1ull << lane) - 1ull;
   int matches = 0;
   // Uniform trip count: every lane reaches the ballots below. Latency is
   // hidden by the workgroup's 16 waves rather than by prefetching (which would
   // cost KMAX more registers per thread).
   for (int m0 = 0; m0 < npairs; m0 += THREADS) {
-    const int m = m0 + threadIdx.x;
+    const int m = m0 + tid;
     uint32_t ids[KMAX];
     if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids);
 #pragma unroll
@@ -331,7 +354,7 @@ __device__ __forceinline__ void vote_phase(int k, int pad_total, const Lds& s, c
   }
   __syncthreads();
   const int qn = s.scal[kQueue];
-  for (int q = threadIdx.x; q < qn; q += THREADS) {
+  for (int q = tid; q < qn; q += THREADS) {
     const int i = s.queue[q];
     const int own = lab[i];
     int best = full_vote<KMAX>(k, pad_total, nbr, scores, lab, i);
@@ -420,6 +443,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   // scores become the current and the best state.
   for (int step = p.init ? -1 : 0; step < p.cycle_count; ++step) {
     const bool evaluate = step < 0;
+    const int tid = opaque_tid();
     const int cycle = p.cycle_begin + step;
     const int cur = s.scal[kCur];
     const unsigned short* cur_lab = s.lab(cur);
@@ -435,11 +459,11 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // The rows at positions [0, take) of a keyed permutation are chosen:
         // marked in the proposal buffer, which holds only label codes otherwise.
         const Permutation perm(draw(key, cycle, 2), bits);
-        for (int y = threadIdx.x; y < take; y += THREADS) {
+        for (int y = tid; y < take; y += THREADS) {
           prop[perm.preimage(static_cast<uint32_t>(y), static_cast<uint32_t>(n))] = kChosenMark;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += THREADS) {
+        for (int i = tid; i < n; i += THREADS) {
           const bool chosen = prop[i] == kChosenMark;
           const int guide = cur_pred[i];
           const int label = chosen ? guide : static_cast<int>(cur_lab[i]);
@@ -453,7 +477,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         int fkey[kRowsPerThread], old[kRowsPerThread];
 #pragma unroll
         for (int r = 0; r < kRowsPerThread; ++r) {
-          const int i = threadIdx.x + r * THREADS;
+          const int i = tid + r * THREADS;
           fkey[r] = i < n ? s.queue[i] : -1;
           old[r] = fkey[r] >= 0 ? atomicAdd(&flows[fkey[r]], 1) : 1;
         }
@@ -490,7 +514,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // candidate comparison -- surplus (exactly, as cnt*n - size_src*size_dst),
         // then flow, then target size, then the smaller label.
         const int np = s.scal[kPairs];
-        for (int q = threadIdx.x; q < np; q += THREADS) {
+        for (int q = tid; q < np; q += THREADS) {
           const int idx = s.pairs[q];
           // Read and clear in one L2 atomic: the propose-phase atomics of this
           // launch bypass the vector L1, which may hold an older copy.
@@ -511,7 +535,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       __syncthreads();
       if (eligible) {
         const double nn = fmax(1.0, static_cast<double>(n));
-        for (int e = threadIdx.x; e < present; e += THREADS) {
+        for (int e = tid; e < present; e += THREADS) {
           const int src = s.list[e];
           const unsigned long long k64 = s.best_key[src];
           if (k64 == 0ull) continue;
@@ -529,10 +553,10 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // (core.cpp:1188-1202); one is drawn in ascending label order with weights
         // max(surplus, DBL_MIN), through a block-wide prefix sum.
         int local_multi = 0;
-        for (int e = threadIdx.x; e < present; e += THREADS) local_multi |= s.nsrc[s.list[e]] > 1 ? 1 : 0;
+        for (int e = tid; e < present; e += THREADS) local_multi |= s.nsrc[s.list[e]] > 1 ? 1 : 0;
         const bool multi = __syncthreads_or(local_multi) != 0;
         const int per = (present + THREADS - 1) / THREADS;
-        const int b0 = min(present, static_cast<int>(threadIdx.x) * per), b1 = min(present, b0 + per);
+        const int b0 = min(present, static_cast<int>(tid) * per), b1 = min(present, b0 + per);
         double part = 0.0;
         for (int e = b0; e < b1; ++e) {
           const int t = s.list[e];
@@ -565,7 +589,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             acc += w;
           }
           __syncthreads();
-          if (threadIdx.x == 0) {
+          if (tid == 0) {
             int e = s.scal[kPick];
             if (e == INT_MAX) {
               // u at or past the last boundary: the last eligible target, as the
@@ -580,7 +604,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         const int tgt = s.scal[kChosen];
         if (tgt >= 0) {
           int v = 0;
-          for (int e = threadIdx.x; e < present; e += THREADS) v += s.point[s.list[e]] == tgt ? 1 : 0;
+          for (int e = tid; e < present; e += THREADS) v += s.point[s.list[e]] == tgt ? 1 : 0;
           if (v) atomicAdd(&s.scal[kVanished], v);
         }
         __syncthreads();
@@ -588,12 +612,12 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // must keep two classes or more and lose at least one (core.cpp:1219-1223).
         const int vanished = s.scal[kVanished];
         if (tgt >= 0 && vanished > 0 && present - vanished > 1) {
-          for (int e = threadIdx.x; e < present; e += THREADS) {
+          for (int e = tid; e < present; e += THREADS) {
             const int c = s.list[e];
             if (s.point[c] == tgt) s.remap[c] = tgt;
           }
           __syncthreads();
-          for (int i = threadIdx.x; i < n; i += THREADS) {
+          for (int i = tid; i < n; i += THREADS) {
             const int r = s.remap[prop[i]];
             if (r >= 0) prop[i] = static_cast<unsigned short>(r);
           }
@@ -601,13 +625,13 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       }
       // Class counts of the proposal after absorption, from the sizes; then
       // reset this cycle's class tables (present classes only).
-      for (int e = threadIdx.x; e < present; e += THREADS) {
+      for (int e = tid; e < present; e += THREADS) {
         const int c = s.list[e];
         const int r = s.remap[c];
         atomicAdd(&s.counts[r >= 0 ? r : c], s.sizes[c]);
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < present; e += THREADS) {
+      for (int e = tid; e < present; e += THREADS) {
         const int c = s.list[e];
         s.weight[c] = 0ull;
         s.best_key[c] = 0ull;
@@ -617,12 +641,12 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         s.point[c] = -1;
         s.remap[c] = -1;
       }
-      if (threadIdx.x == 0) s.scal[kPairs] = 0;
+      if (tid == 0) s.scal[kPairs] = 0;
       __syncthreads();
     }
 
     // -- vote -------------------------------------------------------------------
-    vote_phase<KMAX, THREADS>(k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate);
+    vote_phase<KMAX, THREADS>(tid, k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate);
 
     // -- accept -----------------------------------------------------------------
     if (wave == 0) {
@@ -646,7 +670,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           const double cooling = 1.0 - static_cast<double>(cycle + 1) / static_cast<double>(max(1, p.cycles));
           const double temperature = fmax(1.0e-9, 0.10 * fmax(0.0, 1.0 - cur_acc) * cooling);
           bool take = score >= cur_score;
-          if (!take && temperature > 1.0e-9) take = unit(draw(key, cycle, 4)) < exp((score - cur_score) / temperature);
+          if (!take && temperature > 1.0e-9) {
+            take = unit(draw(key, cycle, 4)) < accept_exp((score - cur_score) / temperature);
+          }
           if (take) {
             s.state[0] = acc;
             s.state[1] = score;
@@ -660,14 +686,14 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     __syncthreads();
     const int dec = s.scal[kDecision];
     if (dec & kEvoImproved) {
-      for (int i = threadIdx.x; i < n; i += THREADS) {
+      for (int i = tid; i < n; i += THREADS) {
         p.best[base + i] = prop[i];
         p.bestpred[base + i] = prop_pred[i];
       }
     }
-    for (int c = threadIdx.x; c < C; c += THREADS) s.counts[c] = 0;
+    for (int c = tid; c < C; c += THREADS) s.counts[c] = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       if (dec & kEvoAccepted) s.scal[kCur] = cur ^ 1;
       s.scal[kMatches] = 0;
       s.scal[kQueue] = 0;
@@ -727,7 +753,7 @@ static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStrea
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s) {
   const size_t shmem = evo_lds_bytes(p.classes, p.max_rows);
   if (shmem > kEvoMaxLds || p.max_rows > kEvoMaxRows || p.classes > 1024 || p.k < 1 || p.k > 64 ||
-      p.pad_total % 256 != 0) {
+      p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31)) {
     return hipErrorInvalidValue;
   }
   if (p.k <= 8) return launch_persistent_k<8>(p, shmem, s);

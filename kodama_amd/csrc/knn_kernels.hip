@@ -125,8 +125,8 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
   if (MODE == 0) {
     q_begin = 0;
     q_count = prm.query_rows;
-    t_begin = 0;
-    t_count = prm.train_rows;
+    t_begin = prm.slice_rows > 0 ? static_cast<int>(blockIdx.y) * prm.slice_rows : 0;
+    t_count = prm.slice_rows > 0 ? min(prm.slice_rows, prm.train_rows - t_begin) : prm.train_rows;
   } else {
     q_begin = prm.run_offsets[run];
     q_count = prm.run_offsets[run + 1] - q_begin;
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
       const int d = e - r * p;
       float v = 0.0f;
       if (r < nt) {
-        const int trow = MODE == 0 ? base + r : prm.rows[t_begin + base + r];
+        const int trow = MODE == 0 ? t_begin + base + r : prm.rows[t_begin + base + r];
         v = prm.train[static_cast<size_t>(trow) * p + d];
       }
       tile[r * PMAX + d] = v;
@@ -181,7 +181,8 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     __syncthreads();
     if (active) {
       for (int r = 0; r < nt; ++r) {
-        const int cand = base + r;
+        // Global train index in MODE 0, run-local row in MODE 1.
+        const int cand = MODE == 0 ? t_begin + base + r : base + r;
         if (MODE == 1) {
           if (tile_fold[r] == my_fold) continue;
         } else {
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
       if (j < k) {
-        const size_t o = static_cast<size_t>(qi) * k + j;
+        const size_t o = (static_cast<size_t>(blockIdx.y) * prm.query_rows + qi) * k + j;
         prm.out_index[o] = ki[j];
         prm.out_dist[o] = ki[j] >= 0 ? kd[j] : INFINITY;
       }
@@ -431,6 +432,53 @@ __global__ void vote_reset_kernel(int* counts, int* firsts, int* matches, int ru
   if (i < runs) matches[i] = 0;
 }
 
+// Merge of per-slice top-k lists, one wave per query: lane s holds the head of
+// list s; each round the wave takes the smallest (distance, index) key over the
+// heads and the owning lane advances. Keys are unique (every train index lies
+// in one slice), so exactly one lane wins each round.
+__global__ void __launch_bounds__(kBlock) knn_merge_slices_kernel(const int* __restrict__ idx,
+                                                                  const float* __restrict__ dist, int slices,
+                                                                  int query_rows, int k, int* __restrict__ out_index,
+                                                                  float* __restrict__ out_dist) {
+  const int q = blockIdx.x * (kBlock / 64) + static_cast<int>(threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (q >= query_rows) return;  // whole waves; no block barrier below
+  const size_t stride = static_cast<size_t>(query_rows) * k;
+  const size_t row = static_cast<size_t>(q) * k;
+  auto key_at = [&](int h) -> uint64_t {
+    if (lane >= slices || h >= k) return ~0ull;
+    const size_t o = static_cast<size_t>(lane) * stride + row + h;
+    const int i = idx[o];
+    if (i < 0) return ~0ull;
+    // Order-preserving map of the float onto unsigned bits, index below it.
+    const uint32_t b = __float_as_uint(dist[o]);
+    const uint32_t ord = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return (static_cast<uint64_t>(ord) << 32) | static_cast<uint32_t>(i);
+  };
+  int head = 0;
+  uint64_t mine = key_at(0);
+  for (int j = 0; j < k; ++j) {
+    uint64_t best = mine;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint64_t other = __shfl_xor(best, m);
+      best = other < best ? other : best;
+    }
+    if (lane == 0) {
+      if (best == ~0ull) {
+        out_index[row + j] = -1;
+        out_dist[row + j] = INFINITY;
+      } else {
+        const uint32_t ord = static_cast<uint32_t>(best >> 32);
+        const uint32_t b = (ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord;
+        out_index[row + j] = static_cast<int>(static_cast<uint32_t>(best));
+        out_dist[row + j] = __uint_as_float(b);
+      }
+    }
+    if (best != ~0ull && mine == best) mine = key_at(++head);
+  }
+}
+
 // ---- host launchers --------------------------------------------------------
 
 template <int PMAX, int KMAX>
@@ -473,6 +521,16 @@ static void launch_vote_k(const VoteParams& prm, dim3 grid, size_t shmem, hipStr
   else if (prm.k <= 16) hipLaunchKernelGGL((vote_kernel<16, STAGE>), grid, block, shmem, stream, prm);
   else if (prm.k <= 32) hipLaunchKernelGGL((vote_kernel<32, STAGE>), grid, block, shmem, stream, prm);
   else hipLaunchKernelGGL((vote_kernel<64, STAGE>), grid, block, shmem, stream, prm);
+}
+
+hipError_t launch_knn_merge_slices(const int* idx, const float* dist, int slices, int query_rows, int k,
+                                   int* out_index, float* out_dist, hipStream_t stream) {
+  if (slices < 1 || slices > 64 || k < 1 || k > kMaxK) return hipErrorInvalidValue;
+  if (query_rows == 0) return hipSuccess;
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(knn_merge_slices_kernel, dim3((query_rows + per_block - 1) / per_block), dim3(kBlock), 0, stream,
+                     idx, dist, slices, query_rows, k, out_index, out_dist);
+  return hipGetLastError();
 }
 
 hipError_t launch_vote(const VoteParams& prm, int blocks, hipStream_t stream) {

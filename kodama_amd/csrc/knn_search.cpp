@@ -1,6 +1,7 @@
 // Host driver of the exact kNN on the device: the split-BF16 MFMA candidates
 // with the exact rerank where they apply, the register scan otherwise and for
 // the queries the rerank flags.
+#include <algorithm>
 #include <vector>
 
 #include "../../include/kodama_hip.h"
@@ -10,11 +11,23 @@
 
 namespace kodama_hip {
 
+namespace {
+
+constexpr int kScanBlock = 256;          // queries per register-scan block
+constexpr int kScanTargetBlocks = 512;   // two blocks per CU
+constexpr int kMaxSlices = 64;           // lists one merge wave combines
+constexpr int kSliceAlign = 64;          // LDS tile rows of the scan
+
+}  // namespace
+
 void device_exact_knn(const float* d_train, int train_rows, const float* d_query, int query_rows, int dims, int k,
                       int metric, const int* d_exclude, int* d_idx, float* d_dist, float train_norm2_max,
                       hipStream_t s, int* mfma_fallbacks) {
   const bool euclid = metric == KODAMA_HIP_EUCLIDEAN;
   if (mfma_fallbacks) *mfma_fallbacks = -1;  // -1: the MFMA path was not used
+  // Register scan. Few queries (one block per 256) would leave most CUs idle,
+  // so the train rows are cut into slices scanned by separate blocks, whose
+  // lists are merged after: about 512 blocks in all, at most 64 slices.
   auto scan = [&](const float* q, int rows, const int* ex, int* idx, float* dist) {
     KnnTopkParams prm;
     prm.train = d_train;
@@ -26,7 +39,24 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
     prm.exclude = ex;
     prm.out_index = idx;
     prm.out_dist = dist;
-    check(launch_knn_topk(prm, 0, euclid, rows, 1, s), "knn_topk launch");
+    const int gx = (rows + kScanBlock - 1) / kScanBlock;
+    int slices = std::min(kMaxSlices, std::max(1, (kScanTargetBlocks + gx - 1) / gx));
+    int slice_rows = (train_rows + slices - 1) / slices;
+    slice_rows = std::max(kSliceAlign, (slice_rows + kSliceAlign - 1) / kSliceAlign * kSliceAlign);
+    slices = (train_rows + slice_rows - 1) / slice_rows;
+    if (slices <= 1) {
+      check(launch_knn_topk(prm, 0, euclid, rows, 1, s), "knn_topk launch");
+      return;
+    }
+    DeviceBuffer<int> part_idx(static_cast<size_t>(slices) * rows * k);
+    DeviceBuffer<float> part_dist(part_idx.size());
+    prm.slice_rows = slice_rows;
+    prm.out_index = part_idx.get();
+    prm.out_dist = part_dist.get();
+    check(launch_knn_topk(prm, 0, euclid, rows, slices, s), "knn_topk launch");
+    check(launch_knn_merge_slices(part_idx.get(), part_dist.get(), slices, rows, k, idx, dist, s),
+          "knn_merge_slices launch");
+    check(hipStreamSynchronize(s), "knn scan sync");  // the partial lists are freed on return
   };
   if (dims > 64 || k > kMfmaMaxK || train_rows < kMfmaMinTrain || query_rows == 0) {
     scan(d_query, query_rows, d_exclude, d_idx, d_dist);

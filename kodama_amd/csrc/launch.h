@@ -13,7 +13,10 @@ constexpr int kMaxK = 64;       // neighbours kept per row by the register top-k
 constexpr int kMaxDims = 256;   // feature dimensions of the register query path
 
 // Exact kNN. MODE 0 (query vs train): query/train row-major, out_index/out_dist
-// [query_rows x k], internal distances. MODE 1 (per-fold tables of a batch of
+// [query_rows x k], internal distances; with slice_rows > 0 block row y scans
+// train rows [y * slice_rows, (y + 1) * slice_rows) into the lists
+// [y][query_rows x k] (global train indices), which launch_knn_merge_slices
+// combines. MODE 1 (per-fold tables of a batch of
 // runs): rows[] are the data rows of each run, concatenated with run_offsets;
 // a row's candidates are the rows of its run in another fold; output goes to the
 // transposed padded vote layout [k x pad_total] as padded row ids and scores.
@@ -25,6 +28,7 @@ struct KnnTopkParams {
   int dims = 0;
   int k = 0;
   const int* exclude = nullptr;      // MODE 0, per query, may be null
+  int slice_rows = 0;                // MODE 0, train rows per block row (0: all)
   const int* rows = nullptr;         // MODE 1
   const int* folds = nullptr;        // MODE 1, per concatenated row
   const int* run_offsets = nullptr;  // MODE 1, [runs + 1]
@@ -65,6 +69,10 @@ struct VoteParams {
 
 hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,
                            hipStream_t stream);
+// The k smallest by (distance, index) of `slices` lists [slices][query_rows x k]
+// (each ascending, -1 = empty) -> [query_rows x k], -1 / +inf past the end.
+hipError_t launch_knn_merge_slices(const int* idx, const float* dist, int slices, int query_rows, int k,
+                                   int* out_index, float* out_dist, hipStream_t stream);
 hipError_t launch_vote(const VoteParams& prm, int blocks, hipStream_t stream);
 hipError_t launch_vote_reset(int* counts, int* firsts, int* matches, int runs, int classes,
                              hipStream_t stream);

@@ -55,6 +55,22 @@ def test_exact_knn_exclusion_and_padding(oracle):
     assert not np.any(idx[:, :19] == np.arange(20)[:, None])
 
 
+@pytest.mark.parametrize("metric", ["euclidean", "inner_product"])
+def test_sliced_scan_merge_with_ties_across_slices(oracle, metric):
+    """Few queries against many train rows: the scan cuts the train rows into
+    slices and merges their lists. 40 distinct rows repeated 75 times, copies
+    spread over every slice, so equal distances meet in the merge and only the
+    index breaks the tie; k = 64 fills each slice list."""
+    rng = np.random.default_rng(11)
+    base = rng.normal(size=(40, 90)).astype(np.float32)
+    train = np.tile(base, (75, 1))
+    query = np.concatenate([base[:20], rng.normal(size=(13, 90)).astype(np.float32)])
+    ex = (np.arange(33, dtype=np.int32) * 37) % train.shape[0]
+    idx, dist = K.exact_knn_search(train, query, 64, metric, exclude=ex)
+    oi, od = oracle.exact_knn(train, query, 64, K._native.METRICS[metric], exclude=ex)
+    assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+
+
 def test_vote_graph_matches_oracle_with_ties(oracle):
     rng = np.random.default_rng(3)
     n, k = 900, 12
