@@ -172,6 +172,25 @@ def graph_bench(rows: int, k: int) -> int:
     return 0
 
 
+def matrix_bench(rows: int, runs: int, cycles: int, k: int, forms) -> int:
+    """End-to-end KODAMA.matrix (kNN) of the 20-cluster mixture (config-3
+    generator, p = 50): L = 3000 landmarks per run, splitting 300, global graph
+    of 100 neighbours; one JSON line per form with the stage timings."""
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    x, _ = D.config3(n=rows)
+    K.KODAMA_matrix(x[:4000], M=2, Tcycle=2, landmarks=1000, k=k)  # warm-up (code objects, allocator)
+    for form in forms:
+        t0 = time.perf_counter()
+        out = K.KODAMA_matrix(x, M=runs, Tcycle=cycles, landmarks=3000, splitting=300, folds=5, k=k,
+                              graph_neighbors=100, seed=1234, form=form)
+        wall = time.perf_counter() - t0
+        print(json.dumps({"matrix_rows": rows, "runs": runs, "cycles": cycles, "k": k, "form": form,
+                          "seconds": wall, "mean_acc": float(np.mean(out["acc"])),
+                          "stage_seconds": out["timings"]}), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
     """HBM bytes per launch from the committed counter summary, if it covers
     this kernel on this workload."""
@@ -204,11 +223,15 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
     ap.add_argument("--graph-rows", type=int, default=0, help="time the exact k-NN graph of this many rows and exit")
+    ap.add_argument("--matrix-rows", type=int, default=0,
+                    help="time end-to-end KODAMA.matrix on this many rows (runs, steps, forms apply) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
     if args.graph_rows > 0:
         return graph_bench(args.graph_rows, args.k)
+    if args.matrix_rows > 0:
+        return matrix_bench(args.matrix_rows, args.runs, args.steps, args.k, [f for f in args.forms.split(",") if f])
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported"}))
         return 2

@@ -120,7 +120,8 @@ int kodama_hip_last_knn_fallbacks(void);
  * smaller train index, -1 / +inf beyond min(k, available). query_train_indices
  * (may be NULL) excludes one train index per query. Distances are summed in the
  * reference CPU kernel's order (native_knn.cpp:41-95), so they are bit-identical
- * to the reference's exhaustive search. */
+ * to the reference's exhaustive search. k <= 128 up to 128 dimensions, k <= 64
+ * beyond. */
 int kodama_hip_exact_knn_search(const float* train, int32_t train_rows, const float* query,
                                 int32_t query_rows, int32_t dimensions, int32_t k,
                                 int32_t metric, int32_t device,
@@ -212,7 +213,8 @@ int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const in
 
 /* kodama::KODAMAKNNGraph_CUDA (kodama.hpp:1524-1527; graph_cluster.cpp:790-837):
  * exact self-kNN graph, k = min(k, n-1), self excluded; 1-based ids and public
- * distances (sqrt of squared L2 for Euclidean). Output [n x min(k, n-1)]. */
+ * distances (sqrt of squared L2 for Euclidean). Output [n x min(k, n-1)];
+ * min(k, n-1) <= 128 up to 128 dimensions, <= 64 beyond. */
 int kodama_hip_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_t metric,
                          int32_t device, int32_t* indices_one_based, float* distances);
 
@@ -248,6 +250,49 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
                               const kodama_hip_core_options* options, int32_t mode,
                               int32_t* clbest, double* accbest, double* vect_acc,
                               double* timings);
+
+/* ---- KODAMA.matrix orchestration -------------------------------------------
+ * Flat subset of kodama::KODAMAMatrixOptions (kodama.hpp:532-566) for data
+ * without spatial coordinates, constraints or starting labels. */
+typedef struct kodama_hip_matrix_options {
+  int32_t runs;             /* M */
+  int32_t cycles;           /* Tcycle */
+  int32_t landmarks;        /* <= 0: 10000; >= n: ceil(0.75 n); then clamped to [2, n-1] */
+  int32_t splitting;        /* k-means classes; <= 0: 100 below 40000 rows, else 300 */
+  int32_t folds;            /* CV folds of every Core run (reference default 5) */
+  int32_t k;                /* knn.k: Core neighbours and projection votes, 1..64 */
+  int32_t metric;           /* KODAMA_HIP_COSINE / _INNER_PRODUCT / _EUCLIDEAN */
+  int32_t graph_neighbors;  /* global graph width; <= 0: 100 */
+  uint64_t seed;            /* KODAMAMatrixOptions::seed (reference default 1234) */
+  int32_t form;             /* 0 = plain (reference streams per run), 1 = device-resident */
+  int32_t device;
+  int32_t n_threads;        /* host threads of the plain form (0 = automatic) */
+} kodama_hip_matrix_options;
+
+/* Landmarks per run and global graph width that kodama_hip_kodama_matrix
+ * resolves for n rows (the sizes of its optional outputs). */
+int kodama_hip_matrix_sizes(int32_t n, const kodama_hip_matrix_options* options, int32_t* landmarks,
+                            int32_t* graph_neighbors);
+
+/* kodama::KODAMAMatrix_CUDA with classifier kNN (kodama.hpp:1077-1083;
+ * run_kodama_matrix, kodama_matrix.cpp:3324-4051; run_iteration, 2870-3322).
+ * A shared k-means atlas of all rows (mt19937_64(seed + 1)); for run r = 1..M
+ * an mt19937_64(seed + r) that skips one draw, quota-samples L landmarks over
+ * the atlas strata and draws k-means starting labels on them; CoreKNN on the
+ * landmarks (per-fold exact neighbours, non-stratified folds, seed + r); every
+ * other row takes the majority label of the first k landmarks among its global
+ * graph neighbours (exact self-kNN graph). The k-means runs on the device in
+ * the reference's float order, so atlas, landmarks and starting labels equal
+ * the reference CPU build's.
+ * Outputs: res [runs x n] labels, acc [runs] (accbest), v [runs x cycles]
+ * (may be NULL), landmark_rows [runs x L] (may be NULL), the global graph
+ * [n x graph_neighbors] 1-based with public distances (both NULL to skip),
+ * timings[6] (may be NULL): atlas, graph, landmarks + starting labels, Core
+ * precompute, Core evolution, projection (seconds). */
+int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p,
+                             const kodama_hip_matrix_options* options, int32_t* res, double* acc,
+                             double* v, int32_t* landmark_rows, int32_t* graph_indices_one_based,
+                             float* graph_distances, double* timings);
 
 #ifdef __cplusplus
 }

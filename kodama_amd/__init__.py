@@ -271,6 +271,48 @@ def core_plslda_batch(data, run_rows, initial_labels, options: CoreOptions, fold
                 steps=int(timings[2]), eval_kernel_seconds=float(timings[3]))
 
 
+# ---- KODAMA.matrix ----------------------------------------------------------------
+
+def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5, k=30,
+                  metric="euclidean", graph_neighbors=100, seed=1234, form="plain", gpu_device=0,
+                  n_threads=0, dissimilarity=False) -> dict:
+    """KODAMA.matrix with the kNN classifier on the HIP backend
+    (KODAMAMatrix_CUDA contract; kodama_matrix.cpp:3324-4051): shared k-means
+    landmark atlas, per-run quota landmarks and k-means starting labels, CoreKNN
+    on the landmarks for all M runs in one batch (form "plain" replays the
+    reference streams, "device" keeps the evolution on the GPU), projection of
+    the other rows through the exact global graph.
+
+    Returns res [M x n], acc [M], v [M x Tcycle], the landmark rows [M x L], the
+    1-based global graph (indices, distances) and per-stage timings; with
+    ``dissimilarity`` the graph is rewritten by the KODAMA dissimilarity.
+    """
+    x = _matrix(data)
+    n = x.shape[0]
+    o = N.MatrixOptions(runs=int(M), cycles=int(Tcycle), landmarks=int(landmarks), splitting=int(splitting),
+                        folds=int(folds), k=int(k), metric=_metric(metric), graph_neighbors=int(graph_neighbors),
+                        seed=int(seed), form={"plain": 0, "device": 1}[form], device=int(gpu_device),
+                        n_threads=int(n_threads))
+    L = C.c_int32(0)
+    K = C.c_int32(0)
+    N.call("kodama_hip_matrix_sizes", C.c_int32(n), C.byref(o), C.byref(L), C.byref(K))
+    res = np.zeros((M, n), np.int32)
+    acc = np.zeros(M, np.float64)
+    v = np.zeros((M, max(int(Tcycle), 1)), np.float64)
+    land = np.zeros((M, L.value), np.int32)
+    gi = np.zeros((n, K.value), np.int32)
+    gd = np.zeros((n, K.value), np.float32)
+    timings = np.zeros(6, np.float64)
+    N.call("kodama_hip_kodama_matrix", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o), N.ptr(res),
+           N.ptr(acc), N.ptr(v), N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))
+    out = dict(res=res, acc=acc, v=v[:, : int(Tcycle)].copy(), landmarks=land, knn_indices=gi, knn_distances=gd,
+               timings=dict(zip(("atlas", "graph", "landmarks", "core_precompute", "core_evolution",
+                                 "projection"), timings.tolist())))
+    if dissimilarity:
+        out["knn_indices"], out["knn_distances"] = KODAMA_dissimilarity(gi, gd, res, gpu_device)
+    return out
+
+
 # ---- graph and dissimilarity ----------------------------------------------------
 
 def KODAMA_knn_graph(data, k=30, metric="euclidean", gpu_device=0):

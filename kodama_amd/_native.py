@@ -81,6 +81,16 @@ def core_options(**kw) -> CoreOptions:
     return o
 
 
+class MatrixOptions(C.Structure):
+    """Mirror of ``kodama_hip_matrix_options``."""
+
+    _fields_ = [
+        ("runs", C.c_int32), ("cycles", C.c_int32), ("landmarks", C.c_int32), ("splitting", C.c_int32),
+        ("folds", C.c_int32), ("k", C.c_int32), ("metric", C.c_int32), ("graph_neighbors", C.c_int32),
+        ("seed", C.c_uint64), ("form", C.c_int32), ("device", C.c_int32), ("n_threads", C.c_int32),
+    ]
+
+
 class CoreResult(C.Structure):
     _fields_ = [
         ("clbest", C.POINTER(C.c_int32)), ("clbest_dirty", C.POINTER(C.c_int32)),

@@ -30,7 +30,8 @@ int kodama_hip_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_
     require(n >= 2 && p >= 1, "KODAMAKNNGraph requires at least two rows.");
     require(metric >= KODAMA_HIP_COSINE && metric <= KODAMA_HIP_EUCLIDEAN, "unknown metric.");
     const int kk = std::max(1, std::min(k, n - 1));
-    require(kk <= kMaxK, "graph neighbours above 64 are not supported by the HIP kNN kernel.");
+    require(kk <= (p <= 128 ? kMaxScanK : kMaxK),
+            "graph neighbours above 128 (above 64 beyond 128 feature columns) are not supported by the HIP kNN kernel.");
     require(p <= kMaxDims, "the HIP kNN kernel supports 1..256 feature columns.");
     // Cosine rows are normalised as the reference's exact search expects
     // (native_knn.cpp:1005-1035).

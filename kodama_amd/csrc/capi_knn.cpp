@@ -53,7 +53,8 @@ int kodama_hip_exact_knn_search(const float* train, int32_t train_rows, const fl
     require(train != nullptr && query != nullptr, "null matrix.");
     require(train_rows >= 1 && query_rows >= 0 && dimensions >= 1 && k >= 1,
             "invalid exact kNN dimensions or neighbor count.");
-    require(k <= kMaxK, "k above 64 is not supported by the HIP kNN kernel.");
+    require(k <= (dimensions <= 128 ? kMaxScanK : kMaxK),
+            "k above 128 (above 64 beyond 128 feature columns) is not supported by the HIP kNN kernel.");
     require(dimensions <= kMaxDims, "the HIP kNN kernel supports 1..256 feature columns.");
     select_device(device);
     Stream st(device);

@@ -481,16 +481,19 @@ __global__ void __launch_bounds__(kBlock) knn_merge_slices_kernel(const int* __r
 
 // ---- host launchers --------------------------------------------------------
 
-template <int PMAX, int KMAX>
+// MODES = 1: query vs train only; 2: both modes.
+template <int PMAX, int KMAX, int MODES = 2>
 static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid, int grid_x, int grid_y,
                                  hipStream_t s) {
   const dim3 grid(grid_x, grid_y), block(kBlock);
   if (mode == 0) {
     if (euclid) hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, true>), grid, block, 0, s, prm);
     else hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, false>), grid, block, 0, s, prm);
-  } else {
+  } else if constexpr (MODES == 2) {
     if (euclid) hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 1, true>), grid, block, 0, s, prm);
     else hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 1, false>), grid, block, 0, s, prm);
+  } else {
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -500,12 +503,17 @@ static hipError_t launch_topk_p(const KnnTopkParams& prm, int mode, bool euclid,
   if (prm.k <= 8) return launch_topk_pk<PMAX, 8>(prm, mode, euclid, gx, gy, s);
   if (prm.k <= 16) return launch_topk_pk<PMAX, 16>(prm, mode, euclid, gx, gy, s);
   if (prm.k <= 32) return launch_topk_pk<PMAX, 32>(prm, mode, euclid, gx, gy, s);
-  return launch_topk_pk<PMAX, 64>(prm, mode, euclid, gx, gy, s);
+  if (prm.k <= 64) return launch_topk_pk<PMAX, 64>(prm, mode, euclid, gx, gy, s);
+  // 65..128 neighbours: query vs train only, up to 128 dimensions (the lists
+  // then sit partly in the accumulation registers; beyond they would spill).
+  if constexpr (PMAX <= 128) return launch_topk_pk<PMAX, kMaxScanK, 1>(prm, mode, euclid, gx, gy, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,
                            hipStream_t stream) {
-  if (prm.k < 1 || prm.k > kMaxK || prm.dims < 1 || prm.dims > kMaxDims) return hipErrorInvalidValue;
+  if (prm.k < 1 || prm.dims < 1 || prm.dims > kMaxDims) return hipErrorInvalidValue;
+  if (prm.k > (mode == 0 && prm.dims <= 128 ? kMaxScanK : kMaxK)) return hipErrorInvalidValue;
   const int gx = (max_queries + kBlock - 1) / kBlock;
   if (gx == 0 || runs == 0) return hipSuccess;
   if (prm.dims <= 32) return launch_topk_p<32>(prm, mode, euclid, gx, runs, stream);
@@ -525,7 +533,7 @@ static void launch_vote_k(const VoteParams& prm, dim3 grid, size_t shmem, hipStr
 
 hipError_t launch_knn_merge_slices(const int* idx, const float* dist, int slices, int query_rows, int k,
                                    int* out_index, float* out_dist, hipStream_t stream) {
-  if (slices < 1 || slices > 64 || k < 1 || k > kMaxK) return hipErrorInvalidValue;
+  if (slices < 1 || slices > 64 || k < 1 || k > kMaxScanK) return hipErrorInvalidValue;
   if (query_rows == 0) return hipSuccess;
   const int per_block = kBlock / 64;
   hipLaunchKernelGGL(knn_merge_slices_kernel, dim3((query_rows + per_block - 1) / per_block), dim3(kBlock), 0, stream,

@@ -10,6 +10,7 @@
 namespace kodama_hip {
 
 constexpr int kMaxK = 64;       // neighbours kept per row by the register top-k
+constexpr int kMaxScanK = 128;  // query-vs-train scan with at most 128 dimensions
 constexpr int kMaxDims = 256;   // feature dimensions of the register query path
 
 // Exact kNN. MODE 0 (query vs train): query/train row-major, out_index/out_dist

@@ -1,0 +1,30 @@
+// Host side of the KODAMA.matrix orchestration (matrix.cpp): the landmark atlas
+// and per-run landmarks, k-means on the device, the batched Core and the
+// projection of the other rows.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "engine.h"
+
+namespace kodama_hip {
+
+// Lloyd k-means with the reference's seeding, float order, empty-cluster
+// splitting and stopping rule (kodama_matrix.cpp:664-799); one draw of `rng`.
+// h_x / d_x: the same rows [n x p] on the host and on the device. Returns
+// 1-based labels.
+IVec device_kmeans(const float* h_x, const float* d_x, int n, int p, int k, std::mt19937_64& rng, int max_iter,
+                   hipStream_t s);
+
+// Rows of each 1-based stratum in ascending order (kodama_matrix.cpp:800-824).
+void index_strata(const IVec& labels, int strata, IVec& offsets, IVec& rows);
+
+// Quota sample of `target` rows over shuffled strata, sorted
+// (kodama_matrix.cpp:856-944).
+IVec quota_sample_landmarks(const IVec& offsets, const IVec& rows, int target, std::mt19937_64& rng);
+
+}  // namespace kodama_hip

@@ -1244,6 +1244,272 @@ void dissimilarity(int32_t* idx, float* dist, int samples, int K, const int32_t*
   }
 }
 
+// ---------------------------------------------------------------------------
+// KODAMA.matrix orchestration (kodama_matrix.cpp): the shared k-means landmark
+// atlas, per-run quota landmarks and k-means starting labels, the Core on the
+// landmarks and the projection of the other rows through the global graph.
+// ---------------------------------------------------------------------------
+
+// dot_float32 as the reference's portable x86-64 build computes it
+// (kodama_matrix.cpp:91-124; SSE without FMA): 16 running partials over blocks
+// of 16 (product rounded, then sum rounded), lanes (s[l]+s[4+l])+(s[8+l]+s[12+l]),
+// then ((l0+l1)+l2)+l3, then the remaining products one at a time.
+float sse_dot(const float* a, const float* b, int p) {
+  float s[16];
+  for (int j = 0; j < 16; ++j) s[j] = 0.0f;
+  int i = 0;
+  for (; i + 15 < p; i += 16) {
+    for (int j = 0; j < 16; ++j) {
+      const float prod = a[i + j] * b[i + j];
+      s[j] = s[j] + prod;
+    }
+  }
+  float lane[4];
+  for (int l = 0; l < 4; ++l) lane[l] = (s[l] + s[4 + l]) + (s[8 + l] + s[12 + l]);
+  float sum = ((lane[0] + lane[1]) + lane[2]) + lane[3];
+  for (; i < p; ++i) {
+    const float prod = a[i] * b[i];
+    sum = sum + prod;
+  }
+  return sum;
+}
+
+// FaissCompatibleRandom (kodama_matrix.cpp:535-549): mt19937 of the low 32 bits.
+struct FaissRng {
+  std::mt19937 g;
+  explicit FaissRng(std::uint64_t seed) : g(static_cast<std::uint32_t>(seed)) {}
+  int integer(int m) { return static_cast<int>(g() % static_cast<std::uint32_t>(m)); }
+  float uniform() { return static_cast<float>(g()) / static_cast<float>(std::mt19937::max()); }
+};
+
+// faiss_compatible_permutation (kodama_matrix.cpp:551-564).
+IVec faiss_permutation(int n, std::uint64_t seed) {
+  IVec perm(static_cast<size_t>(n));
+  std::iota(perm.begin(), perm.end(), 0);
+  FaissRng r(seed);
+  for (int i = 0; i + 1 < n; ++i) std::swap(perm[i], perm[i + r.integer(n - i)]);
+  return perm;
+}
+
+// Float Fenwick tree of the empty-cluster donor weights (kodama_matrix.cpp:566-603).
+struct Fenwick {
+  std::vector<float> tree, vals;
+  explicit Fenwick(const std::vector<float>& v) : tree(v.size() + 1, 0.0f), vals(v.size(), 0.0f) {
+    for (size_t i = 0; i < v.size(); ++i) set(i, v[i]);
+  }
+  void set(size_t i, float v) {
+    const float delta = v - vals[i];
+    vals[i] = v;
+    for (++i; i < tree.size(); i += i & (~i + 1)) tree[i] += delta;
+  }
+  float total() const {
+    float r = 0.0f;
+    for (size_t i = vals.size(); i > 0; i -= i & (~i + 1)) r += tree[i];
+    return r;
+  }
+  int select(float target) const {
+    size_t i = 0, step = 1;
+    float prefix = 0.0f;
+    while ((step << 1) < tree.size()) step <<= 1;
+    for (; step > 0; step >>= 1) {
+      const size_t nx = i + step;
+      if (nx < tree.size() && prefix + tree[nx] <= target) {
+        i = nx;
+        prefix += tree[nx];
+      }
+    }
+    return i < vals.size() ? static_cast<int>(i) : -1;
+  }
+};
+
+// split_empty_kmeans_clusters (kodama_matrix.cpp:605-662).
+void split_empty(int rows, int p, int k, std::vector<float>& sizes, std::vector<float>& cent) {
+  if (rows <= k) return;
+  const float eps = 1.0f / 1024.0f;
+  FaissRng r(1234u);
+  std::vector<float> w(static_cast<size_t>(k));
+  for (int c = 0; c < k; ++c) w[c] = std::max(0.0f, sizes[c] - 1.0f);
+  Fenwick tree(w);
+  for (int e = 0; e < k; ++e) {
+    if (sizes[e] != 0.0f) continue;
+    const float total = tree.total();
+    int d = total > 0.0f ? tree.select(r.uniform() * total) : -1;
+    if (d < 0) {
+      d = 0;
+      for (int c = 1; c < k; ++c) if (sizes[c] > sizes[d]) d = c;
+    }
+    float* ce = cent.data() + static_cast<size_t>(e) * p;
+    float* cd = cent.data() + static_cast<size_t>(d) * p;
+    std::copy_n(cd, p, ce);
+    for (int j = 0; j < p; ++j) {
+      if ((j & 1) == 0) {
+        ce[j] *= 1.0f + eps;
+        cd[j] *= 1.0f - eps;
+      } else {
+        ce[j] *= 1.0f - eps;
+        cd[j] *= 1.0f + eps;
+      }
+    }
+    sizes[e] = sizes[d] / 2.0f;
+    sizes[d] -= sizes[e];
+    tree.set(static_cast<size_t>(e), std::max(0.0f, sizes[e] - 1.0f));
+    tree.set(static_cast<size_t>(d), std::max(0.0f, sizes[d] - 1.0f));
+  }
+}
+
+// kmeans_labels, CPU path (kodama_matrix.cpp:664-799): 1-based labels.
+IVec kmeans_labels(const float* x, int n, int p, int k, std::mt19937_64& rng, int max_iter) {
+  k = std::max(1, std::min(k, n));
+  const std::uint64_t kseed = rng() & 0x7fffffffULL;
+  const IVec order = faiss_permutation(n, kseed + 1u);
+  std::vector<float> cent(static_cast<size_t>(k) * p), cn(static_cast<size_t>(k)), pn(static_cast<size_t>(n)),
+    sizes(static_cast<size_t>(k), 0.0f);
+  for (int c = 0; c < k; ++c) std::copy_n(x + static_cast<size_t>(order[c]) * p, p, cent.data() + static_cast<size_t>(c) * p);
+  for (int i = 0; i < n; ++i) pn[i] = sse_dot(x + static_cast<size_t>(i) * p, x + static_cast<size_t>(i) * p, p);
+  IVec asg(static_cast<size_t>(n), -1);
+  auto assign = [&]() {
+    for (int c = 0; c < k; ++c) {
+      const float* cc = cent.data() + static_cast<size_t>(c) * p;
+      cn[c] = sse_dot(cc, cc, p);
+    }
+    int changed = 0;
+    for (int i = 0; i < n; ++i) {
+      int best = 0;
+      float bd = std::numeric_limits<float>::infinity();
+      const float* xi = x + static_cast<size_t>(i) * p;
+      for (int c = 0; c < k; ++c) {
+        const float dot = sse_dot(xi, cent.data() + static_cast<size_t>(c) * p, p);
+        const float sum = pn[i] + cn[c];
+        const float twice = 2.0f * dot;
+        const float d = sum - twice;
+        if (d < bd || (d == bd && c < best)) {
+          bd = d;
+          best = c;
+        }
+      }
+      if (asg[i] != best) {
+        asg[i] = best;
+        ++changed;
+      }
+    }
+    return changed;
+  };
+  for (int it = 0; it < std::max(1, max_iter); ++it) {
+    const int changed = assign();
+    std::fill(cent.begin(), cent.end(), 0.0f);
+    std::fill(sizes.begin(), sizes.end(), 0.0f);
+    for (int i = 0; i < n; ++i) {
+      const int c = asg[i];
+      sizes[c] += 1.0f;
+      float* cc = cent.data() + static_cast<size_t>(c) * p;
+      const float* xi = x + static_cast<size_t>(i) * p;
+      for (int j = 0; j < p; ++j) cc[j] += xi[j];
+    }
+    for (int c = 0; c < k; ++c) {
+      if (sizes[c] == 0.0f) continue;
+      const float scale = 1.0f / sizes[c];
+      float* cc = cent.data() + static_cast<size_t>(c) * p;
+      for (int j = 0; j < p; ++j) cc[j] *= scale;
+    }
+    split_empty(n, p, k, sizes, cent);
+    if (changed == 0) break;
+  }
+  assign();
+  IVec labels(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) labels[i] = asg[i] + 1;
+  return labels;
+}
+
+// index_strata (kodama_matrix.cpp:800-824): rows of each stratum, ascending.
+void index_strata(const IVec& labels, int strata, IVec& offsets, IVec& rows) {
+  offsets.assign(static_cast<size_t>(strata) + 1, 0);
+  rows.assign(labels.size(), 0);
+  for (int l : labels) {
+    if (l < 1 || l > strata) throw std::invalid_argument("stratum label out of range.");
+    ++offsets[l];
+  }
+  for (int s = 1; s <= strata; ++s) offsets[s] += offsets[s - 1];
+  IVec cur = offsets;
+  for (size_t i = 0; i < labels.size(); ++i) rows[cur[labels[i] - 1]++] = static_cast<int>(i);
+}
+
+// quota_sample_landmarks with shuffled strata (kodama_matrix.cpp:856-944):
+// floor quotas, residual slots by a systematic draw over the shuffled strata,
+// std::sample inside each stratum; sorted rows.
+IVec quota_sample(const IVec& offsets, const IVec& rows, int target, std::mt19937_64& rng) {
+  const int samples = static_cast<int>(rows.size());
+  target = std::max(1, std::min(target, samples));
+  const int strata = static_cast<int>(offsets.size()) - 1;
+  IVec quota(static_cast<size_t>(strata), 0), order;
+  std::vector<long double> frac(static_cast<size_t>(strata), 0.0L);
+  int assigned = 0;
+  for (int s = 0; s < strata; ++s) {
+    const int count = offsets[s + 1] - offsets[s];
+    if (count <= 0) continue;
+    order.push_back(s);
+    const long double expected =
+      static_cast<long double>(target) * static_cast<long double>(count) / static_cast<long double>(samples);
+    const int base = std::min(count, static_cast<int>(std::floor(expected)));
+    quota[s] = base;
+    frac[s] = expected - static_cast<long double>(base);
+    assigned += base;
+  }
+  std::shuffle(order.begin(), order.end(), rng);
+  const int residual = target - assigned;
+  if (residual > 0) {
+    const long double offset = std::generate_canonical<long double, 64>(rng);
+    long double cum = 0.0L;
+    int selected = 0;
+    std::vector<char> up(static_cast<size_t>(strata), 0);
+    for (int s : order) {
+      cum += frac[s];
+      const long double next = offset + static_cast<long double>(selected);
+      if (selected < residual && next < cum) {
+        ++quota[s];
+        up[s] = 1;
+        ++selected;
+      }
+    }
+    while (selected < residual) {
+      int best = -1;
+      long double bf = -1.0L;
+      for (int s : order) {
+        if (up[s]) continue;
+        if (frac[s] > bf) {
+          bf = frac[s];
+          best = s;
+        }
+      }
+      if (best < 0) throw std::runtime_error("landmark quota allocation failed.");
+      ++quota[best];
+      up[best] = 1;
+      ++selected;
+    }
+  }
+  IVec out;
+  out.reserve(static_cast<size_t>(target));
+  for (int s = 0; s < strata; ++s) {
+    if (quota[s] <= 0) continue;
+    std::sample(rows.cbegin() + offsets[s], rows.cbegin() + offsets[s + 1], std::back_inserter(out), quota[s], rng);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+// majority_label (kodama_matrix.cpp:169-181): most votes, then smallest label.
+int majority_label(const IVec& v) {
+  std::map<int, int> counts;
+  for (int x : v) counts[x] += 1;
+  int best = counts.begin()->first, bc = counts.begin()->second;
+  for (const auto& kv : counts) {
+    if (kv.second > bc || (kv.second == bc && kv.first < best)) {
+      best = kv.first;
+      bc = kv.second;
+    }
+  }
+  return best;
+}
+
 void copy_fold_out(const IVec& folds, int32_t* out) {
   if (out) std::memcpy(out, folds.data(), sizeof(int32_t) * folds.size());
 }
@@ -1477,6 +1743,133 @@ int oracle_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labe
                              : plsldacv<double>(x, n, p, l, cons, &o, &first_folds, nullptr, nullptr);
     }, out);
     copy_fold_out(first_folds, out->fold_assignments);
+  });
+}
+
+
+int oracle_kmeans_labels(const float* x, int32_t n, int32_t p, int32_t k, uint64_t rng_seed, int32_t discard,
+                         int32_t max_iter, int32_t* labels) {
+  return guarded([&] {
+    std::mt19937_64 rng(rng_seed);
+    for (int i = 0; i < discard; ++i) (void)rng();
+    const IVec l = kmeans_labels(x, n, p, k, rng, max_iter);
+    std::memcpy(labels, l.data(), sizeof(int32_t) * l.size());
+  });
+}
+
+int oracle_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata, int32_t target, uint64_t rng_seed,
+                        int32_t discard, int32_t* rows_out, int32_t* count_out) {
+  return guarded([&] {
+    IVec off, rows;
+    index_strata(IVec(strata_labels, strata_labels + n), strata, off, rows);
+    std::mt19937_64 rng(rng_seed);
+    for (int i = 0; i < discard; ++i) (void)rng();
+    const IVec r = quota_sample(off, rows, target, rng);
+    std::memcpy(rows_out, r.data(), sizeof(int32_t) * r.size());
+    *count_out = static_cast<int32_t>(r.size());
+  });
+}
+
+// KODAMA.matrix, kNN classifier, no spatial data, no constraints or starting
+// labels (kodama_matrix.cpp:3324-4051 with run_iteration 2870-3322): the
+// shared k-means atlas (mt19937_64(seed + 1)), per run r = 1..M an
+// mt19937_64(seed + r) that skips one draw, quota-samples the landmarks and
+// draws the k-means starting labels; CoreKNN on the landmarks with per-fold
+// exact neighbours; each other row takes the majority label of the landmarks
+// among its first graph neighbours (exact global graph, self excluded), at
+// most k votes, clbest[0] when none.
+int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, int32_t* res,
+                         double* acc, double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
+  return guarded([&] {
+    if (n < 3) throw std::invalid_argument("KODAMAMatrix requires at least 3 rows.");
+    int L = o->landmarks > 0 ? o->landmarks : 10000;
+    if (L >= n) L = static_cast<int>(std::ceil(static_cast<double>(n) * 0.75));
+    L = std::max(2, std::min(L, n - 1));
+    const int split = o->splitting > 0 ? o->splitting : (n < 40000 ? 100 : 300);
+    const int coarse_k = std::max(2, std::min(split, n));
+    const int requested = o->graph_neighbors > 0 ? o->graph_neighbors : 100;
+    const int neighbors = std::max(1, static_cast<int>(std::floor(std::min(
+      {static_cast<double>(L), static_cast<double>(n) * 0.75 - 1.0, static_cast<double>(requested)}))));
+    std::mt19937_64 shared(o->seed + 1u);
+    const IVec atlas = kmeans_labels(x, n, p, coarse_k, shared, 10);
+    IVec off, rows;
+    index_strata(atlas, coarse_k, off, rows);
+    const std::vector<float> xs = prepare_rows(x, n, p, o->metric, 1);
+    std::vector<int32_t> gi(static_cast<size_t>(n) * neighbors);
+    std::vector<float> gd(gi.size());
+    IVec self(static_cast<size_t>(n));
+    std::iota(self.begin(), self.end(), 0);
+    exact_knn(xs.data(), n, xs.data(), n, p, neighbors, o->metric, self.data(), gi.data(), gd.data());
+    *landmarks_used = L;
+    for (int r = 1; r <= o->runs; ++r) {
+      std::mt19937_64 rng(o->seed + static_cast<std::uint64_t>(r));
+      (void)rng();
+      const IVec land = quota_sample(off, rows, L, rng);
+      std::vector<float> xl(static_cast<size_t>(L) * p);
+      for (int i = 0; i < L; ++i) std::copy_n(x + static_cast<size_t>(land[i]) * p, p, xl.data() + static_cast<size_t>(i) * p);
+      const int init_k = std::max(2, std::min(split, L));
+      const IVec xw = kmeans_labels(xl.data(), L, p, init_k, rng, 10);
+      IVec cons(static_cast<size_t>(L));
+      std::iota(cons.begin(), cons.end(), 1);
+      oracle_core_options co{};
+      co.cycles = o->cycles;
+      co.classifier = 1;
+      co.seed = o->seed + static_cast<std::uint64_t>(r);
+      co.evolutionary_search = 1;
+      co.guarded_diversity = 1;
+      co.many_to_one_absorption = 1;
+      co.adaptive_proposal_size = 1;
+      co.prediction_guidance = co.policy_adaptive_proposal_size = co.transition_proposal = 1;
+      co.stochastic_acceptance = co.diversity_multiplier = co.pls_transition_coarsening = 1;
+      co.pls_fragmentation_penalty = co.reject_single_class = 1;
+      co.folds = o->folds;
+      co.stratified = 0;
+      co.cv_seed = o->seed + static_cast<std::uint64_t>(r);
+      co.k = std::max(1, o->k);
+      co.metric = o->metric;
+      const IVec folds = make_folds(xw, cons, co.folds, false, co.cv_seed);
+      const KnnTables t = precompute_tables(xl.data(), L, p, folds, co.k, co.metric);
+      Flags f = flags_of(&co);
+      f.knn = true;
+      const int T = std::max(0, o->cycles);
+      IVec clbest(static_cast<size_t>(L)), dirty(static_cast<size_t>(L)), cvp(static_cast<size_t>(L)),
+        psize(static_cast<size_t>(T) + 1), active(static_cast<size_t>(T) + 1);
+      std::vector<double> vacc(static_cast<size_t>(T) + 1), vscore(static_cast<size_t>(T) + 1);
+      std::vector<uint8_t> a1(static_cast<size_t>(T) + 1), a2(static_cast<size_t>(T) + 1), a3(static_cast<size_t>(T) + 1);
+      int32_t counters[ORACLE_N_COUNTERS];
+      oracle_core_result cr{};
+      cr.clbest = clbest.data();
+      cr.clbest_dirty = dirty.data();
+      cr.cvpredbest = cvp.data();
+      cr.vect_acc = vacc.data();
+      cr.vect_score = vscore.data();
+      cr.proposal_size = psize.data();
+      cr.active_classes = active.data();
+      cr.accepted = a1.data();
+      cr.improving_acceptance = a2.data();
+      cr.temperature_acceptance = a3.data();
+      cr.counters = counters;
+      evolve(L, xw, cons, IVec(), f, [&](const IVec& l) { return vote_predict(t, l); }, &cr);
+      int32_t* out = res + static_cast<size_t>(r - 1) * n;
+      IVec local(static_cast<size_t>(n), -1);
+      for (int i = 0; i < L; ++i) local[land[i]] = i;
+      for (int i = 0; i < n; ++i) {
+        if (local[i] >= 0) {
+          out[i] = clbest[local[i]];
+          continue;
+        }
+        IVec votes;
+        for (int j = 0; j < neighbors && static_cast<int>(votes.size()) < co.k; ++j) {
+          const int g = gi[static_cast<size_t>(i) * neighbors + j];
+          if (g < 0 || g >= n) continue;
+          if (local[g] >= 0) votes.push_back(clbest[local[g]]);
+        }
+        out[i] = votes.empty() ? clbest[0] : majority_label(votes);
+      }
+      acc[r - 1] = cr.accbest;
+      for (int c = 0; c < T; ++c) v[static_cast<size_t>(r - 1) * T + c] = vacc[c];
+      if (landmark_rows) std::memcpy(landmark_rows + static_cast<size_t>(r - 1) * L, land.data(), sizeof(int32_t) * L);
+    }
   });
 }
 

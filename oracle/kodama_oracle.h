@@ -126,6 +126,33 @@ int oracle_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labe
                        const int32_t* constrain, const int32_t* fixed,
                        const oracle_core_options* opt, int32_t precision,
                        oracle_core_result* out);
+/* KODAMA.matrix building blocks (kodama_matrix.cpp): k-means labels (1-based)
+ * from an mt19937_64(rng_seed) that first skips `discard` draws; quota-sampled
+ * landmark rows of the strata given as 1-based labels. */
+int oracle_kmeans_labels(const float* x, int32_t n, int32_t p, int32_t k, uint64_t rng_seed,
+                         int32_t discard, int32_t max_iter, int32_t* labels);
+int oracle_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata, int32_t target,
+                        uint64_t rng_seed, int32_t discard, int32_t* rows_out, int32_t* count_out);
+
+/* KODAMAMatrixOptions subset of the kNN pipeline (kodama.hpp:532-566). */
+typedef struct oracle_matrix_options {
+  int32_t runs;
+  int32_t cycles;
+  int32_t landmarks;
+  int32_t splitting;
+  int32_t folds;
+  int32_t k;
+  int32_t metric;
+  int32_t graph_neighbors;
+  uint64_t seed;
+} oracle_matrix_options;
+
+/* res [runs x n], acc [runs], v [runs x cycles], landmark_rows [runs x L] (may
+ * be NULL), *landmarks_used = L. */
+int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* options,
+                         int32_t* res, double* acc, double* v, int32_t* landmark_rows,
+                         int32_t* landmarks_used);
+
 const char* oracle_last_error(void);
 
 /* ---- reference driver (oracle/_ref/libkodama_ref.so) ---------------------- */

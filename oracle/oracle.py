@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libkodama_ref.so")
+REF_MATRIX_SO = os.path.join(HERE, "_ref", "libkodama_ref_matrix.so")
 
 N_COUNTERS = 17
 COUNTER_NAMES = (
@@ -189,7 +190,37 @@ class _Api:
         return pred, folds
 
 
-class Oracle(_Api):
+class MatrixOptions(C.Structure):
+    """Mirror of ``oracle_matrix_options``."""
+
+    _fields_ = [
+        ("runs", C.c_int32), ("cycles", C.c_int32), ("landmarks", C.c_int32), ("splitting", C.c_int32),
+        ("folds", C.c_int32), ("k", C.c_int32), ("metric", C.c_int32), ("graph_neighbors", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class _MatrixBlocks:
+    """k-means labels and quota landmarks, exported by the oracle and by the
+    reference matrix driver under the same names."""
+
+    def kmeans_labels(self, x, k, seed, discard=0, max_iter=10):
+        x = _f32(x)
+        out = np.zeros(x.shape[0], np.int32)
+        self._l.call("kmeans_labels", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     C.c_int32(k), C.c_uint64(seed), C.c_int32(discard), C.c_int32(max_iter), _p(out, C.c_int32))
+        return out
+
+    def quota_sample(self, strata_labels, strata, target, seed, discard=0):
+        lab = _i32(strata_labels)
+        out = np.zeros(lab.size, np.int32)
+        cnt = C.c_int32(0)
+        self._l.call("quota_sample", _p(lab, C.c_int32), C.c_int32(lab.size), C.c_int32(strata),
+                     C.c_int32(target), C.c_uint64(seed), C.c_int32(discard), _p(out, C.c_int32), C.byref(cnt))
+        return out[: cnt.value].copy()
+
+
+class Oracle(_Api, _MatrixBlocks):
     def __init__(self):
         super().__init__(_Lib(ORACLE_SO, "oracle_"))
         L = self._l.lib
@@ -311,6 +342,42 @@ class Oracle(_Api):
                      _ptr_or_null(_i32(fixed), C.c_int32), C.byref(opt), C.c_int32(precision),
                      C.byref(buf.struct))
         return buf.result()
+
+
+def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5, k=30, metric=2,
+                          graph_neighbors=100, seed=1234):
+    x = _f32(x)
+    n = x.shape[0]
+    o = MatrixOptions(runs=runs, cycles=cycles, landmarks=landmarks, splitting=splitting, folds=folds, k=k,
+                      metric=metric, graph_neighbors=graph_neighbors, seed=seed)
+    L = landmarks if 0 < landmarks < n else (int(np.ceil(0.75 * n)) if landmarks >= n else 10000)
+    L = max(2, min(L, n - 1))
+    res = np.zeros((runs, n), np.int32)
+    acc = np.zeros(runs, np.float64)
+    v = np.zeros((runs, max(cycles, 1)), np.float64)
+    land = np.zeros((runs, L), np.int32)
+    used = C.c_int32(0)
+    self._l.call("kodama_matrix", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o),
+                 _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(land, C.c_int32), C.byref(used))
+    assert used.value == L
+    return dict(res=res, acc=acc, v=v[:, :cycles].copy(), landmarks=land)
+
+
+Oracle.kodama_matrix = _oracle_kodama_matrix
+
+
+class ReferenceMatrix(_MatrixBlocks):
+    """The reference's own k-means and landmark sampling (oracle/_ref), where built."""
+
+    def __init__(self):
+        self._l = _Lib(REF_MATRIX_SO, "ref_matrix_")
+        self._l.prefix = "ref_"
+        self._l._err = self._l.lib.ref_matrix_last_error
+        self._l._err.restype = C.c_char_p
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_MATRIX_SO)
 
 
 class Reference(_Api):

@@ -41,7 +41,8 @@ def test_library_exports_every_declared_function():
 
 
 @pytest.mark.parametrize("struct,cls", [("kodama_hip_core_options", "CoreOptions"),
-                                        ("kodama_hip_core_result", "CoreResult")])
+                                        ("kodama_hip_core_result", "CoreResult"),
+                                        ("kodama_hip_matrix_options", "MatrixOptions")])
 def test_ctypes_structs_match_the_header(tmp_path, struct, cls):
     from kodama_amd import _native as N
     ct = getattr(N, cls)

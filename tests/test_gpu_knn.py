@@ -71,6 +71,28 @@ def test_sliced_scan_merge_with_ties_across_slices(oracle, metric):
     assert np.array_equal(idx, oi) and np.array_equal(dist, od)
 
 
+@pytest.mark.parametrize("queries", [40, 1500])
+def test_exact_knn_beyond_64_neighbours(oracle, queries):
+    """k = 100 (the default global-graph width of KODAMA.matrix): the
+    query-vs-train scan keeps up to 128 neighbours for up to 128 dimensions;
+    40 queries take the sliced scan with its merge, 1500 the plain one."""
+    rng = np.random.default_rng(queries)
+    train = rng.normal(size=(2600, 50)).astype(np.float32)
+    train[1300:1400] = train[:100]  # exact duplicates: ties broken by index
+    query = rng.normal(size=(queries, 50)).astype(np.float32)
+    for metric in ("euclidean", "inner_product"):
+        idx, dist = K.exact_knn_search(train, query, 100, metric)
+        oi, od = oracle.exact_knn(train, query, 100, K._native.METRICS[metric])
+        assert np.array_equal(idx, oi) and np.array_equal(dist, od)
+
+
+def test_graph_with_100_neighbours(oracle):
+    x, _, _ = D.config1(n=1800, seed=23)
+    g = K.KODAMA_knn_graph(x, k=100)
+    idx, dist = oracle.knn_graph(x, 100, 2)
+    assert np.array_equal(g["indices"], idx) and np.array_equal(g["distances"], dist)
+
+
 def test_vote_graph_matches_oracle_with_ties(oracle):
     rng = np.random.default_rng(3)
     n, k = 900, 12

@@ -1,0 +1,71 @@
+"""KODAMA.matrix (kNN classifier) on the HIP backend against the oracle's
+restatement of run_kodama_matrix / run_iteration: the k-means atlas and
+starting labels (device k-means in the reference's float order), the quota
+landmarks, the batched CoreKNN and the graph projection. The plain form is
+bit-identical (labels, accuracies, per-cycle accuracy, landmark rows); the
+device-resident form shares the landmarks and starting labels and is held to
+the plain form's accuracy."""
+import numpy as np
+import pytest
+
+import kodama_amd as K
+from kodama_amd import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "p10_euclidean": dict(n=1200, p=10, runs=3, cycles=8, landmarks=400, splitting=40, folds=5, k=10,
+                          metric="euclidean", graph_neighbors=60, seed=7),
+    "p50_cosine": dict(n=1500, p=50, runs=2, cycles=6, landmarks=500, splitting=30, folds=4, k=12,
+                       metric="cosine", graph_neighbors=40, seed=1234),
+    "default_graph_width": dict(n=2400, p=33, runs=2, cycles=5, landmarks=800, splitting=0, folds=5, k=30,
+                                metric="euclidean", graph_neighbors=100, seed=3),
+}
+
+
+def _data(n, p, seed):
+    x, _ = D.gaussian_mixture(n, p, 6, seed=seed)
+    return x
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_plain_matrix_matches_oracle(oracle, name):
+    c = dict(CASES[name])
+    x = _data(c.pop("n"), c.pop("p"), 5)
+    metric = c.pop("metric")
+    got = K.KODAMA_matrix(x, M=c["runs"], Tcycle=c["cycles"], landmarks=c["landmarks"], splitting=c["splitting"],
+                          folds=c["folds"], k=c["k"], metric=metric, graph_neighbors=c["graph_neighbors"],
+                          seed=c["seed"], form="plain")
+    want = oracle.kodama_matrix(x, metric=K._native.METRICS[metric], **c)
+    assert np.array_equal(got["landmarks"], want["landmarks"])
+    assert np.array_equal(got["res"], want["res"])
+    assert np.array_equal(got["acc"], want["acc"])
+    assert np.array_equal(got["v"], want["v"])
+
+
+def test_device_form_shares_landmarks_and_accuracy(oracle):
+    c = dict(CASES["p10_euclidean"], runs=4, cycles=20)
+    x = _data(c.pop("n"), c.pop("p"), 5)
+    c.pop("metric")
+    kw = dict(M=c["runs"], Tcycle=c["cycles"], landmarks=c["landmarks"], splitting=c["splitting"], folds=c["folds"],
+              k=c["k"], graph_neighbors=c["graph_neighbors"], seed=c["seed"])
+    plain = K.KODAMA_matrix(x, form="plain", **kw)
+    dev = K.KODAMA_matrix(x, form="device", **kw)
+    assert np.array_equal(plain["landmarks"], dev["landmarks"])
+    assert np.array_equal(plain["knn_indices"], dev["knn_indices"])
+    # same starting labels and judge, different proposal draws
+    assert abs(dev["acc"].mean() - plain["acc"].mean()) < 0.05
+    n = x.shape[0]
+    for r in range(c["runs"]):
+        assert dev["res"][r].min() >= 1 and np.unique(dev["res"][r]).size > 1
+    assert dev["res"].shape == (c["runs"], n)
+
+
+def test_matrix_graph_and_dissimilarity(oracle):
+    c = dict(CASES["p10_euclidean"])
+    x = _data(c["n"], c["p"], 5)
+    out = K.KODAMA_matrix(x, M=2, Tcycle=4, landmarks=400, splitting=40, folds=5, k=10, graph_neighbors=60, seed=7,
+                          dissimilarity=True)
+    gi, gd = oracle.knn_graph(x, 60, 2)
+    want_i, want_d = oracle.dissimilarity(gi, gd, out["res"])
+    assert np.array_equal(out["knn_indices"], want_i) and np.array_equal(out["knn_distances"], want_d)

@@ -13,7 +13,9 @@ checker of the HIP path:
 
 CPU only: nothing here touches a GPU.
 """
+import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -319,3 +321,77 @@ def test_fresh_dissimilarity_matches_reference(oracle, reference):
     a = oracle.dissimilarity(idx, dist, res)
     b = reference.dissimilarity(idx, dist, res)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ---- KODAMA.matrix building blocks (kodama_matrix.cpp:91-124, 535-944) -----------
+
+def _matrix_golden():
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as MG
+    return MG, golden("matrix_blocks.npz")
+
+
+def test_kmeans_matches_reference_golden(oracle):
+    """Seeding permutation, SSE-order distances, sequential centroid sums,
+    empty-cluster splits (the duplicates case) and the early stop all show in
+    the labels; the fixtures come from the reference's own kmeans_labels."""
+    MG, g = _matrix_golden()
+    for name, (x, k, seed, discard, iters) in MG.matrix_inputs().items():
+        assert np.array_equal(oracle.kmeans_labels(x, k, seed, discard, iters), g[f"kmeans_{name}"]), name
+
+
+def test_quota_landmarks_match_reference_golden(oracle):
+    MG, g = _matrix_golden()
+    labels = {name: g[f"kmeans_{name}"] for name in MG.matrix_inputs()}
+    for i, (name, strata, target, seed, discard) in enumerate(MG.QUOTA_CASES):
+        rows = oracle.quota_sample(labels[name], strata, target, seed, discard)
+        assert np.array_equal(rows, g[f"quota_{i}"]), i
+        # invariants: sorted distinct rows, every stratum within one of its share
+        assert rows.size == target and np.all(np.diff(rows) > 0)
+        lab = labels[name]
+        share = target * np.bincount(lab, minlength=strata + 1)[1:] / lab.size
+        got = np.bincount(lab[rows], minlength=strata + 1)[1:]
+        assert np.all(got >= np.floor(share)) and np.all(got <= np.floor(share) + 1)
+
+
+def test_kmeans_closed_cases(oracle):
+    """Two groups 100 standard deviations apart split exactly into two
+    clusters from any seeding (two seeds in one group pull the second centroid
+    over within a step); k = n puts every distinct row in a class of its own;
+    k = 1 gives one class."""
+    rng = np.random.default_rng(3)
+    truth = np.repeat([0, 1], [70, 130])
+    centres = rng.normal(size=(2, 12)) * 100.0
+    x = (centres[truth] + rng.normal(size=(200, 12))).astype(np.float32)
+    for seed in range(1, 9):
+        lab = oracle.kmeans_labels(x, 2, seed, 0, 20)
+        assert len(set(zip(truth.tolist(), lab.tolist()))) == 2, seed
+    y = rng.normal(size=(40, 5)).astype(np.float32)
+    assert sorted(oracle.kmeans_labels(y, 40, 3).tolist()) == list(range(1, 41))
+    assert np.all(oracle.kmeans_labels(y, 1, 3) == 1)
+
+
+def test_identity_constraint_equals_no_constraint(oracle):
+    """KODAMA.matrix passes constrain = 1..L to each Core run; one group per
+    row in row order draws the same streams as singleton groups, so the batched
+    engine (which takes no constraint) replays it."""
+    x, y, _ = D.config1(n=300, seed=4)
+    init = (y % 4 + 1).astype(np.int32)
+    opt = O.core_options(cycles=6, k=8, folds=5, seed=7, cv_seed=7, **MATRIX_RUN)
+    a = oracle.core_knn(x, init, opt)
+    b = oracle.core_knn(x, init, opt, constrain=np.arange(1, 301, dtype=np.int32))
+    for key in ("clbest", "vect_acc", "accepted", "fold_assignments"):
+        assert np.array_equal(a[key], b[key]), key
+
+
+def test_fresh_kmeans_and_quota_match_reference(oracle):
+    if not O.ReferenceMatrix.available():
+        pytest.skip("oracle/_ref (the compiled reference core) is not built here")
+    ref = O.ReferenceMatrix()
+    x, _, _ = D.config1(n=900, seed=13)
+    for k, seed, discard, iters in ((50, 11, 0, 10), (9, 12, 1, 10), (300, 13, 0, 4)):
+        a = oracle.kmeans_labels(x, k, seed, discard, iters)
+        assert np.array_equal(a, ref.kmeans_labels(x, k, seed, discard, iters)), k
+    lab = oracle.kmeans_labels(x, 50, 5)
+    for target in (10, 200, 777):
+        assert np.array_equal(oracle.quota_sample(lab, 50, target, 3, 1), ref.quota_sample(lab, 50, target, 3, 1))
