@@ -10,14 +10,20 @@ evaluation with its proposal and accept/reject; steps/s = M * cycles / wall time
 of the label-evolution phase (host proposals, device votes and syncs included;
 the one-time per-fold exact kNN precompute is excluded and reported apart).
 
-Two forms run on the same inputs:
+Three forms run on the same inputs:
   plain   host proposals with the reference's random streams (per run
           bit-identical to CoreKNN), one batched device vote per cycle;
   device  proposal, absorption, vote, objective and acceptance on the device:
           one persistent launch (one workgroup per run, run state in LDS) for
-          all cycles, counter-based random streams.
-`value` is the device form's steps/s; both forms' mean final CV accuracy is
-reported ("equal accuracy" = within 0.5 points).
+          all cycles, counter-based random streams;
+  learned the device form with the learned proposal: per cycle the network
+          scores the rows whose label differs from their prediction, then one
+          persistent launch runs the cycle (a quarter of the cycles stay
+          uniform). The network is first trained, untimed, on the plain
+          form's accept/reject traces over 16 other runs.
+`value` is the device form's steps/s; every form's mean final CV accuracy is
+reported ("equal accuracy" = within 0.5 points), with the cycles each form
+needs to reach the plain form's final accuracy minus half a point.
 
 The roofline object covers the dominant kernel of the reported form, timed with
 HIP events on its launch stream:
@@ -66,27 +72,29 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10):
+def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10, salt: int = 0):
+    """The batched-runs workload; `salt` moves the landmark, fold and
+    starting-label streams (the data matrix stays the same)."""
     import kodama_amd as K
     from kodama_amd import datasets as D
     x, _ = D.config3(n=n, p=p)
-    sizes = D.randint(6, 1, runs, 1000, 5000)
+    sizes = D.randint(6 + salt, 1, runs, 1000, 5000)
     rows, fold_ids, init = [], [], []
     for r in range(runs):
         L = int(sizes[r])
-        keys = D.uniform(6, 100 + r, n)
+        keys = D.uniform(6 + salt, 100 + r, n)
         rw = np.sort(np.argsort(keys, kind="stable")[:L]).astype(np.int32)
         # Uneven folds: Dirichlet(1) shares, at least one row per fold.
-        e = -np.log1p(-D.uniform(7, r, folds))
+        e = -np.log1p(-D.uniform(7 + salt, r, folds))
         share = e / e.sum()
         cnt = np.maximum(1, np.floor(share * (L - folds)).astype(np.int64) + 1)
         cnt[np.argmax(cnt)] += L - cnt.sum()
-        perm = np.argsort(D.uniform(7, 1000 + r, L), kind="stable")
+        perm = np.argsort(D.uniform(7 + salt, 1000 + r, L), kind="stable")
         f = np.empty(L, np.int32)
         f[perm] = np.repeat(np.arange(folds, dtype=np.int32), cnt)
         # Starting labels: nearest of min(300, L) random landmark centres.
         nc = min(300, L)
-        centres = x[rw[np.argsort(D.uniform(8, r, L), kind="stable")[:nc]]]
+        centres = x[rw[np.argsort(D.uniform(8 + salt, r, L), kind="stable")[:nc]]]
         idx, _ = K.exact_knn_search(centres, x[rw], 1, "euclidean")
         rows.append(rw)
         fold_ids.append(f)
@@ -218,7 +226,7 @@ def main() -> int:
     ap.add_argument("--runs", type=int, default=100)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--k", type=int, default=30)
-    ap.add_argument("--forms", default="plain,device")
+    ap.add_argument("--forms", default="plain,device,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
@@ -246,20 +254,43 @@ def main() -> int:
                 guarded_diversity=1, many_to_one_absorption=1, n_threads=0)
     forms = [f for f in args.forms.split(",") if f]
     res = {}
+    curves = {}
+    params = None
     for form in forms:
+        extra = {}
+        if form == "learned":
+            # The network is trained (untimed) on the accept/reject traces of
+            # the plain form over 16 other runs (other landmarks, folds and
+            # starting labels of the same data).
+            t0 = time.perf_counter()
+            tx, trows, tfolds, tinit = make_workload(16, args.n, 50, args.k, salt=100)
+            feat, tgt, _ = K.mlp_collect_traces(tx, trows, tinit, K.core_options(cycles=args.steps, **base),
+                                                fold_ids=tfolds)
+            params, losses = K.mlp_train(feat, tgt, epochs=20, batch=256, learning_rate=0.05, momentum=0.9, seed=5)
+            extra = dict(trace_samples=int(tgt.size), accepted_share=float(np.mean(tgt != 30)),
+                         final_train_loss=float(losses[-1]), training_seconds=time.perf_counter() - t0)
+        run = (lambda o, rr, ii, ff: K.core_knn_batch_learned(x, rr, ii, o, params, fallback=0.25, fold_ids=ff)) \
+            if form == "learned" else \
+            (lambda o, rr, ii, ff, form=form: K.core_knn_batch(x, rr, ii, o, mode=form, fold_ids=ff))
         if args.warmup > 0:
-            K.core_knn_batch(x, rows[:8], init[:8], K.core_options(cycles=args.warmup, **base),
-                             mode=form, fold_ids=fold_ids[:8])
-        out = K.core_knn_batch(x, rows, init, K.core_options(cycles=args.steps, **base), mode=form,
-                               fold_ids=fold_ids)
+            run(K.core_options(cycles=args.warmup, **base), rows[:8], init[:8], fold_ids[:8])
+        out = run(K.core_options(cycles=args.steps, **base), rows, init, fold_ids)
         sps = args.runs * args.steps / out["evolution_seconds"]
+        curves[form] = out["vect_acc"].mean(axis=0)
         res[form] = dict(steps_per_s=sps, evolution_seconds=out["evolution_seconds"],
                          precompute_seconds=out["precompute_seconds"],
                          mean_accbest=float(np.mean(out["accbest"])),
-                         vote_kernel_ms=out["vote_seconds"] * 1e3)
+                         vote_kernel_ms=out.get("vote_seconds", out.get("device_seconds", 0.0)) * 1e3, **extra)
         log(f"{form}: {sps:,.0f} steps/s, evolution {out['evolution_seconds']:.3f}s, "
             f"precompute {out['precompute_seconds']:.3f}s, mean accbest {res[form]['mean_accbest']:.4f}, "
-            f"vote kernel {out['vote_seconds'] * 1e3:.3f} ms")
+            f"device kernel {res[form]['vote_kernel_ms']:.3f} ms")
+    # Cycles each form needs before its mean best accuracy reaches the plain
+    # form's final mean best accuracy minus half a point.
+    if "plain" in res:
+        goal = res["plain"]["mean_accbest"] - 0.005
+        for form, c in curves.items():
+            hit = np.flatnonzero(c >= goal)
+            res[form]["cycles_to_plain_accuracy"] = int(hit[0]) + 1 if hit.size else None
     head = "device" if "device" in res else forms[0]
     kern_s = res[head]["vote_kernel_ms"] * 1e-3
     if head == "device":

@@ -294,6 +294,66 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p,
                              double* v, int32_t* landmark_rows, int32_t* graph_indices_one_based,
                              float* graph_distances, double* timings);
 
+/* ---- learned proposal -------------------------------------------------------
+ * No counterpart in the reference: the learned proposal that the device-resident
+ * evolution can use in place of the uniform one, for callers that train it on
+ * the plain method's accept/reject traces and run it. A 64-64-64-32 MLP (ReLU)
+ * over per-row features of the neighbour vote: [0,30) the vote counts of the
+ * row's 30 most voted labels (count desc, code asc) over k, [30,61) one-hot of
+ * its own label's rank (60: not voted), 61 the margin (top - own count) / k,
+ * 62 a constant 1. Outputs: j < 30 relabel to the label of rank j, 30 keep;
+ * ranks that do not exist or hold the row's own label are masked. Weights
+ * and inputs are rounded to bf16, sums are FP32. params: fp32 [10400] =
+ * W1[64x64] b1[64] W2[64x64] b2[64] W3[64x32] b3[32], W row-major [in][out]. */
+int kodama_hip_mlp_init(uint64_t seed, float* params);
+
+/* Features of a table's rows: neighbors [rows x k] row ids (-1 missing),
+ * labels [rows] codes. features [rows x 64], ranked [rows x 30] (-1 past the
+ * voted labels). */
+int kodama_hip_mlp_features(const int32_t* neighbors, const int32_t* labels, int32_t rows, int32_t k,
+                            int32_t device, float* features, int32_t* ranked);
+
+/* Forward pass: probs [rows x 32] (masked outputs 0). */
+int kodama_hip_mlp_forward(const float* params, const float* features, int32_t rows, int32_t device,
+                           float* probs);
+
+/* Gradient of the mean cross-entropy over the rows (targets 0..30) with
+ * respect to params (the bf16-rounded weights' gradient, applied to the fp32
+ * master); *loss = the mean loss (loss may be NULL). */
+int kodama_hip_mlp_gradients(const float* params, const float* features, const int32_t* targets,
+                             int32_t rows, int32_t device, float* grads, float* loss);
+
+/* The plain batched evolution (kodama_hip_core_knn_batch mode 0) recording its
+ * accept/reject samples: every row a proposal relabels (to its prediction),
+ * its features under the labels the proposal started from, and its target
+ * (the rank of the new label when the proposal was accepted, 30 otherwise).
+ * features [max_samples x 64], targets [max_samples]; *samples = recorded;
+ * accbest [runs] may be NULL. */
+int kodama_hip_mlp_collect_traces(const float* x, int32_t n, int32_t p, int32_t runs,
+                                  const int32_t* run_offsets, const int32_t* landmark_rows,
+                                  const int32_t* initial_labels, const int32_t* fold_ids,
+                                  const kodama_hip_core_options* options, int32_t max_samples,
+                                  float* features, int32_t* targets, int32_t* samples, double* accbest);
+
+/* Minibatch SGD with momentum on the device (deterministic for a seed):
+ * params in/out; loss_per_epoch [epochs] may be NULL. */
+int kodama_hip_mlp_train(const float* features, const int32_t* targets, int32_t samples, int32_t epochs,
+                         int32_t batch, float learning_rate, float momentum, uint64_t seed,
+                         int32_t device, float* params, double* loss_per_epoch);
+
+/* kodama_hip_core_knn_batch in the device-resident form with the learned
+ * proposal: each cycle the network scores the rows whose label differs from
+ * their prediction; the persistent kernel chooses rows with probability
+ * take x score / (sum of scores) and relabels them to the network's target;
+ * with probability `fallback` a cycle uses the uniform proposal instead. The
+ * CV accuracy stays the only acceptance criterion. Outputs as the batch. */
+int kodama_hip_core_knn_batch_learned(const float* x, int32_t n, int32_t p, int32_t runs,
+                                      const int32_t* run_offsets, const int32_t* landmark_rows,
+                                      const int32_t* initial_labels, const int32_t* fold_ids,
+                                      const kodama_hip_core_options* options, const float* params,
+                                      float fallback, int32_t* clbest, double* accbest, double* vect_acc,
+                                      double* timings);
+
 #ifdef __cplusplus
 }
 #endif

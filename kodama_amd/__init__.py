@@ -271,6 +271,108 @@ def core_plslda_batch(data, run_rows, initial_labels, options: CoreOptions, fold
                 steps=int(timings[2]), eval_kernel_seconds=float(timings[3]))
 
 
+# ---- learned proposal ---------------------------------------------------------------
+
+MLP_PARAMS = 10400
+
+
+def mlp_init(seed=1) -> np.ndarray:
+    """He-normal weights, zero biases (fp32 [10400])."""
+    out = np.zeros(MLP_PARAMS, np.float32)
+    N.call("kodama_hip_mlp_init", C.c_uint64(int(seed)), N.ptr(out))
+    return out
+
+
+def mlp_features(neighbors, labels, gpu_device=0):
+    """Device features [rows x 64] and ranked voted codes [rows x 30] of a table."""
+    nbr = N.i32(neighbors)
+    lab = N.i32(labels)
+    rows, k = nbr.shape
+    feat = np.zeros((rows, 64), np.float32)
+    ranked = np.zeros((rows, 30), np.int32)
+    N.call("kodama_hip_mlp_features", N.ptr(nbr), N.ptr(lab), C.c_int32(rows), C.c_int32(k), C.c_int32(gpu_device),
+           N.ptr(feat), N.ptr(ranked))
+    return feat, ranked
+
+
+def mlp_forward(params, features, gpu_device=0) -> np.ndarray:
+    par = N.f32(params)
+    feat = N.f32(features)
+    probs = np.zeros((feat.shape[0], 32), np.float32)
+    N.call("kodama_hip_mlp_forward", N.ptr(par), N.ptr(feat), C.c_int32(feat.shape[0]), C.c_int32(gpu_device),
+           N.ptr(probs))
+    return probs
+
+
+def mlp_gradients(params, features, targets, gpu_device=0):
+    """Mean cross-entropy gradient [10400] and loss."""
+    par = N.f32(params)
+    feat = N.f32(features)
+    tgt = N.i32(targets)
+    grads = np.zeros(MLP_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    N.call("kodama_hip_mlp_gradients", N.ptr(par), N.ptr(feat), N.ptr(tgt), C.c_int32(feat.shape[0]),
+           C.c_int32(gpu_device), N.ptr(grads), N.ptr(loss))
+    return grads, float(loss[0])
+
+
+def _batch_args(data, run_rows, initial_labels, fold_ids):
+    x = _matrix(data)
+    M = len(run_rows)
+    offsets = np.zeros(M + 1, np.int32)
+    offsets[1:] = np.cumsum([len(r) for r in run_rows])
+    rows = N.i32(np.concatenate([np.asarray(r) for r in run_rows]))
+    init = N.i32(np.concatenate([np.asarray(l) for l in initial_labels]))
+    folds = None if fold_ids is None else N.i32(np.concatenate([np.asarray(f) for f in fold_ids]))
+    return x, M, offsets, rows, init, folds
+
+
+def mlp_collect_traces(data, run_rows, initial_labels, options: CoreOptions, fold_ids=None, max_samples=200000):
+    """Accept/reject samples of the plain batched evolution: features [S x 64],
+    targets [S] (rank of the new label when accepted, 30 when rejected)."""
+    x, M, offsets, rows, init, folds = _batch_args(data, run_rows, initial_labels, fold_ids)
+    feat = np.zeros((max_samples, 64), np.float32)
+    tgt = np.zeros(max_samples, np.int32)
+    count = C.c_int32(0)
+    acc = np.zeros(M, np.float64)
+    N.call("kodama_hip_mlp_collect_traces", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+           N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), C.c_int32(max_samples),
+           N.ptr(feat), N.ptr(tgt), C.byref(count), N.ptr(acc))
+    c = count.value
+    return feat[:c].copy(), tgt[:c].copy(), acc
+
+
+def mlp_train(features, targets, params=None, epochs=10, batch=256, learning_rate=0.05, momentum=0.9, seed=1,
+              gpu_device=0):
+    """Device SGD; returns (params, mean loss per epoch)."""
+    feat = N.f32(features)
+    tgt = N.i32(targets)
+    par = np.array(mlp_init(seed) if params is None else params, dtype=np.float32, copy=True)
+    loss = np.zeros(max(int(epochs), 1), np.float64)
+    N.call("kodama_hip_mlp_train", N.ptr(feat), N.ptr(tgt), C.c_int32(feat.shape[0]), C.c_int32(int(epochs)),
+           C.c_int32(int(batch)), C.c_float(learning_rate), C.c_float(momentum), C.c_uint64(int(seed)),
+           C.c_int32(gpu_device), N.ptr(par), N.ptr(loss))
+    return par, loss[: int(epochs)]
+
+
+def core_knn_batch_learned(data, run_rows, initial_labels, options: CoreOptions, params, fallback=0.25,
+                           fold_ids=None):
+    """core_knn_batch in the device-resident form with the learned proposal."""
+    x, M, offsets, rows, init, folds = _batch_args(data, run_rows, initial_labels, fold_ids)
+    par = N.f32(params)
+    clbest = np.zeros(int(offsets[-1]), np.int32)
+    accbest = np.zeros(M, np.float64)
+    vect = np.zeros(M * max(options.cycles, 1), np.float64)
+    timings = np.zeros(4, np.float64)
+    N.call("kodama_hip_core_knn_batch_learned", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+           N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), N.ptr(par),
+           C.c_float(fallback), N.ptr(clbest), N.ptr(accbest), N.ptr(vect), N.ptr(timings))
+    return dict(clbest=[clbest[offsets[r]:offsets[r + 1]].copy() for r in range(M)], accbest=accbest,
+                vect_acc=vect[: M * options.cycles].reshape(M, options.cycles),
+                precompute_seconds=float(timings[0]), evolution_seconds=float(timings[1]),
+                steps=int(timings[2]), device_seconds=float(timings[3]))
+
+
 # ---- KODAMA.matrix ----------------------------------------------------------------
 
 def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5, k=30,

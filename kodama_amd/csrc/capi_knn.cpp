@@ -13,9 +13,11 @@
 #include "../../include/kodama_hip.h"
 #include "capi_common.h"
 #include "engine.h"
+#include "batch_api.h"
 #include "evolve_device.h"
 #include "knn_mfma.h"
 #include "launch.h"
+#include "mlp.h"
 
 using namespace kodama_hip;
 
@@ -43,6 +45,91 @@ struct kodama_hip_vote_graph {
   std::unique_ptr<KnnBatch> batch;
   int samples = 0;
 };
+
+namespace kodama_hip {
+
+void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, float fallback, TraceSink* trace,
+                   int32_t* clbest, double* accbest, double* vect_acc, double* timings) {
+  const float* x = q.x;
+  const int n = q.n, p = q.p, runs = q.runs;
+  const kodama_hip_core_options* options = q.options;
+  require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
+  require(runs >= 1, "runs must be positive.");
+  require(q.run_offsets != nullptr && q.landmark_rows != nullptr && q.initial_labels != nullptr, "null input.");
+  require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
+  require(options->cycles >= 0, "Core cycles must be non-negative.");
+  require(mode != 2 || mlp_params != nullptr, "the learned form needs network parameters.");
+  select_device(options->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  const IVec offsets(q.run_offsets, q.run_offsets + runs + 1);
+  require(offsets[0] == 0, "run_offsets[0] must be 0.");
+  for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
+  const int total = offsets[runs];
+  const IVec rows(q.landmark_rows, q.landmark_rows + total);
+  for (int v : rows) require(v >= 0 && v < n, "landmark row outside the data matrix.");
+  std::vector<IVec> folds(static_cast<size_t>(runs)), values(static_cast<size_t>(runs)),
+    codes(static_cast<size_t>(runs));
+  IVec fold_concat(static_cast<size_t>(total));
+  int classes = 1;
+  for (int r = 0; r < runs; ++r) {
+    const int L = offsets[r + 1] - offsets[r];
+    const int* lab = q.initial_labels + offsets[r];
+    folds[r] = q.fold_ids ? IVec(q.fold_ids + offsets[r], q.fold_ids + offsets[r + 1])
+                          : make_folds(IVec(lab, lab + L), IVec(), options->folds, options->stratified != 0,
+                                       options->cv_seed + static_cast<std::uint64_t>(r));
+    {
+      // Two folds or more: every row then has training rows, so the exact
+      // search gives it at least one neighbour.
+      const auto mm = std::minmax_element(folds[r].begin(), folds[r].end());
+      require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
+    }
+    encode_labels(lab, L, values[r], codes[r]);
+    classes = std::max(classes, static_cast<int>(values[r].size()));
+    std::copy(folds[r].begin(), folds[r].end(), fold_concat.begin() + offsets[r]);
+  }
+  const std::vector<float> xs = prepare_rows(x, n, p, options->metric, 0);
+  KnnBatch batch(options->device, xs.data(), n, p, offsets, rows, fold_concat, options->k, options->metric,
+                 classes);
+  const double t_pre = seconds_since(t0);
+  const auto t1 = std::chrono::steady_clock::now();
+  double vote_seconds = 0.0;
+  if (mode == 0) {
+    std::vector<RunEvolution> ev;
+    ev.reserve(static_cast<size_t>(runs));
+    for (int r = 0; r < runs; ++r) {
+      CoreFlags f = CoreFlags::from(*options);
+      f.knn = true;
+      f.seed = options->seed + static_cast<std::uint64_t>(r);
+      ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
+    }
+    if (trace) evolve_plain_traced(ev, batch, folds, options->cycles, options->n_threads, *trace);
+    else evolve_plain(ev, batch, folds, options->cycles, options->n_threads);
+    vote_seconds = batch.vote_ms_total() * 1e-3 / std::max(1, batch.vote_calls());
+    for (int r = 0; r < runs; ++r) {
+      const IVec& b = ev[r].best_codes();
+      for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];
+      accbest[r] = ev[r].accbest();
+      if (vect_acc) std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(), vect_acc + static_cast<size_t>(r) * options->cycles);
+    }
+  } else {
+    DeviceBuffer<float> d_params(mode == 2 ? kMlpParams : 0);
+    if (mode == 2) d_params.upload(mlp_params, kMlpParams, batch.stream());
+    DeviceEvolution dev(batch, values, codes, folds, *options);
+    if (mode == 2) dev.run_learned(options->cycles, d_params.get(), fallback);
+    else dev.run(options->cycles);
+    vote_seconds = dev.kernel_seconds();
+    dev.export_results(clbest, accbest, vect_acc);
+  }
+  const double t_evo = seconds_since(t1);
+  if (timings) {
+    timings[0] = t_pre;
+    timings[1] = t_evo;
+    timings[2] = static_cast<double>(runs) * options->cycles;
+    timings[3] = vote_seconds;
+  }
+}
+
+}  // namespace kodama_hip
 
 extern "C" {
 
@@ -262,76 +349,9 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
                               const kodama_hip_core_options* options, int32_t mode, int32_t* clbest,
                               double* accbest, double* vect_acc, double* timings) {
   return guarded([&] {
-    require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
-    require(runs >= 1, "runs must be positive.");
-    require(run_offsets != nullptr && landmark_rows != nullptr && initial_labels != nullptr, "null input.");
-    require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
     require(mode == 0 || mode == 1, "mode must be 0 (plain) or 1 (device-resident).");
-    require(options->cycles >= 0, "Core cycles must be non-negative.");
-    select_device(options->device);
-    const auto t0 = std::chrono::steady_clock::now();
-    const IVec offsets(run_offsets, run_offsets + runs + 1);
-    require(offsets[0] == 0, "run_offsets[0] must be 0.");
-    for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
-    const int total = offsets[runs];
-    const IVec rows(landmark_rows, landmark_rows + total);
-    for (int v : rows) require(v >= 0 && v < n, "landmark row outside the data matrix.");
-    std::vector<IVec> folds(static_cast<size_t>(runs)), values(static_cast<size_t>(runs)),
-      codes(static_cast<size_t>(runs));
-    IVec fold_concat(static_cast<size_t>(total));
-    int classes = 1;
-    for (int r = 0; r < runs; ++r) {
-      const int L = offsets[r + 1] - offsets[r];
-      const int* lab = initial_labels + offsets[r];
-      folds[r] = fold_ids ? IVec(fold_ids + offsets[r], fold_ids + offsets[r + 1])
-                          : make_folds(IVec(lab, lab + L), IVec(), options->folds, options->stratified != 0,
-                                       options->cv_seed + static_cast<std::uint64_t>(r));
-      {
-        // Two folds or more: every row then has training rows, so the exact
-        // search gives it at least one neighbour.
-        const auto mm = std::minmax_element(folds[r].begin(), folds[r].end());
-        require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
-      }
-      encode_labels(lab, L, values[r], codes[r]);
-      classes = std::max(classes, static_cast<int>(values[r].size()));
-      std::copy(folds[r].begin(), folds[r].end(), fold_concat.begin() + offsets[r]);
-    }
-    const std::vector<float> xs = prepare_rows(x, n, p, options->metric, 0);
-    KnnBatch batch(options->device, xs.data(), n, p, offsets, rows, fold_concat, options->k, options->metric,
-                   classes);
-    const double t_pre = seconds_since(t0);
-    const auto t1 = std::chrono::steady_clock::now();
-    double vote_seconds = 0.0;
-    if (mode == 0) {
-      std::vector<RunEvolution> ev;
-      ev.reserve(static_cast<size_t>(runs));
-      for (int r = 0; r < runs; ++r) {
-        CoreFlags f = CoreFlags::from(*options);
-        f.knn = true;
-        f.seed = options->seed + static_cast<std::uint64_t>(r);
-        ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
-      }
-      evolve_plain(ev, batch, folds, options->cycles, options->n_threads);
-      vote_seconds = batch.vote_ms_total() * 1e-3 / std::max(1, batch.vote_calls());
-      for (int r = 0; r < runs; ++r) {
-        const IVec& b = ev[r].best_codes();
-        for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];
-        accbest[r] = ev[r].accbest();
-        if (vect_acc) std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(), vect_acc + static_cast<size_t>(r) * options->cycles);
-      }
-    } else {
-      DeviceEvolution dev(batch, values, codes, folds, *options);
-      dev.run(options->cycles);
-      vote_seconds = dev.kernel_seconds();
-      dev.export_results(clbest, accbest, vect_acc);
-    }
-    const double t_evo = seconds_since(t1);
-    if (timings) {
-      timings[0] = t_pre;
-      timings[1] = t_evo;
-      timings[2] = static_cast<double>(runs) * options->cycles;
-      timings[3] = vote_seconds;
-    }
+    BatchRequest q{x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options};
+    run_knn_batch(q, mode, nullptr, 0.0f, nullptr, clbest, accbest, vect_acc, timings);
   });
 }
 

@@ -818,7 +818,7 @@ void apply_fold_fallbacks(const IVec& empty_rows, const IVec& folds, const int* 
 }  // namespace
 
 void evolve_plain(std::vector<RunEvolution>& runs, KnnBatch& batch, const std::vector<IVec>& folds,
-                  int cycles, int n_threads) {
+                  int cycles, int n_threads, const CycleHook& hook) {
   const int M = static_cast<int>(runs.size());
   const int C = batch.classes();
   const int threads = resolve_threads(n_threads, M);
@@ -852,8 +852,10 @@ void evolve_plain(std::vector<RunEvolution>& runs, KnnBatch& batch, const std::v
       std::copy(cl.begin(), cl.end(), h_labels + batch.pad_offset(r));
     }
     batch.vote_all();
+    if (hook) hook(cycle, false);
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
     for (int r = 0; r < M; ++r) finish_run(r, cycle, h_labels + batch.pad_offset(r));
+    if (hook) hook(cycle, true);
   }
 }
 

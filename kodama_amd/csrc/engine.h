@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <random>
 #include <vector>
 
@@ -69,6 +70,16 @@ class RunEvolution {
   const IVec& best_codes() const { return best_; }
   const std::vector<double>& vect_acc() const { return vacc_; }
   const IVec& label_values() const { return label_values_; }
+  // Trace access for the learned proposal: the current state a proposal starts
+  // from, the groups it sampled (the first proposal_size(cycle) entries of
+  // sampled_groups()) and whether the cycle's proposal became the current state.
+  const IVec& current_labels() const { return cur_; }
+  const IVec& current_predictions() const { return curpred_; }
+  const IVec& sampled_groups() const { return order_; }
+  int proposal_size(int cycle) const { return psize_[cycle]; }
+  bool accepted(int cycle) const { return accepted_[cycle] != 0; }
+  bool fixed_row(int i) const { return fixed_[i] == 1; }
+  bool singleton_groups() const { return singletons_; }
 
  private:
   struct Flows;  // per-class transition counts
@@ -170,9 +181,13 @@ void encode_labels(const int* labels, int n, IVec& values, IVec& codes);
 // Drive plain (host-proposal) evolution of every run in `runs` against the
 // batch's device vote: one vote launch per cycle for all runs. run i of `runs`
 // is batch run i. Rows without neighbours get the training-fold majority label
-// (core.cpp:274-309). n_threads host workers run the proposals.
+// (core.cpp:274-309). n_threads host workers run the proposals. `hook`
+// (optional) is called on the host thread once per cycle after the vote of the
+// proposals (after_finish false: the runs still hold their pre-proposal
+// state) and once after every run has accepted or rejected (true).
+using CycleHook = std::function<void(int cycle, bool after_finish)>;
 void evolve_plain(std::vector<RunEvolution>& runs, KnnBatch& batch, const std::vector<IVec>& folds,
-                  int cycles, int n_threads);
+                  int cycles, int n_threads, const CycleHook& hook = CycleHook());
 
 int resolve_threads(int requested, int runs);
 

@@ -9,6 +9,7 @@
 
 #include "evolve_launch.h"
 #include "launch.h"
+#include "mlp_launch.h"
 
 namespace kodama_hip {
 
@@ -35,6 +36,10 @@ struct DeviceEvolution::Impl {
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
+  // Learned proposal buffers, allocated on first use.
+  DeviceBuffer<float> learn_score, learn_feat;
+  DeviceBuffer<unsigned short> learn_target, cand_ranked;
+  DeviceBuffer<int> cand_row, cand_count;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -159,6 +164,64 @@ void DeviceEvolution::run(int cycles) {
   t.stop(s);
   kernel_seconds_ = t.ms() * 1e-3;
   m.initialized = true;
+  m.done_cycles += cycles;
+}
+
+void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallback) {
+  Impl& m = *impl_;
+  if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
+  const hipStream_t s = m.batch.stream();
+  const size_t pad = static_cast<size_t>(m.batch.pad_total());
+  if (m.learn_score.size() != pad) {
+    m.learn_score.resize(pad);
+    m.learn_target.resize(pad);
+    m.learn_feat.resize(pad * kMlpIn);
+    m.cand_ranked.resize(pad * kMlpRanks);
+    m.cand_row.resize(pad);
+    m.cand_count.resize(1);
+  }
+  MlpScoreParams sp;
+  sp.params = d_params;
+  sp.neighbors = m.prm.neighbors;
+  sp.labels = m.prm.cur;
+  sp.preds = m.prm.curpred;
+  sp.run_offsets = m.prm.run_offsets;
+  sp.pad_offsets = m.prm.pad_offsets;
+  sp.block_run = m.batch.d_block_run().get();
+  sp.pad_total = m.prm.pad_total;
+  sp.k = m.prm.k;
+  sp.feat = m.learn_feat.get();
+  sp.cand_row = m.cand_row.get();
+  sp.cand_ranked = m.cand_ranked.get();
+  sp.cand_count = m.cand_count.get();
+  sp.score = m.learn_score.get();
+  sp.target = m.learn_target.get();
+  EventTimer t;
+  t.start(s);
+  if (!m.initialized) {
+    // Evaluation of the starting labels alone: the network needs their predictions.
+    EvoParams p = m.prm;
+    p.init = 1;
+    p.cycle_begin = m.done_cycles;
+    p.cycle_count = 0;
+    check(launch_evo_persistent(p, s), "evo_persistent");
+    m.initialized = true;
+  }
+  for (int c = 0; c < cycles; ++c) {
+    check(launch_evo_zero(m.cand_count.get(), 1, s), "evo_zero");
+    check(launch_mlp_candidates(sp, s), "mlp_candidates");
+    check(launch_mlp_score(sp, s), "mlp_score");
+    EvoParams p = m.prm;
+    p.init = 0;
+    p.cycle_begin = m.done_cycles + c;
+    p.cycle_count = 1;
+    p.learn_score = m.learn_score.get();
+    p.learn_target = m.learn_target.get();
+    p.learn_fallback = fallback;
+    check(launch_evo_persistent(p, s), "evo_persistent");
+  }
+  t.stop(s);
+  kernel_seconds_ = t.ms() * 1e-3;
   m.done_cycles += cycles;
 }
 

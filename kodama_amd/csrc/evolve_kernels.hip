@@ -456,17 +456,41 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       {
         const int hi = proposal_hi(n, cycle, p.cycles);
         const int take = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
-        // The rows at positions [0, take) of a keyed permutation are chosen:
-        // marked in the proposal buffer, which holds only label codes otherwise.
-        const Permutation perm(draw(key, cycle, 2), bits);
-        for (int y = tid; y < take; y += THREADS) {
-          prop[perm.preimage(static_cast<uint32_t>(y), static_cast<uint32_t>(n))] = kChosenMark;
+        // A learned cycle (the network's scores are given, and the fallback
+        // draw does not pick the uniform proposal) chooses each row with
+        // probability take * score / (sum of scores), capped at 1, and relabels
+        // it to the network's target; otherwise the rows at positions
+        // [0, take) of a keyed permutation are chosen and take their
+        // prediction. Chosen rows are marked in the proposal buffer, which
+        // holds only label codes otherwise.
+        const bool learned = p.learn_score != nullptr && unit(draw(key, cycle, 5)) >= p.learn_fallback;
+        if (learned) {
+          const float* score = p.learn_score + base;
+          float part = 0.0f;
+          for (int i = tid; i < n; i += THREADS) part += score[i];
+#pragma unroll
+          for (int off = kWave / 2; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+          if (lane == 0) s.wave_total[wave] = part;
+          __syncthreads();
+          double total = 0.0;
+          for (int w = 0; w < THREADS / kWave; ++w) total += s.wave_total[w];
+          const uint64_t row_key = draw(key, cycle, 6);
+          for (int i = tid; i < n; i += THREADS) {
+            const double u = unit(mix64(row_key + static_cast<uint64_t>(i)));
+            if (score[i] > 0.0f && u * total < static_cast<double>(take) * score[i]) prop[i] = kChosenMark;
+          }
+        } else {
+          const Permutation perm(draw(key, cycle, 2), bits);
+          for (int y = tid; y < take; y += THREADS) {
+            prop[perm.preimage(static_cast<uint32_t>(y), static_cast<uint32_t>(n))] = kChosenMark;
+          }
         }
         __syncthreads();
         for (int i = tid; i < n; i += THREADS) {
           const bool chosen = prop[i] == kChosenMark;
           const int guide = cur_pred[i];
-          const int label = chosen ? guide : static_cast<int>(cur_lab[i]);
+          const int label = chosen ? (learned ? static_cast<int>(p.learn_target[base + i]) : guide)
+                                   : static_cast<int>(cur_lab[i]);
           prop[i] = static_cast<unsigned short>(label);
           atomicAdd(&s.sizes[label], 1);
           if (label == guide) atomicAdd(&s.stay[label], 1);

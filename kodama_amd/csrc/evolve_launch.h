@@ -40,6 +40,12 @@ struct EvoParams {
   double* state = nullptr;                   // [runs x kEvoStateStride]
   int* flows = nullptr;                      // [runs x classes x classes], zero between cycles
   double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
+  // Learned proposal (null: every cycle uniform): per padded row the
+  // network's selection score and relabel target, and the probability that a
+  // cycle falls back to the uniform proposal.
+  const float* learn_score = nullptr;
+  const unsigned short* learn_target = nullptr;
+  float learn_fallback = 0.0f;
 };
 
 size_t evo_lds_bytes(int classes, int max_rows);

@@ -395,3 +395,52 @@ def test_fresh_kmeans_and_quota_match_reference(oracle):
     lab = oracle.kmeans_labels(x, 50, 5)
     for target in (10, 200, 777):
         assert np.array_equal(oracle.quota_sample(lab, 50, target, 3, 1), ref.quota_sample(lab, 50, target, 3, 1))
+
+
+# ---- learned proposal (no reference counterpart): the float64 restatement ----------
+
+def test_mlp_features_hand_built():
+    import mlp_oracle as MO
+    lab = np.array([5, 2, 2, 5, 7, 5, 2, 9], np.int32)
+    nbr = np.array([[1, 2, 3, 4, 5, 6],     # votes 2,2,5,7,5,2 own 5 -> (2:3) (5:2) (7:1)
+                    [7, -1, -1, 0, 0, 0],   # votes 9,5,5,5 own 2 (not voted)
+                    [1, 1, 1, 1, 1, 1],     # all 2 (own): one rank, margin 0
+                    [-1, -1, -1, -1, -1, -1],
+                    [0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0]], np.int32)
+    f, r = MO.features(nbr, lab)
+    assert np.array_equal(r[0, :4], [2, 5, 7, -1])
+    assert np.allclose(f[0, :3], [3 / 6, 2 / 6, 1 / 6]) and f[0, 3] == 0 and f[0, 31] == 1 and f[0, 61] == np.float32(1 / 6)
+    assert np.array_equal(r[1, :3], [5, 9, -1]) and f[1, 60] == 1 and f[1, 61] == np.float32(3 / 6)
+    assert f[2, 30] == 1 and f[2, 61] == 0 and np.allclose(f[2, :2], [1, 0])
+    assert np.all(f[3, :30] == 0) and f[3, 60] == 1 and f[3, 61] == 0
+    assert np.all(f[:, 62] == 1) and np.all(f[:, 30:61].sum(axis=1) == 1)
+    v = MO.valid_outputs(f)
+    assert v[0, 0] and not v[0, 1] and v[0, 2] and not v[0, 3] and v[0, 30] and not v[0, 31]
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """Central differences of the float64 loss with respect to the weights the
+    network computes with (bf16-rounded values taken as the variables) and the
+    biases: a dropped ReLU mask, a transposed product or a missing 1/n fails."""
+    import mlp_oracle as MO
+    rng = np.random.default_rng(0)
+    nbr = rng.integers(-1, 60, size=(60, 12)).astype(np.int32)
+    lab = rng.integers(0, 7, size=60).astype(np.int32)
+    feat, _ = MO.features(nbr, lab)
+    valid = MO.valid_outputs(feat)
+    tgt = np.array([rng.choice(np.flatnonzero(v)) for v in valid], np.int32)
+    params = (rng.normal(size=MO.PARAMS) * 0.3).astype(np.float32)
+    w = MO.unpack(params)
+    loss, g = MO.loss_and_grads_of(w, feat, tgt)
+    names = (("w1", MO.W1), ("b1", MO.B1), ("w2", MO.W2), ("b2", MO.B2), ("w3", MO.W3), ("b3", MO.B3))
+    for name, off in names:
+        flat = w[name].ravel()
+        for idx in rng.choice(flat.size, size=6, replace=False):
+            h = 1e-6
+            wp = {k: v.copy() for k, v in w.items()}
+            wm = {k: v.copy() for k, v in w.items()}
+            wp[name].ravel()[idx] += h
+            wm[name].ravel()[idx] -= h
+            fd = (MO.loss_and_grads_of(wp, feat, tgt)[0] - MO.loss_and_grads_of(wm, feat, tgt)[0]) / (2 * h)
+            assert abs(fd - g[off + idx]) <= 1e-6 + 1e-5 * abs(fd), (name, idx, fd, g[off + idx])
+    assert np.isfinite(loss)
