@@ -39,7 +39,8 @@ struct DeviceEvolution::Impl {
   // Learned proposal buffers, allocated on first use.
   DeviceBuffer<float> learn_score, learn_feat;
   DeviceBuffer<unsigned short> learn_target, cand_ranked;
-  DeviceBuffer<int> cand_row, cand_count;
+  DeviceBuffer<int> cand_row, cand_count;  // cand_count: two alternating counters
+  int learn_parity = 0;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -170,6 +171,9 @@ void DeviceEvolution::run(int cycles) {
 void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallback) {
   Impl& m = *impl_;
   if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
+  if (m.initialized && m.cand_count.size() == 0) {
+    throw InvalidArgument("the learned proposal continues only an evolution it started.");
+  }
   const hipStream_t s = m.batch.stream();
   const size_t pad = static_cast<size_t>(m.batch.pad_total());
   if (m.learn_score.size() != pad) {
@@ -178,37 +182,44 @@ void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallb
     m.learn_feat.resize(pad * kMlpIn);
     m.cand_ranked.resize(pad * kMlpRanks);
     m.cand_row.resize(pad);
-    m.cand_count.resize(1);
   }
   MlpScoreParams sp;
   sp.params = d_params;
   sp.neighbors = m.prm.neighbors;
   sp.labels = m.prm.cur;
-  sp.preds = m.prm.curpred;
   sp.run_offsets = m.prm.run_offsets;
   sp.pad_offsets = m.prm.pad_offsets;
   sp.block_run = m.batch.d_block_run().get();
   sp.pad_total = m.prm.pad_total;
   sp.k = m.prm.k;
+  sp.cand_rows = m.cand_row.get();
   sp.feat = m.learn_feat.get();
-  sp.cand_row = m.cand_row.get();
   sp.cand_ranked = m.cand_ranked.get();
-  sp.cand_count = m.cand_count.get();
   sp.score = m.learn_score.get();
   sp.target = m.learn_target.get();
   EventTimer t;
   t.start(s);
   if (!m.initialized) {
-    // Evaluation of the starting labels alone: the network needs their predictions.
+    // Evaluation of the starting labels alone: it lists the first candidates.
+    m.cand_count.resize(2);
+    check(launch_evo_zero(m.cand_count.get(), 1, s), "evo_zero");
+    m.learn_parity = 0;
     EvoParams p = m.prm;
     p.init = 1;
     p.cycle_begin = m.done_cycles;
     p.cycle_count = 0;
+    p.cand_rows = m.cand_row.get();
+    p.cand_count = m.cand_count.get();
     check(launch_evo_persistent(p, s), "evo_persistent");
     m.initialized = true;
   }
   for (int c = 0; c < cycles; ++c) {
-    check(launch_evo_zero(m.cand_count.get(), 1, s), "evo_zero");
+    // Counters alternate: this cycle's candidates were listed at cur, the
+    // launch below lists the next ones at nxt (zeroed by the features kernel).
+    int* cur = m.cand_count.get() + m.learn_parity;
+    int* nxt = m.cand_count.get() + (m.learn_parity ^ 1);
+    sp.cand_count = cur;
+    sp.next_count = nxt;
     check(launch_mlp_candidates(sp, s), "mlp_candidates");
     check(launch_mlp_score(sp, s), "mlp_score");
     EvoParams p = m.prm;
@@ -218,7 +229,10 @@ void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallb
     p.learn_score = m.learn_score.get();
     p.learn_target = m.learn_target.get();
     p.learn_fallback = fallback;
+    p.cand_rows = m.cand_row.get();
+    p.cand_count = nxt;
     check(launch_evo_persistent(p, s), "evo_persistent");
+    m.learn_parity ^= 1;
   }
   t.stop(s);
   kernel_seconds_ = t.ms() * 1e-3;

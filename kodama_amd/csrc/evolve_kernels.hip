@@ -465,9 +465,12 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // holds only label codes otherwise.
         const bool learned = p.learn_score != nullptr && unit(draw(key, cycle, 5)) >= p.learn_fallback;
         if (learned) {
+          // Scores are current for the rows whose label differs from their
+          // prediction (scored after the previous launch); other rows keep
+          // stale values and are skipped.
           const float* score = p.learn_score + base;
           float part = 0.0f;
-          for (int i = tid; i < n; i += THREADS) part += score[i];
+          for (int i = tid; i < n; i += THREADS) part += cur_lab[i] != cur_pred[i] ? score[i] : 0.0f;
 #pragma unroll
           for (int off = kWave / 2; off >= 1; off >>= 1) part += __shfl_xor(part, off);
           if (lane == 0) s.wave_total[wave] = part;
@@ -477,7 +480,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           const uint64_t row_key = draw(key, cycle, 6);
           for (int i = tid; i < n; i += THREADS) {
             const double u = unit(mix64(row_key + static_cast<uint64_t>(i)));
-            if (score[i] > 0.0f && u * total < static_cast<double>(take) * score[i]) prop[i] = kChosenMark;
+            if (cur_lab[i] != cur_pred[i] && score[i] > 0.0f && u * total < static_cast<double>(take) * score[i]) {
+              prop[i] = kChosenMark;
+            }
           }
         } else {
           const Permutation perm(draw(key, cycle, 2), bits);
@@ -726,11 +731,28 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     __syncthreads();
   }
 
-  // Write the run state back for the next launch.
+  // Write the run state back for the next launch; with the learned proposal,
+  // list the rows whose label differs from their prediction (the rows the
+  // network scores before the next cycle). Uniform trip count for the ballot.
   const int cur = s.scal[kCur];
-  for (int i = threadIdx.x; i < n; i += THREADS) {
-    p.cur[base + i] = s.lab(cur)[i];
-    p.curpred[base + i] = s.pred(cur)[i];
+  for (int i0 = 0; i0 < n; i0 += THREADS) {
+    const int i = i0 + static_cast<int>(threadIdx.x);
+    bool cand = false;
+    if (i < n) {
+      const unsigned short l = s.lab(cur)[i], pr = s.pred(cur)[i];
+      p.cur[base + i] = l;
+      p.curpred[base + i] = pr;
+      cand = l != pr;
+    }
+    if (p.cand_rows != nullptr) {
+      const uint64_t mask = __ballot(cand);
+      if (mask != 0ull) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(p.cand_count, __popcll(mask));
+        slot = __shfl(slot, 0) + __popcll(mask & ((1ull << lane) - 1ull));
+        if (cand) p.cand_rows[slot] = base + i;
+      }
+    }
   }
   if (threadIdx.x < 4) p.state[static_cast<size_t>(run) * kEvoStateStride + threadIdx.x] = s.state[threadIdx.x];
 }

@@ -46,6 +46,10 @@ struct EvoParams {
   const float* learn_score = nullptr;
   const unsigned short* learn_target = nullptr;
   float learn_fallback = 0.0f;
+  // Candidate list written at the end of a launch (null: none): padded rows
+  // whose label differs from their prediction, appended at *cand_count.
+  int* cand_rows = nullptr;
+  int* cand_count = nullptr;
 };
 
 size_t evo_lds_bytes(int classes, int max_rows);

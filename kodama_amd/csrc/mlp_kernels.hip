@@ -291,32 +291,23 @@ __global__ void __launch_bounds__(kBlock) mlp_sgd_kernel(float* __restrict__ par
 
 template <int KMAX>
 __global__ void __launch_bounds__(kBlock) mlp_candidates_kernel(MlpScoreParams p) {
-  const int row = blockIdx.x * kBlock + threadIdx.x;
-  const int run = p.block_run[blockIdx.x];
-  const int base = p.pad_offsets[run];
-  const int n = p.run_offsets[run + 1] - p.run_offsets[run];
-  const int i = row - base;
-  const bool valid = i < n;
-  const uint32_t lab = valid ? p.labels[row] : 0u;
-  const bool cand = valid && lab != p.preds[row];
-  p.score[row] = 0.0f;
-  p.target[row] = static_cast<unsigned short>(lab);
-  const uint64_t mask = __ballot(cand);
-  if (mask == 0ull) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  int slot = 0;
-  if (lane == 0) slot = atomicAdd(p.cand_count, __popcll(mask));
-  slot = __shfl(slot, 0) + __popcll(mask & ((1ull << lane) - 1ull));
-  if (!cand) return;
-  uint32_t code[KMAX];
+  const int count = *p.cand_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p.next_count = 0;
+  const int stride = gridDim.x * kBlock;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < count; c += stride) {
+    const int row = p.cand_rows[c];
+    const int run = p.block_run[row / kBlock];
+    const int base = p.pad_offsets[run];
+    const int n = p.run_offsets[run + 1] - p.run_offsets[run];
+    uint32_t code[KMAX];
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const int id = j < p.k ? static_cast<int>(p.neighbors[static_cast<size_t>(j) * p.pad_total + row]) : n;
-    code[j] = id < n ? static_cast<uint32_t>(p.labels[base + id]) : kNone;
+    for (int j = 0; j < KMAX; ++j) {
+      const int id = j < p.k ? static_cast<int>(p.neighbors[static_cast<size_t>(j) * p.pad_total + row]) : n;
+      code[j] = id < n ? static_cast<uint32_t>(p.labels[base + id]) : kNone;
+    }
+    row_features<KMAX>(code, p.k, p.labels[row], p.feat + static_cast<size_t>(c) * kMlpIn,
+                       p.cand_ranked + static_cast<size_t>(c) * kMlpRanks);
   }
-  row_features<KMAX>(code, p.k, lab, p.feat + static_cast<size_t>(slot) * kMlpIn,
-                     p.cand_ranked + static_cast<size_t>(slot) * kMlpRanks);
-  p.cand_row[slot] = row;
 }
 
 __global__ void __launch_bounds__(kBlock) mlp_score_kernel(MlpScoreParams p) {
@@ -342,11 +333,15 @@ __global__ void __launch_bounds__(kBlock) mlp_score_kernel(MlpScoreParams p) {
       key = other > key ? other : key;
     }
     const float keep = __shfl(prob, kMlpKeep);
-    if (lane == 0 && key != 0ull) {
-      const int best = kMlpRanks - static_cast<int>(key & 0xFFFFFFFFull);
-      const int row = p.cand_row[c];
-      p.score[row] = 1.0f - keep;
-      p.target[row] = p.cand_ranked[static_cast<size_t>(c) * kMlpRanks + best];
+    if (lane == 0) {
+      const int row = p.cand_rows[c];
+      if (key != 0ull) {
+        const int best = kMlpRanks - static_cast<int>(key & 0xFFFFFFFFull);
+        p.score[row] = 1.0f - keep;
+        p.target[row] = p.cand_ranked[static_cast<size_t>(c) * kMlpRanks + best];
+      } else {
+        p.score[row] = 0.0f;
+      }
     }
     wave_sync();
   }
@@ -392,8 +387,7 @@ hipError_t launch_mlp_sgd(float* params, float* velocity, const float* partial, 
 
 hipError_t launch_mlp_candidates(const MlpScoreParams& p, hipStream_t s) {
   if (p.k < 1 || p.k > 64 || p.pad_total % kBlock != 0) return hipErrorInvalidValue;
-  if (p.pad_total == 0) return hipSuccess;
-  const dim3 grid(p.pad_total / kBlock), block(kBlock);
+  const dim3 grid(256), block(kBlock);
   if (p.k <= 8) hipLaunchKernelGGL((mlp_candidates_kernel<8>), grid, block, 0, s, p);
   else if (p.k <= 16) hipLaunchKernelGGL((mlp_candidates_kernel<16>), grid, block, 0, s, p);
   else if (p.k <= 32) hipLaunchKernelGGL((mlp_candidates_kernel<32>), grid, block, 0, s, p);

@@ -52,24 +52,26 @@ hipError_t launch_mlp_grad(const float* params, const float* feat, const int* ta
 hipError_t launch_mlp_sgd(float* params, float* velocity, const float* partial, int count, float lr, float momentum,
                           float* loss_out, hipStream_t s);
 
-// Per cycle of the learned device evolution, over the padded rows of all runs:
-// rows whose label differs from their prediction are scored by the network
-// (score = 1 - p(keep), target = label of the most probable valid rank);
-// other rows get score 0. cand_count is reset by the caller.
+// Per cycle of the learned device evolution: the persistent kernel lists the
+// rows whose label differs from their prediction (cand_rows, *cand_count);
+// launch_mlp_candidates computes their features (and zeroes *next_count, the
+// counter the next persistent launch appends to), launch_mlp_score runs the
+// network: score = 1 - p(keep) and target = the label of the most probable
+// valid rank (score 0 when no rank is valid).
 struct MlpScoreParams {
   const float* params = nullptr;
   const unsigned short* neighbors = nullptr;  // [k x pad_total] run-local ids (n = none)
   const unsigned short* labels = nullptr;     // [pad_total] current codes
-  const unsigned short* preds = nullptr;      // [pad_total] current predictions
   const int* run_offsets = nullptr;
   const int* pad_offsets = nullptr;
   const int* block_run = nullptr;             // [pad_total / 256]
   int pad_total = 0;
   int k = 0;
-  float* feat = nullptr;                      // [pad_total x 64] candidate features (compacted)
-  int* cand_row = nullptr;                    // [pad_total] padded row of each candidate
+  const int* cand_rows = nullptr;             // [pad_total] padded rows of the candidates
+  const int* cand_count = nullptr;
+  int* next_count = nullptr;
+  float* feat = nullptr;                      // [pad_total x 64] candidate features
   unsigned short* cand_ranked = nullptr;      // [pad_total x 30]
-  int* cand_count = nullptr;                  // one counter
   float* score = nullptr;                     // [pad_total]
   unsigned short* target = nullptr;           // [pad_total]
 };

@@ -8,7 +8,6 @@ import pytest
 
 import kodama_amd as K
 import mlp_oracle as MO
-from kodama_amd import datasets as D
 
 from test_gpu_device_evolution import OPTS, _centre_batch
 
