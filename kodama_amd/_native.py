@@ -163,6 +163,10 @@ def lib():
     return _lib
 
 
+def lib_available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
 def call(name: str, *args) -> None:
     L = lib()
     rc = getattr(L, name)(*args)

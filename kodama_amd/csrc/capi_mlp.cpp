@@ -63,9 +63,11 @@ int kodama_hip_mlp_forward(const float* params, const float* features, int32_t r
     const hipStream_t s = st.get();
     DeviceBuffer<float> d_par(kMlpParams), d_feat(static_cast<size_t>(rows) * kMlpIn),
       d_prob(static_cast<size_t>(rows) * kMlpOut);
+    DeviceBuffer<uint32_t> d_packed(kMlpPacked);
     d_par.upload(params, kMlpParams, s);
     d_feat.upload(features, d_feat.size(), s);
-    check(launch_mlp_forward(d_par.get(), d_feat.get(), rows, d_prob.get(), s), "mlp_forward");
+    check(launch_mlp_pack(d_par.get(), d_packed.get(), s), "mlp_pack");
+    check(launch_mlp_forward(d_packed.get(), d_feat.get(), rows, d_prob.get(), s), "mlp_forward");
     d_prob.download(probs, d_prob.size(), s);
     st.sync();
   });

@@ -40,6 +40,7 @@ struct DeviceEvolution::Impl {
   DeviceBuffer<float> learn_score, learn_feat;
   DeviceBuffer<unsigned short> learn_target, cand_ranked;
   DeviceBuffer<int> cand_row, cand_count;  // cand_count: two alternating counters
+  DeviceBuffer<uint32_t> packed;           // network weights in the scoring kernel's layout
   int learn_parity = 0;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
@@ -183,8 +184,10 @@ void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallb
     m.cand_ranked.resize(pad * kMlpRanks);
     m.cand_row.resize(pad);
   }
+  if (m.packed.size() == 0) m.packed.resize(kMlpPacked);
+  check(launch_mlp_pack(d_params, m.packed.get(), s), "mlp_pack");
   MlpScoreParams sp;
-  sp.params = d_params;
+  sp.packed = m.packed.get();
   sp.neighbors = m.prm.neighbors;
   sp.labels = m.prm.cur;
   sp.run_offsets = m.prm.run_offsets;

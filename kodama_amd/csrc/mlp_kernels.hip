@@ -160,7 +160,98 @@ __device__ float forward_wave(const MlpLds& w, const float* __restrict__ feat, f
   return e / sum;
 }
 
+// The forward pass with the weights held in registers (the scoring and test
+// kernels): lane j keeps column j of each matrix as bf16 pairs (rows 2p and
+// 2p + 1 in one register), so each layer is 64 LDS-broadcast multiply-adds
+// per lane with no weight traffic. Same operation order as forward_wave.
+struct PackedWeights {
+  uint32_t w1[kMlpIn / 2], w2[kMlpHidden / 2], w3[kMlpHidden / 2];
+  float b1, b2, b3;
+
+  __device__ void load(const uint32_t* __restrict__ packed, int lane) {
+#pragma unroll
+    for (int q = 0; q < kMlpIn / 2; ++q) w1[q] = packed[(0 * 32 + q) * kWave + lane];
+#pragma unroll
+    for (int q = 0; q < kMlpHidden / 2; ++q) w2[q] = packed[(1 * 32 + q) * kWave + lane];
+#pragma unroll
+    for (int q = 0; q < kMlpHidden / 2; ++q) w3[q] = packed[(2 * 32 + q) * kWave + lane];
+    b1 = __uint_as_float(packed[3 * 32 * kWave + lane]);
+    b2 = __uint_as_float(packed[3 * 32 * kWave + kMlpHidden + lane]);
+    b3 = __uint_as_float(packed[3 * 32 * kWave + 2 * kMlpHidden + (lane & (kMlpOut - 1))]);
+  }
+};
+
+__device__ __forceinline__ float lo_bf16(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi_bf16(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+__device__ float forward_packed(const PackedWeights& w, const float* __restrict__ feat, float* xs, float* h1s,
+                                float* h2s, int lane) {
+  xs[lane] = bf16_round(feat[lane]);
+  wave_sync();
+  float a = w.b1;
+#pragma unroll
+  for (int q = 0; q < kMlpIn / 2; ++q) {
+    a = fmaf(xs[2 * q], lo_bf16(w.w1[q]), a);
+    a = fmaf(xs[2 * q + 1], hi_bf16(w.w1[q]), a);
+  }
+  h1s[lane] = fmaxf(a, 0.0f);
+  wave_sync();
+  a = w.b2;
+#pragma unroll
+  for (int q = 0; q < kMlpHidden / 2; ++q) {
+    a = fmaf(h1s[2 * q], lo_bf16(w.w2[q]), a);
+    a = fmaf(h1s[2 * q + 1], hi_bf16(w.w2[q]), a);
+  }
+  h2s[lane] = fmaxf(a, 0.0f);
+  wave_sync();
+  float z = w.b3;
+#pragma unroll
+  for (int q = 0; q < kMlpHidden / 2; ++q) {
+    z = fmaf(h2s[2 * q], lo_bf16(w.w3[q]), z);
+    z = fmaf(h2s[2 * q + 1], hi_bf16(w.w3[q]), z);
+  }
+  const bool valid = lane < kMlpOut && output_valid(feat, lane);
+  float m = valid ? z : -INFINITY;
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  const float e = valid ? expf(z - m) : 0.0f;
+  float sum = e;
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+  return e / sum;
+}
+
 }  // namespace
+
+// Parameters -> the packed layout of PackedWeights: [3][32][64] bf16 pairs
+// (matrix, row pair, column), then b1, b2, b3 as fp32 bits.
+__global__ void __launch_bounds__(kBlock) mlp_pack_kernel(const float* __restrict__ params,
+                                                          uint32_t* __restrict__ packed) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  constexpr int kPairs = 3 * 32 * kWave;
+  if (e >= kMlpPacked) return;
+  if (e < kPairs) {
+    const int m = e / (32 * kWave), q = (e / kWave) % 32, j = e % kWave;
+    float lo = 0.0f, hi = 0.0f;
+    if (m == 0) {
+      lo = params[kMlpW1 + (2 * q) * kMlpHidden + j];
+      hi = params[kMlpW1 + (2 * q + 1) * kMlpHidden + j];
+    } else if (m == 1) {
+      lo = params[kMlpW2 + (2 * q) * kMlpHidden + j];
+      hi = params[kMlpW2 + (2 * q + 1) * kMlpHidden + j];
+    } else if (j < kMlpOut) {
+      lo = params[kMlpW3 + (2 * q) * kMlpOut + j];
+      hi = params[kMlpW3 + (2 * q + 1) * kMlpOut + j];
+    }
+    packed[e] = (__float_as_uint(bf16_round(lo)) >> 16) | (__float_as_uint(bf16_round(hi)) & 0xFFFF0000u);
+  } else {
+    const int b = e - kPairs;
+    const float v = b < kMlpHidden ? params[kMlpB1 + b]
+                                   : (b < 2 * kMlpHidden ? params[kMlpB2 + b - kMlpHidden]
+                                                         : params[kMlpB3 + b - 2 * kMlpHidden]);
+    packed[e] = __float_as_uint(v);
+  }
+}
 
 // ---- features of caller tables (tests, traces) ---------------------------------
 
@@ -181,18 +272,17 @@ __global__ void __launch_bounds__(kBlock) mlp_table_features_kernel(const int* _
                      ranked + static_cast<size_t>(r) * kMlpRanks);
 }
 
-__global__ void __launch_bounds__(kBlock) mlp_forward_kernel(const float* __restrict__ params,
+__global__ void __launch_bounds__(kBlock) mlp_forward_kernel(const uint32_t* __restrict__ packed,
                                                               const float* __restrict__ feat, int rows,
                                                               float* __restrict__ probs) {
-  __shared__ MlpLds w;
   __shared__ float vec[kWavesPerBlock][3][kWave];
-  load_weights(params, w);
-  __syncthreads();
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  PackedWeights w;
+  w.load(packed, lane);
   const int waves = gridDim.x * kWavesPerBlock;
   for (int s = blockIdx.x * kWavesPerBlock + wave; s < rows; s += waves) {
     const float* f = feat + static_cast<size_t>(s) * kMlpIn;
-    const float p = forward_wave(w, f, vec[wave][0], vec[wave][1], vec[wave][2], lane, nullptr);
+    const float p = forward_packed(w, f, vec[wave][0], vec[wave][1], vec[wave][2], lane);
     if (lane < kMlpOut) probs[static_cast<size_t>(s) * kMlpOut + lane] = p;
     wave_sync();
   }
@@ -289,6 +379,9 @@ __global__ void __launch_bounds__(kBlock) mlp_sgd_kernel(float* __restrict__ par
 
 // ---- the learned proposal inside the evolution -----------------------------------
 
+// Features of the listed candidates, one thread each (their gathers overlap
+// across the many threads in flight); also zeroes the counter the next
+// persistent launch appends to.
 template <int KMAX>
 __global__ void __launch_bounds__(kBlock) mlp_candidates_kernel(MlpScoreParams p) {
   const int count = *p.cand_count;
@@ -310,20 +403,22 @@ __global__ void __launch_bounds__(kBlock) mlp_candidates_kernel(MlpScoreParams p
   }
 }
 
+// The network over the candidates, one wave per candidate, weights in
+// registers: score = 1 - p(keep), target = the label of the most probable
+// valid rank (score 0 when no rank is valid).
 __global__ void __launch_bounds__(kBlock) mlp_score_kernel(MlpScoreParams p) {
-  __shared__ MlpLds w;
   __shared__ float vec[kWavesPerBlock][3][kWave];
   const int count = *p.cand_count;
-  if (static_cast<int>(blockIdx.x) * kWavesPerBlock >= count) return;  // uniform: no work for this block
-  load_weights(p.params, w);
-  __syncthreads();
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (static_cast<int>(blockIdx.x) * kWavesPerBlock + wave >= count) return;  // whole wave idle; no block barrier
+  PackedWeights w;
+  w.load(p.packed, lane);
   const int waves = gridDim.x * kWavesPerBlock;
   for (int c = blockIdx.x * kWavesPerBlock + wave; c < count; c += waves) {
-    const float* f = p.feat + static_cast<size_t>(c) * kMlpIn;
-    const float prob = forward_wave(w, f, vec[wave][0], vec[wave][1], vec[wave][2], lane, nullptr);
+    const float* feat = p.feat + static_cast<size_t>(c) * kMlpIn;
+    const float prob = forward_packed(w, feat, vec[wave][0], vec[wave][1], vec[wave][2], lane);
     // Most probable valid relabel rank (smallest rank on ties).
-    const bool rank_ok = lane < kMlpRanks && output_valid(f, lane);
+    const bool rank_ok = lane < kMlpRanks && output_valid(feat, lane);
     uint64_t key = rank_ok ? (static_cast<uint64_t>(__float_as_uint(prob)) << 32) |
                                static_cast<uint32_t>(kMlpRanks - lane)
                            : 0ull;
@@ -361,10 +456,15 @@ hipError_t launch_mlp_features(const int* nbr, const int* labels, int rows, int 
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_forward(const float* params, const float* feat, int rows, float* probs, hipStream_t s) {
+hipError_t launch_mlp_pack(const float* params, uint32_t* packed, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3((kMlpPacked + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params, packed);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const uint32_t* packed, const float* feat, int rows, float* probs, hipStream_t s) {
   if (rows == 0) return hipSuccess;
   const int blocks = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks < 1024 ? blocks : 1024), dim3(kBlock), 0, s, params, feat, rows,
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks < 1024 ? blocks : 1024), dim3(kBlock), 0, s, packed, feat, rows,
                      probs);
   return hipGetLastError();
 }
@@ -387,7 +487,7 @@ hipError_t launch_mlp_sgd(float* params, float* velocity, const float* partial, 
 
 hipError_t launch_mlp_candidates(const MlpScoreParams& p, hipStream_t s) {
   if (p.k < 1 || p.k > 64 || p.pad_total % kBlock != 0) return hipErrorInvalidValue;
-  const dim3 grid(256), block(kBlock);
+  const dim3 grid(512), block(kBlock);
   if (p.k <= 8) hipLaunchKernelGGL((mlp_candidates_kernel<8>), grid, block, 0, s, p);
   else if (p.k <= 16) hipLaunchKernelGGL((mlp_candidates_kernel<16>), grid, block, 0, s, p);
   else if (p.k <= 32) hipLaunchKernelGGL((mlp_candidates_kernel<32>), grid, block, 0, s, p);
@@ -396,7 +496,8 @@ hipError_t launch_mlp_candidates(const MlpScoreParams& p, hipStream_t s) {
 }
 
 hipError_t launch_mlp_score(const MlpScoreParams& p, hipStream_t s) {
-  hipLaunchKernelGGL(mlp_score_kernel, dim3(512), dim3(kBlock), 0, s, p);
+  if (p.k < 1 || p.k > kWave || p.pad_total % kBlock != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(mlp_score_kernel, dim3(1024), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }
 

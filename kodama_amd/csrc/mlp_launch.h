@@ -38,9 +38,15 @@ constexpr int kMlpGradWaves = 64;             // waves per training step (fixed 
 hipError_t launch_mlp_features(const int* nbr, const int* labels, int rows, int k, float* feat,
                                unsigned short* ranked, hipStream_t s);
 
+// Parameters -> the per-lane packed form the scoring and forward kernels keep
+// in registers: [3][32][64] bf16 pairs (rows 2q and 2q + 1 of column j of W1,
+// W2, W3), then b1, b2, b3 as fp32 bits.
+constexpr int kMlpPacked = 3 * 32 * 64 + 2 * kMlpHidden + kMlpOut;
+hipError_t launch_mlp_pack(const float* params, uint32_t* packed, hipStream_t s);
+
 // Forward pass of `rows` feature rows with bf16-rounded weights and inputs and
 // FP32 accumulation: probabilities [rows x 32] (masked entries 0).
-hipError_t launch_mlp_forward(const float* params, const float* feat, int rows, float* probs, hipStream_t s);
+hipError_t launch_mlp_forward(const uint32_t* packed, const float* feat, int rows, float* probs, hipStream_t s);
 
 // One minibatch: per-wave partial gradients of the mean cross-entropy of the
 // samples idx[0..count) (targets in 0..30) -> partial [kMlpGradWaves x
@@ -54,12 +60,13 @@ hipError_t launch_mlp_sgd(float* params, float* velocity, const float* partial, 
 
 // Per cycle of the learned device evolution: the persistent kernel lists the
 // rows whose label differs from their prediction (cand_rows, *cand_count);
-// launch_mlp_candidates computes their features (and zeroes *next_count, the
-// counter the next persistent launch appends to), launch_mlp_score runs the
-// network: score = 1 - p(keep) and target = the label of the most probable
-// valid rank (score 0 when no rank is valid).
+// launch_mlp_candidates computes their features (one thread each) and zeroes
+// *next_count, the counter the next persistent launch appends to;
+// launch_mlp_score runs the network (one wave per row): score = 1 - p(keep),
+// target = the label of the most probable valid rank (score 0 when no rank is
+// valid).
 struct MlpScoreParams {
-  const float* params = nullptr;
+  const uint32_t* packed = nullptr;           // launch_mlp_pack of the parameters
   const unsigned short* neighbors = nullptr;  // [k x pad_total] run-local ids (n = none)
   const unsigned short* labels = nullptr;     // [pad_total] current codes
   const int* run_offsets = nullptr;
