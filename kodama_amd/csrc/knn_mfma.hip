@@ -169,23 +169,14 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
     }
   }
   // This lane's output queries.
-  float qn[4], slack[4];
+  float qn[4];
   int excl[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int q = q_tile + 4 * grp + r;
     qn[r] = q < prm.query_rows ? prm.query_norm2[q] : 0.0f;
     excl[r] = (q < prm.query_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
-    slack[r] = 2.0f * (prm.error_scale * (qn[r] + prm.train_norm2_max));
   }
-  // Pruning bound per query: when each of the query's 16 column lists holds
-  // `depth` = ceil(k / 16) values <= B, at least k candidates are <= B, so the
-  // final A_k <= B and a candidate above B + 2E can never be re-scored nor
-  // decide a list overflow; it is skipped. B starts at +inf and only falls.
-  const int depth = (prm.k + 15) / 16;
-  float bound[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) bound[r] = INFINITY;
   float kd[4][kList];
   int ki[4][kList];
 #pragma unroll
@@ -247,21 +238,10 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
         for (int r = 0; r < 4; ++r) {
           if (j != excl[r]) {
             const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
-            if (v <= bound[r]) list_insert(kd[r], ki[r], v, j);
+            list_insert(kd[r], ki[r], v, j);
           }
         }
       }
-    }
-    // Refresh the bounds: the largest depth-th value over the 16 column lists
-    // (lanes of one k group share their queries), plus the slack.
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float b = kd[r][0];
-#pragma unroll
-      for (int s2 = 1; s2 < kList; ++s2) b = s2 == depth - 1 ? kd[r][s2] : b;
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) b = fmaxf(b, __shfl_xor(b, off));
-      bound[r] = fminf(bound[r], b + slack[r]);
     }
   }
 
