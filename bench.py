@@ -273,7 +273,7 @@ def main() -> int:
             if form == "learned" else \
             (lambda o, rr, ii, ff, form=form: K.core_knn_batch(x, rr, ii, o, mode=form, fold_ids=ff))
         if args.warmup > 0:
-            run(K.core_options(cycles=args.warmup, **base), rows[:8], init[:8], fold_ids[:8])
+            run(K.core_options(cycles=args.warmup, **base), rows, init, fold_ids)
         out = run(K.core_options(cycles=args.steps, **base), rows, init, fold_ids)
         sps = args.runs * args.steps / out["evolution_seconds"]
         curves[form] = out["vect_acc"].mean(axis=0)

@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstddef>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -61,6 +63,47 @@ class DeviceBuffer {
  private:
   T* p_ = nullptr;
   size_t n_ = 0;
+};
+
+// Stream-ordered device memory from the device's default memory pool. The
+// pool keeps freed memory (its release threshold is raised once per device),
+// so an evolution that follows another of similar size allocates and frees
+// without a trip to the driver.
+class PoolBuffer {
+ public:
+  PoolBuffer() = default;
+  ~PoolBuffer() { release(); }
+  PoolBuffer(const PoolBuffer&) = delete;
+  PoolBuffer& operator=(const PoolBuffer&) = delete;
+  void allocate(size_t bytes, int device, hipStream_t s) {
+    release();
+    retain_pool(device);
+    check(hipMallocAsync(&p_, bytes, s), "hipMallocAsync");
+    bytes_ = bytes;
+    s_ = s;
+  }
+  void release() {
+    if (p_ != nullptr) (void)hipFreeAsync(p_, s_);
+    p_ = nullptr;
+    bytes_ = 0;
+  }
+  unsigned char* get() const { return static_cast<unsigned char*>(p_); }
+  size_t size() const { return bytes_; }
+
+ private:
+  static void retain_pool(int device) {
+    static std::atomic<bool> done[64];
+    if (device < 0 || device >= 64 || done[device].load(std::memory_order_acquire)) return;
+    hipMemPool_t pool = nullptr;
+    check(hipDeviceGetDefaultMemPool(&pool, device), "hipDeviceGetDefaultMemPool");
+    uint64_t threshold = UINT64_MAX;  // setting it twice from two threads is harmless
+    check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold), "hipMemPoolSetAttribute");
+    done[device].store(true, std::memory_order_release);
+  }
+
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+  hipStream_t s_ = nullptr;
 };
 
 template <class T>

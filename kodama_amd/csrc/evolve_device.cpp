@@ -30,7 +30,7 @@ struct DeviceEvolution::Impl {
   KnnBatch& batch;
   std::vector<IVec> values;
   int runs = 0, classes = 0, cycles_cap = 0;
-  DeviceBuffer<unsigned char> arena;
+  PoolBuffer arena;
   size_t up_begin = 0, up_bytes = 0, down_begin = 0, down_bytes = 0;
   size_t off_best = 0, off_state = 0, off_history = 0;
   EvoParams prm;
@@ -98,7 +98,10 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.up_bytes = off_key + sizeof(uint64_t) * m.runs - off_cur;
   m.down_begin = m.off_best;
   m.down_bytes = lay.total - m.off_best;
-  m.arena.resize(lay.total);
+  int device = 0;
+  check(hipGetDevice(&device), "hipGetDevice");
+  // Rounded up to 2 MiB so that evolutions of similar size reuse one pool block.
+  m.arena.allocate((lay.total + (2u << 20) - 1) / (2u << 20) * (2u << 20), device, s);
   unsigned char* base = m.arena.get();
 
   // Host image of the upload region: starting labels (their predictions start
