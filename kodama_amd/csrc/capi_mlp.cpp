@@ -63,7 +63,7 @@ int kodama_hip_mlp_forward(const float* params, const float* features, int32_t r
     const hipStream_t s = st.get();
     DeviceBuffer<float> d_par(kMlpParams), d_feat(static_cast<size_t>(rows) * kMlpIn),
       d_prob(static_cast<size_t>(rows) * kMlpOut);
-    DeviceBuffer<uint32_t> d_packed(kMlpPacked);
+    DeviceBuffer<uint32_t> d_packed(kMlpTiled);
     d_par.upload(params, kMlpParams, s);
     d_feat.upload(features, d_feat.size(), s);
     check(launch_mlp_pack(d_par.get(), d_packed.get(), s), "mlp_pack");

@@ -36,12 +36,7 @@ struct DeviceEvolution::Impl {
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
-  // Learned proposal buffers, allocated on first use.
-  DeviceBuffer<float> learn_score, learn_feat;
-  DeviceBuffer<unsigned short> learn_target, cand_ranked;
-  DeviceBuffer<int> cand_row, cand_count;  // cand_count: two alternating counters
-  DeviceBuffer<uint32_t> packed;           // network weights in the scoring kernel's layout
-  int learn_parity = 0;
+  DeviceBuffer<uint32_t> packed;  // the learned proposal's weights (launch_mlp_pack tiled layout)
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -175,73 +170,24 @@ void DeviceEvolution::run(int cycles) {
 void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallback) {
   Impl& m = *impl_;
   if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
-  if (m.initialized && m.cand_count.size() == 0) {
-    throw InvalidArgument("the learned proposal continues only an evolution it started.");
-  }
   const hipStream_t s = m.batch.stream();
-  const size_t pad = static_cast<size_t>(m.batch.pad_total());
-  if (m.learn_score.size() != pad) {
-    m.learn_score.resize(pad);
-    m.learn_target.resize(pad);
-    m.learn_feat.resize(pad * kMlpIn);
-    m.cand_ranked.resize(pad * kMlpRanks);
-    m.cand_row.resize(pad);
+  if (evo_lds_bytes(m.classes, m.batch.max_rows(), true) > kEvoMaxLds) {
+    throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
   }
-  if (m.packed.size() == 0) m.packed.resize(kMlpPacked);
+  if (m.packed.size() == 0) m.packed.resize(kMlpTiled);
   check(launch_mlp_pack(d_params, m.packed.get(), s), "mlp_pack");
-  MlpScoreParams sp;
-  sp.packed = m.packed.get();
-  sp.neighbors = m.prm.neighbors;
-  sp.labels = m.prm.cur;
-  sp.run_offsets = m.prm.run_offsets;
-  sp.pad_offsets = m.prm.pad_offsets;
-  sp.block_run = m.batch.d_block_run().get();
-  sp.pad_total = m.prm.pad_total;
-  sp.k = m.prm.k;
-  sp.cand_rows = m.cand_row.get();
-  sp.feat = m.learn_feat.get();
-  sp.cand_ranked = m.cand_ranked.get();
-  sp.score = m.learn_score.get();
-  sp.target = m.learn_target.get();
+  EvoParams p = m.prm;
+  p.init = m.initialized ? 0 : 1;
+  p.cycle_begin = m.done_cycles;
+  p.cycle_count = cycles;
+  p.learn_packed = m.packed.get();
+  p.learn_fallback = fallback;
   EventTimer t;
   t.start(s);
-  if (!m.initialized) {
-    // Evaluation of the starting labels alone: it lists the first candidates.
-    m.cand_count.resize(2);
-    check(launch_evo_zero(m.cand_count.get(), 1, s), "evo_zero");
-    m.learn_parity = 0;
-    EvoParams p = m.prm;
-    p.init = 1;
-    p.cycle_begin = m.done_cycles;
-    p.cycle_count = 0;
-    p.cand_rows = m.cand_row.get();
-    p.cand_count = m.cand_count.get();
-    check(launch_evo_persistent(p, s), "evo_persistent");
-    m.initialized = true;
-  }
-  for (int c = 0; c < cycles; ++c) {
-    // Counters alternate: this cycle's candidates were listed at cur, the
-    // launch below lists the next ones at nxt (zeroed by the features kernel).
-    int* cur = m.cand_count.get() + m.learn_parity;
-    int* nxt = m.cand_count.get() + (m.learn_parity ^ 1);
-    sp.cand_count = cur;
-    sp.next_count = nxt;
-    check(launch_mlp_candidates(sp, s), "mlp_candidates");
-    check(launch_mlp_score(sp, s), "mlp_score");
-    EvoParams p = m.prm;
-    p.init = 0;
-    p.cycle_begin = m.done_cycles + c;
-    p.cycle_count = 1;
-    p.learn_score = m.learn_score.get();
-    p.learn_target = m.learn_target.get();
-    p.learn_fallback = fallback;
-    p.cand_rows = m.cand_row.get();
-    p.cand_count = nxt;
-    check(launch_evo_persistent(p, s), "evo_persistent");
-    m.learn_parity ^= 1;
-  }
+  check(launch_evo_persistent(p, s), "evo_persistent");
   t.stop(s);
   kernel_seconds_ = t.ms() * 1e-3;
+  m.initialized = true;
   m.done_cycles += cycles;
 }
 

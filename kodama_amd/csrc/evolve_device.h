@@ -17,10 +17,11 @@ class DeviceEvolution {
   DeviceEvolution& operator=(const DeviceEvolution&) = delete;
   // Runs `cycles` more cycles of every run in one persistent launch; blocks.
   void run(int cycles);
-  // Runs `cycles` more cycles with the learned proposal: per cycle the network
+  // Runs `cycles` more cycles with the learned proposal (the persistent
+  // kernel's learned variant, one launch): in each learned cycle the network
   // (device params [kMlpParams]) scores the rows whose label differs from
-  // their prediction, then one persistent launch runs the cycle; a cycle falls
-  // back to the uniform proposal with probability `fallback`. Blocks.
+  // their prediction; a cycle falls back to the uniform proposal with
+  // probability `fallback`. Blocks.
   void run_learned(int cycles, const float* d_params, float fallback);
   // Duration of the last launch (HIP events on the batch stream).
   double kernel_seconds() const { return kernel_seconds_; }

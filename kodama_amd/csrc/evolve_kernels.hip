@@ -32,6 +32,8 @@
 #include <cstdint>
 
 #include "evolve_launch.h"
+#include "mlp_device.h"
+#include "mlp_launch.h"
 
 namespace kodama_hip {
 
@@ -40,6 +42,9 @@ namespace {
 constexpr int kThreads = kEvoThreads;
 constexpr int kWave = 64;
 constexpr int kWaves = kThreads / kWave;
+// Workgroup of the learned variant up to 32 neighbours (the largest learned one).
+constexpr int kLearnedThreads = kThreads * 3 / 4;
+constexpr int kLearnedWaves = kLearnedThreads / kWave;
 // Label code of the sentinel slot lab[n] of each label buffer: a missing
 // neighbour points there (local id n) and matches no label (codes <= 1023).
 constexpr uint32_t kSentinel = 0xFFFFu;
@@ -172,6 +177,13 @@ struct Lds {
   int* scal;                    // [16] scalars
   unsigned short* labels;       // [4 x lmax] label buffers 0, 1 then prediction buffers 0, 1
   int lmax;                     // rows of the largest run + the sentinel slot, even
+  // Learned variant only: the network's tiled weights [kMlpTiled] (16-byte
+  // aligned), and per wave (kLearnedWaves) a tile of 16 rows x 64 features
+  // (bf16 bits; first the rows' neighbour codes) and 16 rows x 32 slots
+  // (ranked codes 0..29, the row's index in slot 31).
+  uint32_t* mlp_w;
+  unsigned short* mlp_tile;
+  unsigned short* mlp_rank;
   // Buffer b of the labels / predictions (the current state and the proposal
   // swap on acceptance); plain arithmetic keeps the struct in registers.
   __device__ unsigned short* lab(int b) const { return labels + b * lmax; }
@@ -203,6 +215,10 @@ __device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows) {
   s.scal = s.queue + max_rows;
   s.labels = reinterpret_cast<unsigned short*>(s.scal + 16);
   s.lmax = label_slots(max_rows);
+  const size_t used = static_cast<size_t>(reinterpret_cast<unsigned char*>(s.labels + 4 * s.lmax) - base);
+  s.mlp_w = reinterpret_cast<uint32_t*>(base + ((used + 15) & ~static_cast<size_t>(15)));
+  s.mlp_tile = reinterpret_cast<unsigned short*>(s.mlp_w + kMlpTiled);
+  s.mlp_rank = s.mlp_tile + kLearnedWaves * mlp::kTileRows * kWave;
   return s;
 }
 
@@ -397,11 +413,11 @@ __device__ __forceinline__ double objective(const Lds& s, bool guarded, int C, i
 
 }  // namespace
 
-template <int KMAX, int THREADS>
+template <int KMAX, int THREADS, bool LEARNED>
 __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   // Rows per thread of the largest run, for the register arrays of the propose phase.
   constexpr int kRowsPerThread = (kEvoMaxRows + THREADS - 1) / THREADS;
-  extern __shared__ double lds_raw[];
+  extern __shared__ __align__(16) double lds_raw[];
   const int C = p.classes;
   const Lds s = carve(reinterpret_cast<unsigned char*>(lds_raw), C, p.max_rows);
   const int run = blockIdx.x;
@@ -437,6 +453,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   }
   if (threadIdx.x < 2) s.lab(threadIdx.x)[n] = static_cast<unsigned short>(kSentinel);
   if (threadIdx.x < 4) s.state[threadIdx.x] = p.state[static_cast<size_t>(run) * kEvoStateStride + threadIdx.x];
+  if (LEARNED) {
+    for (int e = threadIdx.x; e < kMlpTiled; e += THREADS) s.mlp_w[e] = p.learn_packed[e];
+  }
   __syncthreads();
 
   // Step -1 evaluates the starting labels in place: their predictions and
@@ -456,21 +475,85 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       {
         const int hi = proposal_hi(n, cycle, p.cycles);
         const int take = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
-        // A learned cycle (the network's scores are given, and the fallback
-        // draw does not pick the uniform proposal) chooses each row with
-        // probability take * score / (sum of scores), capped at 1, and relabels
-        // it to the network's target; otherwise the rows at positions
-        // [0, take) of a keyed permutation are chosen and take their
-        // prediction. Chosen rows are marked in the proposal buffer, which
-        // holds only label codes otherwise.
-        const bool learned = p.learn_score != nullptr && unit(draw(key, cycle, 5)) >= p.learn_fallback;
+        // A learned cycle (learned variant, unless the fallback draw picks the
+        // uniform proposal) first scores the rows whose label differs from
+        // their prediction; the others score 0. Each wave takes 64-row blocks
+        // and their candidates 16 at a time: the tile's neighbour codes are
+        // gathered (all loads in flight together), each row's features are
+        // built by the wave, and one matrix-core forward pass scores the 16
+        // rows. The target label goes straight into the proposal buffer and
+        // the score into the row queue. Each row is then chosen with
+        // probability take * score / (sum of scores), capped at 1; the others
+        // get their label back. Otherwise the rows at positions [0, take) of a
+        // keyed permutation are marked and take their prediction. The proposal
+        // buffer holds only label codes otherwise.
+        const bool learned = LEARNED && unit(draw(key, cycle, 5)) >= p.learn_fallback;
         if (learned) {
-          // Scores are current for the rows whose label differs from their
-          // prediction (scored after the previous launch); other rows keep
-          // stale values and are skipped.
-          const float* score = p.learn_score + base;
+          const int ln = tid & (kWave - 1);  // from opaque_tid: not hoisted out of the cycle loop
+          float* score = reinterpret_cast<float*>(s.queue);
+          unsigned short* tile = s.mlp_tile + wave * mlp::kTileRows * kWave;
+          unsigned short* rk = s.mlp_rank + wave * mlp::kTileRows * 32;
+          const uint64_t below = (1ull << ln) - 1ull;
+          const int g = ln >> 4, slot = ln & 15;
+          for (int b0 = wave * kWave; b0 < n; b0 += THREADS) {  // uniform in the wave
+            const int i = b0 + ln;
+            const bool cand = i < n && cur_lab[i] != cur_pred[i];
+            if (i < n && !cand) score[i] = 0.0f;
+            for (uint64_t left = __ballot(cand); left;) {
+              // The block's next (up to) 16 candidates, in ascending order.
+              const int pos = __popcll(left & below);
+              const bool mine = ((left >> ln) & 1ull) != 0ull && pos < mlp::kTileRows;
+              const uint64_t took = __ballot(mine);
+              left &= ~took;
+              const int cnt = __popcll(took);
+              if (mine) rk[pos * 32 + 31] = static_cast<unsigned short>(i);
+              uint32_t ids[mlp::kTileRows];
+              {
+                uint64_t t = took;
+#pragma unroll
+                for (int m = 0; m < mlp::kTileRows; ++m) {
+                  const int row = b0 + (t ? __ffsll(static_cast<unsigned long long>(t)) - 1 : 0);
+                  t &= t - 1ull;
+                  ids[m] = m < cnt && ln < k ? static_cast<uint32_t>(nbr[static_cast<uint32_t>(ln * pad_total + row)])
+                                               : static_cast<uint32_t>(n);
+                }
+              }
+#pragma unroll
+              for (int m = 0; m < mlp::kTileRows; ++m) {
+                const uint32_t code = ids[m] < static_cast<uint32_t>(n) ? cur_lab[ids[m]] : mlp::kNone;
+                tile[m * kWave + ln] = static_cast<unsigned short>(m < cnt ? code : 0u);
+              }
+              mlp::wave_sync();
+              {
+                uint64_t t = took;
+                for (int m = 0; m < cnt; ++m) {
+                  const int row = b0 + __ffsll(static_cast<unsigned long long>(t)) - 1;
+                  t &= t - 1ull;
+                  const uint32_t code = tile[m * kWave + ln];
+                  mlp::wave_sync();  // every code read before the row becomes features
+                  mlp::wave_row_features(code, k, cur_lab[row], ln, tile + m * kWave, rk + m * 32);
+                }
+              }
+              const unsigned short* frow = tile + slot * kWave;
+              const uint4 x[2] = {*reinterpret_cast<const uint4*>(frow + 8 * g),
+                                  *reinterpret_cast<const uint4*>(frow + 32 + 8 * g)};
+              const uint32_t valid = mlp::tile_valid(frow, ln);
+              float prob[8];
+              mlp::tile_forward(s.mlp_w, x, valid, ln, prob);
+              float sc;
+              int rank;
+              mlp::tile_pick(prob, valid, ln, &sc, &rank);
+              if (g == 0 && slot < cnt) {
+                const int row = rk[slot * 32 + 31];
+                score[row] = sc;
+                if (rank >= 0) prop[row] = rk[slot * 32 + rank];
+              }
+              mlp::wave_sync();  // the tile is reused by the next group
+            }
+          }
+          __syncthreads();
           float part = 0.0f;
-          for (int i = tid; i < n; i += THREADS) part += cur_lab[i] != cur_pred[i] ? score[i] : 0.0f;
+          for (int i = tid; i < n; i += THREADS) part += score[i];
 #pragma unroll
           for (int off = kWave / 2; off >= 1; off >>= 1) part += __shfl_xor(part, off);
           if (lane == 0) s.wave_total[wave] = part;
@@ -480,9 +563,8 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           const uint64_t row_key = draw(key, cycle, 6);
           for (int i = tid; i < n; i += THREADS) {
             const double u = unit(mix64(row_key + static_cast<uint64_t>(i)));
-            if (cur_lab[i] != cur_pred[i] && score[i] > 0.0f && u * total < static_cast<double>(take) * score[i]) {
-              prop[i] = kChosenMark;
-            }
+            const bool chosen = score[i] > 0.0f && u * total < static_cast<double>(take) * score[i];
+            if (!chosen) prop[i] = cur_lab[i];
           }
         } else {
           const Permutation perm(draw(key, cycle, 2), bits);
@@ -494,8 +576,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         for (int i = tid; i < n; i += THREADS) {
           const bool chosen = prop[i] == kChosenMark;
           const int guide = cur_pred[i];
-          const int label = chosen ? (learned ? static_cast<int>(p.learn_target[base + i]) : guide)
-                                   : static_cast<int>(cur_lab[i]);
+          const int label = learned ? static_cast<int>(prop[i]) : (chosen ? guide : static_cast<int>(cur_lab[i]));
           prop[i] = static_cast<unsigned short>(label);
           atomicAdd(&s.sizes[label], 1);
           if (label == guide) atomicAdd(&s.stay[label], 1);
@@ -731,28 +812,11 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     __syncthreads();
   }
 
-  // Write the run state back for the next launch; with the learned proposal,
-  // list the rows whose label differs from their prediction (the rows the
-  // network scores before the next cycle). Uniform trip count for the ballot.
+  // Write the run state back for the next launch.
   const int cur = s.scal[kCur];
-  for (int i0 = 0; i0 < n; i0 += THREADS) {
-    const int i = i0 + static_cast<int>(threadIdx.x);
-    bool cand = false;
-    if (i < n) {
-      const unsigned short l = s.lab(cur)[i], pr = s.pred(cur)[i];
-      p.cur[base + i] = l;
-      p.curpred[base + i] = pr;
-      cand = l != pr;
-    }
-    if (p.cand_rows != nullptr) {
-      const uint64_t mask = __ballot(cand);
-      if (mask != 0ull) {
-        int slot = 0;
-        if (lane == 0) slot = atomicAdd(p.cand_count, __popcll(mask));
-        slot = __shfl(slot, 0) + __popcll(mask & ((1ull << lane) - 1ull));
-        if (cand) p.cand_rows[slot] = base + i;
-      }
-    }
+  for (int i = threadIdx.x; i < n; i += THREADS) {
+    p.cur[base + i] = s.lab(cur)[i];
+    p.curpred[base + i] = s.pred(cur)[i];
   }
   if (threadIdx.x < 4) p.state[static_cast<size_t>(run) * kEvoStateStride + threadIdx.x] = s.state[threadIdx.x];
 }
@@ -777,35 +841,46 @@ __global__ void evo_zero_kernel(int* a, size_t n) {
 
 // ---- launchers --------------------------------------------------------------
 
-size_t evo_lds_bytes(int classes, int max_rows) {
-  return sizeof(double) * (static_cast<size_t>(classes) + kWaves + 4) +
-         sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
-         sizeof(int) * (7 * static_cast<size_t>(classes) + 2 * static_cast<size_t>(max_rows) + 16) +
-         sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
+size_t evo_lds_bytes(int classes, int max_rows, bool learned) {
+  const size_t base = sizeof(double) * (static_cast<size_t>(classes) + kWaves + 4) +
+                      sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
+                      sizeof(int) * (7 * static_cast<size_t>(classes) + 2 * static_cast<size_t>(max_rows) + 16) +
+                      sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
+  if (!learned) return base;
+  return base + 16 + sizeof(uint32_t) * kMlpTiled +
+         sizeof(unsigned short) * kLearnedWaves * mlp::kTileRows * (kWave + 32);
 }
 
 // 1024 threads per workgroup (16 waves hide the gather latency) up to 32
-// neighbours; 512 for 64, whose id and key arrays need 256 registers.
-template <int KMAX>
+// neighbours; 512 for 64, whose id and key arrays need 256 registers; 768 for
+// the learned variant up to 32 (its matrix-core tile pass needs about 140
+// registers, over the 128 of 16 waves).
+template <int KMAX, bool LEARNED>
 static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStream_t s) {
-  constexpr int threads = KMAX <= 32 ? kThreads : kThreads / 2;
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
+  constexpr int threads = KMAX > 32 ? kThreads / 2 : (LEARNED ? kLearnedThreads : kThreads);
+  const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED>);
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads>), dim3(p.runs), dim3(threads), shmem, s, p);
+  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED>), dim3(p.runs), dim3(threads), shmem, s, p);
   return hipGetLastError();
 }
 
+template <bool LEARNED>
+static hipError_t launch_persistent(const EvoParams& p, size_t shmem, hipStream_t s) {
+  if (p.k <= 8) return launch_persistent_k<8, LEARNED>(p, shmem, s);
+  if (p.k <= 16) return launch_persistent_k<16, LEARNED>(p, shmem, s);
+  if (p.k <= 32) return launch_persistent_k<32, LEARNED>(p, shmem, s);
+  return launch_persistent_k<64, LEARNED>(p, shmem, s);
+}
+
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s) {
-  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows);
+  const bool learned = p.learn_packed != nullptr;
+  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned);
   if (shmem > kEvoMaxLds || p.max_rows > kEvoMaxRows || p.classes > 1024 || p.k < 1 || p.k > 64 ||
       p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31)) {
     return hipErrorInvalidValue;
   }
-  if (p.k <= 8) return launch_persistent_k<8>(p, shmem, s);
-  if (p.k <= 16) return launch_persistent_k<16>(p, shmem, s);
-  if (p.k <= 32) return launch_persistent_k<32>(p, shmem, s);
-  return launch_persistent_k<64>(p, shmem, s);
+  return learned ? launch_persistent<true>(p, shmem, s) : launch_persistent<false>(p, shmem, s);
 }
 
 hipError_t launch_evo_narrow(const int* nbr32, const int* run_offsets, const int* pad_offsets, const int* block_run,

@@ -40,19 +40,14 @@ struct EvoParams {
   double* state = nullptr;                   // [runs x kEvoStateStride]
   int* flows = nullptr;                      // [runs x classes x classes], zero between cycles
   double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
-  // Learned proposal (null: every cycle uniform): per padded row the
-  // network's selection score and relabel target, and the probability that a
-  // cycle falls back to the uniform proposal.
-  const float* learn_score = nullptr;
-  const unsigned short* learn_target = nullptr;
+  // Learned proposal (null: every cycle uniform): the network in the tiled
+  // layout of launch_mlp_pack, and the probability that a cycle falls back to
+  // the uniform proposal.
+  const uint32_t* learn_packed = nullptr;
   float learn_fallback = 0.0f;
-  // Candidate list written at the end of a launch (null: none): padded rows
-  // whose label differs from their prediction, appended at *cand_count.
-  int* cand_rows = nullptr;
-  int* cand_count = nullptr;
 };
 
-size_t evo_lds_bytes(int classes, int max_rows);
+size_t evo_lds_bytes(int classes, int max_rows, bool learned = false);
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
 // 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids; a
 // missing neighbour (-1) becomes the run's row count, its sentinel slot.

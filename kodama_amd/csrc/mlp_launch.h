@@ -38,15 +38,21 @@ constexpr int kMlpGradWaves = 64;             // waves per training step (fixed 
 hipError_t launch_mlp_features(const int* nbr, const int* labels, int rows, int k, float* feat,
                                unsigned short* ranked, hipStream_t s);
 
-// Parameters -> the per-lane packed form the scoring and forward kernels keep
-// in registers: [3][32][64] bf16 pairs (rows 2q and 2q + 1 of column j of W1,
-// W2, W3), then b1, b2, b3 as fp32 bits.
-constexpr int kMlpPacked = 3 * 32 * 64 + 2 * kMlpHidden + kMlpOut;
-hipError_t launch_mlp_pack(const float* params, uint32_t* packed, hipStream_t s);
+// Parameters -> the tiled form of the matrix-core forward pass: the A
+// operands of v_mfma_f32_16x16x32_bf16 per lane, 8 bf16 (one uint4) each --
+// layer 1 [4 unit tiles][2 K blocks] (ops 0..7), layer 2 [4][2] (8..15), layer
+// 3 [2 output tiles][2] (16..19), as [op][64 lanes] -- then b1, b2, b3 as fp32.
+// Lane l of op (t, b) holds unit 16t + l % 16 against inputs 32b + 8g + e
+// (layer 1) or 32b + 4g + e, 32b + 16 + 4g + e - 4 (layers 2 and 3), e = 0..7,
+// g = l / 16.
+constexpr int kMlpTileOps = 20;
+constexpr int kMlpTiled = kMlpTileOps * 64 * 4 + 2 * kMlpHidden + kMlpOut;  // 32-bit words
+hipError_t launch_mlp_pack(const float* params, uint32_t* tiled, hipStream_t s);
 
-// Forward pass of `rows` feature rows with bf16-rounded weights and inputs and
-// FP32 accumulation: probabilities [rows x 32] (masked entries 0).
-hipError_t launch_mlp_forward(const uint32_t* packed, const float* feat, int rows, float* probs, hipStream_t s);
+// Forward pass of `rows` feature rows, 16 rows per matrix-core tile, with
+// bf16 weights and inputs and FP32 sums: probabilities [rows x 32] (masked
+// entries 0).
+hipError_t launch_mlp_forward(const uint32_t* tiled, const float* feat, int rows, float* probs, hipStream_t s);
 
 // One minibatch: per-wave partial gradients of the mean cross-entropy of the
 // samples idx[0..count) (targets in 0..30) -> partial [kMlpGradWaves x
@@ -57,32 +63,5 @@ hipError_t launch_mlp_grad(const float* params, const float* feat, const int* ta
 // velocity + grad; params -= lr * velocity. loss_out (may be null) = mean loss.
 hipError_t launch_mlp_sgd(float* params, float* velocity, const float* partial, int count, float lr, float momentum,
                           float* loss_out, hipStream_t s);
-
-// Per cycle of the learned device evolution: the persistent kernel lists the
-// rows whose label differs from their prediction (cand_rows, *cand_count);
-// launch_mlp_candidates computes their features (one thread each) and zeroes
-// *next_count, the counter the next persistent launch appends to;
-// launch_mlp_score runs the network (one wave per row): score = 1 - p(keep),
-// target = the label of the most probable valid rank (score 0 when no rank is
-// valid).
-struct MlpScoreParams {
-  const uint32_t* packed = nullptr;           // launch_mlp_pack of the parameters
-  const unsigned short* neighbors = nullptr;  // [k x pad_total] run-local ids (n = none)
-  const unsigned short* labels = nullptr;     // [pad_total] current codes
-  const int* run_offsets = nullptr;
-  const int* pad_offsets = nullptr;
-  const int* block_run = nullptr;             // [pad_total / 256]
-  int pad_total = 0;
-  int k = 0;
-  const int* cand_rows = nullptr;             // [pad_total] padded rows of the candidates
-  const int* cand_count = nullptr;
-  int* next_count = nullptr;
-  float* feat = nullptr;                      // [pad_total x 64] candidate features
-  unsigned short* cand_ranked = nullptr;      // [pad_total x 30]
-  float* score = nullptr;                     // [pad_total]
-  unsigned short* target = nullptr;           // [pad_total]
-};
-hipError_t launch_mlp_candidates(const MlpScoreParams& p, hipStream_t s);
-hipError_t launch_mlp_score(const MlpScoreParams& p, hipStream_t s);
 
 }  // namespace kodama_hip

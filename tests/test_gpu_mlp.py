@@ -88,8 +88,9 @@ def test_traces_leave_the_plain_trajectory_unchanged():
 
 
 def test_fallback_one_is_the_uniform_device_form():
-    """With every cycle falling back to the uniform proposal, the per-cycle
-    launches of the learned form reproduce the single persistent launch."""
+    """With every cycle falling back to the uniform proposal, the learned
+    variant of the persistent kernel reproduces the uniform one exactly (its
+    extra per-cycle draw is a separate counter stream)."""
     x, rows, init, folds = _centre_batch(n_runs=6)
     o = K.core_options(**OPTS)
     uni = K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=folds)
