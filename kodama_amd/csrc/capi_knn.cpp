@@ -112,10 +112,8 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
       if (vect_acc) std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(), vect_acc + static_cast<size_t>(r) * options->cycles);
     }
   } else {
-    DeviceBuffer<float> d_params(mode == 2 ? kMlpParams : 0);
-    if (mode == 2) d_params.upload(mlp_params, kMlpParams, batch.stream());
     DeviceEvolution dev(batch, values, codes, folds, *options);
-    if (mode == 2) dev.run_learned(options->cycles, d_params.get(), fallback);
+    if (mode == 2) dev.run_learned(options->cycles, mlp_params, fallback);
     else dev.run(options->cycles);
     vote_seconds = dev.kernel_seconds();
     dev.export_results(clbest, accbest, vect_acc);

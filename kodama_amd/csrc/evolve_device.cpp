@@ -32,11 +32,10 @@ struct DeviceEvolution::Impl {
   int runs = 0, classes = 0, cycles_cap = 0;
   PoolBuffer arena;
   size_t up_begin = 0, up_bytes = 0, down_begin = 0, down_bytes = 0;
-  size_t off_best = 0, off_state = 0, off_history = 0;
+  size_t off_best = 0, off_state = 0, off_history = 0, off_params = 0, off_packed = 0;
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
-  DeviceBuffer<uint32_t> packed;  // the learned proposal's weights (launch_mlp_pack tiled layout)
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -89,10 +88,13 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.off_best = lay.take(label_bytes);  // download: best .. history
   m.off_state = lay.take(sizeof(double) * m.runs * kEvoStateStride);
   m.off_history = lay.take(sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap);
+  // The learned proposal's network: fp32 parameters and their tiled form.
+  m.off_params = lay.take(sizeof(float) * kMlpParams);
+  m.off_packed = lay.take(sizeof(uint32_t) * kMlpTiled);
   m.up_begin = off_cur;
   m.up_bytes = off_key + sizeof(uint64_t) * m.runs - off_cur;
   m.down_begin = m.off_best;
-  m.down_bytes = lay.total - m.off_best;
+  m.down_bytes = m.off_history + sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap - m.off_best;
   int device = 0;
   check(hipGetDevice(&device), "hipGetDevice");
   // Rounded up to 2 MiB so that evolutions of similar size reuse one pool block.
@@ -167,20 +169,22 @@ void DeviceEvolution::run(int cycles) {
   m.done_cycles += cycles;
 }
 
-void DeviceEvolution::run_learned(int cycles, const float* d_params, float fallback) {
+void DeviceEvolution::run_learned(int cycles, const float* params, float fallback) {
   Impl& m = *impl_;
   if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
   const hipStream_t s = m.batch.stream();
   if (evo_lds_bytes(m.classes, m.batch.max_rows(), true) > kEvoMaxLds) {
     throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
   }
-  if (m.packed.size() == 0) m.packed.resize(kMlpTiled);
-  check(launch_mlp_pack(d_params, m.packed.get(), s), "mlp_pack");
+  float* d_params = reinterpret_cast<float*>(m.arena.get() + m.off_params);
+  uint32_t* d_packed = reinterpret_cast<uint32_t*>(m.arena.get() + m.off_packed);
+  check(hipMemcpyAsync(d_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s), "upload params");
+  check(launch_mlp_pack(d_params, d_packed, s), "mlp_pack");
   EvoParams p = m.prm;
   p.init = m.initialized ? 0 : 1;
   p.cycle_begin = m.done_cycles;
   p.cycle_count = cycles;
-  p.learn_packed = m.packed.get();
+  p.learn_packed = d_packed;
   p.learn_fallback = fallback;
   EventTimer t;
   t.start(s);

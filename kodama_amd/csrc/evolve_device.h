@@ -19,10 +19,10 @@ class DeviceEvolution {
   void run(int cycles);
   // Runs `cycles` more cycles with the learned proposal (the persistent
   // kernel's learned variant, one launch): in each learned cycle the network
-  // (device params [kMlpParams]) scores the rows whose label differs from
-  // their prediction; a cycle falls back to the uniform proposal with
-  // probability `fallback`. Blocks.
-  void run_learned(int cycles, const float* d_params, float fallback);
+  // (host params [kMlpParams], copied into the arena) scores the rows whose
+  // label differs from their prediction; a cycle falls back to the uniform
+  // proposal with probability `fallback`. Blocks.
+  void run_learned(int cycles, const float* params, float fallback);
   // Duration of the last launch (HIP events on the batch stream).
   double kernel_seconds() const { return kernel_seconds_; }
   void export_results(int* clbest, double* accbest, double* vect_acc);

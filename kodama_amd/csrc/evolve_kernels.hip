@@ -43,7 +43,7 @@ constexpr int kThreads = kEvoThreads;
 constexpr int kWave = 64;
 constexpr int kWaves = kThreads / kWave;
 // Workgroup of the learned variant up to 32 neighbours (the largest learned one).
-constexpr int kLearnedThreads = kThreads * 3 / 4;
+constexpr int kLearnedThreads = kThreads / 2;
 constexpr int kLearnedWaves = kLearnedThreads / kWave;
 // Label code of the sentinel slot lab[n] of each label buffer: a missing
 // neighbour points there (local id n) and matches no label (codes <= 1023).
@@ -178,19 +178,20 @@ struct Lds {
   unsigned short* labels;       // [4 x lmax] label buffers 0, 1 then prediction buffers 0, 1
   int lmax;                     // rows of the largest run + the sentinel slot, even
   // Learned variant only: the network's tiled weights [kMlpTiled] (16-byte
-  // aligned), and per wave (kLearnedWaves) a tile of 16 rows x 64 features
-  // (bf16 bits; first the rows' neighbour codes) and 16 rows x 32 slots
-  // (ranked codes 0..29, the row's index in slot 31).
+  // aligned); per wave (kLearnedWaves) a tile of 16 rows x 64 features (bf16
+  // bits) and 16 rows x 32 slots (ranked codes 0..29, the row's index in slot
+  // 31); bf16(c / k) for c = 0..64.
   uint32_t* mlp_w;
   unsigned short* mlp_tile;
   unsigned short* mlp_rank;
+  unsigned short* mlp_frac;
   // Buffer b of the labels / predictions (the current state and the proposal
   // swap on acceptance); plain arithmetic keeps the struct in registers.
   __device__ unsigned short* lab(int b) const { return labels + b * lmax; }
   __device__ unsigned short* pred(int b) const { return labels + (2 + b) * lmax; }
 };
 
-enum Scalar { kPresent, kPairs, kMatches, kChosen, kVanished, kPick, kDecision, kCur, kQueue };
+enum Scalar { kPresent, kPairs, kMatches, kChosen, kVanished, kPick, kDecision, kCur, kQueue, kCands };
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
 
@@ -219,6 +220,7 @@ __device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows) {
   s.mlp_w = reinterpret_cast<uint32_t*>(base + ((used + 15) & ~static_cast<size_t>(15)));
   s.mlp_tile = reinterpret_cast<unsigned short*>(s.mlp_w + kMlpTiled);
   s.mlp_rank = s.mlp_tile + kLearnedWaves * mlp::kTileRows * kWave;
+  s.mlp_frac = s.mlp_rank + kLearnedWaves * mlp::kTileRows * 32;
   return s;
 }
 
@@ -333,15 +335,21 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   const uint64_t below = (1ull << lane) - 1ull;
   int matches = 0;
   // Uniform trip count: every lane reaches the ballots below. Latency is
-  // hidden by the workgroup's 16 waves rather than by prefetching (which would
-  // cost KMAX more registers per thread).
-  for (int m0 = 0; m0 < npairs; m0 += THREADS) {
-    const int m = m0 + tid;
-    uint32_t ids[KMAX];
-    if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids);
+  // hidden by the workgroup's 16 waves; with 8 waves (and the registers that
+  // leaves) each thread loads two pairs' ids before using either.
+  constexpr int kBatch = THREADS <= 512 && KMAX <= 32 ? 2 : 1;
+  for (int m0 = 0; m0 < npairs; m0 += kBatch * THREADS) {
+    uint32_t ids_b[kBatch][KMAX];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = 2 * m + h;
+    for (int b = 0; b < kBatch; ++b) {
+      const int m = m0 + b * THREADS + tid;
+      if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids_b[b]);
+    }
+#pragma unroll
+    for (int hb = 0; hb < 2 * kBatch; ++hb) {
+      const int h = hb & 1;
+      const uint32_t(&ids)[KMAX] = ids_b[hb >> 1];
+      const int i = 2 * (m0 + (hb >> 1) * THREADS + tid) + h;
       bool queued = false;
       if (i < n) {
         const int own = lab[i];
@@ -455,6 +463,10 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   if (threadIdx.x < 4) s.state[threadIdx.x] = p.state[static_cast<size_t>(run) * kEvoStateStride + threadIdx.x];
   if (LEARNED) {
     for (int e = threadIdx.x; e < kMlpTiled; e += THREADS) s.mlp_w[e] = p.learn_packed[e];
+    if (threadIdx.x <= 64) {
+      s.mlp_frac[threadIdx.x] =
+        static_cast<unsigned short>(mlp::bf16_bits(__fdiv_rn(static_cast<float>(threadIdx.x), static_cast<float>(k))));
+    }
   }
   __syncthreads();
 
@@ -477,63 +489,91 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         const int take = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
         // A learned cycle (learned variant, unless the fallback draw picks the
         // uniform proposal) first scores the rows whose label differs from
-        // their prediction; the others score 0. Each wave takes 64-row blocks
-        // and their candidates 16 at a time: the tile's neighbour codes are
-        // gathered (all loads in flight together), each row's features are
-        // built by the wave, and one matrix-core forward pass scores the 16
-        // rows. The target label goes straight into the proposal buffer and
-        // the score into the row queue. Each row is then chosen with
-        // probability take * score / (sum of scores), capped at 1; the others
-        // get their label back. Otherwise the rows at positions [0, take) of a
-        // keyed permutation are marked and take their prediction. The proposal
-        // buffer holds only label codes otherwise.
+        // their prediction; the others score 0. The candidates are listed
+        // (in the flow-key buffer, free until the flows are counted) and
+        // dealt to the waves in tiles of 16; one lane per row computes the
+        // row's vote (a register sort of its neighbours' codes), and a
+        // matrix-core forward pass scores each tile. The target label
+        // goes straight into the proposal buffer and the score into the row
+        // queue. Each row is then chosen with probability take * score / (sum
+        // of scores), capped at 1; the others get their label back. Otherwise
+        // the rows at positions [0, take) of a keyed permutation are marked
+        // and take their prediction. The proposal buffer holds only label
+        // codes otherwise.
         const bool learned = LEARNED && unit(draw(key, cycle, 5)) >= p.learn_fallback;
         if (learned) {
           const int ln = tid & (kWave - 1);  // from opaque_tid: not hoisted out of the cycle loop
           float* score = reinterpret_cast<float*>(s.queue);
+          int* cands = s.pairs;
+          {
+            const uint64_t below = (1ull << ln) - 1ull;
+            for (int b0 = wave * kWave; b0 < n; b0 += THREADS) {  // uniform in the wave
+              const int i = b0 + ln;
+              const bool cand = i < n && cur_lab[i] != cur_pred[i];
+              if (i < n && !cand) score[i] = 0.0f;
+              const uint64_t mask = __ballot(cand);
+              if (mask) {
+                int at = 0;
+                if (ln == 0) at = atomicAdd(&s.scal[kCands], __popcll(mask));
+                at = __shfl(at, 0);
+                if (cand) cands[at + __popcll(mask & below)] = i;
+              }
+            }
+          }
+          __syncthreads();
+          const int nc = s.scal[kCands];
           unsigned short* tile = s.mlp_tile + wave * mlp::kTileRows * kWave;
           unsigned short* rk = s.mlp_rank + wave * mlp::kTileRows * 32;
-          const uint64_t below = (1ull << ln) - 1ull;
           const int g = ln >> 4, slot = ln & 15;
-          for (int b0 = wave * kWave; b0 < n; b0 += THREADS) {  // uniform in the wave
-            const int i = b0 + ln;
-            const bool cand = i < n && cur_lab[i] != cur_pred[i];
-            if (i < n && !cand) score[i] = 0.0f;
-            for (uint64_t left = __ballot(cand); left;) {
-              // The block's next (up to) 16 candidates, in ascending order.
-              const int pos = __popcll(left & below);
-              const bool mine = ((left >> ln) & 1ull) != 0ull && pos < mlp::kTileRows;
-              const uint64_t took = __ballot(mine);
-              left &= ~took;
-              const int cnt = __popcll(took);
-              if (mine) rk[pos * 32 + 31] = static_cast<unsigned short>(i);
-              uint32_t ids[mlp::kTileRows];
-              {
-                uint64_t t = took;
+          // Tiles of 16 candidates are dealt to the waves in turn; lane group
+          // g of a wave holds tile t0 + g * waves.
+          constexpr int kNw = THREADS / kWave;
+          for (int t0 = wave; t0 * mlp::kTileRows < nc; t0 += 4 * kNw) {  // uniform in the wave
+            const int c = (t0 + g * kNw) * mlp::kTileRows + slot;
+            const bool have = c < nc;
+            const int row = have ? cands[c] : 0;
+            // Neighbour codes, eight loads in flight at a time.
+            uint32_t vkey[KMAX];
 #pragma unroll
-                for (int m = 0; m < mlp::kTileRows; ++m) {
-                  const int row = b0 + (t ? __ffsll(static_cast<unsigned long long>(t)) - 1 : 0);
-                  t &= t - 1ull;
-                  ids[m] = m < cnt && ln < k ? static_cast<uint32_t>(nbr[static_cast<uint32_t>(ln * pad_total + row)])
-                                               : static_cast<uint32_t>(n);
-                }
+            for (int j0 = 0; j0 < KMAX; j0 += 8) {
+#pragma unroll
+              for (int j = j0; j < j0 + 8 && j < KMAX; ++j) {
+                vkey[j] = have && j < k ? static_cast<uint32_t>(nbr[static_cast<uint32_t>(j * pad_total + row)])
+                                        : static_cast<uint32_t>(n);
               }
 #pragma unroll
-              for (int m = 0; m < mlp::kTileRows; ++m) {
-                const uint32_t code = ids[m] < static_cast<uint32_t>(n) ? cur_lab[ids[m]] : mlp::kNone;
-                tile[m * kWave + ln] = static_cast<unsigned short>(m < cnt ? code : 0u);
+              for (int j = j0; j < j0 + 8 && j < KMAX; ++j) {
+                vkey[j] = vkey[j] < static_cast<uint32_t>(n) ? cur_lab[vkey[j]] : mlp::kNone;
+              }
+              __builtin_amdgcn_sched_barrier(0);
+            }
+            const mlp::RowVote vote = mlp::row_vote<KMAX>(vkey, cur_lab[row]);
+            for (int q = 0; q < 4 && (t0 + q * kNw) * mlp::kTileRows < nc; ++q) {  // uniform in the wave
+              const int cnt = min(mlp::kTileRows, nc - (t0 + q * kNw) * mlp::kTileRows);
+              if (g == q) {
+                // This lane's row becomes tile row `slot` (zeros past the list).
+                const auto frac = [&](int c) { return __uint_as_float(static_cast<uint32_t>(s.mlp_frac[c]) << 16); };
+#pragma unroll
+                for (int v4 = 0; v4 < kMlpIn / 8; ++v4) {
+                  uint32_t word[4];
+#pragma unroll
+                  for (int w = 0; w < 4; ++w) {
+                    const int j = 8 * v4 + 2 * w;
+                    const uint32_t lo = mlp::bf16_bits(mlp::feature<KMAX>(vkey, vote, j, frac));
+                    const uint32_t hi = mlp::bf16_bits(mlp::feature<KMAX>(vkey, vote, j + 1, frac));
+                    word[w] = have ? lo | hi << 16 : 0u;
+                  }
+                  *reinterpret_cast<uint4*>(tile + slot * kWave + 8 * v4) = make_uint4(word[0], word[1], word[2], word[3]);
+                  __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int r = 0; r < kMlpRanks; ++r) {
+                  rk[slot * 32 + r] =
+                    static_cast<unsigned short>(r < KMAX && r < vote.distinct ? vkey[r < KMAX ? r : 0] & 0xFFFFu : mlp::kNone);
+                }
+                rk[slot * 32 + 31] = static_cast<unsigned short>(row);
               }
               mlp::wave_sync();
-              {
-                uint64_t t = took;
-                for (int m = 0; m < cnt; ++m) {
-                  const int row = b0 + __ffsll(static_cast<unsigned long long>(t)) - 1;
-                  t &= t - 1ull;
-                  const uint32_t code = tile[m * kWave + ln];
-                  mlp::wave_sync();  // every code read before the row becomes features
-                  mlp::wave_row_features(code, k, cur_lab[row], ln, tile + m * kWave, rk + m * 32);
-                }
-              }
               const unsigned short* frow = tile + slot * kWave;
               const uint4 x[2] = {*reinterpret_cast<const uint4*>(frow + 8 * g),
                                   *reinterpret_cast<const uint4*>(frow + 32 + 8 * g)};
@@ -544,14 +584,15 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
               int rank;
               mlp::tile_pick(prob, valid, ln, &sc, &rank);
               if (g == 0 && slot < cnt) {
-                const int row = rk[slot * 32 + 31];
-                score[row] = sc;
-                if (rank >= 0) prop[row] = rk[slot * 32 + rank];
+                const int r = rk[slot * 32 + 31];
+                score[r] = sc;
+                if (rank >= 0) prop[r] = rk[slot * 32 + rank];
               }
               mlp::wave_sync();  // the tile is reused by the next group
             }
           }
           __syncthreads();
+          if (tid == 0) s.scal[kCands] = 0;
           float part = 0.0f;
           for (int i = tid; i < n; i += THREADS) part += score[i];
 #pragma unroll
@@ -848,7 +889,7 @@ size_t evo_lds_bytes(int classes, int max_rows, bool learned) {
                       sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
   if (!learned) return base;
   return base + 16 + sizeof(uint32_t) * kMlpTiled +
-         sizeof(unsigned short) * kLearnedWaves * mlp::kTileRows * (kWave + 32);
+         sizeof(unsigned short) * (kLearnedWaves * mlp::kTileRows * (kWave + 32) + 66);
 }
 
 // 1024 threads per workgroup (16 waves hide the gather latency) up to 32

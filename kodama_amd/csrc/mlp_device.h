@@ -1,7 +1,8 @@
 // Device code of the learned proposal shared by mlp_kernels.hip (tests,
 // training, the table features) and the learned variant of the persistent
-// evolution kernel (evolve_kernels.hip): the row features computed by one
-// wave, and the forward pass of a tile of 16 rows on the matrix cores.
+// evolution kernel (evolve_kernels.hip): the neighbour vote of a row computed
+// by one lane (from which its features follow), and the forward pass of a
+// tile of 16 rows on the matrix cores.
 //
 // Tile forward pass: every layer is D^T = W^T X^T on v_mfma_f32_16x16x32_bf16
 // with the weights as the A operand (lane l: unit 16t + l % 16 of unit tile t)
@@ -46,59 +47,79 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void put(float* f, int j, float v) { f[j] = v; }
-__device__ __forceinline__ void put(unsigned short* f, int j, float v) {
-  f[j] = static_cast<unsigned short>(bf16_bits(v));
+// Ascending bitonic sort of N (a power of two) values held in registers. A
+// scheduling barrier after each group of four compare-exchanges keeps the
+// sort in place (N + 8 live values); unconstrained interleaving of a stage's
+// exchanges doubles the registers.
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint32_t (&a)[N]) {
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      int done = 0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const uint32_t x = a[i], y = a[j];
+          const uint32_t lo = min(x, y), hi = max(x, y);
+          a[i] = (i & size) == 0 ? lo : hi;
+          a[j] = (i & size) == 0 ? hi : lo;
+          if (++done % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+  }
 }
 
-// Features of one row, computed by one wave: lane j holds the code of
-// neighbour j (kNone when missing or j >= k). The distinct codes are visited
-// once each (the lowest remaining lane leads its code), which gives every
-// lane the count of its code and marks the first lane of each code as its
-// head; a head's key is (255 - count) << 16 | code and its rank among the
-// heads' keys is the code's rank (count desc, code asc). feat[64] (float, or
-// bf16 bits for the tile) and ranked[30] may live in LDS or global memory; the
-// wave writes them and returns after a wave barrier.
-template <class F>
-__device__ __forceinline__ void wave_row_features(uint32_t code, int k, uint32_t own, int lane, F* feat,
-                                                  unsigned short* ranked) {
-  const bool real = code != kNone;
-  int count = 0;
-  bool head = false;
-  for (uint64_t left = __ballot(real); left;) {
-    const int lead = __ffsll(static_cast<unsigned long long>(left)) - 1;
-    const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(code), lead));
-    const uint64_t same = __ballot(code == c);
-    if (code == c) count = __popcll(same);
-    head = head || lane == lead;
-    left &= ~same;
+// The neighbour vote of one row, computed by one lane: a[] holds the codes of
+// its neighbours (kNone when missing; N >= k entries). Sorting the codes gives
+// each voted code's count as a run length; the run ends become keys
+// (255 - count) << 16 | code, in place, and sorting the keys ranks the voted
+// codes by count desc, code asc. On return a[r] is the key of rank r
+// (0xFFFFFFFF past the voted codes).
+struct RowVote {
+  int distinct;   // voted codes
+  int own_rank;   // rank of the row's own label if voted and < 30, else 30
+  int own_count;  // votes for the row's own label
+  int top;        // votes of rank 0 (0 when nothing is voted)
+};
+template <int N>
+__device__ __forceinline__ RowVote row_vote(uint32_t (&a)[N], uint32_t own) {
+  bitonic_sort<N>(a);
+  RowVote v{0, kMlpRanks, 0, 0};
+  int start = 0;
+  uint32_t prev = 0xFFFFFFFFu, cur = a[0];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const uint32_t next = j + 1 < N ? a[j + 1] : 0xFFFFFFFFu;
+    if (cur != prev) start = j;
+    const int len = j - start + 1;
+    const bool voted = cur != next && cur != kNone;
+    a[j] = voted ? (static_cast<uint32_t>(255 - len) << 16) | cur : 0xFFFFFFFFu;
+    v.distinct += voted ? 1 : 0;
+    v.own_count = voted && cur == own ? len : v.own_count;
+    prev = cur;
+    cur = next;
   }
-  const uint32_t key = head ? ((255u - static_cast<uint32_t>(count)) << 16) | code : 0xFFFFFFFFu;
-  const uint64_t heads = __ballot(head);
-  int rank = 0;
-  for (uint64_t h = heads; h; h &= h - 1) {
-    const int at = __ffsll(static_cast<unsigned long long>(h)) - 1;
-    rank += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(key), at)) < key ? 1 : 0;
+  bitonic_sort<N>(a);
+  v.top = a[0] != 0xFFFFFFFFu ? 255 - static_cast<int>(a[0] >> 16) : 0;
+#pragma unroll
+  for (int r = 0; r < (N < kMlpRanks ? N : kMlpRanks); ++r) {
+    if (a[r] != 0xFFFFFFFFu && (a[r] & 0xFFFFu) == own) v.own_rank = r;
   }
-  const int distinct = __popcll(heads);
-  const int own_count = __popcll(__ballot(real && code == own));
-  const uint64_t top_mask = __ballot(head && rank == 0);
-  const int top = top_mask ? __shfl(count, __ffsll(static_cast<unsigned long long>(top_mask)) - 1) : 0;
-  const uint64_t own_mask = __ballot(head && code == own && rank < kMlpRanks);
-  const int own_rank = own_mask ? __shfl(rank, __ffsll(static_cast<unsigned long long>(own_mask)) - 1) : kMlpRanks;
-  put(feat, lane, 0.0f);
-  if (lane >= distinct && lane < kMlpRanks) ranked[lane] = static_cast<unsigned short>(kNone);
-  wave_sync();
-  if (head && rank < kMlpRanks) {
-    put(feat, rank, __fdiv_rn(static_cast<float>(count), static_cast<float>(k)));
-    ranked[rank] = static_cast<unsigned short>(code);
-  }
-  if (lane == 0) {
-    put(feat, kMlpRanks + own_rank, 1.0f);
-    put(feat, 61, __fdiv_rn(static_cast<float>(top - own_count), static_cast<float>(k)));
-    put(feat, 62, 1.0f);
-  }
-  wave_sync();
+  return v;
+}
+
+// Feature j of a row from its vote (see mlp_launch.h), through frac(c) =
+// c / k rounded to FP32 (c = 0..64).
+template <int N, class Frac>
+__device__ __forceinline__ float feature(const uint32_t (&key)[N], const RowVote& v, int j, Frac frac) {
+  if (j < kMlpRanks) return j < N && j < v.distinct ? frac(255 - static_cast<int>(key[j < N ? j : 0] >> 16)) : 0.0f;
+  if (j <= 2 * kMlpRanks) return j - kMlpRanks == v.own_rank ? 1.0f : 0.0f;
+  if (j == 61) return frac(v.top - v.own_count);
+  return j == 62 ? 1.0f : 0.0f;
 }
 
 // Output validity: keep, or an existing rank that is not the row's own label.

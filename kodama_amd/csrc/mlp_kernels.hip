@@ -116,18 +116,29 @@ __global__ void __launch_bounds__(kBlock) mlp_pack_kernel(const float* __restric
 
 // ---- features of caller tables (tests, traces) ---------------------------------
 
-// One wave per row.
+// One lane per row: the row vote of mlp_device.h over N >= k neighbours.
+template <int N>
 __global__ void __launch_bounds__(kBlock) mlp_table_features_kernel(const int* __restrict__ nbr,
                                                                      const int* __restrict__ labels, int rows, int k,
                                                                      float* __restrict__ feat,
                                                                      unsigned short* __restrict__ ranked) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int waves = gridDim.x * kWavesPerBlock;
-  for (int r = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave; r < rows; r += waves) {
-    const int id = lane < k ? nbr[static_cast<size_t>(r) * k + lane] : -1;
-    const uint32_t code = id >= 0 && id < rows ? static_cast<uint32_t>(labels[id]) : mlp::kNone;
-    mlp::wave_row_features(code, k, static_cast<uint32_t>(labels[r]), lane, feat + static_cast<size_t>(r) * kMlpIn,
-                           ranked + static_cast<size_t>(r) * kMlpRanks);
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  uint32_t key[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const int id = j < k ? nbr[static_cast<size_t>(r) * k + j] : -1;
+    key[j] = id >= 0 && id < rows ? static_cast<uint32_t>(labels[id]) : mlp::kNone;
+  }
+  const mlp::RowVote v = mlp::row_vote<N>(key, static_cast<uint32_t>(labels[r]));
+  const auto frac = [k](int c) { return __fdiv_rn(static_cast<float>(c), static_cast<float>(k)); };
+  float* f = feat + static_cast<size_t>(r) * kMlpIn;
+#pragma unroll
+  for (int j = 0; j < kMlpIn; ++j) f[j] = mlp::feature<N>(key, v, j, frac);
+#pragma unroll
+  for (int q = 0; q < kMlpRanks; ++q) {
+    ranked[static_cast<size_t>(r) * kMlpRanks + q] =
+      static_cast<unsigned short>(q < N && q < v.distinct ? key[q < N ? q : 0] & 0xFFFFu : mlp::kNone);
   }
 }
 
@@ -260,9 +271,16 @@ hipError_t launch_mlp_features(const int* nbr, const int* labels, int rows, int 
                                unsigned short* ranked, hipStream_t s) {
   if (k < 1 || k > kWave) return hipErrorInvalidValue;
   if (rows == 0) return hipSuccess;
-  const int blocks = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
-  hipLaunchKernelGGL(mlp_table_features_kernel, dim3(blocks < 2048 ? blocks : 2048), dim3(kBlock), 0, s, nbr, labels,
-                     rows, k, feat, ranked);
+  const dim3 grid((rows + kBlock - 1) / kBlock);
+  if (k <= 8) {
+    hipLaunchKernelGGL(mlp_table_features_kernel<8>, grid, dim3(kBlock), 0, s, nbr, labels, rows, k, feat, ranked);
+  } else if (k <= 16) {
+    hipLaunchKernelGGL(mlp_table_features_kernel<16>, grid, dim3(kBlock), 0, s, nbr, labels, rows, k, feat, ranked);
+  } else if (k <= 32) {
+    hipLaunchKernelGGL(mlp_table_features_kernel<32>, grid, dim3(kBlock), 0, s, nbr, labels, rows, k, feat, ranked);
+  } else {
+    hipLaunchKernelGGL(mlp_table_features_kernel<64>, grid, dim3(kBlock), 0, s, nbr, labels, rows, k, feat, ranked);
+  }
   return hipGetLastError();
 }
 
