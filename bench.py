@@ -16,11 +16,11 @@ Three forms run on the same inputs:
   device  proposal, absorption, vote, objective and acceptance on the device:
           one persistent launch (one workgroup per run, run state in LDS) for
           all cycles, counter-based random streams;
-  learned the device form with the learned proposal: per cycle the network
-          scores the rows whose label differs from their prediction, then one
-          persistent launch runs the cycle (a quarter of the cycles stay
-          uniform). The network is first trained, untimed, on the plain
-          form's accept/reject traces over 16 other runs.
+  learned the device form with the learned proposal, also one persistent
+          launch: in each learned cycle the network scores the rows whose
+          label differs from their prediction inside the kernel (a quarter of
+          the cycles stay uniform). The network is first trained, untimed, on
+          the plain form's accept/reject traces over 16 other runs.
 `value` is the device form's steps/s; every form's mean final CV accuracy is
 reported ("equal accuracy" = within 0.5 points), with the cycles each form
 needs to reach the plain form's final accuracy minus half a point.

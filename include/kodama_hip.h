@@ -303,7 +303,9 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p,
  * its own label's rank (60: not voted), 61 the margin (top - own count) / k,
  * 62 a constant 1. Outputs: j < 30 relabel to the label of rank j, 30 keep;
  * ranks that do not exist or hold the row's own label are masked. Weights
- * and inputs are rounded to bf16, sums are FP32. params: fp32 [10400] =
+ * and inputs are rounded to bf16, sums are FP32 (the forward pass runs 16 rows
+ * per matrix-core tile, hidden activations entering as three bf16 parts, which
+ * gives the FP32 sums up to their order). params: fp32 [10400] =
  * W1[64x64] b1[64] W2[64x64] b2[64] W3[64x32] b3[32], W row-major [in][out]. */
 int kodama_hip_mlp_init(uint64_t seed, float* params);
 
@@ -342,11 +344,14 @@ int kodama_hip_mlp_train(const float* features, const int32_t* targets, int32_t 
                          int32_t device, float* params, double* loss_per_epoch);
 
 /* kodama_hip_core_knn_batch in the device-resident form with the learned
- * proposal: each cycle the network scores the rows whose label differs from
- * their prediction; the persistent kernel chooses rows with probability
- * take x score / (sum of scores) and relabels them to the network's target;
- * with probability `fallback` a cycle uses the uniform proposal instead. The
- * CV accuracy stays the only acceptance criterion. Outputs as the batch. */
+ * proposal, one persistent launch for all cycles: in each cycle the kernel
+ * scores the rows whose label differs from their prediction with the network,
+ * chooses rows with probability take x score / (sum of scores) and relabels
+ * them to the network's target; with probability `fallback` a cycle uses the
+ * uniform proposal instead. The CV accuracy stays the only acceptance
+ * criterion. EINVAL when the network (21 KB) and the per-wave tiles (24 KB)
+ * do not fit one CU's LDS beside the largest run's state. Outputs as the
+ * batch. */
 int kodama_hip_core_knn_batch_learned(const float* x, int32_t n, int32_t p, int32_t runs,
                                       const int32_t* run_offsets, const int32_t* landmark_rows,
                                       const int32_t* initial_labels, const int32_t* fold_ids,
