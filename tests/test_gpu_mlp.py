@@ -14,7 +14,7 @@ from test_gpu_device_evolution import OPTS, _centre_batch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("k", [6, 30, 64])
+@pytest.mark.parametrize("k", [6, 12, 30, 64])
 def test_features_match_oracle(k):
     """Bit-identical features and ranked labels: missing neighbours, ties of
     counts (smaller code first), own label voted or not, more than 30 voted labels."""
@@ -99,17 +99,30 @@ def test_fallback_one_is_the_uniform_device_form():
     assert np.array_equal(uni["vect_acc"], lrn["vect_acc"])
 
 
-def test_learned_accuracy_is_the_vote_of_the_returned_labels(oracle):
+@pytest.mark.parametrize("k", [5, 15, 30, 40])
+def test_learned_accuracy_is_the_vote_of_the_returned_labels(oracle, k):
+    """Every width of the learned variant (8, 16, 32 and 64 neighbour slots)."""
     x, rows, init, folds = _centre_batch(n_runs=6)
-    o = K.core_options(**OPTS)
+    o = K.core_options(**dict(OPTS, k=k))
     out = K.core_knn_batch_learned(x, rows, init, o, K.mlp_init(2), fallback=0.25, fold_ids=folds)
     for r in range(len(rows)):
-        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], OPTS["k"])
+        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], k)
         cl = out["clbest"][r]
         pred = oracle.knn_vote(nbr, score, folds[r], cl)
         assert out["accbest"][r] == np.mean(pred == cl), r
         assert set(np.unique(cl)) <= set(np.unique(init[r]))
         assert np.all(np.diff(out["vect_acc"][r]) >= 0)
+
+
+def test_learned_is_deterministic():
+    """The candidate list is filled by atomics in any order; each row's score
+    depends only on its own features, so two launches agree exactly."""
+    x, rows, init, folds = _centre_batch(n_runs=6)
+    o = K.core_options(**OPTS)
+    a = K.core_knn_batch_learned(x, rows, init, o, K.mlp_init(3), fallback=0.25, fold_ids=folds)
+    b = K.core_knn_batch_learned(x, rows, init, o, K.mlp_init(3), fallback=0.25, fold_ids=folds)
+    assert all(np.array_equal(u, v) for u, v in zip(a["clbest"], b["clbest"]))
+    assert np.array_equal(a["vect_acc"], b["vect_acc"])
 
 
 def test_learned_proposal_gains_over_uniform():
