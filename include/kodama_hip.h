@@ -112,6 +112,33 @@ const char* kodama_hip_version(void);
  * -1 when the exact scan served every query. Results are identical either way. */
 int kodama_hip_last_knn_fallbacks(void);
 
+/* Per-fold kNN tables of a batch of runs: replaces precompute_knn_cv_cuda
+ * (core.cpp:694-782), which builds one run's tables, for `runs` runs at once.
+ * Run r owns rows[run_offsets[r] .. run_offsets[r + 1]) (data row ids) with
+ * fold ids fold_ids (same indexing); each row's k nearest rows of its run in
+ * another fold, rows prepared as CoreKNN prepares them (cosine rows
+ * normalised, core.cpp:159-193). Output [total x k] row-major: run-local row
+ * ids in (distance, id) order and scores = minus the internal distance, -1 /
+ * -inf past the available rows -- the reference's per-fold exact tables,
+ * bit-identical. path: 1 the register scan in the reference's summation
+ * order; 2 split-BF16 MFMA candidates with the exact FP32 rerank (p <= 64,
+ * k <= 32; otherwise the scan); 0 the process-wide setting below. Both paths
+ * give identical tables. *path_used (may be NULL) = the path that ran. */
+int kodama_hip_knn_fold_tables(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
+                               const int32_t* rows, const int32_t* fold_ids, int32_t k, int32_t metric,
+                               int32_t path, int32_t device, int32_t* neighbors, float* scores,
+                               int32_t* path_used);
+
+/* No counterpart in the reference: the process-wide path of the per-fold
+ * tables that every kNN classifier call builds (KNNCV, CoreKNN, the batches,
+ * KODAMA.matrix): 1 scan, 2 MFMA, 0 (default) tuned -- the first table of a
+ * shape bucket (columns, k class, metric, log2 of the largest run and of all
+ * rows, folds) runs both paths, timed, and the faster serves the bucket for
+ * the rest of the process. kodama_hip_last_fold_table_path returns the path
+ * the calling thread's last table build used (0 before any). */
+int kodama_hip_set_fold_table_path(int32_t path);
+int kodama_hip_last_fold_table_path(void);
+
 /* ---- device seam: exact kNN ------------------------------------------------
  * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).
  * train [train_rows x dimensions], query [query_rows x dimensions] row-major

@@ -434,6 +434,41 @@ def last_knn_fallbacks() -> int:
     return int(N.lib().kodama_hip_last_knn_fallbacks())
 
 
+_TABLE_PATHS = {"tuned": 0, "scan": 1, "mfma": 2}
+
+
+def knn_fold_tables(data, run_rows, fold_ids, k, metric="euclidean", path="tuned", gpu_device=0):
+    """Per-fold kNN tables of a batch of runs (precompute_knn_cv, core.cpp:694-782,
+    for all runs at once). run_rows / fold_ids: one array per run. Returns
+    (neighbours, scores, path used): lists of [L_r x k] run-local ids (-1
+    missing) and scores (minus the internal distance)."""
+    x = _matrix(data)
+    M = len(run_rows)
+    offsets = np.zeros(M + 1, np.int32)
+    offsets[1:] = np.cumsum([len(r) for r in run_rows])
+    rows = N.i32(np.concatenate([np.asarray(r) for r in run_rows]))
+    folds = N.i32(np.concatenate([np.asarray(f) for f in fold_ids]))
+    total = int(offsets[-1])
+    nbr = np.zeros((total, k), np.int32)
+    score = np.zeros((total, k), np.float32)
+    used = C.c_int32(0)
+    N.call("kodama_hip_knn_fold_tables", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+           N.ptr(offsets), N.ptr(rows), N.ptr(folds), C.c_int32(k), C.c_int32(_metric(metric)),
+           C.c_int32(_TABLE_PATHS[path]), C.c_int32(gpu_device), N.ptr(nbr), N.ptr(score), C.byref(used))
+    split = lambda a: [a[offsets[r]:offsets[r + 1]] for r in range(M)]
+    return split(nbr), split(score), {1: "scan", 2: "mfma"}[used.value]
+
+
+def set_fold_table_path(path="tuned"):
+    """Process-wide path of the per-fold tables ("tuned", "scan" or "mfma")."""
+    N.call("kodama_hip_set_fold_table_path", C.c_int32(_TABLE_PATHS[path]))
+
+
+def last_fold_table_path():
+    """Path of this thread's last per-fold table build ("scan", "mfma"; None before any)."""
+    return {0: None, 1: "scan", 2: "mfma"}[int(N.lib().kodama_hip_last_fold_table_path())]
+
+
 def KODAMA_dissimilarity(indices, distances, res, gpu_device=0):
     """KODAMADissimilarityInPlace on a 1-based graph; returns corrected copies."""
     idx = np.array(indices, dtype=np.int32, order="C", copy=True)

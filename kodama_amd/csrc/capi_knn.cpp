@@ -353,4 +353,42 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
   });
 }
 
+// precompute_knn_cv_cuda (core.cpp:694-782) for a batch of runs.
+int kodama_hip_knn_fold_tables(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
+                               const int32_t* rows, const int32_t* fold_ids, int32_t k, int32_t metric,
+                               int32_t path, int32_t device, int32_t* neighbors, float* scores,
+                               int32_t* path_used) {
+  return guarded([&] {
+    require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
+    require(runs >= 1, "runs must be positive.");
+    require(run_offsets != nullptr && rows != nullptr && fold_ids != nullptr, "null input.");
+    require(neighbors != nullptr && scores != nullptr, "null output.");
+    require(path >= 0 && path <= 2, "path must be 0 (tuned), 1 (scan) or 2 (MFMA).");
+    select_device(device);
+    const IVec offsets(run_offsets, run_offsets + runs + 1);
+    require(offsets[0] == 0, "run_offsets[0] must be 0.");
+    for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r], "every run needs at least one row.");
+    const int total = offsets[runs];
+    const IVec rw(rows, rows + total), folds(fold_ids, fold_ids + total);
+    for (int v : rw) require(v >= 0 && v < n, "run row outside the data matrix.");
+    for (int f : folds) require(f >= 0, "invalid fold id.");
+    const std::vector<float> xs = prepare_rows(x, n, p, metric, 0);
+    KnnBatch batch(device, xs.data(), n, p, offsets, rw, folds, k, metric, 1, path);
+    for (int r = 0; r < runs; ++r) {
+      batch.download_tables(r, neighbors + static_cast<size_t>(offsets[r]) * k,
+                            scores + static_cast<size_t>(offsets[r]) * k);
+    }
+    if (path_used) *path_used = last_fold_table_path();
+  });
+}
+
+int kodama_hip_set_fold_table_path(int32_t path) {
+  return guarded([&] {
+    require(path >= 0 && path <= 2, "path must be 0 (tuned), 1 (scan) or 2 (MFMA).");
+    set_fold_table_path(path);
+  });
+}
+
+int kodama_hip_last_fold_table_path(void) { return last_fold_table_path(); }
+
 }  // extern "C"

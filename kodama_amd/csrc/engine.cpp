@@ -12,6 +12,8 @@
 
 #include <omp.h>
 
+#include "capi_common.h"
+#include "knn_mfma.h"
 #include "launch.h"
 
 namespace kodama_hip {
@@ -695,7 +697,7 @@ KnnBatch::KnnBatch(int device, const int* neighbors, const float* scores, int sa
 }
 
 KnnBatch::KnnBatch(int device, const float* x, int n, int p, const IVec& run_offsets, const IVec& rows,
-                   const IVec& folds, int k, int metric, int classes)
+                   const IVec& folds, int k, int metric, int classes, int table_path)
     : stream_(device), runs_(static_cast<int>(run_offsets.size()) - 1), k_(k), classes_(classes),
       run_offsets_(run_offsets) {
   if (k < 1) throw InvalidArgument("KNNOptions::k must be positive.");
@@ -714,18 +716,24 @@ KnnBatch::KnnBatch(int device, const float* x, int n, int p, const IVec& run_off
   d_rows.upload(rows.data(), rows.size(), s);
   d_folds.upload(folds.data(), folds.size(), s);
 
-  KnnTopkParams prm;
-  prm.train = d_x.get();
-  prm.dims = p;
-  prm.k = k;
-  prm.rows = d_rows.get();
-  prm.folds = d_folds.get();
-  prm.run_offsets = d_run_off_.get();
-  prm.pad_offsets = d_pad_off_.get();
-  prm.pad_total = pad_total_;
-  prm.out_index = d_nbr_.get();
-  prm.out_dist = d_score_.get();
-  check(launch_knn_topk(prm, 1, metric == KODAMA_HIP_EUCLIDEAN, max_rows, runs_, s), "knn_topk launch");
+  FoldTableJob job;
+  job.d_x = d_x.get();
+  job.p = p;
+  job.k = k;
+  job.metric = metric;
+  job.d_rows = d_rows.get();
+  job.d_folds = d_folds.get();
+  job.run_offsets = run_offsets;
+  job.d_run_offsets = d_run_off_.get();
+  job.d_pad_offsets = d_pad_off_.get();
+  job.pad_total = pad_total_;
+  job.max_rows = max_rows;
+  for (int f : folds) job.folds = std::max(job.folds, f + 1);
+  job.norm2_max = max_row_norm2(x, n, p);
+  job.path = table_path;
+  job.d_nbr = d_nbr_.get();
+  job.d_score = d_score_.get();
+  device_fold_tables(job, s);
   find_empty_rows();
 }
 

@@ -119,8 +119,9 @@ class KnnBatch {
  public:
   // x: prepared rows [n x p] (host). Run r owns rows[run_offsets[r] ..
   // run_offsets[r+1]) (data row ids) with fold ids `folds` (same indexing).
+  // table_path: how the per-fold tables are built (device_fold_tables).
   KnnBatch(int device, const float* x, int n, int p, const IVec& run_offsets, const IVec& rows,
-           const IVec& folds, int k, int metric, int classes);
+           const IVec& folds, int k, int metric, int classes, int table_path = 0);
   // One run from caller tables [samples x k] (local ids, -1 / -inf padded).
   KnnBatch(int device, const int* neighbors, const float* scores, int samples, int k, int classes);
 

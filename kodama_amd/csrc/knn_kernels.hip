@@ -129,9 +129,9 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     t_count = prm.slice_rows > 0 ? min(prm.slice_rows, prm.train_rows - t_begin) : prm.train_rows;
   } else {
     q_begin = prm.run_offsets[run];
-    q_count = prm.run_offsets[run + 1] - q_begin;
     t_begin = q_begin;
-    t_count = q_count;
+    t_count = prm.run_offsets[run + 1] - q_begin;
+    q_count = prm.qlist != nullptr ? prm.qlist_offsets[run + 1] - prm.qlist_offsets[run] : t_count;
   }
   const int qi = blockIdx.x * kBlock + threadIdx.x;
   if (blockIdx.x * kBlock >= q_count) return;  // whole block idle for this run
@@ -139,15 +139,17 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
   const int p = prm.dims;
   const int k = prm.k;
 
+  // MODE 1: the run-local row of this query (all rows, or the listed ones).
+  const int lq = MODE == 1 && active && prm.qlist != nullptr ? prm.qlist[prm.qlist_offsets[run] + qi] : qi;
   float q[PMAX];
   int my_fold = -1;
   int excluded = -1;
   if (active) {
-    const int qrow = MODE == 0 ? qi : prm.rows[q_begin + qi];
+    const int qrow = MODE == 0 ? qi : prm.rows[q_begin + lq];
     const float* src = (MODE == 0 ? prm.query : prm.train) + static_cast<size_t>(qrow) * p;
 #pragma unroll
     for (int d = 0; d < PMAX; ++d) q[d] = d < p ? src[d] : 0.0f;
-    if (MODE == 1) my_fold = prm.folds[q_begin + qi];
+    if (MODE == 1) my_fold = prm.folds[q_begin + lq];
     if (MODE == 0 && prm.exclude != nullptr) excluded = prm.exclude[qi];
   } else {
 #pragma unroll
@@ -206,7 +208,7 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     }
   } else {
     // Transposed, padded vote layout: column j of run r at padded row.
-    const int prow = prm.pad_offsets[run] + qi;
+    const int prow = prm.pad_offsets[run] + lq;
     const int pad_base = prm.pad_offsets[run];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstddef>
+#include <vector>
 
 namespace kodama_hip {
 
@@ -28,6 +29,11 @@ struct KnnMfmaParams {
   int dims = 0;
   int k = 0;
   const int* exclude = nullptr;          // [query_rows] train index to skip, may be null
+  // Per-fold tables of a batch of runs (launch_knn_mfma_tables): query = train
+  // = the concatenated rows of all runs; each row's candidates are the rows of
+  // its run in another fold. Indices stay concatenated.
+  const int* run_offsets = nullptr;      // [runs + 1]
+  const int* folds = nullptr;            // [rows]
   float* cand_value = nullptr;           // [query_rows x 16 x kMfmaListLen]
   int* cand_index = nullptr;
   int* flags = nullptr;                  // [query_rows] 1 = needs the exact scan
@@ -38,6 +44,10 @@ struct KnnMfmaParams {
 hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi, unsigned short* lo, float* norm2,
                              hipStream_t s);
 hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s);
+hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int runs, int max_rows, hipStream_t s);
+hipError_t launch_publish_tables(const int* idx, const float* dist, const int* flags, const int* run_offsets,
+                                 int runs, const int* pad_offsets, int pad_total, int rows, int k, int* out_index,
+                                 float* out_score, hipStream_t s);
 // rows[i] of src [* x p] -> dst [count x p]; and the inverse scatter of [count x k] results.
 hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s);
 hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, const int* rows, int count, int k,
@@ -51,5 +61,34 @@ hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, con
 void device_exact_knn(const float* d_train, int train_rows, const float* d_query, int query_rows, int dims, int k,
                       int metric, const int* d_exclude, int* d_idx, float* d_dist, float train_norm2_max,
                       hipStream_t s, int* mfma_fallbacks = nullptr);
+
+// Per-fold kNN tables of a batch of runs (precompute_knn_cv_cuda,
+// core.cpp:694-782, for all runs at once) into the transposed padded vote
+// layout [k x pad_total] (padded row ids, scores = minus the internal
+// distance; -1 / -inf past the available rows). Two paths give identical
+// tables: 1 the register scan, 2 split-BF16 MFMA candidates with the exact
+// rerank (p <= 64, k <= 32; flagged rows rescanned). path 0 follows the
+// process-wide setting (set_fold_table_path), whose 0 tunes: on the first
+// table of a shape bucket (p, k class, metric, log2 of the largest run and of
+// all rows, folds) both paths run and timed, and the faster serves that bucket
+// from then on. Returns the path used.
+struct FoldTableJob {
+  const float* d_x = nullptr;       // prepared rows [n x p]
+  int p = 0, k = 0, metric = 0;
+  const int* d_rows = nullptr;      // concatenated data rows of the runs
+  const int* d_folds = nullptr;     // fold of each concatenated row
+  std::vector<int> run_offsets;     // host copy, [runs + 1]
+  const int* d_run_offsets = nullptr;
+  const int* d_pad_offsets = nullptr;
+  int pad_total = 0, max_rows = 0, folds = 0;
+  float norm2_max = 0.0f;           // bound of the rows' squared norms
+  int path = 0;
+  int* d_nbr = nullptr;
+  float* d_score = nullptr;
+};
+int device_fold_tables(const FoldTableJob& job, hipStream_t s);
+void set_fold_table_path(int path);
+int fold_table_path();
+int last_fold_table_path();
 
 }  // namespace kodama_hip

@@ -142,15 +142,24 @@ __global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, 
   norm2[r] = s;
 }
 
-template <bool EUCLID>
+template <bool EUCLID, bool TABLES>
 __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
   __shared__ unsigned short s_hi[kChunk * kRowStride];
   __shared__ unsigned short s_lo[kChunk * kRowStride];
   __shared__ float s_tn[kChunk];
+  __shared__ int s_fold[kChunk];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const int grp = lane >> 4;       // k group of the operands, query group of the output
   const int col = lane & 15;       // A row / B column / output column
+  // Query and train rows: the two sets, or (TABLES) the rows of run
+  // blockIdx.y on both sides, pairs of one fold skipped.
+  int q_base = 0, q_rows = prm.query_rows, t_base = 0, t_rows = prm.train_rows;
+  if (TABLES) {
+    q_base = t_base = prm.run_offsets[blockIdx.y];
+    q_rows = t_rows = prm.run_offsets[blockIdx.y + 1] - q_base;
+    if (static_cast<int>(blockIdx.x) * kQueriesPerBlock >= q_rows) return;  // whole block idle for this run
+  }
   const int q_tile = blockIdx.x * kQueriesPerBlock + wave * 16;
 
   // A operands: query q_tile + col, dimensions kb * 32 + 8 grp .. + 7.
@@ -159,23 +168,29 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
     const int q = q_tile + col;
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-      if (q < prm.query_rows) {
-        a_hi[kb] = *reinterpret_cast<const uint4*>(prm.query_hi + static_cast<size_t>(q) * 64 + kb * 32 + 8 * grp);
-        a_lo[kb] = *reinterpret_cast<const uint4*>(prm.query_lo + static_cast<size_t>(q) * 64 + kb * 32 + 8 * grp);
+      if (q < q_rows) {
+        const size_t o = static_cast<size_t>(q_base + q) * 64 + kb * 32 + 8 * grp;
+        a_hi[kb] = *reinterpret_cast<const uint4*>(prm.query_hi + o);
+        a_lo[kb] = *reinterpret_cast<const uint4*>(prm.query_lo + o);
       } else {
         a_hi[kb] = make_uint4(0, 0, 0, 0);
         a_lo[kb] = make_uint4(0, 0, 0, 0);
       }
     }
   }
-  // This lane's output queries.
+  // This lane's output queries: norms, and the excluded train index (or, in
+  // TABLES, the query's fold).
   float qn[4];
   int excl[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int q = q_tile + 4 * grp + r;
-    qn[r] = q < prm.query_rows ? prm.query_norm2[q] : 0.0f;
-    excl[r] = (q < prm.query_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
+    qn[r] = q < q_rows ? prm.query_norm2[q_base + q] : 0.0f;
+    if (TABLES) {
+      excl[r] = q < q_rows ? prm.folds[q_base + q] : -1;
+    } else {
+      excl[r] = (q < q_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
+    }
   }
   float kd[4][kList];
   int ki[4][kList];
@@ -188,7 +203,7 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
     }
   }
 
-  for (int c0 = 0; c0 < prm.train_rows; c0 += kChunk) {
+  for (int c0 = 0; c0 < t_rows; c0 += kChunk) {
     __syncthreads();
     for (int e = threadIdx.x; e < kChunk * 16; e += kCandThreads) {
       // 64 rows x (8 hi + 8 lo uint4 parts).
@@ -197,14 +212,16 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
       const bool is_lo = part >= 8;
       const int d8 = (part & 7) * 8;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (tr < prm.train_rows) {
-        const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(tr) * 64 + d8;
+      if (tr < t_rows) {
+        const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(t_base + tr) * 64 + d8;
         v = *reinterpret_cast<const uint4*>(src);
       }
       *reinterpret_cast<uint4*>((is_lo ? s_lo : s_hi) + row * kRowStride + d8) = v;
     }
     if (threadIdx.x < kChunk) {
-      s_tn[threadIdx.x] = c0 + static_cast<int>(threadIdx.x) < prm.train_rows ? prm.train_norm2[c0 + threadIdx.x] : 0.0f;
+      const int tr = c0 + static_cast<int>(threadIdx.x);
+      s_tn[threadIdx.x] = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
+      if (TABLES) s_fold[threadIdx.x] = tr < t_rows ? prm.folds[t_base + tr] : -2;
     }
     __syncthreads();
     f32x4 acc[4];
@@ -232,13 +249,14 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) {
       const int j = c0 + tt * 16 + col;
-      if (j < prm.train_rows) {
+      if (j < t_rows) {
         const float tn = s_tn[tt * 16 + col];
+        const int skip = TABLES ? s_fold[tt * 16 + col] : j;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if (j != excl[r]) {
+          if (skip != excl[r]) {
             const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
-            list_insert(kd[r], ki[r], v, j);
+            list_insert(kd[r], ki[r], v, t_base + j);
           }
         }
       }
@@ -249,8 +267,8 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int q = q_tile + 4 * grp + r;
-    if (q < prm.query_rows) {
-      const size_t base = (static_cast<size_t>(q) * 16 + col) * kList;
+    if (q < q_rows) {
+      const size_t base = (static_cast<size_t>(q_base + q) * 16 + col) * kList;
 #pragma unroll
       for (int s = 0; s < kList; ++s) {
         prm.cand_value[base + s] = kd[r][s];
@@ -367,10 +385,29 @@ hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s)
   const unsigned blocks = static_cast<unsigned>((prm.query_rows + kQueriesPerBlock - 1) / kQueriesPerBlock);
   const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
   if (euclid) {
-    hipLaunchKernelGGL(knn_mfma_candidates_kernel<true>, dim3(blocks), dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false>), dim3(blocks), dim3(kCandThreads), 0, s, prm);
     hipLaunchKernelGGL(knn_rerank_kernel<true>, dim3(rblocks), dim3(256), 0, s, prm);
   } else {
-    hipLaunchKernelGGL(knn_mfma_candidates_kernel<false>, dim3(blocks), dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false>), dim3(blocks), dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL(knn_rerank_kernel<false>, dim3(rblocks), dim3(256), 0, s, prm);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int runs, int max_rows, hipStream_t s) {
+  if (prm.query_rows == 0 || runs == 0) return hipSuccess;
+  if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.run_offsets == nullptr ||
+      prm.folds == nullptr || prm.query_rows != prm.train_rows || max_rows < 1) {
+    return hipErrorInvalidValue;
+  }
+  const dim3 grid(static_cast<unsigned>((max_rows + kQueriesPerBlock - 1) / kQueriesPerBlock),
+                  static_cast<unsigned>(runs));
+  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
+  if (euclid) {
+    hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true>), grid, dim3(kCandThreads), 0, s, prm);
+    hipLaunchKernelGGL(knn_rerank_kernel<true>, dim3(rblocks), dim3(256), 0, s, prm);
+  } else {
+    hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true>), grid, dim3(kCandThreads), 0, s, prm);
     hipLaunchKernelGGL(knn_rerank_kernel<false>, dim3(rblocks), dim3(256), 0, s, prm);
   }
   return hipGetLastError();
@@ -396,6 +433,39 @@ __global__ void scatter_results_kernel(const int* __restrict__ src_idx, const fl
   const int r = static_cast<int>(i / k), j = static_cast<int>(i % k);
   dst_idx[static_cast<size_t>(rows[r]) * k + j] = src_idx[i];
   dst_dist[static_cast<size_t>(rows[r]) * k + j] = src_dist[i];
+}
+
+// Reranked per-fold results [rows x k] (concatenated indices, internal
+// distances) -> the transposed padded vote layout [k x pad_total]: padded row
+// ids and scores (minus the distance); rows the rerank flagged are left to
+// the exact scan.
+__global__ void publish_tables_kernel(const int* __restrict__ idx, const float* __restrict__ dist,
+                                      const int* __restrict__ flags, const int* __restrict__ run_offsets, int runs,
+                                      const int* __restrict__ pad_offsets, int pad_total, int rows, int k,
+                                      int* __restrict__ out_index, float* __restrict__ out_score) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<size_t>(rows) * k) return;
+  const int i = static_cast<int>(e / k), j = static_cast<int>(e % k);
+  if (flags[i]) return;
+  int lo = 0, hi = runs - 1;  // run of row i: run_offsets[run] <= i < run_offsets[run + 1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (run_offsets[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const int id = idx[e];
+  const size_t o = static_cast<size_t>(j) * pad_total + pad_offsets[lo] + (i - run_offsets[lo]);
+  out_index[o] = id >= 0 ? pad_offsets[lo] + (id - run_offsets[lo]) : -1;
+  out_score[o] = id >= 0 ? -dist[e] : -INFINITY;
+}
+
+hipError_t launch_publish_tables(const int* idx, const float* dist, const int* flags, const int* run_offsets,
+                                 int runs, const int* pad_offsets, int pad_total, int rows, int k, int* out_index,
+                                 float* out_score, hipStream_t s) {
+  const size_t total = static_cast<size_t>(rows) * k;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(publish_tables_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, idx,
+                     dist, flags, run_offsets, runs, pad_offsets, pad_total, rows, k, out_index, out_score);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s) {

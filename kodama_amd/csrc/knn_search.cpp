@@ -2,6 +2,12 @@
 // with the exact rerank where they apply, the register scan otherwise and for
 // the queries the rerank flags.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "../../include/kodama_hip.h"
@@ -124,6 +130,149 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
   check(launch_scatter_results(r_idx.get(), r_dist.get(), d_rows.get(), nr, k, d_idx, d_dist, s),
         "scatter_results launch");
   check(hipStreamSynchronize(s), "fallback sync");
+}
+
+namespace {
+
+// Tuning cache of the per-fold tables: shape bucket -> the faster path.
+struct TableShape {
+  int p, kmax, euclid, rows_log2, total_log2, folds;
+  bool operator<(const TableShape& o) const {
+    return std::tie(p, kmax, euclid, rows_log2, total_log2, folds) <
+           std::tie(o.p, o.kmax, o.euclid, o.rows_log2, o.total_log2, o.folds);
+  }
+};
+std::mutex g_tune_mutex;
+std::map<TableShape, int> g_tuned;
+std::atomic<int> g_table_path{0};
+thread_local int g_last_table_path = 0;
+
+int ceil_log2(int v) {
+  int b = 0;
+  while ((1 << b) < v) ++b;
+  return b;
+}
+
+double seconds_of(const std::function<void()>& f, hipStream_t s) {
+  check(hipStreamSynchronize(s), "tuning sync");
+  const auto t0 = std::chrono::steady_clock::now();
+  f();
+  check(hipStreamSynchronize(s), "tuning sync");
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+void set_fold_table_path(int path) { g_table_path.store(path); }
+int fold_table_path() { return g_table_path.load(); }
+int last_fold_table_path() { return g_last_table_path; }
+
+int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
+  const bool euclid = job.metric == KODAMA_HIP_EUCLIDEAN;
+  const int runs = static_cast<int>(job.run_offsets.size()) - 1;
+  const int total = job.run_offsets[runs];
+  const bool mfma_ok = job.p <= 64 && job.k <= kMfmaMaxK;
+  int path = job.path != 0 ? job.path : g_table_path.load();
+  if (path == 2 && !mfma_ok) path = 1;
+
+  // Register scan of every row of every run, or of listed rows only.
+  auto scan = [&](const int* d_qlist, const int* d_qlist_offsets, int max_queries) {
+    KnnTopkParams prm;
+    prm.train = job.d_x;
+    prm.dims = job.p;
+    prm.k = job.k;
+    prm.rows = job.d_rows;
+    prm.folds = job.d_folds;
+    prm.run_offsets = job.d_run_offsets;
+    prm.pad_offsets = job.d_pad_offsets;
+    prm.pad_total = job.pad_total;
+    prm.qlist = d_qlist;
+    prm.qlist_offsets = d_qlist_offsets;
+    prm.out_index = job.d_nbr;
+    prm.out_dist = job.d_score;
+    check(launch_knn_topk(prm, 1, euclid, max_queries, runs, s), "knn_topk launch");
+  };
+  // Split-BF16 MFMA candidates, exact rerank, the flagged rows rescanned.
+  auto mfma = [&]() {
+    DeviceBuffer<float> xc(static_cast<size_t>(total) * job.p);
+    check(launch_gather_rows(job.d_x, job.d_rows, total, job.p, xc.get(), s), "gather_rows launch");
+    DeviceBuffer<unsigned short> hi(static_cast<size_t>(total) * 64), lo(hi.size());
+    DeviceBuffer<float> n2(static_cast<size_t>(total));
+    check(launch_split_rows(xc.get(), total, job.p, hi.get(), lo.get(), n2.get(), s), "split_rows launch");
+    const size_t cand = static_cast<size_t>(total) * 16 * kMfmaListLen;
+    DeviceBuffer<float> c_val(cand), r_dist(static_cast<size_t>(total) * job.k);
+    DeviceBuffer<int> c_idx(cand), flags(static_cast<size_t>(total)), r_idx(r_dist.size());
+    KnnMfmaParams m;
+    m.train = xc.get();
+    m.query = xc.get();
+    m.train_hi = m.query_hi = hi.get();
+    m.train_lo = m.query_lo = lo.get();
+    m.train_norm2 = m.query_norm2 = n2.get();
+    m.train_norm2_max = job.norm2_max;
+    m.train_rows = m.query_rows = total;
+    m.dims = job.p;
+    m.k = job.k;
+    m.run_offsets = job.d_run_offsets;
+    m.folds = job.d_folds;
+    m.cand_value = c_val.get();
+    m.cand_index = c_idx.get();
+    m.flags = flags.get();
+    m.out_index = r_idx.get();
+    m.out_dist = r_dist.get();
+    check(launch_knn_mfma_tables(m, euclid, runs, job.max_rows, s), "knn_mfma_tables launch");
+    check(launch_publish_tables(r_idx.get(), r_dist.get(), flags.get(), job.d_run_offsets, runs, job.d_pad_offsets,
+                                job.pad_total, total, job.k, job.d_nbr, job.d_score, s),
+          "publish_tables launch");
+    std::vector<int> h_flags(static_cast<size_t>(total));
+    flags.download(h_flags.data(), h_flags.size(), s);
+    check(hipStreamSynchronize(s), "fold tables sync");
+    std::vector<int> qlist, qoff(static_cast<size_t>(runs) + 1, 0);
+    int max_q = 0;
+    for (int r = 0; r < runs; ++r) {
+      for (int i = job.run_offsets[r]; i < job.run_offsets[r + 1]; ++i) {
+        if (h_flags[i]) qlist.push_back(i - job.run_offsets[r]);
+      }
+      qoff[r + 1] = static_cast<int>(qlist.size());
+      max_q = std::max(max_q, qoff[r + 1] - qoff[r]);
+    }
+    if (qlist.empty()) return;
+    DeviceBuffer<int> d_ql(qlist.size()), d_qo(qoff.size());
+    d_ql.upload(qlist.data(), qlist.size(), s);
+    d_qo.upload(qoff.data(), qoff.size(), s);
+    scan(d_ql.get(), d_qo.get(), max_q);
+    check(hipStreamSynchronize(s), "fold tables sync");
+  };
+
+  if (path == 0) {
+    if (!mfma_ok) {
+      path = 1;
+    } else {
+      const TableShape key{job.p, job.k <= 8 ? 8 : job.k <= 16 ? 16 : 32, euclid ? 1 : 0, ceil_log2(job.max_rows),
+                           ceil_log2(total), job.folds};
+      {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) path = it->second;
+      }
+      if (path == 0) {
+        // First use of this shape: time both paths (identical tables), keep the faster.
+        const double t_scan = seconds_of([&] { scan(nullptr, nullptr, job.max_rows); }, s);
+        const double t_mfma = seconds_of(mfma, s);
+        path = t_mfma < t_scan ? 2 : 1;
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tuned[key] = path;
+        g_last_table_path = path;
+        return path;
+      }
+    }
+  }
+  if (path == 2) {
+    mfma();
+  } else {
+    scan(nullptr, nullptr, job.max_rows);
+  }
+  g_last_table_path = path;
+  return path;
 }
 
 }  // namespace kodama_hip

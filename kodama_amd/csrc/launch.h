@@ -35,6 +35,10 @@ struct KnnTopkParams {
   const int* run_offsets = nullptr;  // MODE 1, [runs + 1]
   const int* pad_offsets = nullptr;  // MODE 1, [runs + 1]
   int pad_total = 0;                 // MODE 1
+  // MODE 1, optional: only the run-local rows qlist[qlist_offsets[r] ..
+  // qlist_offsets[r + 1]) of each run r are queried (null: every row).
+  const int* qlist = nullptr;
+  const int* qlist_offsets = nullptr;  // [runs + 1]
   int* out_index = nullptr;
   float* out_dist = nullptr;         // MODE 0 distances, MODE 1 scores (-distance)
 };

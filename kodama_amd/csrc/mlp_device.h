@@ -156,6 +156,14 @@ __device__ __forceinline__ f32x4 mfma(uint4 a, uint4 b, f32x4 c) {
                                                  0, 0);
 }
 
+// Keeps the weight loads of one K block next to their MFMAs: neither the IR
+// passes (memory clobber) nor the machine scheduler may hoist the next loads
+// above it. Hoisting all of them would hold 80 registers.
+__device__ __forceinline__ void layer_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // Three bf16 parts of the ReLU of 8 activations, packed as B operands.
 __device__ __forceinline__ void split3(const f32x4& u, const f32x4& v, uint4& hi, uint4& mid, uint4& lo) {
   uint32_t ph[8], pm[8], pl[8];
@@ -185,9 +193,8 @@ __device__ __forceinline__ void tile_forward(const P* ops, const uint4 (&x)[2], 
   const uint4* w = reinterpret_cast<const uint4*>(ops);
   const float* bias = reinterpret_cast<const float*>(ops) + kMlpTileOps * kWave * 4;
   const int g = lane >> 4;
-  // Each layer's bias is loaded just before its MFMAs, and scheduling barriers
-  // keep the weight loads of a K block next to their MFMAs: the pass then
-  // holds about 50 registers (hoisting all weight loads would take 80 alone).
+  // Each layer's bias is loaded just before its MFMAs and layer_barrier
+  // keeps the weight loads of a K block next to their MFMAs.
   f32x4 h1[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -199,7 +206,7 @@ __device__ __forceinline__ void tile_forward(const P* ops, const uint4 (&x)[2], 
 #pragma unroll
     for (int t = 0; t < 4; ++t) h1[t] = mfma(w[(0 + 2 * t + b) * kWave + lane], x[b], h1[t]);
   }
-  __builtin_amdgcn_sched_barrier(0);
+  layer_barrier();
   f32x4 h2[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -217,7 +224,7 @@ __device__ __forceinline__ void tile_forward(const P* ops, const uint4 (&x)[2], 
       h2[t] = mfma(a, mid, h2[t]);
       h2[t] = mfma(a, hi, h2[t]);
     }
-    __builtin_amdgcn_sched_barrier(0);
+    layer_barrier();
   }
   f32x4 z[2];
 #pragma unroll
@@ -236,7 +243,7 @@ __device__ __forceinline__ void tile_forward(const P* ops, const uint4 (&x)[2], 
       z[t] = mfma(a, mid, z[t]);
       z[t] = mfma(a, hi, z[t]);
     }
-    __builtin_amdgcn_sched_barrier(0);
+    layer_barrier();
   }
   // Masked softmax of row l % 16 over the 4 lanes l % 16 + 16g.
   float m = -INFINITY;
