@@ -106,9 +106,10 @@ const char* kodama_hip_version(void);
 
 /* No counterpart in the reference: for callers that check which path the last
  * exact kNN search of the calling thread took (kodama_hip_exact_knn_search,
- * kodama_hip_knn_graph). With at most 64 dimensions, k <= 32 and at least 4096
- * train rows the search runs split-BF16 MFMA candidates with an exact FP32
- * rerank and returns the number of queries that fell back to the exact scan;
+ * kodama_hip_knn_graph, and the per-fold tables of the kNN classifier calls).
+ * With at most 64 dimensions, k <= 32 and (for a search) at least 4096 train
+ * rows the search runs split-BF16 MFMA candidates with an exact FP32 rerank and
+ * returns the number of queries (table rows) that fell back to the exact scan;
  * -1 when the exact scan served every query. Results are identical either way. */
 int kodama_hip_last_knn_fallbacks(void);
 

@@ -731,6 +731,7 @@ KnnBatch::KnnBatch(int device, const float* x, int n, int p, const IVec& run_off
   for (int f : folds) job.folds = std::max(job.folds, f + 1);
   job.norm2_max = max_row_norm2(x, n, p);
   job.path = table_path;
+  job.fallbacks = &last_knn_fallbacks();
   job.d_nbr = d_nbr_.get();
   job.d_score = d_score_.get();
   device_fold_tables(job, s);

@@ -129,9 +129,11 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     t_count = prm.slice_rows > 0 ? min(prm.slice_rows, prm.train_rows - t_begin) : prm.train_rows;
   } else {
     q_begin = prm.run_offsets[run];
-    t_begin = q_begin;
-    t_count = prm.run_offsets[run + 1] - q_begin;
-    q_count = prm.qlist != nullptr ? prm.qlist_offsets[run + 1] - prm.qlist_offsets[run] : t_count;
+    const int L = prm.run_offsets[run + 1] - q_begin;
+    q_count = prm.qlist != nullptr ? prm.qlist_offsets[run + 1] - prm.qlist_offsets[run] : L;
+    // Run-local offset of this block's train rows (a slice, or all of them).
+    t_begin = prm.slice_rows > 0 ? static_cast<int>(blockIdx.z) * prm.slice_rows : 0;
+    t_count = prm.slice_rows > 0 ? min(prm.slice_rows, L - t_begin) : L;
   }
   const int qi = blockIdx.x * kBlock + threadIdx.x;
   if (blockIdx.x * kBlock >= q_count) return;  // whole block idle for this run
@@ -172,19 +174,19 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
       const int d = e - r * p;
       float v = 0.0f;
       if (r < nt) {
-        const int trow = MODE == 0 ? t_begin + base + r : prm.rows[t_begin + base + r];
+        const int trow = MODE == 0 ? t_begin + base + r : prm.rows[q_begin + t_begin + base + r];
         v = prm.train[static_cast<size_t>(trow) * p + d];
       }
       tile[r * PMAX + d] = v;
     }
     if (MODE == 1 && threadIdx.x < kTile) {
-      tile_fold[threadIdx.x] = threadIdx.x < nt ? prm.folds[t_begin + base + threadIdx.x] : -1;
+      tile_fold[threadIdx.x] = threadIdx.x < nt ? prm.folds[q_begin + t_begin + base + threadIdx.x] : -1;
     }
     __syncthreads();
     if (active) {
       for (int r = 0; r < nt; ++r) {
         // Global train index in MODE 0, run-local row in MODE 1.
-        const int cand = MODE == 0 ? t_begin + base + r : base + r;
+        const int cand = t_begin + base + r;
         if (MODE == 1) {
           if (tile_fold[r] == my_fold) continue;
         } else {
@@ -202,6 +204,16 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     for (int j = 0; j < KMAX; ++j) {
       if (j < k) {
         const size_t o = (static_cast<size_t>(blockIdx.y) * prm.query_rows + qi) * k + j;
+        prm.out_index[o] = ki[j];
+        prm.out_dist[o] = ki[j] >= 0 ? kd[j] : INFINITY;
+      }
+    }
+  } else if (prm.slice_rows > 0) {
+    // Slice lists of the listed queries, for the merge.
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+      if (j < k) {
+        const size_t o = (static_cast<size_t>(blockIdx.z) * prm.list_total + prm.qlist_offsets[run] + qi) * k + j;
         prm.out_index[o] = ki[j];
         prm.out_dist[o] = ki[j] >= 0 ? kd[j] : INFINITY;
       }
@@ -486,8 +498,8 @@ __global__ void __launch_bounds__(kBlock) knn_merge_slices_kernel(const int* __r
 // MODES = 1: query vs train only; 2: both modes.
 template <int PMAX, int KMAX, int MODES = 2>
 static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid, int grid_x, int grid_y,
-                                 hipStream_t s) {
-  const dim3 grid(grid_x, grid_y), block(kBlock);
+                                 int grid_z, hipStream_t s) {
+  const dim3 grid(grid_x, grid_y, grid_z), block(kBlock);
   if (mode == 0) {
     if (euclid) hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, true>), grid, block, 0, s, prm);
     else hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, false>), grid, block, 0, s, prm);
@@ -501,27 +513,33 @@ static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid
 }
 
 template <int PMAX>
-static hipError_t launch_topk_p(const KnnTopkParams& prm, int mode, bool euclid, int gx, int gy, hipStream_t s) {
-  if (prm.k <= 8) return launch_topk_pk<PMAX, 8>(prm, mode, euclid, gx, gy, s);
-  if (prm.k <= 16) return launch_topk_pk<PMAX, 16>(prm, mode, euclid, gx, gy, s);
-  if (prm.k <= 32) return launch_topk_pk<PMAX, 32>(prm, mode, euclid, gx, gy, s);
-  if (prm.k <= 64) return launch_topk_pk<PMAX, 64>(prm, mode, euclid, gx, gy, s);
+static hipError_t launch_topk_p(const KnnTopkParams& prm, int mode, bool euclid, int gx, int gy, int gz,
+                                hipStream_t s) {
+  if (prm.k <= 8) return launch_topk_pk<PMAX, 8>(prm, mode, euclid, gx, gy, gz, s);
+  if (prm.k <= 16) return launch_topk_pk<PMAX, 16>(prm, mode, euclid, gx, gy, gz, s);
+  if (prm.k <= 32) return launch_topk_pk<PMAX, 32>(prm, mode, euclid, gx, gy, gz, s);
+  if (prm.k <= 64) return launch_topk_pk<PMAX, 64>(prm, mode, euclid, gx, gy, gz, s);
   // 65..128 neighbours: query vs train only, up to 128 dimensions (the lists
   // then sit partly in the accumulation registers; beyond they would spill).
-  if constexpr (PMAX <= 128) return launch_topk_pk<PMAX, kMaxScanK, 1>(prm, mode, euclid, gx, gy, s);
+  if constexpr (PMAX <= 128) return launch_topk_pk<PMAX, kMaxScanK, 1>(prm, mode, euclid, gx, gy, gz, s);
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,
-                           hipStream_t stream) {
+                           hipStream_t stream, int slices) {
   if (prm.k < 1 || prm.dims < 1 || prm.dims > kMaxDims) return hipErrorInvalidValue;
   if (prm.k > (mode == 0 && prm.dims <= 128 ? kMaxScanK : kMaxK)) return hipErrorInvalidValue;
+  // Sliced per-fold scans write per-listed-query lists: they need the list.
+  if (slices < 1 || (slices > 1 && (mode != 1 || prm.qlist == nullptr || prm.slice_rows <= 0))) {
+    return hipErrorInvalidValue;
+  }
+  if (mode == 1 && prm.slice_rows > 0 && prm.qlist == nullptr) return hipErrorInvalidValue;
   const int gx = (max_queries + kBlock - 1) / kBlock;
   if (gx == 0 || runs == 0) return hipSuccess;
-  if (prm.dims <= 32) return launch_topk_p<32>(prm, mode, euclid, gx, runs, stream);
-  if (prm.dims <= 64) return launch_topk_p<64>(prm, mode, euclid, gx, runs, stream);
-  if (prm.dims <= 128) return launch_topk_p<128>(prm, mode, euclid, gx, runs, stream);
-  return launch_topk_p<256>(prm, mode, euclid, gx, runs, stream);
+  if (prm.dims <= 32) return launch_topk_p<32>(prm, mode, euclid, gx, runs, slices, stream);
+  if (prm.dims <= 64) return launch_topk_p<64>(prm, mode, euclid, gx, runs, slices, stream);
+  if (prm.dims <= 128) return launch_topk_p<128>(prm, mode, euclid, gx, runs, slices, stream);
+  return launch_topk_p<256>(prm, mode, euclid, gx, runs, slices, stream);
 }
 
 template <bool STAGE>

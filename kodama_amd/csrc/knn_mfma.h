@@ -48,6 +48,9 @@ hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int run
 hipError_t launch_publish_tables(const int* idx, const float* dist, const int* flags, const int* run_offsets,
                                  int runs, const int* pad_offsets, int pad_total, int rows, int k, int* out_index,
                                  float* out_score, hipStream_t s);
+hipError_t launch_publish_listed(const int* idx, const float* dist, const int* qlist, const int* qrun,
+                                 const int* pad_offsets, int pad_total, int count, int k, int* out_index,
+                                 float* out_score, hipStream_t s);
 // rows[i] of src [* x p] -> dst [count x p]; and the inverse scatter of [count x k] results.
 hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s);
 hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, const int* rows, int count, int k,
@@ -85,6 +88,7 @@ struct FoldTableJob {
   int path = 0;
   int* d_nbr = nullptr;
   float* d_score = nullptr;
+  int* fallbacks = nullptr;         // rows the MFMA path rescanned (may be null)
 };
 int device_fold_tables(const FoldTableJob& job, hipStream_t s);
 void set_fold_table_path(int path);

@@ -468,6 +468,32 @@ hipError_t launch_publish_tables(const int* idx, const float* dist, const int* f
   return hipGetLastError();
 }
 
+// Merged lists of listed rows [count x k] (run-local ids, distances) -> the
+// padded vote layout.
+__global__ void publish_listed_kernel(const int* __restrict__ idx, const float* __restrict__ dist,
+                                      const int* __restrict__ qlist, const int* __restrict__ qrun,
+                                      const int* __restrict__ pad_offsets, int pad_total, int count, int k,
+                                      int* __restrict__ out_index, float* __restrict__ out_score) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<size_t>(count) * k) return;
+  const int f = static_cast<int>(e / k), j = static_cast<int>(e % k);
+  const int run = qrun[f];
+  const int id = idx[e];
+  const size_t o = static_cast<size_t>(j) * pad_total + pad_offsets[run] + qlist[f];
+  out_index[o] = id >= 0 ? pad_offsets[run] + id : -1;
+  out_score[o] = id >= 0 ? -dist[e] : -INFINITY;
+}
+
+hipError_t launch_publish_listed(const int* idx, const float* dist, const int* qlist, const int* qrun,
+                                 const int* pad_offsets, int pad_total, int count, int k, int* out_index,
+                                 float* out_score, hipStream_t s) {
+  const size_t total = static_cast<size_t>(count) * k;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(publish_listed_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, idx,
+                     dist, qlist, qrun, pad_offsets, pad_total, count, k, out_index, out_score);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_rows(const float* src, const int* rows, int count, int p, float* dst, hipStream_t s) {
   const size_t total = static_cast<size_t>(count) * p;
   if (total == 0) return hipSuccess;

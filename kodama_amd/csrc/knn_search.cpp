@@ -23,6 +23,7 @@ constexpr int kScanBlock = 256;          // queries per register-scan block
 constexpr int kScanTargetBlocks = 512;   // two blocks per CU
 constexpr int kMaxSlices = 64;           // lists one merge wave combines
 constexpr int kSliceAlign = 64;          // LDS tile rows of the scan
+constexpr int kTableTargetBlocks = 2048; // sliced rescans of flagged table rows
 
 }  // namespace
 
@@ -169,6 +170,7 @@ int last_fold_table_path() { return g_last_table_path; }
 
 int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
   const bool euclid = job.metric == KODAMA_HIP_EUCLIDEAN;
+  if (job.fallbacks) *job.fallbacks = -1;  // -1: the register scan served every row
   const int runs = static_cast<int>(job.run_offsets.size()) - 1;
   const int total = job.run_offsets[runs];
   const bool mfma_ok = job.p <= 64 && job.k <= kMfmaMaxK;
@@ -226,20 +228,60 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     std::vector<int> h_flags(static_cast<size_t>(total));
     flags.download(h_flags.data(), h_flags.size(), s);
     check(hipStreamSynchronize(s), "fold tables sync");
-    std::vector<int> qlist, qoff(static_cast<size_t>(runs) + 1, 0);
+    std::vector<int> qlist, qrun, qoff(static_cast<size_t>(runs) + 1, 0);
     int max_q = 0;
     for (int r = 0; r < runs; ++r) {
       for (int i = job.run_offsets[r]; i < job.run_offsets[r + 1]; ++i) {
-        if (h_flags[i]) qlist.push_back(i - job.run_offsets[r]);
+        if (h_flags[i]) {
+          qlist.push_back(i - job.run_offsets[r]);
+          qrun.push_back(r);
+        }
       }
       qoff[r + 1] = static_cast<int>(qlist.size());
       max_q = std::max(max_q, qoff[r + 1] - qoff[r]);
     }
-    if (qlist.empty()) return;
-    DeviceBuffer<int> d_ql(qlist.size()), d_qo(qoff.size());
+    const int nflag = static_cast<int>(qlist.size());
+    if (job.fallbacks) *job.fallbacks = nflag;
+    if (nflag == 0) return;
+    DeviceBuffer<int> d_ql(qlist.size()), d_qo(qoff.size()), d_qr(qrun.size());
     d_ql.upload(qlist.data(), qlist.size(), s);
     d_qo.upload(qoff.data(), qoff.size(), s);
-    scan(d_ql.get(), d_qo.get(), max_q);
+    d_qr.upload(qrun.data(), qrun.size(), s);
+    // A few flagged rows per run would leave one block per run scanning the
+    // whole run; their train rows are cut into slices (about 2048 blocks in
+    // all, at most 64 slices) whose lists are merged.
+    const int gx = (max_q + kScanBlock - 1) / kScanBlock;
+    int slices = std::min(kMaxSlices, std::max(1, (kTableTargetBlocks + gx * runs - 1) / (gx * runs)));
+    int slice_rows = (job.max_rows + slices - 1) / slices;
+    slice_rows = std::max(kSliceAlign, (slice_rows + kSliceAlign - 1) / kSliceAlign * kSliceAlign);
+    slices = (job.max_rows + slice_rows - 1) / slice_rows;
+    if (slices <= 1) {
+      scan(d_ql.get(), d_qo.get(), max_q);
+    } else {
+      DeviceBuffer<int> part_idx(static_cast<size_t>(slices) * nflag * job.k), m_idx(static_cast<size_t>(nflag) * job.k);
+      DeviceBuffer<float> part_dist(part_idx.size()), m_dist(m_idx.size());
+      KnnTopkParams prm;
+      prm.train = job.d_x;
+      prm.dims = job.p;
+      prm.k = job.k;
+      prm.rows = job.d_rows;
+      prm.folds = job.d_folds;
+      prm.run_offsets = job.d_run_offsets;
+      prm.pad_offsets = job.d_pad_offsets;
+      prm.pad_total = job.pad_total;
+      prm.qlist = d_ql.get();
+      prm.qlist_offsets = d_qo.get();
+      prm.list_total = nflag;
+      prm.slice_rows = slice_rows;
+      prm.out_index = part_idx.get();
+      prm.out_dist = part_dist.get();
+      check(launch_knn_topk(prm, 1, euclid, max_q, runs, s, slices), "knn_topk launch");
+      check(launch_knn_merge_slices(part_idx.get(), part_dist.get(), slices, nflag, job.k, m_idx.get(), m_dist.get(), s),
+            "knn_merge_slices launch");
+      check(launch_publish_listed(m_idx.get(), m_dist.get(), d_ql.get(), d_qr.get(), job.d_pad_offsets,
+                                  job.pad_total, nflag, job.k, job.d_nbr, job.d_score, s),
+            "publish_listed launch");
+    }
     check(hipStreamSynchronize(s), "fold tables sync");
   };
 

@@ -36,9 +36,13 @@ struct KnnTopkParams {
   const int* pad_offsets = nullptr;  // MODE 1, [runs + 1]
   int pad_total = 0;                 // MODE 1
   // MODE 1, optional: only the run-local rows qlist[qlist_offsets[r] ..
-  // qlist_offsets[r + 1]) of each run r are queried (null: every row).
+  // qlist_offsets[r + 1]) of each run r are queried (null: every row). With
+  // slice_rows > 0 as well, block z scans the run's rows [z * slice_rows,
+  // (z + 1) * slice_rows) and writes its list of listed query i to
+  // [z][list_total x k] (run-local ids, distances; -1 / +inf), for a merge.
   const int* qlist = nullptr;
   const int* qlist_offsets = nullptr;  // [runs + 1]
+  int list_total = 0;
   int* out_index = nullptr;
   float* out_dist = nullptr;         // MODE 0 distances, MODE 1 scores (-distance)
 };
@@ -73,7 +77,7 @@ struct VoteParams {
 };
 
 hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int runs,
-                           hipStream_t stream);
+                           hipStream_t stream, int slices = 1);
 // The k smallest by (distance, index) of `slices` lists [slices][query_rows x k]
 // (each ascending, -1 = empty) -> [query_rows x k], -1 / +inf past the end.
 hipError_t launch_knn_merge_slices(const int* idx, const float* dist, int slices, int query_rows, int k,
