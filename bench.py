@@ -199,6 +199,29 @@ def matrix_bench(rows: int, runs: int, cycles: int, k: int, forms) -> int:
     return 0
 
 
+def tables_bench(runs: int, n: int, k: int) -> int:
+    """Per-fold kNN tables of the batched-runs workload built by each path
+    (register scan, split-BF16 MFMA + exact rerank), twice each; one JSON line
+    per build with its wall time, the rows the MFMA path rescanned, and whether
+    the tables equal the scan's."""
+    import kodama_amd as K
+    x, rows, fids, _ = make_workload(runs, n, 50, k)
+    ref = None
+    for path in ("scan", "mfma", "scan", "mfma"):
+        t0 = time.perf_counter()
+        nbr, score, used = K.knn_fold_tables(x, rows, fids, k, "euclidean", path=path)
+        wall = time.perf_counter() - t0
+        same = None
+        if ref is None:
+            ref = (nbr, score)
+        else:
+            same = all(np.array_equal(a, b) for a, b in zip(nbr, ref[0])) and \
+                all(np.array_equal(a, b) for a, b in zip(score, ref[1]))
+        print(json.dumps({"tables": path, "runs": runs, "n": n, "k": k, "seconds": wall,
+                          "mfma_fallback_rows": K.last_knn_fallbacks(), "equal_to_scan": same}), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
     """HBM bytes per launch from the committed counter summary, if it covers
     this kernel on this workload."""
@@ -231,6 +254,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
     ap.add_argument("--graph-rows", type=int, default=0, help="time the exact k-NN graph of this many rows and exit")
+    ap.add_argument("--tables", action="store_true",
+                    help="time the per-fold kNN tables of the workload by each path and exit")
     ap.add_argument("--matrix-rows", type=int, default=0,
                     help="time end-to-end KODAMA.matrix on this many rows (runs, steps, forms apply) and exit")
     args = ap.parse_args()
@@ -238,6 +263,8 @@ def main() -> int:
         return copy_probe()
     if args.graph_rows > 0:
         return graph_bench(args.graph_rows, args.k)
+    if args.tables:
+        return tables_bench(args.runs, args.n, args.k)
     if args.matrix_rows > 0:
         return matrix_bench(args.matrix_rows, args.runs, args.steps, args.k, [f for f in args.forms.split(",") if f])
     if args.gpus != 1:
