@@ -72,12 +72,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10, salt: int = 0):
+def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10, salt: int = 0, data=None,
+                  labels=None):
     """The batched-runs workload; `salt` moves the landmark, fold and
-    starting-label streams (the data matrix stays the same)."""
+    starting-label streams (the data matrix stays the same). `data` replaces
+    the 20-cluster mixture, `labels` (one per data row) the nearest-centre
+    starting labels."""
     import kodama_amd as K
     from kodama_amd import datasets as D
-    x, _ = D.config3(n=n, p=p)
+    x = data if data is not None else D.config3(n=n, p=p)[0]
+    n = x.shape[0]
     sizes = D.randint(6 + salt, 1, runs, 1000, 5000)
     rows, fold_ids, init = [], [], []
     for r in range(runs):
@@ -92,12 +96,15 @@ def make_workload(runs: int, n: int, p: int, k: int, folds: int = 10, salt: int 
         perm = np.argsort(D.uniform(7 + salt, 1000 + r, L), kind="stable")
         f = np.empty(L, np.int32)
         f[perm] = np.repeat(np.arange(folds, dtype=np.int32), cnt)
+        rows.append(rw)
+        fold_ids.append(f)
+        if labels is not None:
+            init.append(np.asarray(labels[rw], np.int32))
+            continue
         # Starting labels: nearest of min(300, L) random landmark centres.
         nc = min(300, L)
         centres = x[rw[np.argsort(D.uniform(8 + salt, r, L), kind="stable")[:nc]]]
         idx, _ = K.exact_knn_search(centres, x[rw], 1, "euclidean")
-        rows.append(rw)
-        fold_ids.append(f)
         init.append((idx[:, 0] + 1).astype(np.int32))
     return x, rows, fold_ids, init
 
@@ -222,6 +229,41 @@ def tables_bench(runs: int, n: int, k: int) -> int:
     return 0
 
 
+def learned_curve_bench(runs: int, cycles: int, k: int) -> int:
+    """The learned-proposal configuration: X = 200,000 x 50 (30-cluster Zipf
+    mixture), starting labels the true components with 5 % label noise, M runs
+    on landmark subsets of U[1000, 5000] rows with 10 uneven folds, `cycles`
+    cycles, kNN k. The network is trained (untimed) on the plain form's traces
+    of the 20-cluster workload (16 runs). One JSON line per form (uniform
+    device form, learned form): mean best CV accuracy against wall time, at
+    cycles 1, 2, 5, 10, 20, 50, 100 (wall time pro rata of the single launch)."""
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    base = dict(k=k, folds=10, stratified=0, seed=1, cv_seed=1, evolutionary_search=1, guarded_diversity=1,
+                many_to_one_absorption=1, n_threads=0)
+    tx, trows, tfolds, tinit = make_workload(16, 100000, 50, k, salt=100)
+    feat, tgt, _ = K.mlp_collect_traces(tx, trows, tinit, K.core_options(cycles=50, **base), fold_ids=tfolds)
+    params, _ = K.mlp_train(feat, tgt, epochs=20, batch=256, learning_rate=0.05, momentum=0.9, seed=5)
+    x, noisy, comp = D.config5()
+    x, rows, fids, init = make_workload(runs, x.shape[0], 50, k, salt=200, data=x, labels=noisy)
+    start = float(np.mean([np.mean(i == comp[r]) for i, r in zip(init, rows)]))  # agreement with the components
+    for form in ("device", "learned"):
+        run = (lambda o: K.core_knn_batch_learned(x, rows, init, o, params, fallback=0.25, fold_ids=fids)) \
+            if form == "learned" else (lambda o: K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=fids))
+        run(K.core_options(cycles=5, **base))  # warm-up
+        out = run(K.core_options(cycles=cycles, **base))
+        curve = out["vect_acc"].mean(axis=0)
+        pts = [c for c in (1, 2, 5, 10, 20, 50, 100) if c <= cycles]
+        print(json.dumps({"learned_curve": form, "runs": runs, "cycles": cycles, "k": k,
+                          "landmarks_total": int(sum(len(r) for r in rows)), "label_noise": 0.05,
+                          "initial_agreement_with_components": start, "evolution_seconds": out["evolution_seconds"],
+                          "steps_per_s": runs * cycles / out["evolution_seconds"],
+                          "mean_accbest": float(np.mean(out["accbest"])),
+                          "curve": [{"cycle": c, "seconds": out["evolution_seconds"] * c / cycles,
+                                     "mean_accbest": float(curve[c - 1])} for c in pts]}), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
     """HBM bytes per launch from the committed counter summary, if it covers
     this kernel on this workload."""
@@ -254,6 +296,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
     ap.add_argument("--graph-rows", type=int, default=0, help="time the exact k-NN graph of this many rows and exit")
+    ap.add_argument("--learned-curve", action="store_true",
+                    help="accuracy against wall time of the uniform and learned proposals on the 200k"
+                         " label-noise configuration (--steps cycles, default 100 here) and exit")
     ap.add_argument("--tables", action="store_true",
                     help="time the per-fold kNN tables of the workload by each path and exit")
     ap.add_argument("--matrix-rows", type=int, default=0,
@@ -265,6 +310,8 @@ def main() -> int:
         return graph_bench(args.graph_rows, args.k)
     if args.tables:
         return tables_bench(args.runs, args.n, args.k)
+    if args.learned_curve:
+        return learned_curve_bench(args.runs, 100 if args.steps == 50 else args.steps, args.k)
     if args.matrix_rows > 0:
         return matrix_bench(args.matrix_rows, args.runs, args.steps, args.k, [f for f in args.forms.split(",") if f])
     if args.gpus != 1:
