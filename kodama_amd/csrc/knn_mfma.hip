@@ -278,14 +278,49 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
   }
 }
 
-// One wave per query; 4 queries per 256-thread workgroup.
+// Order-preserving map of a float to an unsigned key (finite values).
+__device__ __forceinline__ uint32_t order_key(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Ascending bitonic sort of 128 keys over a wave: element e is register e / 64
+// of lane e % 64 (a: e = lane, b: e = 64 + lane).
+__device__ __forceinline__ void wave_sort128(uint32_t& a, uint32_t& b, int lane) {
+#pragma unroll
+  for (int size = 2; size <= 128; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 64) {  // size 128: between the two registers, ascending
+        const uint32_t lo = min(a, b), hi = max(a, b);
+        a = lo;
+        b = hi;
+        continue;
+      }
+      const bool lower = (lane & stride) == 0;
+      const uint32_t pa = __shfl_xor(a, stride), pb = __shfl_xor(b, stride);
+      const bool asc_a = (lane & size) == 0;
+      const bool asc_b = ((64 + lane) & size) == 0;
+      a = lower == asc_a ? min(a, pa) : max(a, pa);
+      b = lower == asc_b ? min(b, pb) : max(b, pb);
+    }
+  }
+}
+
+// One wave per query; 4 queries per 256-thread workgroup. Each lane holds two
+// of the query's 128 candidates. A_k (the k-th smallest candidate value; ties
+// of value do not move it) comes from a bitonic sort of the values over the
+// wave; the exact re-scores of the candidates within the bound are ranked by
+// (distance, index) among themselves.
 template <bool EUCLID>
 __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   constexpr int kCand = 16 * kList;  // 128
-  __shared__ float s_val[4][kCand];
-  __shared__ int s_idx[4][kCand];
+  static_assert(kCand == 2 * kWave, "two candidates per lane");
   __shared__ float s_ex[4][kCand];
-  __shared__ int s_sel[4][kCand];
+  __shared__ int s_id[4][kCand];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const int q = blockIdx.x * 4 + wave;
@@ -293,39 +328,21 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   if (!live) return;
   const int k = prm.k;
   const int p = prm.dims;
-  float* val = s_val[wave];
-  int* idx = s_idx[wave];
-  for (int c = lane; c < kCand; c += kWave) {
-    val[c] = prm.cand_value[static_cast<size_t>(q) * kCand + c];
-    idx[c] = prm.cand_index[static_cast<size_t>(q) * kCand + c];
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  // A_k: the candidate of rank k - 1 in (value, index) order among valid ones.
-  int valid = 0;
-  float ak = INFINITY;
-  for (int c = lane; c < kCand; c += kWave) {
-    if (idx[c] < 0) continue;
-    int rank = 0;
-    for (int e = 0; e < kCand; ++e) {
-      if (idx[e] < 0) continue;
-      rank += (val[e] < val[c] || (val[e] == val[c] && idx[e] < idx[c])) ? 1 : 0;
-    }
-    if (rank == k - 1) ak = val[c];
-  }
-  for (int c = lane; c < kCand; c += kWave) valid += idx[c] >= 0 ? 1 : 0;
-#pragma unroll
-  for (int m = kWave / 2; m >= 1; m >>= 1) {
-    valid += __shfl_xor(valid, m);
-    ak = fminf(ak, __shfl_xor(ak, m));
-  }
+  const size_t base = static_cast<size_t>(q) * kCand;
+  const float v0 = prm.cand_value[base + lane], v1 = prm.cand_value[base + kWave + lane];
+  const int i0 = prm.cand_index[base + lane], i1 = prm.cand_index[base + kWave + lane];
+  const int valid = __popcll(__ballot(i0 >= 0)) + __popcll(__ballot(i1 >= 0));
+  uint32_t sa = i0 >= 0 ? order_key(v0) : 0xFFFFFFFFu, sb = i1 >= 0 ? order_key(v1) : 0xFFFFFFFFu;
+  wave_sort128(sa, sb, lane);
+  // Rank k - 1 < 64 (k <= 32) sits in register a of lane k - 1.
+  const float ak = key_value(__shfl(sa, k - 1));
   // Fewer valid candidates than k: every one is kept.
   const float bound = valid >= k ? ak + 2.0f * (prm.error_scale * (prm.query_norm2[q] + prm.train_norm2_max)) : INFINITY;
   // A full list whose largest value is within the bound may have dropped a row.
   bool overflow = false;
   if (lane < 16) {
-    const int last = lane * kList + kList - 1;
-    overflow = idx[last] >= 0 && val[last] <= bound;
+    const size_t last = base + lane * kList + kList - 1;
+    overflow = prm.cand_index[last] >= 0 && prm.cand_value[last] <= bound;
   }
   if (__ballot(overflow) != 0ull) {
     if (lane == 0) prm.flags[q] = 1;
@@ -337,30 +354,40 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   const float* qrow = prm.query + static_cast<size_t>(q) * p;
 #pragma unroll
   for (int d = 0; d < 64; ++d) qv[d] = d < p ? qrow[d] : 0.0f;
+  const bool u0 = i0 >= 0 && v0 <= bound, u1 = i1 >= 0 && v1 <= bound;
+  const float e0 = u0 ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(i0) * p, p) : INFINITY;
+  const float e1 = u1 ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(i1) * p, p) : INFINITY;
+  // Compact the selected candidates into the wave's LDS slice.
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint64_t m0 = __ballot(u0), m1 = __ballot(u1);
+  const int n0 = __popcll(m0), nsel = n0 + __popcll(m1);
   float* ex = s_ex[wave];
-  int* sel = s_sel[wave];
-  for (int c = lane; c < kCand; c += kWave) {
-    const bool use = idx[c] >= 0 && val[c] <= bound;
-    sel[c] = use ? 1 : 0;
-    ex[c] = use ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(idx[c]) * p, p) : INFINITY;
+  int* id = s_id[wave];
+  const int p0 = __popcll(m0 & below), p1 = n0 + __popcll(m1 & below);
+  if (u0) {
+    ex[p0] = e0;
+    id[p0] = i0;
+  }
+  if (u1) {
+    ex[p1] = e1;
+    id[p1] = i1;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  int nsel = 0;
-  for (int c = lane; c < kCand; c += kWave) nsel += sel[c];
-#pragma unroll
-  for (int m = kWave / 2; m >= 1; m >>= 1) nsel += __shfl_xor(nsel, m);
-  for (int c = lane; c < kCand; c += kWave) {
-    if (!sel[c]) continue;
-    int rank = 0;
-    for (int e = 0; e < kCand; ++e) {
-      if (!sel[e]) continue;
-      rank += (ex[e] < ex[c] || (ex[e] == ex[c] && idx[e] < idx[c])) ? 1 : 0;
-    }
-    if (rank < k) {
-      prm.out_index[static_cast<size_t>(q) * k + rank] = idx[c];
-      prm.out_dist[static_cast<size_t>(q) * k + rank] = ex[c];
-    }
+  int r0 = 0, r1 = 0;
+  for (int e = 0; e < nsel; ++e) {
+    const float x = ex[e];
+    const int j = id[e];
+    r0 += (x < e0 || (x == e0 && j < i0)) ? 1 : 0;
+    r1 += (x < e1 || (x == e1 && j < i1)) ? 1 : 0;
+  }
+  if (u0 && r0 < k) {
+    prm.out_index[static_cast<size_t>(q) * k + r0] = i0;
+    prm.out_dist[static_cast<size_t>(q) * k + r0] = e0;
+  }
+  if (u1 && r1 < k) {
+    prm.out_index[static_cast<size_t>(q) * k + r1] = i1;
+    prm.out_dist[static_cast<size_t>(q) * k + r1] = e1;
   }
   for (int s = nsel + lane; s < k; s += kWave) {
     prm.out_index[static_cast<size_t>(q) * k + s] = -1;
