@@ -135,6 +135,13 @@ class KnnBatch {
 
   // Pinned staging: label codes in (padded layout), predictions / reductions out.
   int* labels_host() { return h_labels_.get(); }
+  // Pinned staging of the device-resident evolution (allocated with the batch,
+  // outside its timed region): the upload image and the download of its
+  // results. Null when `bytes` exceeds the staging (the caller then stages in
+  // pageable memory). One evolution of a batch at a time: its upload is
+  // complete once its results are exported (the export synchronises).
+  unsigned char* stage_up(size_t bytes) { return bytes <= h_stage_up_.size() ? h_stage_up_.get() : nullptr; }
+  unsigned char* stage_down(size_t bytes) { return bytes <= h_stage_down_.size() ? h_stage_down_.get() : nullptr; }
   const int* predictions_host() const { return h_pred_.get(); }
   const int* counts_host() const { return h_counts_.get(); }
   const int* firsts_host() const { return h_firsts_.get(); }
@@ -170,6 +177,7 @@ class KnnBatch {
   DeviceBuffer<float> d_score_;
   DeviceBuffer<int> d_counts_, d_firsts_, d_matches_;
   PinnedBuffer<int> h_labels_, h_pred_, h_counts_, h_firsts_, h_matches_;
+  PinnedBuffer<unsigned char> h_stage_up_, h_stage_down_;
   std::vector<IVec> empty_rows_;
   EventTimer timer_;
   float vote_ms_ = 0.0f;

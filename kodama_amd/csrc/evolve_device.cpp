@@ -103,10 +103,19 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
 
   // Host image of the upload region: starting labels (their predictions start
   // equal to them and are rewritten by the first evaluation), widths, keys.
-  std::vector<unsigned char> up(m.up_bytes, 0);
-  auto* h_init = reinterpret_cast<unsigned short*>(up.data());
-  auto* h_bits = reinterpret_cast<int*>(up.data() + (off_bits - off_cur));
-  auto* h_key = reinterpret_cast<uint64_t*>(up.data() + (off_key - off_cur));
+  // Built in the batch's pinned staging when it fits (an asynchronous copy the
+  // launch follows in stream order), else in pageable memory.
+  std::vector<unsigned char> pageable;
+  unsigned char* img = batch.stage_up(m.up_bytes);
+  if (img == nullptr) {
+    pageable.assign(m.up_bytes, 0);
+    img = pageable.data();
+  }
+  auto* h_init = reinterpret_cast<unsigned short*>(img);
+  auto* h_bits = reinterpret_cast<int*>(img + (off_bits - off_cur));
+  auto* h_key = reinterpret_cast<uint64_t*>(img + (off_key - off_cur));
+  // The runs' rows are disjoint slices of the image: filled by host threads.
+#pragma omp parallel for schedule(static)
   for (int r = 0; r < m.runs; ++r) {
     for (size_t i = 0; i < codes[r].size(); ++i) {
       h_init[batch.pad_offset(r) + i] = static_cast<unsigned short>(codes[r][i]);
@@ -117,7 +126,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     h_bits[r] = bits;
     h_key[r] = splitmix(options.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
   }
-  check(hipMemcpyAsync(base + m.up_begin, up.data(), m.up_bytes, hipMemcpyHostToDevice, s), "upload");
+  check(hipMemcpyAsync(base + m.up_begin, img, m.up_bytes, hipMemcpyHostToDevice, s), "upload");
   check(hipMemcpyAsync(base + off_curpred, base + off_cur, label_bytes, hipMemcpyDeviceToDevice, s), "copy");
   check(launch_evo_zero(reinterpret_cast<int*>(base + off_flows), flow_items, s), "evo_zero");
   check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
@@ -146,7 +155,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.state = reinterpret_cast<double*>(base + m.off_state);
   p.flows = reinterpret_cast<int*>(base + off_flows);
   p.history = reinterpret_cast<double*>(base + m.off_history);
-  check(hipStreamSynchronize(s), "setup sync");  // `up` is pageable host memory
+  if (!pageable.empty()) check(hipStreamSynchronize(s), "setup sync");  // the image dies with this scope
   impl_ = impl.release();
 }
 
@@ -198,14 +207,19 @@ void DeviceEvolution::run_learned(int cycles, const float* params, float fallbac
 void DeviceEvolution::export_results(int* clbest, double* accbest, double* vect_acc) {
   Impl& m = *impl_;
   const hipStream_t s = m.batch.stream();
-  std::vector<unsigned char> down(m.down_bytes);
-  check(hipMemcpyAsync(down.data(), m.arena.get() + m.down_begin, m.down_bytes, hipMemcpyDeviceToHost, s),
-        "download");
+  std::vector<unsigned char> pageable;
+  unsigned char* down = m.batch.stage_down(m.down_bytes);
+  if (down == nullptr) {
+    pageable.resize(m.down_bytes);
+    down = pageable.data();
+  }
+  check(hipMemcpyAsync(down, m.arena.get() + m.down_begin, m.down_bytes, hipMemcpyDeviceToHost, s), "download");
   check(hipStreamSynchronize(s), "export sync");
-  const auto* best = reinterpret_cast<const unsigned short*>(down.data() + (m.off_best - m.down_begin));
-  const auto* state = reinterpret_cast<const double*>(down.data() + (m.off_state - m.down_begin));
-  const auto* hist = reinterpret_cast<const double*>(down.data() + (m.off_history - m.down_begin));
+  const auto* best = reinterpret_cast<const unsigned short*>(down + (m.off_best - m.down_begin));
+  const auto* state = reinterpret_cast<const double*>(down + (m.off_state - m.down_begin));
+  const auto* hist = reinterpret_cast<const double*>(down + (m.off_history - m.down_begin));
   const IVec& off = m.batch.run_offsets();
+#pragma omp parallel for schedule(static)
   for (int r = 0; r < m.runs; ++r) {
     const int L = off[r + 1] - off[r];
     const IVec& v = m.values[r];
