@@ -187,6 +187,18 @@ def graph_bench(rows: int, k: int) -> int:
     return 0
 
 
+def train_network(k: int, cycles: int = 50):
+    """The learned proposal's network, trained (untimed) on the plain form's
+    accept/reject traces of 16 runs of the 20-cluster workload."""
+    import kodama_amd as K
+    base = dict(k=k, folds=10, stratified=0, seed=1, cv_seed=1, evolutionary_search=1, guarded_diversity=1,
+                many_to_one_absorption=1, n_threads=0)
+    tx, trows, tfolds, tinit = make_workload(16, 100000, 50, k, salt=100)
+    feat, tgt, _ = K.mlp_collect_traces(tx, trows, tinit, K.core_options(cycles=cycles, **base), fold_ids=tfolds)
+    params, _ = K.mlp_train(feat, tgt, epochs=20, batch=256, learning_rate=0.05, momentum=0.9, seed=5)
+    return params
+
+
 def matrix_bench(rows: int, runs: int, cycles: int, k: int, forms) -> int:
     """End-to-end KODAMA.matrix (kNN) of the 20-cluster mixture (config-3
     generator, p = 50): L = 3000 landmarks per run, splitting 300, global graph
@@ -194,11 +206,12 @@ def matrix_bench(rows: int, runs: int, cycles: int, k: int, forms) -> int:
     import kodama_amd as K
     from kodama_amd import datasets as D
     x, _ = D.config3(n=rows)
+    params = train_network(k) if "learned" in forms else None
     K.KODAMA_matrix(x[:4000], M=2, Tcycle=2, landmarks=1000, k=k)  # warm-up (code objects, allocator)
     for form in forms:
         t0 = time.perf_counter()
         out = K.KODAMA_matrix(x, M=runs, Tcycle=cycles, landmarks=3000, splitting=300, folds=5, k=k,
-                              graph_neighbors=100, seed=1234, form=form)
+                              graph_neighbors=100, seed=1234, form=form, mlp_params=params)
         wall = time.perf_counter() - t0
         print(json.dumps({"matrix_rows": rows, "runs": runs, "cycles": cycles, "k": k, "form": form,
                           "seconds": wall, "mean_acc": float(np.mean(out["acc"])),
@@ -241,9 +254,7 @@ def learned_curve_bench(runs: int, cycles: int, k: int) -> int:
     from kodama_amd import datasets as D
     base = dict(k=k, folds=10, stratified=0, seed=1, cv_seed=1, evolutionary_search=1, guarded_diversity=1,
                 many_to_one_absorption=1, n_threads=0)
-    tx, trows, tfolds, tinit = make_workload(16, 100000, 50, k, salt=100)
-    feat, tgt, _ = K.mlp_collect_traces(tx, trows, tinit, K.core_options(cycles=50, **base), fold_ids=tfolds)
-    params, _ = K.mlp_train(feat, tgt, epochs=20, batch=256, learning_rate=0.05, momentum=0.9, seed=5)
+    params = train_network(k)
     x, noisy, comp = D.config5()
     x, rows, fids, init = make_workload(runs, x.shape[0], 50, k, salt=200, data=x, labels=noisy)
     start = float(np.mean([np.mean(i == comp[r]) for i, r in zip(init, rows)]))  # agreement with the components

@@ -292,9 +292,12 @@ typedef struct kodama_hip_matrix_options {
   int32_t metric;           /* KODAMA_HIP_COSINE / _INNER_PRODUCT / _EUCLIDEAN */
   int32_t graph_neighbors;  /* global graph width; <= 0: 100 */
   uint64_t seed;            /* KODAMAMatrixOptions::seed (reference default 1234) */
-  int32_t form;             /* 0 = plain (reference streams per run), 1 = device-resident */
+  int32_t form;             /* 0 = plain (reference streams per run), 1 = device-resident,
+                               2 = device-resident with the learned proposal */
   int32_t device;
   int32_t n_threads;        /* host threads of the plain form (0 = automatic) */
+  const float* mlp_params;  /* form 2: the network, fp32 [10400] (kodama_hip_mlp_train) */
+  float mlp_fallback;       /* form 2: share of cycles on the uniform proposal */
 } kodama_hip_matrix_options;
 
 /* Landmarks per run and global graph width that kodama_hip_kodama_matrix

@@ -377,13 +377,14 @@ def core_knn_batch_learned(data, run_rows, initial_labels, options: CoreOptions,
 
 def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5, k=30,
                   metric="euclidean", graph_neighbors=100, seed=1234, form="plain", gpu_device=0,
-                  n_threads=0, dissimilarity=False) -> dict:
+                  n_threads=0, dissimilarity=False, mlp_params=None, fallback=0.25) -> dict:
     """KODAMA.matrix with the kNN classifier on the HIP backend
     (KODAMAMatrix_CUDA contract; kodama_matrix.cpp:3324-4051): shared k-means
     landmark atlas, per-run quota landmarks and k-means starting labels, CoreKNN
     on the landmarks for all M runs in one batch (form "plain" replays the
-    reference streams, "device" keeps the evolution on the GPU), projection of
-    the other rows through the exact global graph.
+    reference streams, "device" keeps the evolution on the GPU, "learned" adds
+    the learned proposal of ``mlp_params`` with ``fallback`` uniform cycles),
+    projection of the other rows through the exact global graph.
 
     Returns res [M x n], acc [M], v [M x Tcycle], the landmark rows [M x L], the
     1-based global graph (indices, distances) and per-stage timings; with
@@ -391,10 +392,14 @@ def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5,
     """
     x = _matrix(data)
     n = x.shape[0]
+    if form == "learned" and mlp_params is None:
+        raise ValueError('form "learned" needs mlp_params')
+    par = None if mlp_params is None else np.ascontiguousarray(mlp_params, np.float32)
     o = N.MatrixOptions(runs=int(M), cycles=int(Tcycle), landmarks=int(landmarks), splitting=int(splitting),
                         folds=int(folds), k=int(k), metric=_metric(metric), graph_neighbors=int(graph_neighbors),
-                        seed=int(seed), form={"plain": 0, "device": 1}[form], device=int(gpu_device),
-                        n_threads=int(n_threads))
+                        seed=int(seed), form={"plain": 0, "device": 1, "learned": 2}[form], device=int(gpu_device),
+                        n_threads=int(n_threads), mlp_params=N.ptr(par) if form == "learned" else None,
+                        mlp_fallback=float(fallback))
     L = C.c_int32(0)
     K = C.c_int32(0)
     N.call("kodama_hip_matrix_sizes", C.c_int32(n), C.byref(o), C.byref(L), C.byref(K))

@@ -88,6 +88,7 @@ class MatrixOptions(C.Structure):
         ("runs", C.c_int32), ("cycles", C.c_int32), ("landmarks", C.c_int32), ("splitting", C.c_int32),
         ("folds", C.c_int32), ("k", C.c_int32), ("metric", C.c_int32), ("graph_neighbors", C.c_int32),
         ("seed", C.c_uint64), ("form", C.c_int32), ("device", C.c_int32), ("n_threads", C.c_int32),
+        ("mlp_params", C.POINTER(C.c_float)), ("mlp_fallback", C.c_float),
     ]
 
 

@@ -86,7 +86,9 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     require(o.folds >= 2, "KODAMAMatrixOptions::folds must be at least two.");
     require(o.metric >= KODAMA_HIP_COSINE && o.metric <= KODAMA_HIP_EUCLIDEAN, "unknown metric.");
     require(o.k >= 1 && o.k <= kMaxK, "knn.k must be in 1..64 on the HIP backend.");
-    require(o.form == 0 || o.form == 1, "form must be 0 (plain) or 1 (device-resident).");
+    require(o.form >= 0 && o.form <= 2, "form must be 0 (plain), 1 (device-resident) or 2 (learned).");
+    require(o.form != 2 || o.mlp_params != nullptr, "the learned form needs network parameters.");
+    require(o.form != 2 || (o.mlp_fallback >= 0.0f && o.mlp_fallback <= 1.0f), "fallback must be a probability.");
     const MatrixSizes z = resolve(n, o);
     require(z.neighbors <= (p <= 128 ? kMaxScanK : kMaxK),
             "graph neighbours above 128 (above 64 beyond 128 feature columns) are not supported by the HIP kNN kernel.");
@@ -172,8 +174,12 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     IVec clbest(static_cast<size_t>(total));
     std::vector<double> vect(static_cast<size_t>(o.runs) * std::max(1, o.cycles));
     double core_t[4] = {0, 0, 0, 0};
-    const int rc = kodama_hip_core_knn_batch(x, n, p, o.runs, offsets.data(), land_all.data(), init_all.data(),
-                                             nullptr, &co, o.form, clbest.data(), acc, vect.data(), core_t);
+    const int rc =
+      o.form == 2 ? kodama_hip_core_knn_batch_learned(x, n, p, o.runs, offsets.data(), land_all.data(),
+                                                      init_all.data(), nullptr, &co, o.mlp_params, o.mlp_fallback,
+                                                      clbest.data(), acc, vect.data(), core_t)
+                  : kodama_hip_core_knn_batch(x, n, p, o.runs, offsets.data(), land_all.data(), init_all.data(),
+                                              nullptr, &co, o.form, clbest.data(), acc, vect.data(), core_t);
     if (rc != KODAMA_HIP_OK) {
       const std::string msg = std::string("KODAMAMatrix core: ") + kodama_hip_last_error();
       if (rc == KODAMA_HIP_EINVAL) throw InvalidArgument(msg);

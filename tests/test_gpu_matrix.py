@@ -61,6 +61,26 @@ def test_device_form_shares_landmarks_and_accuracy(oracle):
     assert dev["res"].shape == (c["runs"], n)
 
 
+def test_learned_form_in_the_matrix():
+    """form "learned": with every cycle falling back to the uniform proposal it
+    is the device form exactly; with the network it keeps the landmarks, graph
+    and the accuracy level of the device form."""
+    c = dict(CASES["p10_euclidean"], runs=4, cycles=20)
+    x = _data(c.pop("n"), c.pop("p"), 5)
+    c.pop("metric")
+    kw = dict(M=c["runs"], Tcycle=c["cycles"], landmarks=c["landmarks"], splitting=c["splitting"], folds=c["folds"],
+              k=c["k"], graph_neighbors=c["graph_neighbors"], seed=c["seed"])
+    dev = K.KODAMA_matrix(x, form="device", **kw)
+    params = K.mlp_init(7)
+    same = K.KODAMA_matrix(x, form="learned", mlp_params=params, fallback=1.0, **kw)
+    assert np.array_equal(dev["res"], same["res"]) and np.array_equal(dev["v"], same["v"])
+    lrn = K.KODAMA_matrix(x, form="learned", mlp_params=params, fallback=0.25, **kw)
+    assert np.array_equal(dev["landmarks"], lrn["landmarks"])
+    assert np.array_equal(dev["knn_indices"], lrn["knn_indices"])
+    assert abs(lrn["acc"].mean() - dev["acc"].mean()) < 0.05
+    assert np.all(np.diff(lrn["v"], axis=1) >= 0)
+
+
 def test_matrix_graph_and_dissimilarity(oracle):
     c = dict(CASES["p10_euclidean"])
     x = _data(c["n"], c["p"], 5)
