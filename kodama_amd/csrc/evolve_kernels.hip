@@ -345,6 +345,10 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
       const int m = m0 + b * THREADS + tid;
       if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids_b[b]);
     }
+    // The thread's rows first (no LDS atomic between them, so their lookups
+    // overlap), then one queue allocation for all of the wave's undecided rows.
+    uint64_t mask[2 * kBatch];
+    int queued_total = 0;
 #pragma unroll
     for (int hb = 0; hb < 2 * kBatch; ++hb) {
       const int h = hb & 1;
@@ -367,12 +371,19 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
           queued = true;
         }
       }
-      const uint64_t mask = __ballot(queued);
-      if (mask) {
-        int slot = 0;
-        if (lane == 0) slot = atomicAdd(&s.scal[kQueue], __popcll(mask));
-        slot = __shfl(slot, 0);
-        if (queued) s.queue[slot + __popcll(mask & below)] = i;
+      mask[hb] = __ballot(queued);
+      queued_total += __popcll(mask[hb]);
+    }
+    if (queued_total) {
+      int slot = 0;
+      if (lane == 0) slot = atomicAdd(&s.scal[kQueue], queued_total);
+      slot = __shfl(slot, 0);
+#pragma unroll
+      for (int hb = 0; hb < 2 * kBatch; ++hb) {
+        if ((mask[hb] >> lane) & 1ull) {
+          s.queue[slot + __popcll(mask[hb] & below)] = 2 * (m0 + (hb >> 1) * THREADS + tid) + (hb & 1);
+        }
+        slot += __popcll(mask[hb]);
       }
     }
   }
