@@ -376,6 +376,9 @@ def main() -> int:
         for form, c in curves.items():
             hit = np.flatnonzero(c >= goal)
             res[form]["cycles_to_plain_accuracy"] = int(hit[0]) + 1 if hit.size else None
+            # Wall time to that accuracy, pro rata of the evolution call.
+            res[form]["seconds_to_plain_accuracy"] = \
+                res[form]["evolution_seconds"] * (int(hit[0]) + 1) / args.steps if hit.size else None
     head = "device" if "device" in res else forms[0]
     kern_s = res[head]["vote_kernel_ms"] * 1e-3
     if head == "device":
@@ -414,6 +417,13 @@ def main() -> int:
     if "plain" in res and "device" in res:
         line["accuracy_gap_pt"] = 100.0 * (res["device"]["mean_accbest"] - res["plain"]["mean_accbest"])
         line["speedup_device_vs_plain"] = res["device"]["steps_per_s"] / res["plain"]["steps_per_s"]
+    if "plain" in res and "learned" in res:
+        line["accuracy_gap_learned_pt"] = 100.0 * (res["learned"]["mean_accbest"] - res["plain"]["mean_accbest"])
+        line["speedup_learned_vs_plain"] = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
+        # Time to the plain form's final accuracy minus half a point: the plain
+        # form's own time to it against the learned form's.
+        tp, tl = res["plain"].get("seconds_to_plain_accuracy"), res["learned"].get("seconds_to_plain_accuracy")
+        line["time_to_accuracy_learned_vs_plain"] = tp / tl if tp and tl else None
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(x, rows, init, args.k, args.steps, cores, args.cpu_seconds)
