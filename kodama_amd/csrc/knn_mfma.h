@@ -8,8 +8,9 @@
 
 namespace kodama_hip {
 
-constexpr int kMfmaListLen = 8;       // candidates kept per (query, lane column)
-constexpr int kMfmaMaxK = 32;         // k served by the MFMA path
+constexpr int kMfmaMaxK = 128;        // k served by the MFMA path
+// Candidates kept per (query, lane column): 16 x this per query.
+inline int mfma_list_len(int k) { return k <= 32 ? 8 : 16; }
 constexpr int kMfmaMinTrain = 4096;   // smaller train sets use the register scan
 constexpr float kMfmaErrorScale = 8e-5f;  // E = scale * (|q|^2 + max |t|^2)
 
@@ -34,7 +35,7 @@ struct KnnMfmaParams {
   // its run in another fold. Indices stay concatenated.
   const int* run_offsets = nullptr;      // [runs + 1]
   const int* folds = nullptr;            // [rows]
-  float* cand_value = nullptr;           // [query_rows x 16 x kMfmaListLen]
+  float* cand_value = nullptr;           // [query_rows x 16 x mfma_list_len(k)]
   int* cand_index = nullptr;
   int* flags = nullptr;                  // [query_rows] 1 = needs the exact scan
   int* out_index = nullptr;              // [query_rows x k]

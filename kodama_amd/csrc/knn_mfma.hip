@@ -45,7 +45,6 @@ constexpr int kCandThreads = 512;
 constexpr int kQueriesPerBlock = 128;   // 8 waves x 16 queries
 constexpr int kChunk = 64;              // train rows per LDS stage
 constexpr int kRowStride = 72;          // bf16 per staged row (64 + 8 pad: spreads LDS banks)
-constexpr int kList = kMfmaListLen;     // candidates kept per (query, lane column)
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
   // Round to nearest even (finite inputs).
@@ -60,12 +59,13 @@ __device__ __forceinline__ float bf16_value(unsigned short h) {
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // Ascending list insert; a value equal to the largest kept one is dropped.
-__device__ __forceinline__ void list_insert(float (&d)[kList], int (&id)[kList], float v, int j) {
-  if (!(v < d[kList - 1])) return;
-  d[kList - 1] = v;
-  id[kList - 1] = j;
+template <int LIST>
+__device__ __forceinline__ void list_insert(float (&d)[LIST], int (&id)[LIST], float v, int j) {
+  if (!(v < d[LIST - 1])) return;
+  d[LIST - 1] = v;
+  id[LIST - 1] = j;
 #pragma unroll
-  for (int s = kList - 1; s > 0; --s) {
+  for (int s = LIST - 1; s > 0; --s) {
     if (d[s] < d[s - 1]) {
       const float td = d[s - 1];
       const int ti = id[s - 1];
@@ -142,7 +142,7 @@ __global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, 
   norm2[r] = s;
 }
 
-template <bool EUCLID, bool TABLES>
+template <bool EUCLID, bool TABLES, int LIST>
 __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
   __shared__ unsigned short s_hi[kChunk * kRowStride];
   __shared__ unsigned short s_lo[kChunk * kRowStride];
@@ -192,12 +192,12 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
       excl[r] = (q < q_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
     }
   }
-  float kd[4][kList];
-  int ki[4][kList];
+  float kd[4][LIST];
+  int ki[4][LIST];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
 #pragma unroll
-    for (int s = 0; s < kList; ++s) {
+    for (int s = 0; s < LIST; ++s) {
       kd[r][s] = INFINITY;
       ki[r][s] = -1;
     }
@@ -256,21 +256,21 @@ __global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMf
         for (int r = 0; r < 4; ++r) {
           if (skip != excl[r]) {
             const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
-            list_insert(kd[r], ki[r], v, t_base + j);
+            list_insert<LIST>(kd[r], ki[r], v, t_base + j);
           }
         }
       }
     }
   }
 
-  // Lists: query q owns [16 columns x kList] entries, column-major by lane column.
+  // Lists: query q owns [16 columns x LIST] entries, column-major by lane column.
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int q = q_tile + 4 * grp + r;
     if (q < q_rows) {
-      const size_t base = (static_cast<size_t>(q_base + q) * 16 + col) * kList;
+      const size_t base = (static_cast<size_t>(q_base + q) * 16 + col) * LIST;
 #pragma unroll
-      for (int s = 0; s < kList; ++s) {
+      for (int s = 0; s < LIST; ++s) {
         prm.cand_value[base + s] = kd[r][s];
         prm.cand_index[base + s] = ki[r][s];
       }
@@ -287,38 +287,49 @@ __device__ __forceinline__ float key_value(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// Ascending bitonic sort of 128 keys over a wave: element e is register e / 64
-// of lane e % 64 (a: e = lane, b: e = 64 + lane).
-__device__ __forceinline__ void wave_sort128(uint32_t& a, uint32_t& b, int lane) {
+// Ascending bitonic sort of 64 R keys over a wave: element e is register
+// e / 64 of lane e % 64.
+template <int R>
+__device__ __forceinline__ void wave_sort(uint32_t (&v)[R], int lane) {
+  constexpr int kN = 64 * R;
 #pragma unroll
-  for (int size = 2; size <= 128; size <<= 1) {
+  for (int size = 2; size <= kN; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride == 64) {  // size 128: between the two registers, ascending
-        const uint32_t lo = min(a, b), hi = max(a, b);
-        a = lo;
-        b = hi;
+      if (stride >= 64) {  // between registers of the same lane
+        const int rs = stride / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((r & rs) == 0) {
+            const bool asc = ((r * 64 + lane) & size) == 0;
+            const uint32_t lo = min(v[r], v[r + rs]), hi = max(v[r], v[r + rs]);
+            v[r] = asc ? lo : hi;
+            v[r + rs] = asc ? hi : lo;
+          }
+        }
         continue;
       }
       const bool lower = (lane & stride) == 0;
-      const uint32_t pa = __shfl_xor(a, stride), pb = __shfl_xor(b, stride);
-      const bool asc_a = (lane & size) == 0;
-      const bool asc_b = ((64 + lane) & size) == 0;
-      a = lower == asc_a ? min(a, pa) : max(a, pa);
-      b = lower == asc_b ? min(b, pb) : max(b, pb);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t other = __shfl_xor(v[r], stride);
+        const bool asc = ((r * 64 + lane) & size) == 0;
+        v[r] = lower == asc ? min(v[r], other) : max(v[r], other);
+      }
     }
   }
 }
 
-// One wave per query; 4 queries per 256-thread workgroup. Each lane holds two
-// of the query's 128 candidates. A_k (the k-th smallest candidate value; ties
-// of value do not move it) comes from a bitonic sort of the values over the
-// wave; the exact re-scores of the candidates within the bound are ranked by
-// (distance, index) among themselves.
-template <bool EUCLID>
+// One wave per query; 4 queries per 256-thread workgroup. Each lane holds R =
+// LIST / 4 of the query's 16 LIST candidates. A_k (the k-th smallest
+// candidate value; ties of value do not move it) comes from a bitonic sort of
+// the values over the wave; the exact re-scores of the candidates within the
+// bound are ranked by (distance, index) among themselves.
+template <bool EUCLID, int LIST>
 __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
-  constexpr int kCand = 16 * kList;  // 128
-  static_assert(kCand == 2 * kWave, "two candidates per lane");
+  constexpr int kCand = 16 * LIST;
+  constexpr int R = kCand / kWave;
+  static_assert(R * kWave == kCand, "whole registers of candidates per lane");
   __shared__ float s_ex[4][kCand];
   __shared__ int s_id[4][kCand];
   const int wave = threadIdx.x / kWave;
@@ -329,19 +340,29 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   const int k = prm.k;
   const int p = prm.dims;
   const size_t base = static_cast<size_t>(q) * kCand;
-  const float v0 = prm.cand_value[base + lane], v1 = prm.cand_value[base + kWave + lane];
-  const int i0 = prm.cand_index[base + lane], i1 = prm.cand_index[base + kWave + lane];
-  const int valid = __popcll(__ballot(i0 >= 0)) + __popcll(__ballot(i1 >= 0));
-  uint32_t sa = i0 >= 0 ? order_key(v0) : 0xFFFFFFFFu, sb = i1 >= 0 ? order_key(v1) : 0xFFFFFFFFu;
-  wave_sort128(sa, sb, lane);
-  // Rank k - 1 < 64 (k <= 32) sits in register a of lane k - 1.
-  const float ak = key_value(__shfl(sa, k - 1));
+  float val[R];
+  int idx[R];
+  uint32_t key[R];
+  int valid = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    val[r] = prm.cand_value[base + r * kWave + lane];
+    idx[r] = prm.cand_index[base + r * kWave + lane];
+    valid += __popcll(__ballot(idx[r] >= 0));
+    key[r] = idx[r] >= 0 ? order_key(val[r]) : 0xFFFFFFFFu;
+  }
+  wave_sort<R>(key, lane);
+  // Rank k - 1 sits in register (k - 1) / 64 of lane (k - 1) % 64.
+  uint32_t kth = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) kth = r == (k - 1) / kWave ? key[r] : kth;
+  const float ak = key_value(__shfl(kth, (k - 1) % kWave));
   // Fewer valid candidates than k: every one is kept.
   const float bound = valid >= k ? ak + 2.0f * (prm.error_scale * (prm.query_norm2[q] + prm.train_norm2_max)) : INFINITY;
   // A full list whose largest value is within the bound may have dropped a row.
   bool overflow = false;
   if (lane < 16) {
-    const size_t last = base + lane * kList + kList - 1;
+    const size_t last = base + lane * LIST + LIST - 1;
     overflow = prm.cand_index[last] >= 0 && prm.cand_value[last] <= bound;
   }
   if (__ballot(overflow) != 0ull) {
@@ -349,49 +370,50 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
     return;
   }
   if (lane == 0) prm.flags[q] = 0;
-  // Exact re-score of the candidates within the bound.
+  // Exact re-score of the candidates within the bound, compacted into the
+  // wave's LDS slice.
   float qv[64];
   const float* qrow = prm.query + static_cast<size_t>(q) * p;
 #pragma unroll
   for (int d = 0; d < 64; ++d) qv[d] = d < p ? qrow[d] : 0.0f;
-  const bool u0 = i0 >= 0 && v0 <= bound, u1 = i1 >= 0 && v1 <= bound;
-  const float e0 = u0 ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(i0) * p, p) : INFINITY;
-  const float e1 = u1 ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(i1) * p, p) : INFINITY;
-  // Compact the selected candidates into the wave's LDS slice.
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint64_t m0 = __ballot(u0), m1 = __ballot(u1);
-  const int n0 = __popcll(m0), nsel = n0 + __popcll(m1);
   float* ex = s_ex[wave];
   int* id = s_id[wave];
-  const int p0 = __popcll(m0 & below), p1 = n0 + __popcll(m1 & below);
-  if (u0) {
-    ex[p0] = e0;
-    id[p0] = i0;
-  }
-  if (u1) {
-    ex[p1] = e1;
-    id[p1] = i1;
+  float e[R];
+  bool use[R];
+  int nsel = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    use[r] = idx[r] >= 0 && val[r] <= bound;
+    e[r] = use[r] ? exact_distance<EUCLID>(qv, prm.train + static_cast<size_t>(idx[r]) * p, p) : INFINITY;
+    const uint64_t m = __ballot(use[r]);
+    if (use[r]) {
+      ex[nsel + __popcll(m & below)] = e[r];
+      id[nsel + __popcll(m & below)] = idx[r];
+    }
+    nsel += __popcll(m);
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  int r0 = 0, r1 = 0;
-  for (int e = 0; e < nsel; ++e) {
-    const float x = ex[e];
-    const int j = id[e];
-    r0 += (x < e0 || (x == e0 && j < i0)) ? 1 : 0;
-    r1 += (x < e1 || (x == e1 && j < i1)) ? 1 : 0;
+  int rank[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) rank[r] = 0;
+  for (int t = 0; t < nsel; ++t) {
+    const float x = ex[t];
+    const int j = id[t];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rank[r] += (x < e[r] || (x == e[r] && j < idx[r])) ? 1 : 0;
   }
-  if (u0 && r0 < k) {
-    prm.out_index[static_cast<size_t>(q) * k + r0] = i0;
-    prm.out_dist[static_cast<size_t>(q) * k + r0] = e0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (use[r] && rank[r] < k) {
+      prm.out_index[static_cast<size_t>(q) * k + rank[r]] = idx[r];
+      prm.out_dist[static_cast<size_t>(q) * k + rank[r]] = e[r];
+    }
   }
-  if (u1 && r1 < k) {
-    prm.out_index[static_cast<size_t>(q) * k + r1] = i1;
-    prm.out_dist[static_cast<size_t>(q) * k + r1] = e1;
-  }
-  for (int s = nsel + lane; s < k; s += kWave) {
-    prm.out_index[static_cast<size_t>(q) * k + s] = -1;
-    prm.out_dist[static_cast<size_t>(q) * k + s] = INFINITY;
+  for (int t = nsel + lane; t < k; t += kWave) {
+    prm.out_index[static_cast<size_t>(q) * k + t] = -1;
+    prm.out_dist[static_cast<size_t>(q) * k + t] = INFINITY;
   }
 }
 
@@ -404,20 +426,28 @@ hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi
   return hipGetLastError();
 }
 
+template <int LIST>
+static void launch_mfma_list(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
+  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
+  if (tables) {
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+  } else {
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+  }
+  if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
+  else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
+}
+
 hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s) {
   if (prm.query_rows == 0) return hipSuccess;
   if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.train_rows < 1) {
     return hipErrorInvalidValue;
   }
-  const unsigned blocks = static_cast<unsigned>((prm.query_rows + kQueriesPerBlock - 1) / kQueriesPerBlock);
-  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
-  if (euclid) {
-    hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false>), dim3(blocks), dim3(kCandThreads), 0, s, prm);
-    hipLaunchKernelGGL(knn_rerank_kernel<true>, dim3(rblocks), dim3(256), 0, s, prm);
-  } else {
-    hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false>), dim3(blocks), dim3(kCandThreads), 0, s, prm);
-    hipLaunchKernelGGL(knn_rerank_kernel<false>, dim3(rblocks), dim3(256), 0, s, prm);
-  }
+  const dim3 grid(static_cast<unsigned>((prm.query_rows + kQueriesPerBlock - 1) / kQueriesPerBlock));
+  if (mfma_list_len(prm.k) == 8) launch_mfma_list<8>(prm, euclid, false, grid, s);
+  else launch_mfma_list<16>(prm, euclid, false, grid, s);
   return hipGetLastError();
 }
 
@@ -429,14 +459,8 @@ hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int run
   }
   const dim3 grid(static_cast<unsigned>((max_rows + kQueriesPerBlock - 1) / kQueriesPerBlock),
                   static_cast<unsigned>(runs));
-  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
-  if (euclid) {
-    hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true>), grid, dim3(kCandThreads), 0, s, prm);
-    hipLaunchKernelGGL(knn_rerank_kernel<true>, dim3(rblocks), dim3(256), 0, s, prm);
-  } else {
-    hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true>), grid, dim3(kCandThreads), 0, s, prm);
-    hipLaunchKernelGGL(knn_rerank_kernel<false>, dim3(rblocks), dim3(256), 0, s, prm);
-  }
+  if (mfma_list_len(prm.k) == 8) launch_mfma_list<8>(prm, euclid, true, grid, s);
+  else launch_mfma_list<16>(prm, euclid, true, grid, s);
   return hipGetLastError();
 }
 

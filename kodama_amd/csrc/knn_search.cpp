@@ -81,7 +81,7 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
     q_n2.resize(static_cast<size_t>(query_rows));
     check(launch_split_rows(d_query, query_rows, dims, q_hi.get(), q_lo.get(), q_n2.get(), s), "split_rows launch");
   }
-  const size_t cand = static_cast<size_t>(query_rows) * 16 * kMfmaListLen;
+  const size_t cand = static_cast<size_t>(query_rows) * 16 * mfma_list_len(k);
   DeviceBuffer<float> c_val(cand);
   DeviceBuffer<int> c_idx(cand), flags(static_cast<size_t>(query_rows));
   KnnMfmaParams m;
@@ -201,7 +201,7 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     DeviceBuffer<unsigned short> hi(static_cast<size_t>(total) * 64), lo(hi.size());
     DeviceBuffer<float> n2(static_cast<size_t>(total));
     check(launch_split_rows(xc.get(), total, job.p, hi.get(), lo.get(), n2.get(), s), "split_rows launch");
-    const size_t cand = static_cast<size_t>(total) * 16 * kMfmaListLen;
+    const size_t cand = static_cast<size_t>(total) * 16 * mfma_list_len(job.k);
     DeviceBuffer<float> c_val(cand), r_dist(static_cast<size_t>(total) * job.k);
     DeviceBuffer<int> c_idx(cand), flags(static_cast<size_t>(total)), r_idx(r_dist.size());
     KnnMfmaParams m;

@@ -39,7 +39,7 @@ def _check(oracle, x, rows, fids, k, metric, paths=("scan", "mfma")):
 
 
 @pytest.mark.parametrize("p", [7, 50, 64])
-@pytest.mark.parametrize("k", [5, 30])
+@pytest.mark.parametrize("k", [5, 30, 40])
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])
 def test_fold_tables_match_oracle(oracle, p, k, metric):
     x, _ = D.config3(n=3000, p=p, seed=p)

@@ -21,6 +21,19 @@ def test_graph_via_mfma_matches_oracle(oracle, metric, name):
     assert np.array_equal(g["distances"], dist)
 
 
+@pytest.mark.parametrize("k", [33, 100])
+def test_wide_graph_via_mfma_matches_oracle(oracle, k):
+    """More than 32 neighbours: 16-entry candidate lists per lane column (256
+    candidates per query), the same certification and exact rerank."""
+    x, _ = D.config3(n=6000, p=50, seed=k)
+    g = K.KODAMA_knn_graph(x, k=k)
+    fb = K.last_knn_fallbacks()
+    assert 0 <= fb < 600, fb  # the MFMA path ran and certified most queries
+    idx, dist = oracle.knn_graph(x, k, 2)
+    assert np.array_equal(g["indices"], idx)
+    assert np.array_equal(g["distances"], dist)
+
+
 def test_search_with_exclusion_and_dimension_tail(oracle):
     x, _ = D.config3(n=5700, p=40, seed=5)
     train, query = x[:5000], x[5000:]
@@ -46,7 +59,7 @@ def test_duplicate_rows_take_the_exact_fallback(oracle):
     assert np.array_equal(g["distances"], dist)
 
 
-@pytest.mark.parametrize("p,k,mfma", [(64, 32, True), (65, 16, False), (30, 33, False)])
+@pytest.mark.parametrize("p,k,mfma", [(64, 32, True), (65, 16, False), (30, 33, True), (64, 128, True)])
 def test_path_boundaries(oracle, p, k, mfma):
     x, _ = D.config3(n=4200, p=p, seed=p + k)
     idx, dist = K.exact_knn_search(x[:4100], x[4100:], k, "euclidean")
