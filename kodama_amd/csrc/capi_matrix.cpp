@@ -128,24 +128,28 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     // Per run r = 1..M: mt19937_64(seed + r) skips one draw (the atlas is
     // shared), quota-samples the landmarks and draws the k-means starting
     // labels of the landmark rows (kodama_matrix.cpp:2897, 2937-2949, 3088-3103).
-    IVec offsets(static_cast<size_t>(o.runs) + 1, 0), land_all, init_all;
-    land_all.reserve(static_cast<size_t>(o.runs) * L);
-    init_all.reserve(land_all.capacity());
+    // The quota samples first (host threads, one stream per run), then the
+    // starting-label k-means of all runs in lockstep on the device.
+    IVec offsets(static_cast<size_t>(o.runs) + 1, 0), land_all(static_cast<size_t>(o.runs) * L), init_all;
+    init_all.reserve(land_all.size());
     {
-      std::vector<float> xl(static_cast<size_t>(L) * p);
-      DeviceBuffer<float> d_xl(xl.size());
-      const int init_k = std::max(2, std::min(z.splitting, L));
-      for (int r = 1; r <= o.runs; ++r) {
-        std::mt19937_64 rng(o.seed + static_cast<std::uint64_t>(r));
-        (void)rng();
-        const IVec land = quota_sample_landmarks(strata_off, strata_rows, L, rng);
-        for (int i = 0; i < L; ++i) std::copy_n(x + static_cast<size_t>(land[i]) * p, p, xl.data() + static_cast<size_t>(i) * p);
-        d_xl.upload(xl.data(), xl.size(), s);
-        const IVec xw = device_kmeans(xl.data(), d_xl.get(), L, p, init_k, rng, 10, s);
-        land_all.insert(land_all.end(), land.begin(), land.end());
-        init_all.insert(init_all.end(), xw.begin(), xw.end());
-        offsets[r] = offsets[r - 1] + L;
+      std::vector<std::mt19937_64> rngs(static_cast<size_t>(o.runs));
+#pragma omp parallel for schedule(dynamic)
+      for (int r = 0; r < o.runs; ++r) {
+        rngs[r].seed(o.seed + static_cast<std::uint64_t>(r + 1));
+        (void)rngs[r]();
+        const IVec land = quota_sample_landmarks(strata_off, strata_rows, L, rngs[r]);
+        std::copy(land.begin(), land.end(), land_all.begin() + static_cast<size_t>(r) * L);
       }
+      for (int r = 1; r <= o.runs; ++r) offsets[r] = offsets[r - 1] + L;
+      DeviceBuffer<int> d_land(land_all.size());
+      DeviceBuffer<float> d_xl(land_all.size() * static_cast<size_t>(p));
+      d_land.upload(land_all.data(), land_all.size(), s);
+      check(launch_gather_rows(d_x.get(), d_land.get(), static_cast<int>(land_all.size()), p, d_xl.get(), s),
+            "gather_rows launch");
+      const int init_k = std::max(2, std::min(z.splitting, L));
+      const std::vector<IVec> xw = device_kmeans_batch(x, land_all.data(), d_xl.get(), o.runs, L, p, init_k, rngs, 10, s);
+      for (const IVec& w : xw) init_all.insert(init_all.end(), w.begin(), w.end());
     }
     t[2] = seconds_since(clock);
 

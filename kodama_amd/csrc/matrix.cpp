@@ -147,6 +147,87 @@ IVec device_kmeans(const float* h_x, const float* d_x, int n, int p, int k, std:
   return labels;
 }
 
+std::vector<IVec> device_kmeans_batch(const float* h_x, const int* rows, const float* d_rows, int runs, int n, int p,
+                                      int k, std::vector<std::mt19937_64>& rngs, int max_iter, hipStream_t s) {
+  k = std::max(1, std::min(k, n));
+  const size_t kp = static_cast<size_t>(k) * p;
+  std::vector<float> cent(static_cast<size_t>(runs) * kp), sizes(static_cast<size_t>(runs) * k);
+  for (int r = 0; r < runs; ++r) {
+    const std::uint64_t kseed = rngs[r]() & 0x7fffffffULL;
+    const IVec order = seed_order(n, kseed + 1u);
+    for (int c = 0; c < k; ++c) {
+      const int data_row = rows[static_cast<size_t>(r) * n + order[c]];
+      std::copy_n(h_x + static_cast<size_t>(data_row) * p, p, cent.data() + r * kp + static_cast<size_t>(c) * p);
+    }
+  }
+  DeviceBuffer<float> d_xn(static_cast<size_t>(runs) * n), d_cent(cent.size()), d_cn(static_cast<size_t>(runs) * k),
+    d_sizes(sizes.size());
+  DeviceBuffer<int> d_asg(static_cast<size_t>(runs) * n), d_changed(static_cast<size_t>(runs)),
+    d_active(static_cast<size_t>(runs));
+  check(launch_sse_norms(d_rows, runs * n, p, d_xn.get(), s), "sse_norms launch");
+  check(hipMemsetAsync(d_asg.get(), 0xFF, sizeof(int) * d_asg.size(), s), "memset");  // -1: unassigned
+  // Phase of each problem: Lloyd rounds (assign, update, split), then one
+  // final assignment with the last centroids, then done.
+  enum Phase { kLoop, kFinal, kDone };
+  std::vector<int> phase(static_cast<size_t>(runs), kLoop), rounds(static_cast<size_t>(runs), 0),
+    act(static_cast<size_t>(runs)), changed(static_cast<size_t>(runs));
+  for (;;) {
+    bool any = false;
+    for (int r = 0; r < runs; ++r) {
+      act[r] = phase[r] != kDone ? 1 : 0;
+      any = any || act[r];
+    }
+    if (!any) break;
+    d_cent.upload(cent.data(), cent.size(), s);
+    check(launch_sse_norms(d_cent.get(), runs * k, p, d_cn.get(), s), "sse_norms launch");
+    check(hipMemsetAsync(d_changed.get(), 0, sizeof(int) * runs, s), "memset");
+    d_active.upload(act.data(), act.size(), s);
+    check(launch_kmeans_assign(d_rows, n, p, d_xn.get(), d_cent.get(), d_cn.get(), k, d_asg.get(), d_changed.get(), s,
+                               runs, d_active.get()),
+          "kmeans_assign launch");
+    d_changed.download(changed.data(), changed.size(), s);
+    check(hipStreamSynchronize(s), "kmeans sync");  // act and changed are reused below
+    bool update = false;
+    for (int r = 0; r < runs; ++r) {
+      act[r] = phase[r] == kLoop ? 1 : 0;
+      update = update || act[r];
+    }
+    if (update) {
+      d_active.upload(act.data(), act.size(), s);
+      check(launch_kmeans_update(d_rows, n, p, d_asg.get(), k, d_cent.get(), d_sizes.get(), s, runs, d_active.get()),
+            "kmeans_update launch");
+      d_cent.download(cent.data(), cent.size(), s);
+      d_sizes.download(sizes.data(), sizes.size(), s);
+      check(hipStreamSynchronize(s), "kmeans sync");
+#pragma omp parallel for schedule(dynamic)
+      for (int r = 0; r < runs; ++r) {
+        if (!act[r]) continue;
+        std::vector<float> sz(sizes.begin() + static_cast<size_t>(r) * k, sizes.begin() + static_cast<size_t>(r + 1) * k);
+        std::vector<float> ce(cent.begin() + r * kp, cent.begin() + (r + 1) * kp);
+        split_empty_clusters(n, p, k, sz, ce);
+        std::copy(ce.begin(), ce.end(), cent.begin() + r * kp);
+      }
+    }
+    for (int r = 0; r < runs; ++r) {
+      if (phase[r] == kFinal) {
+        phase[r] = kDone;
+      } else if (phase[r] == kLoop) {
+        ++rounds[r];
+        if (changed[r] == 0 || rounds[r] >= std::max(1, max_iter)) phase[r] = kFinal;
+      }
+    }
+  }
+  std::vector<int> flat(static_cast<size_t>(runs) * n);
+  d_asg.download(flat.data(), flat.size(), s);
+  check(hipStreamSynchronize(s), "kmeans sync");
+  std::vector<IVec> labels(static_cast<size_t>(runs));
+  for (int r = 0; r < runs; ++r) {
+    labels[r].assign(flat.begin() + static_cast<size_t>(r) * n, flat.begin() + static_cast<size_t>(r + 1) * n);
+    for (int& l : labels[r]) ++l;
+  }
+  return labels;
+}
+
 void index_strata(const IVec& labels, int strata, IVec& offsets, IVec& rows) {
   if (strata < 1) throw InvalidArgument("landmark strata count must be positive.");
   offsets.assign(static_cast<size_t>(strata) + 1, 0);

@@ -20,6 +20,14 @@ namespace kodama_hip {
 IVec device_kmeans(const float* h_x, const float* d_x, int n, int p, int k, std::mt19937_64& rng, int max_iter,
                    hipStream_t s);
 
+// device_kmeans of `runs` problems of n rows each, in lockstep: problem r's
+// rows are data rows rows[r * n .. (r + 1) * n) of h_x (host, for the seeds)
+// gathered into d_rows [runs x n x p] (device), its draw comes from rngs[r].
+// Every launch serves all problems still iterating; the results equal runs
+// separate device_kmeans calls.
+std::vector<IVec> device_kmeans_batch(const float* h_x, const int* rows, const float* d_rows, int runs, int n, int p,
+                                      int k, std::vector<std::mt19937_64>& rngs, int max_iter, hipStream_t s);
+
 // Rows of each 1-based stratum in ascending order (kodama_matrix.cpp:800-824).
 void index_strata(const IVec& labels, int strata, IVec& offsets, IVec& rows);
 

@@ -87,10 +87,20 @@ __global__ void __launch_bounds__(kBlock) kmeans_assign_kernel(const float* __re
                                                                 const float* __restrict__ x_norm,
                                                                 const float* __restrict__ centroids,
                                                                 const float* __restrict__ c_norm, int k,
-                                                                int* __restrict__ assign, int* __restrict__ changed) {
+                                                                int* __restrict__ assign, int* __restrict__ changed,
+                                                                const int* __restrict__ active_runs) {
   constexpr int kTile = 8192 / PMAX;  // centroids per 32 KiB LDS tile
   __shared__ float tile[kTile * PMAX];
   __shared__ float tile_norm[kTile];
+  // Problem blockIdx.y of a batch of equal-sized ones (rows n, clusters k).
+  const int run = blockIdx.y;
+  if (active_runs != nullptr && !active_runs[run]) return;  // whole block
+  x += static_cast<size_t>(run) * n * p;
+  x_norm += static_cast<size_t>(run) * n;
+  centroids += static_cast<size_t>(run) * k * p;
+  c_norm += static_cast<size_t>(run) * k;
+  assign += static_cast<size_t>(run) * n;
+  changed += run;
   const int row = blockIdx.x * kBlock + threadIdx.x;
   const bool active = row < n;
   float q[PMAX];
@@ -129,9 +139,16 @@ __global__ void __launch_bounds__(kBlock) kmeans_assign_kernel(const float* __re
 __global__ void __launch_bounds__(kBlock) kmeans_update_kernel(const float* __restrict__ x, int n, int p,
                                                                 const int* __restrict__ assign, int k,
                                                                 float* __restrict__ centroids,
-                                                                float* __restrict__ sizes) {
+                                                                float* __restrict__ sizes,
+                                                                const int* __restrict__ active_runs) {
   constexpr int kChunk = 4 * kBlock;  // rows per ballot round
   __shared__ uint64_t masks[kChunk / 64];
+  const int run = blockIdx.y;
+  if (active_runs != nullptr && !active_runs[run]) return;  // whole block
+  x += static_cast<size_t>(run) * n * p;
+  assign += static_cast<size_t>(run) * n;
+  centroids += static_cast<size_t>(run) * k * p;
+  sizes += static_cast<size_t>(run) * k;
   const int c = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int f = threadIdx.x;
@@ -228,26 +245,28 @@ hipError_t launch_sse_norms(const float* x, int rows, int p, float* out, hipStre
 
 template <int PMAX>
 static void launch_assign_p(const float* x, int n, int p, const float* xn, const float* cent, const float* cn,
-                            int k, int* assign, int* changed, hipStream_t s) {
-  hipLaunchKernelGGL((kmeans_assign_kernel<PMAX>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, x, n, p,
-                     xn, cent, cn, k, assign, changed);
+                            int k, int* assign, int* changed, int runs, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL((kmeans_assign_kernel<PMAX>), dim3((n + kBlock - 1) / kBlock, runs), dim3(kBlock), 0, s, x, n,
+                     p, xn, cent, cn, k, assign, changed, active);
 }
 
 hipError_t launch_kmeans_assign(const float* x, int n, int p, const float* x_norm, const float* centroids,
-                                const float* c_norm, int k, int* assign, int* changed, hipStream_t s) {
-  if (p < 1 || p > 256 || k < 1) return hipErrorInvalidValue;
+                                const float* c_norm, int k, int* assign, int* changed, hipStream_t s, int runs,
+                                const int* active) {
+  if (p < 1 || p > 256 || k < 1 || runs < 1) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  if (p <= 32) launch_assign_p<32>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, s);
-  else if (p <= 64) launch_assign_p<64>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, s);
-  else if (p <= 128) launch_assign_p<128>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, s);
-  else launch_assign_p<256>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, s);
+  if (p <= 32) launch_assign_p<32>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, runs, active, s);
+  else if (p <= 64) launch_assign_p<64>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, runs, active, s);
+  else if (p <= 128) launch_assign_p<128>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, runs, active, s);
+  else launch_assign_p<256>(x, n, p, x_norm, centroids, c_norm, k, assign, changed, runs, active, s);
   return hipGetLastError();
 }
 
 hipError_t launch_kmeans_update(const float* x, int n, int p, const int* assign, int k, float* centroids,
-                                float* sizes, hipStream_t s) {
-  if (p < 1 || p > kBlock || k < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kmeans_update_kernel, dim3(k), dim3(kBlock), 0, s, x, n, p, assign, k, centroids, sizes);
+                                float* sizes, hipStream_t s, int runs, const int* active) {
+  if (p < 1 || p > kBlock || k < 1 || runs < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kmeans_update_kernel, dim3(k, runs), dim3(kBlock), 0, s, x, n, p, assign, k, centroids, sizes,
+                     active);
   return hipGetLastError();
 }
 

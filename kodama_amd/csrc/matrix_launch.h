@@ -14,13 +14,19 @@ hipError_t launch_sse_norms(const float* x, int rows, int p, float* out, hipStre
 
 // Nearest centroid of every row (smallest (|x|^2 + |c|^2) - 2 x.c, first
 // cluster on ties); rows whose assignment changes are counted in *changed.
+// With runs > 1, `runs` problems of n rows and k clusters each, laid out one
+// after another (x, x_norm, centroids, c_norm, assign, changed strided by
+// problem), solved by one launch; active (may be null) skips problems whose
+// entry is 0.
 hipError_t launch_kmeans_assign(const float* x, int n, int p, const float* x_norm, const float* centroids,
-                                const float* c_norm, int k, int* assign, int* changed, hipStream_t s);
+                                const float* c_norm, int k, int* assign, int* changed, hipStream_t s,
+                                int runs = 1, const int* active = nullptr);
 
 // Per cluster: member rows summed in ascending row order, then scaled by
 // 1 / count (clusters without members stay zero); sizes as float counts.
+// Batched as launch_kmeans_assign.
 hipError_t launch_kmeans_update(const float* x, int n, int p, const int* assign, int k, float* centroids,
-                                float* sizes, hipStream_t s);
+                                float* sizes, hipStream_t s, int runs = 1, const int* active = nullptr);
 
 struct ProjectParams {
   const int* graph = nullptr;        // [n x neighbors] 0-based ids, -1 = none
