@@ -107,7 +107,7 @@ const char* kodama_hip_version(void);
 /* No counterpart in the reference: for callers that check which path the last
  * exact kNN search of the calling thread took (kodama_hip_exact_knn_search,
  * kodama_hip_knn_graph, and the per-fold tables of the kNN classifier calls).
- * With at most 64 dimensions, k <= 32 and (for a search) at least 4096 train
+ * With at most 64 dimensions, k <= 128 and (for a search) at least 4096 train
  * rows the search runs split-BF16 MFMA candidates with an exact FP32 rerank and
  * returns the number of queries (table rows) that fell back to the exact scan;
  * -1 when the exact scan served every query. Results are identical either way. */
@@ -123,7 +123,7 @@ int kodama_hip_last_knn_fallbacks(void);
  * -inf past the available rows -- the reference's per-fold exact tables,
  * bit-identical. path: 1 the register scan in the reference's summation
  * order; 2 split-BF16 MFMA candidates with the exact FP32 rerank (p <= 64,
- * k <= 32; otherwise the scan); 0 the process-wide setting below. Both paths
+ * k <= 128; otherwise the scan); 0 the process-wide setting below. Both paths
  * give identical tables. *path_used (may be NULL) = the path that ran. */
 int kodama_hip_knn_fold_tables(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
                                const int32_t* rows, const int32_t* fold_ids, int32_t k, int32_t metric,
