@@ -21,15 +21,15 @@ def test_graph_via_mfma_matches_oracle(oracle, metric, name):
     assert np.array_equal(g["distances"], dist)
 
 
-@pytest.mark.parametrize("k", [33, 100])
-def test_wide_graph_via_mfma_matches_oracle(oracle, k):
+@pytest.mark.parametrize("k,metric,name", [(33, 2, "euclidean"), (100, 2, "euclidean"), (100, 0, "cosine")])
+def test_wide_graph_via_mfma_matches_oracle(oracle, k, metric, name):
     """More than 32 neighbours: 16-entry candidate lists per lane column (256
     candidates per query), the same certification and exact rerank."""
     x, _ = D.config3(n=6000, p=50, seed=k)
-    g = K.KODAMA_knn_graph(x, k=k)
+    g = K.KODAMA_knn_graph(x, k=k, metric=name)
     fb = K.last_knn_fallbacks()
     assert 0 <= fb < 600, fb  # the MFMA path ran and certified most queries
-    idx, dist = oracle.knn_graph(x, k, 2)
+    idx, dist = oracle.knn_graph(x, k, metric)
     assert np.array_equal(g["indices"], idx)
     assert np.array_equal(g["distances"], dist)
 
