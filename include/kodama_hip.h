@@ -325,6 +325,22 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p,
                              double* v, int32_t* landmark_rows, int32_t* graph_indices_one_based,
                              float* graph_distances, double* timings);
 
+/* kodama::KODAMAMatrixFromGraphData_CUDA (kodama_matrix.cpp:4676-4690):
+ * kodama_hip_kodama_matrix with the caller's neighbour graph in place of the
+ * exact global graph. graph_indices / graph_distances_in [n x graph_neighbors]
+ * row-major; index_base 0, 1, or -1 (auto: one-based when every id lies in
+ * 1..n), normalised as normalize_external_graph (kodama_matrix.cpp:1292-1330):
+ * per row the first K = min(graph_neighbors, resolved graph width) valid
+ * neighbours (in range, not the row itself, finite distance), distances
+ * clamped at 0. The graph outputs (may be NULL) are [n x K], 1-based, the
+ * caller's distances. */
+int kodama_hip_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const int32_t* graph_indices,
+                                        const float* graph_distances_in, int32_t graph_neighbors,
+                                        int32_t index_base, const kodama_hip_matrix_options* options,
+                                        int32_t* res, double* acc, double* v, int32_t* landmark_rows,
+                                        int32_t* graph_indices_one_based, float* graph_distances,
+                                        double* timings);
+
 /* ---- learned proposal -------------------------------------------------------
  * No counterpart in the reference: the learned proposal that the device-resident
  * evolution can use in place of the uniform one, for callers that train it on

@@ -420,6 +420,47 @@ def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5,
     return out
 
 
+def KODAMA_matrix_from_graph(data, indices, distances, index_base="auto", M=100, Tcycle=20, landmarks=10000,
+                             splitting=0, folds=5, k=30, metric="euclidean", graph_neighbors=100, seed=1234,
+                             form="plain", gpu_device=0, n_threads=0, mlp_params=None, fallback=0.25) -> dict:
+    """KODAMAMatrixFromGraphData: KODAMA_matrix with the caller's neighbour graph
+    (indices / distances [n x neighbours], index_base "auto", 0 or 1) instead of
+    the exact global graph; returns the same dictionary (graph outputs [n x K],
+    K = min(neighbours, resolved width))."""
+    x = _matrix(data)
+    n = x.shape[0]
+    gi_in = N.i32(np.ascontiguousarray(indices))
+    gd_in = np.ascontiguousarray(distances, np.float32)
+    if gi_in.shape != gd_in.shape or gi_in.shape[0] != n:
+        raise ValueError("indices and distances must both be [n x neighbours]")
+    if form == "learned" and mlp_params is None:
+        raise ValueError('form "learned" needs mlp_params')
+    par = None if mlp_params is None else np.ascontiguousarray(mlp_params, np.float32)
+    o = N.MatrixOptions(runs=int(M), cycles=int(Tcycle), landmarks=int(landmarks), splitting=int(splitting),
+                        folds=int(folds), k=int(k), metric=_metric(metric), graph_neighbors=int(graph_neighbors),
+                        seed=int(seed), form={"plain": 0, "device": 1, "learned": 2}[form], device=int(gpu_device),
+                        n_threads=int(n_threads), mlp_params=N.ptr(par) if form == "learned" else None,
+                        mlp_fallback=float(fallback))
+    L = C.c_int32(0)
+    Kr = C.c_int32(0)
+    N.call("kodama_hip_matrix_sizes", C.c_int32(n), C.byref(o), C.byref(L), C.byref(Kr))
+    K = max(1, min(int(Kr.value), gi_in.shape[1]))
+    res = np.zeros((M, n), np.int32)
+    acc = np.zeros(M, np.float64)
+    v = np.zeros((M, max(int(Tcycle), 1)), np.float64)
+    land = np.zeros((M, L.value), np.int32)
+    gi = np.zeros((n, K), np.int32)
+    gd = np.zeros((n, K), np.float32)
+    timings = np.zeros(6, np.float64)
+    base = {"auto": -1, 0: 0, 1: 1}[index_base]
+    N.call("kodama_hip_kodama_matrix_from_graph", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), N.ptr(gi_in),
+           N.ptr(gd_in), C.c_int32(gi_in.shape[1]), C.c_int32(base), C.byref(o), N.ptr(res), N.ptr(acc), N.ptr(v),
+           N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))
+    return dict(res=res, acc=acc, v=v[:, : int(Tcycle)].copy(), landmarks=land, knn_indices=gi, knn_distances=gd,
+                timings=dict(zip(("atlas", "graph", "landmarks", "core_precompute", "core_evolution",
+                                  "projection"), timings.tolist())))
+
+
 # ---- graph and dissimilarity ----------------------------------------------------
 
 def KODAMA_knn_graph(data, k=30, metric="euclidean", gpu_device=0):

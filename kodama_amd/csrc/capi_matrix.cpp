@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <random>
 #include <string>
@@ -73,10 +74,55 @@ int kodama_hip_matrix_sizes(int32_t n, const kodama_hip_matrix_options* options,
   });
 }
 
-int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_hip_matrix_options* options,
-                             int32_t* res, double* acc, double* v, int32_t* landmark_rows,
-                             int32_t* graph_indices_one_based, float* graph_distances, double* timings) {
-  return guarded([&] {
+}  // extern "C"
+
+namespace {
+
+// A caller-supplied neighbour graph (KODAMAMatrixFromGraphData).
+struct ExternalGraph {
+  const int32_t* indices = nullptr;
+  const float* distances = nullptr;
+  int neighbors = 0;
+  int index_base = -1;  // 0, 1, or -1: one-based when every id is in 1..n
+};
+
+// normalize_external_graph (kodama_matrix.cpp:1292-1330): the first `width`
+// valid neighbours of each row (in range, not the row itself, finite
+// distance), 0-based, distances clamped at 0; -1 / +inf past them.
+void normalize_graph(const ExternalGraph& g, int n, int width, IVec& idx, std::vector<float>& dist) {
+  require(g.indices != nullptr && g.distances != nullptr, "null graph.");
+  require(g.neighbors > 0, "NeighborGraph.neighbors must be positive.");
+  int lo = std::numeric_limits<int>::max(), hi = std::numeric_limits<int>::min();
+  const size_t total = static_cast<size_t>(n) * g.neighbors;
+  for (size_t e = 0; e < total; ++e) {
+    const int id = g.indices[e];
+    if (id < 0) continue;
+    lo = std::min(lo, id);
+    hi = std::max(hi, id);
+  }
+  const bool one_based = g.index_base == 1 || (g.index_base < 0 && lo >= 1 && hi <= n);
+  idx.assign(static_cast<size_t>(n) * width, -1);
+  dist.assign(idx.size(), std::numeric_limits<float>::infinity());
+  for (int i = 0; i < n; ++i) {
+    int col = 0;
+    for (int j = 0; j < g.neighbors && col < width; ++j) {
+      const size_t src = static_cast<size_t>(i) * g.neighbors + j;
+      int id = g.indices[src];
+      if (one_based && id > 0) --id;
+      if (id < 0 || id >= n || id == i) continue;
+      const float d = g.distances[src];
+      if (!std::isfinite(d)) continue;
+      idx[static_cast<size_t>(i) * width + col] = id;
+      dist[static_cast<size_t>(i) * width + col] = std::max(0.0f, d);
+      ++col;
+    }
+  }
+}
+
+void matrix_impl(const float* x, int32_t n, int32_t p, const kodama_hip_matrix_options* options,
+                 const ExternalGraph* ext, int32_t* res, double* acc, double* v, int32_t* landmark_rows,
+                 int32_t* graph_indices_one_based, float* graph_distances, double* timings) {
+  {
     require(x != nullptr && options != nullptr && res != nullptr && acc != nullptr, "null pointer.");
     const kodama_hip_matrix_options& o = *options;
     require(n >= 3, "KODAMAMatrix requires at least 3 rows.");
@@ -109,20 +155,30 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     index_strata(atlas, z.coarse_k, strata_off, strata_rows);
     t[0] = seconds_since(clock);
 
-    // Global graph: exact self-kNN of the prepared rows, self excluded.
-    const std::vector<float> xs = prepare_rows(x, n, p, o.metric, 1);
-    const size_t edges = static_cast<size_t>(n) * z.neighbors;
-    DeviceBuffer<float> d_xs(xs.size()), d_gdist(edges);
-    DeviceBuffer<int> d_self(static_cast<size_t>(n)), d_gidx(edges);
-    {
+    // Global graph: exact self-kNN of the prepared rows, self excluded; or
+    // the caller's graph, normalised.
+    const int K = ext ? std::max(1, std::min(z.neighbors, ext->neighbors)) : z.neighbors;
+    const size_t edges = static_cast<size_t>(n) * K;
+    DeviceBuffer<float> d_gdist(edges);
+    DeviceBuffer<int> d_gidx(edges);
+    IVec ext_idx;
+    std::vector<float> ext_dist;
+    if (ext) {
+      normalize_graph(*ext, n, K, ext_idx, ext_dist);
+      d_gidx.upload(ext_idx.data(), edges, s);
+      d_gdist.upload(ext_dist.data(), edges, s);
+    } else {
+      const std::vector<float> xs = prepare_rows(x, n, p, o.metric, 1);
+      DeviceBuffer<float> d_xs(xs.size());
+      DeviceBuffer<int> d_self(static_cast<size_t>(n));
       IVec self(static_cast<size_t>(n));
       std::iota(self.begin(), self.end(), 0);
       d_xs.upload(xs.data(), xs.size(), s);
       d_self.upload(self.data(), self.size(), s);
       device_exact_knn(d_xs.get(), n, d_xs.get(), n, p, z.neighbors, o.metric, d_self.get(), d_gidx.get(),
                        d_gdist.get(), max_row_norm2(xs.data(), n, p), s, &last_knn_fallbacks());
-      check(hipStreamSynchronize(s), "graph sync");
     }
+    check(hipStreamSynchronize(s), "graph sync");
     t[1] = seconds_since(clock);
 
     // Per run r = 1..M: mt19937_64(seed + r) skips one draw (the atlas is
@@ -203,7 +259,7 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     ProjectParams pp;
     pp.graph = d_gidx.get();
     pp.n = n;
-    pp.neighbors = z.neighbors;
+    pp.neighbors = K;
     pp.landmarks = d_land.get();
     pp.run_offsets = d_off.get();
     pp.clbest = d_cl.get();
@@ -213,9 +269,17 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     check(launch_project(pp, s), "project launch");
     d_res.download(res, d_res.size(), s);
     if (graph_indices_one_based != nullptr && graph_distances != nullptr) {
-      check(launch_graph_publish(d_gidx.get(), d_gdist.get(), edges, o.metric, s), "graph_publish launch");
-      d_gidx.download(graph_indices_one_based, edges, s);
-      d_gdist.download(graph_distances, edges, s);
+      if (ext) {
+        // The caller's distances are public already; ids become 1-based.
+        for (size_t e = 0; e < edges; ++e) {
+          graph_indices_one_based[e] = ext_idx[e] >= 0 ? ext_idx[e] + 1 : ext_idx[e];
+          graph_distances[e] = ext_dist[e];
+        }
+      } else {
+        check(launch_graph_publish(d_gidx.get(), d_gdist.get(), edges, o.metric, s), "graph_publish launch");
+        d_gidx.download(graph_indices_one_based, edges, s);
+        d_gdist.download(graph_distances, edges, s);
+      }
     }
     check(hipStreamSynchronize(s), "projection sync");
     t[5] = seconds_since(clock);
@@ -227,6 +291,39 @@ int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_
     }
     if (landmark_rows != nullptr) std::copy(land_all.begin(), land_all.end(), landmark_rows);
     if (timings != nullptr) std::copy_n(t, 6, timings);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p, const kodama_hip_matrix_options* options,
+                             int32_t* res, double* acc, double* v, int32_t* landmark_rows,
+                             int32_t* graph_indices_one_based, float* graph_distances, double* timings) {
+  return guarded([&] {
+    matrix_impl(x, n, p, options, nullptr, res, acc, v, landmark_rows, graph_indices_one_based, graph_distances,
+                timings);
+  });
+}
+
+// KODAMAMatrixFromGraphData_CUDA (kodama_matrix.cpp:4676-4690).
+int kodama_hip_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const int32_t* graph_indices,
+                                        const float* graph_distances_in, int32_t graph_neighbors,
+                                        int32_t index_base, const kodama_hip_matrix_options* options, int32_t* res,
+                                        double* acc, double* v, int32_t* landmark_rows,
+                                        int32_t* graph_indices_one_based, float* graph_distances,
+                                        double* timings) {
+  return guarded([&] {
+    require(n >= 2, "KODAMAMatrixFromGraphData requires at least two rows.");
+    require(index_base >= -1 && index_base <= 1, "index_base must be 0, 1 or -1 (auto).");
+    ExternalGraph g;
+    g.indices = graph_indices;
+    g.distances = graph_distances_in;
+    g.neighbors = graph_neighbors;
+    g.index_base = index_base;
+    matrix_impl(x, n, p, options, &g, res, acc, v, landmark_rows, graph_indices_one_based, graph_distances,
+                timings);
   });
 }
 

@@ -81,6 +81,31 @@ def test_learned_form_in_the_matrix():
     assert np.all(np.diff(lrn["v"], axis=1) >= 0)
 
 
+def test_matrix_from_graph_data():
+    """KODAMAMatrixFromGraphData: fed the exact graph that KODAMA_matrix
+    publishes (1-based, and the same graph 0-based), the results are those of
+    KODAMA_matrix; a narrower graph is taken as given (width min(neighbours,
+    resolved width)) and rows past their valid neighbours are skipped."""
+    c = dict(CASES["p10_euclidean"], runs=3, cycles=10)
+    x = _data(c.pop("n"), c.pop("p"), 5)
+    c.pop("metric")
+    kw = dict(M=c["runs"], Tcycle=c["cycles"], landmarks=c["landmarks"], splitting=c["splitting"], folds=c["folds"],
+              k=c["k"], graph_neighbors=c["graph_neighbors"], seed=c["seed"])
+    full = K.KODAMA_matrix(x, **kw)
+    gi, gd = full["knn_indices"], full["knn_distances"]
+    one = K.KODAMA_matrix_from_graph(x, gi, gd, **kw)
+    zero = K.KODAMA_matrix_from_graph(x, gi - 1, gd, index_base=0, **kw)
+    for out in (one, zero):
+        assert np.array_equal(out["res"], full["res"]) and np.array_equal(out["v"], full["v"])
+        assert np.array_equal(out["knn_indices"], gi) and np.array_equal(out["knn_distances"], gd)
+    narrow = gi[:, :7].copy()
+    narrow[::5, 3] = 0  # a missing entry (0 in a one-based graph is out of range)
+    nar = K.KODAMA_matrix_from_graph(x, narrow, gd[:, :7], index_base=1, **kw)
+    assert nar["knn_indices"].shape == (x.shape[0], 7)
+    assert np.all(nar["res"] >= 1)
+    assert np.array_equal(nar["landmarks"], full["landmarks"])
+
+
 def test_matrix_graph_and_dissimilarity(oracle):
     c = dict(CASES["p10_euclidean"])
     x = _data(c["n"], c["p"], 5)
