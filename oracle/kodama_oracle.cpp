@@ -1770,6 +1770,45 @@ int oracle_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata,
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// normalize_external_graph (kodama_matrix.cpp:1292-1330): a caller's graph in
+// 0-based ids, `width` wide. One-based when index_base is 1, or when it is -1
+// (auto) and every non-negative id lies in 1..n; per row the first `width`
+// entries that name another row in range with a finite distance, that
+// distance clamped at 0; -1 / +inf past them.
+void normalize_graph(const int32_t* gi, const float* gd, int n, int given, int index_base, int width,
+                     std::vector<int32_t>& idx, std::vector<float>& dist) {
+  if (n < 2) throw std::invalid_argument("NeighborGraph samples must be at least 2.");
+  if (given <= 0 || width <= 0) throw std::invalid_argument("NeighborGraph.neighbors must be positive.");
+  bool any = false, in_one_based_range = true;
+  for (size_t e = 0; e < static_cast<size_t>(n) * given; ++e) {
+    if (gi[e] < 0) continue;
+    any = true;
+    if (gi[e] < 1 || gi[e] > n) in_one_based_range = false;
+  }
+  // No non-negative id at all: the reference's min/max stay at their sentinels
+  // (INT_MAX >= 1 but INT_MIN <= n), so auto detection reads it as one-based.
+  const bool one_based = index_base == 1 || (index_base < 0 && (any ? in_one_based_range : true));
+  idx.assign(static_cast<size_t>(n) * width, -1);
+  dist.assign(idx.size(), std::numeric_limits<float>::infinity());
+  for (int i = 0; i < n; ++i) {
+    int filled = 0;
+    for (int j = 0; j < given && filled < width; ++j) {
+      int id = gi[static_cast<size_t>(i) * given + j];
+      const float d = gd[static_cast<size_t>(i) * given + j];
+      if (one_based && id > 0) id -= 1;
+      const bool keep = id >= 0 && id < n && id != i && std::isfinite(d);
+      if (!keep) continue;
+      idx[static_cast<size_t>(i) * width + filled] = id;
+      dist[static_cast<size_t>(i) * width + filled] = d < 0.0f ? 0.0f : d;
+      ++filled;
+    }
+  }
+}
+
 // KODAMA.matrix, kNN classifier, no spatial data, no constraints or starting
 // labels (kodama_matrix.cpp:3324-4051 with run_iteration 2870-3322): the
 // shared k-means atlas (mt19937_64(seed + 1)), per run r = 1..M an
@@ -1778,8 +1817,9 @@ int oracle_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata,
 // exact neighbours; each other row takes the majority label of the landmarks
 // among its first graph neighbours (exact global graph, self excluded), at
 // most k votes, clbest[0] when none.
-int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, int32_t* res,
-                         double* acc, double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
+int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, const int32_t* ext_idx,
+                const float* ext_dist, int32_t ext_width, int32_t ext_base, int32_t* res, double* acc, double* v,
+                int32_t* landmark_rows, int32_t* landmarks_used) {
   return guarded([&] {
     if (n < 3) throw std::invalid_argument("KODAMAMatrix requires at least 3 rows.");
     int L = o->landmarks > 0 ? o->landmarks : 10000;
@@ -1788,8 +1828,9 @@ int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matr
     const int split = o->splitting > 0 ? o->splitting : (n < 40000 ? 100 : 300);
     const int coarse_k = std::max(2, std::min(split, n));
     const int requested = o->graph_neighbors > 0 ? o->graph_neighbors : 100;
-    const int neighbors = std::max(1, static_cast<int>(std::floor(std::min(
+    const int resolved = std::max(1, static_cast<int>(std::floor(std::min(
       {static_cast<double>(L), static_cast<double>(n) * 0.75 - 1.0, static_cast<double>(requested)}))));
+    const int neighbors = ext_idx != nullptr ? std::max(1, std::min(ext_width, resolved)) : resolved;
     std::mt19937_64 shared(o->seed + 1u);
     const IVec atlas = kmeans_labels(x, n, p, coarse_k, shared, 10);
     IVec off, rows;
@@ -1797,9 +1838,13 @@ int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matr
     const std::vector<float> xs = prepare_rows(x, n, p, o->metric, 1);
     std::vector<int32_t> gi(static_cast<size_t>(n) * neighbors);
     std::vector<float> gd(gi.size());
-    IVec self(static_cast<size_t>(n));
-    std::iota(self.begin(), self.end(), 0);
-    exact_knn(xs.data(), n, xs.data(), n, p, neighbors, o->metric, self.data(), gi.data(), gd.data());
+    if (ext_idx != nullptr) {
+      normalize_graph(ext_idx, ext_dist, n, ext_width, ext_base, neighbors, gi, gd);
+    } else {
+      IVec self(static_cast<size_t>(n));
+      std::iota(self.begin(), self.end(), 0);
+      exact_knn(xs.data(), n, xs.data(), n, p, neighbors, o->metric, self.data(), gi.data(), gd.data());
+    }
     *landmarks_used = L;
     for (int r = 1; r <= o->runs; ++r) {
       std::mt19937_64 rng(o->seed + static_cast<std::uint64_t>(r));
@@ -1870,6 +1915,41 @@ int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matr
       for (int c = 0; c < T; ++c) v[static_cast<size_t>(r - 1) * T + c] = vacc[c];
       if (landmark_rows) std::memcpy(landmark_rows + static_cast<size_t>(r - 1) * L, land.data(), sizeof(int32_t) * L);
     }
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, int32_t* res,
+                         double* acc, double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
+  return matrix_impl(x, n, p, o, nullptr, nullptr, 0, -1, res, acc, v, landmark_rows, landmarks_used);
+}
+
+// KODAMAMatrixFromGraphData (kodama_matrix.cpp:4676-4690, 3552-3556): the
+// caller's graph, normalised to min(its width, the resolved graph width),
+// replaces the exact global graph.
+int oracle_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o,
+                                    const int32_t* graph_indices, const float* graph_distances,
+                                    int32_t graph_neighbors, int32_t index_base, int32_t* res, double* acc,
+                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
+  if (graph_indices == nullptr || graph_distances == nullptr || graph_neighbors <= 0) {
+    return guarded([] { throw std::invalid_argument("null or empty graph."); });
+  }
+  return matrix_impl(x, n, p, o, graph_indices, graph_distances, graph_neighbors, index_base, res, acc, v,
+                     landmark_rows, landmarks_used);
+}
+
+int oracle_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,
+                           int32_t graph_neighbors, int32_t index_base, int32_t width, int32_t* indices_out,
+                           float* distances_out) {
+  return guarded([&] {
+    std::vector<int32_t> gi;
+    std::vector<float> gd;
+    normalize_graph(graph_indices, graph_distances, n, graph_neighbors, index_base, width, gi, gd);
+    std::copy(gi.begin(), gi.end(), indices_out);
+    std::copy(gd.begin(), gd.end(), distances_out);
   });
 }
 

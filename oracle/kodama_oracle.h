@@ -153,6 +153,21 @@ int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matr
                          int32_t* res, double* acc, double* v, int32_t* landmark_rows,
                          int32_t* landmarks_used);
 
+/* The same with the caller's graph [n x graph_neighbors] (index_base 0, 1 or
+ * -1 = auto) in place of the exact global graph. */
+int oracle_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const oracle_matrix_options* options,
+                                    const int32_t* graph_indices, const float* graph_distances,
+                                    int32_t graph_neighbors, int32_t index_base, int32_t* res, double* acc,
+                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used);
+
+/* normalize_external_graph: outputs [n x width], 0-based, -1 / +inf padded. */
+int oracle_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,
+                           int32_t graph_neighbors, int32_t index_base, int32_t width, int32_t* indices_out,
+                           float* distances_out);
+int ref_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,
+                        int32_t graph_neighbors, int32_t index_base, int32_t width, int32_t* indices_out,
+                        float* distances_out);
+
 const char* oracle_last_error(void);
 
 /* ---- reference driver (oracle/_ref/libkodama_ref.so) ---------------------- */

@@ -219,6 +219,17 @@ class _MatrixBlocks:
                      C.c_int32(target), C.c_uint64(seed), C.c_int32(discard), _p(out, C.c_int32), C.byref(cnt))
         return out[: cnt.value].copy()
 
+    def normalize_graph(self, indices, distances, index_base=-1, width=None):
+        gi = _i32(indices)
+        gd = _f32(distances)
+        n, given = gi.shape
+        width = given if width is None else width
+        idx = np.zeros((n, width), np.int32)
+        dist = np.zeros((n, width), np.float32)
+        self._l.call("normalize_graph", _p(gi, C.c_int32), _p(gd, C.c_float), C.c_int32(n), C.c_int32(given),
+                     C.c_int32(index_base), C.c_int32(width), _p(idx, C.c_int32), _p(dist, C.c_float))
+        return idx, dist
+
 
 class Oracle(_Api, _MatrixBlocks):
     def __init__(self):
@@ -345,7 +356,8 @@ class Oracle(_Api, _MatrixBlocks):
 
 
 def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5, k=30, metric=2,
-                          graph_neighbors=100, seed=1234):
+                          graph_neighbors=100, seed=1234, graph=None, index_base=-1):
+    """KODAMA.matrix; with graph = (indices, distances), KODAMAMatrixFromGraphData."""
     x = _f32(x)
     n = x.shape[0]
     o = MatrixOptions(runs=runs, cycles=cycles, landmarks=landmarks, splitting=splitting, folds=folds, k=k,
@@ -357,8 +369,17 @@ def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5
     v = np.zeros((runs, max(cycles, 1)), np.float64)
     land = np.zeros((runs, L), np.int32)
     used = C.c_int32(0)
-    self._l.call("kodama_matrix", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o),
-                 _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(land, C.c_int32), C.byref(used))
+    if graph is None:
+        self._l.call("kodama_matrix", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o),
+                     _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(land, C.c_int32),
+                     C.byref(used))
+    else:
+        gi, gd = _i32(graph[0]), _f32(graph[1])
+        assert gi.shape == gd.shape and gi.shape[0] == n
+        self._l.call("kodama_matrix_from_graph", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]),
+                     C.byref(o), _p(gi, C.c_int32), _p(gd, C.c_float), C.c_int32(gi.shape[1]),
+                     C.c_int32(index_base), _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double),
+                     _p(land, C.c_int32), C.byref(used))
     assert used.value == L
     return dict(res=res, acc=acc, v=v[:, :cycles].copy(), landmarks=land)
 

@@ -65,4 +65,23 @@ int ref_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata, in
   });
 }
 
+// normalize_external_graph (kodama_matrix.cpp:1292-1330); index_base -1 is Auto.
+int ref_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,
+                        int32_t graph_neighbors, int32_t index_base, int32_t width, int32_t* indices_out,
+                        float* distances_out) {
+  return guarded_call([&] {
+    kodama::NeighborGraph g;
+    const size_t total = static_cast<size_t>(n) * graph_neighbors;
+    g.indices.assign(graph_indices, graph_indices + total);
+    g.distances.assign(graph_distances, graph_distances + total);
+    g.neighbors = graph_neighbors;
+    g.index_base = index_base == 0 ? kodama::GraphIndexBase::Zero
+                   : index_base == 1 ? kodama::GraphIndexBase::One : kodama::GraphIndexBase::Auto;
+    const kodama::NeighborGraph out = kodama::normalize_external_graph(g, n, width);
+    if (out.neighbors != width) throw std::runtime_error("unexpected normalised width");
+    std::memcpy(indices_out, out.indices.data(), sizeof(int32_t) * out.indices.size());
+    std::memcpy(distances_out, out.distances.data(), sizeof(float) * out.distances.size());
+  });
+}
+
 }  // extern "C"

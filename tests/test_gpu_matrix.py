@@ -81,11 +81,11 @@ def test_learned_form_in_the_matrix():
     assert np.all(np.diff(lrn["v"], axis=1) >= 0)
 
 
-def test_matrix_from_graph_data():
+def test_matrix_from_graph_data(oracle):
     """KODAMAMatrixFromGraphData: fed the exact graph that KODAMA_matrix
     publishes (1-based, and the same graph 0-based), the results are those of
-    KODAMA_matrix; a narrower graph is taken as given (width min(neighbours,
-    resolved width)) and rows past their valid neighbours are skipped."""
+    KODAMA_matrix; a narrower graph with gaps and non-finite distances gives
+    the oracle's KODAMAMatrixFromGraphData results and normalised graph."""
     c = dict(CASES["p10_euclidean"], runs=3, cycles=10)
     x = _data(c.pop("n"), c.pop("p"), 5)
     c.pop("metric")
@@ -99,11 +99,21 @@ def test_matrix_from_graph_data():
         assert np.array_equal(out["res"], full["res"]) and np.array_equal(out["v"], full["v"])
         assert np.array_equal(out["knn_indices"], gi) and np.array_equal(out["knn_distances"], gd)
     narrow = gi[:, :7].copy()
-    narrow[::5, 3] = 0  # a missing entry (0 in a one-based graph is out of range)
-    nar = K.KODAMA_matrix_from_graph(x, narrow, gd[:, :7], index_base=1, **kw)
-    assert nar["knn_indices"].shape == (x.shape[0], 7)
-    assert np.all(nar["res"] >= 1)
+    narrow[::5, 3] = -1  # missing entries: the row keeps its other neighbours in order
+    ndist = gd[:, :7].copy()
+    ndist[1::5, 2] = np.nan  # non-finite distances are dropped too
+    nar = K.KODAMA_matrix_from_graph(x, narrow, ndist, index_base=1, **kw)
+    gi_n, gd_n = nar["knn_indices"], nar["knn_distances"]
+    assert gi_n.shape == (x.shape[0], 7)
+    for rows, col in ((slice(0, None, 5), 3), (slice(1, None, 5), 2)):
+        assert np.array_equal(gi_n[rows, col:6], narrow[rows, col + 1:7])
+        assert np.all(gi_n[rows, 6] == -1) and np.all(np.isinf(gd_n[rows, 6]))
     assert np.array_equal(nar["landmarks"], full["landmarks"])
+    want = oracle.kodama_matrix(x, metric=K._native.METRICS["euclidean"], graph=(narrow, ndist), index_base=1, **c)
+    for key in ("res", "acc", "v", "landmarks"):
+        assert np.array_equal(nar[key], want[key]), key
+    oi, od = oracle.normalize_graph(narrow, ndist, 1, 7)
+    assert np.array_equal(gi_n, np.where(oi >= 0, oi + 1, -1)) and np.array_equal(gd_n, od)
 
 
 def test_matrix_graph_and_dissimilarity(oracle):

@@ -397,6 +397,53 @@ def test_fresh_kmeans_and_quota_match_reference(oracle):
         assert np.array_equal(oracle.quota_sample(lab, 50, target, 3, 1), ref.quota_sample(lab, 50, target, 3, 1))
 
 
+def test_normalize_graph_closed_case(oracle):
+    """normalize_external_graph on a hand-worked 4-row graph: self entries, an
+    out-of-range id, a NaN and a negative distance (clamped at 0), rows left
+    shorter than the width, and how auto detection picks the base."""
+    gi = np.array([[2, 1, 4, 3], [1, 5, 3, 2], [4, 3, -1, 1], [3, 1, 2, 4]], np.int32)
+    gd = np.array([[0.5, 0.0, np.nan, 2.0], [0.25, 9.0, -1.0, 7.0], [1.0, 0.0, 3.0, 4.0],
+                   [0.1, 0.2, 0.3, 0.4]], np.float32)
+    inf = np.inf
+    idx, dist = oracle.normalize_graph(gi, gd, 1, 3)  # one-based: 5 is out of range
+    assert np.array_equal(idx, [[1, 2, -1], [0, 2, -1], [3, 0, -1], [2, 0, 1]])
+    assert np.array_equal(dist, np.array([[0.5, 2.0, inf], [0.25, 0.0, inf], [1.0, 4.0, inf],
+                                          [0.1, 0.2, 0.3]], np.float32))
+    # Auto: the 5 lies outside 1..4, so the graph reads as zero-based.
+    a_idx, a_dist = oracle.normalize_graph(gi, gd, -1, 3)
+    assert np.array_equal(a_idx[0], [2, 1, 3]) and np.array_equal(a_dist[0], np.float32([0.5, 0.0, 2.0]))
+    # Auto with every id in 1..4: one-based.
+    gi4 = gi.copy()
+    gi4[1, 1] = 4
+    b_idx, b_dist = oracle.normalize_graph(gi4, gd, -1, 3)
+    assert np.array_equal(b_idx[1], [0, 3, 2]) and np.array_equal(b_dist[1], np.float32([0.25, 9.0, 0.0]))
+    # Forced zero-based on the shifted graph: 4 is out of range, -1 is a gap.
+    z_idx, _ = oracle.normalize_graph(gi - 1, gd, 0, 4)
+    assert np.array_equal(z_idx[:2], [[1, 2, -1, -1], [0, 2, -1, -1]])
+
+
+def test_normalize_graph_matches_reference(oracle):
+    if not O.ReferenceMatrix.available():
+        pytest.skip("oracle/_ref (the compiled reference core) is not built here")
+    ref = O.ReferenceMatrix()
+    rng = np.random.default_rng(21)
+    n, w = 300, 12
+    for case in range(8):
+        base = (-1, 0, 1)[case % 3]
+        lo = 1 if case in (0, 3, 6) else 0  # cases 0, 3, 6: a one-based graph
+        gi = rng.integers(lo - 2, n + lo + 2, size=(n, w)).astype(np.int32)
+        gi[np.arange(n) % 7 == 0, 1] = np.arange(n)[np.arange(n) % 7 == 0] + lo  # self entries
+        gd = rng.normal(1.0, 1.0, size=(n, w)).astype(np.float32)
+        gd[rng.random((n, w)) < 0.05] = np.nan
+        gd[rng.random((n, w)) < 0.03] = np.inf
+        if case >= 6:
+            gi = np.clip(gi, lo, n - 1 + lo)  # every id in range: auto detection decides
+        for width in (1, 7, w):
+            a = oracle.normalize_graph(gi, gd, base, width)
+            b = ref.normalize_graph(gi, gd, base, width)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (case, width)
+
+
 # ---- learned proposal (no reference counterpart): the float64 restatement ----------
 
 def test_mlp_features_hand_built():
