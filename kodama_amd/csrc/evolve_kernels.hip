@@ -48,6 +48,7 @@ constexpr int kLearnedWaves = kLearnedThreads / kWave;
 // Label code of the sentinel slot lab[n] of each label buffer: a missing
 // neighbour points there (local id n) and matches no label (codes <= 1023).
 constexpr uint32_t kSentinel = 0xFFFFu;
+static_assert(kSentinel == mlp::kNone, "a missing neighbour has one code in the vote and the features");
 constexpr unsigned short kChosenMark = 0xFFFFu;  // above every label code
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -256,9 +257,25 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
                                          const float* __restrict__ scores, const unsigned short* lab, int i) {
   uint32_t key[KMAX];
   int valid = 0;
+  // All of the row's id loads are in flight before the first lookup, and the
+  // lookups go out eight at a time; ranks past k re-read rank k - 1 and are
+  // masked afterwards, so neither needs a branch.
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
-    const uint32_t c = j < k ? lab[nbr[static_cast<uint32_t>(j * pad_total + i)]] : kSentinel;
+    const int jj = j < k ? j : k - 1;
+    key[j] = nbr[static_cast<uint32_t>(jj * pad_total + i)];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  constexpr int kLook = KMAX < 8 ? KMAX : 8;
+#pragma unroll
+  for (int j0 = 0; j0 < KMAX; j0 += kLook) {
+#pragma unroll
+    for (int j = j0; j < j0 + kLook; ++j) key[j] = lab[key[j]];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    const uint32_t c = j < k ? key[j] : kSentinel;
     valid += c != kSentinel ? 1 : 0;
     key[j] = (c << 6) | static_cast<uint32_t>(j);
   }
@@ -310,11 +327,13 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
 
 // Neighbour ids of rows 2m and 2m+1 (low and high halves), one 32-bit load per
 // neighbour rank; run bases and the table stride are multiples of 256 rows.
+// Ranks past k get the sentinel slot in both halves, so the lookups that
+// follow need no branch.
 template <int KMAX>
 __device__ __forceinline__ void load_pair(const uint32_t* __restrict__ nbr2, int pad2, int k, int m,
-                                          uint32_t (&ids)[KMAX]) {
+                                          uint32_t sentinel2, uint32_t (&ids)[KMAX]) {
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) ids[j] = j < k ? nbr2[static_cast<size_t>(j) * pad2 + m] : 0u;
+  for (int j = 0; j < KMAX; ++j) ids[j] = j < k ? nbr2[static_cast<size_t>(j) * pad2 + m] : sentinel2;
 }
 
 // Vote of the labels in `lab` into `pred` and the number of rows whose
@@ -333,6 +352,7 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   const int npairs = (n + 1) >> 1;
   const int lane = tid & (kWave - 1);
   const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t sentinel2 = static_cast<uint32_t>(n) | static_cast<uint32_t>(n) << 16;
   int matches = 0;
   // Uniform trip count: every lane reaches the ballots below. Latency is
   // hidden by the workgroup's 16 waves; with 8 waves (and the registers that
@@ -343,7 +363,7 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
       const int m = m0 + b * THREADS + tid;
-      if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, ids_b[b]);
+      if (m < npairs) load_pair<KMAX>(nbr2, pad2, k, m, sentinel2, ids_b[b]);
     }
     // The thread's rows first (no LDS atomic between them, so their lookups
     // overlap), then one queue allocation for all of the wave's undecided rows.
@@ -358,11 +378,20 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
       if (i < n) {
         const int own = lab[i];
         if (count_labels) atomicAdd(&s.counts[own], 1);
-        const int code0 = lab[ids[0] >> (16 * h) & 0xFFFFu];
-        int c0 = 1;
+        // Ranks past k read the sentinel slot, which matches code0 only when
+        // code0 is the sentinel itself, and such a row is queued anyway.
+        // Lookups go out eight at a time before their compares.
+        constexpr int kGroup = KMAX < 8 ? KMAX : 8;
+        int code0 = 0, c0 = 0;
 #pragma unroll
-        for (int j = 1; j < KMAX; ++j) {
-          if (j < k) c0 += static_cast<int>(lab[ids[j] >> (16 * h) & 0xFFFFu]) == code0 ? 1 : 0;
+        for (int j0 = 0; j0 < KMAX; j0 += kGroup) {
+          int code[kGroup];
+#pragma unroll
+          for (int t = 0; t < kGroup; ++t) code[t] = lab[ids[j0 + t] >> (16 * h) & 0xFFFFu];
+          if (j0 == 0) code0 = code[0];
+#pragma unroll
+          for (int t = 0; t < kGroup; ++t) c0 += code[t] == code0 ? 1 : 0;
+          __builtin_amdgcn_sched_barrier(0);
         }
         if (code0 != kSentinel && 2 * c0 > k) {
           pred[i] = static_cast<unsigned short>(code0);
@@ -549,13 +578,14 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             for (int j0 = 0; j0 < KMAX; j0 += 8) {
 #pragma unroll
               for (int j = j0; j < j0 + 8 && j < KMAX; ++j) {
-                vkey[j] = have && j < k ? static_cast<uint32_t>(nbr[static_cast<uint32_t>(j * pad_total + row)])
-                                        : static_cast<uint32_t>(n);
+                const int jj = j < k ? j : k - 1;
+                const uint32_t id = nbr[static_cast<uint32_t>(jj * pad_total + row)];
+                vkey[j] = have && j < k ? id : static_cast<uint32_t>(n);
               }
+              // Slot n (a missing neighbour, a rank past k, a lane past the
+              // list) holds the sentinel code, which is mlp::kNone.
 #pragma unroll
-              for (int j = j0; j < j0 + 8 && j < KMAX; ++j) {
-                vkey[j] = vkey[j] < static_cast<uint32_t>(n) ? cur_lab[vkey[j]] : mlp::kNone;
-              }
+              for (int j = j0; j < j0 + 8 && j < KMAX; ++j) vkey[j] = cur_lab[vkey[j]];
               __builtin_amdgcn_sched_barrier(0);
             }
             const mlp::RowVote vote = mlp::row_vote<KMAX>(vkey, cur_lab[row]);
