@@ -53,7 +53,7 @@ def test_device_form_shares_landmarks_and_accuracy(oracle):
     dev = K.KODAMA_matrix(x, form="device", **kw)
     assert np.array_equal(plain["landmarks"], dev["landmarks"])
     assert np.array_equal(plain["knn_indices"], dev["knn_indices"])
-    # same starting labels and judge, different proposal draws
+    # same starting labels and CV evaluation, different proposal draws
     assert abs(dev["acc"].mean() - plain["acc"].mean()) < 0.05
     n = x.shape[0]
     for r in range(c["runs"]):
