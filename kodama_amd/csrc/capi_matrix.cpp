@@ -265,6 +265,9 @@ void matrix_impl(const float* x, int32_t n, int32_t p, const kodama_hip_matrix_o
     pp.clbest = d_cl.get();
     pp.runs = o.runs;
     pp.votes = std::max(1, o.k);
+    pp.max_landmarks = L;
+    DeviceBuffer<int> d_local(static_cast<size_t>(o.runs) * n);
+    pp.local = d_local.get();
     pp.res = d_res.get();
     check(launch_project(pp, s), "project launch");
     d_res.download(res, d_res.size(), s);

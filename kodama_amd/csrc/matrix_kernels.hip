@@ -16,11 +16,13 @@
 // members are found 1024 rows at a time with ballots and added in ascending
 // row order, the order of the reference's sequential sum.
 //
-// project_kernel: one thread per (row, run); landmark lookups by binary search
-// in the run's sorted landmark rows; the votes of up to 64 landmark neighbours
-// sit in the thread's LDS slice.
+// project_kernel: one thread per (row, run); landmark lookups through a
+// per-run map from data row to landmark position (filled by
+// local_fill_kernel and local_scatter_kernel); the votes of up to 64 landmark
+// neighbours sit in the thread's LDS slice.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -183,24 +185,32 @@ __global__ void __launch_bounds__(kBlock) kmeans_update_kernel(const float* __re
   if (threadIdx.x == 0) sizes[c] = count;
 }
 
+__global__ void __launch_bounds__(kBlock) local_fill_kernel(int* __restrict__ local, size_t total) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; e < total;
+       e += static_cast<size_t>(gridDim.x) * kBlock) {
+    local[e] = -1;
+  }
+}
+
+// local[r][landmark row] = its position in run r's landmark list (rows are
+// distinct within a run, so no two threads write one entry).
+__global__ void __launch_bounds__(kBlock) local_scatter_kernel(ProjectParams prm) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  const int r = blockIdx.y;
+  const int begin = prm.run_offsets[r];
+  if (j >= prm.run_offsets[r + 1] - begin) return;
+  prm.local[static_cast<size_t>(r) * prm.n + prm.landmarks[begin + j]] = j;
+}
+
 __global__ void __launch_bounds__(kProjectBlock) project_kernel(ProjectParams prm) {
   __shared__ int votes_lds[kProjectBlock * kMaxVotes];
   const int i = blockIdx.x * kProjectBlock + threadIdx.x;
   const int r = blockIdx.y;
   if (i >= prm.n) return;  // no block barrier below
   const int begin = prm.run_offsets[r];
-  const int L = prm.run_offsets[r + 1] - begin;
-  const int* land = prm.landmarks + begin;
   const int* lab = prm.clbest + begin;
-  auto find = [&](int g) -> int {
-    int lo = 0, hi = L;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (land[mid] < g) lo = mid + 1;
-      else hi = mid;
-    }
-    return lo < L && land[lo] == g ? lo : -1;
-  };
+  const int* local = prm.local + static_cast<size_t>(r) * prm.n;
+  auto find = [&](int g) -> int { return local[g]; };
   int out;
   const int self = find(i);
   if (self >= 0) {
@@ -271,8 +281,17 @@ hipError_t launch_kmeans_update(const float* x, int n, int p, const int* assign,
 }
 
 hipError_t launch_project(const ProjectParams& prm, hipStream_t s) {
-  if (prm.votes < 1 || prm.votes > kMaxVotes || prm.neighbors < 0 || prm.runs < 1) return hipErrorInvalidValue;
+  if (prm.votes < 1 || prm.votes > kMaxVotes || prm.neighbors < 0 || prm.runs < 1 || prm.local == nullptr ||
+      prm.max_landmarks < 1) {
+    return hipErrorInvalidValue;
+  }
   if (prm.n == 0) return hipSuccess;
+  const size_t total = static_cast<size_t>(prm.runs) * prm.n;
+  const size_t fill_blocks = std::min<size_t>((total + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(local_fill_kernel, dim3(static_cast<unsigned>(fill_blocks)), dim3(kBlock), 0, s, prm.local,
+                     total);
+  hipLaunchKernelGGL(local_scatter_kernel, dim3((prm.max_landmarks + kBlock - 1) / kBlock, prm.runs), dim3(kBlock),
+                     0, s, prm);
   hipLaunchKernelGGL(project_kernel, dim3((prm.n + kProjectBlock - 1) / kProjectBlock, prm.runs),
                      dim3(kProjectBlock), 0, s, prm);
   return hipGetLastError();

@@ -37,12 +37,15 @@ struct ProjectParams {
   const int* clbest = nullptr;       // per landmark, same indexing as landmarks
   int runs = 0;
   int votes = 0;                     // most landmark neighbours that vote (1..64)
+  int max_landmarks = 0;             // largest run's landmark count
+  int* local = nullptr;              // workspace [runs x n]: landmark position of each row, or -1
   int* res = nullptr;                // [runs x n]
 };
 
 // res[r][i]: the landmark's own label, else the majority label (smallest on
 // ties) of the first `votes` landmarks in row i's graph neighbours, else the
-// run's first landmark label.
+// run's first landmark label. Fills prm.local first (row -> position in the
+// run's landmark list), so each lookup is one load.
 hipError_t launch_project(const ProjectParams& prm, hipStream_t s);
 
 }  // namespace kodama_hip
