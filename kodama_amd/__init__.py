@@ -452,7 +452,7 @@ def KODAMA_matrix_from_graph(data, indices, distances, index_base="auto", M=100,
     gi = np.zeros((n, K), np.int32)
     gd = np.zeros((n, K), np.float32)
     timings = np.zeros(6, np.float64)
-    base = {"auto": -1, 0: 0, 1: 1}[index_base]
+    base = -1 if index_base == "auto" else int(index_base)
     N.call("kodama_hip_kodama_matrix_from_graph", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), N.ptr(gi_in),
            N.ptr(gd_in), C.c_int32(gi_in.shape[1]), C.c_int32(base), C.byref(o), N.ptr(res), N.ptr(acc), N.ptr(v),
            N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))

@@ -320,6 +320,8 @@ int kodama_hip_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, co
   return guarded([&] {
     require(n >= 2, "KODAMAMatrixFromGraphData requires at least two rows.");
     require(index_base >= -1 && index_base <= 1, "index_base must be 0, 1 or -1 (auto).");
+    require(graph_indices != nullptr && graph_distances_in != nullptr, "null graph.");
+    require(graph_neighbors > 0, "NeighborGraph.neighbors must be positive.");
     ExternalGraph g;
     g.indices = graph_indices;
     g.distances = graph_distances_in;

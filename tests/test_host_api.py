@@ -51,6 +51,26 @@ def test_matrix_argument_errors_are_einval(kw, n, p):
         K.KODAMA_matrix(x, **{dict(runs="M", cycles="Tcycle").get(a, a): v for a, v in kw.items()})
 
 
+@pytest.mark.parametrize("case", ["base", "empty", "rows", "shape"])
+def test_matrix_from_graph_argument_errors(case):
+    """KODAMAMatrixFromGraphData rejects a bad graph before any device work:
+    an index base other than 0, 1 or auto, a graph without columns, too few
+    rows (EINVAL from the C ABI), and mismatched indices / distances."""
+    if not N.lib_available():
+        pytest.skip("libkodama_hip.so is not built")
+    n = 2 if case == "rows" else 50
+    x = np.zeros((n, 4), np.float32)
+    width = 0 if case == "empty" else 5
+    gi = np.ones((n, width), np.int32)
+    gd = np.ones((n, width + (1 if case == "shape" else 0)), np.float32)
+    if case == "shape":
+        with pytest.raises(ValueError):
+            K.KODAMA_matrix_from_graph(x, gi, gd)
+        return
+    with pytest.raises(K.InvalidArgument):
+        K.KODAMA_matrix_from_graph(x, gi, gd, index_base=2 if case == "base" else "auto")
+
+
 def test_mlp_init_is_he_normal_and_deterministic():
     if not N.lib_available():
         pytest.skip("libkodama_hip.so is not built")
