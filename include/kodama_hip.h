@@ -233,6 +233,14 @@ int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t r
  * fold's validation rows in ascending order, and the fold's training classes
  * (label values, ascending). Buffers: weights p x max(max_components, 1),
  * val_scores and discriminants n x n, classes n. */
+/* kodama::PLSLDAPredict_CUDA (kodama.hpp:941-946; CPU semantics plscv.cpp:2976-3005):
+ * fit PLS-LDA on train [n_train x p] with labels, predict test [n_test x p]
+ * into pred [n_test]. options: max_components / fixed_components, center,
+ * scale, device. Empty inputs predict nothing; one training class predicts
+ * itself; a degenerate fit is EINVAL (the reference throws DegeneratePLSFit). */
+int kodama_hip_plslda_predict(const float* train, int32_t n_train, const int32_t* labels, const float* test,
+                              int32_t n_test, int32_t p, const kodama_hip_core_options* options, int32_t* pred);
+
 int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels,
                                   const int32_t* folds, int32_t fold,
                                   const kodama_hip_core_options* options, double* weights,

@@ -131,6 +131,24 @@ def PLSLDACV(data, labels, constrain=None, folds=10, stratified=True, seed=1, nc
                 backend="hip")
 
 
+def PLSLDAPredict(train, labels, test, ncomp=None, center=True, scale=True, backend=None, gpu_device=0):
+    """PLSLDAPredict (plscv.cpp:2976-3005): fit PLS-LDA on the training rows
+    and their labels, return the predicted labels of the test rows."""
+    _backend(backend)
+    tr = _matrix(train)
+    te = _matrix(test)
+    lab = N.i32(labels)
+    if te.shape[1] != tr.shape[1] and te.size:
+        raise ValueError("PLSLDAPredict train/test column mismatch.")
+    ncomp = min(50, tr.shape[1]) if ncomp is None else int(ncomp)
+    opt = core_options(classifier=0, max_components=ncomp, fixed_components=ncomp, center=int(bool(center)),
+                       scale=int(bool(scale)), pls_fold_cache=0, device=gpu_device)
+    pred = np.zeros(te.shape[0], np.int32)
+    N.call("kodama_hip_plslda_predict", N.ptr(tr), C.c_int32(tr.shape[0]), N.ptr(lab), N.ptr(te),
+           C.c_int32(te.shape[0]), C.c_int32(tr.shape[1]), C.byref(opt), N.ptr(pred))
+    return pred
+
+
 def plslda_fold_detail(data, labels, folds, fold, options: CoreOptions) -> dict:
     """One PLS-LDA fold evaluated on the device in FP64: SIMPLS weights,
     validation scores and LDA discriminants (validation rows ascending)."""

@@ -232,6 +232,45 @@ int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t r
   });
 }
 
+// PLSLDAPredict_CPU (plscv.cpp:2976-3005): one PLS-LDA fit on all training
+// rows, applied to the test rows. It is the validation fold of a two-fold
+// evaluation of [train; test] in which the training rows form the other fold:
+// the training rows' mean and sd standardise both, the SIMPLS fit takes the
+// requested components capped by the columns and rows - 1, and a single
+// training class predicts itself; a fit without a usable component is an
+// error, as the reference's DegeneratePLSFit.
+int kodama_hip_plslda_predict(const float* train, int32_t n_train, const int32_t* labels, const float* test,
+                              int32_t n_test, int32_t p, const kodama_hip_core_options* options, int32_t* pred) {
+  return guarded([&] {
+    require(train != nullptr && test != nullptr, "PLSLDAPredict input matrix pointer is null.");
+    require(options != nullptr && labels != nullptr && (pred != nullptr || n_test == 0), "null pointer.");
+    require(n_train >= 0 && n_test >= 0 && p >= 0, "negative size.");
+    if (n_train == 0 || n_test == 0 || p == 0) return;
+    IVec values, codes;
+    encode_labels(labels, n_train, values, codes);
+    if (values.size() == 1) {
+      std::fill(pred, pred + n_test, values[0]);
+      return;
+    }
+    const int n = n_train + n_test;
+    std::vector<float> x(static_cast<size_t>(n) * p);
+    std::copy_n(train, static_cast<size_t>(n_train) * p, x.begin());
+    std::copy_n(test, static_cast<size_t>(n_test) * p, x.begin() + static_cast<size_t>(n_train) * p);
+    IVec all_codes(static_cast<size_t>(n), 0), folds(static_cast<size_t>(n), 0);
+    std::copy(codes.begin(), codes.end(), all_codes.begin());
+    std::fill(folds.begin(), folds.begin() + n_train, 1);
+    std::vector<int32_t> lab_all(static_cast<size_t>(n), labels[0]);
+    std::copy_n(labels, n_train, lab_all.begin());
+    validate(x.data(), n, p, lab_all.data(), options);
+    select_device(options->device);
+    PlsCv pls(options->device, x.data(), n, p, *options, static_cast<int>(values.size()));
+    IVec pred_codes(static_cast<size_t>(n));
+    pls.evaluate(all_codes, folds, pred_codes.data(), nullptr);
+    require(pls.detail(0).rank > 0, "PLSLDAPredict: the PLS fit has no usable component.");
+    for (int i = 0; i < n_test; ++i) pred[i] = values[pred_codes[static_cast<size_t>(n_train) + i]];
+  });
+}
+
 int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels,
                                   const int32_t* folds, int32_t fold, const kodama_hip_core_options* options,
                                   double* weights, int32_t* rank, double* val_scores, double* discriminants,

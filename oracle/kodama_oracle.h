@@ -186,6 +186,8 @@ int ref_core_knn_graph(const int32_t* idx, const float* dist, int32_t samples,
 int ref_knncv(const float* x, int32_t n, int32_t p, const int32_t* labels,
               const int32_t* constrain, const oracle_core_options* opt,
               int32_t* pred, int32_t* folds_out);
+int ref_plslda_predict(const float* train, int32_t n_train, const int32_t* labels, const float* test,
+                       int32_t n_test, int32_t p, const oracle_core_options* opt, int32_t* pred);
 int ref_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
                  const int32_t* constrain, const oracle_core_options* opt,
                  int32_t* pred, int32_t* folds_out, int32_t* selected_components);

@@ -387,6 +387,31 @@ def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5
 Oracle.kodama_matrix = _oracle_kodama_matrix
 
 
+def _oracle_plslda_predict(self, train, labels, test, opt, precision=64):
+    """PLSLDAPredict_CPU (plscv.cpp:2976-3005) restated as the validation fold
+    of a two-fold split of [train; test]: its training rows are all of train,
+    so the standardisation, SIMPLS fit (components capped by the columns and
+    rows - 1) and LDA are the single fit of the reference; the prediction is
+    the first class of largest discriminant. One training class predicts
+    itself; a fit without a component raises (DegeneratePLSFit)."""
+    tr, te, lab = _f32(train), _f32(test), _i32(labels)
+    if tr.shape[0] == 0 or te.shape[0] == 0 or tr.shape[1] == 0:
+        return np.zeros(0, np.int32)
+    classes = np.unique(lab)
+    if classes.size == 1:
+        return np.full(te.shape[0], classes[0], np.int32)
+    x = np.vstack([tr, te])
+    lab_all = np.concatenate([lab, np.full(te.shape[0], lab[0], np.int32)])
+    folds = np.concatenate([np.ones(tr.shape[0], np.int32), np.zeros(te.shape[0], np.int32)])
+    d = self.pls_fold_detail(x, lab_all, folds, 0, opt, precision)
+    if d["rank"] == 0:
+        raise ValueError("PLSLDAPredict: degenerate PLS fit")
+    return d["classes"][np.argmax(d["discriminant"], axis=1)].astype(np.int32)
+
+
+Oracle.plslda_predict = _oracle_plslda_predict
+
+
 class ReferenceMatrix(_MatrixBlocks):
     """The reference's own k-means and landmark sampling (oracle/_ref), where built."""
 
@@ -410,6 +435,15 @@ class Reference(_Api):
     @staticmethod
     def available() -> bool:
         return os.path.exists(REF_SO)
+
+    def plslda_predict(self, train, labels, test, opt: CoreOptions):
+        """kodama::PLSLDAPredict_CPU (plscv.cpp:2976-3005)."""
+        tr, te, lab = _f32(train), _f32(test), _i32(labels)
+        pred = np.zeros(te.shape[0], np.int32)
+        self._l.call("plslda_predict", _p(tr, C.c_float), C.c_int32(tr.shape[0]), _p(lab, C.c_int32),
+                     _p(te, C.c_float), C.c_int32(te.shape[0]), C.c_int32(tr.shape[1]), C.byref(opt),
+                     _p(pred, C.c_int32))
+        return pred
 
     def hnsw_search(self, train, query, k, metric=2, ef_search=4096):
         tr = _f32(train)

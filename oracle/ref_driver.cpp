@@ -222,6 +222,18 @@ int ref_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
   });
 }
 
+// kodama::PLSLDAPredict_CPU (plscv.cpp:2976-3005)
+int ref_plslda_predict(const float* train, int32_t n_train, const int32_t* labels, const float* test,
+                       int32_t n_test, int32_t p, const oracle_core_options* opt, int32_t* pred) {
+  return guarded([&] {
+    const kodama::MatrixView tv(train, static_cast<size_t>(n_train), static_cast<size_t>(p));
+    const kodama::MatrixView sv(test, static_cast<size_t>(n_test), static_cast<size_t>(p));
+    const std::vector<int> out =
+      kodama::PLSLDAPredict_CPU(tv, std::vector<int>(labels, labels + n_train), sv, core_options(opt).pls);
+    std::copy(out.begin(), out.end(), pred);
+  });
+}
+
 // kodama::CorePLSLDA_CPU (core.cpp:1626-1646)
 int ref_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labels,
                     const int32_t* constrain, const int32_t* fixed,

@@ -150,3 +150,35 @@ def test_core_plslda_batch_equals_independent_runs():
         assert np.array_equal(batch["clbest"][r], one["clbest"]), r
         assert batch["accbest"][r] == one["accbest"], r
         assert np.array_equal(batch["vect_acc"][r], one["vect_acc"]), r
+
+
+@pytest.mark.parametrize("ncomp,center,scale", [(5, 1, 1), (2, 1, 1), (8, 1, 0)])
+def test_plslda_predict_matches_reference_and_oracle(oracle, ncomp, center, scale):
+    """PLSLDAPredict on the device: the reference's PLSLDAPredict_CPU outputs
+    (golden, FP32) and the oracle's FP64 restatement, with the true labels and
+    with random 12-class labels."""
+    g = golden("plslda_predict.npz")
+    x2, _ = D.config2()
+    train, test = x2[:1500], x2[1500:2000]
+    opt = O.core_options(classifier=0, max_components=ncomp, fixed_components=ncomp, center=center, scale=scale)
+    for name in ("truth", "noisy"):
+        lab = g[f"labels_{name}"]
+        got = K.PLSLDAPredict(train, lab, test, ncomp=ncomp, center=bool(center), scale=bool(scale))
+        assert np.array_equal(got, g[f"pred_{name}_{ncomp}_{center}{scale}"]), name
+        assert np.array_equal(got, oracle.plslda_predict(train, lab, test, opt, precision=64)), name
+
+
+def test_plslda_predict_edge_cases():
+    """One training class predicts itself, an empty test set predicts
+    nothing, constant data is a degenerate fit (EINVAL), mismatched columns
+    are rejected."""
+    rng = np.random.default_rng(5)
+    train = rng.normal(size=(30, 6)).astype(np.float32)
+    test = rng.normal(size=(9, 6)).astype(np.float32)
+    assert np.array_equal(K.PLSLDAPredict(train, np.full(30, 4), test, ncomp=3), np.full(9, 4))
+    assert K.PLSLDAPredict(train, np.arange(30) % 3, test[:0], ncomp=3).shape == (0,)
+    with pytest.raises(K.InvalidArgument):
+        K.PLSLDAPredict(np.ones((30, 6), np.float32), np.arange(30) % 2, test, ncomp=3)
+    with pytest.raises(ValueError):
+        K.PLSLDAPredict(train, np.arange(30) % 2, test[:, :5], ncomp=3)
+

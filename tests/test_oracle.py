@@ -397,6 +397,50 @@ def test_fresh_kmeans_and_quota_match_reference(oracle):
         assert np.array_equal(oracle.quota_sample(lab, 50, target, 3, 1), ref.quota_sample(lab, 50, target, 3, 1))
 
 
+PREDICT_CASES = [(5, 1, 1), (2, 1, 1), (8, 1, 0)]
+
+
+def _predict_inputs():
+    x2, _ = D.config2()
+    return x2[:1500], x2[1500:2000]
+
+
+@pytest.mark.parametrize("ncomp,center,scale", PREDICT_CASES)
+def test_plslda_predict_matches_reference_golden(oracle, ncomp, center, scale):
+    """The oracle's PLSLDAPredict (one validation fold of [train; test], FP32
+    as the reference) against the reference's own PLSLDAPredict_CPU outputs,
+    with the true labels and with random 12-class labels."""
+    g = golden("plslda_predict.npz")
+    train, test = _predict_inputs()
+    opt = O.core_options(classifier=0, max_components=ncomp, fixed_components=ncomp, center=center, scale=scale)
+    for name in ("truth", "noisy"):
+        got = oracle.plslda_predict(train, g[f"labels_{name}"], test, opt, precision=32)
+        assert np.array_equal(got, g[f"pred_{name}_{ncomp}_{center}{scale}"]), name
+
+
+def test_plslda_predict_closed_cases(oracle):
+    """One training class predicts itself; constant data has no PLS component
+    (the reference throws DegeneratePLSFit); two well-separated classes on one
+    axis are told apart."""
+    rng = np.random.default_rng(3)
+    train = rng.normal(size=(40, 4)).astype(np.float32)
+    test = rng.normal(size=(7, 4)).astype(np.float32)
+    opt = O.core_options(classifier=0, max_components=2, fixed_components=2)
+    assert np.array_equal(oracle.plslda_predict(train, np.full(40, 9), test, opt), np.full(7, 9))
+    with pytest.raises(ValueError):
+        oracle.plslda_predict(np.ones((40, 4), np.float32), np.arange(40) % 2 + 1, test, opt)
+    train[:20, 0] += 20.0
+    lab = np.r_[np.full(20, 3), np.full(20, 5)]
+    probe = np.zeros((2, 4), np.float32)
+    probe[0, 0] = 20.0
+    assert np.array_equal(oracle.plslda_predict(train, lab, probe, opt), [3, 5])
+    if O.Reference.available():
+        ref = O.Reference()
+        assert np.array_equal(ref.plslda_predict(train, lab, probe, opt), [3, 5])
+        assert np.array_equal(oracle.plslda_predict(train, lab, test, opt, precision=32),
+                              ref.plslda_predict(train, lab, test, opt))
+
+
 def test_normalize_graph_closed_case(oracle):
     """normalize_external_graph on a hand-worked 4-row graph: self entries, an
     out-of-range id, a NaN and a negative distance (clamped at 0), rows left
