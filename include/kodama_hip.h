@@ -262,6 +262,24 @@ int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distance
                                      const int32_t* run_labels, int32_t runs,
                                      int32_t device);
 
+/* ---- device-resident graph handle --------------------------------------------
+ * kodama::KODAMAGraphHandle + KODAMAGraphMaterialize (kodama.hpp:481-530;
+ * kodama_matrix.cpp:2422-2490, 4364-4381) for the exact self-kNN graph: the
+ * graph stays on the device in its public form (1-based ids, public
+ * distances) until materialised; the KODAMA dissimilarity (kodama_matrix.cpp:
+ * 2685-2776) rewrites it there. A handle serialises its own calls; release it
+ * once. */
+typedef struct kodama_hip_graph kodama_hip_graph;
+/* k neighbours per row, capped at n - 1 (as kodama_hip_knn_graph). */
+int kodama_hip_graph_build(const float* x, int32_t n, int32_t p, int32_t k, int32_t metric, int32_t device,
+                           kodama_hip_graph** out);
+int kodama_hip_graph_info(const kodama_hip_graph* graph, int32_t* samples, int32_t* neighbors, int32_t* device);
+/* indices_one_based / distances [samples x neighbors]. */
+int kodama_hip_graph_materialize(kodama_hip_graph* graph, int32_t* indices_one_based, float* distances);
+/* run_labels [runs x samples]; runs <= 0 leaves the graph as is. */
+int kodama_hip_graph_dissimilarity(kodama_hip_graph* graph, const int32_t* run_labels, int32_t runs);
+void kodama_hip_graph_release(kodama_hip_graph* graph);
+
 /* ---- batched label evolution over M runs ----------------------------------
  * The M-run scheduler of KODAMAMatrix_CUDA (kodama_matrix.cpp:2651-2682,
  * 3823-3850) around CoreKNN_CUDA (3141-3144), extended to a different landmark

@@ -533,6 +533,55 @@ def last_fold_table_path():
     return {0: None, 1: "scan", 2: "mfma"}[int(N.lib().kodama_hip_last_fold_table_path())]
 
 
+class KODAMAGraphHandle:
+    """KODAMAGraphHandle on the device: the exact self-kNN graph built once and
+    kept on the GPU; materialize() returns it 1-based with public distances,
+    dissimilarity(res) applies the KODAMA dissimilarity in place there."""
+
+    def __init__(self, data, k=100, metric="euclidean", gpu_device=0):
+        x = _matrix(data)
+        self._h = C.c_void_p()
+        N.call("kodama_hip_graph_build", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(int(k)),
+               C.c_int32(_metric(metric)), C.c_int32(gpu_device), C.byref(self._h))
+        s, nb, dev = C.c_int32(), C.c_int32(), C.c_int32()
+        N.call("kodama_hip_graph_info", self._h, C.byref(s), C.byref(nb), C.byref(dev))
+        self.samples, self.neighbors, self.device = s.value, nb.value, dev.value
+
+    def valid(self) -> bool:
+        return bool(self._h)
+
+    def materialize(self):
+        self._check()
+        idx = np.zeros((self.samples, self.neighbors), np.int32)
+        dist = np.zeros((self.samples, self.neighbors), np.float32)
+        N.call("kodama_hip_graph_materialize", self._h, N.ptr(idx), N.ptr(dist))
+        return idx, dist
+
+    def dissimilarity(self, res):
+        self._check()
+        r = N.i32(res)
+        if r.ndim != 2 or r.shape[1] != self.samples:
+            raise ValueError("res must be [runs x samples]")
+        N.call("kodama_hip_graph_dissimilarity", self._h, N.ptr(r), C.c_int32(r.shape[0]))
+
+    def release(self):
+        if self._h:
+            fn = N.lib().kodama_hip_graph_release
+            fn.restype = None
+            fn(self._h)
+            self._h = C.c_void_p()
+
+    def _check(self):
+        if not self._h:
+            raise ValueError("the graph handle was released")
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 def KODAMA_dissimilarity(indices, distances, res, gpu_device=0):
     """KODAMADissimilarityInPlace on a 1-based graph; returns corrected copies."""
     idx = np.array(indices, dtype=np.int32, order="C", copy=True)

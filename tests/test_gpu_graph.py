@@ -61,3 +61,28 @@ def test_dissimilarity_edge_cases(oracle, runs):
         bi, bd = dissimilarity_brute_force(idx, dist, res)
         assert np.array_equal(got_i, bi)
         assert np.array_equal(got_d, bd)
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
+def test_graph_handle_materialize_and_dissimilarity(metric):
+    """The device-resident graph handle: materialised it is the exact graph of
+    KODAMA_knn_graph (1-based, public distances); its in-place dissimilarity
+    equals KODAMA_dissimilarity on the host copy; released twice is harmless."""
+    x, _, _ = D.config1(n=900, seed=17)
+    g = K.KODAMA_knn_graph(x, k=12, metric=metric)
+    want_idx, want_dist = g["indices"], g["distances"]
+    h = K.KODAMAGraphHandle(x, k=12, metric=metric)
+    assert (h.samples, h.neighbors) == (900, 12)
+    idx, dist = h.materialize()
+    assert np.array_equal(idx, want_idx) and np.array_equal(dist, want_dist)
+    res = np.stack([D.randint(60 + r, 1, 900, 0, 5) for r in range(6)]).astype(np.int32)
+    ref_idx, ref_dist = K.KODAMA_dissimilarity(want_idx, want_dist, res)
+    h.dissimilarity(res)
+    idx2, dist2 = h.materialize()
+    assert np.array_equal(idx2, ref_idx) and np.array_equal(dist2, ref_dist)
+    h.release()
+    h.release()
+    assert not h.valid()
+    with pytest.raises(ValueError):
+        h.materialize()
+

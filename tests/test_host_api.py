@@ -71,6 +71,20 @@ def test_matrix_from_graph_argument_errors(case):
         K.KODAMA_matrix_from_graph(x, gi, gd, index_base=2 if case == "base" else "auto")
 
 
+def test_graph_handle_null_is_einval():
+    """A null graph handle is an argument error, and releasing it does nothing."""
+    if not N.lib_available():
+        pytest.skip("libkodama_hip.so is not built")
+    import ctypes as C
+    with pytest.raises(K.InvalidArgument):
+        N.call("kodama_hip_graph_info", C.c_void_p(), None, None, None)
+    with pytest.raises(K.InvalidArgument):
+        N.call("kodama_hip_graph_dissimilarity", C.c_void_p(), None, C.c_int32(1))
+    fn = N.lib().kodama_hip_graph_release
+    fn.restype = None
+    fn(C.c_void_p())
+
+
 def test_mlp_init_is_he_normal_and_deterministic():
     if not N.lib_available():
         pytest.skip("libkodama_hip.so is not built")
