@@ -354,10 +354,10 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t sentinel2 = static_cast<uint32_t>(n) | static_cast<uint32_t>(n) << 16;
   int matches = 0;
-  // Uniform trip count: every lane reaches the ballots below. Latency is
-  // hidden by the workgroup's 16 waves; with 8 waves (and the registers that
-  // leaves) each thread loads two pairs' ids before using either.
-  constexpr int kBatch = THREADS <= 512 && KMAX <= 32 ? 2 : 1;
+  // Uniform trip count: every lane reaches the ballots below. One pair of
+  // rows per thread and pass: with the lookups issued in groups, loading two
+  // pairs' ids first (as the 8-wave learned launch once did) gains nothing.
+  constexpr int kBatch = 1;
   for (int m0 = 0; m0 < npairs; m0 += kBatch * THREADS) {
     uint32_t ids_b[kBatch][KMAX];
 #pragma unroll
