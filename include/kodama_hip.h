@@ -307,7 +307,7 @@ int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs
 
 /* ---- KODAMA.matrix orchestration -------------------------------------------
  * Flat subset of kodama::KODAMAMatrixOptions (kodama.hpp:532-566) for data
- * without spatial coordinates, constraints or starting labels. */
+ * without spatial coordinates. */
 typedef struct kodama_hip_matrix_options {
   int32_t runs;             /* M */
   int32_t cycles;           /* Tcycle */
@@ -319,11 +319,15 @@ typedef struct kodama_hip_matrix_options {
   int32_t graph_neighbors;  /* global graph width; <= 0: 100 */
   uint64_t seed;            /* KODAMAMatrixOptions::seed (reference default 1234) */
   int32_t form;             /* 0 = plain (reference streams per run), 1 = device-resident,
-                               2 = device-resident with the learned proposal */
+                               2 = device-resident with the learned proposal (kNN only) */
   int32_t device;
   int32_t n_threads;        /* host threads of the plain form (0 = automatic) */
   const float* mlp_params;  /* form 2: the network, fp32 [10400] (kodama_hip_mlp_train) */
   float mlp_fallback;       /* form 2: share of cycles on the uniform proposal */
+  int32_t classifier;       /* KODAMAMatrixOptions::classifier: 1 = KNN, 0 = PLS_LDA
+                               (CoreClassifier order) */
+  int32_t components;       /* KODAMAMatrixOptions::components (reference default 50),
+                               capped at min(n, p): the PLS-LDA components */
 } kodama_hip_matrix_options;
 
 /* Landmarks per run and global graph width that kodama_hip_kodama_matrix
@@ -331,41 +335,57 @@ typedef struct kodama_hip_matrix_options {
 int kodama_hip_matrix_sizes(int32_t n, const kodama_hip_matrix_options* options, int32_t* landmarks,
                             int32_t* graph_neighbors);
 
-/* kodama::KODAMAMatrix_CUDA with classifier kNN (kodama.hpp:1077-1083;
- * run_kodama_matrix, kodama_matrix.cpp:3324-4051; run_iteration, 2870-3322).
- * A shared k-means atlas of all rows (mt19937_64(seed + 1)); for run r = 1..M
- * an mt19937_64(seed + r) that skips one draw, quota-samples L landmarks over
- * the atlas strata and draws k-means starting labels on them; CoreKNN on the
- * landmarks (per-fold exact neighbours, non-stratified folds, seed + r); every
- * other row takes the majority label of the first k landmarks among its global
- * graph neighbours (exact self-kNN graph). The k-means runs on the device in
- * the reference's float order, so atlas, landmarks and starting labels equal
- * the reference CPU build's.
+/* kodama::KODAMAMatrix_CUDA (kodama.hpp:1077-1083; run_kodama_matrix,
+ * kodama_matrix.cpp:3324-4051; run_iteration, 2870-3322), without spatial
+ * data. A shared k-means atlas of all rows (mt19937_64(seed + 1)); for run
+ * r = 1..M an mt19937_64(seed + r) that skips one draw and quota-samples L
+ * landmarks over the atlas strata; the landmarks' starting labels are
+ * starting_labels (the reference's starting_labels: the search is then greedy,
+ * plain form only) or the k-means labels of the landmarks. Constraint groups
+ * (constrain, [n], any values; NULL: every row its own group) give their
+ * members their majority starting label, move together in the Core and take
+ * their majority label in the result; fixed rows ([n], nonzero = fixed; NULL:
+ * none) keep their label. Core on the landmarks (non-stratified folds, seed +
+ * r): kNN with per-fold exact neighbours, or PLS-LDA with `components` fixed
+ * components and class coarsening. Every other row takes, for kNN, the
+ * majority label of the first k landmarks among its global graph neighbours
+ * (exact self-kNN graph), for PLS-LDA, the PLSLDAPredict label of a fit on the
+ * run's landmarks. The k-means runs on the device in the reference's float
+ * order, so atlas, landmarks and starting labels equal the reference CPU
+ * build's. The device-resident forms (1, 2) run the kNN classifier without
+ * starting labels, constraint groups or fixed rows.
  * Outputs: res [runs x n] labels, acc [runs] (accbest), v [runs x cycles]
  * (may be NULL), landmark_rows [runs x L] (may be NULL), the global graph
- * [n x graph_neighbors] 1-based with public distances (both NULL to skip),
- * timings[6] (may be NULL): atlas, graph, landmarks + starting labels, Core
- * precompute, Core evolution, projection (seconds). */
+ * [n x graph_neighbors] 1-based with public distances (both NULL to skip; the
+ * PLS-LDA classifier builds it only when asked), timings[6] (may be NULL):
+ * atlas, graph, landmarks + starting labels, Core precompute, Core evolution,
+ * projection (seconds). */
 int kodama_hip_kodama_matrix(const float* x, int32_t n, int32_t p,
-                             const kodama_hip_matrix_options* options, int32_t* res, double* acc,
+                             const kodama_hip_matrix_options* options, const int32_t* starting_labels,
+                             const int32_t* constrain, const int32_t* fixed, int32_t* res, double* acc,
                              double* v, int32_t* landmark_rows, int32_t* graph_indices_one_based,
                              float* graph_distances, double* timings);
 
-/* kodama::KODAMAMatrixFromGraphData_CUDA (kodama_matrix.cpp:4676-4690):
- * kodama_hip_kodama_matrix with the caller's neighbour graph in place of the
- * exact global graph. graph_indices / graph_distances_in [n x graph_neighbors]
- * row-major; index_base 0, 1, or -1 (auto: one-based when every id lies in
- * 1..n), normalised as normalize_external_graph (kodama_matrix.cpp:1292-1330):
- * per row the first K = min(graph_neighbors, resolved graph width) valid
- * neighbours (in range, not the row itself, finite distance), distances
- * clamped at 0. The graph outputs (may be NULL) are [n x K], 1-based, the
- * caller's distances. */
+/* kodama::KODAMAMatrixFromGraphData_CUDA (kodama.hpp:1215-1222;
+ * kodama_matrix.cpp:4672-4690): kodama_hip_kodama_matrix with the caller's
+ * neighbour graph in place of the exact global graph. graph_indices /
+ * graph_distances_in [n x graph_neighbors] row-major; index_base 0, 1, or -1
+ * (auto: one-based when every id lies in 1..n), normalised as
+ * normalize_external_graph (kodama_matrix.cpp:1292-1330): per row the first
+ * K = min(graph_neighbors, resolved graph width) valid neighbours (in range,
+ * not the row itself, finite distance), distances clamped at 0. With the kNN
+ * classifier the graph also drives the Core: each run's landmark subgraph
+ * (subset_graph_to_rows, kodama_matrix.cpp:1332-1366) feeds CoreKNNGraph
+ * (core.cpp:624-692): per landmark row the first min(k, K) landmark
+ * neighbours outside its fold, score -d^2 (Euclidean) or 1 - d. The graph
+ * outputs (may be NULL) are [n x K], 1-based, the caller's distances. */
 int kodama_hip_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const int32_t* graph_indices,
                                         const float* graph_distances_in, int32_t graph_neighbors,
                                         int32_t index_base, const kodama_hip_matrix_options* options,
-                                        int32_t* res, double* acc, double* v, int32_t* landmark_rows,
-                                        int32_t* graph_indices_one_based, float* graph_distances,
-                                        double* timings);
+                                        const int32_t* starting_labels, const int32_t* constrain,
+                                        const int32_t* fixed, int32_t* res, double* acc, double* v,
+                                        int32_t* landmark_rows, int32_t* graph_indices_one_based,
+                                        float* graph_distances, double* timings);
 
 /* ---- learned proposal -------------------------------------------------------
  * No counterpart in the reference: the learned proposal that the device-resident

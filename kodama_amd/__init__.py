@@ -393,76 +393,30 @@ def core_knn_batch_learned(data, run_rows, initial_labels, options: CoreOptions,
 
 # ---- KODAMA.matrix ----------------------------------------------------------------
 
-def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5, k=30,
-                  metric="euclidean", graph_neighbors=100, seed=1234, form="plain", gpu_device=0,
-                  n_threads=0, dissimilarity=False, mlp_params=None, fallback=0.25) -> dict:
-    """KODAMA.matrix with the kNN classifier on the HIP backend
-    (KODAMAMatrix_CUDA contract; kodama_matrix.cpp:3324-4051): shared k-means
-    landmark atlas, per-run quota landmarks and k-means starting labels, CoreKNN
-    on the landmarks for all M runs in one batch (form "plain" replays the
-    reference streams, "device" keeps the evolution on the GPU, "learned" adds
-    the learned proposal of ``mlp_params`` with ``fallback`` uniform cycles),
-    projection of the other rows through the exact global graph.
-
-    Returns res [M x n], acc [M], v [M x Tcycle], the landmark rows [M x L], the
-    1-based global graph (indices, distances) and per-stage timings; with
-    ``dissimilarity`` the graph is rewritten by the KODAMA dissimilarity.
-    """
-    x = _matrix(data)
+def _matrix_call(x, graph, index_base, M, Tcycle, landmarks, splitting, folds, k, metric, graph_neighbors, seed,
+                 form, gpu_device, n_threads, mlp_params, fallback, classifier, components, W, constrain, fixed):
     n = x.shape[0]
     if form == "learned" and mlp_params is None:
         raise ValueError('form "learned" needs mlp_params')
+    if classifier not in ("knn", "pls_lda"):
+        raise ValueError('classifier must be "knn" or "pls_lda"')
     par = None if mlp_params is None else np.ascontiguousarray(mlp_params, np.float32)
+    extra = []
+    for name, a in (("W", W), ("constrain", constrain), ("fixed", fixed)):
+        a = N.i32(a)
+        if a is not None and a.shape != (n,):
+            raise ValueError(f"{name} must hold one value per row")
+        extra.append(a)
     o = N.MatrixOptions(runs=int(M), cycles=int(Tcycle), landmarks=int(landmarks), splitting=int(splitting),
                         folds=int(folds), k=int(k), metric=_metric(metric), graph_neighbors=int(graph_neighbors),
                         seed=int(seed), form={"plain": 0, "device": 1, "learned": 2}[form], device=int(gpu_device),
                         n_threads=int(n_threads), mlp_params=N.ptr(par) if form == "learned" else None,
-                        mlp_fallback=float(fallback))
-    L = C.c_int32(0)
-    K = C.c_int32(0)
-    N.call("kodama_hip_matrix_sizes", C.c_int32(n), C.byref(o), C.byref(L), C.byref(K))
-    res = np.zeros((M, n), np.int32)
-    acc = np.zeros(M, np.float64)
-    v = np.zeros((M, max(int(Tcycle), 1)), np.float64)
-    land = np.zeros((M, L.value), np.int32)
-    gi = np.zeros((n, K.value), np.int32)
-    gd = np.zeros((n, K.value), np.float32)
-    timings = np.zeros(6, np.float64)
-    N.call("kodama_hip_kodama_matrix", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o), N.ptr(res),
-           N.ptr(acc), N.ptr(v), N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))
-    out = dict(res=res, acc=acc, v=v[:, : int(Tcycle)].copy(), landmarks=land, knn_indices=gi, knn_distances=gd,
-               timings=dict(zip(("atlas", "graph", "landmarks", "core_precompute", "core_evolution",
-                                 "projection"), timings.tolist())))
-    if dissimilarity:
-        out["knn_indices"], out["knn_distances"] = KODAMA_dissimilarity(gi, gd, res, gpu_device)
-    return out
-
-
-def KODAMA_matrix_from_graph(data, indices, distances, index_base="auto", M=100, Tcycle=20, landmarks=10000,
-                             splitting=0, folds=5, k=30, metric="euclidean", graph_neighbors=100, seed=1234,
-                             form="plain", gpu_device=0, n_threads=0, mlp_params=None, fallback=0.25) -> dict:
-    """KODAMAMatrixFromGraphData: KODAMA_matrix with the caller's neighbour graph
-    (indices / distances [n x neighbours], index_base "auto", 0 or 1) instead of
-    the exact global graph; returns the same dictionary (graph outputs [n x K],
-    K = min(neighbours, resolved width))."""
-    x = _matrix(data)
-    n = x.shape[0]
-    gi_in = N.i32(np.ascontiguousarray(indices))
-    gd_in = np.ascontiguousarray(distances, np.float32)
-    if gi_in.shape != gd_in.shape or gi_in.shape[0] != n:
-        raise ValueError("indices and distances must both be [n x neighbours]")
-    if form == "learned" and mlp_params is None:
-        raise ValueError('form "learned" needs mlp_params')
-    par = None if mlp_params is None else np.ascontiguousarray(mlp_params, np.float32)
-    o = N.MatrixOptions(runs=int(M), cycles=int(Tcycle), landmarks=int(landmarks), splitting=int(splitting),
-                        folds=int(folds), k=int(k), metric=_metric(metric), graph_neighbors=int(graph_neighbors),
-                        seed=int(seed), form={"plain": 0, "device": 1, "learned": 2}[form], device=int(gpu_device),
-                        n_threads=int(n_threads), mlp_params=N.ptr(par) if form == "learned" else None,
-                        mlp_fallback=float(fallback))
+                        mlp_fallback=float(fallback), classifier=1 if classifier == "knn" else 0,
+                        components=int(components))
     L = C.c_int32(0)
     Kr = C.c_int32(0)
     N.call("kodama_hip_matrix_sizes", C.c_int32(n), C.byref(o), C.byref(L), C.byref(Kr))
-    K = max(1, min(int(Kr.value), gi_in.shape[1]))
+    K = Kr.value if graph is None else max(1, min(int(Kr.value), graph[0].shape[1]))
     res = np.zeros((M, n), np.int32)
     acc = np.zeros(M, np.float64)
     v = np.zeros((M, max(int(Tcycle), 1)), np.float64)
@@ -470,13 +424,69 @@ def KODAMA_matrix_from_graph(data, indices, distances, index_base="auto", M=100,
     gi = np.zeros((n, K), np.int32)
     gd = np.zeros((n, K), np.float32)
     timings = np.zeros(6, np.float64)
-    base = -1 if index_base == "auto" else int(index_base)
-    N.call("kodama_hip_kodama_matrix_from_graph", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), N.ptr(gi_in),
-           N.ptr(gd_in), C.c_int32(gi_in.shape[1]), C.c_int32(base), C.byref(o), N.ptr(res), N.ptr(acc), N.ptr(v),
-           N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))
+    ptrs = [N.ptr(a) for a in extra]
+    outs = (N.ptr(res), N.ptr(acc), N.ptr(v), N.ptr(land), N.ptr(gi), N.ptr(gd), N.ptr(timings))
+    if graph is None:
+        N.call("kodama_hip_kodama_matrix", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o), *ptrs, *outs)
+    else:
+        base = -1 if index_base == "auto" else int(index_base)
+        N.call("kodama_hip_kodama_matrix_from_graph", N.ptr(x), C.c_int32(n), C.c_int32(x.shape[1]),
+               N.ptr(graph[0]), N.ptr(graph[1]), C.c_int32(graph[0].shape[1]), C.c_int32(base), C.byref(o), *ptrs,
+               *outs)
     return dict(res=res, acc=acc, v=v[:, : int(Tcycle)].copy(), landmarks=land, knn_indices=gi, knn_distances=gd,
                 timings=dict(zip(("atlas", "graph", "landmarks", "core_precompute", "core_evolution",
                                   "projection"), timings.tolist())))
+
+
+def KODAMA_matrix(data, M=100, Tcycle=20, landmarks=10000, splitting=0, folds=5, k=30,
+                  metric="euclidean", graph_neighbors=100, seed=1234, form="plain", gpu_device=0,
+                  n_threads=0, dissimilarity=False, mlp_params=None, fallback=0.25, classifier="knn",
+                  components=50, W=None, constrain=None, fixed=None) -> dict:
+    """KODAMA.matrix on the HIP backend (KODAMAMatrix_CUDA contract;
+    kodama_matrix.cpp:3324-4051): shared k-means landmark atlas, per-run quota
+    landmarks and k-means starting labels (or ``W``, the reference's
+    starting_labels, which make the search greedy), ``constrain`` groups and
+    ``fixed`` rows as the reference's, the Core on the landmarks for all M
+    runs in one batch, projection of the other rows.
+
+    classifier "knn": CoreKNN (form "plain" replays the reference streams,
+    "device" keeps the evolution on the GPU, "learned" adds the learned
+    proposal of ``mlp_params`` with ``fallback`` uniform cycles), projection
+    through the exact global graph. classifier "pls_lda": CorePLSLDA with
+    ``components`` components (plain form), projection by PLSLDAPredict.
+
+    Returns res [M x n], acc [M], v [M x Tcycle], the landmark rows [M x L], the
+    1-based global graph (indices, distances) and per-stage timings; with
+    ``dissimilarity`` the graph is rewritten by the KODAMA dissimilarity.
+    """
+    x = _matrix(data)
+    out = _matrix_call(x, None, "auto", M, Tcycle, landmarks, splitting, folds, k, metric, graph_neighbors, seed,
+                       form, gpu_device, n_threads, mlp_params, fallback, classifier, components, W, constrain,
+                       fixed)
+    if dissimilarity:
+        out["knn_indices"], out["knn_distances"] = KODAMA_dissimilarity(out["knn_indices"], out["knn_distances"],
+                                                                        out["res"], gpu_device)
+    return out
+
+
+def KODAMA_matrix_from_graph(data, indices, distances, index_base="auto", M=100, Tcycle=20, landmarks=10000,
+                             splitting=0, folds=5, k=30, metric="euclidean", graph_neighbors=100, seed=1234,
+                             form="plain", gpu_device=0, n_threads=0, mlp_params=None, fallback=0.25,
+                             classifier="knn", components=50, W=None, constrain=None, fixed=None) -> dict:
+    """KODAMAMatrixFromGraphData: KODAMA_matrix with the caller's neighbour graph
+    (indices / distances [n x neighbours], index_base "auto", 0 or 1) in place
+    of the exact global graph. With the kNN classifier the graph drives the
+    Core too: each run's landmark subgraph feeds CoreKNNGraph
+    (kodama_matrix.cpp:3178-3192). Returns the same dictionary (graph outputs
+    [n x K], K = min(neighbours, resolved width))."""
+    x = _matrix(data)
+    gi_in = N.i32(np.ascontiguousarray(indices))
+    gd_in = np.ascontiguousarray(distances, np.float32)
+    if gi_in.shape != gd_in.shape or gi_in.shape[0] != x.shape[0]:
+        raise ValueError("indices and distances must both be [n x neighbours]")
+    return _matrix_call(x, (gi_in, gd_in), index_base, M, Tcycle, landmarks, splitting, folds, k, metric,
+                        graph_neighbors, seed, form, gpu_device, n_threads, mlp_params, fallback, classifier,
+                        components, W, constrain, fixed)
 
 
 # ---- graph and dissimilarity ----------------------------------------------------

@@ -89,6 +89,7 @@ class MatrixOptions(C.Structure):
         ("folds", C.c_int32), ("k", C.c_int32), ("metric", C.c_int32), ("graph_neighbors", C.c_int32),
         ("seed", C.c_uint64), ("form", C.c_int32), ("device", C.c_int32), ("n_threads", C.c_int32),
         ("mlp_params", C.POINTER(C.c_float)), ("mlp_fallback", C.c_float),
+        ("classifier", C.c_int32), ("components", C.c_int32),
     ]
 
 

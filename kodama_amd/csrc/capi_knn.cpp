@@ -53,29 +53,40 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
   const float* x = q.x;
   const int n = q.n, p = q.p, runs = q.runs;
   const kodama_hip_core_options* options = q.options;
-  require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
+  const bool tables = q.table_nbr != nullptr;
+  require(tables || (x != nullptr && n > 0 && p > 0), "invalid data matrix.");
   require(runs >= 1, "runs must be positive.");
-  require(q.run_offsets != nullptr && q.landmark_rows != nullptr && q.initial_labels != nullptr, "null input.");
+  require(q.run_offsets != nullptr && q.initial_labels != nullptr, "null input.");
+  require(tables || q.landmark_rows != nullptr, "null input.");
+  require(!tables || (q.table_score != nullptr && q.fold_ids != nullptr && q.table_k >= 1),
+          "caller tables need scores, fold ids and a width.");
   require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
   require(options->cycles >= 0, "Core cycles must be non-negative.");
   require(mode != 2 || mlp_params != nullptr, "the learned form needs network parameters.");
+  require(mode == 0 || (q.constrain == nullptr && q.fixed == nullptr),
+          "the device-resident evolution supports neither constraint groups nor fixed rows.");
   select_device(options->device);
   const auto t0 = std::chrono::steady_clock::now();
   const IVec offsets(q.run_offsets, q.run_offsets + runs + 1);
   require(offsets[0] == 0, "run_offsets[0] must be 0.");
   for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
   const int total = offsets[runs];
-  const IVec rows(q.landmark_rows, q.landmark_rows + total);
-  for (int v : rows) require(v >= 0 && v < n, "landmark row outside the data matrix.");
+  IVec rows;
+  if (!tables) {
+    rows.assign(q.landmark_rows, q.landmark_rows + total);
+    for (int v : rows) require(v >= 0 && v < n, "landmark row outside the data matrix.");
+  }
   std::vector<IVec> folds(static_cast<size_t>(runs)), values(static_cast<size_t>(runs)),
-    codes(static_cast<size_t>(runs));
+    codes(static_cast<size_t>(runs)), cons(static_cast<size_t>(runs)), fixed(static_cast<size_t>(runs));
   IVec fold_concat(static_cast<size_t>(total));
   int classes = 1;
   for (int r = 0; r < runs; ++r) {
     const int L = offsets[r + 1] - offsets[r];
     const int* lab = q.initial_labels + offsets[r];
+    if (q.constrain) cons[r].assign(q.constrain + offsets[r], q.constrain + offsets[r + 1]);
+    if (q.fixed) fixed[r].assign(q.fixed + offsets[r], q.fixed + offsets[r + 1]);
     folds[r] = q.fold_ids ? IVec(q.fold_ids + offsets[r], q.fold_ids + offsets[r + 1])
-                          : make_folds(IVec(lab, lab + L), IVec(), options->folds, options->stratified != 0,
+                          : make_folds(IVec(lab, lab + L), cons[r], options->folds, options->stratified != 0,
                                        options->cv_seed + static_cast<std::uint64_t>(r));
     {
       // Two folds or more: every row then has training rows, so the exact
@@ -87,9 +98,15 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
     classes = std::max(classes, static_cast<int>(values[r].size()));
     std::copy(folds[r].begin(), folds[r].end(), fold_concat.begin() + offsets[r]);
   }
-  const std::vector<float> xs = prepare_rows(x, n, p, options->metric, 0);
-  KnnBatch batch(options->device, xs.data(), n, p, offsets, rows, fold_concat, options->k, options->metric,
-                 classes);
+  std::unique_ptr<KnnBatch> owned;
+  if (tables) {
+    owned = std::make_unique<KnnBatch>(options->device, q.table_nbr, q.table_score, offsets, q.table_k, classes);
+  } else {
+    const std::vector<float> xs = prepare_rows(x, n, p, options->metric, 0);
+    owned = std::make_unique<KnnBatch>(options->device, xs.data(), n, p, offsets, rows, fold_concat, options->k,
+                                       options->metric, classes);
+  }
+  KnnBatch& batch = *owned;
   const double t_pre = seconds_since(t0);
   const auto t1 = std::chrono::steady_clock::now();
   double vote_seconds = 0.0;
@@ -100,7 +117,7 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
       CoreFlags f = CoreFlags::from(*options);
       f.knn = true;
       f.seed = options->seed + static_cast<std::uint64_t>(r);
-      ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
+      ev.emplace_back(values[r], codes[r], cons[r], fixed[r], f);
     }
     if (trace) evolve_plain_traced(ev, batch, folds, options->cycles, options->n_threads, *trace);
     else evolve_plain(ev, batch, folds, options->cycles, options->n_threads);
@@ -169,7 +186,7 @@ int kodama_hip_build_knn_vote_graph(const int32_t* neighbor_rows, const float* s
     require(samples >= 1 && neighbors >= 1, "invalid vote graph shape.");
     select_device(device);
     auto g = std::make_unique<kodama_hip_vote_graph>();
-    g->batch = std::make_unique<KnnBatch>(device, neighbor_rows, scores, samples, neighbors, 1);
+    g->batch = std::make_unique<KnnBatch>(device, neighbor_rows, scores, IVec{0, samples}, neighbors, 1);
     g->samples = samples;
     *out_graph = g.release();
   });
@@ -333,7 +350,7 @@ int kodama_hip_core_knn_graph(const int32_t* graph_indices, const float* graph_d
     }
     IVec values, codes;
     encode_labels(labels, samples, values, codes);
-    KnnBatch batch(options->device, nbr.data(), score.data(), samples, k, static_cast<int>(values.size()));
+    KnnBatch batch(options->device, nbr.data(), score.data(), IVec{0, samples}, k, static_cast<int>(values.size()));
     std::vector<RunEvolution> runs;
     runs.emplace_back(values, codes, cons, vec_or_empty(fixed, samples), f);
     evolve_plain(runs, batch, std::vector<IVec>{folds}, options->cycles, 1);

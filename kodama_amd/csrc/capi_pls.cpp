@@ -9,6 +9,7 @@
 #include "capi_common.h"
 #include "engine.h"
 #include "pls_engine.h"
+#include "batch_api.h"
 
 using namespace kodama_hip;
 
@@ -120,115 +121,205 @@ int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* 
   });
 }
 
+}  // extern "C"
+
+namespace kodama_hip {
+
 // The M-run form of CorePLSLDA: run r evolves its landmark rows with seed + r
 // and cv_seed + r (kodama_matrix.cpp:3113, 3123-3135); every run's proposals
 // come from its own host engine with the reference's random streams, and one
 // group + evaluation launch per cycle serves all runs x folds: the landmark
 // rows of all runs are concatenated and fold f of run r is fold r * F + f of
-// that matrix. Per run the result equals kodama_hip_core_plslda on its rows.
+// that matrix. Per run the result equals kodama_hip_core_plslda on its rows
+// (with the run's constraint groups and fixed rows, when given).
+void run_plslda_batch(const BatchRequest& q, int32_t* clbest, double* accbest, double* vect_acc,
+                      double* timings) {
+  const float* x = q.x;
+  const int n = q.n, p = q.p, runs = q.runs;
+  const kodama_hip_core_options* options = q.options;
+  const int32_t* fold_ids = q.fold_ids;
+  require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
+  require(runs >= 1, "runs must be positive.");
+  require(q.run_offsets != nullptr && q.landmark_rows != nullptr && q.initial_labels != nullptr, "null input.");
+  require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
+  require(options->max_components >= 1, "PLSOptions::max_components must be positive.");
+  require(options->folds >= 2, "PLS cross-validation needs at least two folds.");
+  require(options->cycles >= 0, "Core cycles must be non-negative.");
+  require(options->stratified == 0 || fold_ids == nullptr, "explicit folds are unstratified.");
+  select_device(options->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  const IVec offsets(q.run_offsets, q.run_offsets + runs + 1);
+  require(offsets[0] == 0, "run_offsets[0] must be 0.");
+  for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
+  const int total = offsets[runs];
+  std::vector<float> xcat(static_cast<size_t>(total) * p);
+  for (int i = 0; i < total; ++i) {
+    const int row = q.landmark_rows[i];
+    require(row >= 0 && row < n, "landmark row outside the data matrix.");
+    std::copy(x + static_cast<size_t>(row) * p, x + static_cast<size_t>(row + 1) * p,
+              xcat.begin() + static_cast<size_t>(i) * p);
+  }
+  std::vector<IVec> values(static_cast<size_t>(runs)), codes(static_cast<size_t>(runs)),
+    cons(static_cast<size_t>(runs)), fixed(static_cast<size_t>(runs));
+  int classes = 1;
+  for (int r = 0; r < runs; ++r) {
+    encode_labels(q.initial_labels + offsets[r], offsets[r + 1] - offsets[r], values[r], codes[r]);
+    classes = std::max(classes, static_cast<int>(values[r].size()));
+    if (q.constrain) cons[r].assign(q.constrain + offsets[r], q.constrain + offsets[r + 1]);
+    if (q.fixed) fixed[r].assign(q.fixed + offsets[r], q.fixed + offsets[r + 1]);
+  }
+  const int F = options->folds;
+  const bool cache = options->stratified == 0;
+  auto run_folds = [&](int r, const IVec& labs) {
+    IVec fo = fold_ids ? IVec(fold_ids + offsets[r], fold_ids + offsets[r + 1])
+                       : make_folds(labs, cons[r], F, options->stratified != 0,
+                                    options->cv_seed + static_cast<std::uint64_t>(r));
+    for (int v : fo) require(v >= 0 && v < std::max(F, 1 << 20), "fold id out of range.");
+    return fo;
+  };
+  int fold_span = F;
+  if (fold_ids) {
+    for (int i = 0; i < total; ++i) fold_span = std::max(fold_span, fold_ids[i] + 1);
+  }
+  IVec gfolds(static_cast<size_t>(total));
+  auto set_folds = [&](int r, const IVec& fo) {
+    for (size_t i = 0; i < fo.size(); ++i) gfolds[offsets[r] + i] = r * fold_span + fo[i];
+  };
+  for (int r = 0; r < runs; ++r) {
+    const IVec fo = run_folds(r, IVec(q.initial_labels + offsets[r], q.initial_labels + offsets[r + 1]));
+    const auto mm = std::minmax_element(fo.begin(), fo.end());
+    require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
+    set_folds(r, fo);
+  }
+  PlsCv pls(options->device, xcat.data(), total, p, *options, classes, offsets);
+  std::vector<RunEvolution> ev;
+  ev.reserve(static_cast<size_t>(runs));
+  for (int r = 0; r < runs; ++r) {
+    CoreFlags f = CoreFlags::from(*options);
+    f.knn = false;
+    f.seed = options->seed + static_cast<std::uint64_t>(r);
+    ev.emplace_back(values[r], codes[r], cons[r], fixed[r], f);
+  }
+  const double t_pre = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto t1 = std::chrono::steady_clock::now();
+  const int threads = resolve_threads(options->n_threads, runs);
+  IVec ccat(static_cast<size_t>(total)), pred(static_cast<size_t>(total));
+  auto evaluate_all = [&](int cycle) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int r = 0; r < runs; ++r) {
+      const IVec& cl = cycle < 0 ? ev[r].initial() : ev[r].propose(cycle);
+      std::copy(cl.begin(), cl.end(), ccat.begin() + offsets[r]);
+      if (!cache) set_folds(r, run_folds(r, values_of(cl, values[r])));
+    }
+    pls.evaluate(ccat, gfolds, pred.data(), nullptr);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int r = 0; r < runs; ++r) {
+      const IVec cl(ccat.begin() + offsets[r], ccat.begin() + offsets[r + 1]);
+      IVec counts, firsts;
+      const Tally t = tally_of(cl, pred.data() + offsets[r], static_cast<int>(values[r].size()), counts, firsts);
+      if (cycle < 0) ev[r].begin(pred.data() + offsets[r], t);
+      else ev[r].finish(cycle, pred.data() + offsets[r], t);
+    }
+  };
+  evaluate_all(-1);
+  for (int cycle = 0; cycle < options->cycles; ++cycle) evaluate_all(cycle);
+  const double t_evo = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+  for (int r = 0; r < runs; ++r) {
+    const IVec& b = ev[r].best_codes();
+    for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];
+    accbest[r] = ev[r].accbest();
+    if (vect_acc) {
+      std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(), vect_acc + static_cast<size_t>(r) * options->cycles);
+    }
+  }
+  if (timings) {
+    timings[0] = t_pre;
+    timings[1] = t_evo;
+    timings[2] = static_cast<double>(runs) * options->cycles;
+    timings[3] = pls.eval_kernel_ms_average() * 1e-3;
+  }
+}
+
+// PLSLDAPredict_CPU (plscv.cpp:2976-3005) for a batch of fits: fit r trains
+// on the rows train_rows[train_off[r] .. train_off[r+1]) of x with their
+// labels and predicts the rows test_rows[test_off[r] .. test_off[r+1]) into
+// pred (same indexing). Each fit is the validation fold of a two-fold split of
+// [train; test]; fits go through the fused evaluation in groups whose rows fit
+// a bounded staging matrix. A fit with one training class predicts it; a fit
+// without a usable component is EINVAL (DegeneratePLSFit).
+void plslda_predict_batch(const float* x, int n, int p, int fits, const IVec& train_off, const int* train_rows,
+                          const int* labels, const IVec& test_off, const int* test_rows,
+                          const kodama_hip_core_options& options, int* pred) {
+  const size_t budget = size_t(1) << 26;  // floats staged per group (256 MiB)
+  int r0 = 0;
+  while (r0 < fits) {
+    size_t rows = 0;
+    int r1 = r0;
+    while (r1 < fits) {
+      const size_t add = static_cast<size_t>(train_off[r1 + 1] - train_off[r1] + test_off[r1 + 1] - test_off[r1]);
+      if (r1 > r0 && (rows + add) * p > budget) break;
+      rows += add;
+      ++r1;
+    }
+    // Stage the group's fits: [train; test] per fit, folds 2 g + 1 (train)
+    // and 2 g (test), label codes of each fit's own classes.
+    IVec offsets{0}, all_codes, folds, ncls;
+    std::vector<IVec> values(static_cast<size_t>(r1 - r0));
+    std::vector<float> xs(rows * p);
+    size_t at = 0;
+    int classes = 1;
+    std::vector<int> live;
+    for (int r = r0; r < r1; ++r) {
+      const int g = r - r0;
+      const int nt = train_off[r + 1] - train_off[r], nq = test_off[r + 1] - test_off[r];
+      IVec codes;
+      encode_labels(labels + train_off[r], nt, values[g], codes);
+      if (nq == 0) continue;
+      require(nt > 0, "PLSLDAPredict: no training rows.");
+      if (values[g].size() == 1) {
+        std::fill(pred + test_off[r], pred + test_off[r + 1], values[g][0]);
+        continue;
+      }
+      classes = std::max(classes, static_cast<int>(values[g].size()));
+      for (int i = 0; i < nt + nq; ++i) {
+        const int row = i < nt ? train_rows[train_off[r] + i] : test_rows[test_off[r] + i - nt];
+        std::copy_n(x + static_cast<size_t>(row) * p, p, xs.begin() + (at + i) * p);
+        all_codes.push_back(i < nt ? codes[i] : 0);
+        folds.push_back(i < nt ? 2 * static_cast<int>(live.size()) + 1 : 2 * static_cast<int>(live.size()));
+      }
+      at += static_cast<size_t>(nt + nq);
+      offsets.push_back(static_cast<int>(at));
+      live.push_back(r);
+    }
+    if (!live.empty()) {
+      PlsCv pls(options.device, xs.data(), static_cast<int>(at), p, options, classes, offsets);
+      IVec pc(at);
+      pls.evaluate(all_codes, folds, pc.data(), nullptr);
+      for (size_t j = 0; j < live.size(); ++j) {
+        const int r = live[j];
+        require(pls.detail(static_cast<int>(2 * j)).rank > 0, "PLSLDAPredict: the PLS fit has no usable component.");
+        const int nt = train_off[r + 1] - train_off[r];
+        const IVec& vals = values[r - r0];
+        for (int i = test_off[r]; i < test_off[r + 1]; ++i) {
+          pred[i] = vals[pc[static_cast<size_t>(offsets[j]) + nt + (i - test_off[r])]];
+        }
+      }
+    }
+    r0 = r1;
+  }
+}
+
+}  // namespace kodama_hip
+
+extern "C" {
+
 int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
                                  const int32_t* landmark_rows, const int32_t* initial_labels,
                                  const int32_t* fold_ids, const kodama_hip_core_options* options,
                                  int32_t* clbest, double* accbest, double* vect_acc, double* timings) {
   return guarded([&] {
-    require(x != nullptr && n > 0 && p > 0, "invalid data matrix.");
-    require(runs >= 1, "runs must be positive.");
-    require(run_offsets != nullptr && landmark_rows != nullptr && initial_labels != nullptr, "null input.");
-    require(options != nullptr && clbest != nullptr && accbest != nullptr, "null options or output.");
-    require(options->max_components >= 1, "PLSOptions::max_components must be positive.");
-    require(options->folds >= 2, "PLS cross-validation needs at least two folds.");
-    require(options->cycles >= 0, "Core cycles must be non-negative.");
-    require(options->stratified == 0 || fold_ids == nullptr, "explicit folds are unstratified.");
-    select_device(options->device);
-    const auto t0 = std::chrono::steady_clock::now();
-    const IVec offsets(run_offsets, run_offsets + runs + 1);
-    require(offsets[0] == 0, "run_offsets[0] must be 0.");
-    for (int r = 0; r < runs; ++r) require(offsets[r + 1] > offsets[r] + 1, "every run needs at least two rows.");
-    const int total = offsets[runs];
-    std::vector<float> xcat(static_cast<size_t>(total) * p);
-    for (int i = 0; i < total; ++i) {
-      const int row = landmark_rows[i];
-      require(row >= 0 && row < n, "landmark row outside the data matrix.");
-      std::copy(x + static_cast<size_t>(row) * p, x + static_cast<size_t>(row + 1) * p,
-                xcat.begin() + static_cast<size_t>(i) * p);
-    }
-    std::vector<IVec> values(static_cast<size_t>(runs)), codes(static_cast<size_t>(runs));
-    int classes = 1;
-    for (int r = 0; r < runs; ++r) {
-      encode_labels(initial_labels + offsets[r], offsets[r + 1] - offsets[r], values[r], codes[r]);
-      classes = std::max(classes, static_cast<int>(values[r].size()));
-    }
-    const int F = options->folds;
-    const bool cache = options->stratified == 0;
-    auto run_folds = [&](int r, const IVec& labs) {
-      IVec fo = fold_ids ? IVec(fold_ids + offsets[r], fold_ids + offsets[r + 1])
-                         : make_folds(labs, IVec(), F, options->stratified != 0,
-                                      options->cv_seed + static_cast<std::uint64_t>(r));
-      for (int v : fo) require(v >= 0 && v < std::max(F, 1 << 20), "fold id out of range.");
-      return fo;
-    };
-    int fold_span = F;
-    if (fold_ids) {
-      for (int i = 0; i < total; ++i) fold_span = std::max(fold_span, fold_ids[i] + 1);
-    }
-    IVec gfolds(static_cast<size_t>(total));
-    auto set_folds = [&](int r, const IVec& fo) {
-      for (size_t i = 0; i < fo.size(); ++i) gfolds[offsets[r] + i] = r * fold_span + fo[i];
-    };
-    for (int r = 0; r < runs; ++r) {
-      const IVec fo = run_folds(r, IVec(initial_labels + offsets[r], initial_labels + offsets[r + 1]));
-      const auto mm = std::minmax_element(fo.begin(), fo.end());
-      require(*mm.first != *mm.second, "every run needs at least two distinct folds.");
-      set_folds(r, fo);
-    }
-    PlsCv pls(options->device, xcat.data(), total, p, *options, classes, offsets);
-    std::vector<RunEvolution> ev;
-    ev.reserve(static_cast<size_t>(runs));
-    for (int r = 0; r < runs; ++r) {
-      CoreFlags f = CoreFlags::from(*options);
-      f.knn = false;
-      f.seed = options->seed + static_cast<std::uint64_t>(r);
-      ev.emplace_back(values[r], codes[r], IVec(), IVec(), f);
-    }
-    const double t_pre = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const auto t1 = std::chrono::steady_clock::now();
-    const int threads = resolve_threads(options->n_threads, runs);
-    IVec ccat(static_cast<size_t>(total)), pred(static_cast<size_t>(total));
-    auto evaluate_all = [&](int cycle) {
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
-      for (int r = 0; r < runs; ++r) {
-        const IVec& cl = cycle < 0 ? ev[r].initial() : ev[r].propose(cycle);
-        std::copy(cl.begin(), cl.end(), ccat.begin() + offsets[r]);
-        if (!cache) set_folds(r, run_folds(r, values_of(cl, values[r])));
-      }
-      pls.evaluate(ccat, gfolds, pred.data(), nullptr);
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
-      for (int r = 0; r < runs; ++r) {
-        const IVec cl(ccat.begin() + offsets[r], ccat.begin() + offsets[r + 1]);
-        IVec counts, firsts;
-        const Tally t = tally_of(cl, pred.data() + offsets[r], static_cast<int>(values[r].size()), counts, firsts);
-        if (cycle < 0) ev[r].begin(pred.data() + offsets[r], t);
-        else ev[r].finish(cycle, pred.data() + offsets[r], t);
-      }
-    };
-    evaluate_all(-1);
-    for (int cycle = 0; cycle < options->cycles; ++cycle) evaluate_all(cycle);
-    const double t_evo = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
-    for (int r = 0; r < runs; ++r) {
-      const IVec& b = ev[r].best_codes();
-      for (size_t i = 0; i < b.size(); ++i) clbest[offsets[r] + i] = values[r][b[i]];
-      accbest[r] = ev[r].accbest();
-      if (vect_acc) {
-        std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(),
-                  vect_acc + static_cast<size_t>(r) * options->cycles);
-      }
-    }
-    if (timings) {
-      timings[0] = t_pre;
-      timings[1] = t_evo;
-      timings[2] = static_cast<double>(runs) * options->cycles;
-      timings[3] = pls.eval_kernel_ms_average() * 1e-3;
-    }
+    BatchRequest q{x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options};
+    run_plslda_batch(q, clbest, accbest, vect_acc, timings);
   });
 }
 

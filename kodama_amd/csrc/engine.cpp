@@ -667,30 +667,29 @@ void KnnBatch::find_empty_rows() {
   }
 }
 
-KnnBatch::KnnBatch(int device, const int* neighbors, const float* scores, int samples, int k, int classes)
-    : stream_(device), runs_(1), k_(k), classes_(std::max(1, classes)), run_offsets_{0, samples} {
+KnnBatch::KnnBatch(int device, const int* neighbors, const float* scores, const IVec& run_offsets, int k,
+                   int classes)
+    : stream_(device), runs_(static_cast<int>(run_offsets.size()) - 1), k_(k), classes_(std::max(1, classes)),
+      run_offsets_(run_offsets) {
   if (k < 1 || k > kMaxK) throw InvalidArgument("neighbours per row must be in 1..64.");
+  if (runs_ < 1) throw InvalidArgument("at least one run is required.");
   allocate(run_offsets_);
-  // Transpose into the padded vote layout; ids become padded row ids (run 0
-  // starts at 0, so local ids are kept).
+  // Transpose into the padded vote layout with padded row ids, filled slots
+  // first so that empty rows are detectable from column 0.
   std::vector<int> tn(static_cast<size_t>(k) * pad_total_, -1);
   std::vector<float> ts(static_cast<size_t>(k) * pad_total_, -INFINITY);
-  for (int i = 0; i < samples; ++i) {
-    for (int j = 0; j < k; ++j) {
-      const int nb = neighbors[static_cast<size_t>(i) * k + j];
-      tn[static_cast<size_t>(j) * pad_total_ + i] = nb >= 0 && nb < samples ? nb : -1;
-      ts[static_cast<size_t>(j) * pad_total_ + i] = scores[static_cast<size_t>(i) * k + j];
-    }
-  }
-  // Keep filled slots first so empty rows are detectable from column 0.
-  for (int i = 0; i < samples; ++i) {
-    int w = 0;
-    for (int j = 0; j < k; ++j) {
-      const size_t o = static_cast<size_t>(j) * pad_total_ + i;
-      if (tn[o] >= 0) {
-        const size_t d = static_cast<size_t>(w) * pad_total_ + i;
-        std::swap(tn[d], tn[o]);
-        std::swap(ts[d], ts[o]);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 0; r < runs_; ++r) {
+    const int L = run_offsets_[r + 1] - run_offsets_[r];
+    for (int i = 0; i < L; ++i) {
+      const size_t src = static_cast<size_t>(run_offsets_[r] + i) * k;
+      const size_t row = static_cast<size_t>(pad_offsets_[r] + i);
+      int w = 0;
+      for (int j = 0; j < k; ++j) {
+        const int nb = neighbors[src + j];
+        if (nb < 0 || nb >= L) continue;
+        tn[static_cast<size_t>(w) * pad_total_ + row] = pad_offsets_[r] + nb;
+        ts[static_cast<size_t>(w) * pad_total_ + row] = scores[src + j];
         ++w;
       }
     }

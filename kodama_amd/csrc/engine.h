@@ -122,8 +122,10 @@ class KnnBatch {
   // table_path: how the per-fold tables are built (device_fold_tables).
   KnnBatch(int device, const float* x, int n, int p, const IVec& run_offsets, const IVec& rows,
            const IVec& folds, int k, int metric, int classes, int table_path = 0);
-  // One run from caller tables [samples x k] (local ids, -1 / -inf padded).
-  KnnBatch(int device, const int* neighbors, const float* scores, int samples, int k, int classes);
+  // Runs from caller tables [run_offsets.back() x k] row-major: run r owns
+  // rows [run_offsets[r], run_offsets[r+1]) with run-local neighbour ids
+  // (-1 / -inf padded).
+  KnnBatch(int device, const int* neighbors, const float* scores, const IVec& run_offsets, int k, int classes);
 
   int runs() const { return runs_; }
   int k() const { return k_; }

@@ -5,6 +5,7 @@
 #include "matrix.h"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <iterator>
 #include <limits>
@@ -302,6 +303,56 @@ IVec quota_sample_landmarks(const IVec& offsets, const IVec& rows, int target, s
   }
   if (static_cast<int>(out.size()) != target) throw DeviceError("landmark quota allocation gave the wrong size.");
   std::sort(out.begin(), out.end());
+  return out;
+}
+
+IVec normalize_constrain(const int32_t* constrain, int n) {
+  IVec out(static_cast<size_t>(n));
+  if (constrain == nullptr) {
+    std::iota(out.begin(), out.end(), 1);
+    return out;
+  }
+  std::map<int, int> ids;
+  for (int i = 0; i < n; ++i) out[i] = ids.emplace(constrain[i], static_cast<int>(ids.size()) + 1).first->second;
+  return out;
+}
+
+bool is_identity_constrain(const IVec& groups) {
+  for (size_t i = 0; i < groups.size(); ++i) {
+    if (groups[i] != static_cast<int>(i + 1)) return false;
+  }
+  return true;
+}
+
+IVec factor_subset(const IVec& values, const IVec& rows) {
+  std::map<int, int> ids;
+  IVec out(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) {
+    out[i] = ids.emplace(values[rows[i]], static_cast<int>(ids.size()) + 1).first->second;
+  }
+  return out;
+}
+
+int majority_label(const IVec& labels) {
+  std::map<int, int> counts;
+  for (int v : labels) ++counts[v];
+  int best = counts.begin()->first, best_n = counts.begin()->second;
+  for (const auto& kv : counts) {
+    if (kv.second > best_n) {  // ascending keys: a tie keeps the smaller label
+      best = kv.first;
+      best_n = kv.second;
+    }
+  }
+  return best;
+}
+
+IVec constrained_majority(const IVec& labels, const IVec& groups) {
+  std::map<int, IVec> members;
+  for (size_t i = 0; i < labels.size(); ++i) members[groups[i]].push_back(labels[i]);
+  std::map<int, int> label_of;
+  for (const auto& kv : members) label_of[kv.first] = majority_label(kv.second);
+  IVec out(labels.size());
+  for (size_t i = 0; i < labels.size(); ++i) out[i] = label_of[groups[i]];
   return out;
 }
 

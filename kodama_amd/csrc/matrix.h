@@ -35,4 +35,15 @@ void index_strata(const IVec& labels, int strata, IVec& offsets, IVec& rows);
 // (kodama_matrix.cpp:856-944).
 IVec quota_sample_landmarks(const IVec& offsets, const IVec& rows, int target, std::mt19937_64& rng);
 
+// Constraint groups of KODAMA.matrix (kodama_matrix.cpp:169-216, 988-1011):
+// normalize_constrain renumbers the caller's values 1.. by first occurrence
+// (none: every row its own group); is_identity_constrain; factor_subset
+// renumbers the groups of `rows` 1.. by first occurrence; constrained_majority
+// gives every member of a group the group's majority label (ties: smaller).
+IVec normalize_constrain(const int32_t* constrain, int n);
+bool is_identity_constrain(const IVec& groups);
+IVec factor_subset(const IVec& values, const IVec& rows);
+int majority_label(const IVec& labels);
+IVec constrained_majority(const IVec& labels, const IVec& groups);
+
 }  // namespace kodama_hip

@@ -1809,19 +1809,146 @@ void normalize_graph(const int32_t* gi, const float* gd, int n, int given, int i
   }
 }
 
-// KODAMA.matrix, kNN classifier, no spatial data, no constraints or starting
-// labels (kodama_matrix.cpp:3324-4051 with run_iteration 2870-3322): the
-// shared k-means atlas (mt19937_64(seed + 1)), per run r = 1..M an
-// mt19937_64(seed + r) that skips one draw, quota-samples the landmarks and
-// draws the k-means starting labels; CoreKNN on the landmarks with per-fold
-// exact neighbours; each other row takes the majority label of the landmarks
-// among its first graph neighbours (exact global graph, self excluded), at
-// most k votes, clbest[0] when none.
+// normalize_constrain / normalize_fixed / is_identity_constrain
+// (kodama_matrix.cpp:183-216): constraint values renumbered 1.. by first
+// occurrence (none: every row its own group), fixed flags as 0/1.
+IVec normalize_constrain(const int32_t* c, int n) {
+  IVec out(static_cast<size_t>(n));
+  if (c == nullptr) {
+    std::iota(out.begin(), out.end(), 1);
+    return out;
+  }
+  std::map<int, int> ids;
+  for (int i = 0; i < n; ++i) {
+    auto it = ids.find(c[i]);
+    if (it == ids.end()) it = ids.emplace(c[i], static_cast<int>(ids.size()) + 1).first;
+    out[i] = it->second;
+  }
+  return out;
+}
+
+bool is_identity(const IVec& c) {
+  for (size_t i = 0; i < c.size(); ++i) if (c[i] != static_cast<int>(i + 1)) return false;
+  return true;
+}
+
+// factor_subset (kodama_matrix.cpp:988-1000): values of `rows` renumbered 1..
+// by first occurrence.
+IVec factor_subset(const IVec& values, const IVec& rows) {
+  std::map<int, int> ids;
+  IVec out(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const int v = values[rows[i]];
+    auto it = ids.find(v);
+    if (it == ids.end()) it = ids.emplace(v, static_cast<int>(ids.size()) + 1).first;
+    out[i] = it->second;
+  }
+  return out;
+}
+
+// constrained_majority (kodama_matrix.cpp:1003-1011): every member of a
+// constraint group takes the group's majority label (ties: smaller label).
+IVec constrained_majority(const IVec& labels, const IVec& groups) {
+  std::map<int, IVec> by;
+  for (size_t i = 0; i < labels.size(); ++i) by[groups[i]].push_back(labels[i]);
+  std::map<int, int> lab;
+  for (const auto& kv : by) lab[kv.first] = majority_label(kv.second);
+  IVec out(labels.size());
+  for (size_t i = 0; i < labels.size(); ++i) out[i] = lab[groups[i]];
+  return out;
+}
+
+// PLSLDAPredict_CPU (plscv.cpp:2976-3005): one fit on every training row (no
+// Gram loading path), validation scores of the test rows; a fit without a
+// component is the reference's DegeneratePLSFit.
+template <class T>
+IVec plslda_predict(const float* train, int nt, const IVec& labels, const float* test, int nq, int p,
+                    const oracle_core_options* o) {
+  if (nt == 0 || nq == 0 || p == 0) return IVec();
+  const IVec classes = sorted_unique(labels);
+  if (classes.size() == 1) return IVec(static_cast<size_t>(nq), classes[0]);
+  std::vector<float> x(static_cast<size_t>(nt + nq) * p);
+  std::copy_n(train, static_cast<size_t>(nt) * p, x.begin());
+  std::copy_n(test, static_cast<size_t>(nq) * p, x.begin() + static_cast<size_t>(nt) * p);
+  IVec lab(static_cast<size_t>(nt + nq), labels[0]), tr(static_cast<size_t>(nt)), te(static_cast<size_t>(nq));
+  std::copy(labels.begin(), labels.end(), lab.begin());
+  std::iota(tr.begin(), tr.end(), 0);
+  std::iota(te.begin(), te.end(), nt);
+  const FoldOut<T> fo = pls_fold<T>(x.data(), p, lab, tr, te, o, false);
+  if (fo.weights.c == 0) throw std::runtime_error("PLSLDAPredict: degenerate PLS fit");
+  return fo.pred;
+}
+
+// CoreKNNGraph's tables (core.cpp:624-692) of a run's landmark subgraph
+// (subset_graph_to_rows, kodama_matrix.cpp:1332-1366): per landmark row the
+// global neighbours that are landmarks, as run-local ids in graph order; the
+// first k of them outside the row's fold vote, score -d^2 (Euclidean) or 1 - d.
+KnnTables subgraph_tables(const int32_t* gi, const float* gd, int n, int width, const IVec& land,
+                          const IVec& folds, int k_opt, int metric) {
+  const int L = static_cast<int>(land.size());
+  IVec local(static_cast<size_t>(n), -1);
+  for (int i = 0; i < L; ++i) local[land[i]] = i;
+  KnnTables t;
+  t.folds = folds;
+  t.width = std::max(1, std::min(k_opt, width));
+  t.nbr.assign(static_cast<size_t>(L) * t.width, -1);
+  t.score.assign(t.nbr.size(), -std::numeric_limits<float>::infinity());
+  for (int i = 0; i < L; ++i) {
+    int col = 0;
+    for (int j = 0; j < width && col < t.width; ++j) {
+      const size_t src = static_cast<size_t>(land[i]) * width + j;
+      const int g = gi[src];
+      if (g < 0 || g >= n) continue;
+      const int nb = local[g];
+      if (nb < 0 || nb == i || folds[nb] == folds[i]) continue;
+      const float d0 = gd[src];
+      if (!(std::isfinite(d0) && d0 >= 0.0f)) continue;
+      t.nbr[static_cast<size_t>(i) * t.width + col] = nb;
+      t.score[static_cast<size_t>(i) * t.width + col] = metric == 2 ? -(d0 * d0) : 1.0f - d0;
+      ++col;
+    }
+  }
+  return t;
+}
+
+// FNV-1a over the little-endian bytes of 32-bit values (the hash of the
+// reference's run diagnostics, kodama_matrix.cpp:1689-1699).
+uint64_t fnv1a(const IVec& v) {
+  uint64_t h = 1469598103934665603ull;
+  for (int x : v) {
+    const uint32_t b = static_cast<uint32_t>(x);
+    for (int s = 0; s < 32; s += 8) {
+      h ^= static_cast<uint8_t>((b >> s) & 0xffu);
+      h *= 1099511628211ull;
+    }
+  }
+  return h;
+}
+
+// KODAMA.matrix without spatial data (kodama_matrix.cpp:3324-4051 with
+// run_iteration 2870-3322), kNN or PLS-LDA classifier: the shared k-means atlas
+// (mt19937_64(seed + 1)); per run r = 1..M an mt19937_64(seed + r) that skips
+// one draw, quota-samples the landmarks and, without starting labels, draws the
+// k-means starting labels (with them: the landmarks' starting labels, and the
+// search is greedy); constraint groups take their majority label before the
+// Core and after the projection. Core: kNN with per-fold exact neighbours, or
+// with the caller's graph subset to the landmarks (CoreKNNGraph); PLS-LDA with
+// `components` fixed components. Projection: kNN majority of the first k
+// landmarks among a row's graph neighbours (clbest[0] when none), or
+// PLSLDAPredict. diag (may be NULL) [runs x 3]: FNV-1a hashes of the landmark
+// rows, the starting labels and the fold ids, as the reference's run
+// diagnostics.
 int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, const int32_t* ext_idx,
-                const float* ext_dist, int32_t ext_width, int32_t ext_base, int32_t* res, double* acc, double* v,
-                int32_t* landmark_rows, int32_t* landmarks_used) {
+                const float* ext_dist, int32_t ext_width, int32_t ext_base, const int32_t* starting_labels,
+                const int32_t* constrain_in, const int32_t* fixed_in, int32_t* res, double* acc, double* v,
+                int32_t* landmark_rows, int32_t* landmarks_used, uint64_t* diag) {
   return guarded([&] {
     if (n < 3) throw std::invalid_argument("KODAMAMatrix requires at least 3 rows.");
+    if (o->runs < 1) throw std::invalid_argument("KODAMAMatrixOptions::runs must be positive.");
+    if (o->cycles < 0) throw std::invalid_argument("KODAMAMatrixOptions::cycles must be non-negative.");
+    if (o->folds < 2) throw std::invalid_argument("KODAMAMatrixOptions::folds must be at least two.");
+    if (o->classifier != 0 && o->classifier != 1) throw std::invalid_argument("Unsupported KODAMA.matrix classifier.");
+    const bool knn = o->classifier == 1;
     int L = o->landmarks > 0 ? o->landmarks : 10000;
     if (L >= n) L = static_cast<int>(std::ceil(static_cast<double>(n) * 0.75));
     L = std::max(2, std::min(L, n - 1));
@@ -1831,16 +1958,25 @@ int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_option
     const int resolved = std::max(1, static_cast<int>(std::floor(std::min(
       {static_cast<double>(L), static_cast<double>(n) * 0.75 - 1.0, static_cast<double>(requested)}))));
     const int neighbors = ext_idx != nullptr ? std::max(1, std::min(ext_width, resolved)) : resolved;
+    const int comps = std::max(1, std::min(o->components, std::min(n, p)));
+    const IVec cons_all = normalize_constrain(constrain_in, n);
+    const bool identity = is_identity(cons_all);
+    IVec fixed_all(static_cast<size_t>(n), 0);
+    if (fixed_in) for (int i = 0; i < n; ++i) fixed_all[i] = fixed_in[i] != 0 ? 1 : 0;
     std::mt19937_64 shared(o->seed + 1u);
     const IVec atlas = kmeans_labels(x, n, p, coarse_k, shared, 10);
     IVec off, rows;
     index_strata(atlas, coarse_k, off, rows);
-    const std::vector<float> xs = prepare_rows(x, n, p, o->metric, 1);
-    std::vector<int32_t> gi(static_cast<size_t>(n) * neighbors);
-    std::vector<float> gd(gi.size());
+    // The global graph serves the kNN classifier only (the caller's graph in
+    // KODAMAMatrixFromGraphData, else the exact self-kNN graph).
+    std::vector<int32_t> gi;
+    std::vector<float> gd;
     if (ext_idx != nullptr) {
       normalize_graph(ext_idx, ext_dist, n, ext_width, ext_base, neighbors, gi, gd);
-    } else {
+    } else if (knn) {
+      const std::vector<float> xs = prepare_rows(x, n, p, o->metric, 1);
+      gi.resize(static_cast<size_t>(n) * neighbors);
+      gd.resize(gi.size());
       IVec self(static_cast<size_t>(n));
       std::iota(self.begin(), self.end(), 0);
       exact_knn(xs.data(), n, xs.data(), n, p, neighbors, o->metric, self.data(), gi.data(), gd.data());
@@ -1852,17 +1988,26 @@ int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_option
       const IVec land = quota_sample(off, rows, L, rng);
       std::vector<float> xl(static_cast<size_t>(L) * p);
       for (int i = 0; i < L; ++i) std::copy_n(x + static_cast<size_t>(land[i]) * p, p, xl.data() + static_cast<size_t>(i) * p);
-      const int init_k = std::max(2, std::min(split, L));
-      const IVec xw = kmeans_labels(xl.data(), L, p, init_k, rng, 10);
       IVec cons(static_cast<size_t>(L));
-      std::iota(cons.begin(), cons.end(), 1);
+      if (identity) std::iota(cons.begin(), cons.end(), 1);
+      else cons = factor_subset(cons_all, land);
+      IVec fx(static_cast<size_t>(L));
+      for (int i = 0; i < L; ++i) fx[i] = fixed_all[land[i]];
+      IVec xw(static_cast<size_t>(L));
+      if (starting_labels != nullptr) {
+        for (int i = 0; i < L; ++i) xw[i] = starting_labels[land[i]];
+      } else {
+        xw = kmeans_labels(xl.data(), L, p, std::max(2, std::min(split, L)), rng, 10);
+      }
+      if (!identity) xw = constrained_majority(xw, cons);
       oracle_core_options co{};
       co.cycles = o->cycles;
-      co.classifier = 1;
+      co.classifier = o->classifier;
       co.seed = o->seed + static_cast<std::uint64_t>(r);
-      co.evolutionary_search = 1;
+      co.evolutionary_search = starting_labels == nullptr ? 1 : 0;
       co.guarded_diversity = 1;
       co.many_to_one_absorption = 1;
+      co.auto_class_coarsening = knn ? 0 : 1;
       co.adaptive_proposal_size = 1;
       co.prediction_guidance = co.policy_adaptive_proposal_size = co.transition_proposal = 1;
       co.stochastic_acceptance = co.diversity_multiplier = co.pls_transition_coarsening = 1;
@@ -1872,10 +2017,14 @@ int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_option
       co.cv_seed = o->seed + static_cast<std::uint64_t>(r);
       co.k = std::max(1, o->k);
       co.metric = o->metric;
+      co.max_components = comps;
+      co.fixed_components = comps;
+      co.center = 1;
+      co.scale = 1;
+      co.pls_fold_cache = 1;
       const IVec folds = make_folds(xw, cons, co.folds, false, co.cv_seed);
-      const KnnTables t = precompute_tables(xl.data(), L, p, folds, co.k, co.metric);
       Flags f = flags_of(&co);
-      f.knn = true;
+      f.knn = knn;
       const int T = std::max(0, o->cycles);
       IVec clbest(static_cast<size_t>(L)), dirty(static_cast<size_t>(L)), cvp(static_cast<size_t>(L)),
         psize(static_cast<size_t>(T) + 1), active(static_cast<size_t>(T) + 1);
@@ -1894,26 +2043,58 @@ int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_option
       cr.improving_acceptance = a2.data();
       cr.temperature_acceptance = a3.data();
       cr.counters = counters;
-      evolve(L, xw, cons, IVec(), f, [&](const IVec& l) { return vote_predict(t, l); }, &cr);
+      if (knn) {
+        const KnnTables t = ext_idx != nullptr
+          ? subgraph_tables(gi.data(), gd.data(), n, neighbors, land, folds, co.k, co.metric)
+          : precompute_tables(xl.data(), L, p, folds, co.k, co.metric);
+        evolve(L, xw, cons, fx, f, [&](const IVec& l) { return vote_predict(t, l); }, &cr);
+      } else {
+        const bool f32 = o->precision == 32;
+        evolve(L, xw, cons, fx, f, [&](const IVec& l) {
+          return f32 ? plsldacv<float>(xl.data(), L, p, l, cons, &co, &folds, nullptr, nullptr)
+                     : plsldacv<double>(xl.data(), L, p, l, cons, &co, &folds, nullptr, nullptr);
+        }, &cr);
+      }
       int32_t* out = res + static_cast<size_t>(r - 1) * n;
       IVec local(static_cast<size_t>(n), -1);
       for (int i = 0; i < L; ++i) local[land[i]] = i;
+      IVec tpoints;
       for (int i = 0; i < n; ++i) {
-        if (local[i] >= 0) {
-          out[i] = clbest[local[i]];
-          continue;
+        if (local[i] >= 0) out[i] = clbest[local[i]];
+        else tpoints.push_back(i);
+      }
+      if (knn) {
+        for (int i : tpoints) {
+          IVec votes;
+          for (int j = 0; j < neighbors && static_cast<int>(votes.size()) < co.k; ++j) {
+            const int g = gi[static_cast<size_t>(i) * neighbors + j];
+            if (g < 0 || g >= n) continue;
+            if (local[g] >= 0) votes.push_back(clbest[local[g]]);
+          }
+          out[i] = votes.empty() ? clbest[0] : majority_label(votes);
         }
-        IVec votes;
-        for (int j = 0; j < neighbors && static_cast<int>(votes.size()) < co.k; ++j) {
-          const int g = gi[static_cast<size_t>(i) * neighbors + j];
-          if (g < 0 || g >= n) continue;
-          if (local[g] >= 0) votes.push_back(clbest[local[g]]);
-        }
-        out[i] = votes.empty() ? clbest[0] : majority_label(votes);
+      } else if (!tpoints.empty()) {
+        std::vector<float> xt(tpoints.size() * static_cast<size_t>(p));
+        for (size_t i = 0; i < tpoints.size(); ++i)
+          std::copy_n(x + static_cast<size_t>(tpoints[i]) * p, p, xt.data() + i * p);
+        const int nt = static_cast<int>(tpoints.size());
+        const IVec pr = o->precision == 32 ? plslda_predict<float>(xl.data(), L, clbest, xt.data(), nt, p, &co)
+                                           : plslda_predict<double>(xl.data(), L, clbest, xt.data(), nt, p, &co);
+        for (int i = 0; i < nt; ++i) out[tpoints[i]] = pr[i];
+      }
+      if (!identity) {
+        const IVec all(out, out + n);
+        const IVec cm = constrained_majority(all, cons_all);
+        std::copy(cm.begin(), cm.end(), out);
       }
       acc[r - 1] = cr.accbest;
       for (int c = 0; c < T; ++c) v[static_cast<size_t>(r - 1) * T + c] = vacc[c];
       if (landmark_rows) std::memcpy(landmark_rows + static_cast<size_t>(r - 1) * L, land.data(), sizeof(int32_t) * L);
+      if (diag) {
+        diag[static_cast<size_t>(r - 1) * 3 + 0] = fnv1a(land);
+        diag[static_cast<size_t>(r - 1) * 3 + 1] = fnv1a(xw);
+        diag[static_cast<size_t>(r - 1) * 3 + 2] = fnv1a(folds);
+      }
     }
   });
 }
@@ -1922,23 +2103,27 @@ int matrix_impl(const float* x, int32_t n, int32_t p, const oracle_matrix_option
 
 extern "C" {
 
-int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o, int32_t* res,
-                         double* acc, double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
-  return matrix_impl(x, n, p, o, nullptr, nullptr, 0, -1, res, acc, v, landmark_rows, landmarks_used);
+int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o,
+                         const int32_t* starting_labels, const int32_t* constrain, const int32_t* fixed,
+                         int32_t* res, double* acc, double* v, int32_t* landmark_rows, int32_t* landmarks_used,
+                         uint64_t* diag) {
+  return matrix_impl(x, n, p, o, nullptr, nullptr, 0, -1, starting_labels, constrain, fixed, res, acc, v,
+                     landmark_rows, landmarks_used, diag);
 }
 
-// KODAMAMatrixFromGraphData (kodama_matrix.cpp:4676-4690, 3552-3556): the
+// KODAMAMatrixFromGraphData (kodama_matrix.cpp:4656-4670, 3526-3556): the
 // caller's graph, normalised to min(its width, the resolved graph width),
-// replaces the exact global graph.
+// replaces the exact global graph, in the Core step as in the projection.
 int oracle_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o,
                                     const int32_t* graph_indices, const float* graph_distances,
-                                    int32_t graph_neighbors, int32_t index_base, int32_t* res, double* acc,
-                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used) {
+                                    int32_t graph_neighbors, int32_t index_base, const int32_t* starting_labels,
+                                    const int32_t* constrain, const int32_t* fixed, int32_t* res, double* acc,
+                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used, uint64_t* diag) {
   if (graph_indices == nullptr || graph_distances == nullptr || graph_neighbors <= 0) {
     return guarded([] { throw std::invalid_argument("null or empty graph."); });
   }
-  return matrix_impl(x, n, p, o, graph_indices, graph_distances, graph_neighbors, index_base, res, acc, v,
-                     landmark_rows, landmarks_used);
+  return matrix_impl(x, n, p, o, graph_indices, graph_distances, graph_neighbors, index_base, starting_labels,
+                     constrain, fixed, res, acc, v, landmark_rows, landmarks_used, diag);
 }
 
 int oracle_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,

@@ -134,7 +134,7 @@ int oracle_kmeans_labels(const float* x, int32_t n, int32_t p, int32_t k, uint64
 int oracle_quota_sample(const int32_t* strata_labels, int32_t n, int32_t strata, int32_t target,
                         uint64_t rng_seed, int32_t discard, int32_t* rows_out, int32_t* count_out);
 
-/* KODAMAMatrixOptions subset of the kNN pipeline (kodama.hpp:532-566). */
+/* KODAMAMatrixOptions subset without spatial data (kodama.hpp:532-566). */
 typedef struct oracle_matrix_options {
   int32_t runs;
   int32_t cycles;
@@ -145,20 +145,35 @@ typedef struct oracle_matrix_options {
   int32_t metric;
   int32_t graph_neighbors;
   uint64_t seed;
+  int32_t classifier;   /* 1 = KNN, 0 = PLS_LDA (CoreClassifier order) */
+  int32_t components;   /* PLS-LDA components (KODAMAMatrixOptions::components) */
+  int32_t precision;    /* PLS-LDA arithmetic: 32 (the reference's float) or 64 */
 } oracle_matrix_options;
 
 /* res [runs x n], acc [runs], v [runs x cycles], landmark_rows [runs x L] (may
- * be NULL), *landmarks_used = L. */
+ * be NULL), *landmarks_used = L; starting_labels / constrain / fixed [n] (each
+ * may be NULL: the reference's empty vectors); diag [runs x 3] (may be NULL):
+ * FNV-1a hashes of each run's landmark rows, starting labels and fold ids. */
 int oracle_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* options,
+                         const int32_t* starting_labels, const int32_t* constrain, const int32_t* fixed,
                          int32_t* res, double* acc, double* v, int32_t* landmark_rows,
-                         int32_t* landmarks_used);
+                         int32_t* landmarks_used, uint64_t* diag);
 
 /* The same with the caller's graph [n x graph_neighbors] (index_base 0, 1 or
- * -1 = auto) in place of the exact global graph. */
+ * -1 = auto) in place of the exact global graph (KODAMAMatrixFromGraphData). */
 int oracle_kodama_matrix_from_graph(const float* x, int32_t n, int32_t p, const oracle_matrix_options* options,
                                     const int32_t* graph_indices, const float* graph_distances,
-                                    int32_t graph_neighbors, int32_t index_base, int32_t* res, double* acc,
-                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used);
+                                    int32_t graph_neighbors, int32_t index_base, const int32_t* starting_labels,
+                                    const int32_t* constrain, const int32_t* fixed, int32_t* res, double* acc,
+                                    double* v, int32_t* landmark_rows, int32_t* landmarks_used, uint64_t* diag);
+
+/* The reference's own KODAMAMatrix_CPU / KODAMAMatrixFromGraphData_CPU
+ * (oracle/_ref, one thread, no visual initialisation, no dissimilarity):
+ * res, acc, v and diag as above; graph_indices NULL for KODAMAMatrix_CPU. */
+int ref_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* options,
+                      const int32_t* graph_indices, const float* graph_distances, int32_t graph_neighbors,
+                      int32_t index_base, const int32_t* starting_labels, const int32_t* constrain,
+                      const int32_t* fixed, int32_t* res, double* acc, double* v, uint64_t* diag);
 
 /* normalize_external_graph: outputs [n x width], 0-based, -1 / +inf padded. */
 int oracle_normalize_graph(const int32_t* graph_indices, const float* graph_distances, int32_t n,

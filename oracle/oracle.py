@@ -196,8 +196,22 @@ class MatrixOptions(C.Structure):
     _fields_ = [
         ("runs", C.c_int32), ("cycles", C.c_int32), ("landmarks", C.c_int32), ("splitting", C.c_int32),
         ("folds", C.c_int32), ("k", C.c_int32), ("metric", C.c_int32), ("graph_neighbors", C.c_int32),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("classifier", C.c_int32), ("components", C.c_int32), ("precision", C.c_int32),
     ]
+
+
+def matrix_options(runs, cycles, landmarks, splitting=0, folds=5, k=30, metric=2, graph_neighbors=100,
+                   seed=1234, classifier="knn", components=50, precision=64) -> MatrixOptions:
+    """KODAMAMatrixOptions defaults (kodama.hpp:532-566); classifier "knn" or
+    "pls_lda"."""
+    return MatrixOptions(runs=runs, cycles=cycles, landmarks=landmarks, splitting=splitting, folds=folds, k=k,
+                         metric=metric, graph_neighbors=graph_neighbors, seed=seed,
+                         classifier=1 if classifier == "knn" else 0, components=components, precision=precision)
+
+
+def _resolved_landmarks(n, landmarks):
+    L = landmarks if 0 < landmarks < n else (int(np.ceil(0.75 * n)) if landmarks >= n else 10000)
+    return max(2, min(L, n - 1))
 
 
 class _MatrixBlocks:
@@ -356,32 +370,38 @@ class Oracle(_Api, _MatrixBlocks):
 
 
 def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5, k=30, metric=2,
-                          graph_neighbors=100, seed=1234, graph=None, index_base=-1):
-    """KODAMA.matrix; with graph = (indices, distances), KODAMAMatrixFromGraphData."""
+                          graph_neighbors=100, seed=1234, graph=None, index_base=-1, classifier="knn",
+                          components=50, precision=64, W=None, constrain=None, fixed=None):
+    """KODAMA.matrix; with graph = (indices, distances), KODAMAMatrixFromGraphData.
+    W / constrain / fixed: the reference's starting_labels, constrain and fixed
+    (one per row, None for empty). diag [runs x 3]: FNV-1a hashes of each run's
+    landmark rows, starting labels and fold ids."""
     x = _f32(x)
     n = x.shape[0]
-    o = MatrixOptions(runs=runs, cycles=cycles, landmarks=landmarks, splitting=splitting, folds=folds, k=k,
-                      metric=metric, graph_neighbors=graph_neighbors, seed=seed)
-    L = landmarks if 0 < landmarks < n else (int(np.ceil(0.75 * n)) if landmarks >= n else 10000)
-    L = max(2, min(L, n - 1))
+    o = matrix_options(runs, cycles, landmarks, splitting, folds, k, metric, graph_neighbors, seed, classifier,
+                       components, precision)
+    L = _resolved_landmarks(n, landmarks)
     res = np.zeros((runs, n), np.int32)
     acc = np.zeros(runs, np.float64)
     v = np.zeros((runs, max(cycles, 1)), np.float64)
     land = np.zeros((runs, L), np.int32)
+    diag = np.zeros((runs, 3), np.uint64)
     used = C.c_int32(0)
+    extra = (_ptr_or_null(_i32(W), C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32),
+             _ptr_or_null(_i32(fixed), C.c_int32))
+    outs = (_p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(land, C.c_int32), C.byref(used),
+            _p(diag, C.c_uint64))
     if graph is None:
         self._l.call("kodama_matrix", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o),
-                     _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(land, C.c_int32),
-                     C.byref(used))
+                     *extra, *outs)
     else:
         gi, gd = _i32(graph[0]), _f32(graph[1])
         assert gi.shape == gd.shape and gi.shape[0] == n
         self._l.call("kodama_matrix_from_graph", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]),
                      C.byref(o), _p(gi, C.c_int32), _p(gd, C.c_float), C.c_int32(gi.shape[1]),
-                     C.c_int32(index_base), _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double),
-                     _p(land, C.c_int32), C.byref(used))
+                     C.c_int32(index_base), *extra, *outs)
     assert used.value == L
-    return dict(res=res, acc=acc, v=v[:, :cycles].copy(), landmarks=land)
+    return dict(res=res, acc=acc, v=v[:, :cycles].copy(), landmarks=land, diag=diag)
 
 
 Oracle.kodama_matrix = _oracle_kodama_matrix
@@ -492,6 +512,32 @@ class Reference(_Api):
         self._l.call("dissimilarity", _p(idx, C.c_int32), _p(dist, C.c_float), C.c_int32(idx.shape[0]),
                      C.c_int32(idx.shape[1]), _p(res, C.c_int32), C.c_int32(res.shape[0]))
         return idx, dist
+
+    def kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5, k=30, metric=2,
+                      graph_neighbors=100, seed=1234, graph=None, index_base=-1, classifier="knn",
+                      components=50, W=None, constrain=None, fixed=None):
+        """kodama::KODAMAMatrix_CPU, or KODAMAMatrixFromGraphData_CPU with graph =
+        (indices, distances); one thread. res, acc, v and the run diagnostics'
+        hashes (diag [runs x 3])."""
+        x = _f32(x)
+        n = x.shape[0]
+        o = matrix_options(runs, cycles, landmarks, splitting, folds, k, metric, graph_neighbors, seed,
+                           classifier, components)
+        res = np.zeros((runs, n), np.int32)
+        acc = np.zeros(runs, np.float64)
+        v = np.zeros((runs, max(cycles, 1)), np.float64)
+        diag = np.zeros((runs, 3), np.uint64)
+        if graph is None:
+            gi, gd, width = None, None, 0
+        else:
+            gi, gd = _i32(graph[0]), _f32(graph[1])
+            width = gi.shape[1]
+        self._l.call("kodama_matrix", _p(x, C.c_float), C.c_int32(n), C.c_int32(x.shape[1]), C.byref(o),
+                     _ptr_or_null(gi, C.c_int32), _ptr_or_null(gd, C.c_float), C.c_int32(width),
+                     C.c_int32(index_base), _ptr_or_null(_i32(W), C.c_int32),
+                     _ptr_or_null(_i32(constrain), C.c_int32), _ptr_or_null(_i32(fixed), C.c_int32),
+                     _p(res, C.c_int32), _p(acc, C.c_double), _p(v, C.c_double), _p(diag, C.c_uint64))
+        return dict(res=res, acc=acc, v=v[:, :cycles].copy(), diag=diag)
 
     def knn_graph(self, x, k, metric=2, n_threads=1):
         x = _f32(x)

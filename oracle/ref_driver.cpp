@@ -290,4 +290,59 @@ int ref_knn_graph(const float* x, int32_t n, int32_t p, int32_t k, int32_t metri
   });
 }
 
+// kodama::KODAMAMatrix_CPU / KODAMAMatrixFromGraphData_CPU
+// (kodama_matrix.cpp:4465-4475, 4656-4670): one thread, no PCA visual
+// initialisation, no graph dissimilarity; the run diagnostics' hashes of the
+// landmark rows, starting labels and fold ids go to diag [runs x 3].
+int ref_kodama_matrix(const float* x, int32_t n, int32_t p, const oracle_matrix_options* o,
+                      const int32_t* graph_indices, const float* graph_distances, int32_t graph_neighbors,
+                      int32_t index_base, const int32_t* starting_labels, const int32_t* constrain,
+                      const int32_t* fixed, int32_t* res, double* acc, double* v, uint64_t* diag) {
+  return guarded([&] {
+    const kodama::MatrixView xv(x, static_cast<size_t>(n), static_cast<size_t>(p));
+    kodama::KODAMAMatrixOptions m;
+    m.runs = o->runs;
+    m.cycles = o->cycles;
+    m.components = o->components;
+    m.landmarks = o->landmarks;
+    m.splitting = o->splitting;
+    m.graph_neighbors = o->graph_neighbors;
+    m.folds = o->folds;
+    m.n_threads = 1;
+    m.seed = o->seed;
+    m.metric = metric_of(o->metric);
+    m.backend = kodama::Backend::CPU;
+    m.classifier = o->classifier == 1 ? kodama::CoreClassifier::KNN : kodama::CoreClassifier::PLS_LDA;
+    m.apply_kodama_dissimilarity = false;
+    m.compute_visual_init = false;
+    m.materialize_graph = false;
+    m.knn.k = o->k;
+    m.knn.n_threads = 1;
+    const std::vector<int> w = vec_or_empty(starting_labels, n), c = vec_or_empty(constrain, n),
+                           f = vec_or_empty(fixed, n);
+    kodama::KODAMAMatrixResult r;
+    if (graph_indices != nullptr) {
+      kodama::NeighborGraph g;
+      g.neighbors = graph_neighbors;
+      g.index_base = index_base == 0 ? kodama::GraphIndexBase::Zero
+                   : index_base == 1 ? kodama::GraphIndexBase::One : kodama::GraphIndexBase::Auto;
+      g.indices.assign(graph_indices, graph_indices + static_cast<size_t>(n) * graph_neighbors);
+      g.distances.assign(graph_distances, graph_distances + static_cast<size_t>(n) * graph_neighbors);
+      r = kodama::KODAMAMatrixFromGraphData_CPU(xv, g, w, c, f, m);
+    } else {
+      r = kodama::KODAMAMatrix_CPU(xv, w, c, f, m);
+    }
+    std::memcpy(res, r.res.data(), sizeof(int32_t) * r.res.size());
+    std::memcpy(acc, r.acc.data(), sizeof(double) * r.acc.size());
+    if (v) std::memcpy(v, r.v.data(), sizeof(double) * r.v.size());
+    if (diag) {
+      for (int i = 0; i < o->runs; ++i) {
+        diag[static_cast<size_t>(i) * 3 + 0] = r.run_diagnostics[i].landmark_rows_hash;
+        diag[static_cast<size_t>(i) * 3 + 1] = r.run_diagnostics[i].initial_labels_hash;
+        diag[static_cast<size_t>(i) * 3 + 2] = r.run_diagnostics[i].fold_assignments_hash;
+      }
+    }
+  });
+}
+
 }  // extern "C"

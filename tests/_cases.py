@@ -45,3 +45,64 @@ def check_invariants(r: dict, cycles: int, evolutionary: bool) -> None:
         assert r["current_state_accepts"] + r["current_state_rejections"] == 0
     vscore = np.asarray(r["vect_score"])
     assert np.all(np.diff(vscore) >= 0), "best score trace must be monotone"
+
+
+# ---- KODAMA.matrix end-to-end cases pinned against the reference's own
+# KODAMAMatrix_CPU / KODAMAMatrixFromGraphData_CPU (deterministic there: the
+# caller's graph drives the kNN Core, and the PLS-LDA classifier uses no graph).
+
+def matrix_case_inputs(name):
+    """Seeded inputs of a matrix case: (x, graph or None, index_base, W,
+    constrain, fixed). Graphs are the oracle's exact public graph (1-based)."""
+    from kodama_amd import datasets as D
+    import oracle as O
+    spec = MATRIX_CASES[name]
+    if spec.get("data") == "pls":
+        x, _ = D.config2(n=2000)
+        x = x[:900]
+    else:
+        x, _, _ = D.config1(n=spec.get("n", 600), seed=21)
+    graph, base = None, -1
+    if spec.get("graph"):
+        gi, gd = O.Oracle().knn_graph(x, spec["graph"], spec.get("metric", 2))
+        if spec.get("gaps"):
+            gi = gi.copy()
+            gd = gd.copy()
+            gi[::5, 3] = -1
+            gd[1::5, 2] = np.nan
+            gi[2::7, 0] = gi[2::7, 1]  # a repeated neighbour
+        if spec.get("zero_based"):
+            gi = np.where(gi > 0, gi - 1, gi)
+            base = 0
+        graph = (gi, gd)
+    n = x.shape[0]
+    rng = np.random.default_rng(spec.get("seed", 7) + 100)
+    W = rng.integers(1, 6, n).astype(np.int32) if spec.get("W") else None
+    cons = (np.arange(n) // 3 + 1000).astype(np.int32) if spec.get("constrain") else None
+    fixed = (rng.random(n) < 0.1).astype(np.int32) if spec.get("fixed") else None
+    return x, graph, base, W, cons, fixed
+
+
+def matrix_case_options(name):
+    spec = MATRIX_CASES[name]
+    return dict(runs=spec.get("runs", 3), cycles=spec.get("cycles", 8), landmarks=spec.get("landmarks", 300),
+                splitting=spec.get("splitting", 20), folds=spec.get("folds", 5), k=spec.get("k", 10),
+                metric=spec.get("metric", 2), graph_neighbors=spec.get("graph_neighbors", 40),
+                seed=spec.get("seed", 7), classifier=spec.get("classifier", "knn"),
+                components=spec.get("components", 5))
+
+
+MATRIX_CASES = {
+    "graph_euclidean": dict(graph=40),
+    "graph_cosine_zero_based": dict(graph=30, metric=0, zero_based=True, k=12, seed=11, folds=4),
+    "graph_narrow_k_above_width": dict(graph=8, k=12, graph_neighbors=40, seed=5),
+    "graph_gaps": dict(graph=20, gaps=True, seed=13, cycles=10),
+    "graph_starting_labels": dict(graph=40, W=True, seed=17),
+    "graph_constrain_fixed": dict(graph=40, constrain=True, fixed=True, seed=19),
+    "graph_all_inputs": dict(graph=25, W=True, constrain=True, fixed=True, seed=23, metric=1),
+    "pls": dict(classifier="pls_lda", data="pls", landmarks=400, seed=3),
+    "pls_starting_labels_constrain": dict(classifier="pls_lda", data="pls", landmarks=300, W=True,
+                                          constrain=True, fixed=True, seed=29, components=3),
+    "pls_more_components": dict(classifier="pls_lda", data="pls", landmarks=500, components=8, folds=4,
+                                cycles=6, seed=31),
+}
