@@ -21,19 +21,24 @@ Three forms run on the same inputs:
           label differs from their prediction inside the kernel (a quarter of
           the cycles stay uniform). The network is first trained, untimed, on
           the plain form's accept/reject traces over 16 other runs.
+A fourth form, pls, runs the batched CorePLSLDA (host proposals with the
+reference streams, one fused FP64 PLS-LDA evaluation of all runs x folds per
+cycle) on the PLS-LDA workload (X = 10,000 x 50, 100 runs of 2000 landmarks,
+10 folds, 5 components); it is reported apart in `pls_lda_form`.
 `value` is the device form's steps/s; every form's mean final CV accuracy is
 reported ("equal accuracy" = within 0.5 points), with the cycles each form
 needs to reach the plain form's final accuracy minus half a point.
 
 The roofline object covers the dominant kernel of the reported form, timed with
-HIP events on its launch stream:
-  device  evo_persistent_kernel: per cycle and landmark row the vote reads its
-          k 16-bit neighbour ids (2k bytes; labels live in LDS); one launch
-          evaluates the starting labels and then runs every cycle, so
-          algorithmic bytes per launch = (cycles + 1) * sum(L) * 2k;
-  plain   vote_kernel: per row k 32-bit ids, k float scores, the row's label and
-          prediction (8k + 28 bytes with the per-run reductions), so bytes per
-          launch = sum(L) * (8k + 28) over the average launch duration.
+HIP events on its launch stream. Algorithmic bytes follow the per-step cost of
+the vote and accept/commit: per landmark row k neighbour ids and k scores of 4
+bytes plus 7 words of label traffic, L * (8k + 28) bytes per evaluation:
+  device  evo_persistent_kernel: one launch evaluates the starting labels and
+          then runs every cycle, so bytes per launch = (cycles + 1) * sum(L) *
+          (8k + 28); the kernel's own layout reads 2k bytes per row (16-bit
+          ids, labels in LDS), reported beside it as *_16bit_ids;
+  plain   vote_kernel: bytes per launch = sum(L) * (8k + 28) over the average
+          launch duration.
 
 `traffic` (HBM bytes per launch of that kernel) comes from the committed
 counter summary profiles/evo_persistent_kernel_traffic.txt when it was
@@ -46,7 +51,7 @@ CoreKNNGraph_CPU on exact 64-NN landmark graphs of the same runs), run on
 `cores` host threads, one run per thread at a time, in passes over all runs
 until at least --cpu-seconds of wall time.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--runs 100]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--runs 100]
     python bench.py --copy-probe     # the 256 MiB copy that calibrates the counters
     python bench.py --graph-rows 1000000   # exact 30-NN graph of 1M x 50 (own JSON line)
 """
@@ -275,6 +280,74 @@ def learned_curve_bench(runs: int, cycles: int, k: int) -> int:
     return 0
 
 
+def pls_workload(runs: int = 100, landmarks: int = 2000, classes: int = 8, seed: int = 13):
+    """The PLS-LDA batched workload: X = 10,000 x 50 latent Gaussian (variance
+    1/i per column) with 8 class-shifted clusters, M runs of 2000 landmarks,
+    10 folds, starting labels the k-means-like nearest of 8 random landmark
+    centres."""
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    x, _ = D.config2()
+    n = x.shape[0]
+    rows, init = [], []
+    for r in range(runs):
+        rw = np.sort(np.argsort(D.uniform(seed, 100 + r, n), kind="stable")[:landmarks]).astype(np.int32)
+        centres = x[rw[np.argsort(D.uniform(seed, 300 + r, landmarks), kind="stable")[:classes]]]
+        idx, _ = K.exact_knn_search(centres, x[rw], 1, "euclidean")
+        rows.append(rw)
+        init.append((idx[:, 0] + 1).astype(np.int32))
+    return x, rows, init
+
+
+def pls_form(args) -> dict:
+    """The batched CorePLSLDA form on the PLS-LDA workload (10 folds, 5
+    components, the KODAMA.matrix per-run flags): steps/s and the fused
+    evaluation kernel against its per-step bytes (one read of each landmark
+    row of p floats plus three words per row)."""
+    import kodama_amd as K
+    x, rows, init = pls_workload(args.runs)
+    kw = dict(classifier=0, folds=10, stratified=0, seed=1, cv_seed=1, max_components=5, fixed_components=5,
+              center=1, scale=1, pls_fold_cache=1, evolutionary_search=1, guarded_diversity=1,
+              many_to_one_absorption=1, auto_class_coarsening=1, n_threads=0)
+    if args.warmup > 0:
+        K.core_plslda_batch(x, rows, init, K.core_options(cycles=min(args.warmup, 2), **kw))
+    out = K.core_plslda_batch(x, rows, init, K.core_options(cycles=args.steps, **kw))
+    L = sum(len(r) for r in rows)
+    kern = out["eval_kernel_seconds"]
+    algo = float(L) * (50 * 4 + 12)  # bytes per evaluation of all runs
+    return dict(workload=f"PLS-LDA: X=10000x50 latent + 8 shifted clusters, M={args.runs}, 2000 landmarks, "
+                         f"10 folds, ncomp=5, Tcycle={args.steps}",
+                steps_per_s=args.runs * args.steps / out["evolution_seconds"],
+                evolution_seconds=out["evolution_seconds"], mean_accbest=float(np.mean(out["accbest"])),
+                eval_kernel_ms=kern * 1e3,
+                roofline={"kernel": "pls_eval_kernel", "bound": "latency (block barriers, FP64)",
+                          "achieved": algo / kern / 1e9 if kern > 0 else 0.0, "unit": "GB/s",
+                          "peak": HBM_PEAK_GBS, "frac": algo / kern / 1e9 / HBM_PEAK_GBS if kern > 0 else 0.0,
+                          "algorithmic_bytes_per_launch": algo, "mfma": "none (class sums)"})
+
+
+def pls_fit_bench(rows: int, ncomp: int) -> int:
+    """General PLS of the one-hot mixture components (20 responses) on rows of
+    the 20-cluster mixture (the 1M x 50 generator): the FP64 matrix-core
+    cross-product kernel against the FP64 matrix peak, and the whole fit."""
+    import kodama_amd as K
+    from kodama_amd import datasets as D
+    x, comp = D.config4(n=rows)
+    y = np.zeros((rows, 20), np.float32)
+    y[np.arange(rows), comp - 1] = 1.0
+    K.PLS(x[:4096], y[:4096], ncomp=ncomp)  # warm-up
+    t0 = time.perf_counter()
+    out = K.PLS(x, y, ncomp=ncomp)
+    wall = time.perf_counter() - t0
+    flops = 2.0 * rows * 50 * (50 + 20)
+    print(json.dumps({"pls_fit_rows": rows, "p": 50, "q": 20, "ncomp": ncomp, "components": out["components"],
+                      "seconds": wall, "device_seconds": out["device_seconds"],
+                      "cross_kernel_ms": out["cross_seconds"] * 1e3,
+                      "cross_tflops": flops / out["cross_seconds"] / 1e12,
+                      "cross_frac_of_fp64_matrix_peak": flops / out["cross_seconds"] / 78.6e12}), flush=True)
+    return 0
+
+
 def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int):
     """HBM bytes per launch from the committed counter summary, if it covers
     this kernel on this workload."""
@@ -297,12 +370,12 @@ def measured_traffic(kernel: str, runs: int, cycles: int, k: int, landmarks: int
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50, help="label-evolution cycles per run (Tcycle)")
+    ap.add_argument("--steps", type=int, default=20, help="label-evolution cycles per run (Tcycle)")
     ap.add_argument("--warmup", type=int, default=5, help="cycles of an untimed warm-up evolution")
     ap.add_argument("--runs", type=int, default=100)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--k", type=int, default=30)
-    ap.add_argument("--forms", default="plain,device,learned")
+    ap.add_argument("--forms", default="plain,device,learned,pls")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum wall time of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter-calibration copy and exit")
@@ -312,6 +385,8 @@ def main() -> int:
                          " label-noise configuration (--steps cycles, default 100 here) and exit")
     ap.add_argument("--tables", action="store_true",
                     help="time the per-fold kNN tables of the workload by each path and exit")
+    ap.add_argument("--pls-fit-rows", type=int, default=0,
+                    help="time the general PLS fit (FP64 matrix-core cross-products) on this many rows and exit")
     ap.add_argument("--matrix-rows", type=int, default=0,
                     help="time end-to-end KODAMA.matrix on this many rows (runs, steps, forms apply) and exit")
     args = ap.parse_args()
@@ -322,7 +397,9 @@ def main() -> int:
     if args.tables:
         return tables_bench(args.runs, args.n, args.k)
     if args.learned_curve:
-        return learned_curve_bench(args.runs, 100 if args.steps == 50 else args.steps, args.k)
+        return learned_curve_bench(args.runs, 100 if args.steps == 20 else args.steps, args.k)
+    if args.pls_fit_rows > 0:
+        return pls_fit_bench(args.pls_fit_rows, 10)
     if args.matrix_rows > 0:
         return matrix_bench(args.matrix_rows, args.runs, args.steps, args.k, [f for f in args.forms.split(",") if f])
     if args.gpus != 1:
@@ -338,6 +415,11 @@ def main() -> int:
     base = dict(k=args.k, folds=10, stratified=0, seed=1, cv_seed=1, evolutionary_search=1,
                 guarded_diversity=1, many_to_one_absorption=1, n_threads=0)
     forms = [f for f in args.forms.split(",") if f]
+    pls = None
+    if "pls" in forms:
+        forms.remove("pls")
+        pls = pls_form(args)
+        log(f"pls: {pls['steps_per_s']:,.0f} steps/s, eval kernel {pls['eval_kernel_ms']:.3f} ms")
     res = {}
     curves = {}
     params = None
@@ -381,14 +463,16 @@ def main() -> int:
                 res[form]["evolution_seconds"] * (int(hit[0]) + 1) / args.steps if hit.size else None
     head = "device" if "device" in res else forms[0]
     kern_s = res[head]["vote_kernel_ms"] * 1e-3
+    evaluations = float(args.steps + 1) if head == "device" else 1.0
     if head == "device":
         kmax = 8 if args.k <= 8 else 16 if args.k <= 16 else 32 if args.k <= 32 else 64
         kernel = f"evo_persistent_kernel<{kmax},{1024 if kmax <= 32 else 512}>"
-        algo_bytes = float(args.steps + 1) * float(L.sum()) * (2 * args.k)
     else:
         kernel = "vote_kernel"
-        algo_bytes = float(L.sum()) * (8 * args.k + 28)
+    algo_bytes = evaluations * float(L.sum()) * (8 * args.k + 28)
+    algo_bytes_16 = evaluations * float(L.sum()) * (2 * args.k)
     achieved = algo_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
+    achieved_16 = algo_bytes_16 / kern_s / 1e9 if kern_s > 0 else 0.0
     traffic = measured_traffic(kernel, args.runs, args.steps, args.k, int(L.sum())) if head == "device" else None
     line = {
         "metric": METRIC,
@@ -404,15 +488,17 @@ def main() -> int:
         "dtype": "i32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "batched M runs: X=100000x50 20-cluster Zipf(1.1) mixture, kNN k=30, M=100, "
-                        "landmarks U[1000,5000], 10 uneven folds, Tcycle=50",
+            "workload": f"batched M runs: X={args.n}x50 20-cluster Zipf(1.1) mixture, kNN k={args.k}, "
+                        f"M={args.runs}, landmarks U[1000,5000], 10 uneven folds, Tcycle={args.steps}",
             "n": args.n, "p": 50, "k": args.k, "runs": args.runs, "cycles": args.steps,
             "landmarks_total": int(L.sum()), "form": head,
         },
         "forms": res,
         "roofline": {"bound": "hbm", "kernel": kernel, "kernel_ms": kern_s * 1e3, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes_per_launch": algo_bytes},
+                     "traffic": traffic, "algorithmic_bytes_per_launch": algo_bytes,
+                     "bytes_per_row_evaluation": 8 * args.k + 28, "evaluations_per_launch": evaluations,
+                     "achieved_16bit_ids": achieved_16, "frac_16bit_ids": achieved_16 / HBM_PEAK_GBS},
     }
     if "plain" in res and "device" in res:
         line["accuracy_gap_pt"] = 100.0 * (res["device"]["mean_accbest"] - res["plain"]["mean_accbest"])
@@ -424,6 +510,12 @@ def main() -> int:
         # form's own time to it against the learned form's.
         tp, tl = res["plain"].get("seconds_to_plain_accuracy"), res["learned"].get("seconds_to_plain_accuracy")
         line["time_to_accuracy_learned_vs_plain"] = tp / tl if tp and tl else None
+        # The same with the network's trace collection and training counted
+        # on the learned side (a one-time cost per trained network).
+        tt = res["learned"].get("training_seconds")
+        line["time_to_accuracy_learned_incl_training_vs_plain"] = tp / (tl + tt) if tp and tl and tt else None
+    if pls is not None:
+        line["pls_lda_form"] = pls
     if not args.no_cpu_baseline:
         cores = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(x, rows, init, args.k, args.steps, cores, args.cpu_seconds)

@@ -205,6 +205,35 @@ int kodama_hip_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* lab
                         int32_t* predicted_labels, int32_t* fold_assignments,
                         int32_t* selected_components);
 
+/* kodama::PLSDACV_CUDA (kodama.hpp:913-918; plscv.cpp:2934-2938): the F-fold
+ * CV of kodama_hip_plsldacv with the PLS-DA head of run_plscv_host
+ * (plscv.cpp:1210-1296, 2430-2442): per fold, the least-squares regression of
+ * the centred one-hot training labels on the training scores, solved by
+ * Gauss-Jordan elimination with partial pivoting, and the class of largest
+ * fitted value. FP64 on the device. */
+int kodama_hip_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                       const int32_t* constrain, const kodama_hip_core_options* options,
+                       int32_t* predicted_labels, int32_t* fold_assignments,
+                       int32_t* selected_components);
+
+/* kodama::PLS_CUDA (kodama.hpp:1377-1381; fit_general_pls, plscv.cpp:2765-2878):
+ * general PLS of y [n x q] on x [n x p]. X centred / scaled as
+ * train_center_scale_float (options center, scale), Y centred; S = X'Y and
+ * the Gram matrix X'X on the matrix cores (FP64), power-refresh SIMPLS with
+ * A = max(1, min(fixed or max components, p, n - 1)) at most, response
+ * weights Q = S'W, coefficients W Q', scores X W and fitted values
+ * ybar + X W Q'. Outputs (float, row-major, PLSFitResult): weights [p x A],
+ * response_loadings [q x A], scores [n x A] (buffers for the requested A),
+ * coefficients [p x q], fitted [n x q], x_center / x_scale [p], y_center [q];
+ * *components = A. timings[2] (may be NULL): the cross-product kernel and the
+ * whole device fit (seconds, HIP events). A fit without a component is
+ * ERUNTIME (the reference's DegeneratePLSFit). */
+int kodama_hip_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
+                   const kodama_hip_core_options* options, float* weights,
+                   float* response_loadings, float* scores, float* coefficients, float* fitted,
+                   float* x_center, float* x_scale, float* y_center, int32_t* components,
+                   double* timings);
+
 /* kodama::CorePLSLDA_CUDA (kodama.hpp:1002-1008; core.cpp:1720-1749). */
 int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* labels,
                            const int32_t* constrain, const int32_t* fixed,
@@ -429,6 +458,18 @@ int kodama_hip_mlp_collect_traces(const float* x, int32_t n, int32_t p, int32_t 
                                   const int32_t* initial_labels, const int32_t* fold_ids,
                                   const kodama_hip_core_options* options, int32_t max_samples,
                                   float* features, int32_t* targets, int32_t* samples, double* accbest);
+
+/* The same run recording outcome samples: at every cycle every row whose
+ * label differs from its prediction, its features under the current labels,
+ * and its target: the rank of the label it holds in the run's final best
+ * labelling (30 when that is its current label; rows whose final label is
+ * not among the voted ranks are skipped). Same buffers and outputs. */
+int kodama_hip_mlp_collect_outcomes(const float* x, int32_t n, int32_t p, int32_t runs,
+                                    const int32_t* run_offsets, const int32_t* landmark_rows,
+                                    const int32_t* initial_labels, const int32_t* fold_ids,
+                                    const kodama_hip_core_options* options, int32_t max_samples,
+                                    float* features, int32_t* targets, int32_t* samples,
+                                    double* accbest);
 
 /* Minibatch SGD with momentum on the device (deterministic for a seed):
  * params in/out; loss_per_epoch [epochs] may be NULL. */

@@ -131,6 +131,63 @@ def PLSLDACV(data, labels, constrain=None, folds=10, stratified=True, seed=1, nc
                 backend="hip")
 
 
+def PLSDACV(data, labels, constrain=None, folds=10, stratified=True, seed=1, ncomp=None,
+            center=True, scale=True, backend=None, n_cores=1, gpu_device=0, max_components=None):
+    """PLS-DA cross-validation (PLSDACV, plscv.cpp:2926-2938): the folds and
+    SIMPLS fits of PLSLDACV with the least-squares PLS-DA head. ncomp fixes
+    the evaluated components (0: max_components)."""
+    _backend(backend)
+    x = _matrix(data)
+    lab = N.i32(labels)
+    ncomp = min(50, x.shape[1]) if ncomp is None else int(ncomp)
+    mc = ncomp if max_components is None else int(max_components)
+    opt = core_options(classifier=0, folds=folds, stratified=int(bool(stratified)), cv_seed=seed,
+                       max_components=mc, fixed_components=ncomp, center=int(bool(center)),
+                       scale=int(bool(scale)), pls_fold_cache=0, device=gpu_device, n_threads=n_cores)
+    pred = np.zeros(x.shape[0], np.int32)
+    fold = np.zeros(x.shape[0], np.int32)
+    sel = C.c_int32(0)
+    N.call("kodama_hip_plsdacv", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), N.ptr(lab),
+           N.ptr(N.i32(constrain)), C.byref(opt), N.ptr(pred), N.ptr(fold), C.byref(sel))
+    return dict(predicted_labels=pred, true_labels=lab.copy(), fold_assignments=fold,
+                selected_components=sel.value, global_accuracy=float(np.mean(pred == lab)),
+                backend="hip")
+
+
+def PLS(x, y, ncomp=10, center=True, scale=True, max_components=None, gpu_device=0) -> dict:
+    """General PLS (PLS_CUDA contract; fit_general_pls, plscv.cpp:2765-2878):
+    weights [p x A], response_loadings [q x A], scores [n x A], coefficients
+    [p x q], fitted [n x q], x_center, x_scale, y_center, components, and the
+    device seconds of the cross-product kernel and of the whole fit."""
+    xm = _matrix(x)
+    ym = np.ascontiguousarray(y, np.float32)
+    if ym.ndim == 1:
+        ym = ym[:, None]
+    n, p = xm.shape
+    q = ym.shape[1]
+    if ym.shape[0] != n:
+        raise ValueError("PLS X and Y must have the same rows.")
+    mc = int(ncomp) if max_components is None else int(max_components)
+    opt = core_options(classifier=0, max_components=mc, fixed_components=int(ncomp), center=int(bool(center)),
+                       scale=int(bool(scale)), device=gpu_device)
+    A = max(1, int(ncomp) if int(ncomp) > 0 else mc)
+    out = dict(weights=np.zeros(p * A, np.float32), response_loadings=np.zeros(q * A, np.float32),
+               scores=np.zeros(n * A, np.float32), coefficients=np.zeros((p, q), np.float32),
+               fitted=np.zeros((n, q), np.float32), x_center=np.zeros(p, np.float32),
+               x_scale=np.zeros(p, np.float32), y_center=np.zeros(q, np.float32))
+    comps = C.c_int32(0)
+    timings = np.zeros(2, np.float64)
+    N.call("kodama_hip_pls", N.ptr(xm), C.c_int32(n), C.c_int32(p), N.ptr(ym), C.c_int32(q), C.byref(opt),
+           *(N.ptr(out[k]) for k in ("weights", "response_loadings", "scores", "coefficients", "fitted",
+                                     "x_center", "x_scale", "y_center")), C.byref(comps), N.ptr(timings))
+    a = comps.value
+    out["weights"] = out["weights"][: p * a].reshape(p, a)
+    out["response_loadings"] = out["response_loadings"][: q * a].reshape(q, a)
+    out["scores"] = out["scores"][: n * a].reshape(n, a)
+    out.update(components=a, cross_seconds=float(timings[0]), device_seconds=float(timings[1]))
+    return out
+
+
 def PLSLDAPredict(train, labels, test, ncomp=None, center=True, scale=True, backend=None, gpu_device=0):
     """PLSLDAPredict (plscv.cpp:2976-3005): fit PLS-LDA on the training rows
     and their labels, return the predicted labels of the test rows."""
@@ -345,15 +402,20 @@ def _batch_args(data, run_rows, initial_labels, fold_ids):
     return x, M, offsets, rows, init, folds
 
 
-def mlp_collect_traces(data, run_rows, initial_labels, options: CoreOptions, fold_ids=None, max_samples=200000):
-    """Accept/reject samples of the plain batched evolution: features [S x 64],
-    targets [S] (rank of the new label when accepted, 30 when rejected)."""
+def mlp_collect_traces(data, run_rows, initial_labels, options: CoreOptions, fold_ids=None, max_samples=200000,
+                       targets="accept"):
+    """Samples of the plain batched evolution: features [S x 64], targets [S].
+    targets "accept": the rows a proposal relabels, target the rank of the new
+    label when the proposal was accepted, 30 when rejected; "outcome": every
+    row whose label differs from its prediction at every cycle, target the
+    rank of its label in the run's final best labelling (30: unchanged)."""
     x, M, offsets, rows, init, folds = _batch_args(data, run_rows, initial_labels, fold_ids)
+    fn = {"accept": "kodama_hip_mlp_collect_traces", "outcome": "kodama_hip_mlp_collect_outcomes"}[targets]
     feat = np.zeros((max_samples, 64), np.float32)
     tgt = np.zeros(max_samples, np.int32)
     count = C.c_int32(0)
     acc = np.zeros(M, np.float64)
-    N.call("kodama_hip_mlp_collect_traces", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
+    N.call(fn, N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
            N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), C.c_int32(max_samples),
            N.ptr(feat), N.ptr(tgt), C.byref(count), N.ptr(acc))
     c = count.value

@@ -102,26 +102,54 @@ int kodama_hip_mlp_gradients(const float* params, const float* features, const i
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+void collect(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
+             const int32_t* landmark_rows, const int32_t* initial_labels, const int32_t* fold_ids,
+             const kodama_hip_core_options* options, int32_t max_samples, bool outcome, float* features,
+             int32_t* targets, int32_t* samples, double* accbest) {
+  require(features != nullptr && targets != nullptr && samples != nullptr, "null pointer.");
+  require(max_samples >= 1, "max_samples must be positive.");
+  BatchRequest q{x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options};
+  TraceSink sink;
+  sink.max_samples = max_samples;
+  sink.outcome = outcome;
+  const int total = run_offsets ? run_offsets[runs] : 0;
+  std::vector<int32_t> clbest(static_cast<size_t>(std::max(total, 1)));
+  std::vector<double> acc(static_cast<size_t>(std::max(runs, 1)));
+  run_knn_batch(q, 0, nullptr, 0.0f, &sink, clbest.data(), acc.data(), nullptr, nullptr);
+  const int count = static_cast<int>(sink.target.size());
+  std::copy(sink.feat.begin(), sink.feat.end(), features);
+  std::copy(sink.target.begin(), sink.target.end(), targets);
+  *samples = count;
+  if (accbest) std::copy_n(acc.data(), runs, accbest);
+}
+
+}  // namespace
+
+extern "C" {
+
 int kodama_hip_mlp_collect_traces(const float* x, int32_t n, int32_t p, int32_t runs, const int32_t* run_offsets,
                                   const int32_t* landmark_rows, const int32_t* initial_labels,
                                   const int32_t* fold_ids, const kodama_hip_core_options* options,
                                   int32_t max_samples, float* features, int32_t* targets, int32_t* samples,
                                   double* accbest) {
   return guarded([&] {
-    require(features != nullptr && targets != nullptr && samples != nullptr, "null pointer.");
-    require(max_samples >= 1, "max_samples must be positive.");
-    BatchRequest q{x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options};
-    TraceSink sink;
-    sink.max_samples = max_samples;
-    const int total = run_offsets ? run_offsets[runs] : 0;
-    std::vector<int32_t> clbest(static_cast<size_t>(std::max(total, 1)));
-    std::vector<double> acc(static_cast<size_t>(std::max(runs, 1)));
-    run_knn_batch(q, 0, nullptr, 0.0f, &sink, clbest.data(), acc.data(), nullptr, nullptr);
-    const int count = static_cast<int>(sink.target.size());
-    std::copy(sink.feat.begin(), sink.feat.end(), features);
-    std::copy(sink.target.begin(), sink.target.end(), targets);
-    *samples = count;
-    if (accbest) std::copy_n(acc.data(), runs, accbest);
+    collect(x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options, max_samples, false,
+            features, targets, samples, accbest);
+  });
+}
+
+int kodama_hip_mlp_collect_outcomes(const float* x, int32_t n, int32_t p, int32_t runs,
+                                    const int32_t* run_offsets, const int32_t* landmark_rows,
+                                    const int32_t* initial_labels, const int32_t* fold_ids,
+                                    const kodama_hip_core_options* options, int32_t max_samples, float* features,
+                                    int32_t* targets, int32_t* samples, double* accbest) {
+  return guarded([&] {
+    collect(x, n, p, runs, run_offsets, landmark_rows, initial_labels, fold_ids, options, max_samples, true,
+            features, targets, samples, accbest);
   });
 }
 

@@ -2,6 +2,7 @@
 // detail used to check scores and discriminants against an FP64 restatement.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -54,27 +55,54 @@ Tally tally_of(const IVec& codes, const int* pred, int classes, IVec& counts, IV
 
 }  // namespace
 
+namespace {
+
+// One F-fold PLS CV with the LDA or the PLS-DA head (run_plscv_host,
+// plscv.cpp:2293-2507).
+void plscv_impl(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                const kodama_hip_core_options* options, bool da, int32_t* predicted_labels,
+                int32_t* fold_assignments, int32_t* selected_components) {
+  validate(x, n, p, labels, options);
+  require(predicted_labels != nullptr, "null output.");
+  require(options->fixed_components >= 0, "PLSOptions::fixed_components must be non-negative.");
+  require(options->fixed_components <= options->max_components,
+          "PLSOptions::fixed_components cannot exceed max_components.");
+  select_device(options->device);
+  IVec values, codes;
+  encode_labels(labels, n, values, codes);
+  const IVec lab(labels, labels + n);
+  const IVec folds = make_folds(lab, vec_or_empty(constrain, n), options->folds, options->stratified != 0,
+                                options->cv_seed);
+  PlsCv pls(options->device, x, n, p, *options, static_cast<int>(values.size()));
+  pls.set_da(da);
+  IVec pred(static_cast<size_t>(n));
+  int sel = 0;
+  pls.evaluate(codes, folds, pred.data(), &sel);
+  for (int i = 0; i < n; ++i) predicted_labels[i] = values[pred[i]];
+  if (fold_assignments) std::copy(folds.begin(), folds.end(), fold_assignments);
+  if (selected_components) *selected_components = sel;
+}
+
+}  // namespace
+
 extern "C" {
 
 int kodama_hip_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
                         const kodama_hip_core_options* options, int32_t* predicted_labels,
                         int32_t* fold_assignments, int32_t* selected_components) {
   return guarded([&] {
-    validate(x, n, p, labels, options);
-    require(predicted_labels != nullptr, "null output.");
-    select_device(options->device);
-    IVec values, codes;
-    encode_labels(labels, n, values, codes);
-    const IVec lab(labels, labels + n);
-    const IVec folds = make_folds(lab, vec_or_empty(constrain, n), options->folds, options->stratified != 0,
-                                  options->cv_seed);
-    PlsCv pls(options->device, x, n, p, *options, static_cast<int>(values.size()));
-    IVec pred(static_cast<size_t>(n));
-    int sel = 0;
-    pls.evaluate(codes, folds, pred.data(), &sel);
-    for (int i = 0; i < n; ++i) predicted_labels[i] = values[pred[i]];
-    if (fold_assignments) std::copy(folds.begin(), folds.end(), fold_assignments);
-    if (selected_components) *selected_components = sel;
+    plscv_impl(x, n, p, labels, constrain, options, false, predicted_labels, fold_assignments, selected_components);
+  });
+}
+
+// PLSDACV_CUDA (plscv.cpp:2934-2938): the same folds and SIMPLS fits with the
+// PLS-DA head: the least-squares regression of the centred one-hot labels on
+// the training scores, the class of largest fitted value.
+int kodama_hip_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                       const kodama_hip_core_options* options, int32_t* predicted_labels,
+                       int32_t* fold_assignments, int32_t* selected_components) {
+  return guarded([&] {
+    plscv_impl(x, n, p, labels, constrain, options, true, predicted_labels, fold_assignments, selected_components);
   });
 }
 
@@ -391,6 +419,41 @@ int kodama_hip_plslda_fold_detail(const float* x, int32_t n, int32_t p, const in
     std::copy(d.val_scores.begin(), d.val_scores.end(), val_scores);
     std::copy(d.discriminants.begin(), d.discriminants.end(), discriminants);
     for (size_t c = 0; c < d.classes.size(); ++c) classes[c] = values[d.classes[c]];
+  });
+}
+
+// PLS_CUDA (plscv.cpp:2899-2905 -> fit_general_pls 2765-2878): general PLS
+// of y [n x q] on x [n x p], FP64 on the device (cross-products on the
+// matrix cores), float outputs as PLSFitResult.
+int kodama_hip_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
+                   const kodama_hip_core_options* options, float* weights, float* response_loadings,
+                   float* scores, float* coefficients, float* fitted, float* x_center, float* x_scale,
+                   float* y_center, int32_t* components, double* timings) {
+  return guarded([&] {
+    require(x != nullptr && y != nullptr && n >= 2 && p >= 1 && q >= 1,
+            "PLS requires non-empty X and Y matrices with at least two rows.");
+    require(options != nullptr, "null options.");
+    require(weights && response_loadings && scores && coefficients && fitted && x_center && x_scale && y_center &&
+              components,
+            "null output.");
+    for (size_t e = 0; e < static_cast<size_t>(n) * q; ++e) {
+      require(std::isfinite(y[e]), "PLS Y must contain only finite values.");
+    }
+    select_device(options->device);
+    const PlsGeneralResult r = pls_general_fit(options->device, x, y, n, p, q, *options);
+    *components = r.components;
+    std::copy(r.weights.begin(), r.weights.end(), weights);
+    std::copy(r.response_loadings.begin(), r.response_loadings.end(), response_loadings);
+    std::copy(r.scores.begin(), r.scores.end(), scores);
+    std::copy(r.coefficients.begin(), r.coefficients.end(), coefficients);
+    std::copy(r.fitted.begin(), r.fitted.end(), fitted);
+    std::copy(r.x_center.begin(), r.x_center.end(), x_center);
+    std::copy(r.x_scale.begin(), r.x_scale.end(), x_scale);
+    std::copy(r.y_center.begin(), r.y_center.end(), y_center);
+    if (timings) {
+      timings[0] = r.cross_ms * 1e-3;
+      timings[1] = r.total_ms * 1e-3;
+    }
   });
 }
 

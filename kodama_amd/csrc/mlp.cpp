@@ -57,7 +57,42 @@ void evolve_plain_traced(std::vector<RunEvolution>& runs, KnnBatch& batch, const
   std::vector<std::vector<int>> rank(static_cast<size_t>(M));
   std::vector<int> codes(static_cast<size_t>(k)), ranked(kMlpRanks);
   float f[kMlpIn];
+  // Outcome samples wait for the final labelling: (run, row, current code,
+  // ranked codes) beside their features.
+  struct Pending {
+    int run, row, cur;
+    int ranked[kMlpRanks];
+  };
+  std::vector<Pending> pending;
+  std::vector<float> pending_feat;
+  auto record_outcomes = [&]() {
+    for (int r = 0; r < M; ++r) {
+      RunEvolution& ev = runs[r];
+      const IVec& cur = ev.current_labels();
+      const IVec& pred = ev.current_predictions();
+      const int base = batch.pad_offset(r);
+      for (int i = 0; i < ev.rows(); ++i) {
+        if (static_cast<int>(pending.size()) >= sink.max_samples) return;
+        if (ev.fixed_row(i) || pred[i] == cur[i]) continue;
+        for (int j = 0; j < k; ++j) {
+          const int id = nbr[static_cast<size_t>(j) * pad + base + i];
+          codes[j] = id >= 0 ? cur[id - base] : -1;
+        }
+        Pending pd;
+        pd.run = r;
+        pd.row = i;
+        pd.cur = cur[i];
+        host_row_features(codes.data(), k, cur[i], f, pd.ranked);
+        pending.push_back(pd);
+        pending_feat.insert(pending_feat.end(), f, f + kMlpIn);
+      }
+    }
+  };
   auto hook = [&](int cycle, bool after_finish) {
+    if (sink.outcome) {
+      if (!after_finish) record_outcomes();
+      return;
+    }
     for (int r = 0; r < M; ++r) {
       RunEvolution& ev = runs[r];
       if (!after_finish) {
@@ -94,6 +129,21 @@ void evolve_plain_traced(std::vector<RunEvolution>& runs, KnnBatch& batch, const
     }
   };
   evolve_plain(runs, batch, folds, cycles, n_threads, hook);
+  if (sink.outcome) {
+    for (size_t e = 0; e < pending.size(); ++e) {
+      const Pending& pd = pending[e];
+      const int fin = runs[pd.run].best_codes()[pd.row];
+      int target = -1;
+      if (fin == pd.cur) {
+        target = kMlpKeep;
+      } else {
+        for (int j = 0; j < kMlpRanks; ++j) if (pd.ranked[j] == fin) target = j;
+      }
+      if (target < 0) continue;
+      sink.feat.insert(sink.feat.end(), pending_feat.begin() + e * kMlpIn, pending_feat.begin() + (e + 1) * kMlpIn);
+      sink.target.push_back(target);
+    }
+  }
 }
 
 void mlp_init(std::uint64_t seed, float* params) {

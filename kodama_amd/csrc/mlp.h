@@ -23,10 +23,18 @@ void host_row_features(const int* codes, int k, int own, float* feat, int* ranke
 // label), its features under the labels the proposal started from, and the
 // target: the rank of its new label when the proposal became the current
 // state, keep (30) otherwise.
+// With outcome targets every row whose label differs from its prediction is
+// a sample at every cycle (features under the current labels), and its target
+// is the rank of the label the row holds in the run's final best labelling
+// (keep when that is its current label; rows whose final label is not among
+// the voted ranks are not recorded): the network then learns which rows end
+// up relabelled, and to which label, rather than the proposal-level accept
+// flag that every row of a proposal shares.
 struct TraceSink {
   std::vector<float> feat;    // [samples x 64]
   std::vector<int> target;    // [samples]
   int max_samples = 0;        // stop recording beyond this many
+  bool outcome = false;       // outcome targets instead of accept / reject
 };
 
 // The plain batched evolution (evolve_plain) with trace recording; runs are

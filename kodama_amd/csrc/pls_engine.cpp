@@ -25,7 +25,7 @@ std::vector<double> pls_start_vector(int p) {
 struct PlsCv::Impl {
   int device = 0, n = 0, p = 0, classes = 0;
   int requested = 0, max_components = 0, rank_cap = 0;
-  bool center = true, scale = true;
+  bool center = true, scale = true, da = false;
   Stream stream;
   DeviceBuffer<float> x;
   DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, s0, gw, linear, constant, val_scores, disc;
@@ -181,6 +181,7 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   e.p = m.p;
   e.folds = F;
   e.requested = m.requested;
+  e.da = m.da ? 1 : 0;
   e.rank_cap = m.rank_cap;
   e.w0 = m.w0.get();
   e.mean = m.mean.get();
@@ -225,6 +226,8 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   if (selected) *selected = std::min(evaluated, m.max_components);
 }
 
+void PlsCv::set_da(bool da) { impl_->da = da; }
+
 double PlsCv::eval_kernel_ms_average() const {
   return impl_->eval_calls ? impl_->eval_ms_total / impl_->eval_calls : 0.0;
 }
@@ -266,6 +269,143 @@ PlsFoldDetail PlsCv::detail(int fold) const {
     for (int c = 0; c < cf; ++c) d.discriminants[static_cast<size_t>(i) * cf + c] = disc[static_cast<size_t>(i) * m.classes + c];
   }
   return d;
+}
+
+PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int n, int p, int q,
+                                 const kodama_hip_core_options& o) {
+  const int requested = o.fixed_components > 0 ? o.fixed_components : o.max_components;
+  if (requested < 1) throw InvalidArgument("PLS component count must be positive.");
+  if (p > kPlsMaxDims) throw InvalidArgument("the HIP PLS kernels support up to 256 predictors.");
+  if (static_cast<size_t>(q) * p > static_cast<size_t>(kPlsMaxSLds) || pls_fit_lds_bytes(p, q) > 160 * 1024 ||
+      (p + q + 15) / 16 * 16 * 16 * sizeof(double) > 64 * 1024) {
+    throw InvalidArgument("the HIP PLS kernels hold responses x predictors <= 8192 and predictors + responses <= 512.");
+  }
+  const int rank_cap = std::max(1, std::min({requested, p, std::max(1, n - 1)}));
+  if (rank_cap > kPlsMaxComponents) throw InvalidArgument("the HIP PLS kernels support up to 64 components.");
+  Stream st(device);
+  const hipStream_t s = st.get();
+  DeviceBuffer<float> dx(static_cast<size_t>(n) * p), dy(static_cast<size_t>(n) * q);
+  dx.upload(x, dx.size(), s);
+  dy.upload(y, dy.size(), s);
+  DeviceBuffer<double> xm(p), xsd(p), ym(q), ysd(q);
+  EventTimer t_all, t_cross;
+  t_all.start(s);
+  // Column statistics in row chunks, reduced in chunk order.
+  const int st_chunks = std::max(1, std::min(2048, (n + 63) / 64));
+  const int st_rows = (n + st_chunks - 1) / st_chunks;
+  DeviceBuffer<double> partial(static_cast<size_t>(st_chunks) * std::max(p, q));
+  auto stats = [&](const float* m, int cols, double* mean, double* sd, int scale) {
+    PlsColStatsParams c;
+    c.m = m;
+    c.n = n;
+    c.cols = cols;
+    c.chunks = st_chunks;
+    c.chunk_rows = st_rows;
+    c.center = o.center != 0 ? 1 : 0;
+    c.mean = mean;
+    c.sd = sd;
+    c.partial = partial.get();
+    c.pass = 0;
+    check(launch_pls_colstats(c, s), "pls_colstats launch");
+    check(launch_pls_colstats_reduce(c, scale, s), "pls_colstats reduce");
+    c.pass = 1;
+    if (scale) check(launch_pls_colstats(c, s), "pls_colstats launch");
+    check(launch_pls_colstats_reduce(c, scale, s), "pls_colstats reduce");
+  };
+  stats(dx.get(), p, xm.get(), xsd.get(), o.scale != 0 ? 1 : 0);
+  stats(dy.get(), q, ym.get(), ysd.get(), 0);
+  // Cross-products on the matrix cores.
+  const int p16 = (p + 15) / 16 * 16, z16 = (p + q + 15) / 16 * 16;
+  const int cr_rows = std::max(16, ((n + 1023) / 1024 + 15) / 16 * 16);
+  const int cr_chunks = (n + cr_rows - 1) / cr_rows;
+  DeviceBuffer<double> cpart(static_cast<size_t>(cr_chunks) * p16 * z16), gram(static_cast<size_t>(p) * p),
+    cross(static_cast<size_t>(q) * p);
+  PlsCrossParams cp;
+  cp.x = dx.get();
+  cp.y = dy.get();
+  cp.n = n;
+  cp.p = p;
+  cp.q = q;
+  cp.chunks = cr_chunks;
+  cp.chunk_rows = cr_rows;
+  cp.x_mean = xm.get();
+  cp.x_sd = xsd.get();
+  cp.y_mean = ym.get();
+  cp.partial = cpart.get();
+  cp.gram = gram.get();
+  cp.cross = cross.get();
+  t_cross.start(s);
+  check(launch_pls_cross(cp, s), "pls_cross launch");
+  t_cross.stop(s);
+  // SIMPLS, response weights, coefficients.
+  const std::vector<double> w0 = pls_start_vector(p);
+  DeviceBuffer<double> dw0(p), W(static_cast<size_t>(p) * kPlsMaxComponents), V(W.size()),
+    Q(static_cast<size_t>(q) * kPlsMaxComponents), B(static_cast<size_t>(p) * q);
+  DeviceBuffer<int> dfit(1);
+  dw0.upload(w0.data(), w0.size(), s);
+  PlsFitParams fp;
+  fp.p = p;
+  fp.q = q;
+  fp.rank_cap = rank_cap;
+  fp.w0 = dw0.get();
+  fp.gram = gram.get();
+  fp.cross = cross.get();
+  fp.weights = W.get();
+  fp.loadings = V.get();
+  fp.yweights = Q.get();
+  fp.coefficients = B.get();
+  fp.fitted = dfit.get();
+  check(launch_pls_fit(fp, s), "pls_fit launch");
+  int A = 0;
+  dfit.download(&A, 1, s);
+  st.sync();
+  if (A < 1) throw DeviceError("PLS: SIMPLS found no supervised latent direction.");
+  DeviceBuffer<float> dsc(static_cast<size_t>(n) * A), dft(static_cast<size_t>(n) * q);
+  DeviceBuffer<double> tbuf(static_cast<size_t>(n) * A);
+  PlsScoreParams sp;
+  sp.x = dx.get();
+  sp.n = n;
+  sp.p = p;
+  sp.q = q;
+  sp.A = A;
+  sp.x_mean = xm.get();
+  sp.x_sd = xsd.get();
+  sp.y_mean = ym.get();
+  sp.weights = W.get();
+  sp.yweights = Q.get();
+  sp.tbuf = tbuf.get();
+  sp.scores = dsc.get();
+  sp.fitted = dft.get();
+  check(launch_pls_scores(sp, s), "pls_scores launch");
+  t_all.stop(s);
+  PlsGeneralResult r;
+  r.components = A;
+  std::vector<double> hw(W.size()), hq(Q.size()), hb(B.size()), hxm(p), hxs(p), hym(q);
+  W.download(hw.data(), hw.size(), s);
+  Q.download(hq.data(), hq.size(), s);
+  B.download(hb.data(), hb.size(), s);
+  xm.download(hxm.data(), hxm.size(), s);
+  xsd.download(hxs.data(), hxs.size(), s);
+  ym.download(hym.data(), hym.size(), s);
+  r.scores.resize(dsc.size());
+  r.fitted.resize(dft.size());
+  dsc.download(r.scores.data(), r.scores.size(), s);
+  dft.download(r.fitted.data(), r.fitted.size(), s);
+  st.sync();
+  r.weights.resize(static_cast<size_t>(p) * A);
+  r.response_loadings.resize(static_cast<size_t>(q) * A);
+  for (int j = 0; j < p; ++j)
+    for (int a = 0; a < A; ++a) r.weights[static_cast<size_t>(j) * A + a] = static_cast<float>(hw[static_cast<size_t>(j) * kPlsMaxComponents + a]);
+  for (int c = 0; c < q; ++c)
+    for (int a = 0; a < A; ++a)
+      r.response_loadings[static_cast<size_t>(c) * A + a] = static_cast<float>(hq[static_cast<size_t>(c) * kPlsMaxComponents + a]);
+  r.coefficients.assign(hb.begin(), hb.end());
+  r.x_center.assign(hxm.begin(), hxm.end());
+  r.x_scale.assign(hxs.begin(), hxs.end());
+  r.y_center.assign(hym.begin(), hym.end());
+  r.cross_ms = t_cross.ms();
+  r.total_ms = t_all.ms();
+  return r;
 }
 
 }  // namespace kodama_hip

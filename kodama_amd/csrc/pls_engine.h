@@ -38,6 +38,9 @@ class PlsCv {
   // max_components). Fold statistics are recomputed when `folds` differs from
   // the previous call's.
   void evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected);
+  // Head of the fold fits: false (default) LDA (PLSLDACV), true the
+  // least-squares PLS-DA head (PLSDACV).
+  void set_da(bool da);
   // Detail of fold `fold` of the last evaluation.
   PlsFoldDetail detail(int fold) const;
   // Average duration of the fused evaluation kernel (HIP events).
@@ -47,6 +50,19 @@ class PlsCv {
   struct Impl;
   Impl* impl_ = nullptr;
 };
+
+// General PLS fit (fit_general_pls, plscv.cpp:2765-2878) on the device in
+// FP64: column statistics, the cross-products Xs'[Xs | Yc] on the matrix
+// cores, power-refresh SIMPLS, response weights, coefficients, scores and
+// fitted values. Outputs as the reference's PLSFitResult (float, row-major).
+struct PlsGeneralResult {
+  int components = 0;
+  std::vector<float> weights, response_loadings, scores, coefficients, fitted, x_center, x_scale, y_center;
+  double cross_ms = 0.0;   // the MFMA cross-product kernel (HIP events)
+  double total_ms = 0.0;   // every kernel of the fit
+};
+PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int n, int p, int q,
+                                 const kodama_hip_core_options& options);
 
 // std::mt19937(1) + std::normal_distribution<float>(0, 1), the first power
 // iteration start of every SIMPLS fit (plscv.cpp:844-846), widened to double.

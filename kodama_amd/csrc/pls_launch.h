@@ -40,6 +40,7 @@ struct PlsEvalParams {
   int p = 0;
   int folds = 0;
   int requested = 0;             // components evaluated (fixed or max)
+  int da = 0;                    // 1: PLS-DA head (least squares on the scores), 0: LDA
   int rank_cap = 0;              // min(max_components, p) before the per-fold n - 1 cap
   const double* w0 = nullptr;    // [p] first power-iteration start (normal draws)
   const double* mean = nullptr;  // fold cache
@@ -62,8 +63,8 @@ struct PlsEvalParams {
   double* loadings = nullptr;        // [folds x p x kPlsMaxComponents]
   double* s0 = nullptr;              // [folds x max_classes x p] S before deflation
   double* gw = nullptr;              // [folds x p x kPlsMaxComponents] G W
-  double* linear = nullptr;          // [folds x max_classes x kPlsMaxComponents]
-  double* constant = nullptr;        // [folds x max_classes]
+  double* linear = nullptr;          // [folds x max_classes x kPlsMaxComponents] (DA: coefficients)
+  double* constant = nullptr;        // [folds x max_classes] (DA: class frequencies)
   // Outputs. components[f] = -1 reports a failed LDA factorisation.
   int* pred_code = nullptr;          // [n] written at validation rows
   int* components = nullptr;         // [folds]
@@ -86,6 +87,81 @@ struct PlsGroupParams {
   int* seg_start = nullptr;          // [folds x (max_classes + 1)]
   int* seg_count = nullptr;          // [folds]
 };
+
+// General PLS (fit_general_pls, plscv.cpp:2765-2878) on the device, FP64.
+// Column statistics of a matrix m [n x cols] in row chunks: pass 0 sums the
+// columns, pass 1 the squared deviations from `mean`; partial [chunks x cols],
+// reduced in chunk order by launch_pls_colstats_reduce into mean (pass 0:
+// sum / n when centring, else 0) or sd (pass 1: sqrt((1 + sum) / max(1, n-1)),
+// non-finite or zero -> 1; the scale accumulator starts at 1 as in
+// train_center_scale_float).
+struct PlsColStatsParams {
+  const float* m = nullptr;
+  int n = 0;
+  int cols = 0;
+  int chunks = 0;
+  int chunk_rows = 0;
+  int pass = 0;
+  int center = 1;
+  double* mean = nullptr;      // [cols]
+  double* sd = nullptr;        // [cols] (pass 1)
+  double* partial = nullptr;   // [chunks x cols]
+};
+
+// Cross-products of the standardised rows: Z = Xs' [Xs | Yc] on the matrix
+// cores (v_mfma_f64_16x16x4f64), one row chunk per workgroup, partial
+// [chunks x p16 x z16] (p16, z16 = p and p + q rounded up to 16), then reduced
+// in chunk order into gram [p x p] and cross [q x p] (row r = X' y_r).
+struct PlsCrossParams {
+  const float* x = nullptr;    // [n x p]
+  const float* y = nullptr;    // [n x q]
+  int n = 0, p = 0, q = 0;
+  int chunks = 0, chunk_rows = 0;  // chunk_rows a multiple of 16
+  const double* x_mean = nullptr;
+  const double* x_sd = nullptr;
+  const double* y_mean = nullptr;
+  double* partial = nullptr;
+  double* gram = nullptr;      // [p x p]
+  double* cross = nullptr;     // [q x p]
+};
+
+// One workgroup: power-refresh SIMPLS of the cross-product (q rows in LDS)
+// with the Gram matrix, then the response weights Q = S0' W and the
+// coefficients W Q'.
+struct PlsFitParams {
+  int p = 0, q = 0;
+  int rank_cap = 0;            // max(1, min(requested, p, n - 1))
+  const double* w0 = nullptr;  // [p]
+  const double* gram = nullptr;
+  const double* cross = nullptr;   // [q x p], undeflated
+  double* weights = nullptr;       // [p x kPlsMaxComponents]
+  double* loadings = nullptr;      // [p x kPlsMaxComponents]
+  double* yweights = nullptr;      // [q x kPlsMaxComponents]
+  double* coefficients = nullptr;  // [p x q]
+  int* fitted = nullptr;           // [1] rank (0: degenerate)
+};
+
+// Scores T = Xs W [n x A] and fitted values ybar + T Q' [n x q], one thread
+// per row, written as float.
+struct PlsScoreParams {
+  const float* x = nullptr;
+  int n = 0, p = 0, q = 0, A = 0;
+  const double* x_mean = nullptr;
+  const double* x_sd = nullptr;
+  const double* y_mean = nullptr;
+  const double* weights = nullptr;   // [p x kPlsMaxComponents]
+  const double* yweights = nullptr;  // [q x kPlsMaxComponents]
+  double* tbuf = nullptr;            // [n x A] scratch: the scores in FP64
+  float* scores = nullptr;           // [n x A]
+  float* fitted = nullptr;           // [n x q]
+};
+
+hipError_t launch_pls_colstats(const PlsColStatsParams& prm, hipStream_t s);
+hipError_t launch_pls_colstats_reduce(const PlsColStatsParams& prm, int scale, hipStream_t s);
+hipError_t launch_pls_cross(const PlsCrossParams& prm, hipStream_t s);
+size_t pls_fit_lds_bytes(int p, int q);
+hipError_t launch_pls_fit(const PlsFitParams& prm, hipStream_t s);
+hipError_t launch_pls_scores(const PlsScoreParams& prm, hipStream_t s);
 
 size_t pls_eval_lds_bytes(int p, int max_classes);
 hipError_t launch_pls_group(const PlsGroupParams& prm, hipStream_t s);

@@ -218,25 +218,31 @@ std::vector<float> prepare_rows(const float* x, int n, int p, int metric, int va
 // min(k, available), as the reference search reports (native_knn.cpp:218-266).
 void exact_knn(const float* train, int nt, const float* query, int nq, int p, int k,
                int metric, const int32_t* exclude, int32_t* out_idx, float* out_dist) {
-  std::vector<std::pair<float, int>> cand;
-  cand.reserve(static_cast<size_t>(nt));
-  for (int q = 0; q < nq; ++q) {
-    cand.clear();
-    const int ex = exclude ? exclude[q] : -1;
-    for (int t = 0; t < nt; ++t) {
-      if (t == ex) continue;
-      cand.emplace_back(distance_of(query + static_cast<size_t>(q) * p,
-                                    train + static_cast<size_t>(t) * p, p, metric), t);
-    }
-    const int kk = std::min<int>(k, static_cast<int>(cand.size()));
-    std::partial_sort(cand.begin(), cand.begin() + kk, cand.end(),
-                      [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
-                        return a.first < b.first || (a.first == b.first && a.second < b.second);
-                      });
-    for (int j = 0; j < k; ++j) {
-      const size_t o = static_cast<size_t>(q) * k + j;
-      out_idx[o] = j < kk ? cand[j].second : -1;
-      out_dist[o] = j < kk ? cand[j].first : std::numeric_limits<float>::infinity();
+  // Queries are independent: one thread per query block, each with its own
+  // candidate list, so the result does not depend on the thread count.
+#pragma omp parallel if (static_cast<long long>(nq) * nt * p > (1ll << 24))
+  {
+    std::vector<std::pair<float, int>> cand;
+    cand.reserve(static_cast<size_t>(nt));
+#pragma omp for schedule(dynamic, 8)
+    for (int q = 0; q < nq; ++q) {
+      cand.clear();
+      const int ex = exclude ? exclude[q] : -1;
+      for (int t = 0; t < nt; ++t) {
+        if (t == ex) continue;
+        cand.emplace_back(distance_of(query + static_cast<size_t>(q) * p,
+                                      train + static_cast<size_t>(t) * p, p, metric), t);
+      }
+      const int kk = std::min<int>(k, static_cast<int>(cand.size()));
+      std::partial_sort(cand.begin(), cand.begin() + kk, cand.end(),
+                        [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+                          return a.first < b.first || (a.first == b.first && a.second < b.second);
+                        });
+      for (int j = 0; j < k; ++j) {
+        const size_t o = static_cast<size_t>(q) * k + j;
+        out_idx[o] = j < kk ? cand[j].second : -1;
+        out_dist[o] = j < kk ? cand[j].first : std::numeric_limits<float>::infinity();
+      }
     }
   }
 }
@@ -909,10 +915,10 @@ struct Degenerate : std::runtime_error { Degenerate() : std::runtime_error("dege
 // normalised, scaled to unit score norm, and the loading is Gram-Schmidt
 // orthogonalised against earlier loadings before S is deflated.
 template <class T>
-Mat<T> simpls_power(const Mat<T>& x, Mat<T> s, int max_components, const Mat<T>* gram) {
+Mat<T> simpls_power(const Mat<T>& x, Mat<T> s, int max_components, const Mat<T>* gram, Mat<T>* qout = nullptr) {
   const int rank_cap = std::min({max_components, x.c, std::max(1, x.r - 1)});
   const Mat<T> s0 = s;
-  Mat<T> w(x.c, rank_cap), vb(x.c, rank_cap);
+  Mat<T> w(x.c, rank_cap), vb(x.c, rank_cap), q(s.c, rank_cap);
   std::vector<T> prev;
   int fitted = 0;
   auto refresh = [](const Mat<T>& cp, std::vector<T>& cand) {
@@ -970,6 +976,9 @@ Mat<T> simpls_power(const Mat<T>& x, Mat<T> s, int max_components, const Mat<T>*
       for (T& v : wt) v *= inv;
       load = tmatvec(x, t);
     }
+    // Response weights q = S0' w of the scaled weight (plscv.cpp:905).
+    const std::vector<T> qa = tmatvec(s0, wt);
+    for (int c = 0; c < s.c; ++c) q(c, a) = qa[c];
     prev = wt;
     for (int b = 0; b < a; ++b) {
       T proj = T(0);
@@ -989,12 +998,109 @@ Mat<T> simpls_power(const Mat<T>& x, Mat<T> s, int max_components, const Mat<T>*
     for (int j = 0; j < s.r; ++j) for (int c = 0; c < s.c; ++c) s(j, c) -= load[j] * vs[c];
     fitted = a + 1;
   }
-  (void)s0;
   if (fitted < 1) throw Degenerate();
+  if (qout) {
+    *qout = Mat<T>(s.c, fitted);
+    for (int a = 0; a < fitted; ++a) for (int c = 0; c < s.c; ++c) (*qout)(c, a) = q(c, a);
+  }
   if (fitted == rank_cap) return w;
   Mat<T> out(x.c, fitted);
   for (int a = 0; a < fitted; ++a) for (int j = 0; j < x.c; ++j) out(j, a) = w(j, a);
   return out;
+}
+
+// Gauss-Jordan elimination with partial pivoting on a + 1e-9 I, columns whose
+// pivot is below 1e-12 skipped (solve_linear_float, plscv.cpp:1346-1377).
+template <class T>
+Mat<T> solve_linear(Mat<T> a, Mat<T> b) {
+  const int n = a.r, m = b.c;
+  for (int i = 0; i < n; ++i) a(i, i) += static_cast<T>(1.0e-9f);
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    T best = std::abs(a(col, col));
+    for (int r = col + 1; r < n; ++r) {
+      const T v = std::abs(a(r, col));
+      if (v > best) { best = v; piv = r; }
+    }
+    if (best < static_cast<T>(1.0e-12f)) continue;
+    if (piv != col) {
+      for (int j = 0; j < n; ++j) std::swap(a(col, j), a(piv, j));
+      for (int j = 0; j < m; ++j) std::swap(b(col, j), b(piv, j));
+    }
+    const T div = a(col, col);
+    for (int j = col; j < n; ++j) a(col, j) /= div;
+    for (int j = 0; j < m; ++j) b(col, j) /= div;
+    for (int r = 0; r < n; ++r) {
+      if (r == col) continue;
+      const T f = a(r, col);
+      if (f == T(0)) continue;
+      for (int j = col; j < n; ++j) a(r, j) -= f * a(col, j);
+      for (int j = 0; j < m; ++j) b(r, j) -= f * b(col, j);
+    }
+  }
+  return b;
+}
+
+// Scores X W[:, :A] with a row-sequential accumulation (transform_pls_scores_float,
+// plscv.cpp:995-1008).
+template <class T>
+Mat<T> transform_scores(const Mat<T>& x, const Mat<T>& w, int A) {
+  Mat<T> out(x.r, A);
+  for (int i = 0; i < x.r; ++i)
+    for (int j = 0; j < x.c; ++j) {
+      const T v = x(i, j);
+      for (int a = 0; a < A; ++a) out(i, a) += v * w(j, a);
+    }
+  return out;
+}
+
+// PLS-DA prediction (plscv.cpp:1210-1296): class frequencies ybar; B solves
+// (T'T) B = T'Y - (T'1) ybar' over the first A training scores (per-class
+// score sums in the working type, the rest accumulated in double); the
+// prediction is the first class of largest ybar_c + t'B[:, c].
+template <class T>
+IVec predict_pls_da(const Mat<T>& tt, const IVec& y, const Mat<T>& tv, const IVec& classes, int A) {
+  const int C = static_cast<int>(classes.size());
+  IVec code(y.size());
+  std::vector<T> ymean(C, T(0));
+  for (size_t i = 0; i < y.size(); ++i) {
+    code[i] = static_cast<int>(std::lower_bound(classes.begin(), classes.end(), y[i]) - classes.begin());
+    ymean[code[i]] += T(1);
+  }
+  const T inv = y.empty() ? T(0) : T(1) / static_cast<T>(y.size());
+  for (T& m : ymean) m *= inv;
+  Mat<T> lhs(A, A), rhs(A, C);
+  std::vector<double> sums(A, 0.0), acc(static_cast<size_t>(A) * A, 0.0);
+  for (int r = 0; r < tt.r; ++r) {
+    const int c = code[r];
+    for (int a = 0; a < A; ++a) {
+      const double v = static_cast<double>(tt(r, a));
+      rhs(a, c) += static_cast<T>(v);
+      sums[a] += v;
+      for (int b = a; b < A; ++b) acc[static_cast<size_t>(a) * A + b] += v * static_cast<double>(tt(r, b));
+    }
+  }
+  for (int a = 0; a < A; ++a)
+    for (int b = a; b < A; ++b) {
+      const T v = static_cast<T>(acc[static_cast<size_t>(a) * A + b]);
+      lhs(a, b) = v;
+      lhs(b, a) = v;
+    }
+  for (int a = 0; a < A; ++a)
+    for (int c = 0; c < C; ++c) rhs(a, c) -= static_cast<T>(sums[a] * static_cast<double>(ymean[c]));
+  const Mat<T> coef = solve_linear(lhs, rhs);
+  IVec pred(static_cast<size_t>(tv.r), classes.front());
+  for (int i = 0; i < tv.r; ++i) {
+    int best = 0;
+    T best_s = -std::numeric_limits<T>::infinity();
+    for (int c = 0; c < C; ++c) {
+      T sc = ymean[c];
+      for (int a = 0; a < A; ++a) sc += tv(i, a) * coef(a, c);
+      if (sc > best_s) { best_s = sc; best = c; }
+    }
+    pred[i] = classes[best];
+  }
+  return pred;
 }
 
 // Cholesky solve (plscv.cpp:1379-1412).
@@ -1096,7 +1202,7 @@ struct FoldOut {
 // 1544-1608). `gram` selects the Gram-matrix loading path of the fold cache.
 template <class T>
 FoldOut<T> pls_fold(const float* x, int p, const IVec& labels, const IVec& train, const IVec& val,
-                    const oracle_core_options* o, bool use_gram) {
+                    const oracle_core_options* o, bool use_gram, bool da = false) {
   FoldOut<T> out;
   std::vector<T> mean, sd;
   center_scale<T>(x, p, train, o->center != 0, o->scale != 0, mean, sd);
@@ -1132,6 +1238,15 @@ FoldOut<T> pls_fold(const float* x, int p, const IVec& labels, const IVec& train
   const int A = std::min(requested, w.c);
   out.components = A;
   out.weights = w;
+  if (da) {
+    // PLS-DA head (plscv.cpp:2430-2442): scores of every fitted component,
+    // the regression over the first A.
+    const Mat<T> tt = transform_scores(xt, w, w.c);
+    const Mat<T> tv = transform_scores(xv, w, w.c);
+    out.val_scores = tv;
+    out.pred = predict_pls_da(tt, y, tv, classes, A);
+    return out;
+  }
   const int C = static_cast<int>(classes.size());
   IVec code(y.size()), counts(C, 0);
   for (size_t i = 0; i < y.size(); ++i) {
@@ -1179,7 +1294,8 @@ FoldOut<T> pls_fold(const float* x, int p, const IVec& labels, const IVec& train
 // and the Gram loading path is used; otherwise folds are drawn from the labels.
 template <class T>
 IVec plsldacv(const float* x, int n, int p, const IVec& labels, const IVec& constrain,
-              const oracle_core_options* o, const IVec* cached_folds, IVec* folds_out, int* selected) {
+              const oracle_core_options* o, const IVec* cached_folds, IVec* folds_out, int* selected,
+              bool da = false) {
   if (o->max_components < 1) throw std::invalid_argument("PLSOptions::max_components must be positive.");
   const bool cache = o->stratified == 0 && o->pls_fold_cache != 0;
   const IVec folds = (cache && cached_folds) ? *cached_folds
@@ -1190,12 +1306,76 @@ IVec plsldacv(const float* x, int n, int p, const IVec& labels, const IVec& cons
   for (int f : sorted_unique(folds)) {
     IVec val, tr;
     for (int i = 0; i < n; ++i) (folds[i] == f ? val : tr).push_back(i);
-    const FoldOut<T> fo = pls_fold<T>(x, p, labels, tr, val, o, cache);
+    const FoldOut<T> fo = pls_fold<T>(x, p, labels, tr, val, o, cache, da);
     for (size_t i = 0; i < val.size(); ++i) pred[val[i]] = fo.pred[i];
     evaluated = std::min(evaluated, fo.components);
   }
   if (selected) *selected = std::min(evaluated, o->max_components);
   return pred;
+}
+
+// General PLS (fit_general_pls, plscv.cpp:2765-2878): X centred / scaled as
+// the training rows of a fold, Y centred by its float column means, S = X'Y
+// summed over rows in order, power-refresh SIMPLS without the Gram path,
+// scores T = X W, coefficients W Q' and fitted values ybar + T Q'.
+template <class T>
+struct PlsFit {
+  Mat<T> w, q, scores, coef, fitted;
+  std::vector<T> xc, xs, yc;
+};
+
+template <class T>
+PlsFit<T> pls_general(const float* x, const float* y, int n, int p, int q, const oracle_core_options* o) {
+  if (x == nullptr || y == nullptr || n < 2 || p < 1 || q < 1) {
+    throw std::invalid_argument("PLS requires non-empty X and Y matrices with at least two rows.");
+  }
+  const int requested = o->fixed_components > 0 ? o->fixed_components : o->max_components;
+  if (requested < 1) throw std::invalid_argument("PLS component count must be positive.");
+  IVec rows(static_cast<size_t>(n));
+  std::iota(rows.begin(), rows.end(), 0);
+  PlsFit<T> f;
+  center_scale<T>(x, p, rows, o->center != 0, o->scale != 0, f.xc, f.xs);
+  const Mat<T> xp = standardize<T>(x, p, rows, f.xc, f.xs);
+  Mat<T> yp(n, q);
+  f.yc.assign(q, T(0));
+  for (int i = 0; i < n; ++i)
+    for (int r = 0; r < q; ++r) {
+      const T v = static_cast<T>(y[static_cast<size_t>(i) * q + r]);
+      if (!std::isfinite(v)) throw std::invalid_argument("PLS Y must contain only finite values.");
+      f.yc[r] += v;
+      yp(i, r) = v;
+    }
+  if (o->center) {
+    const T inv = T(1) / static_cast<T>(n);
+    for (T& v : f.yc) v *= inv;
+    for (int i = 0; i < n; ++i) for (int r = 0; r < q; ++r) yp(i, r) -= f.yc[r];
+  }
+  const int comps = std::max(1, std::min({requested, p, std::max(1, n - 1)}));
+  Mat<T> cross(p, q);
+  for (int j = 0; j < p; ++j)
+    for (int r = 0; r < q; ++r) {
+      T v = T(0);
+      for (int i = 0; i < n; ++i) v += xp(i, j) * yp(i, r);
+      cross(j, r) = v;
+    }
+  f.w = simpls_power<T>(xp, cross, comps, nullptr, &f.q);
+  const int A = f.w.c;
+  f.scores = transform_scores(xp, f.w, A);
+  f.coef = Mat<T>(p, q);
+  for (int j = 0; j < p; ++j)
+    for (int r = 0; r < q; ++r) {
+      T v = T(0);
+      for (int a = 0; a < A; ++a) v += f.w(j, a) * f.q(r, a);
+      f.coef(j, r) = v;
+    }
+  f.fitted = Mat<T>(n, q);
+  for (int i = 0; i < n; ++i)
+    for (int r = 0; r < q; ++r) {
+      T v = f.yc[r];
+      for (int a = 0; a < A; ++a) v += f.scores(i, a) * f.q(r, a);
+      f.fitted(i, r) = v;
+    }
+  return f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1694,6 +1874,47 @@ int oracle_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
     std::memcpy(pred, r.data(), sizeof(int32_t) * n);
     copy_fold_out(folds, folds_out);
     if (selected_components) *selected_components = sel;
+  });
+}
+
+// PLSDACV (plscv.cpp:2926-2928): the PLS-LDA CV with the PLS-DA head.
+int oracle_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels, const int32_t* constrain,
+                   const oracle_core_options* opt, int32_t precision, int32_t* pred, int32_t* folds_out,
+                   int32_t* selected_components) {
+  return guarded([&] {
+    const IVec lab(labels, labels + n);
+    IVec folds;
+    int sel = 1;
+    const IVec r = precision == 32
+      ? plsldacv<float>(x, n, p, lab, to_vec(constrain, n), opt, nullptr, &folds, &sel, true)
+      : plsldacv<double>(x, n, p, lab, to_vec(constrain, n), opt, nullptr, &folds, &sel, true);
+    std::memcpy(pred, r.data(), sizeof(int32_t) * n);
+    copy_fold_out(folds, folds_out);
+    if (selected_components) *selected_components = sel;
+  });
+}
+
+// PLS_CPU (plscv.cpp:2899-2901). Outputs in double: weights [p x A], response
+// loadings [q x A], scores [n x A], coefficients [p x q], fitted [n x q],
+// centres / scales; *components = A (buffers sized for the requested count).
+int oracle_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q, const oracle_core_options* opt,
+               int32_t precision, double* weights, double* response_loadings, double* scores,
+               double* coefficients, double* fitted, double* x_center, double* x_scale, double* y_center,
+               int32_t* components) {
+  return guarded([&] {
+    auto emit = [&](const auto& f) {
+      const int A = f.w.c;
+      *components = A;
+      for (int j = 0; j < p; ++j) for (int a = 0; a < A; ++a) weights[static_cast<size_t>(j) * A + a] = f.w(j, a);
+      for (int r = 0; r < q; ++r) for (int a = 0; a < A; ++a) response_loadings[static_cast<size_t>(r) * A + a] = f.q(r, a);
+      for (int i = 0; i < n; ++i) for (int a = 0; a < A; ++a) scores[static_cast<size_t>(i) * A + a] = f.scores(i, a);
+      for (size_t e = 0; e < f.coef.v.size(); ++e) coefficients[e] = f.coef.v[e];
+      for (size_t e = 0; e < f.fitted.v.size(); ++e) fitted[e] = f.fitted.v[e];
+      for (int j = 0; j < p; ++j) { x_center[j] = f.xc[j]; x_scale[j] = f.xs[j]; }
+      for (int r = 0; r < q; ++r) y_center[r] = f.yc[r];
+    };
+    if (precision == 32) emit(pls_general<float>(x, y, n, p, q, opt));
+    else emit(pls_general<double>(x, y, n, p, q, opt));
   });
 }
 

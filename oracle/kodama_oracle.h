@@ -117,6 +117,26 @@ int oracle_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
                     const int32_t* constrain, const oracle_core_options* opt,
                     int32_t precision, int32_t* pred, int32_t* folds_out,
                     int32_t* selected_components);
+/* PLSDACV: the F-fold PLS-DA CV (same folds and SIMPLS, least-squares head). */
+int oracle_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                   const int32_t* constrain, const oracle_core_options* opt,
+                   int32_t precision, int32_t* pred, int32_t* folds_out,
+                   int32_t* selected_components);
+/* PLS_CPU: general PLS of y [n x q] on x [n x p]; outputs in double, A =
+ * *components columns: weights [p x A], response_loadings [q x A], scores
+ * [n x A], coefficients [p x q], fitted [n x q], x_center / x_scale [p],
+ * y_center [q]. */
+int oracle_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
+               const oracle_core_options* opt, int32_t precision, double* weights,
+               double* response_loadings, double* scores, double* coefficients, double* fitted,
+               double* x_center, double* x_scale, double* y_center, int32_t* components);
+int ref_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                const int32_t* constrain, const oracle_core_options* opt,
+                int32_t* pred, int32_t* folds_out, int32_t* selected_components);
+int ref_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
+            const oracle_core_options* opt, double* weights, double* response_loadings,
+            double* scores, double* coefficients, double* fitted, double* x_center,
+            double* x_scale, double* y_center, int32_t* components);
 int oracle_pls_fold_detail(const float* x, int32_t n, int32_t p, const int32_t* labels,
                            const int32_t* folds, int32_t fold, const oracle_core_options* opt,
                            int32_t precision, double* weights, int32_t* rank,

@@ -407,6 +407,60 @@ def _oracle_kodama_matrix(self, x, runs, cycles, landmarks, splitting=0, folds=5
 Oracle.kodama_matrix = _oracle_kodama_matrix
 
 
+def _pls_buffers(n, p, q, opt):
+    A = max(1, opt.fixed_components if opt.fixed_components > 0 else opt.max_components)
+    return dict(weights=np.zeros(p * A), response_loadings=np.zeros(q * A), scores=np.zeros(n * A),
+                coefficients=np.zeros((p, q)), fitted=np.zeros((n, q)), x_center=np.zeros(p),
+                x_scale=np.zeros(p), y_center=np.zeros(q))
+
+
+def _pls_result(buf, n, p, q, comps):
+    A = comps.value
+    out = dict(buf)
+    out["components"] = A
+    out["weights"] = buf["weights"][: p * A].reshape(p, A)
+    out["response_loadings"] = buf["response_loadings"][: q * A].reshape(q, A)
+    out["scores"] = buf["scores"][: n * A].reshape(n, A)
+    return out
+
+
+def _pls_call(lib, name, x, y, opt, extra=()):
+    x, y = _f32(x), _f32(y)
+    if y.ndim == 1:
+        y = y[:, None]
+    n, p = x.shape
+    q = y.shape[1]
+    buf = _pls_buffers(n, p, q, opt)
+    comps = C.c_int32(0)
+    lib.call(name, _p(x, C.c_float), C.c_int32(n), C.c_int32(p), _p(y, C.c_float), C.c_int32(q), C.byref(opt),
+             *extra, *(_p(buf[k], C.c_double) for k in ("weights", "response_loadings", "scores", "coefficients",
+                                                         "fitted", "x_center", "x_scale", "y_center")),
+             C.byref(comps))
+    return _pls_result(buf, n, p, q, comps)
+
+
+def _oracle_plsdacv(self, x, labels, opt, precision=64, constrain=None):
+    """PLSDACV (plscv.cpp:2926-2928): the PLS-LDA CV folds and SIMPLS with the
+    least-squares PLS-DA head."""
+    x, lab = _f32(x), _i32(labels)
+    pred = np.zeros(lab.size, np.int32)
+    folds = np.zeros(lab.size, np.int32)
+    sel = C.c_int32(0)
+    self._l.call("plsdacv", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), _p(lab, C.c_int32),
+                 _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt), C.c_int32(precision), _p(pred, C.c_int32),
+                 _p(folds, C.c_int32), C.byref(sel))
+    return pred, folds, sel.value
+
+
+def _oracle_pls(self, x, y, opt, precision=64):
+    """General PLS (PLS_CPU, plscv.cpp:2765-2901)."""
+    return _pls_call(self._l, "pls", x, y, opt, (C.c_int32(precision),))
+
+
+Oracle.plsdacv = _oracle_plsdacv
+Oracle.pls = _oracle_pls
+
+
 def _oracle_plslda_predict(self, train, labels, test, opt, precision=64):
     """PLSLDAPredict_CPU (plscv.cpp:2976-3005) restated as the validation fold
     of a two-fold split of [train; test]: its training rows are all of train,
@@ -495,6 +549,21 @@ class Reference(_Api):
                      _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt),
                      _p(pred, C.c_int32), _p(folds, C.c_int32), C.byref(sel))
         return pred, folds, sel.value
+
+    def plsdacv(self, x, labels, opt: CoreOptions, constrain=None):
+        """kodama::PLSDACV_CPU (plscv.cpp:2926-2928)."""
+        x, lab = _f32(x), _i32(labels)
+        pred = np.zeros(lab.size, np.int32)
+        folds = np.zeros(lab.size, np.int32)
+        sel = C.c_int32(0)
+        self._l.call("plsdacv", _p(x, C.c_float), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]),
+                     _p(lab, C.c_int32), _ptr_or_null(_i32(constrain), C.c_int32), C.byref(opt),
+                     _p(pred, C.c_int32), _p(folds, C.c_int32), C.byref(sel))
+        return pred, folds, sel.value
+
+    def pls(self, x, y, opt: CoreOptions):
+        """kodama::PLS_CPU (plscv.cpp:2899-2901), outputs widened to double."""
+        return _pls_call(self._l, "pls", x, y, opt)
 
     def core_plslda(self, x, labels, opt: CoreOptions, constrain=None, fixed=None):
         x = _f32(x)

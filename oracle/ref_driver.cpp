@@ -15,6 +15,7 @@
 
 #include "kodama_oracle.h"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -219,6 +220,43 @@ int ref_plsldacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
     std::memcpy(pred, r.predicted_labels.data(), sizeof(int32_t) * n);
     std::memcpy(folds_out, r.fold_assignments.data(), sizeof(int32_t) * n);
     *selected_components = r.selected_components;
+  });
+}
+
+// kodama::PLSDACV_CPU (plscv.cpp:2926-2928)
+int ref_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labels,
+                const int32_t* constrain, const oracle_core_options* opt,
+                int32_t* pred, int32_t* folds_out, int32_t* selected_components) {
+  return guarded([&] {
+    const kodama::MatrixView xv(x, static_cast<size_t>(n), static_cast<size_t>(p));
+    kodama::PLSOptions po = core_options(opt).pls;
+    po.data_epoch = 0;
+    const kodama::PLSCVResult r = kodama::PLSDACV_CPU(
+      xv, std::vector<int>(labels, labels + n), vec_or_empty(constrain, n), po);
+    std::memcpy(pred, r.predicted_labels.data(), sizeof(int32_t) * n);
+    std::memcpy(folds_out, r.fold_assignments.data(), sizeof(int32_t) * n);
+    *selected_components = r.selected_components;
+  });
+}
+
+// kodama::PLS_CPU (plscv.cpp:2899-2901); outputs widened to double.
+int ref_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
+            const oracle_core_options* opt, double* weights, double* response_loadings,
+            double* scores, double* coefficients, double* fitted, double* x_center,
+            double* x_scale, double* y_center, int32_t* components) {
+  return guarded([&] {
+    const kodama::MatrixView xv(x, static_cast<size_t>(n), static_cast<size_t>(p));
+    const kodama::MatrixView yv(y, static_cast<size_t>(n), static_cast<size_t>(q));
+    const kodama::PLSFitResult r = kodama::PLS_CPU(xv, yv, core_options(opt).pls);
+    *components = r.components;
+    std::copy(r.weights.begin(), r.weights.end(), weights);
+    std::copy(r.response_loadings.begin(), r.response_loadings.end(), response_loadings);
+    std::copy(r.scores.begin(), r.scores.end(), scores);
+    std::copy(r.coefficients.begin(), r.coefficients.end(), coefficients);
+    std::copy(r.fitted.begin(), r.fitted.end(), fitted);
+    std::copy(r.x_center.begin(), r.x_center.end(), x_center);
+    std::copy(r.x_scale.begin(), r.x_scale.end(), x_scale);
+    std::copy(r.y_center.begin(), r.y_center.end(), y_center);
   });
 }
 

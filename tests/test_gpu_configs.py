@@ -1,0 +1,87 @@
+"""The large configurations of the north star on the device, at sizes the
+oracle checks in seconds:
+
+* C4 (1,000,000 x 50 generator, exact 30-NN graph by split-BF16 MFMA
+  candidates and an exact FP32 rerank): the graph of 100,000 rows, every
+  neighbour list of 2,000 seeded queries against the oracle's brute force over
+  all rows (ids and public distances bit-identical); PLS-LDA with 10
+  components on 20,000 of its rows against the FP64 oracle (weights, scores,
+  discriminants within 1e-4 relative).
+* C5 (200,000 x 50, 30-cluster mixture, 5 % label-noise starting labels):
+  the device-resident and learned forms on a 40,000-row slice with uneven
+  landmark runs, held to the invariants the algorithm fixes: the reported
+  accuracy is the oracle's vote of the returned labels, labels stay within
+  the starting classes, the best-accuracy history is monotone and ends at the
+  reported value, and the noise is repaired (accuracy above the starting
+  agreement).
+"""
+import numpy as np
+import pytest
+
+import kodama_amd as K
+import oracle as O
+from kodama_amd import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c4_exact_graph_sampled_against_brute_force(oracle):
+    x, _ = D.config4(n=100000)
+    g = K.KODAMA_knn_graph(x, k=30)
+    gi, gd = g["indices"], g["distances"]
+    assert gi.shape == (100000, 30)
+    assert K.last_knn_fallbacks() >= 0  # the MFMA candidate path served the graph
+    q = np.sort(np.argsort(D.uniform(77, 1, x.shape[0]), kind="stable")[:2000]).astype(np.int32)
+    idx, d2 = oracle.exact_knn(x, x[q], 30, 2, exclude=q)
+    assert np.array_equal(gi[q] - 1, idx)
+    assert np.array_equal(gd[q], np.sqrt(np.maximum(d2, 0.0)).astype(np.float32))
+
+
+def test_c4_plslda_ten_components_within_tolerance(oracle):
+    x, comp = D.config4(n=20000)
+    folds = (np.argsort(D.uniform(78, 1, x.shape[0]), kind="stable") % 10).astype(np.int32)
+    kw = dict(classifier=0, folds=10, stratified=0, max_components=10, fixed_components=10, center=1, scale=1,
+              pls_fold_cache=1)
+    for fold in (0, 7):
+        dev = K.plslda_fold_detail(x, comp, folds, fold, K.core_options(**kw))
+        ref = oracle.pls_fold_detail(x, comp, folds, fold, O.core_options(**kw), precision=64)
+        assert dev["rank"] == ref["rank"] == 10
+        assert np.array_equal(dev["classes"], ref["classes"])
+        for key in ("weights", "val_scores", "discriminant"):
+            a, b = np.asarray(dev[key]), np.asarray(ref[key])
+            scale = np.maximum(np.abs(b), np.max(np.abs(b)) * 1e-6)
+            assert np.max(np.abs(a - b) / scale) <= 1e-4, key
+
+
+def _c5_runs(n_runs=16, n=40000, seed=8):
+    x, noisy, comp = D.config5()
+    x, noisy, comp = x[:n], noisy[:n], comp[:n]
+    rows, init, folds = [], [], []
+    for r in range(n_runs):
+        L = int(D.randint(seed, 300 + r, 1, 1000, 5000)[0])
+        rw = np.sort(np.argsort(D.uniform(seed, 400 + r, n), kind="stable")[:L]).astype(np.int32)
+        rows.append(rw)
+        init.append(noisy[rw].astype(np.int32))
+        folds.append((np.argsort(D.uniform(seed, 500 + r, L), kind="stable") % 10).astype(np.int32))
+    return x, comp, rows, init, folds
+
+
+@pytest.mark.parametrize("form", ["device", "learned"])
+def test_c5_label_noise_forms_invariants(oracle, form):
+    x, comp, rows, init, folds = _c5_runs()
+    o = K.core_options(cycles=30, k=30, folds=10, stratified=0, seed=5, cv_seed=5, evolutionary_search=1,
+                       guarded_diversity=1, many_to_one_absorption=1)
+    if form == "device":
+        out = K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=folds)
+    else:
+        out = K.core_knn_batch_learned(x, rows, init, o, K.mlp_init(11), fallback=0.25, fold_ids=folds)
+    start = []
+    for r in range(len(rows)):
+        nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], 30)
+        cl = out["clbest"][r]
+        assert out["accbest"][r] == np.mean(oracle.knn_vote(nbr, score, folds[r], cl) == cl), r
+        assert set(np.unique(cl)) <= set(np.unique(init[r]))
+        assert np.all(np.diff(out["vect_acc"][r]) >= 0)
+        assert out["vect_acc"][r][-1] == out["accbest"][r]
+        start.append(np.mean(oracle.knn_vote(nbr, score, folds[r], init[r]) == init[r]))
+    assert np.mean(out["accbest"]) > np.mean(start) + 0.01

@@ -64,19 +64,20 @@ def test_dissimilarity_edge_cases(oracle, runs):
 
 
 @pytest.mark.parametrize("metric", ["euclidean", "cosine"])
-def test_graph_handle_materialize_and_dissimilarity(metric):
-    """The device-resident graph handle: materialised it is the exact graph of
-    KODAMA_knn_graph (1-based, public distances); its in-place dissimilarity
-    equals KODAMA_dissimilarity on the host copy; released twice is harmless."""
+def test_graph_handle_materialize_and_dissimilarity(oracle, metric):
+    """The device-resident graph handle against the oracle: materialised it is
+    the oracle's exact public graph (1-based, public distances); after its
+    in-place dissimilarity it is the oracle's dissimilarity rewrite of that
+    graph; released twice is harmless."""
     x, _, _ = D.config1(n=900, seed=17)
-    g = K.KODAMA_knn_graph(x, k=12, metric=metric)
-    want_idx, want_dist = g["indices"], g["distances"]
+    m = K._native.METRICS[metric]
+    want_idx, want_dist = oracle.knn_graph(x, 12, m)
     h = K.KODAMAGraphHandle(x, k=12, metric=metric)
     assert (h.samples, h.neighbors) == (900, 12)
     idx, dist = h.materialize()
     assert np.array_equal(idx, want_idx) and np.array_equal(dist, want_dist)
     res = np.stack([D.randint(60 + r, 1, 900, 0, 5) for r in range(6)]).astype(np.int32)
-    ref_idx, ref_dist = K.KODAMA_dissimilarity(want_idx, want_dist, res)
+    ref_idx, ref_dist = oracle.dissimilarity(want_idx, want_dist, res)
     h.dissimilarity(res)
     idx2, dist2 = h.materialize()
     assert np.array_equal(idx2, ref_idx) and np.array_equal(dist2, ref_dist)
@@ -85,4 +86,3 @@ def test_graph_handle_materialize_and_dissimilarity(metric):
     assert not h.valid()
     with pytest.raises(ValueError):
         h.materialize()
-

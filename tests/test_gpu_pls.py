@@ -182,3 +182,77 @@ def test_plslda_predict_edge_cases():
     with pytest.raises(ValueError):
         K.PLSLDAPredict(train, np.arange(30) % 2, test[:, :5], ncomp=3)
 
+
+
+# ---- PLSDACV and general PLS ----------------------------------------------------
+
+def test_plsdacv_matches_reference_and_oracle(oracle):
+    """PLSDACV on the device (FP64): predictions and selected components equal
+    the reference's PLSDACV_CPU fixtures and the FP64 oracle."""
+    from _cases import pls_general_inputs
+    g = golden("pls_general.npz")
+    x, truth, noisy, _ = pls_general_inputs()
+    for name, lab in (("truth", truth), ("noisy", noisy)):
+        for strat in (1, 0):
+            for nc in (2, 5):
+                key = f"dacv_{name}_s{strat}_c{nc}"
+                got = K.PLSDACV(x, lab, folds=10, stratified=bool(strat), seed=4, ncomp=nc)
+                opt = O.core_options(folds=10, stratified=strat, cv_seed=4, max_components=nc, fixed_components=nc)
+                want, folds, sel = oracle.plsdacv(x, lab, opt, precision=64)
+                assert np.array_equal(got["fold_assignments"], folds), key
+                assert np.array_equal(got["predicted_labels"], want), key
+                assert np.array_equal(got["predicted_labels"], g[key]), key
+                assert got["selected_components"] == sel == int(g[key + "_sel"]), key
+
+
+def test_plsdacv_edge_cases(oracle):
+    """One training class per fold predicts it; a constant matrix gives the
+    degenerate fit's training majority (components 1); both as the oracle."""
+    x = np.random.default_rng(3).standard_normal((60, 5)).astype(np.float32)
+    one = np.full(60, 7, np.int32)
+    got = K.PLSDACV(x, one, folds=5, seed=2, ncomp=3)
+    assert np.all(got["predicted_labels"] == 7)
+    const = np.ones((60, 5), np.float32)
+    lab = (np.arange(60) % 3 + 1).astype(np.int32)
+    got = K.PLSDACV(const, lab, folds=5, seed=2, ncomp=3)
+    opt = O.core_options(folds=5, stratified=1, cv_seed=2, max_components=3, fixed_components=3)
+    want, _, sel = oracle.plsdacv(const, lab, opt, precision=64)
+    assert np.array_equal(got["predicted_labels"], want)
+    assert got["selected_components"] == sel == 1
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_pls_general_matches_oracle_fp64_and_reference(oracle, case):
+    """General PLS on the device (cross-products on the matrix cores, FP64):
+    every output within 1e-4 relative of the FP64 oracle (observed ~1e-12) and
+    of the reference's FP32 PLS_CPU fixture; the same component count."""
+    from _cases import PLS_FIELDS, PLS_GENERAL_CASES, pls_general_inputs, rel_err
+    g = golden("pls_general.npz")
+    x, _, _, y = pls_general_inputs()
+    nc, center, scale = PLS_GENERAL_CASES[case]
+    got = K.PLS(x, y, ncomp=nc, center=bool(center), scale=bool(scale))
+    opt = O.core_options(max_components=nc, fixed_components=nc, center=center, scale=scale)
+    want = oracle.pls(x, y, opt, precision=64)
+    tag = f"pls_{nc}_{center}{scale}"
+    assert got["components"] == want["components"] == int(g[tag + "_components"])
+    for key in PLS_FIELDS:
+        assert rel_err(got[key], want[key]) < 1e-6, (tag, key, rel_err(got[key], want[key]))
+        assert rel_err(got[key], g[f"{tag}_{key}"].reshape(np.asarray(got[key]).shape)) < RTOL, (tag, key)
+
+
+def test_pls_general_large_and_wide(oracle):
+    """200,000 rows (many row chunks of the cross-product kernel) and 70
+    predictors with 20 responses (several 16 x 16 output tiles per wave and
+    padded edges): within 1e-6 relative of the FP64 oracle."""
+    from _cases import PLS_FIELDS, rel_err
+    rng = np.random.default_rng(8)
+    for n, p, q, nc in ((200000, 50, 3, 10), (3000, 70, 20, 6)):
+        x = rng.standard_normal((n, p)).astype(np.float32)
+        y = (x[:, :5] @ rng.standard_normal((5, q)) + rng.standard_normal((n, q))).astype(np.float32)
+        got = K.PLS(x, y, ncomp=nc)
+        want = oracle.pls(x, y, O.core_options(max_components=nc, fixed_components=nc, center=1, scale=1),
+                          precision=64)
+        assert got["components"] == want["components"]
+        for key in PLS_FIELDS:
+            assert rel_err(got[key], want[key]) < 1e-6, (n, p, q, key, rel_err(got[key], want[key]))
+        assert got["cross_seconds"] > 0.0

@@ -147,3 +147,84 @@ def test_knn_graph_brute_force_public_form(oracle):
     order = np.lexsort((np.broadcast_to(np.arange(n), d2.shape), d2), axis=1)[:, :k]
     assert np.array_equal(gi, order + 1)
     assert np.array_equal(gd, np.sqrt(np.take_along_axis(d2, order, 1)).astype(np.float32))
+
+
+# ---- PLSDACV and general PLS (plscv.cpp:1210-1296, 2765-2938) ---------------------
+
+def test_plsdacv_matches_reference_golden(oracle):
+    """The oracle's PLS-DA CV: float arithmetic bit-identical to the
+    reference's PLSDACV_CPU predictions and selected components; the FP64
+    form (the device arithmetic) gives the same predictions here."""
+    from _cases import pls_general_inputs
+    g = golden("pls_general.npz")
+    x, truth, noisy, _ = pls_general_inputs()
+    for name, lab in (("truth", truth), ("noisy", noisy)):
+        for strat in (1, 0):
+            for nc in (2, 5):
+                opt = O.core_options(folds=10, stratified=strat, cv_seed=4, max_components=nc, fixed_components=nc)
+                key = f"dacv_{name}_s{strat}_c{nc}"
+                for prec in (32, 64):
+                    pred, _, sel = oracle.plsdacv(x, lab, opt, precision=prec)
+                    assert np.array_equal(pred, g[key]), (key, prec)
+                    assert sel == int(g[key + "_sel"]), (key, prec)
+
+
+def test_pls_general_matches_reference_golden(oracle):
+    """The oracle's general PLS: float arithmetic bit-identical to the
+    reference's PLS_CPU fit (weights, response loadings, scores, coefficients,
+    fitted values, centres and scales); FP64 within 1e-4 relative of it."""
+    from _cases import PLS_FIELDS, PLS_GENERAL_CASES, pls_general_inputs, rel_err
+    g = golden("pls_general.npz")
+    x, _, _, y = pls_general_inputs()
+    assert np.array_equal(y, g["y"])
+    for nc, center, scale in PLS_GENERAL_CASES:
+        opt = O.core_options(max_components=nc, fixed_components=nc, center=center, scale=scale)
+        f32 = oracle.pls(x, y, opt, precision=32)
+        f64 = oracle.pls(x, y, opt, precision=64)
+        tag = f"pls_{nc}_{center}{scale}"
+        assert f32["components"] == f64["components"] == int(g[tag + "_components"])
+        for key in PLS_FIELDS:
+            want = g[f"{tag}_{key}"].reshape(np.asarray(f32[key]).shape)
+            assert np.array_equal(np.asarray(f32[key], np.float32), want), (tag, key)
+            assert rel_err(f64[key], want) < 1e-4, (tag, key)
+
+
+def test_pls_general_closed_forms(oracle):
+    """Closed forms that pin the general PLS without the reference: one
+    response and one component give w proportional to X'y with unit score
+    norm and q = w'X'y; with as many components as columns the coefficients
+    are the least-squares solution of the standardised problem."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((200, 6)).astype(np.float32)
+    y = (x @ rng.standard_normal(6) + 0.1 * rng.standard_normal(200)).astype(np.float32)
+    opt = O.core_options(max_components=1, fixed_components=1, center=1, scale=1)
+    f = oracle.pls(x, y, opt, precision=64)
+    xs = (x.astype(np.float64) - f["x_center"]) / f["x_scale"]
+    yc = y.astype(np.float64) - f["y_center"][0]
+    s = xs.T @ yc
+    w = f["weights"][:, 0]
+    assert np.allclose(w / np.linalg.norm(w), s / np.linalg.norm(s), atol=1e-10)
+    assert abs(np.linalg.norm(xs @ w) - 1.0) < 1e-10
+    assert abs(f["response_loadings"][0, 0] - w @ s) < 1e-8 * abs(w @ s)
+    opt6 = O.core_options(max_components=6, fixed_components=6, center=1, scale=1)
+    f6 = oracle.pls(x, y, opt6, precision=64)
+    ls = np.linalg.lstsq(xs, yc, rcond=None)[0]
+    assert np.allclose(f6["coefficients"][:, 0], ls, rtol=1e-5, atol=1e-8)
+
+
+def test_plsdacv_and_pls_match_live_reference(oracle, reference):
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((400, 9)).astype(np.float32)
+    lab = rng.integers(1, 5, 400).astype(np.int32)
+    x[lab == 2, 0] += 1.5
+    opt = O.core_options(folds=5, stratified=1, cv_seed=9, max_components=4, fixed_components=3)
+    a = oracle.plsdacv(x, lab, opt, precision=32)
+    b = reference.plsdacv(x, lab, opt)
+    assert all(np.array_equal(u, v) for u, v in zip(a, b))
+    y = rng.standard_normal((400, 3)).astype(np.float32)
+    opt = O.core_options(max_components=4, fixed_components=0, center=1, scale=1)
+    f = oracle.pls(x, y, opt, precision=32)
+    r = reference.pls(x, y, opt)
+    assert f["components"] == r["components"]
+    for key in ("weights", "scores", "coefficients", "fitted"):
+        assert np.array_equal(np.asarray(f[key], np.float32), np.asarray(r[key], np.float32)), key
