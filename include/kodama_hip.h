@@ -291,6 +291,23 @@ int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distance
                                      const int32_t* run_labels, int32_t runs,
                                      int32_t device);
 
+/* No counterpart in the reference: the allocation-free twin of
+ * kodama_hip_dissimilarity_inplace, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same rewrite, on caller-owned DEVICE buffers:
+ * d_indices_one_based / d_distances [samples x neighbors] in place,
+ * d_run_labels [runs x samples]; enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) of `device`, which the call makes current, and not waited
+ * for. The transposed run labels (samples x runs rounded up to 4, int32) live
+ * in a library-owned static device array of 64 MiB per GPU, held until the
+ * process ends: larger inputs are EINVAL. The call allocates nothing (no
+ * hipMalloc, hipFree or pool allocation). Calls that overlap on one GPU share
+ * that array and race; serialise them (one stream, or wait between calls). */
+int kodama_hip_dissimilarity_inplace_noalloc(int32_t* d_indices_one_based, float* d_distances,
+                                             int32_t samples, int32_t neighbors,
+                                             const int32_t* d_run_labels, int32_t runs,
+                                             int32_t device, void* stream);
+
 /* ---- device-resident graph handle --------------------------------------------
  * kodama::KODAMAGraphHandle + KODAMAGraphMaterialize (kodama.hpp:481-530;
  * kodama_matrix.cpp:2422-2490, 4364-4381) for the exact self-kNN graph: the

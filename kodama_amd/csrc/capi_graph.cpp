@@ -84,6 +84,32 @@ int kodama_hip_dissimilarity_inplace(int32_t* indices_one_based, float* distance
   });
 }
 
+// The allocation-free twin of kodama_hip_dissimilarity_inplace: the caller's
+// device buffers, the caller's stream, and the transposed labels in the
+// library's static scratch (no hipMalloc, hipFree or pool call).
+int kodama_hip_dissimilarity_inplace_noalloc(int32_t* d_indices_one_based, float* d_distances, int32_t samples,
+                                             int32_t neighbors, const int32_t* d_run_labels, int32_t runs,
+                                             int32_t device, void* stream) {
+  return guarded([&] {
+    if (runs <= 0 || samples <= 0 || neighbors <= 0) return;  // the reference leaves the graph as is
+    require(d_indices_one_based != nullptr && d_distances != nullptr && d_run_labels != nullptr, "null pointer.");
+    require(dissimilarity_lds_bytes(neighbors) <= 64 * 1024, "dissimilarity supports up to 1365 neighbours per row.");
+    const int stride = (runs + 3) / 4 * 4;
+    select_device(device);
+    int* labels = nullptr;
+    size_t bytes = 0;
+    check(noalloc_scratch(device, &labels, &bytes), "scratch address");
+    require(static_cast<size_t>(samples) * stride * sizeof(int) <= bytes,
+            "samples x runs (rounded up to 4) exceeds the 64 MiB scratch of the allocation-free dissimilarity.");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // The transpose writes every entry the rewrite reads (padding included),
+    // so the scratch needs no clearing between calls.
+    check(launch_labels_transpose(d_run_labels, runs, samples, stride, labels, s), "labels_transpose launch");
+    check(launch_dissimilarity(d_indices_one_based, d_distances, samples, neighbors, labels, stride, true, true, s),
+          "dissimilarity launch");
+  });
+}
+
 // ---- device-resident graph handle --------------------------------------------
 // KODAMAGraphHandle / KODAMAGraphMaterialize (kodama.hpp:481-530,
 // kodama_matrix.cpp:2422-2490, 4364-4381): the exact graph stays on the device

@@ -13,8 +13,15 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "graph_launch.h"
+
+// Scratch of the allocation-free dissimilarity entry point: the transposed run
+// labels [samples x stride] int32 (stride = runs rounded up to 4), 64 MiB per
+// GPU, held until the process ends. Host code never names it directly: the
+// per-device address comes from noalloc_scratch below.
+static __device__ __align__(256) int scratch[kodama_hip::kNoallocScratchBytes / sizeof(int)];
 
 namespace kodama_hip {
 
@@ -145,6 +152,29 @@ hipError_t launch_labels_transpose(const int* res, int runs, int samples, int st
   const dim3 grid(static_cast<unsigned>((samples + 31) / 32), static_cast<unsigned>((stride + 31) / 32));
   hipLaunchKernelGGL(labels_transpose_kernel, grid, dim3(32, 8), 0, s, res, runs, samples, stride, out);
   return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(int device, int** ptr, size_t* bytes) {
+  // Resolved once per device, with that device current: every GPU holds its
+  // own copy of `scratch` at its own address.
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static int* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[device] == nullptr) {
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<int*>(p);
+  }
+  *ptr = addr[device];
+  *bytes = kNoallocScratchBytes;
+  return hipSuccess;
 }
 
 size_t dissimilarity_lds_bytes(int neighbors) {

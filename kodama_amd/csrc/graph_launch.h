@@ -7,6 +7,12 @@
 
 namespace kodama_hip {
 
+// Size of the static device scratch of the allocation-free dissimilarity.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+// Device address and size of that scratch on `device`, which must be the
+// current device (resolved with hipGetSymbolAddress on first use per device).
+hipError_t noalloc_scratch(int device, int** ptr, size_t* bytes);
+
 // In place: internal self-kNN distances and 0-based ids -> public distances and
 // 1-based ids (missing entries: -1 / +inf). metric as KODAMA_HIP_* .
 hipError_t launch_graph_publish(int* idx, float* dist, size_t total, int metric, hipStream_t s);
