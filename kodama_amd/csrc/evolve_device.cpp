@@ -36,6 +36,7 @@ struct DeviceEvolution::Impl {
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
+  bool large = false;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -66,10 +67,13 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.classes = batch.classes();
   m.cycles_cap = std::max(1, options.cycles);
   if (m.classes > 1024) throw InvalidArgument("device-resident evolution supports up to 1024 classes per run.");
-  if (batch.max_rows() > kEvoMaxRows) {
-    throw InvalidArgument("device-resident evolution supports up to 8191 rows per run.");
+  // Runs whose row queue and flow list do not fit LDS beside their labels take
+  // the large-run layout (those two in global scratch), up to 16383 rows.
+  m.large = batch.max_rows() > kEvoMaxRows || evo_lds_bytes(m.classes, batch.max_rows()) > kEvoMaxLds;
+  if (batch.max_rows() > kEvoMaxRowsLarge) {
+    throw InvalidArgument("device-resident evolution supports up to 16383 rows per run.");
   }
-  if (evo_lds_bytes(m.classes, batch.max_rows()) > kEvoMaxLds) {
+  if (evo_lds_bytes(m.classes, batch.max_rows(), false, m.large) > kEvoMaxLds) {
     throw InvalidArgument("device-resident evolution: the run state does not fit the LDS of one compute unit.");
   }
   const size_t flow_items = static_cast<size_t>(m.runs) * m.classes * m.classes;
@@ -91,6 +95,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   // The learned proposal's network: fp32 parameters and their tiled form.
   m.off_params = lay.take(sizeof(float) * kMlpParams);
   m.off_packed = lay.take(sizeof(uint32_t) * kMlpTiled);
+  // Large-run layout: each run's flow list and row queue.
+  const size_t off_large = m.large ? lay.take(sizeof(int) * 2 * static_cast<size_t>(m.runs) * batch.max_rows()) : 0;
   m.up_begin = off_cur;
   m.up_bytes = off_key + sizeof(uint64_t) * m.runs - off_cur;
   m.down_begin = m.off_best;
@@ -155,6 +161,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.state = reinterpret_cast<double*>(base + m.off_state);
   p.flows = reinterpret_cast<int*>(base + off_flows);
   p.history = reinterpret_cast<double*>(base + m.off_history);
+  p.large_scratch = m.large ? reinterpret_cast<int*>(base + off_large) : nullptr;
   if (!pageable.empty()) check(hipStreamSynchronize(s), "setup sync");  // the image dies with this scope
   impl_ = impl.release();
 }
@@ -182,7 +189,7 @@ void DeviceEvolution::run_learned(int cycles, const float* params, float fallbac
   Impl& m = *impl_;
   if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
   const hipStream_t s = m.batch.stream();
-  if (evo_lds_bytes(m.classes, m.batch.max_rows(), true) > kEvoMaxLds) {
+  if (evo_lds_bytes(m.classes, m.batch.max_rows(), true, m.large) > kEvoMaxLds) {
     throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
   }
   float* d_params = reinterpret_cast<float*>(m.arena.get() + m.off_params);

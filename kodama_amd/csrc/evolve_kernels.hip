@@ -196,7 +196,7 @@ enum Scalar { kPresent, kPairs, kMatches, kChosen, kVanished, kPick, kDecision, 
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
 
-__device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows) {
+__device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows, int* large_rows) {
   Lds s;
   double* d = reinterpret_cast<double*>(base);
   s.wl = d;
@@ -212,9 +212,16 @@ __device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows) {
   s.point = i + 4 * C;
   s.list = i + 5 * C;
   s.remap = i + 6 * C;
-  s.pairs = i + 7 * C;
-  s.queue = s.pairs + max_rows;
-  s.scal = s.queue + max_rows;
+  if (large_rows != nullptr) {
+    // Large-run variant: the flow list and the row queue in global scratch.
+    s.pairs = large_rows;
+    s.queue = large_rows + max_rows;
+    s.scal = i + 7 * C;
+  } else {
+    s.pairs = i + 7 * C;
+    s.queue = s.pairs + max_rows;
+    s.scal = s.queue + max_rows;
+  }
   s.labels = reinterpret_cast<unsigned short*>(s.scal + 16);
   s.lmax = label_slots(max_rows);
   const size_t used = static_cast<size_t>(reinterpret_cast<unsigned char*>(s.labels + 4 * s.lmax) - base);
@@ -461,14 +468,16 @@ __device__ __forceinline__ double objective(const Lds& s, bool guarded, int C, i
 
 }  // namespace
 
-template <int KMAX, int THREADS, bool LEARNED>
+template <int KMAX, int THREADS, bool LEARNED, bool LARGE>
 __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
-  // Rows per thread of the largest run, for the register arrays of the propose phase.
+  // Rows per thread and pass of the flow counting, for its register arrays
+  // (one pass covers kEvoMaxRows rows at 1024 threads).
   constexpr int kRowsPerThread = (kEvoMaxRows + THREADS - 1) / THREADS;
   extern __shared__ __align__(16) double lds_raw[];
   const int C = p.classes;
-  const Lds s = carve(reinterpret_cast<unsigned char*>(lds_raw), C, p.max_rows);
   const int run = blockIdx.x;
+  const Lds s = carve(reinterpret_cast<unsigned char*>(lds_raw), C, p.max_rows,
+                      LARGE ? p.large_scratch + static_cast<size_t>(run) * 2 * p.max_rows : nullptr);
   const int n = p.run_offsets[run + 1] - p.run_offsets[run];
   const int base = p.pad_offsets[run];
   const unsigned short* __restrict__ nbr = p.neighbors + base;
@@ -664,18 +673,21 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           if (label == guide) atomicAdd(&s.stay[label], 1);
           s.queue[i] = label == guide ? -1 : label * C + guide;
         }
-        // Flow counts: all of a thread's returning atomics are in flight before
-        // the first result is used; the first increment of an entry lists it.
-        int fkey[kRowsPerThread], old[kRowsPerThread];
+        // Flow counts: all of a thread's returning atomics of a pass are in
+        // flight before the first result is used; the first increment of an
+        // entry lists it. One pass covers kRowsPerThread * THREADS rows.
+        for (int pass0 = 0; pass0 < n; pass0 += kRowsPerThread * THREADS) {
+          int fkey[kRowsPerThread], old[kRowsPerThread];
 #pragma unroll
-        for (int r = 0; r < kRowsPerThread; ++r) {
-          const int i = tid + r * THREADS;
-          fkey[r] = i < n ? s.queue[i] : -1;
-          old[r] = fkey[r] >= 0 ? atomicAdd(&flows[fkey[r]], 1) : 1;
-        }
+          for (int r = 0; r < kRowsPerThread; ++r) {
+            const int i = pass0 + tid + r * THREADS;
+            fkey[r] = i < n ? s.queue[i] : -1;
+            old[r] = fkey[r] >= 0 ? atomicAdd(&flows[fkey[r]], 1) : 1;
+          }
 #pragma unroll
-        for (int r = 0; r < kRowsPerThread; ++r) {
-          if (old[r] == 0) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = fkey[r];
+          for (int r = 0; r < kRowsPerThread; ++r) {
+            if (old[r] == 0) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = fkey[r];
+          }
         }
       }
       __syncthreads();
@@ -711,17 +723,45 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           // Read and clear in one L2 atomic: the propose-phase atomics of this
           // launch bypass the vector L1, which may hold an older copy.
           const int cnt = atomicExch(&flows[idx], 0);
+          if (LARGE) s.queue[q] = cnt;  // the second pass below needs it again
           if (!eligible) continue;
           const int src = idx / C;
           const int d = idx - src * C;
           const int dn = s.sizes[d];
           if (dn == 0) continue;
           const long long surplus_n = static_cast<long long>(cnt) * n - static_cast<long long>(s.sizes[src]) * dn;
-          const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 27)) << 36) |
-                                         (static_cast<unsigned long long>(cnt) << 23) |
-                                         (static_cast<unsigned long long>(dn) << 10) |
-                                         static_cast<unsigned long long>(1023 - d);
-          atomicMax(&s.best_key[src], k64);
+          if (LARGE) {
+            // Up to 16383 rows: surplus in 29 bits, flow and target size in 14
+            // each; the label is settled in a second pass.
+            const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 28)) << 28) |
+                                           (static_cast<unsigned long long>(cnt) << 14) |
+                                           static_cast<unsigned long long>(dn);
+            atomicMax(&s.best_key[src], k64);
+          } else {
+            const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 27)) << 36) |
+                                           (static_cast<unsigned long long>(cnt) << 23) |
+                                           (static_cast<unsigned long long>(dn) << 10) |
+                                           static_cast<unsigned long long>(1023 - d);
+            atomicMax(&s.best_key[src], k64);
+          }
+        }
+        if (LARGE && eligible) {
+          // Second pass: among the targets of a source's best key, the
+          // smallest label (kept in `point`, which is -1 until the consumer).
+          __syncthreads();
+          for (int q = tid; q < np; q += THREADS) {
+            const int idx = s.pairs[q];
+            const int cnt = s.queue[q];
+            const int src = idx / C;
+            const int d = idx - src * C;
+            const int dn = s.sizes[d];
+            if (dn == 0) continue;
+            const long long surplus_n = static_cast<long long>(cnt) * n - static_cast<long long>(s.sizes[src]) * dn;
+            const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 28)) << 28) |
+                                           (static_cast<unsigned long long>(cnt) << 14) |
+                                           static_cast<unsigned long long>(dn);
+            if (k64 == s.best_key[src]) atomicMin(reinterpret_cast<unsigned int*>(&s.point[src]), static_cast<unsigned int>(d));
+          }
         }
       }
       __syncthreads();
@@ -731,9 +771,18 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           const int src = s.list[e];
           const unsigned long long k64 = s.best_key[src];
           if (k64 == 0ull) continue;
-          const long long surplus_n = static_cast<long long>(k64 >> 36) - (1ll << 27);
-          const int cnt = static_cast<int>((k64 >> 23) & 0x1FFFull);
-          const int d = 1023 - static_cast<int>(k64 & 0x3FFull);
+          long long surplus_n;
+          int cnt, d;
+          if (LARGE) {
+            surplus_n = static_cast<long long>(k64 >> 28) - (1ll << 28);
+            cnt = static_cast<int>((k64 >> 14) & 0x3FFFull);
+            d = s.point[src];
+            s.point[src] = -1;  // the smallest label of the best key, read once
+          } else {
+            surplus_n = static_cast<long long>(k64 >> 36) - (1ll << 27);
+            cnt = static_cast<int>((k64 >> 23) & 0x1FFFull);
+            d = 1023 - static_cast<int>(k64 & 0x3FFull);
+          }
           if (surplus_n > 0 && cnt > s.stay[src]) {
             s.point[src] = d;
             atomicAdd(&s.weight[d], static_cast<unsigned long long>(surplus_n));
@@ -923,10 +972,11 @@ __global__ void evo_zero_kernel(int* a, size_t n) {
 
 // ---- launchers --------------------------------------------------------------
 
-size_t evo_lds_bytes(int classes, int max_rows, bool learned) {
+size_t evo_lds_bytes(int classes, int max_rows, bool learned, bool large) {
+  const size_t rows_lds = large ? 0 : 2 * static_cast<size_t>(max_rows);
   const size_t base = sizeof(double) * (static_cast<size_t>(classes) + kWaves + 4) +
                       sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
-                      sizeof(int) * (7 * static_cast<size_t>(classes) + 2 * static_cast<size_t>(max_rows) + 16) +
+                      sizeof(int) * (7 * static_cast<size_t>(classes) + rows_lds + 16) +
                       sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
   if (!learned) return base;
   return base + 16 + sizeof(uint32_t) * kMlpTiled +
@@ -937,32 +987,35 @@ size_t evo_lds_bytes(int classes, int max_rows, bool learned) {
 // neighbours; 512 for 64, whose id and key arrays need 256 registers; 768 for
 // the learned variant up to 32 (its matrix-core tile pass needs about 140
 // registers, over the 128 of 16 waves).
-template <int KMAX, bool LEARNED>
+template <int KMAX, bool LEARNED, bool LARGE>
 static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStream_t s) {
   constexpr int threads = KMAX > 32 ? kThreads / 2 : (LEARNED ? kLearnedThreads : kThreads);
-  const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED>);
+  const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>);
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED>), dim3(p.runs), dim3(threads), shmem, s, p);
+  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>), dim3(p.runs), dim3(threads), shmem, s,
+                     p);
   return hipGetLastError();
 }
 
-template <bool LEARNED>
+template <bool LEARNED, bool LARGE>
 static hipError_t launch_persistent(const EvoParams& p, size_t shmem, hipStream_t s) {
-  if (p.k <= 8) return launch_persistent_k<8, LEARNED>(p, shmem, s);
-  if (p.k <= 16) return launch_persistent_k<16, LEARNED>(p, shmem, s);
-  if (p.k <= 32) return launch_persistent_k<32, LEARNED>(p, shmem, s);
-  return launch_persistent_k<64, LEARNED>(p, shmem, s);
+  if (p.k <= 8) return launch_persistent_k<8, LEARNED, LARGE>(p, shmem, s);
+  if (p.k <= 16) return launch_persistent_k<16, LEARNED, LARGE>(p, shmem, s);
+  if (p.k <= 32) return launch_persistent_k<32, LEARNED, LARGE>(p, shmem, s);
+  return launch_persistent_k<64, LEARNED, LARGE>(p, shmem, s);
 }
 
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s) {
   const bool learned = p.learn_packed != nullptr;
-  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned);
-  if (shmem > kEvoMaxLds || p.max_rows > kEvoMaxRows || p.classes > 1024 || p.k < 1 || p.k > 64 ||
-      p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31)) {
+  const bool large = p.large_scratch != nullptr;
+  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned, large);
+  if (shmem > kEvoMaxLds || p.max_rows > (large ? kEvoMaxRowsLarge : kEvoMaxRows) || p.classes > 1024 ||
+      p.k < 1 || p.k > 64 || p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31)) {
     return hipErrorInvalidValue;
   }
-  return learned ? launch_persistent<true>(p, shmem, s) : launch_persistent<false>(p, shmem, s);
+  if (large) return learned ? launch_persistent<true, true>(p, shmem, s) : launch_persistent<false, true>(p, shmem, s);
+  return learned ? launch_persistent<true, false>(p, shmem, s) : launch_persistent<false, false>(p, shmem, s);
 }
 
 hipError_t launch_evo_narrow(const int* nbr32, const int* run_offsets, const int* pad_offsets, const int* block_run,

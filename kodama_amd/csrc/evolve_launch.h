@@ -15,6 +15,11 @@ constexpr int kEvoAccepted = 1;
 constexpr int kEvoImproved = 2;
 constexpr int kEvoStateStride = 4;  // cur_acc, cur_score, best_acc, best_score
 constexpr int kEvoMaxRows = 8191;   // rows of one run (13-bit fields of the absorption keys)
+// Runs of up to kEvoMaxRowsLarge rows take the large-run variant: the row
+// queue and the distinct flow list live in a per-run global scratch instead
+// of LDS, and the absorption keys are compared in two passes (surplus, flow,
+// target size in 57 bits, then the smallest label).
+constexpr int kEvoMaxRowsLarge = 16383;
 
 struct EvoParams {
   int runs = 0;
@@ -25,7 +30,7 @@ struct EvoParams {
   int init = 0;           // evaluate the starting labels first
   int guarded = 1;        // diversity-guarded objective
   int k = 0;
-  int max_rows = 0;       // largest run, <= kEvoMaxRows
+  int max_rows = 0;       // largest run, <= kEvoMaxRows (kEvoMaxRowsLarge with large_scratch)
   int pad_total = 0;
   const int* run_offsets = nullptr;          // [runs + 1]
   const int* pad_offsets = nullptr;          // [runs + 1]
@@ -45,9 +50,13 @@ struct EvoParams {
   // the uniform proposal.
   const uint32_t* learn_packed = nullptr;
   float learn_fallback = 0.0f;
+  // Large-run variant: [runs x 2 x max_rows] ints (distinct flow entries, then
+  // the row queue of each run); null for the LDS-resident layout.
+  int* large_scratch = nullptr;
 };
 
-size_t evo_lds_bytes(int classes, int max_rows, bool learned = false);
+// LDS of one run-workgroup; `large` leaves the row queue and the flow list out.
+size_t evo_lds_bytes(int classes, int max_rows, bool learned = false, bool large = false);
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
 // 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids; a
 // missing neighbour (-1) becomes the run's row count, its sentinel slot.

@@ -287,14 +287,17 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
   DeviceBuffer<float> dx(static_cast<size_t>(n) * p), dy(static_cast<size_t>(n) * q);
   dx.upload(x, dx.size(), s);
   dy.upload(y, dy.size(), s);
-  DeviceBuffer<double> xm(p), xsd(p), ym(q), ysd(q);
+  // ym centres Y; yreport is the reference's y_center: the column means when
+  // centring, else the column sums (fit_general_pls accumulates them and
+  // divides only when centring), which the fitted values start from.
+  DeviceBuffer<double> xm(p), xsd(p), ym(q), ysd(q), ysum(q);
   EventTimer t_all, t_cross;
   t_all.start(s);
   // Column statistics in row chunks, reduced in chunk order.
   const int st_chunks = std::max(1, std::min(2048, (n + 63) / 64));
   const int st_rows = (n + st_chunks - 1) / st_chunks;
   DeviceBuffer<double> partial(static_cast<size_t>(st_chunks) * std::max(p, q));
-  auto stats = [&](const float* m, int cols, double* mean, double* sd, int scale) {
+  auto stats = [&](const float* m, int cols, double* mean, double* sd, int scale, double* sum) {
     PlsColStatsParams c;
     c.m = m;
     c.n = n;
@@ -304,16 +307,19 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
     c.center = o.center != 0 ? 1 : 0;
     c.mean = mean;
     c.sd = sd;
+    c.sum = sum;
     c.partial = partial.get();
     c.pass = 0;
     check(launch_pls_colstats(c, s), "pls_colstats launch");
     check(launch_pls_colstats_reduce(c, scale, s), "pls_colstats reduce");
     c.pass = 1;
+    c.sum = nullptr;
     if (scale) check(launch_pls_colstats(c, s), "pls_colstats launch");
     check(launch_pls_colstats_reduce(c, scale, s), "pls_colstats reduce");
   };
-  stats(dx.get(), p, xm.get(), xsd.get(), o.scale != 0 ? 1 : 0);
-  stats(dy.get(), q, ym.get(), ysd.get(), 0);
+  stats(dx.get(), p, xm.get(), xsd.get(), o.scale != 0 ? 1 : 0, nullptr);
+  stats(dy.get(), q, ym.get(), ysd.get(), 0, ysum.get());
+  const double* yreport = o.center != 0 ? ym.get() : ysum.get();
   // Cross-products on the matrix cores.
   const int p16 = (p + 15) / 16 * 16, z16 = (p + q + 15) / 16 * 16;
   const int cr_rows = std::max(16, ((n + 1023) / 1024 + 15) / 16 * 16);
@@ -370,7 +376,7 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
   sp.A = A;
   sp.x_mean = xm.get();
   sp.x_sd = xsd.get();
-  sp.y_mean = ym.get();
+  sp.y_mean = yreport;
   sp.weights = W.get();
   sp.yweights = Q.get();
   sp.tbuf = tbuf.get();
@@ -386,7 +392,7 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
   B.download(hb.data(), hb.size(), s);
   xm.download(hxm.data(), hxm.size(), s);
   xsd.download(hxs.data(), hxs.size(), s);
-  ym.download(hym.data(), hym.size(), s);
+  check(hipMemcpyAsync(hym.data(), yreport, sizeof(double) * q, hipMemcpyDeviceToHost, s), "y_center download");
   r.scores.resize(dsc.size());
   r.fitted.resize(dft.size());
   dsc.download(r.scores.data(), r.scores.size(), s);

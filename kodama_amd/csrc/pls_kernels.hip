@@ -653,6 +653,7 @@ __global__ void pls_colstats_reduce_kernel(PlsColStatsParams prm, int scale) {
   for (int k = 0; k < prm.chunks; ++k) t += prm.partial[static_cast<size_t>(k) * prm.cols + c];
   if (prm.pass == 0) {
     prm.mean[c] = prm.center ? t / static_cast<double>(prm.n) : 0.0;
+    if (prm.sum) prm.sum[c] = t;
   } else {
     double sd = 1.0;
     if (scale) {

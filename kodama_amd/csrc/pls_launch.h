@@ -105,6 +105,7 @@ struct PlsColStatsParams {
   int center = 1;
   double* mean = nullptr;      // [cols]
   double* sd = nullptr;        // [cols] (pass 1)
+  double* sum = nullptr;       // [cols] (pass 0, may be null) the column sums
   double* partial = nullptr;   // [chunks x cols]
 };
 

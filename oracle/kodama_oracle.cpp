@@ -10,19 +10,31 @@
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
 // library. The product path never does.
 //
-// Pinning (what checks each function; the tests live in tests/test_oracle_*.py):
-//   oracle_make_folds        vs reference make_folds (oracle/_ref) + fold-size invariants
-//   oracle_distance          vs reference HNSW distances on exhaustive searches (_ref)
-//   oracle_exact_knn         vs brute force on tiny inputs, reference exhaustive HNSW (_ref)
-//   oracle_knn_vote          hand-built ties (count, score, label) + empty-row fallback
+// Pinning (what checks each function; tests/test_oracle.py and
+// tests/test_oracle_matrix.py, fixtures in tests/golden/ written by the
+// compiled reference through oracle/_ref):
+//   oracle_make_folds        reference make_folds fixtures (24 cases) + fold-size invariants
+//   oracle_exact_knn         brute force with integer coordinates and ties; reference exhaustive HNSW fixtures
+//   oracle_knn_vote          hand-built ties (count, score, label) + empty-row fallback; double score sums
 //   oracle_objective         closed form acc*sqrt(1-sum p^2), single-class -inf
-//   oracle_core_knn          trajectories vs reference CoreKNN_CPU (_ref, golden fixtures)
-//   oracle_knncv             vs reference KNNCV_CPU (_ref, golden fixtures)
-//   oracle_knn_graph         brute force on tiny inputs; reference HNSW graph recall
-//   oracle_dissimilarity     brute-force co-assignment counts + reference (_ref)
-//   oracle_plsldacv (fp32)   vs reference PLSLDACV_CPU (_ref), bit-exact predictions
-//   oracle_plsldacv (fp64)   two-class closed form (rank-1 S), w'Gw = 1, orthonormal loadings
-//   oracle_core_plslda       trajectories vs reference CorePLSLDA_CPU (_ref)
+//   oracle_core_knn          CoreKNN_CPU trajectory fixtures (Core API and KODAMA.matrix flags); live reference
+//   oracle_knncv             KNNCV_CPU fixtures (cosine, Euclidean)
+//   oracle_knn_graph         live KODAMAKNNGraph_CPU (three metrics, 3 / 6 / 40 columns); brute force with ties
+//   oracle_dissimilarity     KODAMADissimilarityInPlace fixtures + brute-force co-assignment counts; live reference
+//   oracle_plsldacv (fp32)   PLSLDACV_CPU fixtures, bit-exact; live reference
+//   oracle_plsldacv (fp64)   two-class closed form (rank-1 S), G-orthonormal weights, one-component LDA
+//   oracle_core_plslda       CorePLSLDA_CPU trajectory fixtures (fp32)
+//   oracle_plsdacv           PLSDACV_CPU fixtures, bit-exact (fp32; fp64 the same predictions); live reference
+//   oracle_pls               PLS_CPU fixtures, bit-exact (fp32), fp64 within 1e-4; closed forms (rank one,
+//                            full-rank least squares)
+//   oracle_kmeans_labels,    the reference's own kmeans_labels / quota_sample_landmarks fixtures and live calls
+//   oracle_quota_sample
+//   oracle_normalize_graph   the reference's normalize_external_graph, live, every index base; a hand-worked case
+//   oracle_kodama_matrix,    KODAMAMatrix_CPU (PLS-LDA) and KODAMAMatrixFromGraphData_CPU (kNN) fixtures:
+//   oracle_kodama_matrix_    res, acc, v and the run diagnostics' hashes (landmarks, starting labels, folds)
+//   from_graph               bit-exact, with starting labels, constraint groups and fixed rows; live
+//                            reference on fresh seeds; the exact-graph projection against a numpy brute force
+//   PLSLDAPredict (fp32/64)  PLSLDAPredict_CPU fixtures (true and random labels, 2 / 5 / 8 components)
 
 #include "kodama_oracle.h"
 

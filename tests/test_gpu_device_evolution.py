@@ -80,3 +80,48 @@ def test_device_is_deterministic():
     for r in range(4):
         assert np.array_equal(a["clbest"][r], b["clbest"][r])
     assert np.array_equal(a["accbest"], b["accbest"])
+
+
+def test_device_runs_beyond_8191_rows(oracle):
+    """Runs of 10,000 rows (the reference's default landmarks, kodama.hpp:536)
+    and of 16,383 rows take the large-run layout (row queue and flow list in
+    global scratch, two-pass absorption keys). The uniform and learned forms
+    keep the invariants: the reported accuracy is the oracle's vote of the
+    returned labels, labels stay within the starting classes, the history is
+    monotone and ends at the reported value."""
+    n = 40000
+    x, _ = D.config3(n=n, seed=31)
+    rows, init, folds = [], [], []
+    for r, L in enumerate((10000, 16383)):
+        rw = np.sort(np.argsort(D.uniform(31, 900 + r, n), kind="stable")[:L]).astype(np.int32)
+        c = x[rw[np.argsort(D.uniform(31, 950 + r, L), kind="stable")[:60]]]
+        idx, _ = K.exact_knn_search(c, x[rw], 1, "euclidean")
+        rows.append(rw)
+        init.append((idx[:, 0] + 1).astype(np.int32))
+        folds.append((np.argsort(D.uniform(31, 980 + r, L), kind="stable") % 10).astype(np.int32))
+    o = K.core_options(**dict(OPTS, cycles=12, k=30))
+    dev = K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=folds)
+    lrn = K.core_knn_batch_learned(x, rows[:1], init[:1], o, K.mlp_init(5), fallback=0.25, fold_ids=folds[:1])
+    for out, count in ((dev, 2), (lrn, 1)):
+        for r in range(count):
+            nbr, score = oracle.precompute_knn_cv(x[rows[r]], folds[r], 30)
+            cl = out["clbest"][r]
+            assert out["accbest"][r] == np.mean(oracle.knn_vote(nbr, score, folds[r], cl) == cl), r
+            assert set(np.unique(cl)) <= set(np.unique(init[r]))
+            assert np.all(np.diff(out["vect_acc"][r]) >= 0)
+            assert out["vect_acc"][r][-1] == out["accbest"][r]
+
+
+def test_matrix_device_form_at_default_landmarks():
+    """KODAMA_matrix(form="device") with the default landmarks = 10000 on
+    20,000 rows: the same landmarks and graph as the plain form, labels in
+    range, accuracy at the plain form's level."""
+    x, _ = D.config3(n=20000, seed=41)
+    kw = dict(M=2, Tcycle=6, folds=5, k=10, graph_neighbors=30, seed=5)
+    plain = K.KODAMA_matrix(x, form="plain", **kw)
+    dev = K.KODAMA_matrix(x, form="device", **kw)
+    assert plain["landmarks"].shape == (2, 10000)
+    assert np.array_equal(plain["landmarks"], dev["landmarks"])
+    assert np.array_equal(plain["knn_indices"], dev["knn_indices"])
+    assert dev["res"].shape == (2, 20000) and dev["res"].min() >= 1
+    assert abs(dev["acc"].mean() - plain["acc"].mean()) < 0.05
