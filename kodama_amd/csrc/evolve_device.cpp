@@ -30,13 +30,18 @@ struct DeviceEvolution::Impl {
   KnnBatch& batch;
   std::vector<IVec> values;
   int runs = 0, classes = 0, cycles_cap = 0;
+  uint64_t seed = 0;
   PoolBuffer arena;
   size_t up_begin = 0, up_bytes = 0, down_begin = 0, down_bytes = 0;
   size_t off_best = 0, off_state = 0, off_history = 0, off_params = 0, off_packed = 0;
+  size_t off_cur = 0, off_bits = 0, off_key = 0;
   EvoParams prm;
   int done_cycles = 0;
   bool initialized = false;
   bool large = false;
+  bool learned = false;
+  EventTimer timer;
+  bool timed = false;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -54,9 +59,8 @@ struct ArenaLayout {
 
 }  // namespace
 
-DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values, const std::vector<IVec>& codes,
-                                 const std::vector<IVec>& folds, const kodama_hip_core_options& options) {
-  (void)folds;
+DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values,
+                                 const kodama_hip_core_options& options, const float* params) {
   if (options.evolutionary_search == 0) {
     throw InvalidArgument("the device-resident evolution implements the evolutionary KODAMA.matrix loop.");
   }
@@ -66,6 +70,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.runs = batch.runs();
   m.classes = batch.classes();
   m.cycles_cap = std::max(1, options.cycles);
+  m.seed = options.seed;
+  m.learned = params != nullptr;
   if (m.classes > 1024) throw InvalidArgument("device-resident evolution supports up to 1024 classes per run.");
   // Runs whose row queue and flow list do not fit LDS beside their labels take
   // the large-run layout (those two in global scratch), up to 16383 rows.
@@ -76,6 +82,9 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   if (evo_lds_bytes(m.classes, batch.max_rows(), false, m.large) > kEvoMaxLds) {
     throw InvalidArgument("device-resident evolution: the run state does not fit the LDS of one compute unit.");
   }
+  if (m.learned && evo_lds_bytes(m.classes, batch.max_rows(), true, m.large) > kEvoMaxLds) {
+    throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
+  }
   const size_t flow_items = static_cast<size_t>(m.runs) * m.classes * m.classes;
   const int pad = batch.pad_total();
   const size_t label_bytes = sizeof(unsigned short) * static_cast<size_t>(pad);
@@ -85,9 +94,9 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   const size_t off_nbr = lay.take(sizeof(unsigned short) * static_cast<size_t>(batch.k()) * pad);
   const size_t off_flows = lay.take(sizeof(int) * flow_items);
   const size_t off_curpred = lay.take(label_bytes);
-  const size_t off_cur = lay.take(label_bytes);  // upload: cur .. run_key
-  const size_t off_bits = lay.take(sizeof(int) * m.runs);
-  const size_t off_key = lay.take(sizeof(uint64_t) * m.runs);
+  m.off_cur = lay.take(label_bytes);  // upload: cur .. run_key
+  m.off_bits = lay.take(sizeof(int) * m.runs);
+  m.off_key = lay.take(sizeof(uint64_t) * m.runs);
   const size_t off_bestpred = lay.take(label_bytes);
   m.off_best = lay.take(label_bytes);  // download: best .. history
   m.off_state = lay.take(sizeof(double) * m.runs * kEvoStateStride);
@@ -97,8 +106,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.off_packed = lay.take(sizeof(uint32_t) * kMlpTiled);
   // Large-run layout: each run's flow list and row queue.
   const size_t off_large = m.large ? lay.take(sizeof(int) * 2 * static_cast<size_t>(m.runs) * batch.max_rows()) : 0;
-  m.up_begin = off_cur;
-  m.up_bytes = off_key + sizeof(uint64_t) * m.runs - off_cur;
+  m.up_begin = m.off_cur;
+  m.up_bytes = m.off_key + sizeof(uint64_t) * m.runs - m.off_cur;
   m.down_begin = m.off_best;
   m.down_bytes = m.off_history + sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap - m.off_best;
   int device = 0;
@@ -106,39 +115,16 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   // Rounded up to 2 MiB so that evolutions of similar size reuse one pool block.
   m.arena.allocate((lay.total + (2u << 20) - 1) / (2u << 20) * (2u << 20), device, s);
   unsigned char* base = m.arena.get();
-
-  // Host image of the upload region: starting labels (their predictions start
-  // equal to them and are rewritten by the first evaluation), widths, keys.
-  // Built in the batch's pinned staging when it fits (an asynchronous copy the
-  // launch follows in stream order), else in pageable memory.
-  std::vector<unsigned char> pageable;
-  unsigned char* img = batch.stage_up(m.up_bytes);
-  if (img == nullptr) {
-    pageable.assign(m.up_bytes, 0);
-    img = pageable.data();
-  }
-  auto* h_init = reinterpret_cast<unsigned short*>(img);
-  auto* h_bits = reinterpret_cast<int*>(img + (off_bits - off_cur));
-  auto* h_key = reinterpret_cast<uint64_t*>(img + (off_key - off_cur));
-  // The runs' rows are disjoint slices of the image: filled by host threads.
-#pragma omp parallel for schedule(static)
-  for (int r = 0; r < m.runs; ++r) {
-    for (size_t i = 0; i < codes[r].size(); ++i) {
-      h_init[batch.pad_offset(r) + i] = static_cast<unsigned short>(codes[r][i]);
-    }
-    const int n = static_cast<int>(codes[r].size());
-    int bits = 1;
-    while ((1 << bits) < n) ++bits;
-    h_bits[r] = bits;
-    h_key[r] = splitmix(options.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
-  }
-  check(hipMemcpyAsync(base + m.up_begin, img, m.up_bytes, hipMemcpyHostToDevice, s), "upload");
-  check(hipMemcpyAsync(base + off_curpred, base + off_cur, label_bytes, hipMemcpyDeviceToDevice, s), "copy");
   check(launch_evo_zero(reinterpret_cast<int*>(base + off_flows), flow_items, s), "evo_zero");
   check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
                           batch.d_block_run().get(), reinterpret_cast<unsigned short*>(base + off_nbr),
                           static_cast<size_t>(batch.k()) * pad, pad, s),
         "evo_narrow");
+  if (m.learned) {
+    float* d_params = reinterpret_cast<float*>(base + m.off_params);
+    check(hipMemcpyAsync(d_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s), "upload params");
+    check(launch_mlp_pack(d_params, reinterpret_cast<uint32_t*>(base + m.off_packed), s), "mlp_pack");
+  }
 
   EvoParams& p = m.prm;
   p.runs = m.runs;
@@ -152,9 +138,9 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.pad_offsets = batch.d_pad_offsets().get();
   p.neighbors = reinterpret_cast<const unsigned short*>(base + off_nbr);
   p.scores = batch.d_scores().get();
-  p.run_key = reinterpret_cast<const uint64_t*>(base + off_key);
-  p.perm_bits = reinterpret_cast<const int*>(base + off_bits);
-  p.cur = reinterpret_cast<unsigned short*>(base + off_cur);
+  p.run_key = reinterpret_cast<const uint64_t*>(base + m.off_key);
+  p.perm_bits = reinterpret_cast<const int*>(base + m.off_bits);
+  p.cur = reinterpret_cast<unsigned short*>(base + m.off_cur);
   p.curpred = reinterpret_cast<unsigned short*>(base + off_curpred);
   p.best = reinterpret_cast<unsigned short*>(base + m.off_best);
   p.bestpred = reinterpret_cast<unsigned short*>(base + off_bestpred);
@@ -162,54 +148,76 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.flows = reinterpret_cast<int*>(base + off_flows);
   p.history = reinterpret_cast<double*>(base + m.off_history);
   p.large_scratch = m.large ? reinterpret_cast<int*>(base + off_large) : nullptr;
-  if (!pageable.empty()) check(hipStreamSynchronize(s), "setup sync");  // the image dies with this scope
+  p.learn_packed = m.learned ? reinterpret_cast<const uint32_t*>(base + m.off_packed) : nullptr;
+  // The set-up is complete before the timed evolution starts.
+  check(hipStreamSynchronize(s), "setup sync");
   impl_ = impl.release();
 }
 
 DeviceEvolution::~DeviceEvolution() { delete impl_; }
 
-void DeviceEvolution::run(int cycles) {
+void DeviceEvolution::start(const std::vector<IVec>& codes) {
   Impl& m = *impl_;
+  if (static_cast<int>(codes.size()) != m.runs) throw InvalidArgument("one label vector per run.");
+  const hipStream_t s = m.batch.stream();
+  // Host image of the upload region: starting labels (the kernel's first
+  // evaluation writes their predictions), permutation widths, run keys. Built
+  // in the batch's pinned staging when it fits (an asynchronous copy the
+  // launch follows in stream order), else in pageable memory.
+  std::vector<unsigned char> pageable;
+  unsigned char* img = m.batch.stage_up(m.up_bytes);
+  if (img == nullptr) {
+    pageable.assign(m.up_bytes, 0);
+    img = pageable.data();
+  }
+  auto* h_init = reinterpret_cast<unsigned short*>(img);
+  auto* h_bits = reinterpret_cast<int*>(img + (m.off_bits - m.off_cur));
+  auto* h_key = reinterpret_cast<uint64_t*>(img + (m.off_key - m.off_cur));
+  // The runs' rows are disjoint slices of the image: filled by host threads.
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < m.runs; ++r) {
+    for (size_t i = 0; i < codes[r].size(); ++i) {
+      h_init[m.batch.pad_offset(r) + i] = static_cast<unsigned short>(codes[r][i]);
+    }
+    const int n = static_cast<int>(codes[r].size());
+    int bits = 1;
+    while ((1 << bits) < n) ++bits;
+    h_bits[r] = bits;
+    h_key[r] = splitmix(m.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
+  }
+  check(hipMemcpyAsync(m.arena.get() + m.up_begin, img, m.up_bytes, hipMemcpyHostToDevice, s), "upload");
+  if (!pageable.empty()) check(hipStreamSynchronize(s), "upload sync");  // the image dies with this scope
+}
+
+namespace {
+
+void launch_cycles(DeviceEvolution::Impl& m, int cycles, bool learned, float fallback) {
   if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
   const hipStream_t s = m.batch.stream();
   EvoParams p = m.prm;
   p.init = m.initialized ? 0 : 1;
   p.cycle_begin = m.done_cycles;
   p.cycle_count = cycles;
-  EventTimer t;
-  t.start(s);
+  if (!learned) p.learn_packed = nullptr;
+  p.learn_fallback = fallback;
+  if (!m.timed) m.timer.start(s);  // kernel time spans the first launch to the last
   check(launch_evo_persistent(p, s), "evo_persistent");
-  t.stop(s);
-  kernel_seconds_ = t.ms() * 1e-3;
+  m.timer.stop(s);
+  m.timed = true;
   m.initialized = true;
   m.done_cycles += cycles;
 }
 
-void DeviceEvolution::run_learned(int cycles, const float* params, float fallback) {
-  Impl& m = *impl_;
-  if (cycles < 0 || m.done_cycles + cycles > m.cycles_cap) throw InvalidArgument("more cycles than configured.");
-  const hipStream_t s = m.batch.stream();
-  if (evo_lds_bytes(m.classes, m.batch.max_rows(), true, m.large) > kEvoMaxLds) {
-    throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
-  }
-  float* d_params = reinterpret_cast<float*>(m.arena.get() + m.off_params);
-  uint32_t* d_packed = reinterpret_cast<uint32_t*>(m.arena.get() + m.off_packed);
-  check(hipMemcpyAsync(d_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s), "upload params");
-  check(launch_mlp_pack(d_params, d_packed, s), "mlp_pack");
-  EvoParams p = m.prm;
-  p.init = m.initialized ? 0 : 1;
-  p.cycle_begin = m.done_cycles;
-  p.cycle_count = cycles;
-  p.learn_packed = d_packed;
-  p.learn_fallback = fallback;
-  EventTimer t;
-  t.start(s);
-  check(launch_evo_persistent(p, s), "evo_persistent");
-  t.stop(s);
-  kernel_seconds_ = t.ms() * 1e-3;
-  m.initialized = true;
-  m.done_cycles += cycles;
+}  // namespace
+
+void DeviceEvolution::run(int cycles) { launch_cycles(*impl_, cycles, false, 0.0f); }
+
+void DeviceEvolution::run_learned(int cycles, float fallback) {
+  if (!impl_->learned) throw InvalidArgument("the evolution was set up without a network.");
+  launch_cycles(*impl_, cycles, true, fallback);
 }
+
+double DeviceEvolution::kernel_seconds() const { return impl_->timed ? impl_->timer.ms() * 1e-3 : 0.0; }
 
 void DeviceEvolution::export_results(int* clbest, double* accbest, double* vect_acc) {
   Impl& m = *impl_;

@@ -10,27 +10,33 @@ namespace kodama_hip {
 
 class DeviceEvolution {
  public:
-  DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values, const std::vector<IVec>& codes,
-                  const std::vector<IVec>& folds, const kodama_hip_core_options& options);
+  // Label-independent set-up of a batch's evolution: the state arena, the
+  // 16-bit neighbour table, the cleared flow table and, with `params` (host
+  // [kMlpParams], else null), the learned proposal's network in its tiled
+  // form. Nothing here depends on the starting labels.
+  DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& values, const kodama_hip_core_options& options,
+                  const float* params = nullptr);
   ~DeviceEvolution();
   DeviceEvolution(const DeviceEvolution&) = delete;
   DeviceEvolution& operator=(const DeviceEvolution&) = delete;
-  // Runs `cycles` more cycles of every run in one persistent launch; blocks.
+  // Uploads the starting label codes of every run (same order as `values`).
+  void start(const std::vector<IVec>& codes);
+  // Enqueues `cycles` more cycles of every run in one persistent launch
+  // (the first launch evaluates the starting labels first); with the network
+  // of the constructor the learned variant, whose cycles fall back to the
+  // uniform proposal with probability `fallback`. Does not block.
   void run(int cycles);
-  // Runs `cycles` more cycles with the learned proposal (the persistent
-  // kernel's learned variant, one launch): in each learned cycle the network
-  // (host params [kMlpParams], copied into the arena) scores the rows whose
-  // label differs from their prediction; a cycle falls back to the uniform
-  // proposal with probability `fallback`. Blocks.
-  void run_learned(int cycles, const float* params, float fallback);
-  // Duration of the last launch (HIP events on the batch stream).
-  double kernel_seconds() const { return kernel_seconds_; }
+  void run_learned(int cycles, float fallback);
+  // Downloads and converts the results; blocks until the launches are done.
   void export_results(int* clbest, double* accbest, double* vect_acc);
+  // Time from the start of the first launch to the end of the last (HIP
+  // events on the batch stream); valid after export_results.
+  double kernel_seconds() const;
+
+  struct Impl;
 
  private:
-  struct Impl;
   Impl* impl_ = nullptr;
-  double kernel_seconds_ = 0.0;
 };
 
 }  // namespace kodama_hip

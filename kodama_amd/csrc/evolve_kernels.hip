@@ -505,7 +505,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   if (threadIdx.x < 16) s.scal[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < n; i += THREADS) {
     s.lab(0)[i] = p.cur[base + i];
-    s.pred(0)[i] = p.curpred[base + i];
+    // Before the first evaluation the predictions start equal to the labels
+    // (that evaluation rewrites them).
+    s.pred(0)[i] = p.init ? p.cur[base + i] : p.curpred[base + i];
     s.lab(1)[i] = 0;  // no stale chosen marks in the first proposal buffer
   }
   if (threadIdx.x < 2) s.lab(threadIdx.x)[n] = static_cast<unsigned short>(kSentinel);
