@@ -28,8 +28,9 @@ struct PlsCv::Impl {
   bool center = true, scale = true, da = false;
   Stream stream;
   DeviceBuffer<float> x;
-  DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, s0, gw, linear, constant, val_scores, disc;
-  DeviceBuffer<int> train_asc, train_off, train_sorted, seg_code, seg_start, seg_count, val_rows, val_off;
+  DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, gw, zsum, mu, linear, constant, val_scores, disc;
+  DeviceBuffer<double> fsum;
+  DeviceBuffer<int> train_asc, train_off, seg_code, seg_count, val_rows, val_off, run_first, run_end, fcnt;
   DeviceBuffer<int> codes, pred, comps, rank;
   IVec groups;  // row ranges of the runs: [groups[g], groups[g+1])
   IVec cached_folds, fold_ids, h_train_rows, h_train_off, h_val_rows, h_val_off;
@@ -56,6 +57,27 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
   fold_ids.clear();
   for (const auto& kv : group_of) fold_ids.push_back(kv.first);
   const int F = static_cast<int>(fold_ids.size());
+  // Each fold's training sums are its run's other folds' sums: the folds of a
+  // run must be consecutive in fold-id order.
+  IVec h_first(static_cast<size_t>(F)), h_end(static_cast<size_t>(F));
+  for (int f = 0; f < F;) {
+    const int g = group_of[fold_ids[f]];
+    int e = f + 1;
+    while (e < F && group_of[fold_ids[e]] == g) ++e;
+    for (int t = f; t < e; ++t) {
+      h_first[t] = f;
+      h_end[t] = e;
+    }
+    f = e;
+  }
+  {
+    std::vector<char> seen(groups.size(), 0);
+    for (int f = 0; f < F; ++f) {
+      const int g = group_of[fold_ids[f]];
+      if (h_first[f] == f && seen[g]) throw InvalidArgument("the fold ids of a run must be consecutive.");
+      seen[g] = 1;
+    }
+  }
   h_train_rows.clear();
   h_val_rows.clear();
   h_train_off.assign(1, 0);
@@ -71,30 +93,39 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
   }
   const hipStream_t s = stream.get();
   train_asc.resize(h_train_rows.size());
-  train_sorted.resize(h_train_rows.size());
   train_off.resize(h_train_off.size());
   val_rows.resize(h_val_rows.size());
   val_off.resize(h_val_off.size());
+  run_first.resize(h_first.size());
+  run_end.resize(h_end.size());
   train_asc.upload(h_train_rows.data(), h_train_rows.size(), s);
   train_off.upload(h_train_off.data(), h_train_off.size(), s);
   val_rows.upload(h_val_rows.data(), h_val_rows.size(), s);
   val_off.upload(h_val_off.data(), h_val_off.size(), s);
+  run_first.upload(h_first.data(), h_first.size(), s);
+  run_end.upload(h_end.data(), h_end.size(), s);
   const size_t fp = static_cast<size_t>(F) * p;
+  const size_t fc = static_cast<size_t>(F) * classes;
   mean.resize(fp);
   sd.resize(fp);
   xsum.resize(fp);
   gram.resize(fp * p);
   weights.resize(fp * kPlsMaxComponents);
-  loadings.resize(fp * kPlsMaxComponents);
-  gw.resize(fp * kPlsMaxComponents);
-  s0.resize(static_cast<size_t>(F) * classes * p);
-  linear.resize(static_cast<size_t>(F) * classes * kPlsMaxComponents);
-  constant.resize(static_cast<size_t>(F) * classes);
+  fsum.resize(fc * p);
+  fcnt.resize(fc);
   comps.resize(static_cast<size_t>(F));
   rank.resize(static_cast<size_t>(F));
-  seg_code.resize(static_cast<size_t>(F) * classes);
-  seg_start.resize(static_cast<size_t>(F) * (classes + 1));
+  seg_code.resize(fc);
   seg_count.resize(static_cast<size_t>(F));
+  if (!pls_eval_compact(p, classes, rank_cap)) {
+    // The wide layout's per-fold scratch (the compact one keeps it in LDS).
+    loadings.resize(fp * kPlsMaxComponents);
+    gw.resize(fp * kPlsMaxComponents);
+    zsum.resize(fc * p);
+    mu.resize(fc * kPlsMaxComponents);
+    linear.resize(fc * kPlsMaxComponents);
+    constant.resize(fc);
+  }
   PlsCacheParams c;
   c.x = x.get();
   c.p = p;
@@ -140,7 +171,7 @@ PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_opt
   m->w0.resize(w0.size());
   m->w0.upload(w0.data(), w0.size(), s);
   if (static_cast<size_t>(m->classes) * p > static_cast<size_t>(kPlsMaxSLds) ||
-      pls_eval_lds_bytes(p, m->classes) > 160 * 1024) {
+      pls_eval_lds_bytes(p, m->classes, m->rank_cap, pls_eval_compact(p, m->classes, m->rank_cap)) > 160 * 1024) {
     delete m;
     throw InvalidArgument("the HIP PLS-LDA kernel holds classes x columns <= 8192 in LDS.");
   }
@@ -165,17 +196,16 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   const int F = static_cast<int>(m.fold_ids.size());
   const hipStream_t s = m.stream.get();
   m.codes.upload(codes.data(), codes.size(), s);
-  PlsGroupParams g;
-  g.train.rows = m.train_asc.get();
-  g.train.offsets = m.train_off.get();
-  g.codes = m.codes.get();
-  g.folds = F;
-  g.max_classes = m.classes;
-  g.sorted = m.train_sorted.get();
-  g.seg_code = m.seg_code.get();
-  g.seg_start = m.seg_start.get();
-  g.seg_count = m.seg_count.get();
-  check(launch_pls_group(g, s), "pls_group launch");
+  PlsFoldSumsParams fsp;
+  fsp.x = m.x.get();
+  fsp.p = m.p;
+  fsp.folds = F;
+  fsp.max_classes = m.classes;
+  fsp.codes = m.codes.get();
+  fsp.val.rows = m.val_rows.get();
+  fsp.val.offsets = m.val_off.get();
+  fsp.fsum = m.fsum.get();
+  fsp.fcnt = m.fcnt.get();
   PlsEvalParams e;
   e.x = m.x.get();
   e.p = m.p;
@@ -183,23 +213,25 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   e.requested = m.requested;
   e.da = m.da ? 1 : 0;
   e.rank_cap = m.rank_cap;
+  e.max_classes = m.classes;
   e.w0 = m.w0.get();
   e.mean = m.mean.get();
   e.sd = m.sd.get();
   e.xsum = m.xsum.get();
   e.gram = m.gram.get();
-  e.train.rows = m.train_sorted.get();
-  e.train.offsets = m.train_off.get();
-  e.seg_code = m.seg_code.get();
-  e.seg_start = m.seg_start.get();
-  e.seg_count = m.seg_count.get();
+  e.run_first = m.run_first.get();
+  e.run_end = m.run_end.get();
+  e.fsum = m.fsum.get();
+  e.fcnt = m.fcnt.get();
   e.val.rows = m.val_rows.get();
   e.val.offsets = m.val_off.get();
-  e.max_classes = m.classes;
+  e.seg_code = m.seg_code.get();
+  e.seg_count = m.seg_count.get();
   e.weights = m.weights.get();
   e.loadings = m.loadings.get();
-  e.s0 = m.s0.get();
   e.gw = m.gw.get();
+  e.zsum = m.zsum.get();
+  e.mu = m.mu.get();
   e.linear = m.linear.get();
   e.constant = m.constant.get();
   e.pred_code = m.pred.get();
@@ -208,6 +240,7 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   e.val_scores = m.val_scores.get();
   e.discriminants = m.disc.get();
   m.timer.start(s);
+  check(launch_pls_fold_sums(fsp, s), "pls_fold_sums launch");
   check(launch_pls_eval(e, s), "pls_eval launch");
   m.timer.stop(s);
   m.h_comps.assign(static_cast<size_t>(F), 0);
@@ -250,7 +283,7 @@ PlsFoldDetail PlsCv::detail(int fold) const {
   const int nv = static_cast<int>(d.val_rows.size());
   const int cf = static_cast<int>(d.classes.size());
   const hipStream_t s = m.stream.get();
-  std::vector<double> w(static_cast<size_t>(m.p) * kPlsMaxComponents);
+  std::vector<double> w(static_cast<size_t>(m.p) * kPlsMaxComponents);  // [a * p + j]
   std::vector<double> tv(static_cast<size_t>(nv) * kPlsMaxComponents);
   std::vector<double> disc(static_cast<size_t>(nv) * m.classes);
   m.weights.download(w.data(), w.size(), s, static_cast<size_t>(fold) * m.p * kPlsMaxComponents);
@@ -261,7 +294,7 @@ PlsFoldDetail PlsCv::detail(int fold) const {
   const int A = fitted ? d.components : 0;
   d.weights.assign(static_cast<size_t>(m.p) * (fitted ? d.rank : 0), 0.0);
   for (int j = 0; j < m.p && fitted; ++j)
-    for (int a = 0; a < d.rank; ++a) d.weights[static_cast<size_t>(j) * d.rank + a] = w[static_cast<size_t>(j) * kPlsMaxComponents + a];
+    for (int a = 0; a < d.rank; ++a) d.weights[static_cast<size_t>(j) * d.rank + a] = w[static_cast<size_t>(a) * m.p + j];
   d.val_scores.assign(static_cast<size_t>(nv) * A, 0.0);
   d.discriminants.assign(static_cast<size_t>(nv) * (fitted ? cf : 0), 0.0);
   for (int i = 0; i < nv && fitted; ++i) {

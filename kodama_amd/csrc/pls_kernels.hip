@@ -6,19 +6,25 @@
 // does), column sums of the standardised training rows and their Gram matrix
 // G = Xs'Xs, each summed in training-row order.
 //
-// pls_eval_kernel (one workgroup per fold, once per labelling; the PLS-LDA head,
-// or with `da` the least-squares PLS-DA head of PLSDACV, plscv.cpp:1210-1296):
-//   S      S[c] = sum_{rows of class c} xs - (n_c / n) xsum, from the class
-//          segments of the training rows (host-sorted by class, rows ascending
-//          inside a class), one thread per (class, column) in fixed order;
+// Per labelling, two launches:
+// pls_fold_sums_kernel (one wave per fold): the class sums of the fold's own
+//          rows, sum x[row] per label code, and the class counts. Every row
+//          is read once per evaluation; a fold's training rows are the other
+//          folds of its run, so their class sums are sums of fold sums.
+// pls_eval_kernel (one wave per fold; the PLS-LDA head, or with `da` the
+//          least-squares PLS-DA head of PLSDACV, plscv.cpp:1210-1296):
+//   S      Z[c] = (training class sum - n_c mean) / sd, the class's
+//          standardised column sums, and S[c] = Z[c] - (n_c / n) xsum;
 //   SIMPLS power refresh (two S S' products), normalisation through G, one
 //          sequential Gram-Schmidt pass against earlier loadings, deflation;
-//   LDA    class score means W'(S0[c] + (n_c/n) xsum) / n_c and T'T = W'GW
-//          (equal to the sums over the training scores), pooled covariance,
-//          ridge grid, Cholesky solve;
-//   score  validation scores, discriminants, argmax (strict >, first wins).
-// Reductions are fixed trees (wave butterflies, waves summed in order), so a
-// result never depends on scheduling.
+//   LDA    class score means W'Z[c] / n_c and T'T = W'GW (equal to the sums
+//          over the training scores), pooled covariance, ridge grid,
+//          Cholesky solve;
+//   score  validation scores, discriminants, argmax (strict >, first wins),
+//          one lane per validation row.
+// With one wave per fold the block barriers are a wave's own and a fold's
+// arrays stay in LDS (G too for p <= 64) when they fit 64 KiB. Reductions are
+// fixed butterflies, so a result never depends on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -251,378 +257,553 @@ __global__ void __launch_bounds__(kT) pls_cache_kernel(PlsCacheParams prm) {
 
 // ---- evaluation -------------------------------------------------------------
 
-__global__ void __launch_bounds__(kT) pls_eval_kernel(PlsEvalParams prm) {
+namespace {
+
+constexpr int kE = kPlsEvalThreads;  // one wave per fold
+
+// Sum over the wave; every lane gets the total (fixed butterfly order).
+__device__ __forceinline__ double lane_total(double v) { return wave_sum(v); }
+
+// LDS carve of one fold's wave (doubles first, then ints).
+struct EvalLds {
+  double *S, *wt, *ld, *mean, *sd, *rv, *mat, *tt;
+  double *G, *W, *V, *GW, *Z, *mu, *LIN, *CONST;  // null when in global memory
+  int *segc, *cnt, *scal;
+};
+
+__host__ __device__ inline size_t eval_lds_doubles(int p, int C, int rc, bool compact) {
+  size_t d = static_cast<size_t>(C) * p + 4 * static_cast<size_t>(p) + C + 2 * static_cast<size_t>(rc) * rc;
+  if (compact) {
+    d += (p <= 64 ? static_cast<size_t>(p) * p : 0) + 3 * static_cast<size_t>(rc) * p + static_cast<size_t>(C) * p +
+         2 * static_cast<size_t>(C) * rc + C;
+  }
+  return d;
+}
+
+__device__ EvalLds eval_carve(double* base, int p, int C, int rc, bool compact) {
+  EvalLds s{};
+  double* at = base;
+  auto take = [&](size_t n) {
+    double* r = at;
+    at += n;
+    return r;
+  };
+  s.S = take(static_cast<size_t>(C) * p);
+  s.wt = take(p);
+  s.ld = take(p);
+  s.mean = take(p);
+  s.sd = take(p);
+  s.rv = take(C);
+  s.mat = take(static_cast<size_t>(rc) * rc);
+  s.tt = take(static_cast<size_t>(rc) * rc);
+  if (compact) {
+    s.G = p <= 64 ? take(static_cast<size_t>(p) * p) : nullptr;
+    s.W = take(static_cast<size_t>(rc) * p);
+    s.V = take(static_cast<size_t>(rc) * p);
+    s.GW = take(static_cast<size_t>(rc) * p);
+    s.Z = take(static_cast<size_t>(C) * p);
+    s.mu = take(static_cast<size_t>(C) * rc);
+    s.LIN = take(static_cast<size_t>(C) * rc);
+    s.CONST = take(C);
+  }
+  int* ip = reinterpret_cast<int*>(at);
+  s.segc = ip;
+  s.cnt = ip + C;
+  s.scal = ip + 2 * C;
+  return s;
+}
+
+// Power-refresh SIMPLS (plscv.cpp:816-955) by one wave, lanes over columns:
+// S [m x p] (LDS, row c at S[c * p], deflated in place), G [p x p] symmetric
+// (read as G[j * p + i] so that the lanes of one j are contiguous), weights and
+// loadings component-major (W[a * p + j]). Per component the previous weight
+// (w0 for the first) takes two S'S power steps, is normalised (a failed first
+// refresh restarts from the strongest row of S), scaled to unit score norm
+// through G, its loading G w is Gram-Schmidt orthogonalised once against the
+// earlier loadings and normalised, and S is deflated. Returns the fitted rank
+// (0: degenerate).
+__device__ int simpls_wave(double* S, int m, int p, int rc, const double* G, const double* w0, double* W, double* V,
+                           double* wt, double* ld, double* rv, int* s_int) {
+  const int lane = threadIdx.x;
+  int fitted = 0;
+  auto refresh = [&]() {
+    for (int it = 0; it < 2; ++it) {
+      for (int c = 0; c < m; ++c) {
+        double a = 0.0;
+        for (int j = lane; j < p; j += kE) a += S[c * p + j] * wt[j];
+        a = lane_total(a);
+        if (lane == 0) rv[c] = a;
+      }
+      __syncthreads();
+      for (int j = lane; j < p; j += kE) {
+        double a = 0.0;
+        for (int c = 0; c < m; ++c) a += S[c * p + j] * rv[c];
+        wt[j] = a;
+      }
+      __syncthreads();
+    }
+  };
+  auto norm_of = [&](const double* v) {
+    double part = 0.0;
+    for (int j = lane; j < p; j += kE) part += v[j] * v[j];
+    return lane_total(part);
+  };
+  for (int a = 0; a < rc; ++a) {
+    if (a == 0) {
+      for (int j = lane; j < p; j += kE) wt[j] = w0[j];
+      __syncthreads();
+    }
+    refresh();
+    double ss = norm_of(wt);
+    double nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
+    if (a == 0 && (!isfinite(nrm) || nrm <= kNormFloor)) {
+      // The strongest row of S restarts the power iteration.
+      if (lane == 0) {
+        int strongest = -1;
+        double best = 0.0;
+        for (int c = 0; c < m; ++c) {
+          double sq = 0.0;
+          for (int j = 0; j < p; ++j) sq += S[c * p + j] * S[c * p + j];
+          if (isfinite(sq) && sq > best) {
+            best = sq;
+            strongest = c;
+          }
+        }
+        s_int[0] = strongest >= 0 && best > 1.0e-20 ? strongest : -1;
+      }
+      __syncthreads();
+      const int strongest = s_int[0];
+      if (strongest >= 0) {
+        for (int j = lane; j < p; j += kE) wt[j] = S[strongest * p + j];
+        __syncthreads();
+        refresh();
+        ss = norm_of(wt);
+        nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
+      }
+    }
+    if (!isfinite(nrm) || nrm <= kNormFloor) break;
+    for (int j = lane; j < p; j += kE) wt[j] /= nrm;
+    __syncthreads();
+    for (int i = lane; i < p; i += kE) {
+      double s = 0.0;
+      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(j) * p + i] * wt[j];
+      ld[i] = s;
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int j = lane; j < p; j += kE) part += wt[j] * ld[j];
+    const double tnorm = lane_total(part);
+    if (!isfinite(tnorm) || tnorm <= kTinyFloor) break;
+    const double inv = 1.0 / sqrt(tnorm);
+    for (int j = lane; j < p; j += kE) {
+      wt[j] *= inv;
+      ld[j] *= inv;
+    }
+    __syncthreads();
+    // One sequential Gram-Schmidt pass against the earlier loadings.
+    for (int b = 0; b < a; ++b) {
+      double pp = 0.0;
+      const double* vb = V + static_cast<size_t>(b) * p;
+      for (int j = lane; j < p; j += kE) pp += vb[j] * ld[j];
+      const double proj = lane_total(pp);
+      for (int j = lane; j < p; j += kE) ld[j] -= proj * vb[j];
+      __syncthreads();
+    }
+    const double lnn = norm_of(ld);
+    const double ln = sqrt(fmax(0.0, lnn));
+    if (!isfinite(ln) || ln <= kNormFloor) break;
+    for (int j = lane; j < p; j += kE) {
+      ld[j] /= ln;
+      W[static_cast<size_t>(a) * p + j] = wt[j];
+      V[static_cast<size_t>(a) * p + j] = ld[j];
+    }
+    __syncthreads();
+    for (int c = 0; c < m; ++c) {
+      double s = 0.0;
+      for (int j = lane; j < p; j += kE) s += ld[j] * S[c * p + j];
+      s = lane_total(s);
+      for (int j = lane; j < p; j += kE) S[c * p + j] -= ld[j] * s;
+    }
+    __syncthreads();
+    fitted = a + 1;
+  }
+  return fitted;
+}
+
+}  // namespace
+
+// Class sums of each fold's validation rows: one wave per fold, lanes over
+// columns, rows in ascending order (each sum has one owner lane, so its order
+// is fixed).
+__global__ void __launch_bounds__(kE) pls_fold_sums_kernel(PlsFoldSumsParams prm) {
+  extern __shared__ double fs_lds[];
+  const int f = blockIdx.x, p = prm.p, C = prm.max_classes, lane = threadIdx.x;
+  double* acc = fs_lds;                                                   // [C x p]
+  int* cnt = reinterpret_cast<int*>(fs_lds + static_cast<size_t>(C) * p);  // [C]
+  for (int e = lane; e < C * p; e += kE) acc[e] = 0.0;
+  for (int c = lane; c < C; c += kE) cnt[c] = 0;
+  __syncthreads();
+  const int v0 = prm.val.offsets[f];
+  const int nv = prm.val.offsets[f + 1] - v0;
+  const int* rows = prm.val.rows + v0;
+  constexpr int kU = 8;
+  for (int i0 = 0; i0 < nv; i0 += kU) {
+    int row[kU], code[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) row[u] = i0 + u < nv ? rows[i0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) code[u] = row[u] >= 0 ? prm.codes[row[u]] : 0;
+    for (int j0 = 0; j0 < p; j0 += kE) {
+      const int j = j0 + lane;
+      float xv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) xv[u] = row[u] >= 0 && j < p ? prm.x[static_cast<size_t>(row[u]) * p + j] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (row[u] >= 0 && j < p) acc[code[u] * p + j] += static_cast<double>(xv[u]);
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (row[u] >= 0) ++cnt[code[u]];
+      }
+    }
+  }
+  __syncthreads();
+  double* out = prm.fsum + static_cast<size_t>(f) * C * p;
+  for (int e = lane; e < C * p; e += kE) out[e] = acc[e];
+  int* oc = prm.fcnt + static_cast<size_t>(f) * C;
+  for (int c = lane; c < C; c += kE) oc[c] = cnt[c];
+}
+
+// One wave per fold (see the file comment): S from the class sums of the
+// run's other folds, SIMPLS, the LDA or PLS-DA head, the validation argmax.
+__global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int compact) {
   extern __shared__ double pls_lds[];
-  __shared__ int s_int[4];
   const int f = blockIdx.x;
   const int p = prm.p;
-  const int tid = threadIdx.x;
-  const int t0 = prm.train.offsets[f];
-  const int nt = prm.train.offsets[f + 1] - t0;
-  const int cf = prm.seg_count[f];
-  const int* seg_code = prm.seg_code + static_cast<size_t>(f) * prm.max_classes;
-  const int* seg_start = prm.seg_start + static_cast<size_t>(f) * (prm.max_classes + 1);  // cf + 1 entries
+  const int lane = threadIdx.x;
+  const int C = prm.max_classes;
+  const int rcap = prm.rank_cap;
+  const EvalLds L = eval_carve(pls_lds, p, C, rcap, compact != 0);
+  const size_t fA = static_cast<size_t>(f) * kA;
+  const double* Gg = prm.gram + static_cast<size_t>(f) * p * p;
+  const double* G = L.G ? L.G : Gg;
+  double* Wg = prm.weights + fA * p;
+  double* W = L.W ? L.W : Wg;
+  double* V = L.V ? L.V : prm.loadings + fA * p;
+  double* GW = L.GW ? L.GW : prm.gw + fA * p;
+  double* Z = L.Z ? L.Z : prm.zsum + static_cast<size_t>(f) * C * p;
+  double* MU = L.mu ? L.mu : prm.mu + static_cast<size_t>(f) * C * kA;
+  double* LIN = L.LIN ? L.LIN : prm.linear + static_cast<size_t>(f) * C * kA;
+  double* CONST = L.CONST ? L.CONST : prm.constant + static_cast<size_t>(f) * C;
   const int v0 = prm.val.offsets[f];
   const int nv = prm.val.offsets[f + 1] - v0;
   const int* vrows = prm.val.rows + v0;
-  const int cmax = prm.max_classes;
-  const double* mean = prm.mean + static_cast<size_t>(f) * p;
-  const double* sd = prm.sd + static_cast<size_t>(f) * p;
   const double* xsum = prm.xsum + static_cast<size_t>(f) * p;
-  const double* G = prm.gram + static_cast<size_t>(f) * p * p;
-  double* W = prm.weights + static_cast<size_t>(f) * p * kA;   // [j * kA + a]
-  double* V = prm.loadings + static_cast<size_t>(f) * p * kA;
-  double* S0 = prm.s0 + static_cast<size_t>(f) * cmax * p;  // undeflated S, [c * p + j]
-  double* GW = prm.gw + static_cast<size_t>(f) * p * kA;     // [j * kA + a]
-  double* LIN = prm.linear + static_cast<size_t>(f) * cmax * kA;  // [c * kA + a]
-  double* CONST = prm.constant + static_cast<size_t>(f) * cmax;
   double* TV = prm.val_scores + static_cast<size_t>(v0) * kA;       // [i * kA + a]
-  double* DISC = prm.discriminants + static_cast<size_t>(v0) * cmax;  // [i * cmax + c]
+  double* DISC = prm.discriminants + static_cast<size_t>(v0) * C;    // [i * C + c]
 
-  double* S = pls_lds;                                  // [cmax * p], later LDA rows
-  double* wt = S + static_cast<size_t>(cmax) * p;       // [p]
-  double* ld = wt + p;                                  // [p]
-  double* rv = ld + p;                                  // [cmax]
-  double* red = rv + cmax;                              // [kWaves]
-  double* mat = red + kWaves;                           // [rc * rc] pooled covariance, then its factor
-  double* tt = mat + static_cast<size_t>(prm.rank_cap) * prm.rank_cap;  // [rc * rc] W'GW
-  double* mu = tt + static_cast<size_t>(prm.rank_cap) * prm.rank_cap;   // [cmax * rc] class score means
+  for (int j = lane; j < p; j += kE) {
+    L.mean[j] = prm.mean[static_cast<size_t>(f) * p + j];
+    L.sd[j] = prm.sd[static_cast<size_t>(f) * p + j];
+  }
+  if (L.G) {
+    for (int e = lane; e < p * p; e += kE) L.G[e] = Gg[e];
+  }
+  // Training rows per class: the run's other folds.
+  const int g0 = prm.run_first[f], g1 = prm.run_end[f];
+  for (int c = lane; c < C; c += kE) {
+    int t = 0;
+    for (int g = g0; g < g1; ++g) t += g != f ? prm.fcnt[static_cast<size_t>(g) * C + c] : 0;
+    L.cnt[c] = t;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int s = 0, nt = 0;
+    for (int c = 0; c < C; ++c) {
+      if (L.cnt[c] > 0) {
+        L.segc[s++] = c;
+        nt += L.cnt[c];
+      }
+    }
+    L.scal[0] = s;
+    L.scal[1] = nt;
+    prm.seg_count[f] = s;
+  }
+  __syncthreads();
+  const int cf = L.scal[0];
+  const int nt = L.scal[1];
+  for (int s = lane; s < cf; s += kE) prm.seg_code[static_cast<size_t>(f) * C + s] = L.segc[s];
 
   if (cf <= 1) {
     // One class in the training fold: every validation row gets it.
-    const int code = cf == 1 ? seg_code[0] : 0;
-    for (int i = tid; i < nv; i += kT) prm.pred_code[vrows[i]] = code;
-    if (tid == 0) {
+    const int code = cf == 1 ? L.segc[0] : 0;
+    for (int i = lane; i < nv; i += kE) prm.pred_code[vrows[i]] = code;
+    if (lane == 0) {
       prm.components[f] = 1;
       prm.rank[f] = 0;
     }
     return;
   }
 
-  // -- S = X'(Y - ybar) -----------------------------------------------------------
+  // -- S = X'(Y - ybar) from the class sums ----------------------------------------
+  // Z[c] = sum over the class's training rows of the standardised row
+  // ((sum x - n_c mean) / sd), S[c] = Z[c] - (n_c / n) xsum.
   const double inv_nt = 1.0 / static_cast<double>(nt);
-  for (int q = tid; q < cf * p; q += kT) {
-    const int c = q / p, j = q - c * p;
-    const int* r = prm.train.rows;
-    double acc = 0.0;
-    for (int i = seg_start[c]; i < seg_start[c + 1]; ++i) {
-      acc += (static_cast<double>(prm.x[static_cast<size_t>(r[i]) * p + j]) - mean[j]) / sd[j];
+  for (int ci = 0; ci < cf; ++ci) {
+    const int c = L.segc[ci];
+    const int nc = L.cnt[c];
+    const double ym = static_cast<double>(nc) * inv_nt;
+    for (int j = lane; j < p; j += kE) {
+      double z = 0.0;
+      for (int g = g0; g < g1; ++g) {
+        if (g != f) z += prm.fsum[(static_cast<size_t>(g) * C + c) * p + j];
+      }
+      const double zs = (z - static_cast<double>(nc) * L.mean[j]) / L.sd[j];
+      Z[ci * p + j] = zs;
+      L.S[ci * p + j] = zs - ym * xsum[j];
     }
-    const double ym = static_cast<double>(seg_start[c + 1] - seg_start[c]) * inv_nt;
-    const double s = acc - ym * xsum[j];
-    S[q] = s;
-    S0[q] = s;
   }
   __syncthreads();
 
   // -- power-refresh SIMPLS ----------------------------------------------------
-  const int rc = max(1, min(prm.rank_cap, max(1, nt - 1)));
-  const int fitted = simpls_power_fit(S, cf, p, rc, G, prm.w0, W, V, wt, ld, rv, red, s_int);
-
+  const int rc = max(1, min(rcap, max(1, nt - 1)));
+  const int fitted = simpls_wave(L.S, cf, p, rc, G, prm.w0, W, V, L.wt, L.ld, L.rv, L.scal + 2);
   if (fitted < 1) {
     // Degenerate fit: the training majority (the smaller code on ties).
-    const int code = [&] {
-      int best = seg_code[0], best_n = -1;
+    if (lane == 0) {
+      int best = L.segc[0], best_n = -1;
       for (int c = 0; c < cf; ++c) {
-        const int cnt = seg_start[c + 1] - seg_start[c];
+        const int cnt = L.cnt[L.segc[c]];
         if (cnt > best_n) {
-          best = seg_code[c];
+          best = L.segc[c];
           best_n = cnt;
         }
       }
-      return best;
-    }();
-    for (int i = tid; i < nv; i += kT) prm.pred_code[vrows[i]] = code;
-    if (tid == 0) {
+      L.scal[2] = best;
       prm.components[f] = 1;
       prm.rank[f] = 0;
     }
+    __syncthreads();
+    const int code = L.scal[2];
+    for (int i = lane; i < nv; i += kE) prm.pred_code[vrows[i]] = code;
     return;
   }
   const int A = min(prm.requested, fitted);
-  if (tid == 0) {
+  if (lane == 0) {
     prm.components[f] = A;
     prm.rank[f] = fitted;
   }
+  if (L.W) {
+    for (int e = lane; e < fitted * p; e += kE) Wg[e] = L.W[e];
+  }
 
   // -- class statistics ------------------------------------------------------------
-  // GW = G W [p x A] (global scratch, [j * kA + a]).
-  for (int q = tid; q < p * A; q += kT) {
-    const int i = q / A, a = q - i * A;
-    double s = 0.0;
-    for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(i) * p + j] * W[static_cast<size_t>(j) * kA + a];
-    GW[static_cast<size_t>(i) * kA + a] = s;
-  }
-  if (prm.da) {
-    // PLS-DA: T'(Y - ybar) = W'S0 (class score sums minus ybar times the
-    // score totals), kept in mu as [c * A + a].
-    for (int q = tid; q < cf * A; q += kT) {
-      const int c = q / A, a = q - c * A;
+  // GW = G W (component-major).
+  for (int a = 0; a < A; ++a) {
+    for (int i = lane; i < p; i += kE) {
       double s = 0.0;
-      for (int j = 0; j < p; ++j) s += W[static_cast<size_t>(j) * kA + a] * S0[c * p + j];
-      mu[c * A + a] = s;
+      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(j) * p + i] * W[static_cast<size_t>(a) * p + j];
+      GW[static_cast<size_t>(a) * p + i] = s;
     }
-  } else {
-    // Class score means W'(S0[c] + (n_c / n) xsum) / n_c.
-    for (int q = tid; q < cf * A; q += kT) {
-      const int c = q / A, a = q - c * A;
-      const int cnt = seg_start[c + 1] - seg_start[c];
-      const double ym = static_cast<double>(cnt) * inv_nt;
+  }
+  // PLS-DA: T'(Y - ybar) = W'S0 with S0 = Z - (n_c / n) xsum; LDA: the class
+  // score means W'Z[c] / n_c. Kept in MU as [c * A + a].
+  for (int ci = 0; ci < cf; ++ci) {
+    const int cnt = L.cnt[L.segc[ci]];
+    const double ym = static_cast<double>(cnt) * inv_nt;
+    for (int a = 0; a < A; ++a) {
       double s = 0.0;
-      for (int j = 0; j < p; ++j) s += W[static_cast<size_t>(j) * kA + a] * (S0[c * p + j] + ym * xsum[j]);
-      mu[c * A + a] = cnt > 0 ? s * (1.0 / static_cast<double>(cnt)) : 0.0;
+      for (int j = lane; j < p; j += kE) {
+        const double zj = prm.da ? Z[ci * p + j] - ym * xsum[j] : Z[ci * p + j];
+        s += W[static_cast<size_t>(a) * p + j] * zj;
+      }
+      s = lane_total(s);
+      if (lane == 0) MU[ci * A + a] = prm.da ? s : (cnt > 0 ? s * (1.0 / static_cast<double>(cnt)) : 0.0);
     }
   }
   __syncthreads();
   // T'T = W' G W.
-  for (int q = tid; q < A * A; q += kT) {
-    const int r = q / A, c2 = q - r * A;
-    double s = 0.0;
-    for (int j = 0; j < p; ++j) s += W[static_cast<size_t>(j) * kA + r] * GW[static_cast<size_t>(j) * kA + c2];
-    tt[q] = s;
+  for (int r = 0; r < A; ++r) {
+    for (int c2 = 0; c2 < A; ++c2) {
+      double s = 0.0;
+      for (int j = lane; j < p; j += kE) s += W[static_cast<size_t>(r) * p + j] * GW[static_cast<size_t>(c2) * p + j];
+      s = lane_total(s);
+      if (lane == 0) L.tt[r * A + c2] = s;
+    }
   }
   __syncthreads();
 
+  double* mat = L.mat;
+  double* tt = L.tt;
+  double* mu = MU;
   if (prm.da) {
     // -- PLS-DA regression (plscv.cpp:1220-1296): Gauss-Jordan elimination with
     // partial pivoting on (T'T + 1e-9 I) B = T'(Y - ybar), columns whose pivot
-    // is below 1e-12 skipped; one wave, lanes over matrix entries. mat holds
-    // T'T, mu the right-hand sides (row a of column c at mu[c * A + a]).
-    if (tid < kWave) {
-      const int lane = tid;
-      for (int q = lane; q < A * A; q += kWave) {
-        const int r = q / A, c2 = q - r * A;
-        const int lo = min(r, c2), hi = max(r, c2);
-        mat[q] = tt[hi * A + lo] + (r == c2 ? static_cast<double>(1.0e-9f) : 0.0);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      for (int col = 0; col < A; ++col) {
-        int piv = col;
-        double best = fabs(mat[col * A + col]);
-        for (int r = col + 1; r < A; ++r) {
-          const double v = fabs(mat[r * A + col]);
-          if (v > best) {
-            best = v;
-            piv = r;
-          }
+    // is below 1e-12 skipped; lanes over matrix entries. mat holds T'T, mu the
+    // right-hand sides (row a of column c at mu[c * A + a]).
+    for (int q = lane; q < A * A; q += kE) {
+      const int r = q / A, c2 = q - r * A;
+      const int lo = min(r, c2), hi = max(r, c2);
+      mat[q] = tt[hi * A + lo] + (r == c2 ? static_cast<double>(1.0e-9f) : 0.0);
+    }
+    __syncthreads();
+    for (int col = 0; col < A; ++col) {
+      int piv = col;
+      double best = fabs(mat[col * A + col]);
+      for (int r = col + 1; r < A; ++r) {
+        const double v = fabs(mat[r * A + col]);
+        if (v > best) {
+          best = v;
+          piv = r;
         }
-        if (best < static_cast<double>(1.0e-12f)) continue;
-        if (piv != col) {
-          for (int j = lane; j < A; j += kWave) {
-            const double t = mat[col * A + j];
-            mat[col * A + j] = mat[piv * A + j];
-            mat[piv * A + j] = t;
-          }
-          for (int c = lane; c < cf; c += kWave) {
-            const double t = mu[c * A + col];
-            mu[c * A + col] = mu[c * A + piv];
-            mu[c * A + piv] = t;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        }
-        const double div = mat[col * A + col];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        for (int j = col + lane; j < A; j += kWave) mat[col * A + j] /= div;
-        for (int c = lane; c < cf; c += kWave) mu[c * A + col] /= div;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // Row factors read before any row changes (tt is free once copied).
-        for (int r = lane; r < A; r += kWave) tt[r] = mat[r * A + col];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        for (int q = lane; q < A * (A + cf); q += kWave) {
-          const int r = q / (A + cf), j = q - r * (A + cf);
-          const double fr = tt[r];
-          if (r == col || fr == 0.0) continue;
-          if (j < A) {
-            if (j >= col) mat[r * A + j] -= fr * mat[col * A + j];
-          } else {
-            const int c = j - A;
-            mu[c * A + r] -= fr * mu[c * A + col];
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
       }
-      // Discriminant of class c: ybar_c + t' B[:, c].
-      for (int c = lane; c < cf; c += kWave) {
-        for (int a = 0; a < A; ++a) LIN[static_cast<size_t>(c) * kA + a] = mu[c * A + a];
-        CONST[c] = static_cast<double>(seg_start[c + 1] - seg_start[c]) * inv_nt;
+      if (best < static_cast<double>(1.0e-12f)) continue;
+      if (piv != col) {
+        for (int j = lane; j < A; j += kE) {
+          const double t = mat[col * A + j];
+          mat[col * A + j] = mat[piv * A + j];
+          mat[piv * A + j] = t;
+        }
+        for (int c = lane; c < cf; c += kE) {
+          const double t = mu[c * A + col];
+          mu[c * A + col] = mu[c * A + piv];
+          mu[c * A + piv] = t;
+        }
+        __syncthreads();
       }
+      const double div = mat[col * A + col];
+      __syncthreads();
+      for (int j = col + lane; j < A; j += kE) mat[col * A + j] /= div;
+      for (int c = lane; c < cf; c += kE) mu[c * A + col] /= div;
+      __syncthreads();
+      // Row factors read before any row changes (tt is free once copied).
+      for (int r = lane; r < A; r += kE) tt[r] = mat[r * A + col];
+      __syncthreads();
+      for (int q = lane; q < A * (A + cf); q += kE) {
+        const int r = q / (A + cf), j = q - r * (A + cf);
+        const double fr = tt[r];
+        if (r == col || fr == 0.0) continue;
+        if (j < A) {
+          if (j >= col) mat[r * A + j] -= fr * mat[col * A + j];
+        } else {
+          const int c = j - A;
+          mu[c * A + r] -= fr * mu[c * A + col];
+        }
+      }
+      __syncthreads();
+    }
+    // Discriminant of class c: ybar_c + t' B[:, c].
+    for (int c = lane; c < cf; c += kE) {
+      for (int a = 0; a < A; ++a) LIN[static_cast<size_t>(c) * A + a] = mu[c * A + a];
+      CONST[c] = static_cast<double>(L.cnt[L.segc[c]]) * inv_nt;
     }
     __syncthreads();
   } else {
-    // -- LDA: pooled covariance, ridge grid + Cholesky (one wave; lanes over
-    // rows / right-hand sides) ---------------------------------------------------
+    // -- LDA: pooled covariance, ridge grid + Cholesky (lanes over rows /
+    // right-hand sides) ---------------------------------------------------------------
     const double df = static_cast<double>(max(1, nt - cf));
-    for (int q = tid; q < A * A; q += kT) {
+    for (int q = lane; q < A * A; q += kE) {
       const int r = q / A, c2 = q - r * A;
       const int lo = min(r, c2), hi = max(r, c2);
       double between = 0.0;
       for (int c = 0; c < cf; ++c) {
-        between += static_cast<double>(seg_start[c + 1] - seg_start[c]) * mu[c * A + hi] * mu[c * A + lo];
+        between += static_cast<double>(L.cnt[L.segc[c]]) * mu[c * A + hi] * mu[c * A + lo];
       }
       // The lower triangle is the one computed (symmetric use of T'T).
       mat[q] = (tt[hi * A + lo] - between) / df;
     }
     __syncthreads();
     // The factor is written over tt (lower triangle); the solution of each
-    // class column goes to LIN[c * kA + a].
-    if (tid < kWave) {
-      const int lane = tid;
-      double trace = 0.0;
-      for (int a = 0; a < A; ++a) trace += mat[a * A + a];
-      const double rs = isfinite(trace) && trace > 0.0 ? trace / static_cast<double>(max(1, A)) : 1.0;
-      const float grid[6] = {1e-8f, 1e-6f, 1e-5f, 1e-4f, 1e-3f, 1e-2f};
-      bool ok = false;
-      // tt is free now: it holds the factor while `mat` keeps the covariance.
-      double* lo = tt;
-      for (int g = 0; g < 6 && !ok; ++g) {
-        const double lam = static_cast<double>(grid[g]) * rs;
-        bool good = true;
-        for (int c = 0; c < A && good; ++c) {
-          // Diagonal entry (every lane computes it; lanes agree).
-          double v = mat[c * A + c] + lam;
-          for (int k = 0; k < c; ++k) v -= lo[c * A + k] * lo[c * A + k];
-          if (!isfinite(v) || v <= 0.0) {
-            good = false;
-            break;
-          }
-          const double d = sqrt(v);
-          if (lane == 0) lo[c * A + c] = d;
-          for (int r = c + 1 + lane; r < A; r += kWave) {
-            double w = mat[r * A + c];
-            for (int k = 0; k < c; ++k) w -= lo[r * A + k] * lo[c * A + k];
-            lo[r * A + c] = w / d;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
+    // class column goes to LIN[c * A + a].
+    double trace = 0.0;
+    for (int a = 0; a < A; ++a) trace += mat[a * A + a];
+    const double rs = isfinite(trace) && trace > 0.0 ? trace / static_cast<double>(max(1, A)) : 1.0;
+    const float grid[6] = {1e-8f, 1e-6f, 1e-5f, 1e-4f, 1e-3f, 1e-2f};
+    bool ok = false;
+    double* lo = tt;
+    for (int g = 0; g < 6 && !ok; ++g) {
+      const double lam = static_cast<double>(grid[g]) * rs;
+      bool good = true;
+      for (int c = 0; c < A && good; ++c) {
+        // Diagonal entry (every lane computes it; lanes agree).
+        double v = mat[c * A + c] + lam;
+        for (int k = 0; k < c; ++k) v -= lo[c * A + k] * lo[c * A + k];
+        if (!isfinite(v) || v <= 0.0) {
+          good = false;
+          break;
         }
-        ok = good;
-      }
-      if (lane == 0) s_int[1] = ok ? 1 : 0;
-      if (ok) {
-        for (int c = lane; c < cf; c += kWave) {
-          double* sol = LIN + static_cast<size_t>(c) * kA;
-          for (int r = 0; r < A; ++r) {
-            double v = mu[c * A + r];
-            for (int k = 0; k < r; ++k) v -= lo[r * A + k] * sol[k];
-            sol[r] = v / lo[r * A + r];
-          }
-          for (int r = A - 1; r >= 0; --r) {
-            double v = sol[r];
-            for (int k = r + 1; k < A; ++k) v -= lo[k * A + r] * sol[k];
-            sol[r] = v / lo[r * A + r];
-          }
-          double dm = 0.0;
-          for (int a = 0; a < A; ++a) dm += mu[c * A + a] * sol[a];
-          const int cnt = seg_start[c + 1] - seg_start[c];
-          const double prior = fmax(static_cast<double>(cnt) / static_cast<double>(max(1, nt)),
-                                    static_cast<double>(FLT_MIN));
-          CONST[c] = -0.5 * dm + log(prior);
+        const double d = sqrt(v);
+        __syncthreads();
+        if (lane == 0) lo[c * A + c] = d;
+        for (int r = c + 1 + lane; r < A; r += kE) {
+          double w = mat[r * A + c];
+          for (int k = 0; k < c; ++k) w -= lo[r * A + k] * lo[c * A + k];
+          lo[r * A + c] = w / d;
         }
+        __syncthreads();
       }
+      ok = good;
     }
-    __syncthreads();
-    if (!s_int[1]) {
+    if (!ok) {
       // The host reports the failed factorisation (components = -1).
-      if (tid == 0) prm.components[f] = -1;
+      if (lane == 0) prm.components[f] = -1;
       return;
     }
+    for (int c = lane; c < cf; c += kE) {
+      double* sol = LIN + static_cast<size_t>(c) * A;
+      for (int r = 0; r < A; ++r) {
+        double v = mu[c * A + r];
+        for (int k = 0; k < r; ++k) v -= lo[r * A + k] * sol[k];
+        sol[r] = v / lo[r * A + r];
+      }
+      for (int r = A - 1; r >= 0; --r) {
+        double v = sol[r];
+        for (int k = r + 1; k < A; ++k) v -= lo[k * A + r] * sol[k];
+        sol[r] = v / lo[r * A + r];
+      }
+      double dm = 0.0;
+      for (int a = 0; a < A; ++a) dm += mu[c * A + a] * sol[a];
+      const int cnt = L.cnt[L.segc[c]];
+      const double prior = fmax(static_cast<double>(cnt) / static_cast<double>(max(1, nt)),
+                                static_cast<double>(FLT_MIN));
+      CONST[c] = -0.5 * dm + log(prior);
+    }
+    __syncthreads();
   }
 
-  // -- validation scores and discriminants ----------------------------------------
-  for (int q = tid; q < nv * A; q += kT) {
-    const int i = q / A, a = q - i * A;
+  // -- validation scores, discriminants, argmax (one lane per row) ----------------
+  for (int i = lane; i < nv; i += kE) {
     const size_t row = static_cast<size_t>(vrows[i]) * p;
-    double s = 0.0;
-    for (int j = 0; j < p; ++j) {
-      s += ((static_cast<double>(prm.x[row + j]) - mean[j]) / sd[j]) * W[static_cast<size_t>(j) * kA + a];
+    double* tv = TV + static_cast<size_t>(i) * kA;
+    for (int a0 = 0; a0 < A; a0 += 8) {
+      double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < p; ++j) {
+        const double z = (static_cast<double>(prm.x[row + j]) - L.mean[j]) / L.sd[j];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (a0 + u < A) t[u] += z * W[static_cast<size_t>(a0 + u) * p + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (a0 + u < A) tv[a0 + u] = t[u];
+      }
     }
-    TV[static_cast<size_t>(i) * kA + a] = s;
-  }
-  __syncthreads();
-  for (int q = tid; q < nv * cf; q += kT) {
-    const int i = q / cf, c = q - i * cf;
-    double sc = CONST[c];
-    for (int a = 0; a < A; ++a) sc += TV[static_cast<size_t>(i) * kA + a] * LIN[static_cast<size_t>(c) * kA + a];
-    DISC[static_cast<size_t>(i) * cmax + c] = sc;
-  }
-  __syncthreads();
-  for (int i = tid; i < nv; i += kT) {
     int best = 0;
     double best_s = -INFINITY;
     for (int c = 0; c < cf; ++c) {
-      const double sc = DISC[static_cast<size_t>(i) * cmax + c];
+      double sc = CONST[c];
+      for (int a = 0; a < A; ++a) sc += tv[a] * LIN[static_cast<size_t>(c) * A + a];
+      DISC[static_cast<size_t>(i) * C + c] = sc;
       if (sc > best_s) {
         best_s = sc;
         best = c;
       }
     }
-    prm.pred_code[vrows[i]] = seg_code[best];
-  }
-}
-
-// ---- class grouping -----------------------------------------------------------
-
-// One workgroup per fold: class counts, the segment table (classes ascending),
-// then one wave scatters the rows in their original order -- per 64-row chunk,
-// the lanes of one class take consecutive slots in lane order -- so every class
-// segment lists its rows ascending.
-__global__ void __launch_bounds__(kT) pls_group_kernel(PlsGroupParams prm) {
-  extern __shared__ int grp_lds[];
-  const int C = prm.max_classes;
-  int* cnt = grp_lds;       // [C]
-  int* base = grp_lds + C;  // [C]
-  const int f = blockIdx.x;
-  const int t0 = prm.train.offsets[f];
-  const int nt = prm.train.offsets[f + 1] - t0;
-  const int* rows = prm.train.rows + t0;
-  for (int c = threadIdx.x; c < C; c += kT) cnt[c] = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < nt; i += kT) atomicAdd(&cnt[prm.codes[rows[i]]], 1);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int s = 0, acc = 0;
-    int* code = prm.seg_code + static_cast<size_t>(f) * C;
-    int* start = prm.seg_start + static_cast<size_t>(f) * (C + 1);
-    for (int c = 0; c < C; ++c) {
-      if (cnt[c] == 0) continue;
-      code[s] = c;
-      start[s] = t0 + acc;
-      base[c] = t0 + acc;
-      acc += cnt[c];
-      ++s;
-    }
-    start[s] = t0 + acc;
-    prm.seg_count[f] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < kWave) {
-    const int lane = threadIdx.x;
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int c0 = 0; c0 < nt; c0 += kWave) {
-      const int i = c0 + lane;
-      const int row = i < nt ? rows[i] : -1;
-      const int code = i < nt ? prm.codes[row] : -1;
-      uint64_t remaining = __ballot(i < nt);
-      while (remaining != 0ull) {
-        const int leader = __ffsll(static_cast<unsigned long long>(remaining)) - 1;
-        const int lc = __shfl(code, leader);
-        const uint64_t mask = __ballot(code == lc) & remaining;
-        const int b = base[lc];
-        if ((mask >> lane) & 1ull) prm.sorted[b + __popcll(mask & below)] = row;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (lane == leader) base[lc] = b + __popcll(mask);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        remaining &= ~mask;
-      }
-    }
+    prm.pred_code[vrows[i]] = L.segc[best];
   }
 }
 
@@ -812,19 +993,28 @@ __global__ void pls_scores_kernel(PlsScoreParams prm) {
 
 // ---- launchers ----------------------------------------------------------------
 
-hipError_t launch_pls_group(const PlsGroupParams& prm, hipStream_t s) {
+hipError_t launch_pls_fold_sums(const PlsFoldSumsParams& prm, hipStream_t s) {
   if (prm.folds == 0) return hipSuccess;
-  if (prm.max_classes < 1) return hipErrorInvalidValue;
-  const size_t shmem = 2 * sizeof(int) * static_cast<size_t>(prm.max_classes);
-  if (shmem > 64 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pls_group_kernel, dim3(prm.folds), dim3(kT), shmem, s, prm);
+  if (prm.p < 1 || prm.p > kPlsMaxDims || prm.max_classes < 1 ||
+      static_cast<size_t>(prm.max_classes) * prm.p > kPlsMaxSLds) {
+    return hipErrorInvalidValue;
+  }
+  const size_t shmem = sizeof(double) * static_cast<size_t>(prm.max_classes) * prm.p +
+                       sizeof(int) * static_cast<size_t>(prm.max_classes);
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_fold_sums_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pls_fold_sums_kernel, dim3(prm.folds), dim3(kE), shmem, s, prm);
   return hipGetLastError();
 }
 
-size_t pls_eval_lds_bytes(int p, int max_classes) {
-  const int rc = p < kPlsMaxComponents ? p : kPlsMaxComponents;
-  return sizeof(double) * (static_cast<size_t>(max_classes) * p + 2 * static_cast<size_t>(p) + max_classes +
-                           kWaves + 2 * static_cast<size_t>(rc) * rc + static_cast<size_t>(max_classes) * rc);
+size_t pls_eval_lds_bytes(int p, int max_classes, int rank_cap, bool compact) {
+  const size_t ints = 2 * static_cast<size_t>(max_classes) + 4;
+  return sizeof(double) * eval_lds_doubles(p, max_classes, rank_cap, compact) + sizeof(int) * ints;
+}
+
+bool pls_eval_compact(int p, int max_classes, int rank_cap) {
+  return pls_eval_lds_bytes(p, max_classes, rank_cap, true) <= 64 * 1024;
 }
 
 hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s) {
@@ -837,15 +1027,17 @@ hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s) {
 hipError_t launch_pls_eval(const PlsEvalParams& prm, hipStream_t s) {
   if (prm.folds == 0) return hipSuccess;
   if (prm.p < 1 || prm.p > kPlsMaxDims || prm.rank_cap < 1 || prm.rank_cap > kPlsMaxComponents ||
-      prm.rank_cap > prm.p || prm.requested < 1 || static_cast<size_t>(prm.max_classes) * prm.p > kPlsMaxSLds) {
+      prm.rank_cap > prm.p || prm.requested < 1 || prm.max_classes < 1 ||
+      static_cast<size_t>(prm.max_classes) * prm.p > kPlsMaxSLds) {
     return hipErrorInvalidValue;
   }
-  const size_t shmem = pls_eval_lds_bytes(prm.p, prm.max_classes);
+  const bool compact = pls_eval_compact(prm.p, prm.max_classes, prm.rank_cap);
+  const size_t shmem = pls_eval_lds_bytes(prm.p, prm.max_classes, prm.rank_cap, compact);
   if (shmem > 160 * 1024) return hipErrorInvalidValue;
   const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_eval_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pls_eval_kernel, dim3(prm.folds), dim3(kT), shmem, s, prm);
+  hipLaunchKernelGGL(pls_eval_kernel, dim3(prm.folds), dim3(kE), shmem, s, prm, compact ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -11,7 +11,8 @@
 
 namespace kodama_hip {
 
-constexpr int kPlsThreads = 256;      // one workgroup per fold
+constexpr int kPlsThreads = 256;      // general PLS kernels
+constexpr int kPlsEvalThreads = 64;   // one wave per fold in the cross-validation kernels
 constexpr int kPlsMaxComponents = 64;
 constexpr int kPlsMaxDims = 256;
 constexpr int kPlsMaxSLds = 8192;     // doubles of S = p x classes kept in LDS
@@ -35,6 +36,20 @@ struct PlsCacheParams {
   double* gram = nullptr;        // [folds x p x p]
 };
 
+// Class sums of each fold's validation rows (ascending row order, FP64):
+// fsum[f][c][j] = sum of x[row][j] over the fold's rows of code c, fcnt[f][c]
+// their number. A fold's training sums are its run's other folds' sums.
+struct PlsFoldSumsParams {
+  const float* x = nullptr;
+  int p = 0;
+  int folds = 0;
+  int max_classes = 0;
+  const int* codes = nullptr;        // [rows] label code per row
+  PlsFoldRows val;
+  double* fsum = nullptr;            // [folds x max_classes x p]
+  int* fcnt = nullptr;               // [folds x max_classes]
+};
+
 struct PlsEvalParams {
   const float* x = nullptr;
   int p = 0;
@@ -42,27 +57,29 @@ struct PlsEvalParams {
   int requested = 0;             // components evaluated (fixed or max)
   int da = 0;                    // 1: PLS-DA head (least squares on the scores), 0: LDA
   int rank_cap = 0;              // min(max_components, p) before the per-fold n - 1 cap
+  int max_classes = 0;           // label codes (0 .. max_classes - 1)
   const double* w0 = nullptr;    // [p] first power-iteration start (normal draws)
   const double* mean = nullptr;  // fold cache
   const double* sd = nullptr;
   const double* xsum = nullptr;
   const double* gram = nullptr;
-  // Training rows grouped by class code (ascending code, rows ascending inside
-  // a class), as pls_group_kernel writes them: fold f has seg_count[f]
-  // classes, class s is seg_code[f * max_classes + s] and owns train.rows[i]
-  // for seg_start[f * (max_classes + 1) + s] <= i < seg_start[... + s + 1]
-  // (indices into the concatenated training rows).
-  PlsFoldRows train;
-  const int* seg_code = nullptr;     // [folds x max_classes]
-  const int* seg_start = nullptr;    // [folds x (max_classes + 1)]
-  const int* seg_count = nullptr;    // [folds]
+  // The folds of fold f's run are [run_first[f], run_end[f]); the training
+  // rows of f are the rows of the run's other folds.
+  const int* run_first = nullptr;    // [folds]
+  const int* run_end = nullptr;      // [folds]
+  const double* fsum = nullptr;      // class sums of every fold's rows (PlsFoldSumsParams)
+  const int* fcnt = nullptr;
   PlsFoldRows val;
-  int max_classes = 0;               // label codes (0 .. max_classes - 1)
-  // Scratch (device, per fold).
-  double* weights = nullptr;         // [folds x p x kPlsMaxComponents], [j * kA + a]
-  double* loadings = nullptr;        // [folds x p x kPlsMaxComponents]
-  double* s0 = nullptr;              // [folds x max_classes x p] S before deflation
-  double* gw = nullptr;              // [folds x p x kPlsMaxComponents] G W
+  // Per fold: the label codes present among its training rows, ascending.
+  int* seg_code = nullptr;           // [folds x max_classes]
+  int* seg_count = nullptr;          // [folds]
+  // Per-fold scratch in global memory; the large one-wave layout also keeps
+  // the loadings, G W, class score sums and LDA rows here instead of LDS.
+  double* weights = nullptr;         // [folds x kPlsMaxComponents x p], component-major [a * p + j]
+  double* loadings = nullptr;        // [folds x kPlsMaxComponents x p]
+  double* gw = nullptr;              // [folds x kPlsMaxComponents x p] G W
+  double* zsum = nullptr;            // [folds x max_classes x p] standardised class sums (present classes)
+  double* mu = nullptr;              // [folds x max_classes x kPlsMaxComponents]
   double* linear = nullptr;          // [folds x max_classes x kPlsMaxComponents] (DA: coefficients)
   double* constant = nullptr;        // [folds x max_classes] (DA: class frequencies)
   // Outputs. components[f] = -1 reports a failed LDA factorisation.
@@ -73,19 +90,6 @@ struct PlsEvalParams {
   // [val total x max_classes] (rows in validation-list order).
   double* val_scores = nullptr;
   double* discriminants = nullptr;
-};
-
-// Groups each fold's training rows (ascending, `train`) by the class code of
-// each row (codes[row]) into `sorted`, stably, and writes the segment tables.
-struct PlsGroupParams {
-  PlsFoldRows train;
-  const int* codes = nullptr;        // [rows] label code per row
-  int folds = 0;
-  int max_classes = 0;
-  int* sorted = nullptr;             // [train total]
-  int* seg_code = nullptr;           // [folds x max_classes]
-  int* seg_start = nullptr;          // [folds x (max_classes + 1)]
-  int* seg_count = nullptr;          // [folds]
 };
 
 // General PLS (fit_general_pls, plscv.cpp:2765-2878) on the device, FP64.
@@ -164,8 +168,13 @@ size_t pls_fit_lds_bytes(int p, int q);
 hipError_t launch_pls_fit(const PlsFitParams& prm, hipStream_t s);
 hipError_t launch_pls_scores(const PlsScoreParams& prm, hipStream_t s);
 
-size_t pls_eval_lds_bytes(int p, int max_classes);
-hipError_t launch_pls_group(const PlsGroupParams& prm, hipStream_t s);
+// LDS of one fold's wave: the compact layout keeps the loadings, G W, the
+// class score sums and the LDA rows in LDS (and G when p <= 64), the wide
+// layout keeps them in the global scratch. pls_eval_layout picks the compact
+// one when it fits 64 KiB.
+size_t pls_eval_lds_bytes(int p, int max_classes, int rank_cap, bool compact);
+bool pls_eval_compact(int p, int max_classes, int rank_cap);
+hipError_t launch_pls_fold_sums(const PlsFoldSumsParams& prm, hipStream_t s);
 hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s);
 hipError_t launch_pls_eval(const PlsEvalParams& prm, hipStream_t s);
 
