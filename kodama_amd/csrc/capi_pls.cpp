@@ -228,25 +228,31 @@ void run_plslda_batch(const BatchRequest& q, int32_t* clbest, double* accbest, d
     f.seed = options->seed + static_cast<std::uint64_t>(r);
     ev.emplace_back(values[r], codes[r], cons[r], fixed[r], f);
   }
+  // Folds that do not depend on the labels: the fold cache is part of the
+  // precompute.
+  if (cache) pls.prepare(gfolds);
   const double t_pre = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const auto t1 = std::chrono::steady_clock::now();
   const int threads = resolve_threads(options->n_threads, runs);
-  IVec ccat(static_cast<size_t>(total)), pred(static_cast<size_t>(total));
+  // The proposals are written straight into the engine's pinned staging and
+  // the predictions read from it.
+  int* ccat = pls.staged_codes();
+  const int* pred = pls.staged_pred();
   auto evaluate_all = [&](int cycle) {
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
     for (int r = 0; r < runs; ++r) {
       const IVec& cl = cycle < 0 ? ev[r].initial() : ev[r].propose(cycle);
-      std::copy(cl.begin(), cl.end(), ccat.begin() + offsets[r]);
+      std::copy(cl.begin(), cl.end(), ccat + offsets[r]);
       if (!cache) set_folds(r, run_folds(r, values_of(cl, values[r])));
     }
-    pls.evaluate(ccat, gfolds, pred.data(), nullptr);
+    pls.evaluate_staged(gfolds, nullptr);
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
     for (int r = 0; r < runs; ++r) {
-      const IVec cl(ccat.begin() + offsets[r], ccat.begin() + offsets[r + 1]);
+      const IVec cl(ccat + offsets[r], ccat + offsets[r + 1]);
       IVec counts, firsts;
-      const Tally t = tally_of(cl, pred.data() + offsets[r], static_cast<int>(values[r].size()), counts, firsts);
-      if (cycle < 0) ev[r].begin(pred.data() + offsets[r], t);
-      else ev[r].finish(cycle, pred.data() + offsets[r], t);
+      const Tally t = tally_of(cl, pred + offsets[r], static_cast<int>(values[r].size()), counts, firsts);
+      if (cycle < 0) ev[r].begin(pred + offsets[r], t);
+      else ev[r].finish(cycle, pred + offsets[r], t);
     }
   };
   evaluate_all(-1);

@@ -35,6 +35,7 @@ struct PlsCv::Impl {
   IVec groups;  // row ranges of the runs: [groups[g], groups[g+1])
   IVec cached_folds, fold_ids, h_train_rows, h_train_off, h_val_rows, h_val_off;
   IVec h_comps, h_rank;
+  PinnedBuffer<int> h_codes, h_pred, h_small;  // codes, predictions; components and ranks
   EventTimer timer;
   double eval_ms_total = 0.0;
   int eval_calls = 0;
@@ -177,6 +178,8 @@ PlsCv::PlsCv(int device, const float* x, int n, int p, const kodama_hip_core_opt
   }
   m->codes.resize(static_cast<size_t>(n));
   m->pred.resize(static_cast<size_t>(n));
+  m->h_codes.resize(static_cast<size_t>(n));
+  m->h_pred.resize(static_cast<size_t>(n));
   m->val_scores.resize(static_cast<size_t>(n) * kPlsMaxComponents);
   m->disc.resize(static_cast<size_t>(n) * m->classes);
   impl_ = m;
@@ -186,16 +189,33 @@ PlsCv::~PlsCv() { delete impl_; }
 
 void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected) {
   Impl& m = *impl_;
-  if (static_cast<int>(codes.size()) != m.n || static_cast<int>(folds.size()) != m.n) {
-    throw InvalidArgument("labels and folds must have one entry per row.");
-  }
-  for (int c : codes) {
-    if (c < 0 || c >= m.classes) throw InvalidArgument("label code out of range.");
-  }
+  if (static_cast<int>(codes.size()) != m.n) throw InvalidArgument("labels and folds must have one entry per row.");
+  std::copy(codes.begin(), codes.end(), m.h_codes.get());
+  evaluate_staged(folds, selected);
+  std::copy(m.h_pred.get(), m.h_pred.get() + m.n, pred_codes);
+}
+
+int* PlsCv::staged_codes() { return impl_->h_codes.get(); }
+const int* PlsCv::staged_pred() const { return impl_->h_pred.get(); }
+
+void PlsCv::prepare(const IVec& folds) {
+  Impl& m = *impl_;
+  if (static_cast<int>(folds.size()) != m.n) throw InvalidArgument("labels and folds must have one entry per row.");
+  if (folds != m.cached_folds) m.rebuild_folds(folds);
+  m.stream.sync();
+}
+
+void PlsCv::evaluate_staged(const IVec& folds, int* selected) {
+  Impl& m = *impl_;
+  if (static_cast<int>(folds.size()) != m.n) throw InvalidArgument("labels and folds must have one entry per row.");
+  const int* codes = m.h_codes.get();
+  bool in_range = true;
+  for (int i = 0; i < m.n; ++i) in_range &= codes[i] >= 0 && codes[i] < m.classes;
+  if (!in_range) throw InvalidArgument("label code out of range.");
   if (folds != m.cached_folds) m.rebuild_folds(folds);
   const int F = static_cast<int>(m.fold_ids.size());
   const hipStream_t s = m.stream.get();
-  m.codes.upload(codes.data(), codes.size(), s);
+  m.codes.upload(codes, static_cast<size_t>(m.n), s);
   PlsFoldSumsParams fsp;
   fsp.x = m.x.get();
   fsp.p = m.p;
@@ -243,12 +263,13 @@ void PlsCv::evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int*
   check(launch_pls_fold_sums(fsp, s), "pls_fold_sums launch");
   check(launch_pls_eval(e, s), "pls_eval launch");
   m.timer.stop(s);
-  m.h_comps.assign(static_cast<size_t>(F), 0);
-  m.h_rank.assign(static_cast<size_t>(F), 0);
-  m.pred.download(pred_codes, static_cast<size_t>(m.n), s);
-  m.comps.download(m.h_comps.data(), m.h_comps.size(), s);
-  m.rank.download(m.h_rank.data(), m.h_rank.size(), s);
+  if (m.h_small.size() < 2 * static_cast<size_t>(F)) m.h_small.resize(2 * static_cast<size_t>(F));
+  m.pred.download(m.h_pred.get(), static_cast<size_t>(m.n), s);
+  m.comps.download(m.h_small.get(), static_cast<size_t>(F), s);
+  m.rank.download(m.h_small.get() + F, static_cast<size_t>(F), s);
   m.stream.sync();
+  m.h_comps.assign(m.h_small.get(), m.h_small.get() + F);
+  m.h_rank.assign(m.h_small.get() + F, m.h_small.get() + 2 * F);
   m.eval_ms_total += m.timer.ms();
   ++m.eval_calls;
   int evaluated = m.requested;

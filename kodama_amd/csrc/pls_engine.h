@@ -38,6 +38,15 @@ class PlsCv {
   // max_components). Fold statistics are recomputed when `folds` differs from
   // the previous call's.
   void evaluate(const IVec& codes, const IVec& folds, int* pred_codes, int* selected);
+  // The same on pinned host buffers of n entries: the caller writes the
+  // codes into staged_codes() and reads the predictions from staged_pred()
+  // after the call (no copies through pageable memory).
+  void evaluate_staged(const IVec& folds, int* selected);
+  int* staged_codes();
+  const int* staged_pred() const;
+  // Builds the fold cache for `folds` now (otherwise the first evaluation
+  // with new folds does).
+  void prepare(const IVec& folds);
   // Head of the fold fits: false (default) LDA (PLSLDACV), true the
   // least-squares PLS-DA head (PLSDACV).
   void set_da(bool da);

@@ -27,6 +27,7 @@
 // fixed butterflies, so a result never depends on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -261,21 +262,70 @@ namespace {
 
 constexpr int kE = kPlsEvalThreads;  // one wave per fold
 
-// Sum over the wave; every lane gets the total (fixed butterfly order).
-__device__ __forceinline__ double lane_total(double v) { return wave_sum(v); }
+// Wave sums on the DPP crossbar instead of LDS permutes: within each row of
+// 16 lanes, pairs (quad_perm [1,0,3,2]), quads ([2,3,0,1]), then rotations
+// by 4 and 8 leave the row's sum in every lane of the row; lane 0 of each of
+// the four rows is read into a scalar and the four row sums are added in row
+// order, so every lane gets the same total and the order is fixed.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(b)), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<unsigned>(b >> 32)), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                                     static_cast<unsigned>(lo)));
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b)), lane));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b >> 32)), lane));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// N independent wave sums at once (their cross-lane steps interleave).
+template <int N>
+__device__ __forceinline__ void lane_totals(double (&v)[N]) {
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] += dpp_move<0xB1>(v[u]);   // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] += dpp_move<0x4E>(v[u]);   // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] += dpp_move<0x124>(v[u]);  // row_ror:4
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] += dpp_move<0x128>(v[u]);  // row_ror:8
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    v[u] = ((read_lane(v[u], 0) + read_lane(v[u], 16)) + read_lane(v[u], 32)) + read_lane(v[u], 48);
+  }
+}
+
+// Sum over the wave; every lane gets the total.
+__device__ __forceinline__ double lane_total(double v) {
+  double t[1] = {v};
+  lane_totals(t);
+  return t[0];
+}
+constexpr int kBatch = 8;  // reductions in flight
 
 // LDS carve of one fold's wave (doubles first, then ints).
 struct EvalLds {
   double *S, *wt, *ld, *mean, *sd, *rv, *mat, *tt;
   double *G, *W, *V, *GW, *Z, *mu, *LIN, *CONST;  // null when in global memory
+  double *Wp, *LINp;  // compact: weights [8 x 64] and LDA rows [C x 8], zero padded
   int *segc, *cnt, *scal;
 };
 
+// Columns of the padded per-column arrays (mean, sd: 64 at least, so that the
+// compact validation loop runs over 64 columns without bounds).
+__host__ __device__ inline int eval_cols(int p) { return p > 64 ? p : 64; }
+
 __host__ __device__ inline size_t eval_lds_doubles(int p, int C, int rc, bool compact) {
-  size_t d = static_cast<size_t>(C) * p + 4 * static_cast<size_t>(p) + C + 2 * static_cast<size_t>(rc) * rc;
+  size_t d = static_cast<size_t>(C) * p + 2 * static_cast<size_t>(p) + 2 * static_cast<size_t>(eval_cols(p)) + C +
+             2 * static_cast<size_t>(rc) * rc;
   if (compact) {
-    d += (p <= 64 ? static_cast<size_t>(p) * p : 0) + 3 * static_cast<size_t>(rc) * p + static_cast<size_t>(C) * p +
-         2 * static_cast<size_t>(C) * rc + C;
+    d += static_cast<size_t>(p) * p + 3 * static_cast<size_t>(rc) * p + static_cast<size_t>(C) * p +
+         2 * static_cast<size_t>(C) * rc + C + 8 * 64 + 8 * static_cast<size_t>(C);
   }
   return d;
 }
@@ -291,13 +341,13 @@ __device__ EvalLds eval_carve(double* base, int p, int C, int rc, bool compact) 
   s.S = take(static_cast<size_t>(C) * p);
   s.wt = take(p);
   s.ld = take(p);
-  s.mean = take(p);
-  s.sd = take(p);
+  s.mean = take(eval_cols(p));
+  s.sd = take(eval_cols(p));
   s.rv = take(C);
   s.mat = take(static_cast<size_t>(rc) * rc);
   s.tt = take(static_cast<size_t>(rc) * rc);
   if (compact) {
-    s.G = p <= 64 ? take(static_cast<size_t>(p) * p) : nullptr;
+    s.G = take(static_cast<size_t>(p) * p);
     s.W = take(static_cast<size_t>(rc) * p);
     s.V = take(static_cast<size_t>(rc) * p);
     s.GW = take(static_cast<size_t>(rc) * p);
@@ -305,6 +355,8 @@ __device__ EvalLds eval_carve(double* base, int p, int C, int rc, bool compact) 
     s.mu = take(static_cast<size_t>(C) * rc);
     s.LIN = take(static_cast<size_t>(C) * rc);
     s.CONST = take(C);
+    s.Wp = take(8 * 64);
+    s.LINp = take(8 * static_cast<size_t>(C));
   }
   int* ip = reinterpret_cast<int*>(at);
   s.segc = ip;
@@ -322,17 +374,28 @@ __device__ EvalLds eval_carve(double* base, int p, int C, int rc, bool compact) 
 // through G, its loading G w is Gram-Schmidt orthogonalised once against the
 // earlier loadings and normalised, and S is deflated. Returns the fitted rank
 // (0: degenerate).
-__device__ int simpls_wave(double* S, int m, int p, int rc, const double* G, const double* w0, double* W, double* V,
+__device__ __forceinline__ int simpls_wave(double* S, int m, int p, int rc, const double* G, const double* w0, double* W, double* V,
                            double* wt, double* ld, double* rv, int* s_int) {
   const int lane = threadIdx.x;
   int fitted = 0;
   auto refresh = [&]() {
     for (int it = 0; it < 2; ++it) {
-      for (int c = 0; c < m; ++c) {
-        double a = 0.0;
-        for (int j = lane; j < p; j += kE) a += S[c * p + j] * wt[j];
-        a = lane_total(a);
-        if (lane == 0) rv[c] = a;
+      for (int c0 = 0; c0 < m; c0 += kBatch) {
+        double a[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          a[u] = 0.0;
+          if (c0 + u < m) {
+            for (int j = lane; j < p; j += kE) a[u] += S[(c0 + u) * p + j] * wt[j];
+          }
+        }
+        lane_totals(a);
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            if (c0 + u < m) rv[c0 + u] = a[u];
+          }
+        }
       }
       __syncthreads();
       for (int j = lane; j < p; j += kE) {
@@ -385,9 +448,18 @@ __device__ int simpls_wave(double* S, int m, int p, int rc, const double* G, con
     for (int j = lane; j < p; j += kE) wt[j] /= nrm;
     __syncthreads();
     for (int i = lane; i < p; i += kE) {
-      double s = 0.0;
-      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(j) * p + i] * wt[j];
-      ld[i] = s;
+      // Four partial sums (columns j mod 4), added in order.
+      double q[4] = {0.0, 0.0, 0.0, 0.0};
+      int j = 0;
+      for (; j + 4 <= p; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] += G[static_cast<size_t>(j + u) * p + i] * wt[j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (j + u < p) q[u] += G[static_cast<size_t>(j + u) * p + i] * wt[j + u];
+      }
+      ld[i] = (q[0] + q[1]) + (q[2] + q[3]);
     }
     __syncthreads();
     double part = 0.0;
@@ -418,11 +490,22 @@ __device__ int simpls_wave(double* S, int m, int p, int rc, const double* G, con
       V[static_cast<size_t>(a) * p + j] = ld[j];
     }
     __syncthreads();
-    for (int c = 0; c < m; ++c) {
-      double s = 0.0;
-      for (int j = lane; j < p; j += kE) s += ld[j] * S[c * p + j];
-      s = lane_total(s);
-      for (int j = lane; j < p; j += kE) S[c * p + j] -= ld[j] * s;
+    for (int c0 = 0; c0 < m; c0 += kBatch) {
+      double d[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        d[u] = 0.0;
+        if (c0 + u < m) {
+          for (int j = lane; j < p; j += kE) d[u] += ld[j] * S[(c0 + u) * p + j];
+        }
+      }
+      lane_totals(d);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (c0 + u < m) {
+          for (int j = lane; j < p; j += kE) S[(c0 + u) * p + j] -= ld[j] * d[u];
+        }
+      }
     }
     __syncthreads();
     fitted = a + 1;
@@ -432,72 +515,95 @@ __device__ int simpls_wave(double* S, int m, int p, int rc, const double* G, con
 
 }  // namespace
 
-// Class sums of each fold's validation rows: one wave per fold, lanes over
-// columns, rows in ascending order (each sum has one owner lane, so its order
-// is fixed).
-__global__ void __launch_bounds__(kE) pls_fold_sums_kernel(PlsFoldSumsParams prm) {
+// Class sums of each fold's validation rows: one workgroup per fold of
+// `waves` waves, each summing a contiguous share of the rows (ascending) into
+// its own LDS partial with lanes over columns, so every partial sum has one
+// owner lane and a fixed order; the partials are then added in wave order.
+// Per 64 rows the lanes load the row ids and codes, and sixteen rows' columns
+// are loaded before they are added.
+__global__ void __launch_bounds__(kT) pls_fold_sums_kernel(PlsFoldSumsParams prm, int waves) {
   extern __shared__ double fs_lds[];
-  const int f = blockIdx.x, p = prm.p, C = prm.max_classes, lane = threadIdx.x;
-  double* acc = fs_lds;                                                   // [C x p]
-  int* cnt = reinterpret_cast<int*>(fs_lds + static_cast<size_t>(C) * p);  // [C]
-  for (int e = lane; e < C * p; e += kE) acc[e] = 0.0;
-  for (int c = lane; c < C; c += kE) cnt[c] = 0;
+  const int f = blockIdx.x, p = prm.p, C = prm.max_classes;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const size_t cp = static_cast<size_t>(C) * p;
+  double* acc = fs_lds;                                                          // [waves x C x p]
+  int* cnt = reinterpret_cast<int*>(fs_lds + static_cast<size_t>(waves) * cp);  // [waves x C]
+  for (size_t e = threadIdx.x; e < waves * cp; e += blockDim.x) acc[e] = 0.0;
+  for (int e = threadIdx.x; e < waves * C; e += blockDim.x) cnt[e] = 0;
   __syncthreads();
   const int v0 = prm.val.offsets[f];
   const int nv = prm.val.offsets[f + 1] - v0;
-  const int* rows = prm.val.rows + v0;
-  constexpr int kU = 8;
-  for (int i0 = 0; i0 < nv; i0 += kU) {
-    int row[kU], code[kU];
+  const int share = (nv + waves - 1) / waves;
+  const int r0 = min(nv, wave * share), r1 = min(nv, r0 + share);
+  double* my = acc + wave * cp;
+  for (int i0 = r0; i0 < r1; i0 += kWave) {
+    const int mine = i0 + lane < r1 ? prm.val.rows[v0 + i0 + lane] : -1;
+    const int mycode = mine >= 0 ? prm.codes[mine] : 0;
+    const int cnt_rows = min(kWave, r1 - i0);
+    if (mine >= 0) atomicAdd(&cnt[wave * C + mycode], 1);
+    for (int u0 = 0; u0 < cnt_rows; u0 += 16) {  // uniform in the wave
+      int row[16], code[16];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) row[u] = i0 + u < nv ? rows[i0 + u] : -1;
-#pragma unroll
-    for (int u = 0; u < kU; ++u) code[u] = row[u] >= 0 ? prm.codes[row[u]] : 0;
-    for (int j0 = 0; j0 < p; j0 += kE) {
-      const int j = j0 + lane;
-      float xv[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) xv[u] = row[u] >= 0 && j < p ? prm.x[static_cast<size_t>(row[u]) * p + j] : 0.0f;
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (row[u] >= 0 && j < p) acc[code[u] * p + j] += static_cast<double>(xv[u]);
+      for (int u = 0; u < 16; ++u) {
+        row[u] = __shfl(mine, u0 + u);
+        code[u] = __shfl(mycode, u0 + u);
       }
-    }
-    if (lane == 0) {
+      for (int j0 = 0; j0 < p; j0 += kWave) {
+        const int j = j0 + lane;
+        // Unconditional loads: rows past the batch re-read the batch's first
+        // row and columns past p column p - 1; neither is added.
+        float xv[16];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (row[u] >= 0) ++cnt[code[u]];
+        for (int u = 0; u < 16; ++u) {
+          const int r = u0 + u < cnt_rows ? row[u] : row[0];
+          xv[u] = prm.x[static_cast<size_t>(r) * p + min(j, p - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          if (u0 + u < cnt_rows && j < p) my[code[u] * p + j] += static_cast<double>(xv[u]);
+        }
       }
     }
   }
   __syncthreads();
-  double* out = prm.fsum + static_cast<size_t>(f) * C * p;
-  for (int e = lane; e < C * p; e += kE) out[e] = acc[e];
+  double* out = prm.fsum + static_cast<size_t>(f) * cp;
+  for (size_t e = threadIdx.x; e < cp; e += blockDim.x) {
+    double t = acc[e];
+    for (int w = 1; w < waves; ++w) t += acc[w * cp + e];
+    out[e] = t;
+  }
   int* oc = prm.fcnt + static_cast<size_t>(f) * C;
-  for (int c = lane; c < C; c += kE) oc[c] = cnt[c];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    int t = 0;
+    for (int w = 0; w < waves; ++w) t += cnt[w * C + c];
+    oc[c] = t;
+  }
 }
 
 // One wave per fold (see the file comment): S from the class sums of the
 // run's other folds, SIMPLS, the LDA or PLS-DA head, the validation argmax.
-__global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int compact) {
+template <bool COMPACT>
+__global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm) {
   extern __shared__ double pls_lds[];
   const int f = blockIdx.x;
   const int p = prm.p;
   const int lane = threadIdx.x;
   const int C = prm.max_classes;
   const int rcap = prm.rank_cap;
-  const EvalLds L = eval_carve(pls_lds, p, C, rcap, compact != 0);
+  const EvalLds L = eval_carve(pls_lds, p, C, rcap, COMPACT);
   const size_t fA = static_cast<size_t>(f) * kA;
   const double* Gg = prm.gram + static_cast<size_t>(f) * p * p;
-  const double* G = L.G ? L.G : Gg;
+  // The compact layout's arrays are all LDS (known at compile time, so their
+  // accesses are LDS instructions); the wide layout's are global.
+  const double* G = COMPACT ? L.G : Gg;
   double* Wg = prm.weights + fA * p;
-  double* W = L.W ? L.W : Wg;
-  double* V = L.V ? L.V : prm.loadings + fA * p;
-  double* GW = L.GW ? L.GW : prm.gw + fA * p;
-  double* Z = L.Z ? L.Z : prm.zsum + static_cast<size_t>(f) * C * p;
-  double* MU = L.mu ? L.mu : prm.mu + static_cast<size_t>(f) * C * kA;
-  double* LIN = L.LIN ? L.LIN : prm.linear + static_cast<size_t>(f) * C * kA;
-  double* CONST = L.CONST ? L.CONST : prm.constant + static_cast<size_t>(f) * C;
+  double* W = COMPACT ? L.W : Wg;
+  double* V = COMPACT ? L.V : prm.loadings + fA * p;
+  double* GW = COMPACT ? L.GW : prm.gw + fA * p;
+  double* Z = COMPACT ? L.Z : prm.zsum + static_cast<size_t>(f) * C * p;
+  double* MU = COMPACT ? L.mu : prm.mu + static_cast<size_t>(f) * C * kA;
+  double* LIN = COMPACT ? L.LIN : prm.linear + static_cast<size_t>(f) * C * kA;
+  double* CONST = COMPACT ? L.CONST : prm.constant + static_cast<size_t>(f) * C;
   const int v0 = prm.val.offsets[f];
   const int nv = prm.val.offsets[f + 1] - v0;
   const int* vrows = prm.val.rows + v0;
@@ -505,12 +611,22 @@ __global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int com
   double* TV = prm.val_scores + static_cast<size_t>(v0) * kA;       // [i * kA + a]
   double* DISC = prm.discriminants + static_cast<size_t>(v0) * C;    // [i * C + c]
 
-  for (int j = lane; j < p; j += kE) {
-    L.mean[j] = prm.mean[static_cast<size_t>(f) * p + j];
-    L.sd[j] = prm.sd[static_cast<size_t>(f) * p + j];
+  for (int j = lane; j < eval_cols(p); j += kE) {
+    L.mean[j] = j < p ? prm.mean[static_cast<size_t>(f) * p + j] : 0.0;
+    L.sd[j] = j < p ? prm.sd[static_cast<size_t>(f) * p + j] : 1.0;
   }
-  if (L.G) {
-    for (int e = lane; e < p * p; e += kE) L.G[e] = Gg[e];
+  if (COMPACT) {
+    // Eight loads in flight per lane.
+    for (int e0 = 0; e0 < p * p; e0 += 8 * kE) {
+      double g[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) g[u] = Gg[min(e0 + u * kE + lane, p * p - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * kE + lane;
+        if (e < p * p) L.G[e] = g[u];
+      }
+    }
   }
   // Training rows per class: the run's other folds.
   const int g0 = prm.run_first[f], g1 = prm.run_end[f];
@@ -557,9 +673,20 @@ __global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int com
     const int nc = L.cnt[c];
     const double ym = static_cast<double>(nc) * inv_nt;
     for (int j = lane; j < p; j += kE) {
+      // The other folds' sums in ascending fold order, four loads in flight
+      // (a skipped fold adds 0).
       double z = 0.0;
-      for (int g = g0; g < g1; ++g) {
-        if (g != f) z += prm.fsum[(static_cast<size_t>(g) * C + c) * p + j];
+      for (int gb = g0; gb < g1; gb += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int g = min(gb + u, g1 - 1);
+          v[u] = prm.fsum[(static_cast<size_t>(g) * C + c) * p + j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = gb + u < g1 && gb + u != f ? v[u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) z += v[u];
       }
       const double zs = (z - static_cast<double>(nc) * L.mean[j]) / L.sd[j];
       Z[ci * p + j] = zs;
@@ -596,42 +723,99 @@ __global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int com
     prm.components[f] = A;
     prm.rank[f] = fitted;
   }
-  if (L.W) {
+  if (COMPACT) {
     for (int e = lane; e < fitted * p; e += kE) Wg[e] = L.W[e];
   }
 
   // -- class statistics ------------------------------------------------------------
-  // GW = G W (component-major).
-  for (int a = 0; a < A; ++a) {
+  // Compact layout with up to 8 components: the weights zero-padded to
+  // 8 x 64, so that the products below read them without bounds checks.
+  const bool padded = COMPACT && A <= 8;
+  if (padded) {
+    for (int e = lane; e < 8 * 64; e += kE) {
+      const int u = e >> 6, j = e & 63;
+      L.Wp[e] = u < A && j < p ? W[static_cast<size_t>(u) * p + j] : 0.0;
+    }
+    __syncthreads();
     for (int i = lane; i < p; i += kE) {
-      double s = 0.0;
-      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(j) * p + i] * W[static_cast<size_t>(a) * p + j];
-      GW[static_cast<size_t>(a) * p + i] = s;
+      double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+      for (int j = 0; j < p; ++j) {
+        const double g = G[static_cast<size_t>(j) * p + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += g * L.Wp[u * 64 + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (u < A) GW[static_cast<size_t>(u) * p + i] = s[u];
+      }
+    }
+  }
+  // GW = G W (component-major), eight components per pass over G.
+  for (int a0 = 0; a0 < (padded ? 0 : A); a0 += 8) {
+    for (int i = lane; i < p; i += kE) {
+      double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < p; ++j) {
+        const double g = G[static_cast<size_t>(j) * p + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (a0 + u < A) s[u] += g * W[static_cast<size_t>(a0 + u) * p + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (a0 + u < A) GW[static_cast<size_t>(a0 + u) * p + i] = s[u];
+      }
     }
   }
   // PLS-DA: T'(Y - ybar) = W'S0 with S0 = Z - (n_c / n) xsum; LDA: the class
   // score means W'Z[c] / n_c. Kept in MU as [c * A + a].
-  for (int ci = 0; ci < cf; ++ci) {
-    const int cnt = L.cnt[L.segc[ci]];
-    const double ym = static_cast<double>(cnt) * inv_nt;
-    for (int a = 0; a < A; ++a) {
-      double s = 0.0;
-      for (int j = lane; j < p; j += kE) {
-        const double zj = prm.da ? Z[ci * p + j] - ym * xsum[j] : Z[ci * p + j];
-        s += W[static_cast<size_t>(a) * p + j] * zj;
+  for (int q0 = 0; q0 < cf * A; q0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      v[u] = 0.0;
+      const int q = q0 + u;
+      if (q < cf * A) {
+        const int ci = q / A, a = q - ci * A;
+        const double ym = static_cast<double>(L.cnt[L.segc[ci]]) * inv_nt;
+        for (int j = lane; j < p; j += kE) {
+          const double zj = prm.da ? Z[ci * p + j] - ym * xsum[j] : Z[ci * p + j];
+          v[u] += W[static_cast<size_t>(a) * p + j] * zj;
+        }
       }
-      s = lane_total(s);
-      if (lane == 0) MU[ci * A + a] = prm.da ? s : (cnt > 0 ? s * (1.0 / static_cast<double>(cnt)) : 0.0);
+    }
+    lane_totals(v);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int q = q0 + u;
+        if (q < cf * A) {
+          const int cnt = L.cnt[L.segc[q / A]];
+          MU[q] = prm.da ? v[u] : (cnt > 0 ? v[u] * (1.0 / static_cast<double>(cnt)) : 0.0);
+        }
+      }
     }
   }
   __syncthreads();
   // T'T = W' G W.
-  for (int r = 0; r < A; ++r) {
-    for (int c2 = 0; c2 < A; ++c2) {
-      double s = 0.0;
-      for (int j = lane; j < p; j += kE) s += W[static_cast<size_t>(r) * p + j] * GW[static_cast<size_t>(c2) * p + j];
-      s = lane_total(s);
-      if (lane == 0) L.tt[r * A + c2] = s;
+  for (int q0 = 0; q0 < A * A; q0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      v[u] = 0.0;
+      const int q = q0 + u;
+      if (q < A * A) {
+        const int r = q / A, c2 = q - r * A;
+        for (int j = lane; j < p; j += kE) v[u] += W[static_cast<size_t>(r) * p + j] * GW[static_cast<size_t>(c2) * p + j];
+      }
+    }
+    lane_totals(v);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (q0 + u < A * A) L.tt[q0 + u] = v[u];
+      }
     }
   }
   __syncthreads();
@@ -774,33 +958,90 @@ __global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm, int com
     __syncthreads();
   }
 
+  if (padded) {
+    for (int e = lane; e < cf * 8; e += kE) {
+      const int c = e >> 3, u = e & 7;
+      L.LINp[e] = u < A ? LIN[static_cast<size_t>(c) * A + u] : 0.0;
+    }
+    __syncthreads();
+  }
   // -- validation scores, discriminants, argmax (one lane per row) ----------------
   for (int i = lane; i < nv; i += kE) {
     const size_t row = static_cast<size_t>(vrows[i]) * p;
     double* tv = TV + static_cast<size_t>(i) * kA;
-    for (int a0 = 0; a0 < A; a0 += 8) {
+    int best = 0;
+    double best_s = -INFINITY;
+    if (padded) {
+      // p <= 64, 64 columns in chunks of 16 whose loads are issued one chunk
+      // ahead (columns past p re-read column p - 1, have mean 0, sd 1 and
+      // zero weights, so they add exact zeros); the scores stay in registers
+      // for the discriminants.
       double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int j = 0; j < p; ++j) {
-        const double z = (static_cast<double>(prm.x[row + j]) - L.mean[j]) / L.sd[j];
+      float cur[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (a0 + u < A) t[u] += z * W[static_cast<size_t>(a0 + u) * p + j];
+      for (int v = 0; v < 16; ++v) cur[v] = prm.x[row + min(v, p - 1)];
+#pragma unroll 1
+      for (int j0 = 0; j0 < 64; j0 += 16) {
+        float nxt[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) nxt[v] = prm.x[row + min(j0 + 16 + v, p - 1)];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int j = j0 + v;
+          const double z = (static_cast<double>(cur[v]) - L.mean[j]) / L.sd[j];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] += z * L.Wp[u * 64 + j];
         }
+#pragma unroll
+        for (int v = 0; v < 16; ++v) cur[v] = nxt[v];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (a0 + u < A) tv[a0 + u] = t[u];
+        if (u < A) tv[u] = t[u];
       }
-    }
-    int best = 0;
-    double best_s = -INFINITY;
-    for (int c = 0; c < cf; ++c) {
-      double sc = CONST[c];
-      for (int a = 0; a < A; ++a) sc += tv[a] * LIN[static_cast<size_t>(c) * A + a];
-      DISC[static_cast<size_t>(i) * C + c] = sc;
-      if (sc > best_s) {
-        best_s = sc;
-        best = c;
+      for (int c = 0; c < cf; ++c) {
+        double sc = CONST[c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sc += t[u] * L.LINp[c * 8 + u];
+        DISC[static_cast<size_t>(i) * C + c] = sc;
+        if (sc > best_s) {
+          best_s = sc;
+          best = c;
+        }
+      }
+    } else {
+      for (int a0 = 0; a0 < A; a0 += 8) {
+        double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // Sixteen of the row's loads in flight before their use.
+        for (int j0 = 0; j0 < p; j0 += 16) {
+          float xv[16];
+#pragma unroll
+          for (int v = 0; v < 16; ++v) xv[v] = prm.x[row + min(j0 + v, p - 1)];
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int j = j0 + v;
+            if (j < p) {
+              const double z = (static_cast<double>(xv[v]) - L.mean[j]) / L.sd[j];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                if (a0 + u < A) t[u] += z * W[static_cast<size_t>(a0 + u) * p + j];
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (a0 + u < A) tv[a0 + u] = t[u];
+        }
+      }
+      for (int c = 0; c < cf; ++c) {
+        double sc = CONST[c];
+        for (int a = 0; a < A; ++a) sc += tv[a] * LIN[static_cast<size_t>(c) * A + a];
+        DISC[static_cast<size_t>(i) * C + c] = sc;
+        if (sc > best_s) {
+          best_s = sc;
+          best = c;
+        }
       }
     }
     prm.pred_code[vrows[i]] = L.segc[best];
@@ -999,12 +1240,16 @@ hipError_t launch_pls_fold_sums(const PlsFoldSumsParams& prm, hipStream_t s) {
       static_cast<size_t>(prm.max_classes) * prm.p > kPlsMaxSLds) {
     return hipErrorInvalidValue;
   }
-  const size_t shmem = sizeof(double) * static_cast<size_t>(prm.max_classes) * prm.p +
-                       sizeof(int) * static_cast<size_t>(prm.max_classes);
+  // Up to four waves, as many as keep the partials within 64 KiB (one wave
+  // at the largest class x column tables).
+  const size_t per_wave = sizeof(double) * static_cast<size_t>(prm.max_classes) * prm.p +
+                          sizeof(int) * static_cast<size_t>(prm.max_classes);
+  const int waves = static_cast<int>(std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per_wave)));
+  const size_t shmem = per_wave * waves;
   const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_fold_sums_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pls_fold_sums_kernel, dim3(prm.folds), dim3(kE), shmem, s, prm);
+  hipLaunchKernelGGL(pls_fold_sums_kernel, dim3(prm.folds), dim3(kWave * waves), shmem, s, prm, waves);
   return hipGetLastError();
 }
 
@@ -1014,7 +1259,7 @@ size_t pls_eval_lds_bytes(int p, int max_classes, int rank_cap, bool compact) {
 }
 
 bool pls_eval_compact(int p, int max_classes, int rank_cap) {
-  return pls_eval_lds_bytes(p, max_classes, rank_cap, true) <= 64 * 1024;
+  return p <= 64 && pls_eval_lds_bytes(p, max_classes, rank_cap, true) <= 64 * 1024;
 }
 
 hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s) {
@@ -1034,10 +1279,15 @@ hipError_t launch_pls_eval(const PlsEvalParams& prm, hipStream_t s) {
   const bool compact = pls_eval_compact(prm.p, prm.max_classes, prm.rank_cap);
   const size_t shmem = pls_eval_lds_bytes(prm.p, prm.max_classes, prm.rank_cap, compact);
   if (shmem > 160 * 1024) return hipErrorInvalidValue;
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_eval_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
+  const void* fn = compact ? reinterpret_cast<const void*>(&pls_eval_kernel<true>)
+                           : reinterpret_cast<const void*>(&pls_eval_kernel<false>);
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pls_eval_kernel, dim3(prm.folds), dim3(kE), shmem, s, prm, compact ? 1 : 0);
+  if (compact) {
+    hipLaunchKernelGGL(pls_eval_kernel<true>, dim3(prm.folds), dim3(kE), shmem, s, prm);
+  } else {
+    hipLaunchKernelGGL(pls_eval_kernel<false>, dim3(prm.folds), dim3(kE), shmem, s, prm);
+  }
   return hipGetLastError();
 }
 
