@@ -160,6 +160,37 @@ def cpu_baseline(x, rows, init, k, cycles, cores, min_seconds):
                 wall_seconds=wall, mean_accbest=float(np.mean(accs)))
 
 
+def pls_cpu_baseline(x, rows, init, kw, cycles, cores, min_seconds):
+    """Reference CorePLSLDA_CPU (oracle/_ref) on the runs of the PLS-LDA
+    workload, `cores` runs at a time, until min_seconds have elapsed (at least
+    one group of runs)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if not O.Reference.available():
+        return None
+    eng = O.Reference()
+
+    def one(r):
+        o = O.core_options(cycles=cycles, **kw)
+        o.seed = 1 + r
+        o.cv_seed = 1 + r
+        return eng.core_plslda(x[rows[r]], init[r], o)["accbest"]
+
+    done, t0 = 0, time.perf_counter()
+    accs = []
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        while True:
+            group = [(done + i) % len(rows) for i in range(cores)]
+            accs += list(pool.map(one, group))
+            done += cores
+            if time.perf_counter() - t0 >= min_seconds:
+                break
+    wall = time.perf_counter() - t0
+    return dict(value=done * cycles / wall, unit="steps/s", cores=cores, kind="reference",
+                sample=f"{done} runs x {cycles} cycles of this workload (CorePLSLDA_CPU), {cores} threads",
+                wall_seconds=wall, mean_accbest=float(np.mean(accs)))
+
+
 def copy_probe() -> int:
     """A plain device-to-device copy of 256 MiB, three times: the counter
     calibration (FETCH_SIZE / WRITE_SIZE per KiB actually moved)."""
@@ -314,6 +345,10 @@ def pls_form(args) -> dict:
     out = K.core_plslda_batch(x, rows, init, K.core_options(cycles=args.steps, **kw))
     L = sum(len(r) for r in rows)
     kern = out["eval_kernel_seconds"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        ckw = {k: v for k, v in kw.items() if k != "n_threads"}
+        cpu = pls_cpu_baseline(x, rows, init, ckw, args.steps, min(16, os.cpu_count() or 1), 5.0)
     algo = float(L) * (50 * 4 + 12)  # bytes per evaluation of all runs
     return dict(workload=f"PLS-LDA: X=10000x50 latent + 8 shifted clusters, M={args.runs}, 2000 landmarks, "
                          f"10 folds, ncomp=5, Tcycle={args.steps}",
@@ -323,7 +358,8 @@ def pls_form(args) -> dict:
                 roofline={"kernel": "pls_eval_kernel", "bound": "latency (block barriers, FP64)",
                           "achieved": algo / kern / 1e9 if kern > 0 else 0.0, "unit": "GB/s",
                           "peak": HBM_PEAK_GBS, "frac": algo / kern / 1e9 / HBM_PEAK_GBS if kern > 0 else 0.0,
-                          "algorithmic_bytes_per_launch": algo, "mfma": "none (class sums)"})
+                          "algorithmic_bytes_per_launch": algo, "mfma": "none (class sums)"},
+                cpu_baseline=cpu)
 
 
 def pls_fit_bench(rows: int, ncomp: int) -> int:

@@ -227,7 +227,8 @@ int kodama_hip_plsdacv(const float* x, int32_t n, int32_t p, const int32_t* labe
  * coefficients [p x q], fitted [n x q], x_center / x_scale [p], y_center [q];
  * *components = A. timings[2] (may be NULL): the cross-product kernel and the
  * whole device fit (seconds, HIP events). A fit without a component is
- * ERUNTIME (the reference's DegeneratePLSFit). */
+ * ERUNTIME (the reference's DegeneratePLSFit). Limits (EINVAL beyond): p <= 256,
+ * q x p <= 8192, p + q <= 448, at most 64 components. */
 int kodama_hip_pls(const float* x, int32_t n, int32_t p, const float* y, int32_t q,
                    const kodama_hip_core_options* options, float* weights,
                    float* response_loadings, float* scores, float* coefficients, float* fitted,

@@ -330,12 +330,13 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
   const int requested = o.fixed_components > 0 ? o.fixed_components : o.max_components;
   if (requested < 1) throw InvalidArgument("PLS component count must be positive.");
   if (p > kPlsMaxDims) throw InvalidArgument("the HIP PLS kernels support up to 256 predictors.");
-  if (static_cast<size_t>(q) * p > static_cast<size_t>(kPlsMaxSLds) || pls_fit_lds_bytes(p, q) > 160 * 1024 ||
-      (p + q + 15) / 16 * 16 * 16 * sizeof(double) > 64 * 1024) {
-    throw InvalidArgument("the HIP PLS kernels hold responses x predictors <= 8192 and predictors + responses <= 512.");
-  }
   const int rank_cap = std::max(1, std::min({requested, p, std::max(1, n - 1)}));
   if (rank_cap > kPlsMaxComponents) throw InvalidArgument("the HIP PLS kernels support up to 64 components.");
+  if (static_cast<size_t>(q) * p > static_cast<size_t>(kPlsMaxSLds) ||
+      pls_fit_lds_bytes(p, q, rank_cap, false) > 160 * 1024 ||
+      (p + q + 15) / 16 * 16 * 18 * sizeof(double) > 64 * 1024) {
+    throw InvalidArgument("the HIP PLS kernels hold responses x predictors <= 8192 and predictors + responses <= 448.");
+  }
   Stream st(device);
   const hipStream_t s = st.get();
   DeviceBuffer<float> dx(static_cast<size_t>(n) * p), dy(static_cast<size_t>(n) * q);
@@ -415,13 +416,15 @@ PlsGeneralResult pls_general_fit(int device, const float* x, const float* y, int
   fp.yweights = Q.get();
   fp.coefficients = B.get();
   fp.fitted = dfit.get();
+  DeviceBuffer<double> wscratch(2 * static_cast<size_t>(kPlsMaxComponents) * p);
+  fp.wscratch = wscratch.get();
   check(launch_pls_fit(fp, s), "pls_fit launch");
   int A = 0;
   dfit.download(&A, 1, s);
   st.sync();
   if (A < 1) throw DeviceError("PLS: SIMPLS found no supervised latent direction.");
   DeviceBuffer<float> dsc(static_cast<size_t>(n) * A), dft(static_cast<size_t>(n) * q);
-  DeviceBuffer<double> tbuf(static_cast<size_t>(n) * A);
+  DeviceBuffer<double> tbuf(pls_scores_staged(p, q, A) ? 0 : static_cast<size_t>(n) * A);
   PlsScoreParams sp;
   sp.x = dx.get();
   sp.n = n;

@@ -52,141 +52,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum of v over the block; every thread gets the total. `red` holds kWaves
-// doubles; two barriers make back-to-back calls safe.
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  v = wave_sum(v);
-  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) t += red[w];
-  __syncthreads();
-  return t;
-}
-
 }  // namespace
-
-// Power-refresh SIMPLS (plscv.cpp:816-955) of the cross-product S [m x p]
-// (LDS, row c at S[c * p], deflated in place) with the Gram matrix G [p x p]:
-// per component the previous weight (w0 for the first) takes two S'S power
-// steps, is normalised (a failed first refresh restarts from the strongest
-// row of S), scaled to unit score norm through G, its loading G w is
-// Gram-Schmidt orthogonalised once against the earlier loadings and
-// normalised, and S is deflated. Weights and loadings go to W and V
-// ([j * kA + a], global). Returns the fitted rank (0: degenerate). wt, ld
-// [p], rv [m] and red [kWaves] are LDS scratch, s_int[0] an LDS int. Every
-// thread of the block calls it.
-__device__ int simpls_power_fit(double* S, int m, int p, int rc, const double* G, const double* w0, double* W,
-                                double* V, double* wt, double* ld, double* rv, double* red, int* s_int) {
-  const int tid = threadIdx.x;
-  const int cf = m;
-  int fitted = 0;
-  auto refresh = [&]() {
-    for (int it = 0; it < 2; ++it) {
-      for (int c = tid; c < cf; c += kT) {
-        double a = 0.0;
-        for (int j = 0; j < p; ++j) a += S[c * p + j] * wt[j];
-        rv[c] = a;
-      }
-      __syncthreads();
-      for (int j = tid; j < p; j += kT) {
-        double a = 0.0;
-        for (int c = 0; c < cf; ++c) a += S[c * p + j] * rv[c];
-        wt[j] = a;
-      }
-      __syncthreads();
-    }
-  };
-  auto norm_of = [&](const double* v) {
-    double part = 0.0;
-    for (int j = tid; j < p; j += kT) part += v[j] * v[j];
-    return block_sum(part, red);
-  };
-  for (int a = 0; a < rc; ++a) {
-    if (a == 0) {
-      for (int j = tid; j < p; j += kT) wt[j] = w0[j];
-      __syncthreads();
-    }
-    refresh();
-    double ss = norm_of(wt);
-    double nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
-    if (a == 0 && (!isfinite(nrm) || nrm <= kNormFloor)) {
-      // The strongest column of S restarts the power iteration.
-      if (tid == 0) {
-        int strongest = -1;
-        double best = 0.0;
-        for (int c = 0; c < cf; ++c) {
-          double sq = 0.0;
-          for (int j = 0; j < p; ++j) sq += S[c * p + j] * S[c * p + j];
-          if (isfinite(sq) && sq > best) {
-            best = sq;
-            strongest = c;
-          }
-        }
-        s_int[0] = strongest >= 0 && best > 1.0e-20 ? strongest : -1;
-      }
-      __syncthreads();
-      const int strongest = s_int[0];
-      if (strongest >= 0) {
-        for (int j = tid; j < p; j += kT) wt[j] = S[strongest * p + j];
-        __syncthreads();
-        refresh();
-        ss = norm_of(wt);
-        nrm = isfinite(ss) && ss >= 0.0 ? sqrt(ss) : INFINITY;
-      }
-    }
-    if (!isfinite(nrm) || nrm <= kNormFloor) break;
-    for (int j = tid; j < p; j += kT) wt[j] /= nrm;
-    __syncthreads();
-    for (int i = tid; i < p; i += kT) {
-      double s = 0.0;
-      for (int j = 0; j < p; ++j) s += G[static_cast<size_t>(i) * p + j] * wt[j];
-      ld[i] = s;
-    }
-    __syncthreads();
-    double part = 0.0;
-    for (int j = tid; j < p; j += kT) part += wt[j] * ld[j];
-    const double tnorm = block_sum(part, red);
-    if (!isfinite(tnorm) || tnorm <= kTinyFloor) break;
-    const double inv = 1.0 / sqrt(tnorm);
-    for (int j = tid; j < p; j += kT) {
-      wt[j] *= inv;
-      ld[j] *= inv;
-    }
-    __syncthreads();
-    // One sequential Gram-Schmidt pass against the earlier loadings.
-    for (int b = 0; b < a; ++b) {
-      double pp = 0.0;
-      for (int j = tid; j < p; j += kT) pp += V[static_cast<size_t>(j) * kA + b] * ld[j];
-      const double proj = block_sum(pp, red);
-      for (int j = tid; j < p; j += kT) ld[j] -= proj * V[static_cast<size_t>(j) * kA + b];
-      __syncthreads();
-    }
-    const double lnn = norm_of(ld);
-    const double ln = sqrt(fmax(0.0, lnn));
-    if (!isfinite(ln) || ln <= kNormFloor) break;
-    for (int j = tid; j < p; j += kT) {
-      ld[j] /= ln;
-      W[static_cast<size_t>(j) * kA + a] = wt[j];
-      V[static_cast<size_t>(j) * kA + a] = ld[j];
-    }
-    __syncthreads();
-    for (int c = tid; c < cf; c += kT) {
-      double s = 0.0;
-      for (int j = 0; j < p; ++j) s += ld[j] * S[c * p + j];
-      rv[c] = s;
-    }
-    __syncthreads();
-    for (int q = tid; q < cf * p; q += kT) {
-      const int c = q / p, j = q - c * p;
-      S[q] -= ld[j] * rv[c];
-    }
-    __syncthreads();
-    fitted = a + 1;
-  }
-  return fitted;
-}
 
 // ---- fold cache ---------------------------------------------------------------
 
@@ -1050,29 +916,66 @@ __global__ void __launch_bounds__(kE) pls_eval_kernel(PlsEvalParams prm) {
 
 // ---- general PLS ------------------------------------------------------------------
 
+// One chunk of rows per workgroup: the threads form R = 256 / cols row lanes
+// (at least one) of `cols` column lanes; row lane l sums rows l, l + R, ...
+// of the chunk for its column, four loads in flight, and the R partials are
+// added in row-lane order. Consecutive threads read consecutive columns of
+// one row.
 __global__ void __launch_bounds__(kT) pls_colstats_kernel(PlsColStatsParams prm) {
+  __shared__ double part[kT];
+  const int cols = prm.cols;
   const int r0 = blockIdx.x * prm.chunk_rows;
   const int r1 = min(prm.n, r0 + prm.chunk_rows);
-  for (int c = threadIdx.x; c < prm.cols; c += kT) {
+  for (int c0 = 0; c0 < cols; c0 += kT) {
+    const int width = min(kT, cols - c0);
+    const int lanes = max(1, kT / width);
+    const int rl = threadIdx.x / width, c = c0 + threadIdx.x % width;
+    const bool active = rl < lanes && threadIdx.x < lanes * width;
+    const double mu = prm.pass == 0 ? 0.0 : prm.mean[min(c, cols - 1)];
     double acc = 0.0;
-    if (prm.pass == 0) {
-      for (int r = r0; r < r1; ++r) acc += static_cast<double>(prm.m[static_cast<size_t>(r) * prm.cols + c]);
-    } else {
-      const double mu = prm.mean[c];
-      for (int r = r0; r < r1; ++r) {
-        const double d = static_cast<double>(prm.m[static_cast<size_t>(r) * prm.cols + c]) - mu;
-        acc += d * d;
+    if (active) {
+      int r = r0 + rl;
+      for (; r + 3 * lanes < r1; r += 4 * lanes) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = prm.m[static_cast<size_t>(r + u * lanes) * cols + c];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double d = static_cast<double>(v[u]) - mu;
+          acc += prm.pass == 0 ? static_cast<double>(v[u]) : d * d;
+        }
+      }
+      for (; r < r1; r += lanes) {
+        const double v = static_cast<double>(prm.m[static_cast<size_t>(r) * cols + c]);
+        acc += prm.pass == 0 ? v : (v - mu) * (v - mu);
       }
     }
-    prm.partial[static_cast<size_t>(blockIdx.x) * prm.cols + c] = acc;
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < width) {
+      double t = 0.0;
+      for (int l = 0; l < lanes; ++l) t += part[l * width + threadIdx.x];
+      prm.partial[static_cast<size_t>(blockIdx.x) * cols + c0 + threadIdx.x] = t;
+    }
+    __syncthreads();
   }
 }
 
-__global__ void pls_colstats_reduce_kernel(PlsColStatsParams prm, int scale) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= prm.cols) return;
+// Chunk partials of one column per workgroup: thread t sums chunks t, t + 256,
+// ... in order, then a fixed tree over the threads.
+__global__ void __launch_bounds__(kT) pls_colstats_reduce_kernel(PlsColStatsParams prm, int scale) {
+  __shared__ double red[kT];
+  const int c = blockIdx.x;
   double t = 0.0;
-  for (int k = 0; k < prm.chunks; ++k) t += prm.partial[static_cast<size_t>(k) * prm.cols + c];
+  for (int k = threadIdx.x; k < prm.chunks; k += kT) t += prm.partial[static_cast<size_t>(k) * prm.cols + c];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = kT / 2; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  t = red[0];
   if (prm.pass == 0) {
     prm.mean[c] = prm.center ? t / static_cast<double>(prm.n) : 0.0;
     if (prm.sum) prm.sum[c] = t;
@@ -1088,53 +991,79 @@ __global__ void pls_colstats_reduce_kernel(PlsColStatsParams prm, int scale) {
 
 namespace {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-constexpr int kCrossTilesPerWave = 4;
+constexpr int kCrossTilesPerWave = 8;
 constexpr int kCrossTilesPerBlock = kCrossTilesPerWave * kWaves;
 }  // namespace
 
-// Z = Xs' [Xs | Yc] over one chunk of rows (blockIdx.x) for one group of 16 x
-// 16 output tiles (blockIdx.y). Rows are staged 16 at a time in LDS as
-// doubles (standardised X columns, centred Y columns, zero padding); each
-// wave accumulates up to four tiles with v_mfma_f64_16x16x4f64 (A: lane l
+// Z = Xs' [Xs | Yc] over one chunk of rows (blockIdx.x) for one group of up
+// to 32 of the 16 x 16 output tiles (blockIdx.y). Rows go through LDS 16 at a
+// time as doubles (standardised X columns, centred Y columns, zero padding);
+// each thread holds PER = z16 / 16 elements of a 16-row block, and the next
+// block's elements are loaded (without bounds checks: indices are clamped and
+// the values masked) while the matrix cores work on the current one. Each
+// wave accumulates up to eight tiles with v_mfma_f64_16x16x4f64 (A: lane l
 // holds Z'[tile row l % 16][k = l / 16], B: Z[k = l / 16][tile column l % 16];
 // D: register i of lane l holds row 4 i + l / 16, column l % 16).
+template <int PER>
 __global__ void __launch_bounds__(kT) pls_cross_kernel(PlsCrossParams prm) {
-  extern __shared__ double zst[];  // [16][z16]
-  const int p = prm.p, q = prm.q, zc = p + q;
+  extern __shared__ double zst[];  // [16][z16], then scale [z16] and shift [z16]
+  const int p = prm.p, q = prm.q, zc = p + q, n = prm.n;
   const int p16 = (p + 15) / 16 * 16, z16 = (zc + 15) / 16 * 16;
   const int tiles_c = z16 / 16, tiles = (p16 / 16) * tiles_c;
+  double* scale = zst + 16 * z16;
+  double* shift = scale + z16;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const int r0 = blockIdx.x * prm.chunk_rows;
-  const int r1 = min(prm.n, r0 + prm.chunk_rows);
-  const int tile0 = blockIdx.y * kCrossTilesPerBlock + wave * kCrossTilesPerWave;
+  const int r1 = min(n, r0 + prm.chunk_rows);
+  // Tiles of the block dealt round robin: wave w takes tiles w, w + 4, ...
+  const int tile0 = blockIdx.y * kCrossTilesPerBlock + wave;
+  // Column c of Z is (v - shift[c]) * scale[c]: X columns standardised
+  // (1 / sd), Y columns centred, padding columns zero.
+  for (int c = threadIdx.x; c < z16; c += kT) {
+    scale[c] = c < p ? 1.0 / prm.x_sd[c] : (c < zc ? 1.0 : 0.0);
+    shift[c] = c < p ? prm.x_mean[c] : (c < zc ? prm.y_mean[c - p] : 0.0);
+  }
   f64x4 acc[kCrossTilesPerWave];
 #pragma unroll
   for (int t = 0; t < kCrossTilesPerWave; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
   const int kr = lane / 16, kc = lane % 16;
+  // Element k of this thread in a block: e = threadIdx.x + k * 256 < 16 z16.
+  auto fetch = [&](int rb, float (&xv)[PER], float (&yv)[PER]) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = min(static_cast<int>(threadIdx.x) + k * kT, 16 * z16 - 1);
+      const int rr = e / z16, c = e - rr * z16;
+      const int r = min(rb + rr, n - 1);
+      xv[k] = prm.x[static_cast<size_t>(r) * p + min(c, p - 1)];
+      yv[k] = q > 0 ? prm.y[static_cast<size_t>(r) * q + min(max(c - p, 0), q - 1)] : 0.0f;
+    }
+  };
+  float xv[PER], yv[PER];
+  fetch(r0, xv, yv);
+  __syncthreads();
   for (int rb = r0; rb < r1; rb += 16) {
-    for (int e = threadIdx.x; e < 16 * z16; e += kT) {
-      const int rr = e / z16, c = e - rr * z16, r = rb + rr;
-      double v = 0.0;
-      if (r < r1) {
-        if (c < p) {
-          v = (static_cast<double>(prm.x[static_cast<size_t>(r) * p + c]) - prm.x_mean[c]) / prm.x_sd[c];
-        } else if (c < zc) {
-          v = static_cast<double>(prm.y[static_cast<size_t>(r) * q + (c - p)]) - prm.y_mean[c - p];
-        }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = static_cast<int>(threadIdx.x) + k * kT;
+      if (e < 16 * z16) {
+        const int rr = e / z16, c = e - rr * z16;
+        const double v = c < p ? static_cast<double>(xv[k]) : static_cast<double>(yv[k]);
+        zst[e] = rb + rr < r1 ? (v - shift[c]) * scale[c] : 0.0;
       }
-      zst[e] = v;
     }
     __syncthreads();
+    if (rb + 16 < r1) fetch(rb + 16, xv, yv);  // in flight during the products
 #pragma unroll
     for (int t = 0; t < kCrossTilesPerWave; ++t) {
-      const int tile = tile0 + t;
-      if (tile >= tiles) break;
-      const int tj = tile / tiles_c, tc = tile - tj * tiles_c;
+      const int tile = tile0 + t * kWaves;
+      if (tile < tiles) {
+        const int tj = tile / tiles_c, tc = tile - tj * tiles_c;
 #pragma unroll
-      for (int k0 = 0; k0 < 16; k0 += 4) {
-        const double a = zst[(k0 + kr) * z16 + tj * 16 + kc];
-        const double b = zst[(k0 + kr) * z16 + tc * 16 + kc];
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const double a = zst[(k0 + kr) * z16 + tj * 16 + kc];
+          const double b = zst[(k0 + kr) * z16 + tc * 16 + kc];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
@@ -1142,7 +1071,7 @@ __global__ void __launch_bounds__(kT) pls_cross_kernel(PlsCrossParams prm) {
   double* out = prm.partial + static_cast<size_t>(blockIdx.x) * p16 * z16;
 #pragma unroll
   for (int t = 0; t < kCrossTilesPerWave; ++t) {
-    const int tile = tile0 + t;
+    const int tile = tile0 + t * kWaves;
     if (tile >= tiles) break;
     const int tj = tile / tiles_c, tc = tile - tj * tiles_c;
 #pragma unroll
@@ -1152,50 +1081,162 @@ __global__ void __launch_bounds__(kT) pls_cross_kernel(PlsCrossParams prm) {
   }
 }
 
-// Chunk partials summed in chunk order: Gram [p x p] and cross [q x p].
-__global__ void pls_cross_reduce_kernel(PlsCrossParams prm) {
+// Chunk partials of 16 outputs per workgroup: chunk lane l (16 of them) sums
+// chunks l, l + 16, ... in order for its output, and the 16 partials are
+// added in chunk-lane order. Gram [p x p] and cross [q x p].
+constexpr int kReduceOutputs = 16;
+__global__ void __launch_bounds__(kT) pls_cross_reduce_kernel(PlsCrossParams prm) {
+  __shared__ double part[kT];
   const int p = prm.p, zc = prm.p + prm.q;
   const int p16 = (p + 15) / 16 * 16, z16 = (zc + 15) / 16 * 16;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= p * zc) return;
-  const int j = e / zc, c = e - j * zc;
+  const int lanes = kT / kReduceOutputs;
+  const int o = threadIdx.x % kReduceOutputs, l = threadIdx.x / kReduceOutputs;
+  const int e = blockIdx.x * kReduceOutputs + o;
+  const int ec = min(e, p * zc - 1);
+  const int j = ec / zc, c = ec - j * zc;
   double t = 0.0;
-  for (int k = 0; k < prm.chunks; ++k) t += prm.partial[(static_cast<size_t>(k) * p16 + j) * z16 + c];
-  if (c < p) prm.gram[static_cast<size_t>(j) * p + c] = t;
-  else prm.cross[static_cast<size_t>(c - p) * p + j] = t;
+  for (int k = l; k < prm.chunks; k += lanes) t += prm.partial[(static_cast<size_t>(k) * p16 + j) * z16 + c];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  if (l != 0 || e >= p * zc) return;
+  double v = 0.0;
+  for (int q = 0; q < lanes; ++q) v += part[q * kReduceOutputs + o];
+  if (c < p) prm.gram[static_cast<size_t>(j) * p + c] = v;
+  else prm.cross[static_cast<size_t>(c - p) * p + j] = v;
 }
 
-__global__ void __launch_bounds__(kT) pls_fit_kernel(PlsFitParams prm) {
+// One wave: power-refresh SIMPLS of the cross-product (q rows in LDS) with
+// the Gram matrix, then the response weights Q = S0' W and the coefficients
+// W Q'. The weights and loadings are fitted component-major in `wscratch` and
+// written out as [j * kPlsMaxComponents + a].
+// STAGED: the Gram matrix, the weights and the loadings live in LDS (their
+// accesses are LDS instructions); otherwise they stay in global memory.
+template <bool STAGED>
+__global__ void __launch_bounds__(kE) pls_fit_kernel(PlsFitParams prm) {
   extern __shared__ double fit_lds[];
-  __shared__ int s_int[4];
-  const int p = prm.p, q = prm.q, tid = threadIdx.x;
-  double* S = fit_lds;                            // [q x p]
-  double* wt = S + static_cast<size_t>(q) * p;   // [p]
-  double* ld = wt + p;                            // [p]
-  double* rv = ld + p;                            // [q]
-  double* red = rv + q;                           // [kWaves]
-  for (int e = tid; e < q * p; e += kT) S[e] = prm.cross[e];
-  __syncthreads();
-  const int fitted = simpls_power_fit(S, q, p, prm.rank_cap, prm.gram, prm.w0, prm.weights, prm.loadings, wt, ld,
-                                      rv, red, s_int);
-  // Response weights of the scaled weights against the undeflated S
-  // (plscv.cpp:905), then the coefficients W Q'.
-  for (int e = tid; e < q * fitted; e += kT) {
-    const int c = e / fitted, a = e - c * fitted;
-    double v = 0.0;
-    for (int j = 0; j < p; ++j) v += prm.cross[static_cast<size_t>(c) * p + j] * prm.weights[static_cast<size_t>(j) * kA + a];
-    prm.yweights[static_cast<size_t>(c) * kA + a] = v;
+  const int p = prm.p, q = prm.q, lane = threadIdx.x, rc = prm.rank_cap;
+  double* S = fit_lds;                              // [q x p]
+  double* wt = S + static_cast<size_t>(q) * p;     // [p]
+  double* ld = wt + p;                              // [p]
+  double* rv = ld + p;                              // [q]
+  double* Ql = rv + q;                              // [q x rc] response weights
+  double* Wl = Ql + static_cast<size_t>(q) * rc;    // STAGED: [rc x p] weights, component-major
+  double* Vl = Wl + static_cast<size_t>(rc) * p;    // STAGED: [rc x p] loadings
+  double* Gl = Vl + static_cast<size_t>(rc) * p;    // STAGED: [p x p]
+  int* s_int = reinterpret_cast<int*>(STAGED ? Gl + static_cast<size_t>(p) * p : Wl);  // [4]
+  for (int e = lane; e < q * p; e += kE) S[e] = prm.cross[e];
+  if (STAGED) {
+    for (int e = lane; e < p * p; e += kE) Gl[e] = prm.gram[e];
   }
   __syncthreads();
-  for (int e = tid; e < p * q; e += kT) {
+  double* W = STAGED ? Wl : prm.wscratch;                                    // [a * p + j]
+  double* V = STAGED ? Vl : prm.wscratch + static_cast<size_t>(kA) * p;
+  const double* G = STAGED ? Gl : prm.gram;
+  const int fitted = simpls_wave(S, q, p, rc, G, prm.w0, W, V, wt, ld, rv, s_int);
+  for (int e = lane; e < fitted * p; e += kE) {
+    const int a = e / p, j = e - a * p;
+    prm.weights[static_cast<size_t>(j) * kA + a] = W[e];
+    prm.loadings[static_cast<size_t>(j) * kA + a] = V[e];
+  }
+  // Response weights of the scaled weights against the undeflated S
+  // (plscv.cpp:905), lanes over columns, eight sums at a time.
+  for (int e0 = 0; e0 < q * fitted; e0 += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      v[u] = 0.0;
+      const int e = e0 + u;
+      if (e < q * fitted) {
+        const int c = e / fitted, a = e - c * fitted;
+        for (int j = lane; j < p; j += kE) v[u] += prm.cross[static_cast<size_t>(c) * p + j] * W[static_cast<size_t>(a) * p + j];
+      }
+    }
+    lane_totals(v);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u;
+        if (e < q * fitted) {
+          const int c = e / fitted, a = e - c * fitted;
+          Ql[c * rc + a] = v[u];
+          prm.yweights[static_cast<size_t>(c) * kA + a] = v[u];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Coefficients W Q'.
+  for (int e = lane; e < p * q; e += kE) {
     const int j = e / q, c = e - j * q;
     double v = 0.0;
-    for (int a = 0; a < fitted; ++a) {
-      v += prm.weights[static_cast<size_t>(j) * kA + a] * prm.yweights[static_cast<size_t>(c) * kA + a];
-    }
+    for (int a = 0; a < fitted; ++a) v += W[static_cast<size_t>(a) * p + j] * Ql[c * rc + a];
     prm.coefficients[e] = v;
   }
-  if (tid == 0) prm.fitted[0] = fitted;
+  if (lane == 0) prm.fitted[0] = fitted;
+}
+
+// Scores and fitted values with up to 16 components, one row per thread: the
+// weights, response weights, means and scales staged in LDS, zero padded to
+// 16 components and a multiple of 16 columns (padding columns have mean 0,
+// scale 1 and zero weights, so they add exact zeros), and each row's columns
+// loaded sixteen at a time without bounds checks.
+constexpr int kScoreA = 16;
+__host__ __device__ inline int score_cols(int p) { return (p + 15) / 16 * 16; }
+__host__ __device__ inline size_t score_lds_doubles(int p, int q) {
+  return static_cast<size_t>(score_cols(p)) * (kScoreA + 2) + static_cast<size_t>(q) * (kScoreA + 1);
+}
+
+__global__ void __launch_bounds__(kT) pls_scores_staged_kernel(PlsScoreParams prm) {
+  extern __shared__ double sc_lds[];
+  const int p = prm.p, q = prm.q, A = prm.A, pc = score_cols(p);
+  double* w = sc_lds;                               // [pc x 16]
+  double* qw = w + static_cast<size_t>(pc) * kScoreA;  // [q x 16]
+  double* mean = qw + static_cast<size_t>(q) * kScoreA;  // [pc]
+  double* sd = mean + pc;                           // [pc]
+  double* yb = sd + pc;                             // [q]
+  for (int e = threadIdx.x; e < pc * kScoreA; e += kT) {
+    const int j = e / kScoreA, u = e % kScoreA;
+    w[e] = j < p && u < A ? prm.weights[static_cast<size_t>(j) * kA + u] : 0.0;
+  }
+  for (int e = threadIdx.x; e < q * kScoreA; e += kT) {
+    const int c = e / kScoreA, u = e % kScoreA;
+    qw[e] = u < A ? prm.yweights[static_cast<size_t>(c) * kA + u] : 0.0;
+  }
+  // sd holds the inverse scales here.
+  for (int j = threadIdx.x; j < pc; j += kT) {
+    mean[j] = j < p ? prm.x_mean[j] : 0.0;
+    sd[j] = j < p ? 1.0 / prm.x_sd[j] : 1.0;
+  }
+  for (int c = threadIdx.x; c < q; c += kT) yb[c] = prm.y_mean[c];
+  __syncthreads();
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= prm.n) return;
+  const size_t row = static_cast<size_t>(i) * p;
+  double t[kScoreA];
+#pragma unroll
+  for (int u = 0; u < kScoreA; ++u) t[u] = 0.0;
+  for (int j0 = 0; j0 < pc; j0 += 16) {
+    float xv[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) xv[v] = prm.x[row + min(j0 + v, p - 1)];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int j = j0 + v;
+      const double z = (static_cast<double>(xv[v]) - mean[j]) * sd[j];
+#pragma unroll
+      for (int u = 0; u < kScoreA; ++u) t[u] += z * w[j * kScoreA + u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kScoreA; ++u) {
+    if (u < A) prm.scores[static_cast<size_t>(i) * A + u] = static_cast<float>(t[u]);
+  }
+  for (int c = 0; c < q; ++c) {
+    double v = yb[c];
+#pragma unroll
+    for (int u = 0; u < kScoreA; ++u) v += t[u] * qw[c * kScoreA + u];
+    prm.fitted[static_cast<size_t>(i) * q + c] = static_cast<float>(v);
+  }
 }
 
 __global__ void pls_scores_kernel(PlsScoreParams prm) {
@@ -1299,46 +1340,79 @@ hipError_t launch_pls_colstats(const PlsColStatsParams& prm, hipStream_t s) {
 
 hipError_t launch_pls_colstats_reduce(const PlsColStatsParams& prm, int scale, hipStream_t s) {
   if (prm.cols == 0) return hipSuccess;
-  hipLaunchKernelGGL(pls_colstats_reduce_kernel, dim3((prm.cols + 127) / 128), dim3(128), 0, s, prm, scale);
+  hipLaunchKernelGGL(pls_colstats_reduce_kernel, dim3(prm.cols), dim3(kT), 0, s, prm, scale);
   return hipGetLastError();
 }
 
 hipError_t launch_pls_cross(const PlsCrossParams& prm, hipStream_t s) {
   if (prm.chunks == 0) return hipSuccess;
-  if (prm.p < 1 || prm.q < 0 || prm.chunk_rows % 16 != 0) return hipErrorInvalidValue;
+  if (prm.p < 1 || prm.q < 0 || prm.chunk_rows % 16 != 0 || prm.n < 1) return hipErrorInvalidValue;
   const int p16 = (prm.p + 15) / 16 * 16, z16 = (prm.p + prm.q + 15) / 16 * 16;
   const int tiles = (p16 / 16) * (z16 / 16);
-  const size_t shmem = sizeof(double) * 16 * static_cast<size_t>(z16);
+  const size_t shmem = sizeof(double) * 18 * static_cast<size_t>(z16);
   if (shmem > 64 * 1024) return hipErrorInvalidValue;
   const dim3 grid(static_cast<unsigned>(prm.chunks), static_cast<unsigned>((tiles + kCrossTilesPerBlock - 1) / kCrossTilesPerBlock));
-  hipLaunchKernelGGL(pls_cross_kernel, grid, dim3(kT), shmem, s, prm);
+  const int per = z16 / 16;  // elements per thread of a 16-row block
+  if (per <= 5) {
+    hipLaunchKernelGGL(pls_cross_kernel<5>, grid, dim3(kT), shmem, s, prm);
+  } else if (per <= 9) {
+    hipLaunchKernelGGL(pls_cross_kernel<9>, grid, dim3(kT), shmem, s, prm);
+  } else if (per <= 17) {
+    hipLaunchKernelGGL(pls_cross_kernel<17>, grid, dim3(kT), shmem, s, prm);
+  } else if (per <= 33) {
+    hipLaunchKernelGGL(pls_cross_kernel<33>, grid, dim3(kT), shmem, s, prm);
+  } else {
+    return hipErrorInvalidValue;
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int total = prm.p * (prm.p + prm.q);
-  hipLaunchKernelGGL(pls_cross_reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, s, prm);
+  hipLaunchKernelGGL(pls_cross_reduce_kernel, dim3((total + kReduceOutputs - 1) / kReduceOutputs), dim3(kT), 0, s,
+                     prm);
   return hipGetLastError();
 }
 
-size_t pls_fit_lds_bytes(int p, int q) {
-  return sizeof(double) * (static_cast<size_t>(q) * p + 2 * static_cast<size_t>(p) + q + kWaves);
+size_t pls_fit_lds_bytes(int p, int q, int rc, bool staged) {
+  return sizeof(double) * (static_cast<size_t>(q) * p + 2 * static_cast<size_t>(p) + q + static_cast<size_t>(q) * rc +
+                           (staged ? 2 * static_cast<size_t>(rc) * p + static_cast<size_t>(p) * p : 0)) +
+         4 * sizeof(int);
 }
 
 hipError_t launch_pls_fit(const PlsFitParams& prm, hipStream_t s) {
   if (prm.p < 1 || prm.p > kPlsMaxDims || prm.q < 1 || prm.rank_cap < 1 || prm.rank_cap > kPlsMaxComponents ||
-      static_cast<size_t>(prm.q) * prm.p > kPlsMaxSLds) {
+      static_cast<size_t>(prm.q) * prm.p > kPlsMaxSLds || prm.wscratch == nullptr) {
     return hipErrorInvalidValue;
   }
-  const size_t shmem = pls_fit_lds_bytes(prm.p, prm.q);
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&pls_fit_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
+  // The Gram matrix, weights and loadings join the LDS state when all fits
+  // 150 KiB.
+  const bool staged = pls_fit_lds_bytes(prm.p, prm.q, prm.rank_cap, true) <= 150 * 1024;
+  const size_t shmem = pls_fit_lds_bytes(prm.p, prm.q, prm.rank_cap, staged);
+  if (shmem > 160 * 1024) return hipErrorInvalidValue;
+  const void* fn = staged ? reinterpret_cast<const void*>(&pls_fit_kernel<true>)
+                          : reinterpret_cast<const void*>(&pls_fit_kernel<false>);
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pls_fit_kernel, dim3(1), dim3(kT), shmem, s, prm);
+  if (staged) {
+    hipLaunchKernelGGL(pls_fit_kernel<true>, dim3(1), dim3(kE), shmem, s, prm);
+  } else {
+    hipLaunchKernelGGL(pls_fit_kernel<false>, dim3(1), dim3(kE), shmem, s, prm);
+  }
   return hipGetLastError();
+}
+
+bool pls_scores_staged(int p, int q, int A) {
+  return A <= kScoreA && sizeof(double) * score_lds_doubles(p, q) <= 48 * 1024;
 }
 
 hipError_t launch_pls_scores(const PlsScoreParams& prm, hipStream_t s) {
   if (prm.n == 0) return hipSuccess;
   if (prm.A < 1 || prm.A > kPlsMaxComponents) return hipErrorInvalidValue;
+  if (pls_scores_staged(prm.p, prm.q, prm.A)) {
+    const size_t shmem = sizeof(double) * score_lds_doubles(prm.p, prm.q);
+    hipLaunchKernelGGL(pls_scores_staged_kernel, dim3((prm.n + kT - 1) / kT), dim3(kT), shmem, s, prm);
+    return hipGetLastError();
+  }
+  if (prm.tbuf == nullptr) return hipErrorInvalidValue;
   hipLaunchKernelGGL(pls_scores_kernel, dim3((prm.n + 127) / 128), dim3(128), 0, s, prm);
   return hipGetLastError();
 }

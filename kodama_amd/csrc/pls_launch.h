@@ -130,9 +130,9 @@ struct PlsCrossParams {
   double* cross = nullptr;     // [q x p]
 };
 
-// One workgroup: power-refresh SIMPLS of the cross-product (q rows in LDS)
-// with the Gram matrix, then the response weights Q = S0' W and the
-// coefficients W Q'.
+// One wave: power-refresh SIMPLS of the cross-product (q rows in LDS) with
+// the Gram matrix, then the response weights Q = S0' W and the coefficients
+// W Q'.
 struct PlsFitParams {
   int p = 0, q = 0;
   int rank_cap = 0;            // max(1, min(requested, p, n - 1))
@@ -144,6 +144,7 @@ struct PlsFitParams {
   double* yweights = nullptr;      // [q x kPlsMaxComponents]
   double* coefficients = nullptr;  // [p x q]
   int* fitted = nullptr;           // [1] rank (0: degenerate)
+  double* wscratch = nullptr;      // [2 x kPlsMaxComponents x p] component-major weights and loadings
 };
 
 // Scores T = Xs W [n x A] and fitted values ybar + T Q' [n x q], one thread
@@ -156,7 +157,7 @@ struct PlsScoreParams {
   const double* y_mean = nullptr;
   const double* weights = nullptr;   // [p x kPlsMaxComponents]
   const double* yweights = nullptr;  // [q x kPlsMaxComponents]
-  double* tbuf = nullptr;            // [n x A] scratch: the scores in FP64
+  double* tbuf = nullptr;            // [n x A] scratch: the scores in FP64 (unstaged kernel only)
   float* scores = nullptr;           // [n x A]
   float* fitted = nullptr;           // [n x q]
 };
@@ -164,8 +165,11 @@ struct PlsScoreParams {
 hipError_t launch_pls_colstats(const PlsColStatsParams& prm, hipStream_t s);
 hipError_t launch_pls_colstats_reduce(const PlsColStatsParams& prm, int scale, hipStream_t s);
 hipError_t launch_pls_cross(const PlsCrossParams& prm, hipStream_t s);
-size_t pls_fit_lds_bytes(int p, int q);
+size_t pls_fit_lds_bytes(int p, int q, int rc, bool staged);
 hipError_t launch_pls_fit(const PlsFitParams& prm, hipStream_t s);
+// True when the scores take the LDS-staged kernel (up to 16 components),
+// which needs no tbuf.
+bool pls_scores_staged(int p, int q, int A);
 hipError_t launch_pls_scores(const PlsScoreParams& prm, hipStream_t s);
 
 // LDS of one fold's wave: the compact layout keeps the loadings, G W, the
