@@ -355,7 +355,8 @@ def pls_form(args) -> dict:
                 steps_per_s=args.runs * args.steps / out["evolution_seconds"],
                 evolution_seconds=out["evolution_seconds"], mean_accbest=float(np.mean(out["accbest"])),
                 eval_kernel_ms=kern * 1e3,
-                roofline={"kernel": "pls_eval_kernel", "bound": "latency (block barriers, FP64)",
+                roofline={"kernel": "pls_fold_sums_kernel + pls_eval_kernel",
+                          "bound": "latency (one wave per fold, FP64)",
                           "achieved": algo / kern / 1e9 if kern > 0 else 0.0, "unit": "GB/s",
                           "peak": HBM_PEAK_GBS, "frac": algo / kern / 1e9 / HBM_PEAK_GBS if kern > 0 else 0.0,
                           "algorithmic_bytes_per_launch": algo, "mfma": "none (class sums)"},
