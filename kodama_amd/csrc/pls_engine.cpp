@@ -30,10 +30,12 @@ struct PlsCv::Impl {
   DeviceBuffer<float> x;
   DeviceBuffer<double> w0, mean, sd, xsum, gram, weights, loadings, gw, zsum, mu, linear, constant, val_scores, disc;
   DeviceBuffer<double> fsum;
-  DeviceBuffer<int> train_asc, train_off, seg_code, seg_count, val_rows, val_off, run_first, run_end, fcnt;
+  DeviceBuffer<int> seg_code, seg_count, val_rows, val_off, run_first, run_end, fcnt;
+  DeviceBuffer<int> shift_row, chunk_fold, chunk_begin, chunk_end, chunk_off;
+  DeviceBuffer<double> moment_partial, moments;
   DeviceBuffer<int> codes, pred, comps, rank;
   IVec groups;  // row ranges of the runs: [groups[g], groups[g+1])
-  IVec cached_folds, fold_ids, h_train_rows, h_train_off, h_val_rows, h_val_off;
+  IVec cached_folds, fold_ids, h_val_rows, h_val_off;
   IVec h_comps, h_rank;
   PinnedBuffer<int> h_codes, h_pred, h_small;  // codes, predictions; components and ranks
   EventTimer timer;
@@ -79,32 +81,53 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
       seen[g] = 1;
     }
   }
-  h_train_rows.clear();
   h_val_rows.clear();
-  h_train_off.assign(1, 0);
   h_val_off.assign(1, 0);
-  for (int fid : fold_ids) {
-    const int g = group_of[fid];
-    for (int i = groups[g]; i < groups[g + 1]; ++i) (folds[i] == fid ? h_val_rows : h_train_rows).push_back(i);
-    h_train_off.push_back(static_cast<int>(h_train_rows.size()));
-    h_val_off.push_back(static_cast<int>(h_val_rows.size()));
-  }
+  IVec h_shift(static_cast<size_t>(F));
   for (int f = 0; f < F; ++f) {
-    if (h_train_off[f + 1] == h_train_off[f]) throw std::runtime_error("PLS fold has no training samples.");
+    const int fid = fold_ids[f];
+    const int g = group_of[fid];
+    for (int i = groups[g]; i < groups[g + 1]; ++i) {
+      if (folds[i] == fid) h_val_rows.push_back(i);
+    }
+    h_val_off.push_back(static_cast<int>(h_val_rows.size()));
+    if (h_val_off[f + 1] - h_val_off[f] == groups[g + 1] - groups[g]) {
+      throw std::runtime_error("PLS fold has no training samples.");
+    }
+    h_shift[f] = groups[g];
+  }
+  // Row chunks of each fold's own rows for the moment kernel.
+  IVec h_cf, h_cb, h_ce, h_co(1, 0);
+  for (int f = 0; f < F; ++f) {
+    for (int b = h_val_off[f]; b < h_val_off[f + 1]; b += kPlsMomentRows) {
+      h_cf.push_back(f);
+      h_cb.push_back(b);
+      h_ce.push_back(std::min(b + kPlsMomentRows, h_val_off[f + 1]));
+    }
+    h_co.push_back(static_cast<int>(h_cf.size()));
   }
   const hipStream_t s = stream.get();
-  train_asc.resize(h_train_rows.size());
-  train_off.resize(h_train_off.size());
   val_rows.resize(h_val_rows.size());
   val_off.resize(h_val_off.size());
   run_first.resize(h_first.size());
   run_end.resize(h_end.size());
-  train_asc.upload(h_train_rows.data(), h_train_rows.size(), s);
-  train_off.upload(h_train_off.data(), h_train_off.size(), s);
+  shift_row.resize(h_shift.size());
+  chunk_fold.resize(h_cf.size());
+  chunk_begin.resize(h_cb.size());
+  chunk_end.resize(h_ce.size());
+  chunk_off.resize(h_co.size());
   val_rows.upload(h_val_rows.data(), h_val_rows.size(), s);
   val_off.upload(h_val_off.data(), h_val_off.size(), s);
   run_first.upload(h_first.data(), h_first.size(), s);
   run_end.upload(h_end.data(), h_end.size(), s);
+  shift_row.upload(h_shift.data(), h_shift.size(), s);
+  chunk_fold.upload(h_cf.data(), h_cf.size(), s);
+  chunk_begin.upload(h_cb.data(), h_cb.size(), s);
+  chunk_end.upload(h_ce.data(), h_ce.size(), s);
+  chunk_off.upload(h_co.data(), h_co.size(), s);
+  const size_t z16 = static_cast<size_t>((p + 1 + 15) / 16 * 16);
+  moment_partial.resize(h_cf.size() * z16 * z16);
+  moments.resize(static_cast<size_t>(F) * (p + 1) * (p + 1));
   const size_t fp = static_cast<size_t>(F) * p;
   const size_t fc = static_cast<size_t>(F) * classes;
   mean.resize(fp);
@@ -133,8 +156,18 @@ void PlsCv::Impl::rebuild_folds(const IVec& folds) {
   c.folds = F;
   c.center = center ? 1 : 0;
   c.scale = scale ? 1 : 0;
-  c.train.rows = train_asc.get();
-  c.train.offsets = train_off.get();
+  c.val.rows = val_rows.get();
+  c.val.offsets = val_off.get();
+  c.shift_row = shift_row.get();
+  c.run_first = run_first.get();
+  c.run_end = run_end.get();
+  c.chunks = static_cast<int>(h_cf.size());
+  c.chunk_fold = chunk_fold.get();
+  c.chunk_begin = chunk_begin.get();
+  c.chunk_end = chunk_end.get();
+  c.chunk_off = chunk_off.get();
+  c.partial = moment_partial.get();
+  c.moments = moments.get();
   c.mean = mean.get();
   c.sd = sd.get();
   c.xsum = xsum.get();

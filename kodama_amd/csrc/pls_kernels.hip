@@ -1,10 +1,14 @@
 // PLS-LDA cross-validation on the device (plscv.cpp:2293-2507 with the fold
 // cache 593-698, SIMPLS 816-955 and the LDA 1379-1510), all in FP64.
 //
-// pls_cache_kernel (one workgroup per fold, once per data epoch): training mean
-// and scale (the scale accumulator starts at 1, as train_center_scale_float
-// does), column sums of the standardised training rows and their Gram matrix
-// G = Xs'Xs, each summed in training-row order.
+// Fold cache (once per set of folds): pls_moments_kernel forms the moment
+// matrix of [x - shift | 1] over row chunks of each fold's own rows on the
+// FP64 matrix cores (the shift is one row of the fold's run), batched over
+// all chunks of all folds; pls_moments_reduce_kernel sums each fold's chunks;
+// pls_cache_finalize_kernel takes a fold's training moments as the sum of its
+// run's other folds' and derives the training mean and scale (the scale
+// accumulator starts at 1, as train_center_scale_float does), the column sums
+// of the standardised training rows and their Gram matrix G = Xs'Xs.
 //
 // Per labelling, two launches:
 // pls_fold_sums_kernel (one wave per fold): the class sums of the fold's own
@@ -46,79 +50,156 @@ constexpr int kA = kPlsMaxComponents;
 constexpr double kNormFloor = static_cast<double>(1.0e-10f);
 constexpr double kTinyFloor = static_cast<double>(1.0e-20f);
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = kWave / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
 
 }  // namespace
 
 // ---- fold cache ---------------------------------------------------------------
 
-__global__ void __launch_bounds__(kT) pls_cache_kernel(PlsCacheParams prm) {
-  constexpr int kTile = 16, kChunk = 32;
-  __shared__ double stage[kChunk][2 * kTile + 1];
-  const int f = blockIdx.x;
-  const int p = prm.p;
-  const int t0 = prm.train.offsets[f];
-  const int nt = prm.train.offsets[f + 1] - t0;
-  const int* rows = prm.train.rows + t0;
-  double* mean = prm.mean + static_cast<size_t>(f) * p;
-  double* sd = prm.sd + static_cast<size_t>(f) * p;
-  double* xsum = prm.xsum + static_cast<size_t>(f) * p;
-  double* gram = prm.gram + static_cast<size_t>(f) * p * p;
-  for (int j = threadIdx.x; j < p; j += kT) {
-    double m = 0.0;
-    if (prm.center) {
-      for (int i = 0; i < nt; ++i) m += static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]);
-      m *= nt > 0 ? 1.0 / static_cast<double>(nt) : 0.0;
+namespace {
+typedef double f64x4m __attribute__((ext_vector_type(4)));
+constexpr int kMomentTilesPerWave = 8;
+constexpr int kMomentTilesPerBlock = kMomentTilesPerWave * kWaves;
+}  // namespace
+
+// Moments of one chunk of a fold's own rows (blockIdx.x) for one group of up
+// to 32 of the 16 x 16 output tiles (blockIdx.y): Z'Z with Z = [x - shift | 1]
+// (zero padded to z16 columns), rows through LDS 16 at a time as doubles, the
+// next block's loads in flight during the products (v_mfma_f64_16x16x4f64;
+// the operand layout of pls_cross_kernel). Z'Z holds the shifted cross
+// moments, the shifted column sums (column p) and the row count (p, p).
+template <int PER>
+__global__ void __launch_bounds__(kT) pls_moments_kernel(PlsCacheParams prm) {
+  extern __shared__ double mst[];  // [16][z16]
+  const int p = prm.p, zc = p + 1;
+  const int z16 = (zc + 15) / 16 * 16;
+  const int tiles_c = z16 / 16, tiles = tiles_c * tiles_c;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int k = blockIdx.x;
+  const int f = prm.chunk_fold[k];
+  const int b0 = prm.chunk_begin[k], b1 = prm.chunk_end[k];
+  const int sr = prm.shift_row[f];
+  const int tile0 = blockIdx.y * kMomentTilesPerBlock + wave;
+  f64x4m acc[kMomentTilesPerWave];
+#pragma unroll
+  for (int t = 0; t < kMomentTilesPerWave; ++t) acc[t] = f64x4m{0.0, 0.0, 0.0, 0.0};
+  const int kr = lane / 16, kc = lane % 16;
+  // Element q of this thread in a block: e = threadIdx.x + q * 256 < 16 z16;
+  // indices are clamped (the values are masked when stored).
+  auto fetch = [&](int rb, float (&xv)[PER], float (&sv)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = min(static_cast<int>(threadIdx.x) + q * kT, 16 * z16 - 1);
+      const int rr = e / z16, c = min(e - rr * z16, p - 1);
+      const int row = prm.val.rows[min(rb + rr, b1 - 1)];
+      xv[q] = prm.x[static_cast<size_t>(row) * p + c];
+      sv[q] = prm.x[static_cast<size_t>(sr) * p + c];
     }
+  };
+  float xv[PER], sv[PER];
+  fetch(b0, xv, sv);
+  for (int rb = b0; rb < b1; rb += 16) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = static_cast<int>(threadIdx.x) + q * kT;
+      if (e < 16 * z16) {
+        const int rr = e / z16, c = e - rr * z16;
+        const double v = c < p ? static_cast<double>(xv[q]) - static_cast<double>(sv[q]) : (c == p ? 1.0 : 0.0);
+        mst[e] = rb + rr < b1 ? v : 0.0;
+      }
+    }
+    __syncthreads();
+    if (rb + 16 < b1) fetch(rb + 16, xv, sv);
+#pragma unroll
+    for (int t = 0; t < kMomentTilesPerWave; ++t) {
+      const int tile = tile0 + t * kWaves;
+      if (tile < tiles) {
+        const int tj = tile / tiles_c, tc = tile - tj * tiles_c;
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const double a = mst[(k0 + kr) * z16 + tj * 16 + kc];
+          const double b = mst[(k0 + kr) * z16 + tc * 16 + kc];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* out = prm.partial + static_cast<size_t>(k) * z16 * z16;
+#pragma unroll
+  for (int t = 0; t < kMomentTilesPerWave; ++t) {
+    const int tile = tile0 + t * kWaves;
+    if (tile >= tiles) break;
+    const int tj = tile / tiles_c, tc = tile - tj * tiles_c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      out[static_cast<size_t>(tj * 16 + 4 * i + kr) * z16 + tc * 16 + kc] = acc[t][i];
+    }
+  }
+}
+
+// Moments of each fold (one workgroup per fold): its chunks' partials in
+// chunk order.
+__global__ void __launch_bounds__(kT) pls_moments_reduce_kernel(PlsCacheParams prm) {
+  const int f = blockIdx.x, p = prm.p, zc = p + 1;
+  const int z16 = (zc + 15) / 16 * 16;
+  const int k0 = prm.chunk_off[f], k1 = prm.chunk_off[f + 1];
+  double* out = prm.moments + static_cast<size_t>(f) * zc * zc;
+  for (int e = threadIdx.x; e < zc * zc; e += kT) {
+    const int a = e / zc, b = e - a * zc;
+    double t = 0.0;
+    for (int k = k0; k < k1; ++k) t += prm.partial[(static_cast<size_t>(k) * z16 + a) * z16 + b];
+    out[e] = t;
+  }
+}
+
+// Training statistics of each fold (one workgroup per fold) from the sum T of
+// its run's other folds' moments (fixed fold order): with n = T[p][p],
+// Y = T[.][p] the shifted column sums, d = mean - shift (center: Y / n,
+// otherwise -shift), sum (x - mean)^2 = T[j][j] - 2 d Y + n d^2, the scale
+// sqrt((1 + that) / max(1, n - 1)) (non-finite or zero -> 1), the
+// standardised column sums (Y - n d) / sd and
+// G[a][b] = (T[a][b] - d_a Y_b - d_b Y_a + n d_a d_b) / (sd_a sd_b).
+__global__ void __launch_bounds__(kT) pls_cache_finalize_kernel(PlsCacheParams prm) {
+  __shared__ double st[3 * kPlsMaxDims + 1];  // d, Y, sd, n
+  const int f = blockIdx.x, p = prm.p, zc = p + 1;
+  const int g0 = prm.run_first[f], g1 = prm.run_end[f];
+  auto total = [&](int a, int b) {
+    double t = 0.0;
+    for (int g = g0; g < g1; ++g) {
+      if (g != f) t += prm.moments[(static_cast<size_t>(g) * zc + a) * zc + b];
+    }
+    return t;
+  };
+  double* dv = st;
+  double* yv = st + kPlsMaxDims;
+  double* sdv = st + 2 * kPlsMaxDims;
+  const double n = total(p, p);
+  const int sr = prm.shift_row[f];
+  for (int j = threadIdx.x; j < p; j += kT) {
+    const double y = total(j, p);
+    const double c = static_cast<double>(prm.x[static_cast<size_t>(sr) * p + j]);
+    const double d = prm.center ? y / n : -c;
     double s = 1.0;
     if (prm.scale) {
-      double acc = 1.0;
-      for (int i = 0; i < nt; ++i) {
-        const double d = static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]) - m;
-        acc += d * d;
-      }
-      s = sqrt(acc / fmax(1.0, static_cast<double>(nt - 1)));
+      const double ss = total(j, j) - 2.0 * d * y + n * d * d;
+      s = sqrt((1.0 + ss) / fmax(1.0, n - 1.0));
       s = s > 0.0 && isfinite(s) ? s : 1.0;
     }
-    double xs = 0.0;
-    for (int i = 0; i < nt; ++i) xs += (static_cast<double>(prm.x[static_cast<size_t>(rows[i]) * p + j]) - m) / s;
-    mean[j] = m;
-    sd[j] = s;
-    xsum[j] = xs;
+    dv[j] = d;
+    yv[j] = y;
+    sdv[j] = s;
+    prm.mean[static_cast<size_t>(f) * p + j] = prm.center ? c + d : 0.0;
+    prm.sd[static_cast<size_t>(f) * p + j] = s;
+    prm.xsum[static_cast<size_t>(f) * p + j] = (y - n * d) / s;
   }
   __syncthreads();
-  // Gram in 16 x 16 tiles of the lower triangle; rows staged 32 at a time.
-  const int tiles = (p + kTile - 1) / kTile;
-  const int u = threadIdx.x / kTile, v = threadIdx.x % kTile;
-  for (int ta = 0; ta < tiles; ++ta) {
-    for (int tb = 0; tb <= ta; ++tb) {
-      const int a = ta * kTile + u, b = tb * kTile + v;
-      double acc = 0.0;
-      for (int c0 = 0; c0 < nt; c0 += kChunk) {
-        // Stage the two 16-column blocks of up to 32 standardised rows.
-        for (int e = threadIdx.x; e < kChunk * 2 * kTile; e += kT) {
-          const int r = e / (2 * kTile), q = e % (2 * kTile);
-          const int col = q < kTile ? ta * kTile + q : tb * kTile + (q - kTile);
-          double val = 0.0;
-          if (c0 + r < nt && col < p) {
-            val = (static_cast<double>(prm.x[static_cast<size_t>(rows[c0 + r]) * p + col]) - mean[col]) / sd[col];
-          }
-          stage[r][q] = val;
-        }
-        __syncthreads();
-        const int len = min(kChunk, nt - c0);
-        for (int r = 0; r < len; ++r) acc += stage[r][u] * stage[r][kTile + v];
-        __syncthreads();
-      }
-      if (a < p && b < p && b <= a) {
-        gram[static_cast<size_t>(a) * p + b] = acc;
-        gram[static_cast<size_t>(b) * p + a] = acc;
-      }
-    }
+  double* gram = prm.gram + static_cast<size_t>(f) * p * p;
+  for (int e = threadIdx.x; e < p * p; e += kT) {
+    const int a = e / p, b = e - a * p;
+    if (b > a) continue;
+    const double v = (total(a, b) - dv[a] * yv[b] - dv[b] * yv[a] + n * dv[a] * dv[b]) / (sdv[a] * sdv[b]);
+    gram[static_cast<size_t>(a) * p + b] = v;
+    gram[static_cast<size_t>(b) * p + a] = v;
   }
 }
 
@@ -1305,8 +1386,26 @@ bool pls_eval_compact(int p, int max_classes, int rank_cap) {
 
 hipError_t launch_pls_cache(const PlsCacheParams& prm, hipStream_t s) {
   if (prm.folds == 0) return hipSuccess;
-  if (prm.p < 1 || prm.p > kPlsMaxDims) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pls_cache_kernel, dim3(prm.folds), dim3(kT), 0, s, prm);
+  if (prm.p < 1 || prm.p > kPlsMaxDims || prm.chunks < 1) return hipErrorInvalidValue;
+  const int z16 = (prm.p + 1 + 15) / 16 * 16;
+  const int tiles = (z16 / 16) * (z16 / 16);
+  const size_t shmem = sizeof(double) * 16 * static_cast<size_t>(z16);
+  const dim3 grid(static_cast<unsigned>(prm.chunks),
+                  static_cast<unsigned>((tiles + kMomentTilesPerBlock - 1) / kMomentTilesPerBlock));
+  const int per = z16 / 16;  // elements per thread of a 16-row block
+  if (per <= 5) {
+    hipLaunchKernelGGL(pls_moments_kernel<5>, grid, dim3(kT), shmem, s, prm);
+  } else if (per <= 9) {
+    hipLaunchKernelGGL(pls_moments_kernel<9>, grid, dim3(kT), shmem, s, prm);
+  } else {
+    hipLaunchKernelGGL(pls_moments_kernel<17>, grid, dim3(kT), shmem, s, prm);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pls_moments_reduce_kernel, dim3(prm.folds), dim3(kT), 0, s, prm);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pls_cache_finalize_kernel, dim3(prm.folds), dim3(kT), 0, s, prm);
   return hipGetLastError();
 }
 

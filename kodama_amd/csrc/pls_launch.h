@@ -23,18 +23,36 @@ struct PlsFoldRows {
   const int* offsets = nullptr;  // [folds + 1]
 };
 
+// Fold cache (mean, scale, column sums of the standardised training rows,
+// Gram matrix G = Xs'Xs of every fold's training rows) from moments: each
+// fold's own rows, shifted by one row of their run (the run's first row, so
+// that the moments hold no large offsets), give the moment matrix of
+// [x - shift | 1] on the FP64 matrix cores, in row chunks; a fold's training
+// moments are the sum of its run's other folds' moments, from which mean,
+// scale (the scale accumulator starts at 1), column sums and G follow.
 struct PlsCacheParams {
   const float* x = nullptr;      // [n x p] row-major
   int p = 0;
   int folds = 0;
   int center = 1;
   int scale = 1;
-  PlsFoldRows train;             // any order; sums run in this order
+  PlsFoldRows val;               // each fold's own rows
+  const int* shift_row = nullptr;    // [folds] the row that shifts the fold's run
+  const int* run_first = nullptr;    // [folds] as PlsEvalParams
+  const int* run_end = nullptr;
+  int chunks = 0;                    // row chunks of all folds, fold by fold
+  const int* chunk_fold = nullptr;   // [chunks]
+  const int* chunk_begin = nullptr;  // [chunks] positions in the concatenated own rows
+  const int* chunk_end = nullptr;
+  const int* chunk_off = nullptr;    // [folds + 1] first chunk of each fold
+  double* partial = nullptr;         // [chunks x z16 x z16], z16 = p + 1 rounded up to 16
+  double* moments = nullptr;         // [folds x (p + 1) x (p + 1)]
   double* mean = nullptr;        // [folds x p]
   double* sd = nullptr;          // [folds x p]
   double* xsum = nullptr;        // [folds x p] column sums of the standardised training rows
   double* gram = nullptr;        // [folds x p x p]
 };
+constexpr int kPlsMomentRows = 256;  // rows per moment chunk
 
 // Class sums of each fold's validation rows (ascending row order, FP64):
 // fsum[f][c][j] = sum of x[row][j] over the fold's rows of code c, fcnt[f][c]
