@@ -246,8 +246,9 @@ int kodama_hip_core_plslda(const float* x, int32_t n, int32_t p, const int32_t* 
  * kodama_hip_core_knn_batch; proposals on the host with the reference's random
  * streams (seed + r, folds from cv_seed + r unless fold_ids is given), one
  * device evaluation of all runs x folds per cycle. Per run the result equals
- * kodama_hip_core_plslda on the run's rows. timings[4] as for the kNN batch,
- * timings[3] = average duration of the fused PLS-LDA evaluation kernel. */
+ * kodama_hip_core_plslda on the run's rows. timings[5] as for the kNN batch
+ * (plain form), timings[3] = average duration of the PLS-LDA evaluation
+ * kernels. */
 int kodama_hip_core_plslda_batch(const float* x, int32_t n, int32_t p, int32_t runs,
                                  const int32_t* run_offsets, const int32_t* landmark_rows,
                                  const int32_t* initial_labels, const int32_t* fold_ids,
@@ -339,12 +340,18 @@ void kodama_hip_graph_release(kodama_hip_graph* graph);
  * kodama_hip_core_knn); mode 1 = device-resident (proposal, transition, vote
  * and acceptance on the device, one persistent launch for all cycles with one
  * workgroup per run, counter-based random streams; same algorithm, different
- * random draws; runs of up to 8191 rows and 1024 classes).
+ * random draws; runs of up to 16383 rows and 1024 classes, no constraint
+ * groups or fixed rows).
  * Outputs: clbest (same indexing as landmark_rows), accbest[runs],
- * vect_acc[runs x cycles]; timings[4] (may be NULL): precompute seconds,
- * evolution seconds, steps completed, and the device seconds of the dominant
- * kernel (HIP events on its launch stream): mode 0 the average batched vote
- * launch, mode 1 the single persistent evolution launch. */
+ * vect_acc[runs x cycles]; timings[5] (may be NULL): precompute seconds,
+ * evolution seconds, steps completed, the device seconds of the dominant
+ * kernel (HIP events on its launch stream: mode 0 the average batched vote
+ * launch, mode 1 the persistent evolution launch), and the evolution's wall
+ * seconds with its host copies. Mode 1 times the evolution with its inputs in
+ * device memory (from the launch, the starting labels already uploaded, to
+ * the end of the launch); timings[4] adds the label upload and the result
+ * download and conversion. Mode 0 moves labels between host and device every
+ * cycle by construction, and both figures are the whole loop. */
 int kodama_hip_core_knn_batch(const float* x, int32_t n, int32_t p, int32_t runs,
                               const int32_t* run_offsets, const int32_t* landmark_rows,
                               const int32_t* initial_labels, const int32_t* fold_ids,

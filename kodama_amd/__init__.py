@@ -310,7 +310,7 @@ def core_knn_batch(data, run_rows, initial_labels, options: CoreOptions, mode="p
     clbest = np.zeros(int(offsets[-1]), np.int32)
     accbest = np.zeros(M, np.float64)
     vect = np.zeros(M * max(options.cycles, 1), np.float64)
-    timings = np.zeros(4, np.float64)
+    timings = np.zeros(5, np.float64)
     m = {"plain": 0, "device": 1}[mode]
     N.call("kodama_hip_core_knn_batch", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
            N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), C.c_int32(m),
@@ -318,7 +318,8 @@ def core_knn_batch(data, run_rows, initial_labels, options: CoreOptions, mode="p
     return dict(clbest=[clbest[offsets[r]:offsets[r + 1]].copy() for r in range(M)], accbest=accbest,
                 vect_acc=vect[: M * options.cycles].reshape(M, options.cycles),
                 precompute_seconds=float(timings[0]), evolution_seconds=float(timings[1]),
-                steps=int(timings[2]), vote_seconds=float(timings[3]))
+                steps=int(timings[2]), vote_seconds=float(timings[3]),
+                evolution_seconds_with_copies=float(timings[4]))
 
 
 def core_plslda_batch(data, run_rows, initial_labels, options: CoreOptions, fold_ids=None):
@@ -336,7 +337,7 @@ def core_plslda_batch(data, run_rows, initial_labels, options: CoreOptions, fold
     clbest = np.zeros(int(offsets[-1]), np.int32)
     accbest = np.zeros(M, np.float64)
     vect = np.zeros(M * max(options.cycles, 1), np.float64)
-    timings = np.zeros(4, np.float64)
+    timings = np.zeros(5, np.float64)
     N.call("kodama_hip_core_plslda_batch", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
            N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), N.ptr(clbest),
            N.ptr(accbest), N.ptr(vect), N.ptr(timings))
@@ -443,14 +444,15 @@ def core_knn_batch_learned(data, run_rows, initial_labels, options: CoreOptions,
     clbest = np.zeros(int(offsets[-1]), np.int32)
     accbest = np.zeros(M, np.float64)
     vect = np.zeros(M * max(options.cycles, 1), np.float64)
-    timings = np.zeros(4, np.float64)
+    timings = np.zeros(5, np.float64)
     N.call("kodama_hip_core_knn_batch_learned", N.ptr(x), C.c_int32(x.shape[0]), C.c_int32(x.shape[1]), C.c_int32(M),
            N.ptr(offsets), N.ptr(rows), N.ptr(init), N.ptr(folds), C.byref(options), N.ptr(par),
            C.c_float(fallback), N.ptr(clbest), N.ptr(accbest), N.ptr(vect), N.ptr(timings))
     return dict(clbest=[clbest[offsets[r]:offsets[r + 1]].copy() for r in range(M)], accbest=accbest,
                 vect_acc=vect[: M * options.cycles].reshape(M, options.cycles),
                 precompute_seconds=float(timings[0]), evolution_seconds=float(timings[1]),
-                steps=int(timings[2]), device_seconds=float(timings[3]))
+                steps=int(timings[2]), device_seconds=float(timings[3]),
+                evolution_seconds_with_copies=float(timings[4]))
 
 
 # ---- KODAMA.matrix ----------------------------------------------------------------

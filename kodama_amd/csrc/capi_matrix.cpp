@@ -331,7 +331,7 @@ void matrix_impl(const float* x, int32_t n, int32_t p, const kodama_hip_matrix_o
   const int total = offsets[o.runs];
   IVec clbest(static_cast<size_t>(total));
   std::vector<double> vect(static_cast<size_t>(o.runs) * std::max(1, o.cycles));
-  double core_t[4] = {0, 0, 0, 0};
+  double core_t[5] = {0, 0, 0, 0, 0};
   BatchRequest q{x, n, p, o.runs, offsets.data(), land_all.data(), init_all.data(), nullptr, &co};
   q.constrain = identity ? nullptr : cons_land.data();
   q.fixed = any_fixed ? fixed_land.data() : nullptr;

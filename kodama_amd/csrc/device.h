@@ -169,6 +169,7 @@ class EventTimer {
   }
   void start(hipStream_t s) { check(hipEventRecord(a_, s), "hipEventRecord"); }
   void stop(hipStream_t s) { check(hipEventRecord(b_, s), "hipEventRecord"); }
+  void wait() { check(hipEventSynchronize(b_), "hipEventSynchronize"); }
   float ms() {
     check(hipEventSynchronize(b_), "hipEventSynchronize");
     float v = 0.0f;

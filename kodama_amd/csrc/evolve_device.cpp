@@ -186,7 +186,11 @@ void DeviceEvolution::start(const std::vector<IVec>& codes) {
     h_key[r] = splitmix(m.seed + static_cast<uint64_t>(r) * 0xD1B54A32D192ED03ull);
   }
   check(hipMemcpyAsync(m.arena.get() + m.up_begin, img, m.up_bytes, hipMemcpyHostToDevice, s), "upload");
-  if (!pageable.empty()) check(hipStreamSynchronize(s), "upload sync");  // the image dies with this scope
+  check(hipStreamSynchronize(s), "upload sync");
+}
+
+void DeviceEvolution::wait() {
+  if (impl_->timed) impl_->timer.wait();
 }
 
 namespace {

@@ -19,7 +19,8 @@ class DeviceEvolution {
   ~DeviceEvolution();
   DeviceEvolution(const DeviceEvolution&) = delete;
   DeviceEvolution& operator=(const DeviceEvolution&) = delete;
-  // Uploads the starting label codes of every run (same order as `values`).
+  // Uploads the starting label codes of every run (same order as `values`);
+  // returns once they are on the device.
   void start(const std::vector<IVec>& codes);
   // Enqueues `cycles` more cycles of every run in one persistent launch
   // (the first launch evaluates the starting labels first); with the network
@@ -27,6 +28,8 @@ class DeviceEvolution {
   // uniform proposal with probability `fallback`. Does not block.
   void run(int cycles);
   void run_learned(int cycles, float fallback);
+  // Blocks until the enqueued launches are done (results in device memory).
+  void wait();
   // Downloads and converts the results; blocks until the launches are done.
   void export_results(int* clbest, double* accbest, double* vect_acc);
   // Time from the start of the first launch to the end of the last (HIP
