@@ -140,6 +140,17 @@ int kodama_hip_knn_fold_tables(const float* x, int32_t n, int32_t p, int32_t run
 int kodama_hip_set_fold_table_path(int32_t path);
 int kodama_hip_last_fold_table_path(void);
 
+/* Tuning control of the device-resident evolution (no reference counterpart:
+ * the reference runs one CUDA stream per run). Runs of at least 1024 rows are
+ * split over up to max_group workgroups (0 = automatic, up to 4; 1 = one
+ * workgroup per run) while all the workgroups fit the device at once; the
+ * members replay the same proposal, absorption and acceptance and each votes
+ * a slice of the rows, so the results are identical for every setting.
+ * kodama_hip_last_evolution_workgroups returns the workgroups of the calling
+ * thread's last device-resident evolution (0 before any). */
+int kodama_hip_set_evolution_split(int32_t max_group);
+int kodama_hip_last_evolution_workgroups(void);
+
 /* ---- device seam: exact kNN ------------------------------------------------
  * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).
  * train [train_rows x dimensions], query [query_rows x dimensions] row-major

@@ -21,6 +21,7 @@ __all__ = [
     "KNNCV", "PLSLDACV", "CoreKNN", "CorePLSLDA", "CoreKNNGraph", "KODAMA_knn_graph",
     "KODAMA_dissimilarity", "core_knn", "core_plslda", "core_knn_batch", "exact_knn_search",
     "KnnVoteGraph", "core_options", "device_count", "version",
+    "set_evolution_split", "last_evolution_workgroups",
 ]
 
 
@@ -605,6 +606,18 @@ def set_fold_table_path(path="tuned"):
 def last_fold_table_path():
     """Path of this thread's last per-fold table build ("scan", "mfma"; None before any)."""
     return {0: None, 1: "scan", 2: "mfma"}[int(N.lib().kodama_hip_last_fold_table_path())]
+
+
+def set_evolution_split(max_group=0):
+    """Process-wide workgroups per run of the device-resident evolution: 0 =
+    automatic (large runs over up to 4 workgroups), 1 = one per run, 2-4 = at
+    most that many. Results are identical for every setting."""
+    N.call("kodama_hip_set_evolution_split", C.c_int32(int(max_group)))
+
+
+def last_evolution_workgroups():
+    """Workgroups of this thread's last device-resident evolution (0 before any)."""
+    return int(N.lib().kodama_hip_last_evolution_workgroups())
 
 
 class KODAMAGraphHandle:

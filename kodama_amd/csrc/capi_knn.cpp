@@ -15,6 +15,7 @@
 #include "engine.h"
 #include "batch_api.h"
 #include "evolve_device.h"
+#include "evolve_launch.h"
 #include "knn_mfma.h"
 #include "launch.h"
 #include "mlp.h"
@@ -47,6 +48,8 @@ struct kodama_hip_vote_graph {
 };
 
 namespace kodama_hip {
+
+thread_local int g_last_evolution_workgroups = 0;
 
 void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, float fallback, TraceSink* trace,
                    int32_t* clbest, double* accbest, double* vect_acc, double* timings) {
@@ -147,6 +150,7 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
     else dev->run(options->cycles);
     dev->wait();
     t_evo = seconds_since(tr);
+    g_last_evolution_workgroups = dev->workgroups();
     dev->export_results(clbest, accbest, vect_acc);
   }
   const double t_wall = seconds_since(t1);
@@ -423,5 +427,14 @@ int kodama_hip_set_fold_table_path(int32_t path) {
 }
 
 int kodama_hip_last_fold_table_path(void) { return last_fold_table_path(); }
+
+int kodama_hip_set_evolution_split(int32_t max_group) {
+  return guarded([&] {
+    require(max_group >= 0 && max_group <= kEvoMaxGroup, "max_group must be 0 (automatic) to 4.");
+    set_evolution_split(max_group);
+  });
+}
+
+int kodama_hip_last_evolution_workgroups(void) { return g_last_evolution_workgroups; }
 
 }  // extern "C"

@@ -132,6 +132,7 @@ class KnnBatch {
   int pad_offset(int r) const { return pad_offsets_[r]; }
   int pad_total() const { return pad_total_; }
   int max_rows() const { return max_rows_; }
+  int run_rows(int r) const { return run_offsets_[r + 1] - run_offsets_[r]; }
   int classes() const { return classes_; }
   hipStream_t stream() const { return stream_.get(); }
 

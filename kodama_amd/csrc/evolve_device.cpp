@@ -5,7 +5,9 @@
 #include "evolve_device.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <queue>
 
 #include "evolve_launch.h"
 #include "launch.h"
@@ -14,6 +16,10 @@
 namespace kodama_hip {
 
 namespace {
+std::atomic<int> g_split_max{0};
+constexpr int kMinSliceRows = 512;  // a split run's members vote at least this many rows each
+constexpr int kSpinLimit = 1 << 20;
+
 uint64_t splitmix(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -42,6 +48,11 @@ struct DeviceEvolution::Impl {
   bool learned = false;
   EventTimer timer;
   bool timed = false;
+  // Split runs: the exchange region (xbuf, xmatch, error) cleared before
+  // every launch, and the error word read back at export.
+  int workgroups = 0;
+  size_t off_exchange = 0, exchange_words = 0, off_error = 0;
+  PinnedBuffer<int> h_error;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
 };
@@ -85,7 +96,38 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   if (m.learned && evo_lds_bytes(m.classes, batch.max_rows(), true, m.large) > kEvoMaxLds) {
     throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
   }
-  const size_t flow_items = static_cast<size_t>(m.runs) * m.classes * m.classes;
+  // Split the largest runs over up to kEvoMaxGroup workgroups while the
+  // workgroups stay within what the device holds at once: repeatedly one more
+  // member for the run with the most rows per member, as long as each member
+  // keeps at least kMinSliceRows rows.
+  int device = 0;
+  check(hipGetDevice(&device), "hipGetDevice");
+  IVec group(static_cast<size_t>(m.runs), 1);
+  const int max_group = std::min(kEvoMaxGroup, g_split_max.load() == 0 ? kEvoMaxGroup : g_split_max.load());
+  if (max_group > 1) {
+    int cus = 0;
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    const int capacity = cus * evo_resident_per_cu(batch.k(), m.classes, batch.max_rows(), m.learned, m.large);
+    if (capacity >= m.runs) {
+      std::priority_queue<std::pair<double, int>> pq;
+      for (int r = 0; r < m.runs; ++r) pq.emplace(static_cast<double>(batch.run_rows(r)), r);
+      int total = m.runs;
+      while (total < capacity && !pq.empty()) {
+        const int r = pq.top().second;
+        pq.pop();
+        const int rows = batch.run_rows(r);
+        if (group[r] >= max_group || rows / (group[r] + 1) < kMinSliceRows) continue;
+        ++group[r];
+        ++total;
+        pq.emplace(static_cast<double>(rows) / group[r], r);
+      }
+    }
+  }
+  int workgroups = 0;
+  for (int g : group) workgroups += g;
+  const bool split = workgroups > m.runs;
+  m.workgroups = split ? workgroups : m.runs;
+  const size_t flow_items = static_cast<size_t>(m.workgroups) * m.classes * m.classes;
   const int pad = batch.pad_total();
   const size_t label_bytes = sizeof(unsigned short) * static_cast<size_t>(pad);
   const hipStream_t s = batch.stream();
@@ -104,14 +146,20 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   // The learned proposal's network: fp32 parameters and their tiled form.
   m.off_params = lay.take(sizeof(float) * kMlpParams);
   m.off_packed = lay.take(sizeof(uint32_t) * kMlpTiled);
-  // Large-run layout: each run's flow list and row queue.
-  const size_t off_large = m.large ? lay.take(sizeof(int) * 2 * static_cast<size_t>(m.runs) * batch.max_rows()) : 0;
+  // Large-run layout: each workgroup's flow list and row queue.
+  const size_t off_large =
+    m.large ? lay.take(sizeof(int) * 2 * static_cast<size_t>(m.workgroups) * batch.max_rows()) : 0;
+  // Split runs: workgroup tables and the exchange region.
+  const size_t off_wg = split ? lay.take(sizeof(int) * (2 * static_cast<size_t>(m.workgroups) + m.runs)) : 0;
+  if (split) {
+    m.exchange_words = static_cast<size_t>(pad) + static_cast<size_t>(m.runs) * kEvoMaxGroup + 1;
+    m.off_exchange = lay.take(sizeof(unsigned) * m.exchange_words);
+    m.off_error = m.off_exchange + sizeof(unsigned) * (m.exchange_words - 1);
+  }
   m.up_begin = m.off_cur;
   m.up_bytes = m.off_key + sizeof(uint64_t) * m.runs - m.off_cur;
   m.down_begin = m.off_best;
   m.down_bytes = m.off_history + sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap - m.off_best;
-  int device = 0;
-  check(hipGetDevice(&device), "hipGetDevice");
   // Rounded up to 2 MiB so that evolutions of similar size reuse one pool block.
   m.arena.allocate((lay.total + (2u << 20) - 1) / (2u << 20) * (2u << 20), device, s);
   unsigned char* base = m.arena.get();
@@ -120,6 +168,20 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
                           batch.d_block_run().get(), reinterpret_cast<unsigned short*>(base + off_nbr),
                           static_cast<size_t>(batch.k()) * pad, pad, s),
         "evo_narrow");
+  if (split) {
+    IVec tables;
+    tables.reserve(2 * static_cast<size_t>(m.workgroups) + m.runs);
+    for (int r = 0; r < m.runs; ++r)
+      for (int g = 0; g < group[r]; ++g) tables.push_back(r);
+    for (int r = 0; r < m.runs; ++r)
+      for (int g = 0; g < group[r]; ++g) tables.push_back(g);
+    tables.insert(tables.end(), group.begin(), group.end());
+    check(hipMemcpyAsync(base + off_wg, tables.data(), sizeof(int) * tables.size(), hipMemcpyHostToDevice, s),
+          "upload workgroup tables");
+    // The copy reads a pageable vector: complete it before the vector goes.
+    check(hipStreamSynchronize(s), "workgroup tables sync");
+    m.h_error.resize(1);
+  }
   if (m.learned) {
     float* d_params = reinterpret_cast<float*>(base + m.off_params);
     check(hipMemcpyAsync(d_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, s), "upload params");
@@ -149,6 +211,17 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.history = reinterpret_cast<double*>(base + m.off_history);
   p.large_scratch = m.large ? reinterpret_cast<int*>(base + off_large) : nullptr;
   p.learn_packed = m.learned ? reinterpret_cast<const uint32_t*>(base + m.off_packed) : nullptr;
+  if (split) {
+    const int* wg = reinterpret_cast<const int*>(base + off_wg);
+    p.wg_run = wg;
+    p.wg_member = wg + m.workgroups;
+    p.run_group = wg + 2 * m.workgroups;
+    p.xbuf = reinterpret_cast<unsigned*>(base + m.off_exchange);
+    p.xmatch = p.xbuf + pad;
+    p.error = reinterpret_cast<int*>(base + m.off_error);
+    p.spin_limit = kSpinLimit;
+    p.workgroups = m.workgroups;
+  }
   // The set-up is complete before the timed evolution starts.
   check(hipStreamSynchronize(s), "setup sync");
   impl_ = impl.release();
@@ -204,6 +277,10 @@ void launch_cycles(DeviceEvolution::Impl& m, int cycles, bool learned, float fal
   p.cycle_count = cycles;
   if (!learned) p.learn_packed = nullptr;
   p.learn_fallback = fallback;
+  if (p.wg_run != nullptr) {
+    check(launch_evo_zero(reinterpret_cast<int*>(m.arena.get() + m.off_exchange), m.exchange_words, s),
+          "exchange clear");
+  }
   if (!m.timed) m.timer.start(s);  // kernel time spans the first launch to the last
   check(launch_evo_persistent(p, s), "evo_persistent");
   m.timer.stop(s);
@@ -215,6 +292,11 @@ void launch_cycles(DeviceEvolution::Impl& m, int cycles, bool learned, float fal
 }  // namespace
 
 void DeviceEvolution::run(int cycles) { launch_cycles(*impl_, cycles, false, 0.0f); }
+
+int DeviceEvolution::workgroups() const { return impl_->workgroups; }
+
+void set_evolution_split(int max_group) { g_split_max.store(max_group); }
+int evolution_split() { return g_split_max.load(); }
 
 void DeviceEvolution::run_learned(int cycles, float fallback) {
   if (!impl_->learned) throw InvalidArgument("the evolution was set up without a network.");
@@ -233,7 +315,14 @@ void DeviceEvolution::export_results(int* clbest, double* accbest, double* vect_
     down = pageable.data();
   }
   check(hipMemcpyAsync(down, m.arena.get() + m.down_begin, m.down_bytes, hipMemcpyDeviceToHost, s), "download");
+  if (m.h_error.size() > 0) {
+    check(hipMemcpyAsync(m.h_error.get(), m.arena.get() + m.off_error, sizeof(int), hipMemcpyDeviceToHost, s),
+          "download error word");
+  }
   check(hipStreamSynchronize(s), "export sync");
+  if (m.h_error.size() > 0 && m.h_error[0] != 0) {
+    throw DeviceError("device-resident evolution: a split run's exchange wait ran out.");
+  }
   const auto* best = reinterpret_cast<const unsigned short*>(down + (m.off_best - m.down_begin));
   const auto* state = reinterpret_cast<const double*>(down + (m.off_state - m.down_begin));
   const auto* hist = reinterpret_cast<const double*>(down + (m.off_history - m.down_begin));

@@ -35,11 +35,19 @@ class DeviceEvolution {
   // Time from the start of the first launch to the end of the last (HIP
   // events on the batch stream); valid after export_results.
   double kernel_seconds() const;
+  // Workgroups of the launches (the runs, or more when large runs are split).
+  int workgroups() const;
 
   struct Impl;
 
  private:
   Impl* impl_ = nullptr;
 };
+
+// Largest number of workgroups per run in later evolutions: 0 = automatic
+// (up to kEvoMaxGroup while the device holds them all at once), 1 = one
+// workgroup per run.
+void set_evolution_split(int max_group);
+int evolution_split();
 
 }  // namespace kodama_hip

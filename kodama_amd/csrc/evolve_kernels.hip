@@ -353,10 +353,12 @@ template <int KMAX, int THREADS>
 __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const Lds& s,
                                            const unsigned short* __restrict__ nbr,
                                            const float* __restrict__ scores, const unsigned short* lab,
-                                           unsigned short* pred, int n, bool count_labels) {
+                                           unsigned short* pred, int n, bool count_labels, int pair_lo,
+                                           int pair_hi) {
   const uint32_t* __restrict__ nbr2 = reinterpret_cast<const uint32_t*>(nbr);
   const int pad2 = pad_total >> 1;
-  const int npairs = (n + 1) >> 1;
+  const int npairs = pair_hi;
+  const int lim = min(n, 2 * pair_hi);  // rows past the range are another member's
   const int lane = tid & (kWave - 1);
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t sentinel2 = static_cast<uint32_t>(n) | static_cast<uint32_t>(n) << 16;
@@ -365,7 +367,7 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   // rows per thread and pass: with the lookups issued in groups, loading two
   // pairs' ids first (as the 8-wave learned launch once did) gains nothing.
   constexpr int kBatch = 1;
-  for (int m0 = 0; m0 < npairs; m0 += kBatch * THREADS) {
+  for (int m0 = pair_lo; m0 < npairs; m0 += kBatch * THREADS) {
     uint32_t ids_b[kBatch][KMAX];
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
@@ -382,7 +384,7 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
       const uint32_t(&ids)[KMAX] = ids_b[hb >> 1];
       const int i = 2 * (m0 + (hb >> 1) * THREADS + tid) + h;
       bool queued = false;
-      if (i < n) {
+      if (i < lim) {
         const int own = lab[i];
         if (count_labels) atomicAdd(&s.counts[own], 1);
         // Ranks past k read the sentinel slot, which matches code0 only when
@@ -475,10 +477,17 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   constexpr int kRowsPerThread = (kEvoMaxRows + THREADS - 1) / THREADS;
   extern __shared__ __align__(16) double lds_raw[];
   const int C = p.classes;
-  const int run = blockIdx.x;
+  const int wg = blockIdx.x;
+  const int run = p.wg_run ? p.wg_run[wg] : wg;
+  const int member = p.wg_run ? p.wg_member[wg] : 0;
+  const int group = p.wg_run ? p.run_group[run] : 1;
+  const bool lead = member == 0;  // writes the run's results
   const Lds s = carve(reinterpret_cast<unsigned char*>(lds_raw), C, p.max_rows,
-                      LARGE ? p.large_scratch + static_cast<size_t>(run) * 2 * p.max_rows : nullptr);
+                      LARGE ? p.large_scratch + static_cast<size_t>(wg) * 2 * p.max_rows : nullptr);
   const int n = p.run_offsets[run + 1] - p.run_offsets[run];
+  // This member's slice of rows for the vote (even start, so whole id pairs).
+  const int share = ((n + group - 1) / group + 1) & ~1;
+  const int slice_lo = min(n, member * share), slice_hi = min(n, slice_lo + share);
   const int base = p.pad_offsets[run];
   const unsigned short* __restrict__ nbr = p.neighbors + base;
   const float* __restrict__ scores = p.scores + base;
@@ -486,7 +495,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   const int bits = p.perm_bits[run];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  int* flows = p.flows + static_cast<size_t>(run) * C * C;
+  int* flows = p.flows + static_cast<size_t>(wg) * C * C;
   const int k = p.k;
   const int pad_total = p.pad_total;
   const bool guarded = p.guarded != 0;
@@ -889,7 +898,55 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     }
 
     // -- vote -------------------------------------------------------------------
-    vote_phase<KMAX, THREADS>(tid, k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate);
+    vote_phase<KMAX, THREADS>(tid, k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate && group == 1,
+                              slice_lo >> 1, (slice_hi + 1) >> 1);
+    if (group > 1) {
+      // Exchange the slices' predictions and match counts between the run's
+      // members (see EvoParams). Epochs are 1..65535, never the zero of a
+      // cleared word, and differ between consecutive steps.
+      const unsigned ep = 1u + static_cast<unsigned>((step + 1) % 65535);
+      for (int i = slice_lo + tid; i < slice_hi; i += THREADS) {
+        __hip_atomic_store(&p.xbuf[base + i], (ep << 16) | prop_pred[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (tid == 0) {
+        __hip_atomic_store(&p.xmatch[run * kEvoMaxGroup + member], (ep << 14) | static_cast<unsigned>(s.scal[kMatches]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // Reads wait for the current epoch; a wait that runs out of polls (or
+      // finds another one did) flags the error and stops waiting, so every
+      // member finishes.
+      auto await = [&](unsigned* word, int shift) {
+        unsigned w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int polls = 0;
+        while ((w >> shift) != ep) {
+          if (++polls > p.spin_limit || __hip_atomic_load(p.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __hip_atomic_store(p.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return w;
+      };
+      for (int i = tid; i < n; i += THREADS) {
+        if (i >= slice_lo && i < slice_hi) continue;
+        prop_pred[i] = static_cast<unsigned short>(await(&p.xbuf[base + i], 16) & 0xFFFFu);
+      }
+      if (tid == 0) {
+        int total = 0;
+        for (int m = 0; m < group; ++m) {
+          total += m == member ? s.scal[kMatches]
+                               : static_cast<int>(await(&p.xmatch[run * kEvoMaxGroup + m], 14) & 0x3FFFu);
+        }
+        s.scal[kMatches] = total;
+      }
+      // The evaluation's class counts, over all rows (the sliced vote
+      // counted none).
+      if (evaluate) {
+        for (int i = tid; i < n; i += THREADS) atomicAdd(&s.counts[prop[i]], 1);
+      }
+      __syncthreads();
+    }
 
     // -- accept -----------------------------------------------------------------
     if (wave == 0) {
@@ -921,14 +978,14 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             s.state[1] = score;
             dec |= kEvoAccepted;
           }
-          if (p.history) p.history[static_cast<size_t>(run) * p.cycles + cycle] = s.state[2];
+          if (p.history && lead) p.history[static_cast<size_t>(run) * p.cycles + cycle] = s.state[2];
         }
         s.scal[kDecision] = dec;
       }
     }
     __syncthreads();
     const int dec = s.scal[kDecision];
-    if (dec & kEvoImproved) {
+    if ((dec & kEvoImproved) && lead) {
       for (int i = tid; i < n; i += THREADS) {
         p.best[base + i] = prop[i];
         p.bestpred[base + i] = prop_pred[i];
@@ -946,6 +1003,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   }
 
   // Write the run state back for the next launch.
+  if (!lead) return;
   const int cur = s.scal[kCur];
   for (int i = threadIdx.x; i < n; i += THREADS) {
     p.cur[base + i] = s.lab(cur)[i];
@@ -995,6 +1053,13 @@ static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStrea
   const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>);
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
+  if (p.wg_run != nullptr) {
+    // Split runs wait on each other: all workgroups must be resident at once,
+    // which a cooperative launch guarantees (or refuses).
+    EvoParams arg = p;
+    void* args[] = {&arg};
+    return hipLaunchCooperativeKernel(fn, dim3(p.workgroups), dim3(threads), args, static_cast<unsigned>(shmem), s);
+  }
   hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>), dim3(p.runs), dim3(threads), shmem, s,
                      p);
   return hipGetLastError();
@@ -1008,12 +1073,39 @@ static hipError_t launch_persistent(const EvoParams& p, size_t shmem, hipStream_
   return launch_persistent_k<64, LEARNED, LARGE>(p, shmem, s);
 }
 
+template <int KMAX, bool LEARNED, bool LARGE>
+static int resident_per_cu_k(size_t shmem) {
+  constexpr int threads = KMAX > 32 ? kThreads / 2 : (LEARNED ? kLearnedThreads : kThreads);
+  const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)) != hipSuccess) return 0;
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, threads, shmem) != hipSuccess) return 0;
+  return blocks;
+}
+
+template <bool LEARNED, bool LARGE>
+static int resident_per_cu(int k, size_t shmem) {
+  if (k <= 8) return resident_per_cu_k<8, LEARNED, LARGE>(shmem);
+  if (k <= 16) return resident_per_cu_k<16, LEARNED, LARGE>(shmem);
+  if (k <= 32) return resident_per_cu_k<32, LEARNED, LARGE>(shmem);
+  return resident_per_cu_k<64, LEARNED, LARGE>(shmem);
+}
+
+int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large) {
+  const size_t shmem = evo_lds_bytes(classes, max_rows, learned, large);
+  if (large) return learned ? resident_per_cu<true, true>(k, shmem) : resident_per_cu<false, true>(k, shmem);
+  return learned ? resident_per_cu<true, false>(k, shmem) : resident_per_cu<false, false>(k, shmem);
+}
+
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s) {
   const bool learned = p.learn_packed != nullptr;
   const bool large = p.large_scratch != nullptr;
   const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned, large);
   if (shmem > kEvoMaxLds || p.max_rows > (large ? kEvoMaxRowsLarge : kEvoMaxRows) || p.classes > 1024 ||
-      p.k < 1 || p.k > 64 || p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31)) {
+      p.k < 1 || p.k > 64 || p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31) ||
+      (p.wg_run != nullptr && (p.wg_member == nullptr || p.run_group == nullptr || p.xbuf == nullptr ||
+                               p.xmatch == nullptr || p.error == nullptr || p.workgroups < p.runs ||
+                               p.spin_limit < 1))) {
     return hipErrorInvalidValue;
   }
   if (large) return learned ? launch_persistent<true, true>(p, shmem, s) : launch_persistent<false, true>(p, shmem, s);

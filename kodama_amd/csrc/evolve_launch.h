@@ -43,21 +43,45 @@ struct EvoParams {
   unsigned short* best = nullptr;
   unsigned short* bestpred = nullptr;
   double* state = nullptr;                   // [runs x kEvoStateStride]
-  int* flows = nullptr;                      // [runs x classes x classes], zero between cycles
+  int* flows = nullptr;                      // [workgroups x classes x classes], zero between cycles
   double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
   // Learned proposal (null: every cycle uniform): the network in the tiled
   // layout of launch_mlp_pack, and the probability that a cycle falls back to
   // the uniform proposal.
   const uint32_t* learn_packed = nullptr;
   float learn_fallback = 0.0f;
-  // Large-run variant: [runs x 2 x max_rows] ints (distinct flow entries, then
-  // the row queue of each run); null for the LDS-resident layout.
+  // Large-run variant: [workgroups x 2 x max_rows] ints (distinct flow
+  // entries, then the row queue of each workgroup); null for the
+  // LDS-resident layout.
   int* large_scratch = nullptr;
+  // Split runs (null wg_run: one workgroup per run, workgroup r = run r).
+  // Workgroup b works on run wg_run[b] as member wg_member[b] of
+  // run_group[run] members. Every member replays the same proposal,
+  // absorption and acceptance (identical draws and integer decisions) in its
+  // own LDS and votes only its slice of rows; slices are exchanged through
+  // xbuf ([pad_total] words, (epoch << 16) | prediction) and the match counts
+  // through xmatch ([runs x kEvoMaxGroup], (epoch << 14) | matches), written
+  // and read with device-scope relaxed atomics; a reader waits (bounded by
+  // spin_limit polls) until a word carries the current epoch. Members need to
+  // be resident together: the launch is cooperative. The flow tables and the
+  // large-run scratch are per workgroup. *error is set when a wait ran out.
+  const int* wg_run = nullptr;
+  const int* wg_member = nullptr;
+  const int* run_group = nullptr;
+  unsigned* xbuf = nullptr;
+  unsigned* xmatch = nullptr;
+  int* error = nullptr;
+  int spin_limit = 0;
+  int workgroups = 0;                        // grid size (runs when not split)
 };
+constexpr int kEvoMaxGroup = 4;  // workgroups per run at most
 
 // LDS of one run-workgroup; `large` leaves the row queue and the flow list out.
 size_t evo_lds_bytes(int classes, int max_rows, bool learned = false, bool large = false);
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
+// Workgroups of the persistent kernel's variant for these sizes that one CU
+// holds at once (0 when the query fails).
+int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large);
 // 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids; a
 // missing neighbour (-1) becomes the run's row count, its sentinel slot.
 hipError_t launch_evo_narrow(const int* nbr32, const int* run_offsets, const int* pad_offsets, const int* block_run,
