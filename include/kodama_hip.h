@@ -150,6 +150,13 @@ int kodama_hip_last_fold_table_path(void);
  * thread's last device-resident evolution (0 before any). */
 int kodama_hip_set_evolution_split(int32_t max_group);
 int kodama_hip_last_evolution_workgroups(void);
+/* Split runs wait on each other, so their workgroups must be resident
+ * together: by default they take a cooperative launch, which guarantees it.
+ * cooperative = 0 launches them as an ordinary grid of the same size (for
+ * profilers that cannot follow cooperative launches); the members' waits are
+ * bounded, so a member that is not resident makes the call fail with
+ * ERUNTIME instead of hanging. No reference counterpart. */
+int kodama_hip_set_cooperative_launch(int32_t cooperative);
 
 /* ---- device seam: exact kNN ------------------------------------------------
  * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).

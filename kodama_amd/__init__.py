@@ -21,7 +21,7 @@ __all__ = [
     "KNNCV", "PLSLDACV", "CoreKNN", "CorePLSLDA", "CoreKNNGraph", "KODAMA_knn_graph",
     "KODAMA_dissimilarity", "core_knn", "core_plslda", "core_knn_batch", "exact_knn_search",
     "KnnVoteGraph", "core_options", "device_count", "version",
-    "set_evolution_split", "last_evolution_workgroups",
+    "set_evolution_split", "last_evolution_workgroups", "set_cooperative_launch",
 ]
 
 
@@ -613,6 +613,12 @@ def set_evolution_split(max_group=0):
     automatic (large runs over up to 4 workgroups), 1 = one per run, 2-4 = at
     most that many. Results are identical for every setting."""
     N.call("kodama_hip_set_evolution_split", C.c_int32(int(max_group)))
+
+
+def set_cooperative_launch(cooperative=True):
+    """Split runs launch cooperatively (default) or as an ordinary grid, for
+    profilers that cannot follow cooperative launches."""
+    N.call("kodama_hip_set_cooperative_launch", C.c_int32(1 if cooperative else 0))
 
 
 def last_evolution_workgroups():

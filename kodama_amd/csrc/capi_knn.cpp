@@ -437,4 +437,11 @@ int kodama_hip_set_evolution_split(int32_t max_group) {
 
 int kodama_hip_last_evolution_workgroups(void) { return g_last_evolution_workgroups; }
 
+int kodama_hip_set_cooperative_launch(int32_t cooperative) {
+  return guarded([&] {
+    require(cooperative == 0 || cooperative == 1, "cooperative must be 0 or 1.");
+    set_cooperative_launch(cooperative != 0);
+  });
+}
+
 }  // extern "C"

@@ -17,6 +17,7 @@ namespace kodama_hip {
 
 namespace {
 std::atomic<int> g_split_max{0};
+std::atomic<int> g_cooperative{1};
 constexpr int kMinSliceRows = 512;  // a split run's members vote at least this many rows each
 constexpr int kSpinLimit = 1 << 20;
 
@@ -221,6 +222,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     p.error = reinterpret_cast<int*>(base + m.off_error);
     p.spin_limit = kSpinLimit;
     p.workgroups = m.workgroups;
+    p.cooperative = g_cooperative.load();
   }
   // The set-up is complete before the timed evolution starts.
   check(hipStreamSynchronize(s), "setup sync");
@@ -296,6 +298,7 @@ void DeviceEvolution::run(int cycles) { launch_cycles(*impl_, cycles, false, 0.0
 int DeviceEvolution::workgroups() const { return impl_->workgroups; }
 
 void set_evolution_split(int max_group) { g_split_max.store(max_group); }
+void set_cooperative_launch(bool on) { g_cooperative.store(on ? 1 : 0); }
 int evolution_split() { return g_split_max.load(); }
 
 void DeviceEvolution::run_learned(int cycles, float fallback) {

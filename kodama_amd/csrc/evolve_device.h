@@ -49,5 +49,8 @@ class DeviceEvolution {
 // workgroup per run.
 void set_evolution_split(int max_group);
 int evolution_split();
+// Split runs take a cooperative launch (default) or an ordinary one (for
+// tools that cannot follow cooperative launches).
+void set_cooperative_launch(bool on);
 
 }  // namespace kodama_hip

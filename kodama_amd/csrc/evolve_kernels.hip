@@ -1053,15 +1053,18 @@ static hipError_t launch_persistent_k(const EvoParams& p, size_t shmem, hipStrea
   const void* fn = reinterpret_cast<const void*>(&evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>);
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem));
   if (e != hipSuccess) return e;
-  if (p.wg_run != nullptr) {
+  if (p.wg_run != nullptr && p.cooperative) {
     // Split runs wait on each other: all workgroups must be resident at once,
     // which a cooperative launch guarantees (or refuses).
     EvoParams arg = p;
     void* args[] = {&arg};
     return hipLaunchCooperativeKernel(fn, dim3(p.workgroups), dim3(threads), args, static_cast<unsigned>(shmem), s);
   }
-  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>), dim3(p.runs), dim3(threads), shmem, s,
-                     p);
+  // One workgroup per run, or split runs launched as an ordinary grid (sized
+  // to what the device holds at once; the bounded waits turn a member that is
+  // not resident into an error, not a hang).
+  hipLaunchKernelGGL((evo_persistent_kernel<KMAX, threads, LEARNED, LARGE>),
+                     dim3(p.wg_run != nullptr ? p.workgroups : p.runs), dim3(threads), shmem, s, p);
   return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ struct EvoParams {
   int* error = nullptr;
   int spin_limit = 0;
   int workgroups = 0;                        // grid size (runs when not split)
+  int cooperative = 1;                       // split runs: cooperative launch (else an ordinary one)
 };
 constexpr int kEvoMaxGroup = 4;  // workgroups per run at most
 
