@@ -139,6 +139,17 @@ int kodama_hip_knn_fold_tables(const float* x, int32_t n, int32_t p, int32_t run
  * the calling thread's last table build used (0 before any). */
 int kodama_hip_set_fold_table_path(int32_t path);
 int kodama_hip_last_fold_table_path(void);
+/* The tuner's full schedule (no reference counterpart): path as above; for
+ * the MFMA path the candidates kept per (query, lane column), list = 8 (k <=
+ * 32) or 16, and the waves of 16 queries per workgroup, waves = 4 or 8; 0 =
+ * tuned. With path 0 the first table of a shape bucket times the scan (below
+ * the bucket sizes where MFMA is known to win) and every MFMA schedule the
+ * forced values leave, and the fastest serves the bucket from then on; all
+ * schedules give identical tables. A forced list that excludes the job's k
+ * leaves the scan. kodama_hip_last_fold_table_schedule reports the calling
+ * thread's last build (list and waves 0 for the scan). */
+int kodama_hip_set_fold_table_schedule(int32_t path, int32_t list, int32_t waves);
+int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* waves);
 
 /* Tuning control of the device-resident evolution (no reference counterpart:
  * the reference runs one CUDA stream per run). Runs of at least 1024 rows are

@@ -603,6 +603,21 @@ def set_fold_table_path(path="tuned"):
     N.call("kodama_hip_set_fold_table_path", C.c_int32(_TABLE_PATHS[path]))
 
 
+def set_fold_table_schedule(path="tuned", list=0, waves=0):
+    """Process-wide schedule of the per-fold tables: path ("tuned", "scan",
+    "mfma"), and for the MFMA path the candidate list length (8 or 16) and the
+    waves per workgroup (4 or 8); 0 = tuned. Tables are identical for all."""
+    N.call("kodama_hip_set_fold_table_schedule", C.c_int32(_TABLE_PATHS[path]), C.c_int32(int(list)),
+           C.c_int32(int(waves)))
+
+
+def last_fold_table_schedule():
+    """Schedule of this thread's last per-fold table build: dict(path, list, waves)."""
+    p, l, w = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    N.call("kodama_hip_last_fold_table_schedule", C.byref(p), C.byref(l), C.byref(w))
+    return dict(path={0: None, 1: "scan", 2: "mfma"}[p.value], list=l.value, waves=w.value)
+
+
 def last_fold_table_path():
     """Path of this thread's last per-fold table build ("scan", "mfma"; None before any)."""
     return {0: None, 1: "scan", 2: "mfma"}[int(N.lib().kodama_hip_last_fold_table_path())]

@@ -428,6 +428,25 @@ int kodama_hip_set_fold_table_path(int32_t path) {
 
 int kodama_hip_last_fold_table_path(void) { return last_fold_table_path(); }
 
+int kodama_hip_set_fold_table_schedule(int32_t path, int32_t list, int32_t waves) {
+  return guarded([&] {
+    require(path >= 0 && path <= 2, "path must be 0 (tuned), 1 (scan) or 2 (MFMA).");
+    require(list == 0 || list == 8 || list == 16, "list must be 0 (tuned), 8 or 16.");
+    require(waves == 0 || waves == 4 || waves == 8, "waves must be 0 (tuned), 4 or 8.");
+    set_fold_table_schedule(TableSchedule{path, list, waves});
+  });
+}
+
+int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* waves) {
+  return guarded([&] {
+    require(path != nullptr && list != nullptr && waves != nullptr, "null pointer.");
+    const TableSchedule t = last_fold_table_schedule();
+    *path = t.path;
+    *list = t.list;
+    *waves = t.waves;
+  });
+}
+
 int kodama_hip_set_evolution_split(int32_t max_group) {
   return guarded([&] {
     require(max_group >= 0 && max_group <= kEvoMaxGroup, "max_group must be 0 (automatic) to 4.");

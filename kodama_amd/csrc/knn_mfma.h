@@ -40,7 +40,13 @@ struct KnnMfmaParams {
   int* flags = nullptr;                  // [query_rows] 1 = needs the exact scan
   int* out_index = nullptr;              // [query_rows x k]
   float* out_dist = nullptr;             // internal distances
+  // Schedule: candidates kept per (query, lane column) (0: mfma_list_len(k);
+  // 8 serves k <= 32, 16 any k), and waves of 16 queries per workgroup (4 or
+  // 8). Every schedule gives the same neighbours.
+  int list = 0;
+  int waves = 8;
 };
+inline int mfma_list(const KnnMfmaParams& p) { return p.list != 0 ? p.list : mfma_list_len(p.k); }
 
 hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi, unsigned short* lo, float* norm2,
                              hipStream_t s);
@@ -76,6 +82,16 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
 // table of a shape bucket (p, k class, metric, log2 of the largest run and of
 // all rows, folds) both paths run and timed, and the faster serves that bucket
 // from then on. Returns the path used.
+// Schedule of a per-fold table build: path 1 scan, 2 MFMA (with `list`
+// candidates per lane column and `waves` waves per workgroup); 0 = tuned.
+struct TableSchedule {
+  int path = 0;
+  int list = 0;
+  int waves = 0;
+};
+void set_fold_table_schedule(const TableSchedule& t);
+TableSchedule last_fold_table_schedule();
+
 struct FoldTableJob {
   const float* d_x = nullptr;       // prepared rows [n x p]
   int p = 0, k = 0, metric = 0;

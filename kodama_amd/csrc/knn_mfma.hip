@@ -12,11 +12,13 @@
 // rounding of the norms and of the final combination, and the reference-order
 // rounding of the exact distance.
 //
-// knn_mfma_candidates_kernel: 512 threads, 128 queries per workgroup (8 waves
-// of 16 queries); train rows stream through LDS 64 at a time. MFMA output
+// knn_mfma_candidates_kernel: WAVES (4 or 8) waves of 16 queries per
+// workgroup; train rows stream through LDS 64 at a time. MFMA output
 // lane l holds queries 4 (l / 16) + r (r = 0..3) against train column l % 16 of
-// each 16-row tile, so each lane keeps, per query, a sorted list of its 8
-// smallest candidate values over the train rows of its column class.
+// each 16-row tile, so each lane keeps, per query, a sorted list of its LIST
+// (8 or 16) smallest candidate values over the train rows of its column
+// class. The schedule (LIST, WAVES) does not change the result: the rerank
+// certifies every query against its own lists.
 //
 // knn_rerank_kernel: one wave per query. A_k = the k-th smallest candidate
 // value over the query's 16 lists. Any train row whose exact distance can be
@@ -41,8 +43,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWave = 64;
-constexpr int kCandThreads = 512;
-constexpr int kQueriesPerBlock = 128;   // 8 waves x 16 queries
 constexpr int kChunk = 64;              // train rows per LDS stage
 constexpr int kRowStride = 72;          // bf16 per staged row (64 + 8 pad: spreads LDS banks)
 
@@ -142,8 +142,10 @@ __global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, 
   norm2[r] = s;
 }
 
-template <bool EUCLID, bool TABLES, int LIST>
-__global__ void __launch_bounds__(kCandThreads) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
+template <bool EUCLID, bool TABLES, int LIST, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
+  constexpr int kCandThreads = WAVES * 64;
+  constexpr int kQueriesPerBlock = WAVES * 16;
   __shared__ unsigned short s_hi[kChunk * kRowStride];
   __shared__ unsigned short s_lo[kChunk * kRowStride];
   __shared__ float s_tn[kChunk];
@@ -426,41 +428,53 @@ hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi
   return hipGetLastError();
 }
 
-template <int LIST>
-static void launch_mfma_list(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
-  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
+template <int LIST, int WAVES>
+static void launch_mfma_schedule(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
+  const dim3 block(WAVES * 64);
   if (tables) {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST>), grid, dim3(kCandThreads), 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST, WAVES>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST, WAVES>), grid, block, 0, s, prm);
   } else {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST>), grid, dim3(kCandThreads), 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST>), grid, dim3(kCandThreads), 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST, WAVES>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST, WAVES>), grid, block, 0, s, prm);
   }
+}
+
+template <int LIST>
+static void launch_mfma_list(const KnnMfmaParams& prm, bool euclid, bool tables, int rows, hipStream_t s,
+                             unsigned runs) {
+  const int qpb = prm.waves * 16;
+  const dim3 grid(static_cast<unsigned>((rows + qpb - 1) / qpb), runs);
+  if (prm.waves == 4) launch_mfma_schedule<LIST, 4>(prm, euclid, tables, grid, s);
+  else launch_mfma_schedule<LIST, 8>(prm, euclid, tables, grid, s);
+  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
   if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
   else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
 }
 
+static bool schedule_ok(const KnnMfmaParams& prm) {
+  const int list = mfma_list(prm);
+  return (prm.waves == 4 || prm.waves == 8) && (list == 16 || (list == 8 && prm.k <= 32));
+}
+
 hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s) {
   if (prm.query_rows == 0) return hipSuccess;
-  if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.train_rows < 1) {
+  if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.train_rows < 1 || !schedule_ok(prm)) {
     return hipErrorInvalidValue;
   }
-  const dim3 grid(static_cast<unsigned>((prm.query_rows + kQueriesPerBlock - 1) / kQueriesPerBlock));
-  if (mfma_list_len(prm.k) == 8) launch_mfma_list<8>(prm, euclid, false, grid, s);
-  else launch_mfma_list<16>(prm, euclid, false, grid, s);
+  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, false, prm.query_rows, s, 1);
+  else launch_mfma_list<16>(prm, euclid, false, prm.query_rows, s, 1);
   return hipGetLastError();
 }
 
 hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int runs, int max_rows, hipStream_t s) {
   if (prm.query_rows == 0 || runs == 0) return hipSuccess;
   if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.run_offsets == nullptr ||
-      prm.folds == nullptr || prm.query_rows != prm.train_rows || max_rows < 1) {
+      prm.folds == nullptr || prm.query_rows != prm.train_rows || max_rows < 1 || !schedule_ok(prm)) {
     return hipErrorInvalidValue;
   }
-  const dim3 grid(static_cast<unsigned>((max_rows + kQueriesPerBlock - 1) / kQueriesPerBlock),
-                  static_cast<unsigned>(runs));
-  if (mfma_list_len(prm.k) == 8) launch_mfma_list<8>(prm, euclid, true, grid, s);
-  else launch_mfma_list<16>(prm, euclid, true, grid, s);
+  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, true, max_rows, s, static_cast<unsigned>(runs));
+  else launch_mfma_list<16>(prm, euclid, true, max_rows, s, static_cast<unsigned>(runs));
   return hipGetLastError();
 }
 

@@ -135,7 +135,7 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
 
 namespace {
 
-// Tuning cache of the per-fold tables: shape bucket -> the faster path.
+// Tuning cache of the per-fold tables: shape bucket -> the fastest schedule.
 struct TableShape {
   int p, kmax, euclid, rows_log2, total_log2, folds;
   bool operator<(const TableShape& o) const {
@@ -148,9 +148,9 @@ struct TableShape {
 constexpr int kMfmaSureRows = 2048;
 constexpr long long kMfmaSureTotal = 1ll << 16;
 std::mutex g_tune_mutex;
-std::map<TableShape, int> g_tuned;
-std::atomic<int> g_table_path{0};
-thread_local int g_last_table_path = 0;
+std::map<TableShape, TableSchedule> g_tuned;
+std::atomic<int> g_table_path{0}, g_table_list{0}, g_table_waves{0};
+thread_local TableSchedule g_last_table;
 
 int ceil_log2(int v) {
   int b = 0;
@@ -168,9 +168,15 @@ double seconds_of(const std::function<void()>& f, hipStream_t s) {
 
 }  // namespace
 
-void set_fold_table_path(int path) { g_table_path.store(path); }
+void set_fold_table_path(int path) { set_fold_table_schedule(TableSchedule{path, 0, 0}); }
 int fold_table_path() { return g_table_path.load(); }
-int last_fold_table_path() { return g_last_table_path; }
+int last_fold_table_path() { return g_last_table.path; }
+void set_fold_table_schedule(const TableSchedule& t) {
+  g_table_path.store(t.path);
+  g_table_list.store(t.list);
+  g_table_waves.store(t.waves);
+}
+TableSchedule last_fold_table_schedule() { return g_last_table; }
 
 int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
   const bool euclid = job.metric == KODAMA_HIP_EUCLIDEAN;
@@ -198,14 +204,16 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     prm.out_dist = job.d_score;
     check(launch_knn_topk(prm, 1, euclid, max_queries, runs, s), "knn_topk launch");
   };
-  // Split-BF16 MFMA candidates, exact rerank, the flagged rows rescanned.
-  auto mfma = [&]() {
+  // Split-BF16 MFMA candidates (with `list` candidates per lane column and
+  // `waves` waves of queries per workgroup), exact rerank, the flagged rows
+  // rescanned.
+  auto mfma = [&](int list, int waves) {
     DeviceBuffer<float> xc(static_cast<size_t>(total) * job.p);
     check(launch_gather_rows(job.d_x, job.d_rows, total, job.p, xc.get(), s), "gather_rows launch");
     DeviceBuffer<unsigned short> hi(static_cast<size_t>(total) * 64), lo(hi.size());
     DeviceBuffer<float> n2(static_cast<size_t>(total));
     check(launch_split_rows(xc.get(), total, job.p, hi.get(), lo.get(), n2.get(), s), "split_rows launch");
-    const size_t cand = static_cast<size_t>(total) * 16 * mfma_list_len(job.k);
+    const size_t cand = static_cast<size_t>(total) * 16 * list;
     DeviceBuffer<float> c_val(cand), r_dist(static_cast<size_t>(total) * job.k);
     DeviceBuffer<int> c_idx(cand), flags(static_cast<size_t>(total)), r_idx(r_dist.size());
     KnnMfmaParams m;
@@ -225,6 +233,8 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     m.flags = flags.get();
     m.out_index = r_idx.get();
     m.out_dist = r_dist.get();
+    m.list = list;
+    m.waves = waves;
     check(launch_knn_mfma_tables(m, euclid, runs, job.max_rows, s), "knn_mfma_tables launch");
     check(launch_publish_tables(r_idx.get(), r_dist.get(), flags.get(), job.d_run_offsets, runs, job.d_pad_offsets,
                                 job.pad_total, total, job.k, job.d_nbr, job.d_score, s),
@@ -289,44 +299,71 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     check(hipStreamSynchronize(s), "fold tables sync");
   };
 
+  // MFMA schedules: lists of 8 (k <= 32) and 16 candidates, 4 and 8 waves
+  // per workgroup; a forced list or wave count narrows them.
+  std::vector<TableSchedule> schedules;
+  for (int list : {8, 16}) {
+    if (list == 8 && job.k > 32) continue;
+    if (g_table_list.load() != 0 && list != g_table_list.load()) continue;
+    for (int waves : {8, 4}) {
+      if (g_table_waves.load() != 0 && waves != g_table_waves.load()) continue;
+      schedules.push_back(TableSchedule{2, list, waves});
+    }
+  }
+  // A forced list that the job's k excludes leaves the scan.
+  const bool mfma_usable = mfma_ok && !schedules.empty();
+  TableSchedule choice{path == 2 && !mfma_usable ? 1 : path, 0, 0};
   if (path == 0) {
-    if (!mfma_ok) {
-      path = 1;
+    if (!mfma_usable) {
+      choice = TableSchedule{1, 0, 0};
     } else {
       const TableShape key{job.p, job.k <= 8 ? 8 : job.k <= 16 ? 16 : 32, euclid ? 1 : 0, ceil_log2(job.max_rows),
                            ceil_log2(total), job.folds};
+      bool cached = false;
       {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) path = it->second;
+        if (it != g_tuned.end()) {
+          choice = it->second;
+          cached = true;
+        }
       }
-      if (path == 0 && job.max_rows >= kMfmaSureRows && total >= kMfmaSureTotal) {
-        // Runs this large sit well inside the region where the MFMA path
-        // measured several times faster (7.1 against 29.7 ms for 100 runs of
-        // 1000-5000 rows, p = 50, k = 30): taken without timing the scan.
-        path = 2;
+      if (!cached) {
+        // First use of this shape bucket: time every candidate schedule (they
+        // give identical tables) and keep the fastest. Runs this large sit
+        // well inside the region where the MFMA path measured several times
+        // faster than the scan (7.1 against 29.7 ms for 100 runs of
+        // 1000-5000 rows, p = 50, k = 30): the scan is not timed there.
+        const bool sure = job.max_rows >= kMfmaSureRows && total >= kMfmaSureTotal;
+        double best = 0.0;
+        if (!sure) {
+          best = seconds_of([&] { scan(nullptr, nullptr, job.max_rows); }, s);
+          choice = TableSchedule{1, 0, 0};
+        }
+        for (const TableSchedule& t : schedules) {
+          const double sec = seconds_of([&] { mfma(t.list, t.waves); }, s);
+          if (choice.path == 0 || sec < best) {
+            best = sec;
+            choice = t;
+          }
+        }
         std::lock_guard<std::mutex> lock(g_tune_mutex);
-        g_tuned[key] = path;
-      }
-      if (path == 0) {
-        // First use of this shape: time both paths (identical tables), keep the faster.
-        const double t_scan = seconds_of([&] { scan(nullptr, nullptr, job.max_rows); }, s);
-        const double t_mfma = seconds_of(mfma, s);
-        path = t_mfma < t_scan ? 2 : 1;
-        std::lock_guard<std::mutex> lock(g_tune_mutex);
-        g_tuned[key] = path;
-        g_last_table_path = path;
-        return path;
+        g_tuned[key] = choice;
+        g_last_table = choice;
+        return choice.path;
       }
     }
+  } else if (choice.path == 2) {
+    choice = schedules.front();
   }
-  if (path == 2) {
-    mfma();
+  if (choice.path == 2) {
+    mfma(choice.list, choice.waves);
   } else {
+    choice = TableSchedule{1, 0, 0};
     scan(nullptr, nullptr, job.max_rows);
   }
-  g_last_table_path = path;
-  return path;
+  g_last_table = choice;
+  return choice.path;
 }
 
 }  // namespace kodama_hip

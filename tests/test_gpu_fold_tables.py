@@ -94,3 +94,47 @@ def test_batch_evolution_identical_under_both_paths():
         K.set_fold_table_path("tuned")
     assert all(np.array_equal(a, b) for a, b in zip(out["scan"]["clbest"], out["mfma"]["clbest"]))
     assert np.array_equal(out["scan"]["vect_acc"], out["mfma"]["vect_acc"])
+
+
+@pytest.mark.parametrize("list_,waves", [(8, 4), (8, 8), (16, 4), (16, 8)])
+@pytest.mark.parametrize("k", [12, 30, 40])
+def test_fold_table_schedules_match_oracle(oracle, list_, waves, k):
+    """Every MFMA schedule the tuner chooses from (candidates per lane column,
+    waves per workgroup) gives the oracle's tables; a list of 8 cannot serve
+    k > 32, which then takes the scan."""
+    x, _ = D.config3(n=5000, p=50, seed=21)
+    rows, fids = _runs(x, [1500, 2400, 300], folds=10, seed=k + list_ + waves)
+    want = [oracle.precompute_knn_cv(x[rw], f, k, K._metric("euclidean")) for rw, f in zip(rows, fids)]
+    try:
+        K.set_fold_table_schedule("tuned", list_, waves)
+        nbr, score, used = K.knn_fold_tables(x, rows, fids, k, "euclidean", path="mfma")
+        sched = K.last_fold_table_schedule()
+    finally:
+        K.set_fold_table_schedule("tuned", 0, 0)
+    if list_ == 8 and k > 32:
+        assert used == "scan" and sched["path"] == "scan"
+    else:
+        assert used == "mfma" and sched == dict(path="mfma", list=list_, waves=waves)
+    for r, (wn, ws) in enumerate(want):
+        assert np.array_equal(nbr[r], wn), r
+        assert np.array_equal(score[r], ws), r
+
+
+def test_tuner_times_schedules_and_keeps_its_choice(oracle):
+    """With everything tuned, the first table of a shape bucket times the scan
+    and the four MFMA schedules; the choice is kept for the bucket and the
+    tables are the oracle's either way."""
+    x, _ = D.config3(n=6000, p=40, seed=33)
+    rows, fids = _runs(x, [1800, 2200], folds=8, seed=4)
+    K.set_fold_table_schedule("tuned", 0, 0)
+    first = K.knn_fold_tables(x, rows, fids, 20, "euclidean", path="tuned")
+    s1 = K.last_fold_table_schedule()
+    second = K.knn_fold_tables(x, rows, fids, 20, "euclidean", path="tuned")
+    s2 = K.last_fold_table_schedule()
+    assert s1 == s2 and s1["path"] in ("scan", "mfma")
+    if s1["path"] == "mfma":
+        assert s1["list"] in (8, 16) and s1["waves"] in (4, 8)
+    want = [oracle.precompute_knn_cv(x[rw], f, 20, K._metric("euclidean")) for rw, f in zip(rows, fids)]
+    for out in (first, second):
+        for r, (wn, ws) in enumerate(want):
+            assert np.array_equal(out[0][r], wn) and np.array_equal(out[1][r], ws)
