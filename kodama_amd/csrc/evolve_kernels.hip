@@ -3,14 +3,18 @@
 // prediction-guided singleton relabelling, many-to-one absorption,
 // diversity-guarded objective, evolutionary acceptance).
 //
-// One workgroup (1024 threads; 512 for k > 32) owns one run for a whole launch:
-// the run's current labels and predictions, the proposal, the class tables and
-// the absorption scratch live in LDS (16-bit label codes), so a launch runs any
-// number of cycles with only block barriers between phases. The neighbour table
-// (16-bit local ids, [k x rows]) streams from HBM; flow counts use a per-run HBM
-// table of which only the touched entries are read and reset. Random draws come
-// from a counter-based SplitMix64 hash of (run key, cycle, stream) instead of the
-// host's mt19937_64: the algorithm is the reference's, the draws are not.
+// A workgroup (1024 threads; 512 for k > 32 and the learned variant) holds one
+// run for a whole launch: the run's current labels and predictions, the
+// proposal, the class tables and the absorption scratch live in LDS (16-bit
+// label codes), so a launch runs any number of cycles with only block
+// barriers between phases. The neighbour table (16-bit local ids, [k x rows])
+// streams from HBM; flow counts use a per-workgroup HBM table of which only
+// the touched entries are read and reset. Random draws come from a
+// counter-based SplitMix64 hash of (run key, cycle, stream) instead of the
+// host's mt19937_64: the algorithm is the reference's, the draws are not. A
+// large run may be split over up to four workgroups (see EvoParams): each
+// replays every phase but votes only its slice of rows, and the slices'
+// predictions and match counts are exchanged after the vote.
 //
 // Cycle phases (all within the workgroup):
 //   propose  choose n ~ U{1..hi(cycle)} rows by a keyed permutation and set them
