@@ -932,9 +932,23 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         }
         return w;
       };
-      for (int i = tid; i < n; i += THREADS) {
-        if (i >= slice_lo && i < slice_hi) continue;
-        prop_pred[i] = static_cast<unsigned short>(await(&p.xbuf[base + i], 16) & 0xFFFFu);
+      // Eight of a thread's words in flight at once; a word not yet at the
+      // current epoch is then awaited on its own.
+      constexpr int kAhead = 8;
+      for (int i0 = tid; i0 < n; i0 += kAhead * THREADS) {
+        unsigned w[kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+          const int i = min(i0 + u * THREADS, n - 1);
+          w[u] = __hip_atomic_load(&p.xbuf[base + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+          const int i = i0 + u * THREADS;
+          if (i >= n || (i >= slice_lo && i < slice_hi)) continue;
+          if ((w[u] >> 16) != ep) w[u] = await(&p.xbuf[base + i], 16);
+          prop_pred[i] = static_cast<unsigned short>(w[u] & 0xFFFFu);
+        }
       }
       if (tid == 0) {
         int total = 0;
