@@ -317,7 +317,9 @@ int device_fold_tables(const FoldTableJob& job, hipStream_t s) {
     if (!mfma_usable) {
       choice = TableSchedule{1, 0, 0};
     } else {
-      const TableShape key{job.p, job.k <= 8 ? 8 : job.k <= 16 ? 16 : 32, euclid ? 1 : 0, ceil_log2(job.max_rows),
+      // k enters by its power-of-two bucket (8, 16, 32, 64, 128): the
+      // schedules open to a k differ across 32 (lists of 8 need k <= 32).
+      const TableShape key{job.p, 1 << std::max(3, ceil_log2(job.k)), euclid ? 1 : 0, ceil_log2(job.max_rows),
                            ceil_log2(total), job.folds};
       bool cached = false;
       {

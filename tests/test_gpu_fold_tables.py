@@ -138,3 +138,20 @@ def test_tuner_times_schedules_and_keeps_its_choice(oracle):
     for out in (first, second):
         for r, (wn, ws) in enumerate(want):
             assert np.array_equal(out[0][r], wn) and np.array_equal(out[1][r], ws)
+
+
+def test_tuned_choice_is_kept_per_k_bucket(oracle):
+    """A schedule tuned for k <= 32 (possibly 8 candidates per lane column) is
+    not reused for a k above 32 on the same shape: each k bucket is tuned on
+    its own and both tables are the oracle's."""
+    x, _ = D.config3(n=6000, p=50, seed=47)
+    rows, fids = _runs(x, [2600, 3100], folds=10, seed=9)
+    K.set_fold_table_schedule("tuned", 0, 0)
+    for k in (30, 40):
+        nbr, score, used = K.knn_fold_tables(x, rows, fids, k, "euclidean", path="tuned")
+        sched = K.last_fold_table_schedule()
+        if sched["path"] == "mfma" and k > 32:
+            assert sched["list"] == 16
+        want = [oracle.precompute_knn_cv(x[rw], f, k, K._metric("euclidean")) for rw, f in zip(rows, fids)]
+        for r, (wn, ws) in enumerate(want):
+            assert np.array_equal(nbr[r], wn) and np.array_equal(score[r], ws), (k, r)
