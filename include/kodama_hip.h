@@ -168,6 +168,14 @@ int kodama_hip_last_evolution_workgroups(void);
  * bounded, so a member that is not resident makes the call fail with
  * ERUNTIME instead of hanging. No reference counterpart. */
 int kodama_hip_set_cooperative_launch(int32_t cooperative);
+/* Flow counts of the device-resident evolution's absorption: in an LDS table
+ * when it fits beside the run state (enabled = 1, default) or in a
+ * per-workgroup HBM table (0); the results are identical.
+ * kodama_hip_last_evolution_lds_flows returns 1 when the calling thread's
+ * last device-resident evolution used the LDS table. No reference
+ * counterpart. */
+int kodama_hip_set_lds_flow_table(int32_t enabled);
+int kodama_hip_last_evolution_lds_flows(void);
 
 /* ---- device seam: exact kNN ------------------------------------------------
  * Replaces detail::native_cuda_exact_knn_search (native_cuda_backend.hpp:165-175).

@@ -21,7 +21,8 @@ __all__ = [
     "KNNCV", "PLSLDACV", "CoreKNN", "CorePLSLDA", "CoreKNNGraph", "KODAMA_knn_graph",
     "KODAMA_dissimilarity", "core_knn", "core_plslda", "core_knn_batch", "exact_knn_search",
     "KnnVoteGraph", "core_options", "device_count", "version",
-    "set_evolution_split", "last_evolution_workgroups", "set_cooperative_launch",
+    "set_evolution_split", "last_evolution_workgroups", "set_cooperative_launch", "set_lds_flow_table",
+    "last_evolution_lds_flows",
 ]
 
 
@@ -639,6 +640,18 @@ def set_cooperative_launch(cooperative=True):
 def last_evolution_workgroups():
     """Workgroups of this thread's last device-resident evolution (0 before any)."""
     return int(N.lib().kodama_hip_last_evolution_workgroups())
+
+
+def set_lds_flow_table(enabled=True):
+    """Process-wide: the device-resident evolution counts flows in an LDS
+    table when it fits (default) or in a per-workgroup HBM table. Results are
+    identical either way."""
+    N.call("kodama_hip_set_lds_flow_table", C.c_int32(1 if enabled else 0))
+
+
+def last_evolution_lds_flows():
+    """Whether this thread's last device-resident evolution counted flows in LDS."""
+    return bool(N.lib().kodama_hip_last_evolution_lds_flows())
 
 
 class KODAMAGraphHandle:

@@ -50,6 +50,7 @@ struct kodama_hip_vote_graph {
 namespace kodama_hip {
 
 thread_local int g_last_evolution_workgroups = 0;
+thread_local int g_last_evolution_lds_flows = 0;
 
 void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, float fallback, TraceSink* trace,
                    int32_t* clbest, double* accbest, double* vect_acc, double* timings) {
@@ -151,6 +152,7 @@ void run_knn_batch(const BatchRequest& q, int mode, const float* mlp_params, flo
     dev->wait();
     t_evo = seconds_since(tr);
     g_last_evolution_workgroups = dev->workgroups();
+    g_last_evolution_lds_flows = dev->lds_flow_table() ? 1 : 0;
     dev->export_results(clbest, accbest, vect_acc);
   }
   const double t_wall = seconds_since(t1);
@@ -462,5 +464,14 @@ int kodama_hip_set_cooperative_launch(int32_t cooperative) {
     set_cooperative_launch(cooperative != 0);
   });
 }
+
+int kodama_hip_set_lds_flow_table(int32_t enabled) {
+  return guarded([&] {
+    require(enabled == 0 || enabled == 1, "enabled must be 0 or 1.");
+    set_lds_flow_table(enabled != 0);
+  });
+}
+
+int kodama_hip_last_evolution_lds_flows(void) { return g_last_evolution_lds_flows; }
 
 }  // extern "C"

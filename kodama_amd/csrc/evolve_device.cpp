@@ -18,6 +18,7 @@ namespace kodama_hip {
 namespace {
 std::atomic<int> g_split_max{0};
 std::atomic<int> g_cooperative{1};
+std::atomic<int> g_lds_flows{1};
 constexpr int kMinSliceRows = 512;  // a split run's members vote at least this many rows each
 constexpr int kSpinLimit = 1 << 20;
 
@@ -47,6 +48,7 @@ struct DeviceEvolution::Impl {
   bool initialized = false;
   bool large = false;
   bool learned = false;
+  int hash_bits = 0;  // LDS flow table (0: the per-workgroup HBM table)
   EventTimer timer;
   bool timed = false;
   // Split runs: the exchange region (xbuf, xmatch, error) cleared before
@@ -97,6 +99,12 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   if (m.learned && evo_lds_bytes(m.classes, batch.max_rows(), true, m.large) > kEvoMaxLds) {
     throw InvalidArgument("the learned proposal's network does not fit the LDS of one compute unit beside the run state.");
   }
+  // Flow counts in an LDS table when it fits beside the run state (the
+  // per-workgroup HBM table otherwise, and for the large-run layout).
+  if (!m.large && g_lds_flows.load() != 0) {
+    const int bits = evo_hash_bits(batch.max_rows());
+    if (evo_lds_bytes(m.classes, batch.max_rows(), m.learned, false, bits) <= kEvoMaxLds) m.hash_bits = bits;
+  }
   // Split the largest runs over up to kEvoMaxGroup workgroups while the
   // workgroups stay within what the device holds at once: repeatedly one more
   // member for the run with the most rows per member, as long as each member
@@ -108,7 +116,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   if (max_group > 1) {
     int cus = 0;
     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
-    const int capacity = cus * evo_resident_per_cu(batch.k(), m.classes, batch.max_rows(), m.learned, m.large);
+    const int capacity =
+      cus * evo_resident_per_cu(batch.k(), m.classes, batch.max_rows(), m.learned, m.large, m.hash_bits);
     if (capacity >= m.runs) {
       std::priority_queue<std::pair<double, int>> pq;
       for (int r = 0; r < m.runs; ++r) pq.emplace(static_cast<double>(batch.run_rows(r)), r);
@@ -128,7 +137,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   for (int g : group) workgroups += g;
   const bool split = workgroups > m.runs;
   m.workgroups = split ? workgroups : m.runs;
-  const size_t flow_items = static_cast<size_t>(m.workgroups) * m.classes * m.classes;
+  const size_t flow_items = m.hash_bits > 0 ? 0 : static_cast<size_t>(m.workgroups) * m.classes * m.classes;
   const int pad = batch.pad_total();
   const size_t label_bytes = sizeof(unsigned short) * static_cast<size_t>(pad);
   const hipStream_t s = batch.stream();
@@ -208,7 +217,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.best = reinterpret_cast<unsigned short*>(base + m.off_best);
   p.bestpred = reinterpret_cast<unsigned short*>(base + off_bestpred);
   p.state = reinterpret_cast<double*>(base + m.off_state);
-  p.flows = reinterpret_cast<int*>(base + off_flows);
+  p.flows = m.hash_bits > 0 ? nullptr : reinterpret_cast<int*>(base + off_flows);
+  p.hash_bits = m.hash_bits;
   p.history = reinterpret_cast<double*>(base + m.off_history);
   p.large_scratch = m.large ? reinterpret_cast<int*>(base + off_large) : nullptr;
   p.learn_packed = m.learned ? reinterpret_cast<const uint32_t*>(base + m.off_packed) : nullptr;
@@ -296,9 +306,11 @@ void launch_cycles(DeviceEvolution::Impl& m, int cycles, bool learned, float fal
 void DeviceEvolution::run(int cycles) { launch_cycles(*impl_, cycles, false, 0.0f); }
 
 int DeviceEvolution::workgroups() const { return impl_->workgroups; }
+bool DeviceEvolution::lds_flow_table() const { return impl_->hash_bits > 0; }
 
 void set_evolution_split(int max_group) { g_split_max.store(max_group); }
 void set_cooperative_launch(bool on) { g_cooperative.store(on ? 1 : 0); }
+void set_lds_flow_table(bool on) { g_lds_flows.store(on ? 1 : 0); }
 int evolution_split() { return g_split_max.load(); }
 
 void DeviceEvolution::run_learned(int cycles, float fallback) {

@@ -37,6 +37,8 @@ class DeviceEvolution {
   double kernel_seconds() const;
   // Workgroups of the launches (the runs, or more when large runs are split).
   int workgroups() const;
+  // Whether the launches count flows in the LDS table.
+  bool lds_flow_table() const;
 
   struct Impl;
 
@@ -52,5 +54,8 @@ int evolution_split();
 // Split runs take a cooperative launch (default) or an ordinary one (for
 // tools that cannot follow cooperative launches).
 void set_cooperative_launch(bool on);
+// Later evolutions count flows in an LDS table when it fits beside the run
+// state (default) or always in the per-workgroup HBM table.
+void set_lds_flow_table(bool on);
 
 }  // namespace kodama_hip

@@ -94,7 +94,10 @@ __device__ __forceinline__ uint32_t round_fn(uint32_t r, uint32_t key) {
 // Keyed pseudo-random permutation of [0, n): a four-round unbalanced Feistel
 // network on `bits` = ceil(log2 n) bits (halves of floor and ceil width; each
 // round swaps their widths, four rounds restore them), cycle-walked into
-// [0, n). The walk takes fewer than two rounds of the network on average.
+// [0, n): the row at position y < n is the first of network_inverse(y),
+// network_inverse(network_inverse(y)), ... below n (the orbit of y under the
+// bijection returns below n, so the walk ends). It takes fewer than two rounds
+// of the network on average.
 struct Permutation {
   uint32_t k0, k1, k2, k3;
   int a, b;
@@ -143,14 +146,6 @@ struct Permutation {
     }
     return (l << wr) | r;
   }
-
-  // Row whose position in the permutation of [0, n) is y. Terminates: the
-  // orbit of y < n under the bijection returns below n.
-  __device__ __forceinline__ uint32_t preimage(uint32_t y, uint32_t n) const {
-    uint32_t x = network_inverse(y);
-    while (x >= n) x = network_inverse(x);
-    return x;
-  }
 };
 
 // Upper bound of the proposal size (core.cpp:1237-1253, adaptive form).
@@ -165,7 +160,6 @@ __device__ __forceinline__ int proposal_hi(int groups, int cycle, int cycles) {
 
 // LDS layout of one run-workgroup (8-byte entries first).
 struct Lds {
-  double* wl;                   // [C] eligible weight per present-list entry
   double* wave_total;           // [kWaves]
   double* state;                // [4] cur_acc, cur_score, best_acc, best_score
   unsigned long long* weight;   // [C] absorption weight per target, times n (exact)
@@ -175,13 +169,15 @@ struct Lds {
   int* counts;                  // [C] class counts after absorption (objective)
   int* nsrc;                    // [C] absorbed sources per target
   int* point;                   // [C] target of each source or -1
-  int* list;                    // [C] present classes, ascending
-  int* remap;                   // [C] absorption remap or -1
   int* pairs;                   // [max_rows] distinct flow entries (label * C + prediction)
   int* queue;                   // [max_rows] rows left to the full vote; flow keys while proposing
   int* scal;                    // [16] scalars
   unsigned short* labels;       // [4 x lmax] label buffers 0, 1 then prediction buffers 0, 1
   int lmax;                     // rows of the largest run + the sentinel slot, even
+  // LDS flow table (EvoParams::hash_bits > 0): per slot a flow entry (label *
+  // C + prediction, kEmptySlot when free) and its count.
+  unsigned* hkey;
+  int* hcnt;
   // Learned variant only: the network's tiled weights [kMlpTiled] (16-byte
   // aligned); per wave (kLearnedWaves) a tile of 16 rows x 64 features (bf16
   // bits) and 16 rows x 32 slots (ranked codes 0..29, the row's index in slot
@@ -196,17 +192,18 @@ struct Lds {
   __device__ unsigned short* pred(int b) const { return labels + (2 + b) * lmax; }
 };
 
-enum Scalar { kPresent, kPairs, kMatches, kChosen, kVanished, kPick, kDecision, kCur, kQueue, kCands };
+constexpr unsigned kEmptySlot = 0xFFFFFFFFu;  // above every flow entry (< 1024^2)
+
+enum Scalar { kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands };
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
 
-__device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows, int* large_rows) {
+__device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows, int* large_rows, int hash_bits) {
   Lds s;
   double* d = reinterpret_cast<double*>(base);
-  s.wl = d;
-  s.wave_total = d + C;
-  s.state = d + C + kWaves;
-  s.weight = reinterpret_cast<unsigned long long*>(d + C + kWaves + 4);
+  s.wave_total = d;
+  s.state = d + kWaves;
+  s.weight = reinterpret_cast<unsigned long long*>(d + kWaves + 4);
   s.best_key = s.weight + C;
   int* i = reinterpret_cast<int*>(s.best_key + C);
   s.sizes = i;
@@ -214,21 +211,22 @@ __device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows, i
   s.counts = i + 2 * C;
   s.nsrc = i + 3 * C;
   s.point = i + 4 * C;
-  s.list = i + 5 * C;
-  s.remap = i + 6 * C;
   if (large_rows != nullptr) {
     // Large-run variant: the flow list and the row queue in global scratch.
     s.pairs = large_rows;
     s.queue = large_rows + max_rows;
-    s.scal = i + 7 * C;
+    s.scal = i + 5 * C;
   } else {
-    s.pairs = i + 7 * C;
+    s.pairs = i + 5 * C;
     s.queue = s.pairs + max_rows;
     s.scal = s.queue + max_rows;
   }
   s.labels = reinterpret_cast<unsigned short*>(s.scal + 16);
   s.lmax = label_slots(max_rows);
-  const size_t used = static_cast<size_t>(reinterpret_cast<unsigned char*>(s.labels + 4 * s.lmax) - base);
+  const int slots = hash_bits > 0 ? 1 << hash_bits : 0;
+  s.hkey = reinterpret_cast<unsigned*>(s.labels + 4 * s.lmax);
+  s.hcnt = reinterpret_cast<int*>(s.hkey + slots);
+  const size_t used = static_cast<size_t>(reinterpret_cast<unsigned char*>(s.hcnt + slots) - base);
   s.mlp_w = reinterpret_cast<uint32_t*>(base + ((used + 15) & ~static_cast<size_t>(15)));
   s.mlp_tile = reinterpret_cast<unsigned short*>(s.mlp_w + kMlpTiled);
   s.mlp_rank = s.mlp_tile + kLearnedWaves * mlp::kTileRows * kWave;
@@ -258,11 +256,71 @@ __device__ __forceinline__ void sort_network(uint32_t (&a)[N]) {
   }
 }
 
+// Two 16-bit lanes per register: min and max of each lane (v_pk_min_u16 /
+// v_pk_max_u16).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_min16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
+// Bitonic sort of 2M 16-bit keys held two to a register: element i is half
+// i / M of register i % M. Every comparator sends the minimum to the lower
+// index; each merge of size s opens by comparing i with i ^ (s - 1) (the
+// mirror image, which makes the second half of a bitonic merge ascending too)
+// and continues with i ^ d for d = s/4 .. 1. Comparators between registers
+// act on both halves with one packed min and one packed max; only the
+// opening of the last merge pairs the low half of register r with the high
+// half of register M - 1 - r.
+template <int M>
+__device__ __forceinline__ void sort_packed(uint32_t (&v)[M]) {
+#pragma unroll
+  for (int size = 2; size <= 2 * M; size <<= 1) {
+    if (size <= M) {
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int q = r ^ (size - 1);
+        if (q > r) {
+          const uint32_t a = v[r], b = v[q];
+          v[r] = pk_min16(a, b);
+          v[q] = pk_max16(a, b);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < M / 2; ++r) {
+        const int q = M - 1 - r;
+        const uint32_t a = v[r], b = __builtin_amdgcn_alignbit(v[q], v[q], 16);  // halves of v[q] swapped
+        const uint32_t mn = pk_min16(a, b), mx = pk_max16(a, b);
+        v[r] = (mn & 0xFFFFu) | (mx & 0xFFFF0000u);
+        v[q] = __builtin_amdgcn_alignbit(mx, mn, 16);  // high half of mn, low half of mx
+      }
+    }
+#pragma unroll
+    for (int d = size >> 2; d > 0; d >>= 1) {
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int q = r ^ d;
+        if (q > r) {
+          const uint32_t a = v[r], b = v[q];
+          v[r] = pk_min16(a, b);
+          v[q] = pk_max16(a, b);
+        }
+      }
+    }
+  }
+}
+
 // Full vote of row i (core.cpp:871-911): most votes, then the larger double sum
 // of scores taken in neighbour-rank order, then the smaller label. Neighbours
-// become keys (code << 6 | rank) sorted ascending, so each label's votes form
-// one run with ranks ascending inside it; missing neighbours (the sentinel
-// code) sort last. Returns -1 for a row without neighbours.
+// become keys (code << kRankBits | rank) sorted ascending, so each label's
+// votes form one run with ranks ascending inside it; missing neighbours (the
+// sentinel code) sort last. Up to 32 neighbours a key has 5 rank bits and an
+// 11-bit code (labels up to 1023, the sentinel as 2047) and two keys share a
+// register in the sort; at 64 the keys are 32-bit. Returns -1 for a row
+// without neighbours.
 template <int KMAX>
 __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned short* __restrict__ nbr,
                                          const float* __restrict__ scores, const unsigned short* lab, int i) {
@@ -284,53 +342,69 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
     for (int j = j0; j < j0 + kLook; ++j) key[j] = lab[key[j]];
     __builtin_amdgcn_sched_barrier(0);
   }
+  constexpr bool kPacked = KMAX <= 32;
+  constexpr int kRankBits = kPacked ? 5 : 6;
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
     const uint32_t c = j < k ? key[j] : kSentinel;
     valid += c != kSentinel ? 1 : 0;
-    key[j] = (c << 6) | static_cast<uint32_t>(j);
+    const uint32_t code = kPacked && c == kSentinel ? 2047u : c;
+    key[j] = (code << kRankBits) | static_cast<uint32_t>(j);
   }
-  sort_network<KMAX>(key);
+  if constexpr (kPacked) {
+    constexpr int M = KMAX / 2;
+    uint32_t v[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) v[r] = key[r] | key[r + M] << 16;
+    sort_packed<M>(v);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      key[r] = v[r] & 0xFFFFu;
+      key[r + M] = v[r] >> 16;
+    }
+  } else {
+    sort_network<KMAX>(key);
+  }
+  // Longest run of one code among the valid entries, and whether another run
+  // is as long. Selects, not branches: per-entry branches would cost the wave
+  // an exec-mask update each.
   int best_count = 0, best_code = -1, run = 0;
   bool tie = false;
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
-    run = (j > 0 && (key[j] >> 6) == (key[j - 1] >> 6)) ? run + 1 : 1;
-    if (j < valid) {
-      if (run > best_count) {
-        best_count = run;
-        best_code = static_cast<int>(key[j] >> 6);
-        tie = false;
-      } else if (run == best_count) {
-        tie = true;
-      }
-    }
+    const uint32_t code = key[j] >> kRankBits;
+    run = (j > 0 && code == (key[j - 1] >> kRankBits)) ? run + 1 : 1;
+    const bool in = j < valid;
+    const bool longer = in && run > best_count;
+    tie = !longer && (tie || (in && run == best_count));
+    best_code = longer ? static_cast<int>(code) : best_code;
+    best_count = longer ? run : best_count;
   }
   if (!tie) return best_code;
-  // Ties are rare: the scores are read only here, in groups of eight loads
-  // issued before their sums (missing neighbours read rank 0 and are not
-  // summed). Element offsets fit 32 bits (k * pad_total < 2^31, checked at
-  // launch), so each load is a scalar base plus one offset register.
-  constexpr int kGroup = KMAX < 8 ? KMAX : 8;
+  // The scores are read only here, all of a row's loads in flight before the
+  // first sum: a wave of 64 rows almost always holds a tie, so the tie path's
+  // load latency is paid by the whole wave (missing neighbours read rank 0 and
+  // are not summed). Element offsets fit 32 bits (k * pad_total < 2^31,
+  // checked at launch), so each load is a scalar base plus one offset register.
+  constexpr int kGroup = KMAX;
   double best_sum = -INFINITY, sum = 0.0;
 #pragma unroll
   for (int j0 = 0; j0 < KMAX; j0 += kGroup) {
     float score[kGroup];
 #pragma unroll
     for (int t = 0; t < kGroup; ++t) {
-      const uint32_t rank = j0 + t < valid ? key[j0 + t] & 63u : 0u;
+      const uint32_t rank = j0 + t < valid ? key[j0 + t] & ((1u << kRankBits) - 1u) : 0u;
       score[t] = scores[static_cast<uint32_t>(static_cast<int>(rank) * pad_total + i)];
     }
 #pragma unroll
     for (int t = 0; t < kGroup; ++t) {
       const int j = j0 + t;
-      const bool fresh = j == 0 || (key[j] >> 6) != (key[j - 1] >> 6);
+      const bool fresh = j == 0 || (key[j] >> kRankBits) != (key[j - 1] >> kRankBits);
       run = fresh ? 1 : run + 1;
       sum = (fresh ? 0.0 : sum) + static_cast<double>(score[t]);
-      if (j < valid && run == best_count && sum > best_sum) {
-        best_sum = sum;
-        best_code = static_cast<int>(key[j] >> 6);
-      }
+      const bool better = j < valid && run == best_count && sum > best_sum;
+      best_sum = better ? sum : best_sum;
+      best_code = better ? static_cast<int>(key[j] >> kRankBits) : best_code;
     }
   }
   return best_code;
@@ -450,25 +524,27 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
 // Objective with the diversity guard from the class counts (wave 0).
 __device__ __forceinline__ double objective(const Lds& s, bool guarded, int C, int n, double* acc_out) {
   const int lane = threadIdx.x & (kWave - 1);
+  // Sum of squared class counts in integers (at most n^2 < 2^28), divided
+  // by n^2 once.
   int k = 0;
-  double same = 0.0;
-  const double nn = static_cast<double>(n);
+  unsigned sq = 0u;
   for (int c = lane; c < C; c += kWave) {
     const int v = s.counts[c];
     if (v > 0) {
       ++k;
-      const double q = static_cast<double>(v) / nn;
-      same += q * q;
+      sq += static_cast<unsigned>(v) * static_cast<unsigned>(v);
     }
   }
 #pragma unroll
   for (int m = kWave / 2; m >= 1; m >>= 1) {
     k += __shfl_xor(k, m);
-    same += __shfl_xor(same, m);
+    sq += __shfl_xor(sq, m);
   }
+  const double nn = static_cast<double>(n);
   const double acc = n ? static_cast<double>(s.scal[kMatches]) / nn : 0.0;
   *acc_out = acc;
   if (!guarded) return acc;
+  const double same = n ? static_cast<double>(sq) / (nn * nn) : 0.0;
   return k == 1 ? -INFINITY : acc * sqrt(fmax(0.0, 1.0 - same));
 }
 
@@ -479,6 +555,8 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   // Rows per thread and pass of the flow counting, for its register arrays
   // (one pass covers kEvoMaxRows rows at 1024 threads).
   constexpr int kRowsPerThread = (kEvoMaxRows + THREADS - 1) / THREADS;
+  // Classes per thread of the per-class phases (label codes are below 1024).
+  constexpr int kClassesPerThread = (1024 + THREADS - 1) / THREADS;
   extern __shared__ __align__(16) double lds_raw[];
   const int C = p.classes;
   const int wg = blockIdx.x;
@@ -486,8 +564,11 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   const int member = p.wg_run ? p.wg_member[wg] : 0;
   const int group = p.wg_run ? p.run_group[run] : 1;
   const bool lead = member == 0;  // writes the run's results
+  // LDS flow table (compile-time off for the large-run layout).
+  const int hash_bits = LARGE ? 0 : p.hash_bits;
+  const bool hashed = hash_bits > 0;
   const Lds s = carve(reinterpret_cast<unsigned char*>(lds_raw), C, p.max_rows,
-                      LARGE ? p.large_scratch + static_cast<size_t>(wg) * 2 * p.max_rows : nullptr);
+                      LARGE ? p.large_scratch + static_cast<size_t>(wg) * 2 * p.max_rows : nullptr, hash_bits);
   const int n = p.run_offsets[run + 1] - p.run_offsets[run];
   // This member's slice of rows for the vote (even start, so whole id pairs).
   const int share = ((n + group - 1) / group + 1) & ~1;
@@ -513,9 +594,14 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     s.counts[c] = 0;
     s.nsrc[c] = 0;
     s.point[c] = -1;
-    s.remap[c] = -1;
   }
   if (threadIdx.x < 16) s.scal[threadIdx.x] = 0;
+  if (hashed) {
+    for (int e = threadIdx.x; e < 1 << hash_bits; e += THREADS) {
+      s.hkey[e] = kEmptySlot;
+      s.hcnt[e] = 0;
+    }
+  }
   for (int i = threadIdx.x; i < n; i += THREADS) {
     s.lab(0)[i] = p.cur[base + i];
     // Before the first evaluation the predictions start equal to the labels
@@ -673,9 +759,23 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             if (!chosen) prop[i] = cur_lab[i];
           }
         } else {
+          // Each lane walks the preimages of its positions y = tid, tid +
+          // THREADS, ... one network step per pass, taking its next position
+          // as soon as a walk lands below n: the wave's passes follow the
+          // longest lane's total walk, not the longest walk of every position.
           const Permutation perm(draw(key, cycle, 2), bits);
-          for (int y = tid; y < take; y += THREADS) {
-            prop[perm.preimage(static_cast<uint32_t>(y), static_cast<uint32_t>(n))] = kChosenMark;
+          uint32_t y = static_cast<uint32_t>(tid), x = y;
+          bool live = static_cast<int>(y) < take;
+          while (__ballot(live) != 0ull) {
+            if (live) {
+              x = perm.network_inverse(x);
+              if (x < static_cast<uint32_t>(n)) {
+                prop[x] = kChosenMark;
+                y += THREADS;
+                live = static_cast<int>(y) < take;
+                x = y;
+              }
+            }
           }
         }
         __syncthreads();
@@ -688,58 +788,81 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           if (label == guide) atomicAdd(&s.stay[label], 1);
           s.queue[i] = label == guide ? -1 : label * C + guide;
         }
-        // Flow counts: all of a thread's returning atomics of a pass are in
-        // flight before the first result is used; the first increment of an
-        // entry lists it. One pass covers kRowsPerThread * THREADS rows.
-        for (int pass0 = 0; pass0 < n; pass0 += kRowsPerThread * THREADS) {
-          int fkey[kRowsPerThread], old[kRowsPerThread];
-#pragma unroll
-          for (int r = 0; r < kRowsPerThread; ++r) {
-            const int i = pass0 + tid + r * THREADS;
-            fkey[r] = i < n ? s.queue[i] : -1;
-            old[r] = fkey[r] >= 0 ? atomicAdd(&flows[fkey[r]], 1) : 1;
+        if (hashed) {
+          // Flow counts in the LDS table: open addressing with linear
+          // probing from a multiplicative hash of the entry; the row that
+          // claims a free slot lists the slot. The table has more slots than
+          // the run has rows, so a probe always ends.
+          const unsigned mask = (1u << hash_bits) - 1u;
+          for (int i = tid; i < n; i += THREADS) {
+            const int fk = s.queue[i];
+            if (fk < 0) continue;
+            unsigned h = (static_cast<unsigned>(fk) * 0x9E3779B1u) >> (32 - hash_bits);
+            while (true) {
+              const unsigned old = atomicCAS(&s.hkey[h], kEmptySlot, static_cast<unsigned>(fk));
+              if (old == kEmptySlot || old == static_cast<unsigned>(fk)) {
+                atomicAdd(&s.hcnt[h], 1);
+                if (old == kEmptySlot) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = static_cast<int>(h);
+                break;
+              }
+              h = (h + 1u) & mask;
+            }
           }
+        } else {
+          // Flow counts in the workgroup's HBM table: all of a thread's
+          // returning atomics of a pass are in flight before the first result
+          // is used; the first increment of an entry lists it. One pass
+          // covers kRowsPerThread * THREADS rows.
+          for (int pass0 = 0; pass0 < n; pass0 += kRowsPerThread * THREADS) {
+            int fkey[kRowsPerThread], old[kRowsPerThread];
 #pragma unroll
-          for (int r = 0; r < kRowsPerThread; ++r) {
-            if (old[r] == 0) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = fkey[r];
+            for (int r = 0; r < kRowsPerThread; ++r) {
+              const int i = pass0 + tid + r * THREADS;
+              fkey[r] = i < n ? s.queue[i] : -1;
+              old[r] = fkey[r] >= 0 ? atomicAdd(&flows[fkey[r]], 1) : 1;
+            }
+#pragma unroll
+            for (int r = 0; r < kRowsPerThread; ++r) {
+              if (old[r] == 0) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = fkey[r];
+            }
           }
         }
       }
       __syncthreads();
 
       // -- absorb ---------------------------------------------------------------
-      if (wave == 0) {
-        int running = 0;
-        for (int b = 0; b < C; b += kWave) {
-          const int c = b + lane;
-          const bool here = c < C && s.sizes[c] > 0;
-          const uint64_t mask = __ballot(here);
-          if (here) s.list[running + __popcll(mask & ((1ull << lane) - 1ull))] = c;
-          running += __popcll(mask);
-        }
-        if (lane == 0) {
-          s.scal[kPresent] = running;
-          s.scal[kChosen] = -1;
-          s.scal[kVanished] = 0;
-          s.scal[kPick] = INT_MAX;
-          s.scal[kDecision] = 0;
-        }
-      }
-      __syncthreads();
-      const int present = s.scal[kPresent];
-      const bool eligible = present > 2;
+      // One thread per class (kClassesPerThread consecutive classes per thread
+      // in workgroups of fewer than 1024 threads), so no list of the present
+      // classes is built: the present count comes from one ballot per wave.
       {
+        int here = 0;
+#pragma unroll
+        for (int j = 0; j < kClassesPerThread; ++j) {
+          const int c = tid * kClassesPerThread + j;
+          here += __popcll(__ballot(c < C && s.sizes[c] > 0));
+        }
+        if (lane == 0 && here != 0) atomicAdd(&s.scal[kPresent], here);
         // Best target per source: atomicMax of a key ordered as the reference's
         // candidate comparison -- surplus (exactly, as cnt*n - size_src*size_dst),
-        // then flow, then target size, then the smaller label.
+        // then flow, then target size, then the smaller label. Keys are built
+        // for every flow entry; they are read only when absorption is eligible.
         const int np = s.scal[kPairs];
         for (int q = tid; q < np; q += THREADS) {
-          const int idx = s.pairs[q];
-          // Read and clear in one L2 atomic: the propose-phase atomics of this
-          // launch bypass the vector L1, which may hold an older copy.
-          const int cnt = atomicExch(&flows[idx], 0);
+          int idx, cnt;
+          if (hashed) {
+            // The listed slot: its entry and count, cleared for the next cycle.
+            const int slot = s.pairs[q];
+            idx = static_cast<int>(s.hkey[slot]);
+            cnt = s.hcnt[slot];
+            s.hkey[slot] = kEmptySlot;
+            s.hcnt[slot] = 0;
+          } else {
+            idx = s.pairs[q];
+            // Read and clear in one L2 atomic: the propose-phase atomics of
+            // this launch bypass the vector L1, which may hold an older copy.
+            cnt = atomicExch(&flows[idx], 0);
+          }
           if (LARGE) s.queue[q] = cnt;  // the second pass below needs it again
-          if (!eligible) continue;
           const int src = idx / C;
           const int d = idx - src * C;
           const int dn = s.sizes[d];
@@ -760,30 +883,38 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             atomicMax(&s.best_key[src], k64);
           }
         }
-        if (LARGE && eligible) {
-          // Second pass: among the targets of a source's best key, the
-          // smallest label (kept in `point`, which is -1 until the consumer).
-          __syncthreads();
-          for (int q = tid; q < np; q += THREADS) {
-            const int idx = s.pairs[q];
-            const int cnt = s.queue[q];
-            const int src = idx / C;
-            const int d = idx - src * C;
-            const int dn = s.sizes[d];
-            if (dn == 0) continue;
-            const long long surplus_n = static_cast<long long>(cnt) * n - static_cast<long long>(s.sizes[src]) * dn;
-            const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 28)) << 28) |
-                                           (static_cast<unsigned long long>(cnt) << 14) |
-                                           static_cast<unsigned long long>(dn);
-            if (k64 == s.best_key[src]) atomicMin(reinterpret_cast<unsigned int*>(&s.point[src]), static_cast<unsigned int>(d));
-          }
-        }
       }
       __syncthreads();
+      const int present = s.scal[kPresent];
+      const bool eligible = present > 2;
+      if (LARGE && eligible) {
+        // Second pass: among the targets of a source's best key, the smallest
+        // label (kept in `point`, which is -1 until the consumer).
+        const int np = s.scal[kPairs];
+        for (int q = tid; q < np; q += THREADS) {
+          const int idx = s.pairs[q];
+          const int cnt = s.queue[q];
+          const int src = idx / C;
+          const int d = idx - src * C;
+          const int dn = s.sizes[d];
+          if (dn == 0) continue;
+          const long long surplus_n = static_cast<long long>(cnt) * n - static_cast<long long>(s.sizes[src]) * dn;
+          const unsigned long long k64 = (static_cast<unsigned long long>(surplus_n + (1ll << 28)) << 28) |
+                                         (static_cast<unsigned long long>(cnt) << 14) |
+                                         static_cast<unsigned long long>(dn);
+          if (k64 == s.best_key[src]) atomicMin(reinterpret_cast<unsigned int*>(&s.point[src]), static_cast<unsigned int>(d));
+        }
+        __syncthreads();
+      }
+      int tgt = -1;
       if (eligible) {
-        const double nn = fmax(1.0, static_cast<double>(n));
-        for (int e = tid; e < present; e += THREADS) {
-          const int src = s.list[e];
+        // Each source with a positive surplus that outflows its stayers points
+        // at its best target, which gathers the surplus (times n, exact) and
+        // counts its sources.
+#pragma unroll
+        for (int j = 0; j < kClassesPerThread; ++j) {
+          const int src = tid * kClassesPerThread + j;
+          if (src >= C) continue;
           const unsigned long long k64 = s.best_key[src];
           if (k64 == 0ull) continue;
           long long surplus_n;
@@ -806,101 +937,82 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         }
         __syncthreads();
         // Targets with several sources win over single-source ones
-        // (core.cpp:1188-1202); one is drawn in ascending label order with weights
-        // max(surplus, DBL_MIN), through a block-wide prefix sum.
-        int local_multi = 0;
-        for (int e = tid; e < present; e += THREADS) local_multi |= s.nsrc[s.list[e]] > 1 ? 1 : 0;
-        const bool multi = __syncthreads_or(local_multi) != 0;
-        const int per = (present + THREADS - 1) / THREADS;
-        const int b0 = min(present, static_cast<int>(tid) * per), b1 = min(present, b0 + per);
-        double part = 0.0;
-        for (int e = b0; e < b1; ++e) {
-          const int t = s.list[e];
-          const double w =
-            s.nsrc[t] > 0 && (!multi || s.nsrc[t] > 1) ? fmax(static_cast<double>(s.weight[t]) / nn, DBL_MIN) : 0.0;
-          s.wl[e] = w;
-          part += w;
-        }
-        double incl = part;
+        // (core.cpp:1188-1202); one is drawn in ascending label order with
+        // probability proportional to its surplus. Wave 0 alone: lane l holds
+        // the `per` consecutive classes from l * per (all of its loads in
+        // flight at once, clamped to a valid class and masked), a wave prefix
+        // sum places the lanes' totals, and the lane whose range holds the
+        // draw finds the target. The weights are integers, so every sum is
+        // exact.
+        if (wave == 0) {
+          constexpr int kPerLane = 1024 / kWave;
+          const int ln = tid & (kWave - 1);  // from opaque_tid: addresses not hoisted out of the cycle loop
+          const int per = (C + kWave - 1) / kWave;
+          const int c0 = ln * per;
+          // Bit j of `one` / `several`: class c0 + j has one source / more.
+          unsigned one = 0u, several = 0u;
 #pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const double y = __shfl_up(incl, off);
-          if (lane >= off) incl += y;
-        }
-        if (lane == kWave - 1) s.wave_total[wave] = incl;
-        __syncthreads();
-        double before = 0.0, total = 0.0;
-        for (int w = 0; w < THREADS / kWave; ++w) {
-          if (w < wave) before += s.wave_total[w];
-          total += s.wave_total[w];
-        }
-        if (total > 0.0) {
-          const double u = unit(draw(key, cycle, 3)) * total;
-          double acc = before + incl - part;
-          for (int e = b0; e < b1; ++e) {
-            const double w = s.wl[e];
-            // Neighbouring threads' boundaries can differ by a rounding step; the
-            // smallest matching entry wins, so the pick is deterministic.
-            if (w > 0.0 && u >= acc && u < acc + w) atomicMin(&s.scal[kPick], e);
-            acc += w;
+          for (int j = 0; j < kPerLane; ++j) {
+            const int m = s.nsrc[min(c0 + j, C - 1)];
+            const bool in = j < per && c0 + j < C;
+            one |= (in && m == 1 ? 1u : 0u) << j;
+            several |= (in && m > 1 ? 1u : 0u) << j;
           }
-          __syncthreads();
-          if (tid == 0) {
-            int e = s.scal[kPick];
-            if (e == INT_MAX) {
-              // u at or past the last boundary: the last eligible target, as the
-              // sequential scan would take.
-              for (e = present - 1; e > 0 && s.wl[e] <= 0.0; --e) {
+          const unsigned drawn = __ballot(several != 0u) != 0ull ? several : one | several;
+          unsigned long long part = 0ull;
+#pragma unroll
+          for (int j = 0; j < kPerLane; ++j) {
+            const unsigned long long w = s.weight[min(c0 + j, C - 1)];
+            part += (drawn >> j) & 1u ? w : 0ull;
+          }
+          unsigned long long incl = part;
+#pragma unroll
+          for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned long long y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+          }
+          const unsigned long long total = __shfl(incl, kWave - 1);
+          if (total == 0ull) {
+            if (lane == 0) s.scal[kChosen] = -1;
+          } else {
+            // floor(u * total) for the 64-bit uniform u / 2^64: below total.
+            const unsigned long long r = __umul64hi(draw(key, cycle, 3), total);
+            unsigned long long acc = incl - part;
+            if (r >= acc && r < incl) {
+              int pick = -1;
+#pragma unroll
+              for (int j = 0; j < kPerLane; ++j) {
+                const unsigned long long w = (drawn >> j) & 1u ? s.weight[min(c0 + j, C - 1)] : 0ull;
+                if (pick < 0 && r < acc + w) pick = c0 + j;
+                acc += w;
               }
+              s.scal[kChosen] = pick;
             }
-            s.scal[kChosen] = s.list[e];
           }
         }
         __syncthreads();
-        const int tgt = s.scal[kChosen];
-        if (tgt >= 0) {
-          int v = 0;
-          for (int e = tid; e < present; e += THREADS) v += s.point[s.list[e]] == tgt ? 1 : 0;
-          if (v) atomicAdd(&s.scal[kVanished], v);
-        }
-        __syncthreads();
-        // Every row of a batch run is movable, so each source vanishes; the result
-        // must keep two classes or more and lose at least one (core.cpp:1219-1223).
-        const int vanished = s.scal[kVanished];
-        if (tgt >= 0 && vanished > 0 && present - vanished > 1) {
-          for (int e = tid; e < present; e += THREADS) {
-            const int c = s.list[e];
-            if (s.point[c] == tgt) s.remap[c] = tgt;
-          }
-          __syncthreads();
-          for (int i = tid; i < n; i += THREADS) {
-            const int r = s.remap[prop[i]];
-            if (r >= 0) prop[i] = static_cast<unsigned short>(r);
-          }
+        tgt = s.scal[kChosen];
+      }
+      // Every row of a batch run is movable, so each source of the chosen
+      // target vanishes (nsrc of them); the result must keep two classes or
+      // more and lose at least one (core.cpp:1219-1223).
+      const int vanished = tgt >= 0 ? s.nsrc[tgt] : 0;
+      const bool absorb = vanished > 0 && present - vanished > 1;
+      if (absorb) {
+        for (int i = tid; i < n; i += THREADS) {
+          if (s.point[prop[i]] == tgt) prop[i] = static_cast<unsigned short>(tgt);
         }
       }
-      // Class counts of the proposal after absorption, from the sizes; then
-      // reset this cycle's class tables (present classes only).
-      for (int e = tid; e < present; e += THREADS) {
-        const int c = s.list[e];
-        const int r = s.remap[c];
-        atomicAdd(&s.counts[r >= 0 ? r : c], s.sizes[c]);
+      // Class counts of the proposal after absorption, from the sizes. The
+      // class tables are cleared after the acceptance test.
+#pragma unroll
+      for (int j = 0; j < kClassesPerThread; ++j) {
+        const int c = tid * kClassesPerThread + j;
+        const int v = c < C ? s.sizes[c] : 0;
+        if (v > 0) atomicAdd(&s.counts[absorb && s.point[c] == tgt ? tgt : c], v);
       }
-      __syncthreads();
-      for (int e = tid; e < present; e += THREADS) {
-        const int c = s.list[e];
-        s.weight[c] = 0ull;
-        s.best_key[c] = 0ull;
-        s.sizes[c] = 0;
-        s.stay[c] = 0;
-        s.nsrc[c] = 0;
-        s.point[c] = -1;
-        s.remap[c] = -1;
-      }
-      if (tid == 0) s.scal[kPairs] = 0;
       __syncthreads();
     }
-
     // -- vote -------------------------------------------------------------------
     vote_phase<KMAX, THREADS>(tid, k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate && group == 1,
                               slice_lo >> 1, (slice_hi + 1) >> 1);
@@ -1009,13 +1121,23 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         p.bestpred[base + i] = prop_pred[i];
       }
     }
-    for (int c = tid; c < C; c += THREADS) s.counts[c] = 0;
+    for (int c = tid; c < C; c += THREADS) {
+      s.counts[c] = 0;
+      s.sizes[c] = 0;
+      s.stay[c] = 0;
+      s.nsrc[c] = 0;
+      s.point[c] = -1;
+      s.weight[c] = 0ull;
+      s.best_key[c] = 0ull;
+    }
     __syncthreads();
     if (tid == 0) {
       if (dec & kEvoAccepted) s.scal[kCur] = cur ^ 1;
       s.scal[kMatches] = 0;
       s.scal[kQueue] = 0;
       s.scal[kDecision] = 0;
+      s.scal[kPairs] = 0;
+      s.scal[kPresent] = 0;
     }
     __syncthreads();
   }
@@ -1050,12 +1172,18 @@ __global__ void evo_zero_kernel(int* a, size_t n) {
 
 // ---- launchers --------------------------------------------------------------
 
-size_t evo_lds_bytes(int classes, int max_rows, bool learned, bool large) {
+int evo_hash_bits(int max_rows) {
+  int bits = 1;
+  while ((1 << bits) <= max_rows) ++bits;
+  return bits;
+}
+
+size_t evo_lds_bytes(int classes, int max_rows, bool learned, bool large, int hash_bits) {
   const size_t rows_lds = large ? 0 : 2 * static_cast<size_t>(max_rows);
-  const size_t base = sizeof(double) * (static_cast<size_t>(classes) + kWaves + 4) +
-                      sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
-                      sizeof(int) * (7 * static_cast<size_t>(classes) + rows_lds + 16) +
-                      sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows));
+  const size_t hash_lds = !large && hash_bits > 0 ? 2 * sizeof(int) * (static_cast<size_t>(1) << hash_bits) : 0;
+  const size_t base = sizeof(double) * (kWaves + 4) + sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
+                      sizeof(int) * (5 * static_cast<size_t>(classes) + rows_lds + 16) +
+                      sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows)) + hash_lds;
   if (!learned) return base;
   return base + 16 + sizeof(uint32_t) * kMlpTiled +
          sizeof(unsigned short) * (kLearnedWaves * mlp::kTileRows * (kWave + 32) + 66);
@@ -1112,8 +1240,8 @@ static int resident_per_cu(int k, size_t shmem) {
   return resident_per_cu_k<64, LEARNED, LARGE>(shmem);
 }
 
-int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large) {
-  const size_t shmem = evo_lds_bytes(classes, max_rows, learned, large);
+int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large, int hash_bits) {
+  const size_t shmem = evo_lds_bytes(classes, max_rows, learned, large, hash_bits);
   if (large) return learned ? resident_per_cu<true, true>(k, shmem) : resident_per_cu<false, true>(k, shmem);
   return learned ? resident_per_cu<true, false>(k, shmem) : resident_per_cu<false, false>(k, shmem);
 }
@@ -1121,8 +1249,10 @@ int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool lar
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s) {
   const bool learned = p.learn_packed != nullptr;
   const bool large = p.large_scratch != nullptr;
-  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned, large);
+  const size_t shmem = evo_lds_bytes(p.classes, p.max_rows, learned, large, p.hash_bits);
   if (shmem > kEvoMaxLds || p.max_rows > (large ? kEvoMaxRowsLarge : kEvoMaxRows) || p.classes > 1024 ||
+      p.hash_bits < 0 || (p.hash_bits > 0 && (large || (1 << p.hash_bits) <= p.max_rows || p.hash_bits > 15)) ||
+      (p.hash_bits == 0 && p.flows == nullptr) ||
       p.k < 1 || p.k > 64 || p.pad_total % 256 != 0 || static_cast<long long>(p.k) * p.pad_total >= (1ll << 31) ||
       (p.wg_run != nullptr && (p.wg_member == nullptr || p.run_group == nullptr || p.xbuf == nullptr ||
                                p.xmatch == nullptr || p.error == nullptr || p.workgroups < p.runs ||

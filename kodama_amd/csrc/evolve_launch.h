@@ -44,6 +44,11 @@ struct EvoParams {
   unsigned short* bestpred = nullptr;
   double* state = nullptr;                   // [runs x kEvoStateStride]
   int* flows = nullptr;                      // [workgroups x classes x classes], zero between cycles
+  // LDS flow table instead of `flows` (0: `flows`): 2^hash_bits slots, more
+  // than max_rows, so a cycle's distinct flow entries (one per moved row at
+  // most) never fill it; each slot holds an entry and its count, both cleared
+  // when the absorption reads them. LDS-resident layout only.
+  int hash_bits = 0;
   double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
   // Learned proposal (null: every cycle uniform): the network in the tiled
   // layout of launch_mlp_pack, and the probability that a cycle falls back to
@@ -77,12 +82,16 @@ struct EvoParams {
 };
 constexpr int kEvoMaxGroup = 4;  // workgroups per run at most
 
-// LDS of one run-workgroup; `large` leaves the row queue and the flow list out.
-size_t evo_lds_bytes(int classes, int max_rows, bool learned = false, bool large = false);
+// LDS of one run-workgroup; `large` leaves the row queue and the flow list
+// out; hash_bits > 0 adds the LDS flow table of 2^hash_bits slots.
+size_t evo_lds_bytes(int classes, int max_rows, bool learned = false, bool large = false, int hash_bits = 0);
+// Bits of the LDS flow table for runs of up to max_rows rows: the smallest
+// table with more slots than max_rows.
+int evo_hash_bits(int max_rows);
 hipError_t launch_evo_persistent(const EvoParams& p, hipStream_t s);
 // Workgroups of the persistent kernel's variant for these sizes that one CU
 // holds at once (0 when the query fails).
-int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large);
+int evo_resident_per_cu(int k, int classes, int max_rows, bool learned, bool large, int hash_bits);
 // 32-bit padded neighbour ids of the vote layout -> 16-bit run-local ids; a
 // missing neighbour (-1) becomes the run's row count, its sentinel slot.
 hipError_t launch_evo_narrow(const int* nbr32, const int* run_offsets, const int* pad_offsets, const int* block_run,
