@@ -91,6 +91,42 @@ __device__ __forceinline__ uint32_t round_fn(uint32_t r, uint32_t key) {
   return x;
 }
 
+// Inclusive prefix sums over the 64 lanes of a wave in DPP moves: shifts by
+// 1, 2, 4, 8 inside each row of 16 lanes (lanes shifted in from outside the
+// row read 0), then lane 15 of each row added to the next row and lane 31 to
+// the upper two rows.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+  return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROWS, 0xF, true));
+}
+__device__ __forceinline__ unsigned wave_scan_u32(unsigned v) {
+  v += dpp_u32<0x111, 0xF>(v);
+  v += dpp_u32<0x112, 0xF>(v);
+  v += dpp_u32<0x114, 0xF>(v);
+  v += dpp_u32<0x118, 0xF>(v);
+  v += dpp_u32<0x142, 0xA>(v);
+  v += dpp_u32<0x143, 0xC>(v);
+  return v;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  return static_cast<unsigned long long>(dpp_u32<CTRL, ROWS>(static_cast<unsigned>(v >> 32))) << 32 |
+         dpp_u32<CTRL, ROWS>(static_cast<unsigned>(v));
+}
+__device__ __forceinline__ unsigned long long wave_scan_u64(unsigned long long v) {
+  v += dpp_u64<0x111, 0xF>(v);
+  v += dpp_u64<0x112, 0xF>(v);
+  v += dpp_u64<0x114, 0xF>(v);
+  v += dpp_u64<0x118, 0xF>(v);
+  v += dpp_u64<0x142, 0xA>(v);
+  v += dpp_u64<0x143, 0xC>(v);
+  return v;
+}
+// Wave total of a 32-bit value (the last lane of the inclusive sums).
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+  return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(wave_scan_u32(v)), 63));
+}
+
 // Keyed pseudo-random permutation of [0, n): a four-round unbalanced Feistel
 // network on `bits` = ceil(log2 n) bits (halves of floor and ceil width; each
 // round swaps their widths, four rounds restore them), cycle-walked into
@@ -194,7 +230,12 @@ struct Lds {
 
 constexpr unsigned kEmptySlot = 0xFFFFFFFFu;  // above every flow entry (< 1024^2)
 
-enum Scalar { kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands };
+// kSumSq: sum of the proposal's squared class sizes; kMulti: some target has
+// two sources or more; kVanished: classes absorbed into kChosen (0: none);
+// kMoved / kMovedSq: rows and sum of squared sizes of those classes.
+enum Scalar {
+  kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands, kSumSq, kMulti, kVanished, kMoved, kMovedSq
+};
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
 
@@ -521,24 +562,39 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   __syncthreads();
 }
 
-// Objective with the diversity guard from the class counts (wave 0).
-__device__ __forceinline__ double objective(const Lds& s, bool guarded, int C, int n, double* acc_out) {
+// Objective with the diversity guard (wave 0), from the number of classes
+// and the sum of their squared counts in integers (at most n^2 < 2^28),
+// divided by n^2 once. The evaluation of the starting labels counts them in
+// s.counts; a proposal's come from its class sizes and the absorbed classes:
+// absorbing classes of m rows and squared sizes q into a target of t rows
+// adds (t + m)^2 - t^2 - q.
+__device__ __forceinline__ double objective(const Lds& s, bool guarded, bool evaluate, int C, int n,
+                                            double* acc_out) {
   const int lane = threadIdx.x & (kWave - 1);
-  // Sum of squared class counts in integers (at most n^2 < 2^28), divided
-  // by n^2 once.
   int k = 0;
   unsigned sq = 0u;
-  for (int c = lane; c < C; c += kWave) {
-    const int v = s.counts[c];
-    if (v > 0) {
-      ++k;
-      sq += static_cast<unsigned>(v) * static_cast<unsigned>(v);
+  if (evaluate) {
+    for (int c = lane; c < C; c += kWave) {
+      const int v = s.counts[c];
+      if (v > 0) {
+        ++k;
+        sq += static_cast<unsigned>(v) * static_cast<unsigned>(v);
+      }
     }
-  }
 #pragma unroll
-  for (int m = kWave / 2; m >= 1; m >>= 1) {
-    k += __shfl_xor(k, m);
-    sq += __shfl_xor(sq, m);
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+      k += __shfl_xor(k, m);
+      sq += __shfl_xor(sq, m);
+    }
+  } else {
+    const int vanished = s.scal[kVanished];
+    k = s.scal[kPresent] - vanished;
+    sq = static_cast<unsigned>(s.scal[kSumSq]);
+    if (vanished > 0) {
+      const unsigned t = static_cast<unsigned>(s.sizes[s.scal[kChosen]]);
+      const unsigned m = static_cast<unsigned>(s.scal[kMoved]);
+      sq = sq - static_cast<unsigned>(s.scal[kMovedSq]) + 2u * t * m + m * m;
+    }
   }
   const double nn = static_cast<double>(n);
   const double acc = n ? static_cast<double>(s.scal[kMatches]) / nn : 0.0;
@@ -835,13 +891,22 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       // in workgroups of fewer than 1024 threads), so no list of the present
       // classes is built: the present count comes from one ballot per wave.
       {
+        // The present count and the sum of squared class sizes (for the
+        // objective: at most n^2 < 2^28).
         int here = 0;
+        unsigned sq = 0u;
 #pragma unroll
         for (int j = 0; j < kClassesPerThread; ++j) {
           const int c = tid * kClassesPerThread + j;
-          here += __popcll(__ballot(c < C && s.sizes[c] > 0));
+          const int v = c < C ? s.sizes[c] : 0;
+          here += __popcll(__ballot(v > 0));
+          sq += static_cast<unsigned>(v) * static_cast<unsigned>(v);
         }
-        if (lane == 0 && here != 0) atomicAdd(&s.scal[kPresent], here);
+        sq = wave_sum_u32(sq);
+        if (lane == 0 && here != 0) {
+          atomicAdd(&s.scal[kPresent], here);
+          atomicAdd(reinterpret_cast<unsigned*>(&s.scal[kSumSq]), sq);
+        }
         // Best target per source: atomicMax of a key ordered as the reference's
         // candidate comparison -- surplus (exactly, as cnt*n - size_src*size_dst),
         // then flow, then target size, then the smaller label. Keys are built
@@ -932,59 +997,52 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           if (surplus_n > 0 && cnt > s.stay[src]) {
             s.point[src] = d;
             atomicAdd(&s.weight[d], static_cast<unsigned long long>(surplus_n));
-            atomicAdd(&s.nsrc[d], 1);
+            if (atomicAdd(&s.nsrc[d], 1) == 1) s.scal[kMulti] = 1;  // a second source
           }
         }
         __syncthreads();
         // Targets with several sources win over single-source ones
         // (core.cpp:1188-1202); one is drawn in ascending label order with
-        // probability proportional to its surplus. Wave 0 alone: lane l holds
-        // the `per` consecutive classes from l * per (all of its loads in
-        // flight at once, clamped to a valid class and masked), a wave prefix
-        // sum places the lanes' totals, and the lane whose range holds the
-        // draw finds the target. The weights are integers, so every sum is
-        // exact.
-        if (wave == 0) {
-          constexpr int kPerLane = 1024 / kWave;
-          const int ln = tid & (kWave - 1);  // from opaque_tid: addresses not hoisted out of the cycle loop
-          const int per = (C + kWave - 1) / kWave;
-          const int c0 = ln * per;
-          // Bit j of `one` / `several`: class c0 + j has one source / more.
-          unsigned one = 0u, several = 0u;
-#pragma unroll
-          for (int j = 0; j < kPerLane; ++j) {
-            const int m = s.nsrc[min(c0 + j, C - 1)];
-            const bool in = j < per && c0 + j < C;
-            one |= (in && m == 1 ? 1u : 0u) << j;
-            several |= (in && m > 1 ? 1u : 0u) << j;
-          }
-          const unsigned drawn = __ballot(several != 0u) != 0ull ? several : one | several;
+        // probability proportional to its surplus. Each thread holds
+        // kClassesPerThread consecutive classes; a wave prefix sum (DPP) and
+        // the wave totals place them, and the thread whose range holds the
+        // draw names the target. The weights are integers, so every sum is
+        // exact and the target does not depend on the summation order.
+        {
+          const bool multi = s.scal[kMulti] != 0;
+          unsigned long long w[kClassesPerThread];
           unsigned long long part = 0ull;
 #pragma unroll
-          for (int j = 0; j < kPerLane; ++j) {
-            const unsigned long long w = s.weight[min(c0 + j, C - 1)];
-            part += (drawn >> j) & 1u ? w : 0ull;
+          for (int j = 0; j < kClassesPerThread; ++j) {
+            const int c = tid * kClassesPerThread + j;
+            const int m = c < C ? s.nsrc[c] : 0;
+            w[j] = m > 0 && (!multi || m > 1) ? s.weight[c] : 0ull;
+            part += w[j];
           }
-          unsigned long long incl = part;
+          const unsigned long long incl = wave_scan_u64(part);
+          unsigned long long* wave_sum = reinterpret_cast<unsigned long long*>(s.wave_total);
+          const int wv = tid / kWave;  // from opaque_tid: the address is not kept across the cycle loop
+          if ((tid & (kWave - 1)) == kWave - 1) wave_sum[wv] = incl;
+          __syncthreads();
+          unsigned long long before = 0ull, total = 0ull;
 #pragma unroll
-          for (int off = 1; off < kWave; off <<= 1) {
-            const unsigned long long y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
+          for (int v = 0; v < THREADS / kWave; ++v) {
+            const unsigned long long t = wave_sum[v];
+            before += v < wv ? t : 0ull;
+            total += t;
           }
-          const unsigned long long total = __shfl(incl, kWave - 1);
           if (total == 0ull) {
-            if (lane == 0) s.scal[kChosen] = -1;
+            if (tid == 0) s.scal[kChosen] = -1;
           } else {
             // floor(u * total) for the 64-bit uniform u / 2^64: below total.
             const unsigned long long r = __umul64hi(draw(key, cycle, 3), total);
-            unsigned long long acc = incl - part;
-            if (r >= acc && r < incl) {
+            unsigned long long acc = before + incl - part;
+            if (r >= acc && r < before + incl) {
               int pick = -1;
 #pragma unroll
-              for (int j = 0; j < kPerLane; ++j) {
-                const unsigned long long w = (drawn >> j) & 1u ? s.weight[min(c0 + j, C - 1)] : 0ull;
-                if (pick < 0 && r < acc + w) pick = c0 + j;
-                acc += w;
+              for (int j = 0; j < kClassesPerThread; ++j) {
+                if (pick < 0 && r < acc + w[j]) pick = tid * kClassesPerThread + j;
+                acc += w[j];
               }
               s.scal[kChosen] = pick;
             }
@@ -1002,14 +1060,22 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         for (int i = tid; i < n; i += THREADS) {
           if (s.point[prop[i]] == tgt) prop[i] = static_cast<unsigned short>(tgt);
         }
-      }
-      // Class counts of the proposal after absorption, from the sizes. The
-      // class tables are cleared after the acceptance test.
+        // Rows and squared sizes of the absorbed classes, for the objective.
+        unsigned moved = 0u, moved_sq = 0u;
 #pragma unroll
-      for (int j = 0; j < kClassesPerThread; ++j) {
-        const int c = tid * kClassesPerThread + j;
-        const int v = c < C ? s.sizes[c] : 0;
-        if (v > 0) atomicAdd(&s.counts[absorb && s.point[c] == tgt ? tgt : c], v);
+        for (int j = 0; j < kClassesPerThread; ++j) {
+          const int c = tid * kClassesPerThread + j;
+          const unsigned v = c < C && s.point[c] == tgt ? static_cast<unsigned>(s.sizes[c]) : 0u;
+          moved += v;
+          moved_sq += v * v;
+        }
+        moved = wave_sum_u32(moved);
+        moved_sq = wave_sum_u32(moved_sq);
+        if (lane == 0) {
+          atomicAdd(reinterpret_cast<unsigned*>(&s.scal[kMoved]), moved);
+          atomicAdd(reinterpret_cast<unsigned*>(&s.scal[kMovedSq]), moved_sq);
+        }
+        if (tid == 0) s.scal[kVanished] = vanished;
       }
       __syncthreads();
     }
@@ -1081,7 +1147,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     // -- accept -----------------------------------------------------------------
     if (wave == 0) {
       double acc;
-      const double score = objective(s, guarded, C, n, &acc);
+      const double score = objective(s, guarded, evaluate, C, n, &acc);
       if (lane == 0) {
         int dec = 0;
         if (evaluate) {
@@ -1138,6 +1204,11 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       s.scal[kDecision] = 0;
       s.scal[kPairs] = 0;
       s.scal[kPresent] = 0;
+      s.scal[kSumSq] = 0;
+      s.scal[kMulti] = 0;
+      s.scal[kVanished] = 0;
+      s.scal[kMoved] = 0;
+      s.scal[kMovedSq] = 0;
     }
     __syncthreads();
   }
