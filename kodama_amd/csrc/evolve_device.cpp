@@ -162,7 +162,9 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   // Split runs: workgroup tables and the exchange region.
   const size_t off_wg = split ? lay.take(sizeof(int) * (2 * static_cast<size_t>(m.workgroups) + m.runs)) : 0;
   if (split) {
-    m.exchange_words = static_cast<size_t>(pad) + static_cast<size_t>(m.runs) * kEvoMaxGroup + 1;
+    // Two halves (alternate steps) of the predictions and match counts, then
+    // the error word.
+    m.exchange_words = 2 * (static_cast<size_t>(pad) + static_cast<size_t>(m.runs) * kEvoMaxGroup) + 1;
     m.off_exchange = lay.take(sizeof(unsigned) * m.exchange_words);
     m.off_error = m.off_exchange + sizeof(unsigned) * (m.exchange_words - 1);
   }
@@ -228,7 +230,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     p.wg_member = wg + m.workgroups;
     p.run_group = wg + 2 * m.workgroups;
     p.xbuf = reinterpret_cast<unsigned*>(base + m.off_exchange);
-    p.xmatch = p.xbuf + pad;
+    p.xmatch = p.xbuf + 2 * static_cast<size_t>(pad);
     p.error = reinterpret_cast<int*>(base + m.off_error);
     p.spin_limit = kSpinLimit;
     p.workgroups = m.workgroups;

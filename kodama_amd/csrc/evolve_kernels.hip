@@ -76,6 +76,13 @@ __device__ __forceinline__ int opaque_tid() {
   asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(static_cast<int>(threadIdx.x)));
   return t;
 }
+// The same for a wave-uniform value: values computed from it (a conversion to
+// double, say) are recomputed each cycle instead of held across the loop.
+__device__ __forceinline__ int opaque_uniform(int v) {
+  int t;
+  asm volatile("s_mov_b32 %0, %1" : "=s"(t) : "s"(v));
+  return t;
+}
 
 // exp of the acceptance test, out of line: one lane calls it once per cycle,
 // and inlined its polynomial constants would be hoisted into registers.
@@ -233,8 +240,12 @@ constexpr unsigned kEmptySlot = 0xFFFFFFFFu;  // above every flow entry (< 1024^
 // kSumSq: sum of the proposal's squared class sizes; kMulti: some target has
 // two sources or more; kVanished: classes absorbed into kChosen (0: none);
 // kMoved / kMovedSq: rows and sum of squared sizes of those classes.
+// kPending (split runs): bit 0, the other slices' predictions of the current
+// state are still to be read (from exchange half kPendingPar, epoch
+// kPendingEp); bit 1, the current state is a new best to be written.
 enum Scalar {
-  kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands, kSumSq, kMulti, kVanished, kMoved, kMovedSq
+  kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands, kSumSq, kMulti, kVanished, kMoved, kMovedSq,
+  kPending, kPendingEp, kPendingPar
 };
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
@@ -568,9 +579,9 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
 // s.counts; a proposal's come from its class sizes and the absorbed classes:
 // absorbing classes of m rows and squared sizes q into a target of t rows
 // adds (t + m)^2 - t^2 - q.
-__device__ __forceinline__ double objective(const Lds& s, bool guarded, bool evaluate, int C, int n,
+__device__ __forceinline__ double objective(const Lds& s, bool guarded, bool evaluate, int C, int n, int tid,
                                             double* acc_out) {
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = tid & (kWave - 1);  // from opaque_tid: the count addresses are not kept across the cycle loop
   int k = 0;
   unsigned sq = 0u;
   if (evaluate) {
@@ -676,6 +687,76 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   }
   __syncthreads();
 
+  // Split runs (see EvoParams): a word of the exchange carries its step's
+  // epoch (1..65535, never the zero of a cleared word, different for
+  // consecutive steps); a read waits for the expected epoch. A wait that runs
+  // out of polls (or finds that another one did) flags the error and stops
+  // waiting, so every member finishes.
+  auto await = [&](unsigned* word, int shift, unsigned ep) {
+    unsigned w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int polls = 0;
+    while ((w >> shift) != ep) {
+      if (++polls > p.spin_limit || __hip_atomic_load(p.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        __hip_atomic_store(p.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+      w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return w;
+  };
+  // The other members' predictions of a step (epoch ep, exchange half par)
+  // into `pred`, for this thread's rows i = tid + j * THREADS only (the rows
+  // every per-row loop gives the thread, so no barrier is needed before the
+  // thread reads them). Eight words in flight at once (the first eight may
+  // have been loaded earlier into `first`); a word not yet at the epoch is
+  // then awaited on its own.
+  constexpr int kAhead = 8;
+  auto exchange_half = [&](int par) { return p.xbuf + static_cast<size_t>(par) * p.pad_total + base; };
+  auto load_words = [&](unsigned (&w)[kAhead], const unsigned* xb, int i0) {
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      const int i = min(i0 + u * THREADS, n - 1);
+      w[u] = __hip_atomic_load(&xb[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
+  auto fill_preds = [&](unsigned short* pred, unsigned ep, int par, int tid, unsigned (&first)[kAhead],
+                        bool loaded) {
+    const unsigned* xb = exchange_half(par);
+    for (int i0 = tid; i0 < n; i0 += kAhead * THREADS) {
+      unsigned w[kAhead];
+      if (i0 == tid && loaded) {
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) w[u] = first[u];
+      } else {
+        load_words(w, xb, i0);
+      }
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) {
+        const int i = i0 + u * THREADS;
+        if (i >= n || (i >= slice_lo && i < slice_hi)) continue;
+        if ((w[u] >> 16) != ep) w[u] = await(const_cast<unsigned*>(&xb[i]), 16, ep);
+        pred[i] = static_cast<unsigned short>(w[u] & 0xFFFFu);
+      }
+    }
+  };
+  // A split run's step ends once the members have the match totals: the
+  // predictions of the other slices are read when first needed, by the next
+  // step's relabelling (loaded before its marking, stored after) or before
+  // the write-back, and only when the step's proposal became the current
+  // state (kPending). The best state of an improving step is written then
+  // too.
+  auto settle = [&](int pending, int tid, unsigned (&first)[kAhead], bool loaded) {
+    const int cur = s.scal[kCur];
+    fill_preds(s.pred(cur), static_cast<unsigned>(s.scal[kPendingEp]), s.scal[kPendingPar], tid, first, loaded);
+    if ((pending & 2) && lead) {
+      for (int i = tid; i < n; i += THREADS) {
+        p.best[base + i] = s.lab(cur)[i];
+        p.bestpred[base + i] = s.pred(cur)[i];
+      }
+    }
+  };
+
   // Step -1 evaluates the starting labels in place: their predictions and
   // scores become the current and the best state.
   for (int step = p.init ? -1 : 0; step < p.cycle_count; ++step) {
@@ -691,7 +772,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     if (!evaluate) {
       // -- propose --------------------------------------------------------------
       {
-        const int hi = proposal_hi(n, cycle, p.cycles);
+        const int hi = proposal_hi(opaque_uniform(n), cycle, p.cycles);
         const int take = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
         // A learned cycle (learned variant, unless the fallback draw picks the
         // uniform proposal) first scores the rows whose label differs from
@@ -707,6 +788,11 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // and take their prediction. The proposal buffer holds only label
         // codes otherwise.
         const bool learned = LEARNED && unit(draw(key, cycle, 5)) >= p.learn_fallback;
+        // The learned scoring lists its candidates by their predictions.
+        const int pending = s.scal[kPending];
+        unsigned first[kAhead];
+        if (LEARNED && (pending & 1)) settle(pending, tid, first, false);
+        if (!LEARNED && (pending & 1)) load_words(first, exchange_half(s.scal[kPendingPar]), tid);
         if (learned) {
           const int ln = tid & (kWave - 1);  // from opaque_tid: not hoisted out of the cycle loop
           float* score = reinterpret_cast<float*>(s.queue);
@@ -835,6 +921,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           }
         }
         __syncthreads();
+        if (!LEARNED && (pending & 1)) settle(pending, tid, first, true);
         for (int i = tid; i < n; i += THREADS) {
           const bool chosen = prop[i] == kChosenMark;
           const int guide = cur_pred[i];
@@ -1082,57 +1169,25 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     // -- vote -------------------------------------------------------------------
     vote_phase<KMAX, THREADS>(tid, k, pad_total, s, nbr, scores, prop, prop_pred, n, evaluate && group == 1,
                               slice_lo >> 1, (slice_hi + 1) >> 1);
+    unsigned ep = 0;
+    int par = 0;
     if (group > 1) {
-      // Exchange the slices' predictions and match counts between the run's
-      // members (see EvoParams). Epochs are 1..65535, never the zero of a
-      // cleared word, and differ between consecutive steps.
-      const unsigned ep = 1u + static_cast<unsigned>((step + 1) % 65535);
+      // Publish this slice's predictions and match count in the step's half
+      // of the exchange (halves alternate, so a member a step ahead never
+      // overwrites words another still reads), then gather the match totals.
+      ep = 1u + static_cast<unsigned>((step + 1) % 65535);
+      par = (step + 1) & 1;
+      unsigned* xb = p.xbuf + static_cast<size_t>(par) * pad_total + base;
+      unsigned* xm = p.xmatch + (static_cast<size_t>(par) * p.runs + run) * kEvoMaxGroup;
       for (int i = slice_lo + tid; i < slice_hi; i += THREADS) {
-        __hip_atomic_store(&p.xbuf[base + i], (ep << 16) | prop_pred[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&xb[i], (ep << 16) | prop_pred[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if (tid == 0) {
-        __hip_atomic_store(&p.xmatch[run * kEvoMaxGroup + member], (ep << 14) | static_cast<unsigned>(s.scal[kMatches]),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      // Reads wait for the current epoch; a wait that runs out of polls (or
-      // finds another one did) flags the error and stops waiting, so every
-      // member finishes.
-      auto await = [&](unsigned* word, int shift) {
-        unsigned w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int polls = 0;
-        while ((w >> shift) != ep) {
-          if (++polls > p.spin_limit || __hip_atomic_load(p.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_store(p.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-          w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return w;
-      };
-      // Eight of a thread's words in flight at once; a word not yet at the
-      // current epoch is then awaited on its own.
-      constexpr int kAhead = 8;
-      for (int i0 = tid; i0 < n; i0 += kAhead * THREADS) {
-        unsigned w[kAhead];
-#pragma unroll
-        for (int u = 0; u < kAhead; ++u) {
-          const int i = min(i0 + u * THREADS, n - 1);
-          w[u] = __hip_atomic_load(&p.xbuf[base + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-#pragma unroll
-        for (int u = 0; u < kAhead; ++u) {
-          const int i = i0 + u * THREADS;
-          if (i >= n || (i >= slice_lo && i < slice_hi)) continue;
-          if ((w[u] >> 16) != ep) w[u] = await(&p.xbuf[base + i], 16);
-          prop_pred[i] = static_cast<unsigned short>(w[u] & 0xFFFFu);
-        }
-      }
-      if (tid == 0) {
+        __hip_atomic_store(&xm[member], (ep << 14) | static_cast<unsigned>(s.scal[kMatches]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
         int total = 0;
         for (int m = 0; m < group; ++m) {
-          total += m == member ? s.scal[kMatches]
-                               : static_cast<int>(await(&p.xmatch[run * kEvoMaxGroup + m], 14) & 0x3FFFu);
+          total += m == member ? s.scal[kMatches] : static_cast<int>(await(&xm[m], 14, ep) & 0x3FFFu);
         }
         s.scal[kMatches] = total;
       }
@@ -1147,7 +1202,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     // -- accept -----------------------------------------------------------------
     if (wave == 0) {
       double acc;
-      const double score = objective(s, guarded, evaluate, C, n, &acc);
+      const double score = objective(s, guarded, evaluate, C, n, tid, &acc);
       if (lane == 0) {
         int dec = 0;
         if (evaluate) {
@@ -1177,11 +1232,16 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           if (p.history && lead) p.history[static_cast<size_t>(run) * p.cycles + cycle] = s.state[2];
         }
         s.scal[kDecision] = dec;
+        if (group > 1) {
+          s.scal[kPending] = (evaluate || (dec & kEvoAccepted) ? 1 : 0) | (dec & kEvoImproved ? 2 : 0);
+          s.scal[kPendingEp] = static_cast<int>(ep);
+          s.scal[kPendingPar] = par;
+        }
       }
     }
     __syncthreads();
     const int dec = s.scal[kDecision];
-    if ((dec & kEvoImproved) && lead) {
+    if (group == 1 && (dec & kEvoImproved) && lead) {
       for (int i = tid; i < n; i += THREADS) {
         p.best[base + i] = prop[i];
         p.bestpred[base + i] = prop_pred[i];
@@ -1215,6 +1275,10 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
 
   // Write the run state back for the next launch.
   if (!lead) return;
+  if (s.scal[kPending] & 1) {
+    unsigned first[kAhead];
+    settle(s.scal[kPending], static_cast<int>(threadIdx.x), first, false);
+  }
   const int cur = s.scal[kCur];
   for (int i = threadIdx.x; i < n; i += THREADS) {
     p.cur[base + i] = s.lab(cur)[i];

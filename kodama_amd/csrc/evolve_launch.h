@@ -64,12 +64,18 @@ struct EvoParams {
   // run_group[run] members. Every member replays the same proposal,
   // absorption and acceptance (identical draws and integer decisions) in its
   // own LDS and votes only its slice of rows; slices are exchanged through
-  // xbuf ([pad_total] words, (epoch << 16) | prediction) and the match counts
-  // through xmatch ([runs x kEvoMaxGroup], (epoch << 14) | matches), written
-  // and read with device-scope relaxed atomics; a reader waits (bounded by
-  // spin_limit polls) until a word carries the current epoch. Members need to
-  // be resident together: the launch is cooperative. The flow tables and the
-  // large-run scratch are per workgroup. *error is set when a wait ran out.
+  // xbuf ([2 x pad_total] words, (epoch << 16) | prediction) and the match
+  // counts through xmatch ([2 x runs x kEvoMaxGroup], (epoch << 14) |
+  // matches), written and read with device-scope relaxed atomics; a reader
+  // waits (bounded by spin_limit polls) until a word carries the expected
+  // epoch. Steps alternate between the two halves. The match totals are
+  // gathered before the acceptance test; the other slices' predictions only
+  // when the proposal became the current state, after the next step's
+  // marking. A member can overwrite a half only after every member has
+  // published the next step's matches, which each does after reading the
+  // half. Members need to be resident together: the launch is cooperative.
+  // The flow tables and the large-run scratch are per workgroup. *error is
+  // set when a wait ran out.
   const int* wg_run = nullptr;
   const int* wg_member = nullptr;
   const int* run_group = nullptr;
