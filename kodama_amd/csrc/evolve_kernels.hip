@@ -76,13 +76,6 @@ __device__ __forceinline__ int opaque_tid() {
   asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(static_cast<int>(threadIdx.x)));
   return t;
 }
-// The same for a wave-uniform value: values computed from it (a conversion to
-// double, say) are recomputed each cycle instead of held across the loop.
-__device__ __forceinline__ int opaque_uniform(int v) {
-  int t;
-  asm volatile("s_mov_b32 %0, %1" : "=s"(t) : "s"(v));
-  return t;
-}
 
 // exp of the acceptance test, out of line: one lane calls it once per cycle,
 // and inlined its polynomial constants would be hoisted into registers.
@@ -145,15 +138,16 @@ struct Permutation {
   uint32_t k0, k1, k2, k3;
   int a, b;
 
-  __device__ Permutation(uint64_t key, int bits) {
+  // Round keys from a 64-bit key: its halves, then the halves of mix64(key).
+  __device__ static void round_keys(uint64_t key, uint32_t (&k)[4]) {
     const uint64_t h = mix64(key);
-    k0 = static_cast<uint32_t>(key);
-    k1 = static_cast<uint32_t>(key >> 32);
-    k2 = static_cast<uint32_t>(h);
-    k3 = static_cast<uint32_t>(h >> 32);
-    a = bits >> 1;
-    b = bits - a;
+    k[0] = static_cast<uint32_t>(key);
+    k[1] = static_cast<uint32_t>(key >> 32);
+    k[2] = static_cast<uint32_t>(h);
+    k[3] = static_cast<uint32_t>(h >> 32);
   }
+  __device__ Permutation(uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, int bits)
+      : k0(r0), k1(r1), k2(r2), k3(r3), a(bits >> 1), b(bits - (bits >> 1)) {}
 
   __device__ __forceinline__ uint32_t network(uint32_t x) const {
     int wl = a, wr = b;
@@ -243,10 +237,15 @@ constexpr unsigned kEmptySlot = 0xFFFFFFFFu;  // above every flow entry (< 1024^
 // kPending (split runs): bit 0, the other slices' predictions of the current
 // state are still to be read (from exchange half kPendingPar, epoch
 // kPendingEp); bit 1, the current state is a new best to be written.
+// kTake .. kAbsorbHi: the next cycle's draws, made once per workgroup (see
+// plan_cycle).
 enum Scalar {
   kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands, kSumSq, kMulti, kVanished, kMoved, kMovedSq,
-  kPending, kPendingEp, kPendingPar
+  kPending, kPendingEp, kPendingPar, kTake, kPerm0, kPerm1, kPerm2, kPerm3, kLearnCycle, kRowKeyLo, kRowKeyHi,
+  kAbsorbLo, kAbsorbHi, kScalars
 };
+constexpr int kScalarSlots = 32;
+static_assert(kScalars <= kScalarSlots, "scalar slots");
 
 __host__ __device__ constexpr int label_slots(int max_rows) { return (max_rows + 2) & ~1; }
 
@@ -273,7 +272,7 @@ __device__ __forceinline__ Lds carve(unsigned char* base, int C, int max_rows, i
     s.queue = s.pairs + max_rows;
     s.scal = s.queue + max_rows;
   }
-  s.labels = reinterpret_cast<unsigned short*>(s.scal + 16);
+  s.labels = reinterpret_cast<unsigned short*>(s.scal + kScalarSlots);
   s.lmax = label_slots(max_rows);
   const int slots = hash_bits > 0 ? 1 << hash_bits : 0;
   s.hkey = reinterpret_cast<unsigned*>(s.labels + 4 * s.lmax);
@@ -662,7 +661,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     s.nsrc[c] = 0;
     s.point[c] = -1;
   }
-  if (threadIdx.x < 16) s.scal[threadIdx.x] = 0;
+  if (threadIdx.x < kScalarSlots) s.scal[threadIdx.x] = 0;
   if (hashed) {
     for (int e = threadIdx.x; e < 1 << hash_bits; e += THREADS) {
       s.hkey[e] = kEmptySlot;
@@ -757,6 +756,35 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     }
   };
 
+  // A cycle's draws (proposal size, permutation round keys, the learned
+  // variant's fallback choice and row key, the absorption draw) are made by
+  // one thread of the workgroup and read from LDS: made by every wave, their
+  // 64-bit hashing would queue on the compute unit's one scalar unit. A lane
+  // of the last wave plans the next cycle during the acceptance test (thread
+  // 0 the first one, before the loop); the barriers that follow publish it.
+  auto plan_cycle = [&](int cycle) {
+    const int hi = proposal_hi(n, cycle, p.cycles);
+    s.scal[kTake] = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
+    uint32_t rk[4];
+    Permutation::round_keys(draw(key, cycle, 2), rk);
+    s.scal[kPerm0] = static_cast<int>(rk[0]);
+    s.scal[kPerm1] = static_cast<int>(rk[1]);
+    s.scal[kPerm2] = static_cast<int>(rk[2]);
+    s.scal[kPerm3] = static_cast<int>(rk[3]);
+    s.scal[kLearnCycle] = LEARNED && unit(draw(key, cycle, 5)) >= p.learn_fallback ? 1 : 0;
+    const uint64_t row_key = LEARNED ? draw(key, cycle, 6) : 0ull;
+    s.scal[kRowKeyLo] = static_cast<int>(static_cast<uint32_t>(row_key));
+    s.scal[kRowKeyHi] = static_cast<int>(static_cast<uint32_t>(row_key >> 32));
+    const uint64_t absorb = draw(key, cycle, 3);
+    s.scal[kAbsorbLo] = static_cast<int>(static_cast<uint32_t>(absorb));
+    s.scal[kAbsorbHi] = static_cast<int>(static_cast<uint32_t>(absorb >> 32));
+  };
+  const auto scal_u64 = [&](int lo) {
+    return static_cast<uint64_t>(static_cast<uint32_t>(s.scal[lo + 1])) << 32 | static_cast<uint32_t>(s.scal[lo]);
+  };
+  if (!p.init && threadIdx.x == 0 && p.cycle_count > 0) plan_cycle(p.cycle_begin);
+  __syncthreads();
+
   // Step -1 evaluates the starting labels in place: their predictions and
   // scores become the current and the best state.
   for (int step = p.init ? -1 : 0; step < p.cycle_count; ++step) {
@@ -772,8 +800,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     if (!evaluate) {
       // -- propose --------------------------------------------------------------
       {
-        const int hi = proposal_hi(opaque_uniform(n), cycle, p.cycles);
-        const int take = min(hi, 1 + static_cast<int>(unit(draw(key, cycle, 1)) * hi));
+        const int take = s.scal[kTake];
         // A learned cycle (learned variant, unless the fallback draw picks the
         // uniform proposal) first scores the rows whose label differs from
         // their prediction; the others score 0. The candidates are listed
@@ -787,7 +814,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // the rows at positions [0, take) of a keyed permutation are marked
         // and take their prediction. The proposal buffer holds only label
         // codes otherwise.
-        const bool learned = LEARNED && unit(draw(key, cycle, 5)) >= p.learn_fallback;
+        const bool learned = LEARNED && s.scal[kLearnCycle] != 0;
         // The learned scoring lists its candidates by their predictions.
         const int pending = s.scal[kPending];
         unsigned first[kAhead];
@@ -894,7 +921,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           __syncthreads();
           double total = 0.0;
           for (int w = 0; w < THREADS / kWave; ++w) total += s.wave_total[w];
-          const uint64_t row_key = draw(key, cycle, 6);
+          const uint64_t row_key = scal_u64(kRowKeyLo);
           for (int i = tid; i < n; i += THREADS) {
             const double u = unit(mix64(row_key + static_cast<uint64_t>(i)));
             const bool chosen = score[i] > 0.0f && u * total < static_cast<double>(take) * score[i];
@@ -905,7 +932,8 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           // THREADS, ... one network step per pass, taking its next position
           // as soon as a walk lands below n: the wave's passes follow the
           // longest lane's total walk, not the longest walk of every position.
-          const Permutation perm(draw(key, cycle, 2), bits);
+          const Permutation perm(static_cast<uint32_t>(s.scal[kPerm0]), static_cast<uint32_t>(s.scal[kPerm1]),
+                                 static_cast<uint32_t>(s.scal[kPerm2]), static_cast<uint32_t>(s.scal[kPerm3]), bits);
           uint32_t y = static_cast<uint32_t>(tid), x = y;
           bool live = static_cast<int>(y) < take;
           while (__ballot(live) != 0ull) {
@@ -1122,7 +1150,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             if (tid == 0) s.scal[kChosen] = -1;
           } else {
             // floor(u * total) for the 64-bit uniform u / 2^64: below total.
-            const unsigned long long r = __umul64hi(draw(key, cycle, 3), total);
+            const unsigned long long r = __umul64hi(scal_u64(kAbsorbLo), total);
             unsigned long long acc = before + incl - part;
             if (r >= acc && r < before + incl) {
               int pick = -1;
@@ -1200,6 +1228,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     }
 
     // -- accept -----------------------------------------------------------------
+    // The last wave plans the next cycle meanwhile (this cycle's draws are
+    // all read by now).
+    if (wave == THREADS / kWave - 1 && lane == 0 && step + 1 < p.cycle_count) plan_cycle(cycle + 1);
     if (wave == 0) {
       double acc;
       const double score = objective(s, guarded, evaluate, C, n, tid, &acc);
@@ -1317,7 +1348,7 @@ size_t evo_lds_bytes(int classes, int max_rows, bool learned, bool large, int ha
   const size_t rows_lds = large ? 0 : 2 * static_cast<size_t>(max_rows);
   const size_t hash_lds = !large && hash_bits > 0 ? 2 * sizeof(int) * (static_cast<size_t>(1) << hash_bits) : 0;
   const size_t base = sizeof(double) * (kWaves + 4) + sizeof(unsigned long long) * 2 * static_cast<size_t>(classes) +
-                      sizeof(int) * (5 * static_cast<size_t>(classes) + rows_lds + 16) +
+                      sizeof(int) * (5 * static_cast<size_t>(classes) + rows_lds + kScalarSlots) +
                       sizeof(unsigned short) * 4 * static_cast<size_t>(label_slots(max_rows)) + hash_lds;
   if (!learned) return base;
   return base + 16 + sizeof(uint32_t) * kMlpTiled +
