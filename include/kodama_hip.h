@@ -153,7 +153,7 @@ int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* w
 
 /* Tuning control of the device-resident evolution (no reference counterpart:
  * the reference runs one CUDA stream per run). Runs of at least 1024 rows are
- * split over up to max_group workgroups (0 = automatic, up to 4; 1 = one
+ * split over up to max_group workgroups (0 = automatic, up to 8; 1 = one
  * workgroup per run) while all the workgroups fit the device at once; the
  * members replay the same proposal, absorption and acceptance and each votes
  * a slice of the rows, so the results are identical for every setting.

@@ -626,7 +626,7 @@ def last_fold_table_path():
 
 def set_evolution_split(max_group=0):
     """Process-wide workgroups per run of the device-resident evolution: 0 =
-    automatic (large runs over up to 4 workgroups), 1 = one per run, 2-4 = at
+    automatic (large runs over up to 8 workgroups), 1 = one per run, 2-8 = at
     most that many. Results are identical for every setting."""
     N.call("kodama_hip_set_evolution_split", C.c_int32(int(max_group)))
 

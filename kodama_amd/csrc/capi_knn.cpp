@@ -451,7 +451,7 @@ int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* w
 
 int kodama_hip_set_evolution_split(int32_t max_group) {
   return guarded([&] {
-    require(max_group >= 0 && max_group <= kEvoMaxGroup, "max_group must be 0 (automatic) to 4.");
+    require(max_group >= 0 && max_group <= kEvoMaxGroup, "max_group must be 0 (automatic) to 8.");
     set_evolution_split(max_group);
   });
 }

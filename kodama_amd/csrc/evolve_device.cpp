@@ -20,6 +20,8 @@ std::atomic<int> g_split_max{0};
 std::atomic<int> g_cooperative{1};
 std::atomic<int> g_lds_flows{1};
 constexpr int kMinSliceRows = 512;  // a split run's members vote at least this many rows each
+// Time per row of the replayed phases relative to the vote's.
+constexpr double kReplayShare = 0.33;
 constexpr int kSpinLimit = 1 << 20;
 
 uint64_t splitmix(uint64_t z) {
@@ -107,8 +109,11 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   }
   // Split the largest runs over up to kEvoMaxGroup workgroups while the
   // workgroups stay within what the device holds at once: repeatedly one more
-  // member for the run with the most rows per member, as long as each member
-  // keeps at least kMinSliceRows rows.
+  // member for the run with the longest estimated cycle, as long as each
+  // member keeps at least kMinSliceRows rows. Every member replays the
+  // proposal, absorption and acceptance over all of the run's rows and votes
+  // its slice, so a cycle takes about rows * (kReplayShare + 1 / members)
+  // vote-row times (kReplayShare measured on the 100-run batch, k = 30).
   int device = 0;
   check(hipGetDevice(&device), "hipGetDevice");
   IVec group(static_cast<size_t>(m.runs), 1);
@@ -120,7 +125,10 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
       cus * evo_resident_per_cu(batch.k(), m.classes, batch.max_rows(), m.learned, m.large, m.hash_bits);
     if (capacity >= m.runs) {
       std::priority_queue<std::pair<double, int>> pq;
-      for (int r = 0; r < m.runs; ++r) pq.emplace(static_cast<double>(batch.run_rows(r)), r);
+      const auto cycle_cost = [&](int r) {
+        return static_cast<double>(batch.run_rows(r)) * (kReplayShare + 1.0 / group[r]);
+      };
+      for (int r = 0; r < m.runs; ++r) pq.emplace(cycle_cost(r), r);
       int total = m.runs;
       while (total < capacity && !pq.empty()) {
         const int r = pq.top().second;
@@ -129,7 +137,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
         if (group[r] >= max_group || rows / (group[r] + 1) < kMinSliceRows) continue;
         ++group[r];
         ++total;
-        pq.emplace(static_cast<double>(rows) / group[r], r);
+        pq.emplace(cycle_cost(r), r);
       }
     }
   }

@@ -12,7 +12,7 @@
 // the touched entries are read and reset. Random draws come from a
 // counter-based SplitMix64 hash of (run key, cycle, stream) instead of the
 // host's mt19937_64: the algorithm is the reference's, the draws are not. A
-// large run may be split over up to four workgroups (see EvoParams): each
+// large run may be split over up to eight workgroups (see EvoParams): each
 // replays every phase but votes only its slice of rows, and the slices'
 // predictions and match counts are exchanged after the vote.
 //

@@ -86,7 +86,7 @@ struct EvoParams {
   int workgroups = 0;                        // grid size (runs when not split)
   int cooperative = 1;                       // split runs: cooperative launch (else an ordinary one)
 };
-constexpr int kEvoMaxGroup = 4;  // workgroups per run at most
+constexpr int kEvoMaxGroup = 8;  // workgroups per run at most
 
 // LDS of one run-workgroup; `large` leaves the row queue and the flow list
 // out; hash_bits > 0 adds the LDS flow table of 2^hash_bits slots.

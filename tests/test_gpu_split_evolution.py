@@ -92,7 +92,7 @@ def test_split_large_runs_equal_and_capped():
     o = K.core_options(**dict(OPTS, cycles=5, k=30))
     run = lambda: K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=folds)
     one, split, workgroups = _both(run, len(rows))
-    assert workgroups == 2 * 4
+    assert workgroups == 2 * 8
     _assert_equal(one, split)
     K.set_evolution_split(2)
     capped = run()
@@ -102,4 +102,4 @@ def test_split_large_runs_equal_and_capped():
 
 def test_split_control_rejects_out_of_range():
     with pytest.raises(ValueError):
-        K.set_evolution_split(5)
+        K.set_evolution_split(9)
