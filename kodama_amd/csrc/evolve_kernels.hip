@@ -201,9 +201,9 @@ struct Lds {
   double* state;                // [4] cur_acc, cur_score, best_acc, best_score
   unsigned long long* weight;   // [C] absorption weight per target, times n (exact)
   unsigned long long* best_key; // [C] packed best target per source
-  int* sizes;                   // [C] class sizes of the proposal
-  int* stay;                    // [C] rows whose prediction equals their label, per label
-  int* counts;                  // [C] class counts after absorption (objective)
+  int* sizes;                   // [C] class sizes of the proposal (the current state's before relabelling)
+  int* stay;                    // [C] rows per label whose prediction differs (the outflow)
+  int* counts;                  // [C] class counts of the current state
   int* nsrc;                    // [C] absorbed sources per target
   int* point;                   // [C] target of each source or -1
   int* pairs;                   // [max_rows] distinct flow entries (label * C + prediction)
@@ -783,7 +783,13 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     return static_cast<uint64_t>(static_cast<uint32_t>(s.scal[lo + 1])) << 32 | static_cast<uint32_t>(s.scal[lo]);
   };
   if (!p.init && threadIdx.x == 0 && p.cycle_count > 0) plan_cycle(p.cycle_begin);
+  // The class counts of the current state (a launch that starts with the
+  // evaluation counts them in its vote), then the first proposal's sizes.
+  if (!p.init) {
+    for (int i = threadIdx.x; i < n; i += THREADS) atomicAdd(&s.counts[s.lab(0)[i]], 1);
+  }
   __syncthreads();
+  for (int c = threadIdx.x; c < C; c += THREADS) s.sizes[c] = s.counts[c];
 
   // Step -1 evaluates the starting labels in place: their predictions and
   // scores become the current and the best state.
@@ -950,13 +956,18 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         }
         __syncthreads();
         if (!LEARNED && (pending & 1)) settle(pending, tid, first, true);
+        // The sizes hold the current state's class counts: only a row whose
+        // label changes moves one from its old class to its new one.
         for (int i = tid; i < n; i += THREADS) {
           const bool chosen = prop[i] == kChosenMark;
           const int guide = cur_pred[i];
-          const int label = learned ? static_cast<int>(prop[i]) : (chosen ? guide : static_cast<int>(cur_lab[i]));
+          const int old = cur_lab[i];
+          const int label = learned ? static_cast<int>(prop[i]) : (chosen ? guide : old);
           prop[i] = static_cast<unsigned short>(label);
-          atomicAdd(&s.sizes[label], 1);
-          if (label == guide) atomicAdd(&s.stay[label], 1);
+          if (label != old) {
+            atomicAdd(&s.sizes[label], 1);
+            atomicSub(&s.sizes[old], 1);
+          }
           s.queue[i] = label == guide ? -1 : label * C + guide;
         }
         if (hashed) {
@@ -1045,6 +1056,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           if (LARGE) s.queue[q] = cnt;  // the second pass below needs it again
           const int src = idx / C;
           const int d = idx - src * C;
+          atomicAdd(&s.stay[src], cnt);  // the source's outflow
           const int dn = s.sizes[d];
           if (dn == 0) continue;
           const long long surplus_n = static_cast<long long>(cnt) * n - static_cast<long long>(s.sizes[src]) * dn;
@@ -1109,7 +1121,8 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             cnt = static_cast<int>((k64 >> 23) & 0x1FFFull);
             d = 1023 - static_cast<int>(k64 & 0x3FFull);
           }
-          if (surplus_n > 0 && cnt > s.stay[src]) {
+          // Stayers: the rows whose prediction is their label.
+          if (surplus_n > 0 && cnt > s.sizes[src] - s.stay[src]) {
             s.point[src] = d;
             atomicAdd(&s.weight[d], static_cast<unsigned long long>(surplus_n));
             if (atomicAdd(&s.nsrc[d], 1) == 1) s.scal[kMulti] = 1;  // a second source
@@ -1278,14 +1291,24 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         p.bestpred[base + i] = prop_pred[i];
       }
     }
-    for (int c = tid; c < C; c += THREADS) {
-      s.counts[c] = 0;
-      s.sizes[c] = 0;
-      s.stay[c] = 0;
-      s.nsrc[c] = 0;
-      s.point[c] = -1;
-      s.weight[c] = 0ull;
-      s.best_key[c] = 0ull;
+    // An accepted proposal's class counts become the current state's: its
+    // sizes, with the absorbed classes moved into the target. The next
+    // proposal's sizes start from the current counts.
+    {
+      const bool took = !evaluate && (dec & kEvoAccepted) != 0;
+      const int vanished = s.scal[kVanished], tgt = s.scal[kChosen], moved = s.scal[kMoved];
+      for (int c = tid; c < C; c += THREADS) {
+        if (took) {
+          const bool gone = vanished > 0 && s.point[c] == tgt;
+          s.counts[c] = gone ? 0 : s.sizes[c] + (vanished > 0 && c == tgt ? moved : 0);
+        }
+        s.sizes[c] = s.counts[c];
+        s.stay[c] = 0;
+        s.nsrc[c] = 0;
+        s.point[c] = -1;
+        s.weight[c] = 0ull;
+        s.best_key[c] = 0ull;
+      }
     }
     __syncthreads();
     if (tid == 0) {
