@@ -242,7 +242,7 @@ constexpr unsigned kEmptySlot = 0xFFFFFFFFu;  // above every flow entry (< 1024^
 enum Scalar {
   kPresent, kPairs, kMatches, kChosen, kDecision, kCur, kQueue, kCands, kSumSq, kMulti, kVanished, kMoved, kMovedSq,
   kPending, kPendingEp, kPendingPar, kTake, kPerm0, kPerm1, kPerm2, kPerm3, kLearnCycle, kRowKeyLo, kRowKeyHi,
-  kAbsorbLo, kAbsorbHi, kScalars
+  kAbsorbLo, kAbsorbHi, kAnyTarget, kScalars
 };
 constexpr int kScalarSlots = 32;
 static_assert(kScalars <= kScalarSlots, "scalar slots");
@@ -968,29 +968,30 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             atomicAdd(&s.sizes[label], 1);
             atomicSub(&s.sizes[old], 1);
           }
-          s.queue[i] = label == guide ? -1 : label * C + guide;
-        }
-        if (hashed) {
-          // Flow counts in the LDS table: open addressing with linear
-          // probing from a multiplicative hash of the entry; the row that
-          // claims a free slot lists the slot. The table has more slots than
-          // the run has rows, so a probe always ends.
-          const unsigned mask = (1u << hash_bits) - 1u;
-          for (int i = tid; i < n; i += THREADS) {
-            const int fk = s.queue[i];
-            if (fk < 0) continue;
-            unsigned h = (static_cast<unsigned>(fk) * 0x9E3779B1u) >> (32 - hash_bits);
-            while (true) {
-              const unsigned old = atomicCAS(&s.hkey[h], kEmptySlot, static_cast<unsigned>(fk));
-              if (old == kEmptySlot || old == static_cast<unsigned>(fk)) {
-                atomicAdd(&s.hcnt[h], 1);
-                if (old == kEmptySlot) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = static_cast<int>(h);
-                break;
+          const int fk = label == guide ? -1 : label * C + guide;
+          if (hashed) {
+            // Flow counts in the LDS table, right here: open addressing with
+            // linear probing from a multiplicative hash of the entry; the row
+            // that claims a free slot lists the slot. The table has more
+            // slots than the run has rows, so a probe always ends.
+            if (fk >= 0) {
+              const unsigned mask = (1u << hash_bits) - 1u;
+              unsigned h = (static_cast<unsigned>(fk) * 0x9E3779B1u) >> (32 - hash_bits);
+              while (true) {
+                const unsigned old_key = atomicCAS(&s.hkey[h], kEmptySlot, static_cast<unsigned>(fk));
+                if (old_key == kEmptySlot || old_key == static_cast<unsigned>(fk)) {
+                  atomicAdd(&s.hcnt[h], 1);
+                  if (old_key == kEmptySlot) s.pairs[atomicAdd(&s.scal[kPairs], 1)] = static_cast<int>(h);
+                  break;
+                }
+                h = (h + 1u) & mask;
               }
-              h = (h + 1u) & mask;
             }
+          } else {
+            s.queue[i] = fk;
           }
-        } else {
+        }
+        if (!hashed) {
           // Flow counts in the workgroup's HBM table: all of a thread's
           // returning atomics of a pass are in flight before the first result
           // is used; the first increment of an entry lists it. One pass
@@ -1126,6 +1127,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
             s.point[src] = d;
             atomicAdd(&s.weight[d], static_cast<unsigned long long>(surplus_n));
             if (atomicAdd(&s.nsrc[d], 1) == 1) s.scal[kMulti] = 1;  // a second source
+            s.scal[kAnyTarget] = 1;
           }
         }
         __syncthreads();
@@ -1135,8 +1137,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
         // kClassesPerThread consecutive classes; a wave prefix sum (DPP) and
         // the wave totals place them, and the thread whose range holds the
         // draw names the target. The weights are integers, so every sum is
-        // exact and the target does not depend on the summation order.
-        {
+        // exact and the target does not depend on the summation order. No
+        // target at all: nothing to draw.
+        if (s.scal[kAnyTarget] != 0) {
           const bool multi = s.scal[kMulti] != 0;
           unsigned long long w[kClassesPerThread];
           unsigned long long part = 0ull;
@@ -1175,9 +1178,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
               s.scal[kChosen] = pick;
             }
           }
+          __syncthreads();
+          tgt = s.scal[kChosen];
         }
-        __syncthreads();
-        tgt = s.scal[kChosen];
       }
       // Every row of a batch run is movable, so each source of the chosen
       // target vanishes (nsrc of them); the result must keep two classes or
@@ -1323,6 +1326,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
       s.scal[kVanished] = 0;
       s.scal[kMoved] = 0;
       s.scal[kMovedSq] = 0;
+      s.scal[kAnyTarget] = 0;
     }
     __syncthreads();
   }
