@@ -369,8 +369,8 @@ __device__ __forceinline__ void sort_packed(uint32_t (&v)[M]) {
 // become keys (code << kRankBits | rank) sorted ascending, so each label's
 // votes form one run with ranks ascending inside it; missing neighbours (the
 // sentinel code) sort last. Up to 32 neighbours a key has 5 rank bits and an
-// 11-bit code (labels up to 1023, the sentinel as 2047) and two keys share a
-// register in the sort; at 64 the keys are 32-bit. Returns -1 for a row
+// 11-bit code (labels up to 1023, a missing neighbour as 1024 + rank) and two
+// keys share a register in the sort; at 64 the keys are 32-bit. Returns -1 for a row
 // without neighbours.
 template <int KMAX>
 __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned short* __restrict__ nbr,
@@ -399,7 +399,10 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
   for (int j = 0; j < KMAX; ++j) {
     const uint32_t c = j < k ? key[j] : kSentinel;
     valid += c != kSentinel ? 1 : 0;
-    const uint32_t code = kPacked && c == kSentinel ? 2047u : c;
+    // Packed keys: a missing neighbour gets a code of its own above every
+    // label (1024 + rank), so missing neighbours never form a run longer than
+    // one.
+    const uint32_t code = kPacked && c == kSentinel ? 1024u + static_cast<uint32_t>(j) : c;
     key[j] = (code << kRankBits) | static_cast<uint32_t>(j);
   }
   if constexpr (kPacked) {
@@ -416,20 +419,49 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
   } else {
     sort_network<KMAX>(key);
   }
-  // Longest run of one code among the valid entries, and whether another run
-  // is as long. Selects, not branches: per-entry branches would cost the wave
-  // an exec-mask update each.
+  // Longest run of one code among the valid entries (the smallest such code)
+  // and whether another run is as long, in integer selects: branches per
+  // entry would cost the wave an exec-mask update each.
   int best_count = 0, best_code = -1, run = 0;
   bool tie = false;
+  if constexpr (kPacked) {
+    // Two passes: the longest run, then how many runs reach it and the first
+    // of them. Missing neighbours sort last as runs of one, so they reach the
+    // longest run only when it is 1: then a tie means two present votes.
+    uint32_t prev = 0xFFFFFFFFu;
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    const uint32_t code = key[j] >> kRankBits;
-    run = (j > 0 && code == (key[j - 1] >> kRankBits)) ? run + 1 : 1;
-    const bool in = j < valid;
-    const bool longer = in && run > best_count;
-    tie = !longer && (tie || (in && run == best_count));
-    best_code = longer ? static_cast<int>(code) : best_code;
-    best_count = longer ? run : best_count;
+    for (int j = 0; j < KMAX; ++j) {
+      const uint32_t code = key[j] >> kRankBits;
+      run = code == prev ? run + 1 : 1;
+      prev = code;
+      best_count = max(best_count, run);
+    }
+    int hits = 0;
+    uint32_t first = 0xFFFFFFFFu;
+    prev = 0xFFFFFFFFu;
+    run = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+      const uint32_t code = key[j] >> kRankBits;
+      run = code == prev ? run + 1 : 1;
+      prev = code;
+      const bool hit = run == best_count;
+      hits += hit ? 1 : 0;
+      first = min(first, hit ? code : 0xFFFFFFFFu);
+    }
+    best_code = valid > 0 ? static_cast<int>(first) : -1;
+    tie = best_count == 1 ? valid > 1 : hits > 1;
+  } else {
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+      const uint32_t code = key[j] >> kRankBits;
+      run = (j > 0 && code == (key[j - 1] >> kRankBits)) ? run + 1 : 1;
+      const bool in = j < valid;
+      const bool longer = in && run > best_count;
+      tie = !longer && (tie || (in && run == best_count));
+      best_code = longer ? static_cast<int>(code) : best_code;
+      best_count = longer ? run : best_count;
+    }
   }
   if (!tie) return best_code;
   // The scores are read only here, all of a row's loads in flight before the
