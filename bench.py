@@ -263,16 +263,19 @@ def matrix_bench(rows: int, runs: int, cycles: int, k: int, forms) -> int:
 def tables_bench(runs: int, n: int, k: int) -> int:
     """Per-fold kNN tables of the batched-runs workload built by every schedule
     the tuner chooses from (register scan; split-BF16 MFMA + exact rerank with
-    8 or 16 candidates per lane column and 4 or 8 waves per workgroup), twice
-    each, then the tuned build; one JSON line per build with its wall time, the
-    rows the MFMA path rescanned, and whether the tables equal the scan's."""
+    8 or 16 candidates per lane column and 4 or 8 waves per workgroup, the
+    lists as packed keys or as separate values and indices), twice each, then
+    the tuned build; one JSON line per build with its wall time, the rows the
+    MFMA path rescanned, and whether the tables equal the scan's."""
     import kodama_amd as K
     x, rows, fids, _ = make_workload(runs, n, 50, k)
     ref = None
-    builds = [("scan", 0, 0)] + [("mfma", l, w) for l in (8, 16) if l == 16 or k <= 32 for w in (8, 4)]
+    builds = [("scan", 0, 0, True)] + [("mfma", l, w, pk) for l in (8, 16) if l == 16 or k <= 32 for w in (8, 4)
+                                       for pk in (True, False)]
     try:
-        for path, lst, waves in builds * 2 + [("tuned", 0, 0)]:
+        for path, lst, waves, packed in builds * 2 + [("tuned", 0, 0, True)]:
             K.set_fold_table_schedule("tuned", lst, waves)
+            K.set_mfma_packed_lists(packed)
             t0 = time.perf_counter()
             nbr, score, used = K.knn_fold_tables(x, rows, fids, k, "euclidean", path=path)
             wall = time.perf_counter() - t0
@@ -282,11 +285,12 @@ def tables_bench(runs: int, n: int, k: int) -> int:
             else:
                 same = all(np.array_equal(a, b) for a, b in zip(nbr, ref[0])) and \
                     all(np.array_equal(a, b) for a, b in zip(score, ref[1]))
-            print(json.dumps({"tables": path, "schedule": K.last_fold_table_schedule(), "runs": runs, "n": n,
-                              "k": k, "seconds": wall, "mfma_fallback_rows": K.last_knn_fallbacks(),
-                              "equal_to_scan": same}), flush=True)
+            print(json.dumps({"tables": path, "schedule": K.last_fold_table_schedule(), "packed_lists": packed,
+                              "runs": runs, "n": n, "k": k, "seconds": wall,
+                              "mfma_fallback_rows": K.last_knn_fallbacks(), "equal_to_scan": same}), flush=True)
     finally:
         K.set_fold_table_schedule("tuned", 0, 0)
+        K.set_mfma_packed_lists(True)
     return 0
 
 

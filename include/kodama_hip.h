@@ -150,6 +150,13 @@ int kodama_hip_last_fold_table_path(void);
  * thread's last build (list and waves 0 for the scan). */
 int kodama_hip_set_fold_table_schedule(int32_t path, int32_t list, int32_t waves);
 int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* waves);
+/* Candidate lists of the MFMA kNN path (no reference counterpart), process
+ * wide: on = 1 (the default) keeps each candidate as one 32-bit key, the
+ * value's order bits with the row's index within its lane column in the low
+ * bits, wherever a search's train rows (a batch's largest run) number at most
+ * 32768; on = 0 keeps separate values and indices. Both give the same
+ * neighbours. */
+int kodama_hip_set_mfma_packed_lists(int32_t on);
 
 /* Tuning control of the device-resident evolution (no reference counterpart:
  * the reference runs one CUDA stream per run). Runs of at least 1024 rows are

@@ -612,6 +612,13 @@ def set_fold_table_schedule(path="tuned", list=0, waves=0):
            C.c_int32(int(waves)))
 
 
+def set_mfma_packed_lists(on=True):
+    """Process-wide form of the MFMA kNN candidate lists: packed 32-bit keys
+    (value order bits and column index; the default, where the train rows
+    number at most 32768) or separate values and indices. Same neighbours."""
+    N.call("kodama_hip_set_mfma_packed_lists", C.c_int32(1 if on else 0))
+
+
 def last_fold_table_schedule():
     """Schedule of this thread's last per-fold table build: dict(path, list, waves)."""
     p, l, w = C.c_int32(0), C.c_int32(0), C.c_int32(0)

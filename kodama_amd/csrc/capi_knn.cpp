@@ -439,6 +439,13 @@ int kodama_hip_set_fold_table_schedule(int32_t path, int32_t list, int32_t waves
   });
 }
 
+int kodama_hip_set_mfma_packed_lists(int32_t on) {
+  return guarded([&] {
+    require(on == 0 || on == 1, "on must be 0 or 1.");
+    set_mfma_packed_lists(on);
+  });
+}
+
 int kodama_hip_last_fold_table_schedule(int32_t* path, int32_t* list, int32_t* waves) {
   return guarded([&] {
     require(path != nullptr && list != nullptr && waves != nullptr, "null pointer.");

@@ -45,7 +45,20 @@ struct KnnMfmaParams {
   // 8). Every schedule gives the same neighbours.
   int list = 0;
   int waves = 8;
+  // Candidate lists as packed 32-bit keys (set by the launchers): the
+  // order-preserving key of the candidate value with its low `index_bits`
+  // bits replaced by the train row's index within its lane column (row / 16),
+  // in cand_index; -1 keeps separate values and indices. `runs` is the batch's
+  // run count for the rerank's run lookup.
+  int index_bits = -1;
+  int runs = 0;
 };
+// Largest column index width of the packed lists: the key keeps 21 of the
+// value's 32 order bits, a relative quantisation of at most 2^-12.
+constexpr int kMfmaMaxIndexBits = 11;
+// Packed lists on (1, the default, where the train rows fit) or off (0).
+void set_mfma_packed_lists(int on);
+int mfma_packed_lists();
 inline int mfma_list(const KnnMfmaParams& p) { return p.list != 0 ? p.list : mfma_list_len(p.k); }
 
 hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi, unsigned short* lo, float* norm2,

@@ -31,6 +31,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <atomic>
 #include <cstdint>
 
 #include "knn_mfma.h"
@@ -57,6 +58,33 @@ __device__ __forceinline__ float bf16_value(unsigned short h) {
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Order-preserving map of a float to an unsigned key (finite values).
+__device__ __forceinline__ uint32_t order_key(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Ascending insert into a sorted list of distinct packed keys, branch-free:
+// slot s takes the median of (old s - 1, old s, key), which shifts the tail
+// up by one from the key's position; a key above the last one (or the empty
+// key 0xFFFFFFFF) leaves the list as it is.
+// v_med3_u32 (the compiler forms a min and a max from the same expression).
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+template <int LIST>
+__device__ __forceinline__ void key_insert(uint32_t (&kk)[LIST], uint32_t key) {
+#pragma unroll
+  for (int s = LIST - 1; s > 0; --s) kk[s] = med3_u32(kk[s - 1], kk[s], key);
+  kk[0] = min(kk[0], key);
+}
 
 // Ascending list insert; a value equal to the largest kept one is dropped.
 template <int LIST>
@@ -125,24 +153,42 @@ __device__ __forceinline__ float exact_distance(const float (&q)[64], const floa
 
 }  // namespace
 
-// Rows -> bf16 hi / lo parts [rows x 64] and FP32 squared norms. One thread per row.
+// Rows -> bf16 hi / lo parts [rows x 64] and FP32 squared norms. Eight
+// threads per row (a row's eight are in one wave), eight dimensions each: one
+// 16-byte store per part, so a wave writes eight whole rows. The norm is the
+// sum of the eight partial sums over an xor tree (the same value in all eight
+// lanes); its rounding is within the candidate error bound like any order's.
 __global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, unsigned short* __restrict__ hi,
                                   unsigned short* __restrict__ lo, float* __restrict__ norm2) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const float* src = x + static_cast<size_t>(r) * p;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = e >> 3, part = e & 7;
+  const bool live = r < rows;
   float s = 0.0f;
-  for (int d = 0; d < 64; ++d) {
-    const float v = d < p ? src[d] : 0.0f;
-    const unsigned short h = bf16_bits(v);
-    hi[static_cast<size_t>(r) * 64 + d] = h;
-    lo[static_cast<size_t>(r) * 64 + d] = bf16_bits(v - bf16_value(h));
-    s = fmaf(v, v, s);
+  if (live) {
+    const float* src = x + static_cast<size_t>(r) * p;
+    uint32_t h[4], l[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int d = part * 8 + 2 * t;
+      const float v0 = d < p ? src[d] : 0.0f;
+      const float v1 = d + 1 < p ? src[d + 1] : 0.0f;
+      const unsigned short h0 = bf16_bits(v0), h1 = bf16_bits(v1);
+      h[t] = h0 | static_cast<uint32_t>(h1) << 16;
+      l[t] = bf16_bits(v0 - bf16_value(h0)) | static_cast<uint32_t>(bf16_bits(v1 - bf16_value(h1))) << 16;
+      s = fmaf(v0, v0, s);
+      s = fmaf(v1, v1, s);
+    }
+    const size_t o = static_cast<size_t>(r) * 64 + part * 8;
+    *reinterpret_cast<uint4*>(hi + o) = make_uint4(h[0], h[1], h[2], h[3]);
+    *reinterpret_cast<uint4*>(lo + o) = make_uint4(l[0], l[1], l[2], l[3]);
   }
-  norm2[r] = s;
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 4);
+  if (live && part == 0) norm2[r] = s;
 }
 
-template <bool EUCLID, bool TABLES, int LIST, int WAVES>
+template <bool EUCLID, bool TABLES, int LIST, int WAVES, bool PACKED>
 __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
   constexpr int kCandThreads = WAVES * 64;
   constexpr int kQueriesPerBlock = WAVES * 16;
@@ -194,14 +240,23 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
       excl[r] = (q < q_rows && prm.exclude != nullptr) ? prm.exclude[q] : -1;
     }
   }
-  float kd[4][LIST];
-  int ki[4][LIST];
+  // Unpacked lists: values and indices; packed: one key per entry (the
+  // index arrays are not used).
+  float kd[4][PACKED ? 1 : LIST];
+  int ki[4][PACKED ? 1 : LIST];
+  uint32_t kk[4][PACKED ? LIST : 1];
+  const uint32_t imask = PACKED ? (1u << prm.index_bits) - 1u : 0u;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    if constexpr (PACKED) {
 #pragma unroll
-    for (int s = 0; s < LIST; ++s) {
-      kd[r][s] = INFINITY;
-      ki[r][s] = -1;
+      for (int s = 0; s < LIST; ++s) kk[r][s] = 0xFFFFFFFFu;
+    } else {
+#pragma unroll
+      for (int s = 0; s < LIST; ++s) {
+        kd[r][s] = INFINITY;
+        ki[r][s] = -1;
+      }
     }
   }
 
@@ -248,17 +303,37 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
       for (int tt = 0; tt < 4; ++tt)
         acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a_lo[kb]), as_bf16x8(b_hi[tt]), acc[tt], 0, 0, 0);
     }
+    if constexpr (PACKED) {
+      // Row j = c0 + 16 tt + col is row j / 16 = c0 / 16 + tt of its lane
+      // column, the same for every lane. A candidate that is excluded, past
+      // the rows or not below +inf (the unpacked lists drop those too) gets
+      // the empty key.
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-      const int j = c0 + tt * 16 + col;
-      if (j < t_rows) {
+      for (int tt = 0; tt < 4; ++tt) {
+        const int j = c0 + tt * 16 + col;
+        const uint32_t m = static_cast<uint32_t>((c0 >> 4) + tt);
         const float tn = s_tn[tt * 16 + col];
         const int skip = TABLES ? s_fold[tt * 16 + col] : j;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if (skip != excl[r]) {
-            const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
-            list_insert<LIST>(kd[r], ki[r], v, t_base + j);
+          const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
+          const bool ok = j < t_rows && skip != excl[r] && v < INFINITY;
+          key_insert<LIST>(kk[r], ok ? ((order_key(v) & ~imask) | m) : 0xFFFFFFFFu);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) {
+        const int j = c0 + tt * 16 + col;
+        if (j < t_rows) {
+          const float tn = s_tn[tt * 16 + col];
+          const int skip = TABLES ? s_fold[tt * 16 + col] : j;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (skip != excl[r]) {
+              const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
+              list_insert<LIST>(kd[r], ki[r], v, t_base + j);
+            }
           }
         }
       }
@@ -271,22 +346,19 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     const int q = q_tile + 4 * grp + r;
     if (q < q_rows) {
       const size_t base = (static_cast<size_t>(q_base + q) * 16 + col) * LIST;
+      if constexpr (PACKED) {
+        uint32_t* keys = reinterpret_cast<uint32_t*>(prm.cand_index);
 #pragma unroll
-      for (int s = 0; s < LIST; ++s) {
-        prm.cand_value[base + s] = kd[r][s];
-        prm.cand_index[base + s] = ki[r][s];
+        for (int s = 0; s < LIST; ++s) keys[base + s] = kk[r][s];
+      } else {
+#pragma unroll
+        for (int s = 0; s < LIST; ++s) {
+          prm.cand_value[base + s] = kd[r][s];
+          prm.cand_index[base + s] = ki[r][s];
+        }
       }
     }
   }
-}
-
-// Order-preserving map of a float to an unsigned key (finite values).
-__device__ __forceinline__ uint32_t order_key(float v) {
-  const uint32_t b = __float_as_uint(v);
-  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float key_value(uint32_t k) {
-  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
 // Ascending bitonic sort of 64 R keys over a wave: element e is register
@@ -327,7 +399,16 @@ __device__ __forceinline__ void wave_sort(uint32_t (&v)[R], int lane) {
 // candidate value; ties of value do not move it) comes from a bitonic sort of
 // the values over the wave; the exact re-scores of the candidates within the
 // bound are ranked by (distance, index) among themselves.
-template <bool EUCLID, int LIST>
+//
+// Packed lists: a key stands for a value between key_value(key with its index
+// bits cleared) and key_value(key with them set), and its row is 16 x its
+// index bits + the lane column of its list, from the start of the query's run.
+// The k smallest keys' rows then have exact distances <= hi(k-th key) + E, so
+// every row of the true k nearest has a value <= A = hi(k-th key) + 2E; the
+// candidates re-scored are those with lo(key) <= A, and a full list whose
+// last key has lo <= A may have dropped such a row (the keys of a list are
+// distinct, its rows being).
+template <bool EUCLID, int LIST, bool PACKED>
 __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   constexpr int kCand = 16 * LIST;
   constexpr int R = kCand / kWave;
@@ -336,36 +417,62 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   __shared__ int s_id[4][kCand];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  const int q = blockIdx.x * 4 + wave;
+  // Wave-uniform query (readfirstlane): its row, norm and run come through
+  // scalar loads.
+  const int q = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 + wave);
   const bool live = q < prm.query_rows;  // whole waves; no block barrier below
   if (!live) return;
   const int k = prm.k;
   const int p = prm.dims;
   const size_t base = static_cast<size_t>(q) * kCand;
-  float val[R];
+  const uint32_t* __restrict__ ckey = reinterpret_cast<const uint32_t*>(prm.cand_index);
+  const uint32_t imask = PACKED ? (1u << prm.index_bits) - 1u : 0u;
+  int t_base = 0;  // packed rows count from the start of the query's run
+  if (PACKED && prm.runs > 0) {
+    int lo = 0, hi = prm.runs - 1;  // run_offsets[lo] <= q < run_offsets[lo + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prm.run_offsets[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    t_base = prm.run_offsets[lo];
+  }
+  float val[R];  // the candidate value (packed: its lower limit)
   int idx[R];
   uint32_t key[R];
   int valid = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    val[r] = prm.cand_value[base + r * kWave + lane];
-    idx[r] = prm.cand_index[base + r * kWave + lane];
+    if constexpr (PACKED) {
+      const int slot = r * kWave + lane;
+      const uint32_t kv = ckey[base + slot];
+      idx[r] = kv != 0xFFFFFFFFu ? t_base + 16 * static_cast<int>(kv & imask) + slot / LIST : -1;
+      val[r] = key_value(kv & ~imask);
+      key[r] = kv;
+    } else {
+      val[r] = prm.cand_value[base + r * kWave + lane];
+      idx[r] = prm.cand_index[base + r * kWave + lane];
+      key[r] = idx[r] >= 0 ? order_key(val[r]) : 0xFFFFFFFFu;
+    }
     valid += __popcll(__ballot(idx[r] >= 0));
-    key[r] = idx[r] >= 0 ? order_key(val[r]) : 0xFFFFFFFFu;
   }
   wave_sort<R>(key, lane);
   // Rank k - 1 sits in register (k - 1) / 64 of lane (k - 1) % 64.
   uint32_t kth = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) kth = r == (k - 1) / kWave ? key[r] : kth;
-  const float ak = key_value(__shfl(kth, (k - 1) % kWave));
+  const float ak = key_value(__shfl(kth, (k - 1) % kWave) | imask);
   // Fewer valid candidates than k: every one is kept.
   const float bound = valid >= k ? ak + 2.0f * (prm.error_scale * (prm.query_norm2[q] + prm.train_norm2_max)) : INFINITY;
   // A full list whose largest value is within the bound may have dropped a row.
   bool overflow = false;
   if (lane < 16) {
     const size_t last = base + lane * LIST + LIST - 1;
-    overflow = prm.cand_index[last] >= 0 && prm.cand_value[last] <= bound;
+    if constexpr (PACKED) {
+      const uint32_t kv = ckey[last];
+      overflow = kv != 0xFFFFFFFFu && key_value(kv & ~imask) <= bound;
+    } else {
+      overflow = prm.cand_index[last] >= 0 && prm.cand_value[last] <= bound;
+    }
   }
   if (__ballot(overflow) != 0ull) {
     if (lane == 0) prm.flags[q] = 1;
@@ -423,33 +530,56 @@ hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi
                              hipStream_t s) {
   if (rows == 0) return hipSuccess;
   if (p < 1 || p > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(split_rows_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, s, x, rows, p,
-                     hi, lo, norm2);
+  const size_t threads = static_cast<size_t>(rows) * 8;
+  hipLaunchKernelGGL(split_rows_kernel, dim3(static_cast<unsigned>((threads + 255) / 256)), dim3(256), 0, s, x, rows,
+                     p, hi, lo, norm2);
   return hipGetLastError();
 }
 
-template <int LIST, int WAVES>
+template <int LIST, int WAVES, bool PACKED>
 static void launch_mfma_schedule(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
   const dim3 block(WAVES * 64);
   if (tables) {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST, WAVES>), grid, block, 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST, WAVES>), grid, block, 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
   } else {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST, WAVES>), grid, block, 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST, WAVES>), grid, block, 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
   }
 }
 
+template <int LIST, bool PACKED>
+static void launch_mfma_packing(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
+  if (prm.waves == 4) launch_mfma_schedule<LIST, 4, PACKED>(prm, euclid, tables, grid, s);
+  else launch_mfma_schedule<LIST, 8, PACKED>(prm, euclid, tables, grid, s);
+  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
+  if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST, PACKED>), dim3(rblocks), dim3(256), 0, s, prm);
+  else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST, PACKED>), dim3(rblocks), dim3(256), 0, s, prm);
+}
+
+static std::atomic<int> g_packed_lists{1};
+
+void set_mfma_packed_lists(int on) { g_packed_lists.store(on != 0 ? 1 : 0); }
+int mfma_packed_lists() { return g_packed_lists.load(); }
+
+// `train_rows` rows of one search (the largest run for a batch of tables):
+// packed lists when their column indices (row / 16) fit kMfmaMaxIndexBits.
 template <int LIST>
-static void launch_mfma_list(const KnnMfmaParams& prm, bool euclid, bool tables, int rows, hipStream_t s,
+static void launch_mfma_list(KnnMfmaParams prm, bool euclid, bool tables, int rows, int train_rows, hipStream_t s,
                              unsigned runs) {
   const int qpb = prm.waves * 16;
   const dim3 grid(static_cast<unsigned>((rows + qpb - 1) / qpb), runs);
-  if (prm.waves == 4) launch_mfma_schedule<LIST, 4>(prm, euclid, tables, grid, s);
-  else launch_mfma_schedule<LIST, 8>(prm, euclid, tables, grid, s);
-  const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
-  if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
-  else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST>), dim3(rblocks), dim3(256), 0, s, prm);
+  const unsigned top = static_cast<unsigned>((train_rows + 15) / 16 - 1);  // largest column index
+  int bits = 0;
+  while (bits < 32 && (top >> bits) != 0u) ++bits;
+  prm.runs = tables ? static_cast<int>(runs) : 0;
+  if (g_packed_lists.load() != 0 && bits <= kMfmaMaxIndexBits) {
+    prm.index_bits = bits;
+    launch_mfma_packing<LIST, true>(prm, euclid, tables, grid, s);
+  } else {
+    prm.index_bits = -1;
+    launch_mfma_packing<LIST, false>(prm, euclid, tables, grid, s);
+  }
 }
 
 static bool schedule_ok(const KnnMfmaParams& prm) {
@@ -462,8 +592,8 @@ hipError_t launch_knn_mfma(const KnnMfmaParams& prm, bool euclid, hipStream_t s)
   if (prm.dims < 1 || prm.dims > 64 || prm.k < 1 || prm.k > kMfmaMaxK || prm.train_rows < 1 || !schedule_ok(prm)) {
     return hipErrorInvalidValue;
   }
-  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, false, prm.query_rows, s, 1);
-  else launch_mfma_list<16>(prm, euclid, false, prm.query_rows, s, 1);
+  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, false, prm.query_rows, prm.train_rows, s, 1);
+  else launch_mfma_list<16>(prm, euclid, false, prm.query_rows, prm.train_rows, s, 1);
   return hipGetLastError();
 }
 
@@ -473,8 +603,8 @@ hipError_t launch_knn_mfma_tables(const KnnMfmaParams& prm, bool euclid, int run
       prm.folds == nullptr || prm.query_rows != prm.train_rows || max_rows < 1 || !schedule_ok(prm)) {
     return hipErrorInvalidValue;
   }
-  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, true, max_rows, s, static_cast<unsigned>(runs));
-  else launch_mfma_list<16>(prm, euclid, true, max_rows, s, static_cast<unsigned>(runs));
+  if (mfma_list(prm) == 8) launch_mfma_list<8>(prm, euclid, true, max_rows, max_rows, s, static_cast<unsigned>(runs));
+  else launch_mfma_list<16>(prm, euclid, true, max_rows, max_rows, s, static_cast<unsigned>(runs));
   return hipGetLastError();
 }
 
