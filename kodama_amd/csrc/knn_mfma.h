@@ -48,14 +48,19 @@ struct KnnMfmaParams {
   // Candidate lists as packed 32-bit keys (set by the launchers): the
   // order-preserving key of the candidate value with its low `index_bits`
   // bits replaced by the train row's index within its lane column (row / 16),
-  // in cand_index; -1 keeps separate values and indices. `runs` is the batch's
-  // run count for the rerank's run lookup.
+  // in cand_index; or, for larger train sets, values quantised to the same
+  // bits (their lower limits in cand_value) with rows; -1 keeps separate
+  // exact values and indices. `runs` is the batch's run count for the
+  // rerank's run lookup.
   int index_bits = -1;
   int runs = 0;
 };
 // Largest column index width of the packed lists: the key keeps 21 of the
 // value's 32 order bits, a relative quantisation of at most 2^-12.
 constexpr int kMfmaMaxIndexBits = 11;
+// Index bits of the per-segment keys of larger train sets (8-entry lists):
+// segments of 8192 rows, values kept to 23 order bits (2^-14 relative).
+constexpr int kMfmaSegmentBits = 9;
 // Packed lists on (1, the default, where the train rows fit) or off (0).
 void set_mfma_packed_lists(int on);
 int mfma_packed_lists();

@@ -86,6 +86,36 @@ __device__ __forceinline__ void key_insert(uint32_t (&kk)[LIST], uint32_t key) {
   kk[0] = min(kk[0], key);
 }
 
+// Merge of a segment's packed keys (ascending) into the ascending (value key,
+// row) pairs: the key of column row m stands for row `row0` + 16 m. The
+// half-cleaner min(pairs[s], keys[LIST - 1 - s]) keeps the LIST smallest as a
+// bitonic sequence, which the clean-up stages sort.
+template <int LIST>
+__device__ __forceinline__ void merge_segment(uint64_t (&k2)[LIST], const uint32_t (&kk)[LIST], uint32_t imask,
+                                              uint32_t row0) {
+  uint64_t c[LIST];
+#pragma unroll
+  for (int s = 0; s < LIST; ++s) {
+    const uint32_t key = kk[LIST - 1 - s];
+    const uint64_t pair = key == 0xFFFFFFFFu ? ~0ull
+                                             : static_cast<uint64_t>(key & ~imask) << 32 | (row0 + 16u * (key & imask));
+    c[s] = min(k2[s], pair);
+  }
+#pragma unroll
+  for (int stride = LIST / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+    for (int s = 0; s < LIST; ++s) {
+      if ((s & stride) == 0) {
+        const uint64_t lo = min(c[s], c[s + stride]), hi = max(c[s], c[s + stride]);
+        c[s] = lo;
+        c[s + stride] = hi;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < LIST; ++s) k2[s] = c[s];
+}
+
 // Ascending list insert; a value equal to the largest kept one is dropped.
 template <int LIST>
 __device__ __forceinline__ void list_insert(float (&d)[LIST], int (&id)[LIST], float v, int j) {
@@ -188,7 +218,13 @@ __global__ void split_rows_kernel(const float* __restrict__ x, int rows, int p, 
   if (live && part == 0) norm2[r] = s;
 }
 
-template <bool EUCLID, bool TABLES, int LIST, int WAVES, bool PACKED>
+// PACK 0: lists of values and indices. 1: lists of packed keys (the train
+// rows fit the index bits). 2: packed keys per segment of 16 << index_bits
+// train rows, merged at each segment's end into a sorted list of (value key,
+// row) pairs, the value key being the packed key with its index bits cleared:
+// within a segment the packed order is this pair order, so the merged list
+// keeps the LIST smallest pairs of all rows.
+template <bool EUCLID, bool TABLES, int LIST, int WAVES, int PACK>
 __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfmaParams prm) {
   constexpr int kCandThreads = WAVES * 64;
   constexpr int kQueriesPerBlock = WAVES * 16;
@@ -241,16 +277,23 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     }
   }
   // Unpacked lists: values and indices; packed: one key per entry (the
-  // index arrays are not used).
-  float kd[4][PACKED ? 1 : LIST];
-  int ki[4][PACKED ? 1 : LIST];
-  uint32_t kk[4][PACKED ? LIST : 1];
-  const uint32_t imask = PACKED ? (1u << prm.index_bits) - 1u : 0u;
+  // index arrays are not used); segments: the merged pairs beside them.
+  constexpr bool kPacked = PACK != 0;
+  float kd[4][kPacked ? 1 : LIST];
+  int ki[4][kPacked ? 1 : LIST];
+  uint32_t kk[4][kPacked ? LIST : 1];
+  uint64_t k2[4][PACK == 2 ? LIST : 1];
+  const uint32_t imask = kPacked ? (1u << prm.index_bits) - 1u : 0u;
+  const int seg_mask = PACK == 2 ? (16 << prm.index_bits) - 1 : 0;  // segment rows - 1 (a power of two)
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    if constexpr (PACKED) {
+    if constexpr (kPacked) {
 #pragma unroll
       for (int s = 0; s < LIST; ++s) kk[r][s] = 0xFFFFFFFFu;
+      if constexpr (PACK == 2) {
+#pragma unroll
+        for (int s = 0; s < LIST; ++s) k2[r][s] = ~0ull;
+      }
     } else {
 #pragma unroll
       for (int s = 0; s < LIST; ++s) {
@@ -260,27 +303,66 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     }
   }
 
-  for (int c0 = 0; c0 < t_rows; c0 += kChunk) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kChunk * 16; e += kCandThreads) {
-      // 64 rows x (8 hi + 8 lo uint4 parts).
+  // Staging of 64 train rows x (8 hi + 8 lo 16-byte parts). The global
+  // search streams its train rows from beyond L2, so it loads the next chunk
+  // into registers while this one is scored; a batch's runs sit in L2 and
+  // its kernels keep the registers for occupancy.
+  constexpr bool kPrefetch = !TABLES;
+  constexpr int kStage = kChunk * 16 / kCandThreads;  // parts per thread and chunk
+  uint4 pre[kPrefetch ? kStage : 1];
+  float pre_tn = 0.0f;
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < kStage; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * kCandThreads;
       const int row = e >> 4, part = e & 15;
       const int tr = c0 + row;
-      const bool is_lo = part >= 8;
-      const int d8 = (part & 7) * 8;
-      uint4 v = make_uint4(0, 0, 0, 0);
+      pre[i] = make_uint4(0, 0, 0, 0);
       if (tr < t_rows) {
-        const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(t_base + tr) * 64 + d8;
-        v = *reinterpret_cast<const uint4*>(src);
+        const unsigned short* src =
+            (part >= 8 ? prm.train_lo : prm.train_hi) + static_cast<size_t>(t_base + tr) * 64 + (part & 7) * 8;
+        pre[i] = *reinterpret_cast<const uint4*>(src);
       }
-      *reinterpret_cast<uint4*>((is_lo ? s_lo : s_hi) + row * kRowStride + d8) = v;
     }
     if (threadIdx.x < kChunk) {
       const int tr = c0 + static_cast<int>(threadIdx.x);
-      s_tn[threadIdx.x] = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
-      if (TABLES) s_fold[threadIdx.x] = tr < t_rows ? prm.folds[t_base + tr] : -2;
+      pre_tn = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
+    }
+  };
+  if constexpr (kPrefetch) fetch(0);
+  for (int c0 = 0; c0 < t_rows; c0 += kChunk) {
+    __syncthreads();
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int i = 0; i < kStage; ++i) {
+        const int e = static_cast<int>(threadIdx.x) + i * kCandThreads;
+        const int row = e >> 4, part = e & 15;
+        *reinterpret_cast<uint4*>((part >= 8 ? s_lo : s_hi) + row * kRowStride + (part & 7) * 8) = pre[i];
+      }
+      if (threadIdx.x < kChunk) s_tn[threadIdx.x] = pre_tn;
+    } else {
+      for (int e = threadIdx.x; e < kChunk * 16; e += kCandThreads) {
+        const int row = e >> 4, part = e & 15;
+        const int tr = c0 + row;
+        const bool is_lo = part >= 8;
+        const int d8 = (part & 7) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (tr < t_rows) {
+          const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(t_base + tr) * 64 + d8;
+          v = *reinterpret_cast<const uint4*>(src);
+        }
+        *reinterpret_cast<uint4*>((is_lo ? s_lo : s_hi) + row * kRowStride + d8) = v;
+      }
+      if (threadIdx.x < kChunk) {
+        const int tr = c0 + static_cast<int>(threadIdx.x);
+        s_tn[threadIdx.x] = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
+        if (TABLES) s_fold[threadIdx.x] = tr < t_rows ? prm.folds[t_base + tr] : -2;
+      }
     }
     __syncthreads();
+    if constexpr (kPrefetch) {
+      if (c0 + kChunk < t_rows) fetch(c0 + kChunk);  // in flight during the scoring below
+    }
     f32x4 acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -303,15 +385,16 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
       for (int tt = 0; tt < 4; ++tt)
         acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a_lo[kb]), as_bf16x8(b_hi[tt]), acc[tt], 0, 0, 0);
     }
-    if constexpr (PACKED) {
-      // Row j = c0 + 16 tt + col is row j / 16 = c0 / 16 + tt of its lane
-      // column, the same for every lane. A candidate that is excluded, past
-      // the rows or not below +inf (the unpacked lists drop those too) gets
-      // the empty key.
+    if constexpr (kPacked) {
+      // Row j = c0 + 16 tt + col is row (j - seg0) / 16 = (c0 - seg0) / 16 + tt
+      // of its lane column within the segment starting at row seg0, the same
+      // for every lane. A candidate that is excluded, past the rows or not
+      // below +inf (the unpacked lists drop those too) gets the empty key.
+      const int seg0 = PACK == 2 ? c0 & ~seg_mask : 0;
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt) {
         const int j = c0 + tt * 16 + col;
-        const uint32_t m = static_cast<uint32_t>((c0 >> 4) + tt);
+        const uint32_t m = static_cast<uint32_t>(((c0 - seg0) >> 4) + tt);
         const float tn = s_tn[tt * 16 + col];
         const int skip = TABLES ? s_fold[tt * 16 + col] : j;
 #pragma unroll
@@ -319,6 +402,17 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
           const float v = EUCLID ? (qn[r] + tn) - 2.0f * acc[tt][r] : -acc[tt][r];
           const bool ok = j < t_rows && skip != excl[r] && v < INFINITY;
           key_insert<LIST>(kk[r], ok ? ((order_key(v) & ~imask) | m) : 0xFFFFFFFFu);
+        }
+      }
+      if constexpr (PACK == 2) {
+        // Segment end (a uniform condition): fold the keys into the pairs.
+        if (((c0 + kChunk) & seg_mask) == 0 || c0 + kChunk >= t_rows) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            merge_segment<LIST>(k2[r], kk[r], imask, static_cast<uint32_t>(t_base + seg0 + col));
+#pragma unroll
+            for (int s = 0; s < LIST; ++s) kk[r][s] = 0xFFFFFFFFu;
+          }
         }
       }
     } else {
@@ -346,10 +440,18 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     const int q = q_tile + 4 * grp + r;
     if (q < q_rows) {
       const size_t base = (static_cast<size_t>(q_base + q) * 16 + col) * LIST;
-      if constexpr (PACKED) {
+      if constexpr (PACK == 1) {
         uint32_t* keys = reinterpret_cast<uint32_t*>(prm.cand_index);
 #pragma unroll
         for (int s = 0; s < LIST; ++s) keys[base + s] = kk[r][s];
+      } else if constexpr (PACK == 2) {
+        // The value's lower limit (the rerank knows the index bits) and the row.
+#pragma unroll
+        for (int s = 0; s < LIST; ++s) {
+          const bool full = k2[r][s] != ~0ull;
+          prm.cand_value[base + s] = full ? key_value(static_cast<uint32_t>(k2[r][s] >> 32)) : INFINITY;
+          prm.cand_index[base + s] = full ? static_cast<int>(static_cast<uint32_t>(k2[r][s])) : -1;
+        }
       } else {
 #pragma unroll
         for (int s = 0; s < LIST; ++s) {
@@ -407,7 +509,8 @@ __device__ __forceinline__ void wave_sort(uint32_t (&v)[R], int lane) {
 // every row of the true k nearest has a value <= A = hi(k-th key) + 2E; the
 // candidates re-scored are those with lo(key) <= A, and a full list whose
 // last key has lo <= A may have dropped such a row (the keys of a list are
-// distinct, its rows being).
+// distinct, its rows being). Lists merged over segments hold (value lower
+// limit, row) with the same limits: the same bound and tests apply.
 template <bool EUCLID, int LIST, bool PACKED>
 __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   constexpr int kCand = 16 * LIST;
@@ -426,7 +529,7 @@ __global__ void __launch_bounds__(256) knn_rerank_kernel(KnnMfmaParams prm) {
   const int p = prm.dims;
   const size_t base = static_cast<size_t>(q) * kCand;
   const uint32_t* __restrict__ ckey = reinterpret_cast<const uint32_t*>(prm.cand_index);
-  const uint32_t imask = PACKED ? (1u << prm.index_bits) - 1u : 0u;
+  const uint32_t imask = prm.index_bits >= 0 ? (1u << prm.index_bits) - 1u : 0u;
   int t_base = 0;  // packed rows count from the start of the query's run
   if (PACKED && prm.runs > 0) {
     int lo = 0, hi = prm.runs - 1;  // run_offsets[lo] <= q < run_offsets[lo + 1]
@@ -536,25 +639,25 @@ hipError_t launch_split_rows(const float* x, int rows, int p, unsigned short* hi
   return hipGetLastError();
 }
 
-template <int LIST, int WAVES, bool PACKED>
+template <int LIST, int WAVES, int PACK>
 static void launch_mfma_schedule(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
   const dim3 block(WAVES * 64);
   if (tables) {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, true, LIST, WAVES, PACK>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, true, LIST, WAVES, PACK>), grid, block, 0, s, prm);
   } else {
-    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
-    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST, WAVES, PACKED>), grid, block, 0, s, prm);
+    if (euclid) hipLaunchKernelGGL((knn_mfma_candidates_kernel<true, false, LIST, WAVES, PACK>), grid, block, 0, s, prm);
+    else hipLaunchKernelGGL((knn_mfma_candidates_kernel<false, false, LIST, WAVES, PACK>), grid, block, 0, s, prm);
   }
 }
 
-template <int LIST, bool PACKED>
+template <int LIST, int PACK>
 static void launch_mfma_packing(const KnnMfmaParams& prm, bool euclid, bool tables, dim3 grid, hipStream_t s) {
-  if (prm.waves == 4) launch_mfma_schedule<LIST, 4, PACKED>(prm, euclid, tables, grid, s);
-  else launch_mfma_schedule<LIST, 8, PACKED>(prm, euclid, tables, grid, s);
+  if (prm.waves == 4) launch_mfma_schedule<LIST, 4, PACK>(prm, euclid, tables, grid, s);
+  else launch_mfma_schedule<LIST, 8, PACK>(prm, euclid, tables, grid, s);
   const unsigned rblocks = static_cast<unsigned>((prm.query_rows + 3) / 4);
-  if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST, PACKED>), dim3(rblocks), dim3(256), 0, s, prm);
-  else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST, PACKED>), dim3(rblocks), dim3(256), 0, s, prm);
+  if (euclid) hipLaunchKernelGGL((knn_rerank_kernel<true, LIST, PACK == 1>), dim3(rblocks), dim3(256), 0, s, prm);
+  else hipLaunchKernelGGL((knn_rerank_kernel<false, LIST, PACK == 1>), dim3(rblocks), dim3(256), 0, s, prm);
 }
 
 static std::atomic<int> g_packed_lists{1};
@@ -563,7 +666,9 @@ void set_mfma_packed_lists(int on) { g_packed_lists.store(on != 0 ? 1 : 0); }
 int mfma_packed_lists() { return g_packed_lists.load(); }
 
 // `train_rows` rows of one search (the largest run for a batch of tables):
-// packed lists when their column indices (row / 16) fit kMfmaMaxIndexBits.
+// packed lists when their column indices (row / 16) fit kMfmaMaxIndexBits;
+// past that, 8-entry lists are packed per segment of kMfmaSegmentBits index
+// bits (16 << kMfmaSegmentBits rows) and merged into (value, row) pairs.
 template <int LIST>
 static void launch_mfma_list(KnnMfmaParams prm, bool euclid, bool tables, int rows, int train_rows, hipStream_t s,
                              unsigned runs) {
@@ -573,13 +678,21 @@ static void launch_mfma_list(KnnMfmaParams prm, bool euclid, bool tables, int ro
   int bits = 0;
   while (bits < 32 && (top >> bits) != 0u) ++bits;
   prm.runs = tables ? static_cast<int>(runs) : 0;
-  if (g_packed_lists.load() != 0 && bits <= kMfmaMaxIndexBits) {
+  const bool packed = g_packed_lists.load() != 0;
+  if (packed && bits <= kMfmaMaxIndexBits) {
     prm.index_bits = bits;
-    launch_mfma_packing<LIST, true>(prm, euclid, tables, grid, s);
-  } else {
-    prm.index_bits = -1;
-    launch_mfma_packing<LIST, false>(prm, euclid, tables, grid, s);
+    launch_mfma_packing<LIST, 1>(prm, euclid, tables, grid, s);
+    return;
   }
+  if constexpr (LIST == 8) {
+    if (packed) {
+      prm.index_bits = kMfmaSegmentBits;
+      launch_mfma_packing<LIST, 2>(prm, euclid, tables, grid, s);
+      return;
+    }
+  }
+  prm.index_bits = -1;
+  launch_mfma_packing<LIST, 0>(prm, euclid, tables, grid, s);
 }
 
 static bool schedule_ok(const KnnMfmaParams& prm) {

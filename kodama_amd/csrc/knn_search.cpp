@@ -104,6 +104,7 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
   m.flags = flags.get();
   m.out_index = d_idx;
   m.out_dist = d_dist;
+  m.waves = 4;  // two 4-wave blocks per CU overlap each other's staging barriers
   check(launch_knn_mfma(m, euclid, s), "knn_mfma launch");
   std::vector<int> h_flags(static_cast<size_t>(query_rows));
   flags.download(h_flags.data(), h_flags.size(), s);

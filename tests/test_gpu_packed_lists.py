@@ -4,7 +4,8 @@ separate values and indices: both must give the oracle's neighbours bit for
 bit, for the global search, the per-fold tables of a batch of runs (the
 rerank finds each query's run), every list length, negative candidate values
 (inner product), and train sets at the widest packed index (32768 rows) and
-one row past it (separate lists)."""
+past it (8-entry lists packed per 8192-row segment and merged into (value,
+row) pairs at each segment's end; 16-entry lists separate)."""
 import numpy as np
 import pytest
 
@@ -53,8 +54,8 @@ def test_fold_table_lists_match_oracle(oracle, packed, list_):
 @pytest.mark.parametrize("train_rows", [32768, 32769])
 def test_widest_packed_index_and_past_it(oracle, train_rows):
     """32768 train rows fill 11 index bits (column rows 0..2047); one more takes
-    the separate lists. Queries near the last train rows make the top column
-    indices decode into neighbours."""
+    the segments (five, the last of one row). Queries near the last train rows
+    make the top column indices decode into neighbours."""
     x, _ = D.config3(n=train_rows + 400, p=32, seed=train_rows)
     train = x[:train_rows]
     rng = np.random.default_rng(7)
@@ -68,6 +69,24 @@ def test_widest_packed_index_and_past_it(oracle, train_rows):
         assert np.array_equal(idx, want_idx), packed
         assert np.array_equal(dist, want_dist), packed
     assert np.any(want_idx[:300] >= train_rows - 16)
+
+
+@pytest.mark.parametrize("k,metric,name", [(20, 1, "inner_product"), (32, 2, "euclidean"), (40, 2, "euclidean")])
+def test_segmented_lists_with_exclusion(oracle, k, metric, name):
+    """40000 train rows: k <= 32 merges segment keys into pairs (negative
+    values for the inner product), k = 40 keeps separate 16-entry lists; every
+    query excludes one train row."""
+    x, _ = D.config3(n=40600, p=50, seed=k)
+    train, query = x[:40000], x[40000:]
+    ex = (np.arange(600, dtype=np.int32) * 61) % 40000
+    want_idx, want_dist = oracle.exact_knn(train, query, k, metric, exclude=ex)
+    for packed in (True, False):
+        K.set_mfma_packed_lists(packed)
+        idx, dist = K.exact_knn_search(train, query, k, name, exclude=ex)
+        assert K.last_knn_fallbacks() < 300, packed
+        assert np.array_equal(idx, want_idx), packed
+        assert np.array_equal(dist, want_dist), packed
+        assert not np.any(idx == ex[:, None])
 
 
 def test_duplicates_fall_back_with_packed_lists(oracle):
