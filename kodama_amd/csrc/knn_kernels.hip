@@ -31,6 +31,7 @@ namespace kodama_hip {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWave = 64;
 
 // Candidate rows per LDS tile: 16 KiB of candidates for up to 64 dimensions,
 // 32 KiB beyond.
@@ -114,8 +115,10 @@ __device__ __forceinline__ void topk_insert(float (&kd)[KMAX], int (&ki)[KMAX], 
 // MODE 0: queries vs a separate train set, optional per-query excluded train
 // index. MODE 1: every run's rows against the rows of the same run whose fold
 // differs (the per-fold tables of core.cpp:463-584, all folds at once).
-template <int PMAX, int KMAX, int MODE, bool EUCLID>
-__global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
+// BLOCK: threads (queries) per block; the listed rescans of flagged table rows
+// (a few queries per run) take one wave, so idle waves hold no CU slots.
+template <int PMAX, int KMAX, int MODE, bool EUCLID, int BLOCK = kBlock>
+__global__ void __launch_bounds__(BLOCK) knn_topk_kernel(KnnTopkParams prm) {
   constexpr int kTile = tile_rows<PMAX>();
   __shared__ float tile[kTile * PMAX];
   __shared__ int tile_fold[kTile];
@@ -135,8 +138,9 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
     t_begin = prm.slice_rows > 0 ? static_cast<int>(blockIdx.z) * prm.slice_rows : 0;
     t_count = prm.slice_rows > 0 ? min(prm.slice_rows, L - t_begin) : L;
   }
-  const int qi = blockIdx.x * kBlock + threadIdx.x;
-  if (blockIdx.x * kBlock >= q_count) return;  // whole block idle for this run
+  static_assert(kTile <= BLOCK, "one thread per tile row stages the folds");
+  const int qi = blockIdx.x * BLOCK + threadIdx.x;
+  if (blockIdx.x * BLOCK >= q_count) return;  // whole block idle for this run
   const bool active = qi < q_count;
   const int p = prm.dims;
   const int k = prm.k;
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) knn_topk_kernel(KnnTopkParams prm) {
   for (int base = 0; base < t_count; base += kTile) {
     const int nt = min(kTile, t_count - base);
     __syncthreads();
-    for (int e = threadIdx.x; e < kTile * p; e += kBlock) {
+    for (int e = threadIdx.x; e < kTile * p; e += BLOCK) {
       const int r = e / p;
       const int d = e - r * p;
       float v = 0.0f;
@@ -497,9 +501,17 @@ __global__ void __launch_bounds__(kBlock) knn_merge_slices_kernel(const int* __r
 
 // MODES = 1: query vs train only; 2: both modes.
 template <int PMAX, int KMAX, int MODES = 2>
-static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid, int grid_x, int grid_y,
+static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid, int max_queries, int grid_y,
                                  int grid_z, hipStream_t s) {
-  const dim3 grid(grid_x, grid_y, grid_z), block(kBlock);
+  if constexpr (MODES == 2) {
+    if (mode == 1 && prm.qlist != nullptr) {
+      const dim3 grid((max_queries + kWave - 1) / kWave, grid_y, grid_z), block(kWave);
+      if (euclid) hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 1, true, kWave>), grid, block, 0, s, prm);
+      else hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 1, false, kWave>), grid, block, 0, s, prm);
+      return hipGetLastError();
+    }
+  }
+  const dim3 grid((max_queries + kBlock - 1) / kBlock, grid_y, grid_z), block(kBlock);
   if (mode == 0) {
     if (euclid) hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, true>), grid, block, 0, s, prm);
     else hipLaunchKernelGGL((knn_topk_kernel<PMAX, KMAX, 0, false>), grid, block, 0, s, prm);
@@ -515,6 +527,7 @@ static hipError_t launch_topk_pk(const KnnTopkParams& prm, int mode, bool euclid
 template <int PMAX>
 static hipError_t launch_topk_p(const KnnTopkParams& prm, int mode, bool euclid, int gx, int gy, int gz,
                                 hipStream_t s) {
+  // gx: the most queries of one run (the blocks along x follow the block size).
   if (prm.k <= 8) return launch_topk_pk<PMAX, 8>(prm, mode, euclid, gx, gy, gz, s);
   if (prm.k <= 16) return launch_topk_pk<PMAX, 16>(prm, mode, euclid, gx, gy, gz, s);
   if (prm.k <= 32) return launch_topk_pk<PMAX, 32>(prm, mode, euclid, gx, gy, gz, s);
@@ -534,7 +547,7 @@ hipError_t launch_knn_topk(const KnnTopkParams& prm, int mode, bool euclid, int 
     return hipErrorInvalidValue;
   }
   if (mode == 1 && prm.slice_rows > 0 && prm.qlist == nullptr) return hipErrorInvalidValue;
-  const int gx = (max_queries + kBlock - 1) / kBlock;
+  const int gx = max_queries;
   if (gx == 0 || runs == 0) return hipSuccess;
   if (prm.dims <= 32) return launch_topk_p<32>(prm, mode, euclid, gx, runs, slices, stream);
   if (prm.dims <= 64) return launch_topk_p<64>(prm, mode, euclid, gx, runs, slices, stream);

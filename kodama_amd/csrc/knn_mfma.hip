@@ -303,14 +303,14 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     }
   }
 
-  // Staging of 64 train rows x (8 hi + 8 lo 16-byte parts). The global
-  // search streams its train rows from beyond L2, so it loads the next chunk
-  // into registers while this one is scored; a batch's runs sit in L2 and
-  // its kernels keep the registers for occupancy.
-  constexpr bool kPrefetch = !TABLES;
+  // Staging of 64 train rows x (8 hi + 8 lo 16-byte parts): the next chunk
+  // is loaded into registers while this one is scored, so the loads' latency
+  // (L2 for a batch's runs, beyond it for a large train set) is not paid at
+  // the barrier.
   constexpr int kStage = kChunk * 16 / kCandThreads;  // parts per thread and chunk
-  uint4 pre[kPrefetch ? kStage : 1];
+  uint4 pre[kStage];
   float pre_tn = 0.0f;
+  int pre_fold = -2;
   auto fetch = [&](int c0) {
 #pragma unroll
     for (int i = 0; i < kStage; ++i) {
@@ -327,42 +327,24 @@ __global__ void __launch_bounds__(WAVES * 64) knn_mfma_candidates_kernel(KnnMfma
     if (threadIdx.x < kChunk) {
       const int tr = c0 + static_cast<int>(threadIdx.x);
       pre_tn = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
+      if (TABLES) pre_fold = tr < t_rows ? prm.folds[t_base + tr] : -2;
     }
   };
-  if constexpr (kPrefetch) fetch(0);
+  fetch(0);
   for (int c0 = 0; c0 < t_rows; c0 += kChunk) {
     __syncthreads();
-    if constexpr (kPrefetch) {
 #pragma unroll
-      for (int i = 0; i < kStage; ++i) {
-        const int e = static_cast<int>(threadIdx.x) + i * kCandThreads;
-        const int row = e >> 4, part = e & 15;
-        *reinterpret_cast<uint4*>((part >= 8 ? s_lo : s_hi) + row * kRowStride + (part & 7) * 8) = pre[i];
-      }
-      if (threadIdx.x < kChunk) s_tn[threadIdx.x] = pre_tn;
-    } else {
-      for (int e = threadIdx.x; e < kChunk * 16; e += kCandThreads) {
-        const int row = e >> 4, part = e & 15;
-        const int tr = c0 + row;
-        const bool is_lo = part >= 8;
-        const int d8 = (part & 7) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (tr < t_rows) {
-          const unsigned short* src = (is_lo ? prm.train_lo : prm.train_hi) + static_cast<size_t>(t_base + tr) * 64 + d8;
-          v = *reinterpret_cast<const uint4*>(src);
-        }
-        *reinterpret_cast<uint4*>((is_lo ? s_lo : s_hi) + row * kRowStride + d8) = v;
-      }
-      if (threadIdx.x < kChunk) {
-        const int tr = c0 + static_cast<int>(threadIdx.x);
-        s_tn[threadIdx.x] = tr < t_rows ? prm.train_norm2[t_base + tr] : 0.0f;
-        if (TABLES) s_fold[threadIdx.x] = tr < t_rows ? prm.folds[t_base + tr] : -2;
-      }
+    for (int i = 0; i < kStage; ++i) {
+      const int e = static_cast<int>(threadIdx.x) + i * kCandThreads;
+      const int row = e >> 4, part = e & 15;
+      *reinterpret_cast<uint4*>((part >= 8 ? s_lo : s_hi) + row * kRowStride + (part & 7) * 8) = pre[i];
+    }
+    if (threadIdx.x < kChunk) {
+      s_tn[threadIdx.x] = pre_tn;
+      if (TABLES) s_fold[threadIdx.x] = pre_fold;
     }
     __syncthreads();
-    if constexpr (kPrefetch) {
-      if (c0 + kChunk < t_rows) fetch(c0 + kChunk);  // in flight during the scoring below
-    }
+    if (c0 + kChunk < t_rows) fetch(c0 + kChunk);  // in flight during the scoring below
     f32x4 acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
