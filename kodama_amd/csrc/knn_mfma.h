@@ -83,8 +83,11 @@ hipError_t launch_scatter_results(const int* src_idx, const float* src_dist, con
 
 // Exact kNN of device-resident rows (internal distances, (distance, index)
 // order, -1 / +inf beyond the available rows), bit-identical to the register
-// scan: the MFMA path when dims <= 64, k <= 32 and train_rows >= 4096,
-// otherwise (and for every flagged query) the register scan. `train_norm2_max`
+// scan: the MFMA path when dims <= 64, k <= 128 and train_rows >= 4096,
+// otherwise the register scan. Queries the 8-entry lists (k <= 32) cannot
+// certify take, when there are 4096 or more, a second MFMA pass with 16-entry
+// lists; the rest (those it flags again, fewer, or of 16-entry lists) take
+// the register scan. `train_norm2_max`
 // bounds the squared norms of the train rows (prepared rows for cosine).
 void device_exact_knn(const float* d_train, int train_rows, const float* d_query, int query_rows, int dims, int k,
                       int metric, const int* d_exclude, int* d_idx, float* d_dist, float train_norm2_max,

@@ -24,6 +24,7 @@ constexpr int kScanTargetBlocks = 512;   // two blocks per CU
 constexpr int kMaxSlices = 64;           // lists one merge wave combines
 constexpr int kSliceAlign = 64;          // LDS tile rows of the scan
 constexpr int kTableTargetBlocks = 2048; // sliced rescans of flagged table rows
+constexpr int kMfmaSecondPassMin = 4096; // flagged queries for a 16-entry MFMA pass before the scan
 
 }  // namespace
 
@@ -128,7 +129,61 @@ void device_exact_knn(const float* d_train, int train_rows, const float* d_query
     ex_sub.resize(sub.size());
     ex_sub.upload(sub.data(), sub.size(), s);
   }
-  scan(q_sub.get(), nr, d_exclude ? ex_sub.get() : nullptr, r_idx.get(), r_dist.get());
+  const int* d_ex_sub = d_exclude ? ex_sub.get() : nullptr;
+  if (mfma_list(m) == 8 && nr >= kMfmaSecondPassMin) {
+    // Second pass over the flagged queries with 16 candidates per lane column
+    // (most certify there); only queries flagged again take the scan. Its
+    // workgroups stream every train row, so it pays once the flagged queries
+    // would keep the sliced scan busy for longer.
+    DeviceBuffer<unsigned short> s_hi(static_cast<size_t>(nr) * 64), s_lo(s_hi.size());
+    DeviceBuffer<float> s_n2(static_cast<size_t>(nr));
+    check(launch_split_rows(q_sub.get(), nr, dims, s_hi.get(), s_lo.get(), s_n2.get(), s), "split_rows launch");
+    const size_t cand2 = static_cast<size_t>(nr) * 16 * 16;
+    DeviceBuffer<float> c2_val(cand2);
+    DeviceBuffer<int> c2_idx(cand2), flags2(static_cast<size_t>(nr));
+    KnnMfmaParams m2 = m;
+    m2.query = q_sub.get();
+    m2.query_hi = s_hi.get();
+    m2.query_lo = s_lo.get();
+    m2.query_norm2 = s_n2.get();
+    m2.query_rows = nr;
+    m2.exclude = d_ex_sub;
+    m2.cand_value = c2_val.get();
+    m2.cand_index = c2_idx.get();
+    m2.flags = flags2.get();
+    m2.out_index = r_idx.get();
+    m2.out_dist = r_dist.get();
+    m2.list = 16;
+    check(launch_knn_mfma(m2, euclid, s), "knn_mfma launch");
+    std::vector<int> h_flags2(static_cast<size_t>(nr));
+    flags2.download(h_flags2.data(), h_flags2.size(), s);
+    check(hipStreamSynchronize(s), "knn_mfma sync");
+    std::vector<int> redo2;
+    for (int i = 0; i < nr; ++i) if (h_flags2[i]) redo2.push_back(i);
+    if (mfma_fallbacks) *mfma_fallbacks = static_cast<int>(redo2.size());
+    if (!redo2.empty()) {
+      const int n2 = static_cast<int>(redo2.size());
+      DeviceBuffer<int> d_rows2(static_cast<size_t>(n2));
+      d_rows2.upload(redo2.data(), redo2.size(), s);
+      DeviceBuffer<float> q_sub2(static_cast<size_t>(n2) * dims), r2_dist(static_cast<size_t>(n2) * k);
+      DeviceBuffer<int> r2_idx(static_cast<size_t>(n2) * k), ex_sub2;
+      check(launch_gather_rows(q_sub.get(), d_rows2.get(), n2, dims, q_sub2.get(), s), "gather_rows launch");
+      if (d_exclude) {
+        std::vector<int> h_ex(static_cast<size_t>(query_rows)), sub2(redo2.size());
+        check(hipMemcpyAsync(h_ex.data(), d_exclude, h_ex.size() * sizeof(int), hipMemcpyDeviceToHost, s),
+              "download");
+        check(hipStreamSynchronize(s), "exclusion sync");
+        for (int i = 0; i < n2; ++i) sub2[i] = h_ex[redo[redo2[i]]];
+        ex_sub2.resize(sub2.size());
+        ex_sub2.upload(sub2.data(), sub2.size(), s);
+      }
+      scan(q_sub2.get(), n2, d_exclude ? ex_sub2.get() : nullptr, r2_idx.get(), r2_dist.get());
+      check(launch_scatter_results(r2_idx.get(), r2_dist.get(), d_rows2.get(), n2, k, r_idx.get(), r_dist.get(), s),
+            "scatter_results launch");
+    }
+  } else {
+    scan(q_sub.get(), nr, d_ex_sub, r_idx.get(), r_dist.get());
+  }
   check(launch_scatter_results(r_idx.get(), r_dist.get(), d_rows.get(), nr, k, d_idx, d_dist, s),
         "scatter_results launch");
   check(hipStreamSynchronize(s), "fallback sync");

@@ -103,6 +103,22 @@ def test_duplicates_fall_back_with_packed_lists(oracle):
     assert np.array_equal(g["distances"], dist)
 
 
+@pytest.mark.parametrize("packed", [True, False])
+def test_second_pass_certifies_what_eight_entries_cannot(oracle, packed):
+    """150 copies of each of 40 rows: every lane column holds about 9 copies
+    at distance 0, so every 8-entry list fills inside the bound and all 6000
+    queries are flagged; the second pass with 16-entry lists certifies them
+    (ties at distance 0 ranked by index) and nothing reaches the scan."""
+    base, _ = D.config3(n=40, p=24, seed=5)
+    x = np.repeat(base, 150, axis=0)
+    K.set_mfma_packed_lists(packed)
+    g = K.KODAMA_knn_graph(x, k=20)
+    assert K.last_knn_fallbacks() == 0
+    idx, dist = oracle.knn_graph(x, 20, 2)
+    assert np.array_equal(g["indices"], idx)
+    assert np.array_equal(g["distances"], dist)
+
+
 def test_packed_list_control_rejects_out_of_range():
     with pytest.raises(ValueError):
         K.N.call("kodama_hip_set_mfma_packed_lists", K.C.c_int32(2))
