@@ -648,9 +648,10 @@ void KnnBatch::allocate(const IVec& run_offsets) {
   h_firsts_.resize(static_cast<size_t>(runs_) * classes_);
   h_matches_.resize(static_cast<size_t>(runs_));
   // Device-evolution staging: 16-bit labels per padded row plus per-run words
-  // (keys, widths, state), and the accuracy history of up to 128 cycles.
+  // (keys, widths, state, the elapsed ticks of up to eight workgroups), and
+  // the accuracy history of up to 128 cycles.
   h_stage_up_.resize(2 * static_cast<size_t>(pad_total_) + 16 * static_cast<size_t>(runs_) + 1024);
-  h_stage_down_.resize(2 * static_cast<size_t>(pad_total_) + (32 + 8 * 128) * static_cast<size_t>(runs_) + 1024);
+  h_stage_down_.resize(2 * static_cast<size_t>(pad_total_) + (32 + 64 + 8 * 128) * static_cast<size_t>(runs_) + 1024);
   std::fill(h_labels_.get(), h_labels_.get() + pad_total_, 0);
   std::fill(h_pred_.get(), h_pred_.get() + pad_total_, 0);
 }

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <queue>
 
 #include "evolve_launch.h"
@@ -23,6 +25,16 @@ constexpr int kMinSliceRows = 512;  // a split run's members vote at least this 
 // Time per row of the replayed phases relative to the vote's.
 constexpr double kReplayShare = 0.33;
 constexpr int kSpinLimit = 1 << 20;
+
+// Measured cost of each run in earlier evolutions of a batch shape (key: run
+// count, k, classes, layout and every run's rows): wall-clock ticks per
+// evaluation divided by (kReplayShare + 1 / members), so that it
+// extrapolates to other member counts. The member assignment of the next evolution of
+// that shape starts from them: the rows alone miss how many of a run's rows
+// need the full vote.
+std::mutex g_cost_mutex;
+std::map<std::vector<int>, std::vector<double>> g_run_costs;
+constexpr size_t kMaxCostShapes = 64;
 
 uint64_t splitmix(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -51,6 +63,9 @@ struct DeviceEvolution::Impl {
   bool large = false;
   bool learned = false;
   int hash_bits = 0;  // LDS flow table (0: the per-workgroup HBM table)
+  std::vector<int> shape_key;  // the batch shape of the measured run costs
+  IVec group;                  // members per run
+  size_t off_time = 0;         // per-workgroup elapsed ticks
   EventTimer timer;
   bool timed = false;
   // Split runs: the exchange region (xbuf, xmatch, error) cleared before
@@ -114,8 +129,18 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   // proposal, absorption and acceptance over all of the run's rows and votes
   // its slice, so a cycle takes about rows * (kReplayShare + 1 / members)
   // vote-row times (kReplayShare measured on the 100-run batch, k = 30).
+  // After an evolution of the same batch shape the rows give way to each
+  // run's measured time in that model (results do not depend on it).
   int device = 0;
   check(hipGetDevice(&device), "hipGetDevice");
+  m.shape_key = {m.runs, batch.k(), m.classes, m.learned ? 1 : 0, m.large ? 1 : 0, m.hash_bits};
+  for (int r = 0; r < m.runs; ++r) m.shape_key.push_back(batch.run_rows(r));
+  std::vector<double> measured;
+  {
+    std::lock_guard<std::mutex> lock(g_cost_mutex);
+    const auto it = g_run_costs.find(m.shape_key);
+    if (it != g_run_costs.end()) measured = it->second;
+  }
   IVec group(static_cast<size_t>(m.runs), 1);
   const int max_group = std::min(kEvoMaxGroup, g_split_max.load() == 0 ? kEvoMaxGroup : g_split_max.load());
   if (max_group > 1) {
@@ -126,7 +151,8 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     if (capacity >= m.runs) {
       std::priority_queue<std::pair<double, int>> pq;
       const auto cycle_cost = [&](int r) {
-        return static_cast<double>(batch.run_rows(r)) * (kReplayShare + 1.0 / group[r]);
+        const double unit = measured.empty() ? static_cast<double>(batch.run_rows(r)) : measured[r];
+        return unit * (kReplayShare + 1.0 / group[r]);
       };
       for (int r = 0; r < m.runs; ++r) pq.emplace(cycle_cost(r), r);
       int total = m.runs;
@@ -161,6 +187,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.off_best = lay.take(label_bytes);  // download: best .. history
   m.off_state = lay.take(sizeof(double) * m.runs * kEvoStateStride);
   m.off_history = lay.take(sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap);
+  m.off_time = lay.take(sizeof(unsigned long long) * static_cast<size_t>(m.workgroups));
   // The learned proposal's network: fp32 parameters and their tiled form.
   m.off_params = lay.take(sizeof(float) * kMlpParams);
   m.off_packed = lay.take(sizeof(uint32_t) * kMlpTiled);
@@ -179,11 +206,13 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.up_begin = m.off_cur;
   m.up_bytes = m.off_key + sizeof(uint64_t) * m.runs - m.off_cur;
   m.down_begin = m.off_best;
-  m.down_bytes = m.off_history + sizeof(double) * static_cast<size_t>(m.runs) * m.cycles_cap - m.off_best;
+  m.down_bytes = m.off_time + sizeof(unsigned long long) * static_cast<size_t>(m.workgroups) - m.off_best;
   // Rounded up to 2 MiB so that evolutions of similar size reuse one pool block.
   m.arena.allocate((lay.total + (2u << 20) - 1) / (2u << 20) * (2u << 20), device, s);
   unsigned char* base = m.arena.get();
   check(launch_evo_zero(reinterpret_cast<int*>(base + off_flows), flow_items, s), "evo_zero");
+  check(launch_evo_zero(reinterpret_cast<int*>(base + m.off_time), 2 * static_cast<size_t>(m.workgroups), s),
+        "evo_zero");
   check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
                           batch.d_block_run().get(), reinterpret_cast<unsigned short*>(base + off_nbr),
                           static_cast<size_t>(batch.k()) * pad, pad, s),
@@ -230,6 +259,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   p.flows = m.hash_bits > 0 ? nullptr : reinterpret_cast<int*>(base + off_flows);
   p.hash_bits = m.hash_bits;
   p.history = reinterpret_cast<double*>(base + m.off_history);
+  p.wg_time = reinterpret_cast<unsigned long long*>(base + m.off_time);
   p.large_scratch = m.large ? reinterpret_cast<int*>(base + off_large) : nullptr;
   p.learn_packed = m.learned ? reinterpret_cast<const uint32_t*>(base + m.off_packed) : nullptr;
   if (split) {
@@ -244,6 +274,7 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
     p.workgroups = m.workgroups;
     p.cooperative = g_cooperative.load();
   }
+  m.group = group;
   // The set-up is complete before the timed evolution starts.
   check(hipStreamSynchronize(s), "setup sync");
   impl_ = impl.release();
@@ -351,6 +382,25 @@ void DeviceEvolution::export_results(int* clbest, double* accbest, double* vect_
   const auto* best = reinterpret_cast<const unsigned short*>(down + (m.off_best - m.down_begin));
   const auto* state = reinterpret_cast<const double*>(down + (m.off_state - m.down_begin));
   const auto* hist = reinterpret_cast<const double*>(down + (m.off_history - m.down_begin));
+  // Each run's cost: its slowest member's ticks per evaluation, per unit of
+  // the assignment's model; averaged with an earlier measurement of the shape.
+  if (m.done_cycles > 0) {
+    const auto* ticks = reinterpret_cast<const unsigned long long*>(down + (m.off_time - m.down_begin));
+    std::vector<double> cost(static_cast<size_t>(m.runs));
+    for (int r = 0, w = 0; r < m.runs; ++r) {
+      unsigned long long t = 0;
+      for (int g = 0; g < m.group[r]; ++g, ++w) t = std::max(t, ticks[w]);
+      cost[r] = static_cast<double>(t) / (m.done_cycles + 1) / (kReplayShare + 1.0 / m.group[r]);
+    }
+    std::lock_guard<std::mutex> lock(g_cost_mutex);
+    auto it = g_run_costs.find(m.shape_key);
+    if (it != g_run_costs.end()) {
+      for (int r = 0; r < m.runs; ++r) it->second[r] = 0.5 * (it->second[r] + cost[r]);
+    } else {
+      if (g_run_costs.size() >= kMaxCostShapes) g_run_costs.clear();
+      g_run_costs.emplace(m.shape_key, std::move(cost));
+    }
+  }
   const IVec& off = m.batch.run_offsets();
 #pragma omp parallel for schedule(static)
   for (int r = 0; r < m.runs; ++r) {

@@ -683,6 +683,9 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
   const int pad_total = p.pad_total;
   const bool guarded = p.guarded != 0;
 
+  // The workgroup's elapsed time: its start subtracted here, its end added
+  // before the write-back (one thread, one word per workgroup).
+  if (threadIdx.x == 0 && p.wg_time != nullptr) p.wg_time[wg] -= wall_clock64();
   // Per-launch set-up: clear the class tables, load the run state.
   for (int c = threadIdx.x; c < C; c += THREADS) {
     s.weight[c] = 0ull;
@@ -1363,6 +1366,7 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
     __syncthreads();
   }
 
+  if (threadIdx.x == 0 && p.wg_time != nullptr) p.wg_time[wg] += wall_clock64();
   // Write the run state back for the next launch.
   if (!lead) return;
   if (s.scal[kPending] & 1) {

@@ -50,6 +50,10 @@ struct EvoParams {
   // when the absorption reads them. LDS-resident layout only.
   int hash_bits = 0;
   double* history = nullptr;                 // [runs x cycles] best accuracy per cycle
+  // [workgroups] elapsed wall-clock ticks (100 MHz) summed over launches, for
+  // the member assignment of later evolutions of the same batch shape; may
+  // be null.
+  unsigned long long* wg_time = nullptr;
   // Learned proposal (null: every cycle uniform): the network in the tiled
   // layout of launch_mlp_pack, and the probability that a cycle falls back to
   // the uniform proposal.
