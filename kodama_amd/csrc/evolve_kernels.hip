@@ -370,9 +370,11 @@ __device__ __forceinline__ void sort_packed(uint32_t (&v)[M]) {
 // votes form one run with ranks ascending inside it; missing neighbours (the
 // sentinel code) sort last. Up to 32 neighbours a key has 5 rank bits and an
 // 11-bit code (labels up to 1023, a missing neighbour as 1024 + rank) and two
-// keys share a register in the sort; at 64 the keys are 32-bit. Returns -1 for a row
-// without neighbours.
-template <int KMAX>
+// keys share a register in the sort; at 64 the keys are 32-bit. Returns -1
+// for a row without neighbours. TWO_PASS picks the packed keys' two-pass run
+// scan (fewer instructions); the 512-thread learned launch, whose scoring
+// holds more registers, keeps the one-pass scan, which does not spill there.
+template <int KMAX, bool TWO_PASS>
 __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned short* __restrict__ nbr,
                                          const float* __restrict__ scores, const unsigned short* lab, int i) {
   uint32_t key[KMAX];
@@ -399,10 +401,12 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
   for (int j = 0; j < KMAX; ++j) {
     const uint32_t c = j < k ? key[j] : kSentinel;
     valid += c != kSentinel ? 1 : 0;
-    // Packed keys: a missing neighbour gets a code of its own above every
-    // label (1024 + rank), so missing neighbours never form a run longer than
-    // one.
-    const uint32_t code = kPacked && c == kSentinel ? 1024u + static_cast<uint32_t>(j) : c;
+    // Packed keys: a missing neighbour's code is above every label; for the
+    // two-pass scan one of its own (1024 + rank), so missing neighbours never
+    // form a run longer than one (the one-pass scan stops at the present
+    // ones and takes 2047 for all).
+    const uint32_t missing = TWO_PASS ? 1024u + static_cast<uint32_t>(j) : 2047u;
+    const uint32_t code = kPacked && c == kSentinel ? missing : c;
     key[j] = (code << kRankBits) | static_cast<uint32_t>(j);
   }
   if constexpr (kPacked) {
@@ -424,7 +428,7 @@ __device__ __forceinline__ int full_vote(int k, int pad_total, const unsigned sh
   // entry would cost the wave an exec-mask update each.
   int best_count = 0, best_code = -1, run = 0;
   bool tie = false;
-  if constexpr (kPacked) {
+  if constexpr (kPacked && TWO_PASS) {
     // Two passes: the longest run, then how many runs reach it and the first
     // of them. Missing neighbours sort last as runs of one, so they reach the
     // longest run only when it is 1: then a tie means two present votes.
@@ -591,7 +595,7 @@ __device__ __forceinline__ void vote_phase(int tid, int k, int pad_total, const 
   for (int q = tid; q < qn; q += THREADS) {
     const int i = s.queue[q];
     const int own = lab[i];
-    int best = full_vote<KMAX>(k, pad_total, nbr, scores, lab, i);
+    int best = full_vote<KMAX, THREADS == 1024>(k, pad_total, nbr, scores, lab, i);
     // Every row of a batch run has a neighbour (each run has two folds or
     // more); a row without one would keep its label.
     if (best < 0) best = own;

@@ -41,3 +41,20 @@ def test_c3_batch_device_form(oracle):
         cl = dev["clbest"][r]
         assert dev["accbest"][r] == np.mean(oracle.knn_vote(nbr, score, folds[r], cl) == cl), r
     assert abs(float(np.mean(dev["accbest"])) - float(np.mean(plain["accbest"]))) < 0.005
+
+
+def test_c3_member_assignment_from_measured_times():
+    """The second evolution of a batch shape assigns the split runs' members
+    from the first one's measured run times (256 workgroups across 100 runs,
+    so the costs decide which runs get them); the results are identical
+    whatever the assignment, and a third evolution refines it again."""
+    x, rows, folds, init = bench.make_workload(100, 100000, 50, 30, salt=7)
+    o = K.core_options(**dict(OPTS, cycles=4))
+    K.set_evolution_split(0)
+    outs = [K.core_knn_batch(x, rows, init, o, mode="device", fold_ids=folds) for _ in range(3)]
+    assert K.last_evolution_workgroups() > len(rows)
+    for out in outs[1:]:
+        for a, b in zip(outs[0]["clbest"], out["clbest"]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(outs[0]["accbest"], out["accbest"])
+        assert np.array_equal(outs[0]["vect_acc"], out["vect_acc"])
