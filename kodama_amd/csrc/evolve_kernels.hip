@@ -127,16 +127,22 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
   return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(wave_scan_u32(v)), 63));
 }
 
-// Keyed pseudo-random permutation of [0, n): a four-round unbalanced Feistel
-// network on `bits` = ceil(log2 n) bits (halves of floor and ceil width; each
-// round swaps their widths, four rounds restore them), cycle-walked into
-// [0, n): the row at position y < n is the first of network_inverse(y),
+// Keyed pseudo-random permutation of [0, n): a four-round Feistel network
+// over the mixed-radix domain [0, A * B), A = 2^a with a = floor(bits / 2)
+// (bits = ceil(log2 n)) and B = ceil(n / A), a position x being the pair
+// (hi, lo) = (x >> a, x mod A). The rounds add keyed functions of one half to
+// the other, modulo its radix: lo += F0(hi), hi += F1(lo), lo += F2(hi),
+// hi += F3(lo) (an F for hi scaled into [0, B) by a high multiply), each
+// undone by a subtraction. It is cycle-walked into [0, n): the row at
+// position y < n is the first of network_inverse(y),
 // network_inverse(network_inverse(y)), ... below n (the orbit of y under the
-// bijection returns below n, so the walk ends). It takes fewer than two rounds
-// of the network on average.
+// bijection returns below n, so the walk ends). The domain exceeds n by less
+// than A, about sqrt(n), so nearly every walk takes one step (a power-of-two
+// domain of up to 2n took up to twice as many on average).
 struct Permutation {
   uint32_t k0, k1, k2, k3;
-  int a, b;
+  int a;       // bits of lo
+  uint32_t b;  // radix of hi
 
   // Round keys from a 64-bit key: its halves, then the halves of mix64(key).
   __device__ static void round_keys(uint64_t key, uint32_t (&k)[4]) {
@@ -146,42 +152,23 @@ struct Permutation {
     k[2] = static_cast<uint32_t>(h);
     k[3] = static_cast<uint32_t>(h >> 32);
   }
-  __device__ Permutation(uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, int bits)
-      : k0(r0), k1(r1), k2(r2), k3(r3), a(bits >> 1), b(bits - (bits >> 1)) {}
+  __device__ Permutation(uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, int bits, int n)
+      : k0(r0), k1(r1), k2(r2), k3(r3), a(bits >> 1),
+        b(static_cast<uint32_t>((n + (1 << (bits >> 1)) - 1) >> (bits >> 1))) {}
 
-  __device__ __forceinline__ uint32_t network(uint32_t x) const {
-    int wl = a, wr = b;
-    uint32_t l = x >> wr, r = x & ((1u << wr) - 1u);
-    const uint32_t keys[4] = {k0, k1, k2, k3};
-#pragma unroll
-    for (int round = 0; round < 4; ++round) {
-      const uint32_t nr = (l ^ round_fn(r, keys[round])) & ((1u << wl) - 1u);
-      l = r;
-      r = nr;
-      const int t = wl;
-      wl = wr;
-      wr = t;
-    }
-    return (l << wr) | r;
+  __device__ __forceinline__ uint32_t sub_hi(uint32_t hi, uint32_t lo, uint32_t key) const {
+    const uint32_t t = __umulhi(round_fn(lo, key), b);  // in [0, b)
+    return hi >= t ? hi - t : hi + b - t;
   }
 
-  // Undoes the rounds in reverse order: round t mapped (l, r) of widths
-  // (wl, wr) to (r, l ^ F_t(r)) of widths (wr, wl).
   __device__ __forceinline__ uint32_t network_inverse(uint32_t y) const {
-    int wl = a, wr = b;
-    uint32_t l = y >> wr, r = y & ((1u << wr) - 1u);
-    const uint32_t keys[4] = {k0, k1, k2, k3};
-#pragma unroll
-    for (int round = 3; round >= 0; --round) {
-      const uint32_t old_r = l;
-      const uint32_t old_l = (r ^ round_fn(old_r, keys[round])) & ((1u << wr) - 1u);
-      l = old_l;
-      r = old_r;
-      const int t = wl;
-      wl = wr;
-      wr = t;
-    }
-    return (l << wr) | r;
+    const uint32_t mask = (1u << a) - 1u;
+    uint32_t hi = y >> a, lo = y & mask;
+    hi = sub_hi(hi, lo, k3);
+    lo = (lo - round_fn(hi, k2)) & mask;
+    hi = sub_hi(hi, lo, k1);
+    lo = (lo - round_fn(hi, k0)) & mask;
+    return (hi << a) | lo;
   }
 };
 
@@ -978,7 +965,8 @@ __global__ void __launch_bounds__(THREADS) evo_persistent_kernel(EvoParams p) {
           // as soon as a walk lands below n: the wave's passes follow the
           // longest lane's total walk, not the longest walk of every position.
           const Permutation perm(static_cast<uint32_t>(s.scal[kPerm0]), static_cast<uint32_t>(s.scal[kPerm1]),
-                                 static_cast<uint32_t>(s.scal[kPerm2]), static_cast<uint32_t>(s.scal[kPerm3]), bits);
+                                 static_cast<uint32_t>(s.scal[kPerm2]), static_cast<uint32_t>(s.scal[kPerm3]), bits,
+                                 n);
           uint32_t y = static_cast<uint32_t>(tid), x = y;
           bool live = static_cast<int>(y) < take;
           while (__ballot(live) != 0ull) {
