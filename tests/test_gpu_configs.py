@@ -1,5 +1,5 @@
-"""The large configurations of the north star on the device, at sizes the
-oracle checks in seconds:
+"""The large configurations (a 1M-row exact graph workload and a 200k-row
+label-noise workload) on the device, at sizes the oracle checks in seconds:
 
 * C4 (1,000,000 x 50 generator, exact 30-NN graph by split-BF16 MFMA
   candidates and an exact FP32 rerank): the graph of 100,000 rows, every
