@@ -72,6 +72,7 @@ struct DeviceEvolution::Impl {
   // every launch, and the error word read back at export.
   int workgroups = 0;
   size_t off_exchange = 0, exchange_words = 0, off_error = 0;
+  bool exchange_clear = false;  // cleared since the last launch
   PinnedBuffer<int> h_error;
 
   explicit Impl(KnnBatch& b) : batch(b) {}
@@ -211,6 +212,10 @@ DeviceEvolution::DeviceEvolution(KnnBatch& batch, const std::vector<IVec>& value
   m.arena.allocate((lay.total + (2u << 20) - 1) / (2u << 20) * (2u << 20), device, s);
   unsigned char* base = m.arena.get();
   check(launch_evo_zero(reinterpret_cast<int*>(base + off_flows), flow_items, s), "evo_zero");
+  if (split) {  // the first launch's exchange region, cleared with the set-up
+    check(launch_evo_zero(reinterpret_cast<int*>(base + m.off_exchange), m.exchange_words, s), "exchange clear");
+    m.exchange_clear = true;
+  }
   check(launch_evo_zero(reinterpret_cast<int*>(base + m.off_time), 2 * static_cast<size_t>(m.workgroups), s),
         "evo_zero");
   check(launch_evo_narrow(batch.d_neighbors().get(), batch.d_run_offsets().get(), batch.d_pad_offsets().get(),
@@ -330,10 +335,11 @@ void launch_cycles(DeviceEvolution::Impl& m, int cycles, bool learned, float fal
   p.cycle_count = cycles;
   if (!learned) p.learn_packed = nullptr;
   p.learn_fallback = fallback;
-  if (p.wg_run != nullptr) {
+  if (p.wg_run != nullptr && !m.exchange_clear) {
     check(launch_evo_zero(reinterpret_cast<int*>(m.arena.get() + m.off_exchange), m.exchange_words, s),
           "exchange clear");
   }
+  m.exchange_clear = false;  // the launch leaves tagged words behind
   if (!m.timed) m.timer.start(s);  // kernel time spans the first launch to the last
   check(launch_evo_persistent(p, s), "evo_persistent");
   m.timer.stop(s);
