@@ -210,21 +210,37 @@ def copy_probe() -> int:
     return 0
 
 
-def graph_bench(rows: int, k: int) -> int:
+def graph_bench(rows: int, k: int, lists_ab: bool = False) -> int:
     """Exact self-kNN graph of the 20-cluster mixture (config-4 generator,
-    p = 50) through the split-BF16 MFMA path; one JSON line."""
+    p = 50) through the split-BF16 MFMA path; one JSON line. With lists_ab,
+    the graph is built with packed candidate lists, separate ones and packed
+    again (one line each, and whether the graphs are identical)."""
     import kodama_amd as K
     from kodama_amd import datasets as D
     x, _ = D.config4(n=rows)
     K.KODAMA_knn_graph(x[:8192], k=k)  # warm-up (code objects, allocator)
-    t0 = time.perf_counter()
-    g = K.KODAMA_knn_graph(x, k=k)
-    wall = time.perf_counter() - t0
-    fb = K.last_knn_fallbacks()
-    flops = 2.0 * rows * rows * 50 * 3  # three bf16 products per MAC (hi.hi, hi.lo, lo.hi)
-    print(json.dumps({"graph_rows": rows, "k": k, "p": 50, "seconds": wall, "fallback_queries": fb,
-                      "candidate_bf16_tflops_per_s": flops / wall / 1e12,
-                      "neighbors": int(g["neighbors"])}), flush=True)
+    forms = (True, False, True) if lists_ab else (True,)
+    first = None
+    try:
+        for packed in forms:
+            K.set_mfma_packed_lists(packed)
+            t0 = time.perf_counter()
+            g = K.KODAMA_knn_graph(x, k=k)
+            wall = time.perf_counter() - t0
+            fb = K.last_knn_fallbacks()
+            flops = 2.0 * rows * rows * 50 * 3  # three bf16 products per MAC (hi.hi, hi.lo, lo.hi)
+            line = {"graph_rows": rows, "k": k, "p": 50, "seconds": wall, "fallback_queries": fb,
+                    "candidate_bf16_tflops_per_s": flops / wall / 1e12, "neighbors": int(g["neighbors"])}
+            if lists_ab:
+                line["packed_lists"] = packed
+                if first is None:
+                    first = (g["indices"], g["distances"])
+                else:
+                    line["identical"] = bool(np.array_equal(first[0], g["indices"]) and
+                                             np.array_equal(first[1], g["distances"]))
+            print(json.dumps(line), flush=True)
+    finally:
+        K.set_mfma_packed_lists(True)
     return 0
 
 
@@ -444,6 +460,8 @@ def main() -> int:
                     help="launch split evolution runs as an ordinary grid instead of cooperatively "
                          "(rocprofv3 ends with a fault after tracing cooperative launches and writes nothing)")
     ap.add_argument("--graph-rows", type=int, default=0, help="time the exact k-NN graph of this many rows and exit")
+    ap.add_argument("--lists-ab", action="store_true",
+                    help="with --graph-rows: build the graph with packed and with separate candidate lists")
     ap.add_argument("--learned-curve", action="store_true",
                     help="accuracy against wall time of the uniform and learned proposals on the 200k"
                          " label-noise configuration (--steps cycles, default 100 here) and exit")
@@ -457,7 +475,7 @@ def main() -> int:
     if args.copy_probe:
         return copy_probe()
     if args.graph_rows > 0:
-        return graph_bench(args.graph_rows, args.k)
+        return graph_bench(args.graph_rows, args.k, args.lists_ab)
     if args.tables:
         return tables_bench(args.runs, args.n, args.k)
     if args.learned_curve:
